@@ -1,0 +1,1338 @@
+// mgx_api.hip -- C ABI of libmgx.so (include/mgx.h): device-level kernels, device-resident
+// structured matrices, and the solver handle (stationary V-cycle, Jacobi PCG, MG-PCG).
+//
+// The solver drivers restate the reference host loops:
+//   OpenCLCsrGeometricMultigridSolver.cs:145-285 (Initialize) and :334-536 (Solve)
+//   OpenCLDuViGeometricMultigridSolver.cs:172-391, :455-751
+//   OpenCLCsrPcgSolver.cs:84-218, :250-331
+// with the MI355X changes: coarse solve on the device (dense SPD inverse, one GEMV instead of a
+// host LDL^T round trip per cycle), scalars (alpha, beta, norms) kept on the device, and the
+// stop test evaluated on the device so that several PCG iterations are queued per host sync.
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <memory>
+#include <new>
+
+#include "mgx_internal.h"
+
+namespace mgx {
+
+static thread_local std::string g_error;
+void set_error(const std::string &msg) { g_error = msg; }
+
+template <class F>
+static int guard(F &&f)
+{
+    try {
+        f();
+        return MGX_OK;
+    } catch (const Error &e) {
+        set_error(e.msg);
+        return e.code;
+    } catch (const std::bad_alloc &) {
+        set_error("host out of memory");
+        return MGX_ERR_OUT_OF_MEMORY;
+    } catch (const std::exception &e) {
+        set_error(e.what());
+        return MGX_ERR_DEVICE;
+    }
+}
+
+// ------------------------------------------------------------------ device buffers
+template <typename T>
+struct DBuf {
+    T *p = nullptr;
+    size_t n = 0;
+    DBuf() = default;
+    DBuf(const DBuf &) = delete;
+    DBuf &operator=(const DBuf &) = delete;
+    DBuf(DBuf &&o) noexcept : p(o.p), n(o.n) { o.p = nullptr; o.n = 0; }
+    DBuf &operator=(DBuf &&o) noexcept
+    {
+        if (this != &o) { reset(); p = o.p; n = o.n; o.p = nullptr; o.n = 0; }
+        return *this;
+    }
+    ~DBuf() { reset(); }
+    void reset()
+    {
+        if (p) (void)hipFree(p);
+        p = nullptr; n = 0;
+    }
+    void alloc(size_t count)
+    {
+        reset();
+        if (count == 0) count = 1;
+        MGX_HIP(hipMalloc((void **)&p, sizeof(T) * count));
+        n = count;
+    }
+    void upload(const T *h, size_t count, hipStream_t s)
+    {
+        alloc(count);
+        if (count) MGX_HIP(hipMemcpyAsync(p, h, sizeof(T) * count, hipMemcpyHostToDevice, s));
+    }
+};
+
+struct DevCsr {
+    int32_t nrows = 0, ncols = 0;
+    int64_t nnz = 0;
+    DBuf<int32_t> rp_, ci_;
+    DBuf<double> v_;
+    DBuf<float> v32_;
+    // views (may point into another object's storage)
+    int32_t *rp = nullptr, *ci = nullptr;
+    double *v = nullptr;
+    float *v32 = nullptr;
+    int lanes = 16;
+    void upload(const mgx_csr_host &h, hipStream_t s)
+    {
+        nrows = h.nrows; ncols = h.ncols; nnz = h.nnz;
+        rp_.upload(h.rowptr, (size_t)nrows + 1, s);
+        ci_.upload(h.col, (size_t)nnz, s);
+        v_.upload(h.val, (size_t)nnz, s);
+        rp = rp_.p; ci = ci_.p; v = v_.p;
+        lanes = lanes_for(nnz, nrows);
+    }
+    void view_of(const DevCsr &o)
+    {
+        nrows = o.nrows; ncols = o.ncols; nnz = o.nnz;
+        rp = o.rp; ci = o.ci; v = o.v; lanes = o.lanes;
+    }
+    void make_f32(hipStream_t s)
+    {
+        v32_.alloc((size_t)nnz);
+        launch_convert_f64_f32(nnz, v, v32_.p, nullptr, s);
+        v32 = v32_.p;
+    }
+};
+
+struct DevDuvi {
+    mgx_duvi d{};
+    DBuf<int32_t> row_blocks, row_dist, block_col;
+    DBuf<uint8_t> dist, vi;
+    DBuf<double> values;
+    void encode_upload(const DevCsr &A, const std::vector<int32_t> &hrp, const std::vector<int32_t> &hci,
+                       const std::vector<double> &hv, double tol, int mode, hipStream_t s)
+    {
+        mgx_duvi_host h;
+        duvi_encode(A.nrows, hrp.data(), hci.data(), hv.data(), tol, mode, &h);
+        row_blocks.upload(h.row_blocks, (size_t)h.nrows + 1, s);
+        row_dist.upload(h.row_dist, (size_t)h.nrows + 1, s);
+        block_col.upload(h.block_col, (size_t)h.nblocks, s);
+        dist.upload(h.dist, (size_t)h.nnz, s);
+        if (h.vi_bytes) vi.upload((const uint8_t *)h.vi, (size_t)h.nnz * h.vi_bytes, s);
+        values.upload(h.values, (size_t)h.nuniq, s);
+        MGX_HIP(hipStreamSynchronize(s));
+        d.nrows = h.nrows; d.vi_bytes = h.vi_bytes; d.nnz = h.nnz; d.nblocks = h.nblocks; d.nuniq = h.nuniq;
+        d.row_blocks = row_blocks.p; d.row_dist = row_dist.p; d.block_col = block_col.p; d.dist = dist.p;
+        d.vi = h.vi_bytes ? vi.p : nullptr; d.values = values.p;
+        mgx_duvi_host_free(&h);
+    }
+};
+
+static void download_csr(const DevCsr &A, std::vector<int32_t> &rp, std::vector<int32_t> &ci, std::vector<double> &v,
+                         hipStream_t s)
+{
+    rp.resize((size_t)A.nrows + 1); ci.resize((size_t)A.nnz); v.resize((size_t)A.nnz);
+    MGX_HIP(hipMemcpyAsync(rp.data(), A.rp, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipMemcpyAsync(ci.data(), A.ci, sizeof(int32_t) * ci.size(), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipMemcpyAsync(v.data(), A.v, sizeof(double) * v.size(), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+}
+
+}  // namespace mgx
+
+// ====================================================================== opaque handle types
+struct mgx_matrix_s {
+    mgx::DevCsr A;
+};
+
+using namespace mgx;
+
+// small fill / combine kernels used by the drivers
+__global__ void k_set_i32(int32_t *p, int32_t v) { *p = v; }
+
+__global__ __launch_bounds__(kBlock) void k_resid_from_ax(int32_t n, const double *__restrict__ b,
+                                                          const double *__restrict__ ax, double *__restrict__ r,
+                                                          const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        r[i] = b[i] - ax[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_resid_flag_from_ax(int32_t n, const double *__restrict__ b,
+                                                               const double *__restrict__ ax, double *__restrict__ r,
+                                                               double tol, int32_t *flag)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double ri = b[i] - ax[i];
+        r[i] = ri;
+        const double c = fabs(ri);
+        if (ri != ri || c > 1e50) atomicOr(flag, 2);
+        else if (c > tol) atomicAnd(flag, 2);
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void k_jacobi_from_ax(int32_t n, const double *__restrict__ pre,
+                                                           const double *__restrict__ b, const double *__restrict__ x,
+                                                           const double *__restrict__ ax, double *__restrict__ y,
+                                                           const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        y[i] = x[i] + pre[i] * (b[i] - ax[i]);
+}
+
+__global__ __launch_bounds__(kBlock) void k_scale_into(int32_t n, const double *__restrict__ d,
+                                                       const double *__restrict__ r, double *__restrict__ z,
+                                                       const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        z[i] = d[i] * r[i];
+}
+
+__global__ __launch_bounds__(kBlock) void k_pcg_norms(int32_t n, const double *__restrict__ b,
+                                                      const double *__restrict__ r, Ctl *ctl, double *partials)
+{
+    double sb = 0, sr = 0, m = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double bi = b[i], ri = r[i];
+        sb += bi * bi; sr += ri * ri;
+        m = (ri != ri) ? ri : (m != m ? m : (fabs(ri) > m ? fabs(ri) : m));
+    }
+    // grid reduction (shared helper lives in mgx_kernels.hip; restated here for this kernel)
+    __shared__ double sh[3][kBlock / 64];
+    __shared__ int is_last;
+    for (int o = 32; o > 0; o >>= 1) {
+        sb += __shfl_xor(sb, o, 64); sr += __shfl_xor(sr, o, 64);
+        const double t = __shfl_xor(m, o, 64);
+        m = (m != m) ? m : ((t != t) ? t : (t > m ? t : m));
+    }
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    if (l == 0) { sh[0][w] = sb; sh[1][w] = sr; sh[2][w] = m; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double a = 0, c = 0, mm = 0;
+        for (int i = 0; i < kBlock / 64; ++i) {
+            a += sh[0][i]; c += sh[1][i];
+            const double t = sh[2][i];
+            mm = (mm != mm) ? mm : ((t != t) ? t : (t > mm ? t : mm));
+        }
+        __hip_atomic_store(&partials[3 * blockIdx.x + 0], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[3 * blockIdx.x + 1], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[3 * blockIdx.x + 2], mm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t t = __hip_atomic_fetch_add(&ctl->counter[4], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last || threadIdx.x != 0) return;
+    double a = 0, c = 0, mm = 0;
+    for (int i = 0; i < (int)gridDim.x; ++i) {
+        a += __hip_atomic_load(&partials[3 * i + 0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        c += __hip_atomic_load(&partials[3 * i + 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        const double t = __hip_atomic_load(&partials[3 * i + 2], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        mm = (mm != mm) ? mm : ((t != t) ? t : (t > mm ? t : mm));
+    }
+    __hip_atomic_store(&ctl->counter[4], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    ctl->bnorm2 = a;
+    ctl->rnorm2 = c;
+    ctl->rmax = mm;
+    ctl->iter = 0;
+    int done = 0, status = 0;
+    if (!(mm == mm) || mm > 1e50) { done = 1; status = -1; }
+    else if (ctl->stop == MGX_STOP_REL_L2 ? (c <= ctl->tol * ctl->tol * a) : (mm <= ctl->tol)) { done = 1; status = 1; }
+    else if (ctl->max_it <= 0) done = 1;
+    ctl->status = status;
+    ctl->done = done;
+}
+
+static inline int grid_vec(int64_t n)
+{
+    int64_t nb = (n + kBlock * 4 - 1) / (kBlock * 4);
+    return (int)std::max<int64_t>(1, std::min<int64_t>(nb, 2048));
+}
+
+// ====================================================================== solver
+namespace mgx {
+
+struct Level {
+    int32_t n = 0;
+    DevCsr A, R, P;                        // R: to level+1, P: from level+1
+    mgx_matrix extA = nullptr, extR = nullptr, extP = nullptr;
+    std::vector<int32_t> hArp, hAci, hRrp, hRci, hPrp, hPci;   // staged host copies
+    std::vector<double> hAv, hRv, hPv;
+    bool staged = false;
+    DBuf<double> pre, x, b, r, t, w;       // omega D^-1 and work vectors (w: DuVi scratch)
+    DBuf<float> pre32, x32, b32, r32, t32;
+    int ncolors = 0;
+    std::vector<DBuf<int32_t>> color_nodes;
+    std::vector<int32_t> color_count;
+    DevDuvi dA, dR, dP;
+};
+
+}  // namespace mgx
+
+struct mgx_solver_s {
+    mgx_options opt{};
+    std::vector<mgx::Level> lv;
+    std::vector<double> b_host;
+    int32_t n = 0, nrhs = 1;
+    bool initialized = false;
+    hipStream_t stream = nullptr;
+    mgx::DBuf<double> coarse_inv;
+    mgx::DBuf<double> bdev, xdev;          // level-0 rhs / solution (nrhs columns)
+    mgx::DBuf<double> p, q, z, r;          // PCG vectors
+    mgx::DBuf<double> partials;
+    mgx::DBuf<mgx::Ctl> ctl;
+    mgx::Ctl *ctl_host = nullptr;          // pinned
+    mgx::DBuf<int32_t> flag;
+    int32_t *flag_host = nullptr;          // pinned
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int check_every = 4;
+    size_t partials_tail = 0;
+    std::vector<hipEvent_t> prof_events;   // pairs around level-0 A*p launches
+    int prof_used = 0;
+};
+
+namespace mgx {
+
+static int nlevels_of(const mgx_solver_s *s)
+{
+    if (s->opt.method == MGX_METHOD_PCG_JACOBI) return 1;
+    return (int)s->lv.size();
+}
+
+// ---------------------------------------------------------------- SpMV dispatch by format
+static void level_spmv(mgx_solver_s *S, Level &L, DevCsr &M, DevDuvi &D, const double *x, double *y, bool clear,
+                       const int32_t *done)
+{
+    if (S->opt.format == MGX_FORMAT_DUVI) launch_duvi_spmv(D.d, x, y, clear, done, S->stream);
+    else launch_spmv(M.nrows, M.rp, M.ci, M.v, x, y, clear, M.lanes, done, S->stream);
+}
+
+static void level_residual(mgx_solver_s *S, Level &L, const double *b, const double *x, double *r, double *tmp,
+                           const int32_t *done)
+{
+    if (S->opt.format == MGX_FORMAT_DUVI) {
+        launch_duvi_spmv(L.dA.d, x, tmp, true, done, S->stream);
+        hipLaunchKernelGGL(k_resid_from_ax, dim3(grid_vec(L.n)), dim3(kBlock), 0, S->stream, L.n, b, tmp, r, done);
+        MGX_HIP(hipGetLastError());
+    } else launch_residual(L.n, L.A.rp, L.A.ci, L.A.v, b, x, r, L.A.lanes, done, S->stream);
+}
+
+static void level_jacobi(mgx_solver_s *S, Level &L, const double *b, const double *x, double *y, double *tmp,
+                         const int32_t *done)
+{
+    if (S->opt.format == MGX_FORMAT_DUVI) {
+        launch_duvi_spmv(L.dA.d, x, tmp, true, done, S->stream);
+        hipLaunchKernelGGL(k_jacobi_from_ax, dim3(grid_vec(L.n)), dim3(kBlock), 0, S->stream, L.n, L.pre.p, b, x, tmp,
+                           y, done);
+        MGX_HIP(hipGetLastError());
+    } else launch_jacobi(L.n, L.pre.p, L.A.rp, L.A.ci, L.A.v, b, x, y, L.A.lanes, done, S->stream);
+}
+
+static void level_gs(mgx_solver_s *S, Level &L, const double *b, double *x, bool backward, const int32_t *done)
+{
+    if (!backward)
+        for (int c = 0; c < L.ncolors; ++c)
+            launch_gs_color(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v, b, x, false, done, S->stream);
+    else
+        for (int c = L.ncolors - 1; c >= 0; --c)
+            launch_gs_color(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v, b, x, true, done, S->stream);
+}
+
+// ---------------------------------------------------------------- initialize
+static void stage_host(std::vector<int32_t> &rp, std::vector<int32_t> &ci, std::vector<double> &v, const mgx_csr_host *h)
+{
+    rp.assign(h->rowptr, h->rowptr + h->nrows + 1);
+    ci.assign(h->col, h->col + h->nnz);
+    v.assign(h->val, h->val + h->nnz);
+}
+
+static void check_csr_host(const mgx_csr_host *h, const char *what)
+{
+    MGX_REQUIRE(h && h->rowptr && h->nrows >= 0 && h->ncols >= 0 && h->nnz >= 0, MGX_ERR_INVALID_ARGUMENT,
+                std::string(what) + ": null or negative-size matrix");
+    MGX_REQUIRE(h->nnz == 0 || (h->col && h->val), MGX_ERR_INVALID_ARGUMENT, std::string(what) + ": null arrays");
+    MGX_REQUIRE(h->rowptr[0] == 0 && (int64_t)h->rowptr[h->nrows] == h->nnz, MGX_ERR_INVALID_ARGUMENT,
+                std::string(what) + ": rowptr[0] must be 0 and rowptr[nrows] == nnz");
+    for (int32_t i = 0; i < h->nrows; ++i)
+        MGX_REQUIRE(h->rowptr[i + 1] >= h->rowptr[i], MGX_ERR_INVALID_ARGUMENT, std::string(what) + ": rowptr not monotone");
+    for (int64_t k = 0; k < h->nnz; ++k)
+        MGX_REQUIRE(h->col[k] >= 0 && h->col[k] < h->ncols, MGX_ERR_INVALID_ARGUMENT,
+                    std::string(what) + ": column index out of range");
+}
+
+// power-method weight of RelaxedJacobiPreconditioner (GeometricMultigridSolver.cs:80-121)
+static double power_bound(const std::vector<int32_t> &rp, const std::vector<int32_t> &ci, const std::vector<double> &v,
+                          const std::vector<double> &invd)
+{
+    const int32_t n = (int32_t)invd.size();
+    const double percent = 0.05;
+    std::vector<double> x(n, 1 / std::sqrt((double)n)), y(n);
+    double l = 1;
+    for (int it = 0; it < 100000; ++it) {
+        for (int32_t i = 0; i < n; ++i) {
+            double s = 0;
+            for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s += v[k] * x[ci[k]];
+            y[i] = s * invd[i];
+        }
+        double l2 = 0;
+        for (double t : y) l2 += t * t;
+        l2 = std::sqrt(l2);
+        for (int32_t i = 0; i < n; ++i) x[i] = y[i] * (1 / l2);
+        const double ratio = l2 / l;
+        if (ratio > 1 - percent && ratio < 1 + percent) {
+            l = ratio > 1 ? l2 * std::pow(ratio, 1 / std::sqrt(percent)) : (l2 * 9 + l) / 10;
+            break;
+        }
+        l = l2;
+    }
+    return l;
+}
+
+static void initialize(mgx_solver_s *S)
+{
+    hipStream_t st = S->stream;
+    const int L = nlevels_of(S);
+    MGX_REQUIRE(L >= 1 && (int)S->lv.size() >= L, MGX_ERR_INVALID_OPERATION, "levels not set");
+    if (S->opt.method == MGX_METHOD_GMG) {
+        MGX_REQUIRE(S->opt.depth >= 1 && L == S->opt.depth + 1, MGX_ERR_INVALID_ARGUMENT,
+                    "GMG: the number of levels must be depth + 1");
+        MGX_REQUIRE(S->opt.smoother == MGX_SMOOTHER_GS || S->opt.level_iterations % 2 == 0, MGX_ERR_INVALID_ARGUMENT,
+                    "Jacobi ping-pong needs an even number of smoother iterations");
+    }
+    if (S->opt.method == MGX_METHOD_MGPCG) MGX_REQUIRE(L >= 2, MGX_ERR_INVALID_ARGUMENT, "MG-PCG needs >= 2 levels");
+    MGX_REQUIRE(!(S->opt.format == MGX_FORMAT_DUVI && S->opt.smoother == MGX_SMOOTHER_GS), MGX_ERR_NOT_IMPLEMENTED,
+                "multicolour Gauss-Seidel runs on the CSR format only");
+    MGX_REQUIRE(S->opt.precision == MGX_PRECISION_FP64, MGX_ERR_NOT_IMPLEMENTED, "precision form not available");
+    // upload / attach level matrices
+    for (int l = 0; l < L; ++l) {
+        Level &V = S->lv[l];
+        const bool coarsest = (l == L - 1);
+        if (V.extA) {
+            V.A.view_of(V.extA->A);
+            if (!coarsest && S->opt.method != MGX_METHOD_PCG_JACOBI) {
+                MGX_REQUIRE(V.extR && V.extP, MGX_ERR_INVALID_ARGUMENT, "R and P required below the coarsest level");
+                V.R.view_of(V.extR->A); V.P.view_of(V.extP->A);
+            }
+        } else {
+            MGX_REQUIRE(V.staged, MGX_ERR_INVALID_OPERATION, "level matrix not set");
+            mgx_csr_host h{(int32_t)(V.hArp.size() - 1), (int32_t)(V.hArp.size() - 1), (int64_t)V.hAci.size(),
+                           V.hArp.data(), V.hAci.data(), V.hAv.data()};
+            V.A.upload(h, st);
+            if (!coarsest && S->opt.method != MGX_METHOD_PCG_JACOBI) {
+                MGX_REQUIRE(!V.hRrp.empty() && !V.hPrp.empty(), MGX_ERR_INVALID_ARGUMENT, "R and P required");
+                const int32_t nc = (int32_t)(V.hRrp.size() - 1);
+                mgx_csr_host hr{nc, V.A.nrows, (int64_t)V.hRci.size(), V.hRrp.data(), V.hRci.data(), V.hRv.data()};
+                mgx_csr_host hp{V.A.nrows, nc, (int64_t)V.hPci.size(), V.hPrp.data(), V.hPci.data(), V.hPv.data()};
+                V.R.upload(hr, st); V.P.upload(hp, st);
+            }
+        }
+        V.n = V.A.nrows;
+        MGX_REQUIRE(V.A.nrows == V.A.ncols, MGX_ERR_INVALID_ARGUMENT, "level matrix must be square");
+        if (l > 0) MGX_REQUIRE(S->lv[l - 1].R.nrows == V.n || S->opt.method == MGX_METHOD_PCG_JACOBI,
+                               MGX_ERR_INVALID_ARGUMENT, "R rows must match the next level's size");
+    }
+    S->n = S->lv[0].n;
+    MGX_REQUIRE((int64_t)S->b_host.size() == (int64_t)S->n * S->nrhs, MGX_ERR_INVALID_OPERATION,
+                "right-hand side not set (mgx_solver_set_rhs) or of the wrong size");
+    // host copies needed for setup (diagonals, weights, colouring, DuVi, coarse factor)
+    std::vector<std::vector<int32_t>> hrp(L), hci(L);
+    std::vector<std::vector<double>> hv(L);
+    for (int l = 0; l < L; ++l) {
+        Level &V = S->lv[l];
+        if (V.staged && !V.extA) { hrp[l] = V.hArp; hci[l] = V.hAci; hv[l] = V.hAv; }
+        else download_csr(V.A, hrp[l], hci[l], hv[l], st);
+    }
+    // Jacobi weights
+    const int nsmooth = (S->opt.method == MGX_METHOD_PCG_JACOBI) ? 1 : L - 1;
+    std::vector<std::vector<double>> invd(nsmooth);
+    for (int l = 0; l < nsmooth; ++l) {
+        const int32_t n = S->lv[l].n;
+        invd[l].resize(n);
+        for (int32_t i = 0; i < n; ++i) {
+            double d = 0;
+            for (int32_t k = hrp[l][i]; k < hrp[l][i + 1]; ++k) if (hci[l][k] == i) d = hv[l][k];
+            MGX_REQUIRE(d != 0, MGX_ERR_INVALID_ARGUMENT, "zero diagonal entry");
+            invd[l][i] = 1 / d;
+        }
+    }
+    if (S->opt.method != MGX_METHOD_PCG_JACOBI && S->opt.smoother == MGX_SMOOTHER_JACOBI) {
+        if (S->opt.omega > 0) {
+            for (auto &d : invd) for (auto &t : d) t *= S->opt.omega;
+        } else if (S->opt.coarse_relaxation) {
+            // RelaxateJacobiPreconditioners: 2 / lambda of the coarsest matrix, GeometricMultigridSolver.cs:144-167,209-214
+            const auto &rp = hrp[L - 1];
+            const auto &vv = hv[L - 1];
+            double lam = 0;
+            for (int32_t i = 0; i + 1 < (int32_t)rp.size(); ++i) {
+                double s = 0, m = 0;
+                for (int32_t k = rp[i]; k < rp[i + 1]; ++k) { const double a = std::fabs(vv[k]); s += a; m = std::max(m, a); }
+                lam = std::max(lam, s / m);
+            }
+            const double w = 2 / lam;
+            for (auto &d : invd) for (auto &t : d) t *= w;
+        } else {
+            for (int l = 0; l < nsmooth; ++l) {
+                const double lam = 1.2 * power_bound(hrp[l], hci[l], hv[l], invd[l]);
+                const double w = 2 / lam;
+                for (auto &t : invd[l]) t *= w;
+            }
+        }
+    }
+    for (int l = 0; l < nsmooth; ++l) S->lv[l].pre.upload(invd[l].data(), invd[l].size(), st);
+    // multicolour GS node lists
+    if (S->opt.smoother == MGX_SMOOTHER_GS && S->opt.method != MGX_METHOD_PCG_JACOBI) {
+        for (int l = 0; l < L - 1; ++l) {
+            Level &V = S->lv[l];
+            MGX_REQUIRE(V.n % 3 == 0, MGX_ERR_INVALID_ARGUMENT, "Gauss-Seidel node blocks need 3 dofs per node");
+            std::vector<int32_t> color;
+            V.ncolors = block_coloring(V.n / 3, 3, hrp[l].data(), hci[l].data(), color);
+            std::vector<std::vector<int32_t>> lists(V.ncolors);
+            for (int32_t k = 0; k < (int32_t)color.size(); ++k) lists[color[k]].push_back(k);
+            V.color_nodes.clear();
+            V.color_nodes = std::vector<DBuf<int32_t>>(V.ncolors);
+            V.color_count.assign(V.ncolors, 0);
+            for (int c = 0; c < V.ncolors; ++c) {
+                V.color_nodes[c].upload(lists[c].data(), lists[c].size(), st);
+                V.color_count[c] = (int32_t)lists[c].size();
+            }
+            MGX_HIP(hipStreamSynchronize(st));
+        }
+    }
+    // DuVi encodings
+    if (S->opt.format == MGX_FORMAT_DUVI) {
+        for (int l = 0; l < L; ++l) {
+            Level &V = S->lv[l];
+            V.dA.encode_upload(V.A, hrp[l], hci[l], hv[l], S->opt.duvi_tolerance, S->opt.duvi_mode, st);
+            if (l < L - 1) {
+                std::vector<int32_t> a, b; std::vector<double> c;
+                download_csr(V.R, a, b, c, st);
+                V.dR.encode_upload(V.R, a, b, c, S->opt.duvi_tolerance, S->opt.duvi_mode, st);
+                download_csr(V.P, a, b, c, st);
+                V.dP.encode_upload(V.P, a, b, c, S->opt.duvi_tolerance, S->opt.duvi_mode, st);
+            }
+        }
+    }
+    // coarse inverse
+    if (S->opt.method != MGX_METHOD_PCG_JACOBI) {
+        const int32_t nc = S->lv[L - 1].n;
+        MGX_REQUIRE(nc <= 20000, MGX_ERR_INVALID_ARGUMENT, "coarsest level too large for the dense direct solve");
+        std::vector<double> D((size_t)nc * nc, 0.0);
+        for (int32_t i = 0; i < nc; ++i)
+            for (int32_t k = hrp[L - 1][i]; k < hrp[L - 1][i + 1]; ++k) D[(size_t)i * nc + hci[L - 1][k]] = hv[L - 1][k];
+        MGX_REQUIRE(dense_spd_inverse(nc, D), MGX_ERR_NOT_SPD, "coarsest matrix is not symmetric positive definite");
+        S->coarse_inv.upload(D.data(), D.size(), st);
+        MGX_HIP(hipStreamSynchronize(st));
+    }
+    // work vectors
+    for (int l = 0; l < L; ++l) {
+        Level &V = S->lv[l];
+        V.x.alloc(V.n); V.b.alloc(V.n); V.r.alloc(V.n); V.t.alloc(V.n); V.w.alloc(V.n);
+    }
+    S->bdev.upload(S->b_host.data(), S->b_host.size(), st);
+    S->xdev.alloc((size_t)S->n * S->nrhs);
+    S->p.alloc(S->n); S->q.alloc(S->n); S->z.alloc(S->n); S->r.alloc(S->n);
+    // partial sums: enough for the largest row-kernel grid (2 rows per block minimum)
+    size_t maxgrid = 2048;
+    for (int l = 0; l < L; ++l) maxgrid = std::max<size_t>(maxgrid, (size_t)S->lv[l].n / 2 + 64);
+    S->partials.alloc(3 * maxgrid + 16);
+    S->partials_tail = 3 * maxgrid;
+    S->ctl.alloc(1);
+    MGX_HIP(hipMemsetAsync(S->ctl.p, 0, sizeof(Ctl), st));
+    S->flag.alloc(1);
+    MGX_HIP(hipStreamSynchronize(st));
+    S->initialized = true;
+}
+
+// ---------------------------------------------------------------- stationary V-cycle (reference semantics)
+static void solve_gmg(mgx_solver_s *S, const double *b, double *x, bool has_x0, mgx_stats *st)
+{
+    const int L = (int)S->lv.size();
+    const int depth = S->opt.depth;
+    hipStream_t s = S->stream;
+    double *X0 = x;
+    std::vector<double *> X(L), B(L);
+    X[0] = X0; B[0] = const_cast<double *>(b);
+    for (int l = 1; l < L; ++l) { X[l] = S->lv[l].x.p; B[l] = S->lv[l].b.p; }
+    double *R = S->lv[0].r.p;       // shared residual / ping-pong buffer (bufferOfVectorR)
+    double *tmp = S->lv[0].t.p;
+    bool x_given = has_x0;
+    const int jac = S->opt.smoother == MGX_SMOOTHER_JACOBI;
+    for (int cycle = 0;; ++cycle) {
+        int cur = 0;
+        for (int step = 0; step < 2 * depth; ++step) {
+            const bool down = step < depth;
+            Level &V = S->lv[cur];
+            if (cur == L - 1) {
+                // coarse: b[last] <- A^-1 b[last] (OpenCLCsrGeometricMultigridSolver.cs:364-367), then x[last-1] += P b[last]
+                launch_dense_gemv(V.n, S->coarse_inv.p, B[cur], tmp, false, nullptr, s);
+                MGX_HIP(hipMemcpyAsync(B[cur], tmp, sizeof(double) * V.n, hipMemcpyDeviceToDevice, s));
+                Level &F = S->lv[cur - 1];
+                level_spmv(S, F, F.P, F.dP, B[cur], X[cur - 1], false, nullptr);
+            } else {
+                const bool ld = step == 0 ? false : (step - 1 < depth);
+                const bool reset = (cur == 0 && !x_given) || (cur != 0 && ld);
+                const int it = S->opt.level_iterations;
+                if (!jac) {
+                    if (reset) launch_fill_zero(V.n, X[cur], nullptr, s);
+                    for (int i = 0; i < it; ++i) level_gs(S, V, B[cur], X[cur], false, nullptr);
+                } else {
+                    int i = 0;
+                    if (reset) {
+                        launch_jacobi_initial(V.n, V.pre.p, B[cur], R, nullptr, s);
+                        level_jacobi(S, V, B[cur], R, X[cur], tmp, nullptr);
+                        i = 2;
+                    }
+                    for (; i < it; i += 2) {
+                        level_jacobi(S, V, B[cur], X[cur], R, tmp, nullptr);
+                        level_jacobi(S, V, B[cur], R, X[cur], tmp, nullptr);
+                    }
+                }
+                if (down) {
+                    if (cur == 0) {
+                        hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, S->flag.p, 1);
+                        if (S->opt.format == MGX_FORMAT_DUVI) {
+                            launch_duvi_spmv(V.dA.d, X[0], tmp, true, nullptr, s);
+                            hipLaunchKernelGGL(k_resid_flag_from_ax, dim3(grid_vec(V.n)), dim3(kBlock), 0, s, V.n, B[0],
+                                               tmp, R, S->opt.tolerance, S->flag.p);
+                        } else
+                            launch_residual_flag(V.n, V.A.rp, V.A.ci, V.A.v, B[0], X[0], R, S->opt.tolerance, S->flag.p,
+                                                 V.A.lanes, s);
+                        MGX_HIP(hipMemcpyAsync(S->flag_host, S->flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+                        MGX_HIP(hipStreamSynchronize(s));
+                        x_given = true;
+                        const int32_t f = *S->flag_host;
+                        const bool converged = f == 1, failed = (f & 2) == 2;
+                        if (converged || failed || cycle > S->opt.max_iterations) {
+                            st->iterations = cycle;
+                            st->converged = failed ? -1 : (converged ? 1 : 0);
+                            return;
+                        }
+                    } else level_residual(S, V, B[cur], X[cur], R, tmp, nullptr);
+                    Level &C = S->lv[cur + 1];
+                    (void)C;
+                    level_spmv(S, V, V.R, V.dR, R, B[cur + 1], true, nullptr);
+                } else {
+                    Level &F = S->lv[cur - 1];
+                    level_spmv(S, F, F.P, F.dP, X[cur], X[cur - 1], false, nullptr);
+                }
+            }
+            cur += down ? 1 : -1;
+        }
+    }
+}
+
+// ---------------------------------------------------------------- preconditioners for PCG
+// One symmetric V-cycle z = M r (zero initial guess on every level; R = P^T is the caller's).
+static void vcycle(mgx_solver_s *S, int l, const double *b, double *x, const int32_t *done)
+{
+    const int L = (int)S->lv.size();
+    Level &V = S->lv[l];
+    hipStream_t s = S->stream;
+    if (l == L - 1) { launch_dense_gemv(V.n, S->coarse_inv.p, b, x, false, done, s); return; }
+    const int nu = S->opt.level_iterations;
+    double *cur;
+    if (S->opt.smoother == MGX_SMOOTHER_GS) {
+        launch_fill_zero(V.n, x, done, s);
+        for (int i = 0; i < nu; ++i) level_gs(S, V, b, x, false, done);
+        cur = x;
+    } else {
+        // 1 initial + (2 nu - 1) sweeps alternate t -> x -> t ...; an odd count ends in x.
+        launch_jacobi_initial(V.n, V.pre.p, b, V.t.p, done, s);
+        cur = V.t.p;
+        for (int i = 1; i < nu; ++i) {
+            double *o = (cur == x) ? V.t.p : x;
+            level_jacobi(S, V, b, cur, o, V.w.p, done);
+            cur = o;
+        }
+    }
+    level_residual(S, V, b, cur, V.r.p, V.w.p, done);
+    Level &C = S->lv[l + 1];
+    level_spmv(S, V, V.R, V.dR, V.r.p, C.b.p, true, done);
+    vcycle(S, l + 1, C.b.p, C.x.p, done);
+    level_spmv(S, V, V.P, V.dP, C.x.p, cur, false, done);
+    if (S->opt.smoother == MGX_SMOOTHER_GS) {
+        for (int i = 0; i < nu; ++i) level_gs(S, V, b, x, true, done);
+    } else {
+        for (int i = 0; i < nu; ++i) {
+            double *o = (cur == x) ? V.t.p : x;
+            level_jacobi(S, V, b, cur, o, V.w.p, done);
+            cur = o;
+        }
+        if (cur != x) MGX_HIP(hipMemcpyAsync(x, cur, sizeof(double) * V.n, hipMemcpyDeviceToDevice, s));
+    }
+}
+
+static void apply_prec(mgx_solver_s *S, const double *r, double *z, const int32_t *done)
+{
+    if (S->opt.method == MGX_METHOD_PCG_JACOBI) {
+        Level &V = S->lv[0];
+        hipLaunchKernelGGL(k_scale_into, dim3(grid_vec(V.n)), dim3(kBlock), 0, S->stream, V.n, V.pre.p, r, z, done);
+        MGX_HIP(hipGetLastError());
+    } else {
+        // level 0 of the V-cycle reads b from r and writes z
+        vcycle(S, 0, r, z, done);
+    }
+}
+
+static void solve_pcg(mgx_solver_s *S, const double *b, double *x, bool has_x0, mgx_stats *st)
+{
+    hipStream_t s = S->stream;
+    Level &V = S->lv[0];
+    const int32_t n = S->n;
+    Ctl *ctl = S->ctl.p;
+    const int32_t *done = &ctl->done;
+    // control block: tolerances
+    Ctl init{};
+    init.stop = S->opt.stop;
+    init.tol = S->opt.tolerance;
+    init.max_it = S->opt.max_iterations;
+    if (S->opt.method == MGX_METHOD_PCG_JACOBI) init.max_it = S->opt.max_iterations + 1;   // "currentIteration > MaxIterations"
+    *S->ctl_host = init;
+    MGX_HIP(hipMemcpyAsync(ctl, S->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, s));
+    // r = b - A x0 (or b), x = x0 (or 0)
+    if (has_x0) {
+        launch_residual(n, V.A.rp, V.A.ci, V.A.v, b, x, S->r.p, V.A.lanes, nullptr, s);
+    } else {
+        MGX_HIP(hipMemcpyAsync(S->r.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
+        launch_fill_zero(n, x, nullptr, s);
+    }
+    hipLaunchKernelGGL(k_pcg_norms, dim3(grid_vec(n)), dim3(kBlock), 0, s, n, b, S->r.p, ctl, S->partials.p);
+    MGX_HIP(hipGetLastError());
+    apply_prec(S, S->r.p, S->z.p, done);
+    launch_dot_beta(n, S->r.p, S->z.p, ctl, S->partials.p, true, s);
+    launch_update_p(n, S->z.p, S->p.p, ctl, true, s);
+    const int chunk = S->check_every;
+    S->prof_used = 0;
+    for (int it = 0;; ++it) {
+        const bool prof = S->opt.profile && (size_t)(2 * S->prof_used + 2) <= S->prof_events.size();
+        if (prof) MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used], s));
+        launch_spmv_dot_alpha(n, V.A.rp, V.A.ci, V.A.v, S->p.p, S->q.p, V.A.lanes, ctl, S->partials.p, s);
+        if (prof) { MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used + 1], s)); ++S->prof_used; }
+        launch_pcg_update(n, S->p.p, S->q.p, x, S->r.p, ctl, S->partials.p, s);
+        apply_prec(S, S->r.p, S->z.p, done);
+        launch_dot_beta(n, S->r.p, S->z.p, ctl, S->partials.p, false, s);
+        launch_update_p(n, S->z.p, S->p.p, ctl, false, s);
+        if ((it + 1) % chunk == 0 || it + 1 >= init.max_it) {
+            MGX_HIP(hipMemcpyAsync(S->ctl_host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+            MGX_HIP(hipStreamSynchronize(s));
+            if (S->ctl_host->done) break;
+            if (it + 1 >= init.max_it + chunk) break;   // safety: the device test stops at max_it
+        }
+    }
+    const Ctl &c = *S->ctl_host;
+    st->converged = c.status;
+    st->iterations = (S->opt.method == MGX_METHOD_PCG_JACOBI) ? std::max(0, c.iter - 1) : c.iter;
+    if (S->opt.profile) {
+        // launches queued after the stop fired return at once; count only the real ones
+        const int real = std::min(S->prof_used, c.iter);
+        double tot = 0;
+        for (int i = 0; i < real; ++i) {
+            float ms = 0;
+            MGX_HIP(hipEventElapsedTime(&ms, S->prof_events[2 * i], S->prof_events[2 * i + 1]));
+            tot += ms;
+        }
+        st->spmv_launches = real;
+        st->spmv_ms = tot;
+    }
+}
+
+static void finish_stats(mgx_solver_s *S, const double *b, const double *x, mgx_stats *st)
+{
+    // true residual ||b - A x|| (one extra SpMV, FP64)
+    Level &V = S->lv[0];
+    hipStream_t s = S->stream;
+    launch_residual(V.n, V.A.rp, V.A.ci, V.A.v, b, x, S->q.p, V.A.lanes, nullptr, s);
+    double *res = S->partials.p + S->partials_tail;
+    launch_dot(V.n, S->q.p, S->q.p, res, S->partials.p, &S->ctl.p->counter[5], s);
+    launch_dot(V.n, b, b, res + 1, S->partials.p, &S->ctl.p->counter[5], s);
+    double h[2];
+    MGX_HIP(hipMemcpyAsync(h, res, sizeof(h), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    st->residual_norm = std::sqrt(h[0]);
+    st->rel_residual = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
+}
+
+static void solve_device(mgx_solver_s *S, const double *b, double *x, bool has_x0, mgx_stats *stats)
+{
+    MGX_REQUIRE(S->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
+    MGX_REQUIRE(stats, MGX_ERR_INVALID_ARGUMENT, "stats must not be NULL");
+    hipStream_t s = S->stream;
+    for (int k = 0; k < S->nrhs; ++k) {
+        const double *bk = b + (size_t)k * S->n;
+        double *xk = x + (size_t)k * S->n;
+        mgx_stats st{};
+        MGX_HIP(hipEventRecord(S->ev0, s));
+        if (S->opt.method == MGX_METHOD_GMG) solve_gmg(S, bk, xk, has_x0, &st);
+        else solve_pcg(S, bk, xk, has_x0, &st);
+        MGX_HIP(hipEventRecord(S->ev1, s));
+        MGX_HIP(hipEventSynchronize(S->ev1));
+        float ms = 0;
+        MGX_HIP(hipEventElapsedTime(&ms, S->ev0, S->ev1));
+        st.solve_ms = ms;
+        finish_stats(S, bk, xk, &st);
+        stats[k] = st;
+    }
+}
+
+}  // namespace mgx
+
+// ====================================================================== C ABI
+extern "C" {
+
+const char *mgx_last_error(void) { return mgx::g_error.c_str(); }
+int32_t mgx_version(void) { return 0x000100; }
+int32_t mgx_device_count(void)
+{
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) return 0;
+    return n;
+}
+
+static int32_t read_nnz(int32_t nrows, const int32_t *rowptr, hipStream_t s)
+{
+    int32_t h[2];
+    MGX_HIP(hipMemcpyAsync(&h[0], rowptr, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipMemcpyAsync(&h[1], rowptr + nrows, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    return h[1] - h[0];
+}
+
+#define MGX_CHECK_ROWS(nrows, rowptr) \
+    MGX_REQUIRE((nrows) >= 0 && ((nrows) == 0 || (rowptr)), MGX_ERR_INVALID_ARGUMENT, "invalid matrix arguments")
+
+int mgx_csr_spmv(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val, const double *x,
+                 double *y, int32_t clear, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_ROWS(nrows, rowptr);
+        if (nrows == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        const int lanes = lanes_for(read_nnz(nrows, rowptr, s), nrows);
+        launch_spmv(nrows, rowptr, col, val, x, y, clear != 0, lanes, nullptr, s);
+    });
+}
+
+int mgx_csr_spmv_f32(int32_t nrows, const int32_t *rowptr, const int32_t *col, const float *val, const float *x,
+                     float *y, int32_t clear, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_ROWS(nrows, rowptr);
+        if (nrows == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        const int lanes = lanes_for(read_nnz(nrows, rowptr, s), nrows);
+        launch_spmv_f32(nrows, rowptr, col, val, x, y, clear != 0, lanes, nullptr, s);
+    });
+}
+
+int mgx_csr_residual(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val, const double *b,
+                     const double *x, double *r, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_ROWS(nrows, rowptr);
+        if (nrows == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        launch_residual(nrows, rowptr, col, val, b, x, r, lanes_for(read_nnz(nrows, rowptr, s), nrows), nullptr, s);
+    });
+}
+
+int mgx_csr_residual_check(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val,
+                           const double *b, const double *x, double *r, double tol, int32_t *flag, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_ROWS(nrows, rowptr);
+        MGX_REQUIRE(flag, MGX_ERR_INVALID_ARGUMENT, "flag must not be NULL");
+        if (nrows == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        launch_residual_flag(nrows, rowptr, col, val, b, x, r, tol, flag, lanes_for(read_nnz(nrows, rowptr, s), nrows), s);
+    });
+}
+
+int mgx_jacobi_initial(int32_t n, const double *pre, const double *b, double *x, void *stream)
+{
+    return guard([&] {
+        MGX_REQUIRE(n >= 0, MGX_ERR_INVALID_ARGUMENT, "negative size");
+        launch_jacobi_initial(n, pre, b, x, nullptr, (hipStream_t)stream);
+    });
+}
+
+int mgx_csr_jacobi(int32_t n, const double *pre, const int32_t *rowptr, const int32_t *col, const double *val,
+                   const double *b, const double *x, double *y, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_ROWS(n, rowptr);
+        if (n == 0) return;
+        hipStream_t s = (hipStream_t)stream;
+        launch_jacobi(n, pre, rowptr, col, val, b, x, y, lanes_for(read_nnz(n, rowptr, s), n), nullptr, s);
+    });
+}
+
+int mgx_csr_gs_color(int32_t nnodes, const int32_t *nodes, const int32_t *rowptr, const int32_t *col,
+                     const double *val, const double *b, double *x, int32_t backward, void *stream)
+{
+    return guard([&] {
+        MGX_REQUIRE(nnodes >= 0, MGX_ERR_INVALID_ARGUMENT, "negative size");
+        launch_gs_color(nnodes, nodes, rowptr, col, val, b, x, backward != 0, nullptr, (hipStream_t)stream);
+    });
+}
+
+int mgx_dot(int32_t n, const double *a, const double *b, double *result, void *stream)
+{
+    return guard([&] {
+        MGX_REQUIRE(n >= 0 && result, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        hipStream_t s = (hipStream_t)stream;
+        DBuf<double> partials; partials.alloc(3 * 2048);
+        DBuf<uint32_t> counter; counter.alloc(1);
+        MGX_HIP(hipMemsetAsync(counter.p, 0, sizeof(uint32_t), s));
+        launch_dot(n, a, b, result, partials.p, counter.p, s);
+        MGX_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int mgx_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t clear, void *stream)
+{
+    return guard([&] {
+        MGX_REQUIRE(A && A->nrows >= 0, MGX_ERR_INVALID_ARGUMENT, "invalid DuVi matrix");
+        MGX_REQUIRE(A->vi_bytes == 0 || A->vi_bytes == 1 || A->vi_bytes == 2 || A->vi_bytes == 4,
+                    MGX_ERR_INVALID_ARGUMENT, "vi_bytes must be 0, 1, 2 or 4");
+        launch_duvi_spmv(*A, x, y, clear != 0, nullptr, (hipStream_t)stream);
+    });
+}
+
+int mgx_duvi_encode(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val, double tol,
+                    int32_t mode, mgx_duvi_host *out)
+{
+    return guard([&] {
+        MGX_REQUIRE(out && nrows >= 0 && rowptr, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        MGX_REQUIRE(mode == 0 || mode == 4 || mode == 8, MGX_ERR_INVALID_ARGUMENT, "mode must be 0, 4 or 8");
+        duvi_encode(nrows, rowptr, col, val, tol, mode, out);
+    });
+}
+
+void mgx_duvi_host_free(mgx_duvi_host *m)
+{
+    if (!m) return;
+    std::free(m->row_blocks); std::free(m->row_dist); std::free(m->block_col); std::free(m->dist);
+    std::free(m->vi); std::free(m->values);
+    std::memset(m, 0, sizeof *m);
+}
+
+int mgx_simp_gradient(int32_t nx, int32_t ny, int32_t nz, double nu, const double *rho, double penal, double Emin,
+                      double E0, const double *u, double *grad, void *stream)
+{
+    return guard([&] {
+        MGX_REQUIRE(nx > 0 && ny > 0 && nz > 0 && rho && u && grad, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        double Ke0[576];
+        ke_hex8(1.0, nu, Ke0);
+        hipStream_t s = (hipStream_t)stream;
+        launch_simp_gradient(nx, ny, nz, Ke0, rho, penal, Emin, E0, u, grad, s);
+        MGX_HIP(hipStreamSynchronize(s));
+    });
+}
+
+// ------------------------------------------------------------------ matrices
+static void assemble_into(DevCsr &A, int nx, int ny, int nz, const double *Ke, const double *dens_dev,
+                          const double *Kel_dev, hipStream_t s)
+{
+    const int64_t nrows = 3LL * nx * (ny + 1) * (nz + 1);
+    const int64_t nnz = cantilever_nnz(nx, ny, nz);
+    MGX_REQUIRE(nrows < INT32_MAX && nnz < INT32_MAX, MGX_ERR_INVALID_ARGUMENT, "mesh too large for int32 CSR");
+    A.nrows = A.ncols = (int32_t)nrows; A.nnz = nnz;
+    A.rp_.alloc(nrows + 1); A.ci_.alloc(nnz); A.v_.alloc(nnz);
+    A.rp = A.rp_.p; A.ci = A.ci_.p; A.v = A.v_.p;
+    A.lanes = lanes_for(nnz, (int32_t)nrows);
+    DBuf<int32_t> len; len.alloc(nrows);
+    launch_assemble_count(nx, ny, nz, len.p, s);
+    exclusive_scan_i32(len.p, A.rp, (int32_t)nrows, s);
+    launch_assemble_fill(nx, ny, nz, Ke, dens_dev, Kel_dev, A.rp, A.ci, A.v, s);
+    MGX_HIP(hipStreamSynchronize(s));
+}
+
+static void upload_host_csr(DevCsr &A, const HostCsr &h, hipStream_t s)
+{
+    mgx_csr_host v{h.nrows, h.ncols, (int64_t)h.col.size(), h.rowptr.data(), h.col.data(), h.val.data()};
+    A.upload(v, s);
+    MGX_HIP(hipStreamSynchronize(s));
+}
+
+int mgx_matrix_cantilever(int32_t nx, int32_t ny, int32_t nz, double E, double nu, const double *dens, mgx_matrix *out)
+{
+    return guard([&] {
+        MGX_REQUIRE(out && nx > 0 && ny > 0 && nz > 0, MGX_ERR_INVALID_ARGUMENT, "invalid mesh");
+        auto m = std::make_unique<mgx_matrix_s>();
+        double Ke[576];
+        ke_hex8(E, nu, Ke);
+        hipStream_t s = nullptr;
+        DBuf<double> d;
+        if (dens) d.upload(dens, (size_t)nx * ny * nz, s);
+        assemble_into(m->A, nx, ny, nz, Ke, dens ? d.p : nullptr, nullptr, s);
+        *out = m.release();
+    });
+}
+
+int mgx_matrix_transfer_reference(int32_t nx, int32_t ny, int32_t nz, mgx_matrix *R, mgx_matrix *P)
+{
+    return guard([&] {
+        MGX_REQUIRE(R && P && nx > 0 && ny > 0 && nz > 0, MGX_ERR_INVALID_ARGUMENT, "invalid mesh");
+        HostCsr hr, hp;
+        transfer_reference(nx, ny, nz, hr, hp);
+        auto r = std::make_unique<mgx_matrix_s>(), p = std::make_unique<mgx_matrix_s>();
+        upload_host_csr(r->A, hr, nullptr);
+        upload_host_csr(p->A, hp, nullptr);
+        *R = r.release(); *P = p.release();
+    });
+}
+
+int mgx_matrix_transfer_trilinear(int32_t nx, int32_t ny, int32_t nz, mgx_matrix *R, mgx_matrix *P)
+{
+    return guard([&] {
+        MGX_REQUIRE(R && P && nx > 1 && ny > 1 && nz > 1 && nx % 2 == 0 && ny % 2 == 0 && nz % 2 == 0,
+                    MGX_ERR_INVALID_ARGUMENT, "trilinear transfer needs even element counts");
+        HostCsr hr, hp;
+        transfer_trilinear(nx, ny, nz, hr, hp);
+        auto r = std::make_unique<mgx_matrix_s>(), p = std::make_unique<mgx_matrix_s>();
+        upload_host_csr(r->A, hr, nullptr);
+        upload_host_csr(p->A, hp, nullptr);
+        *R = r.release(); *P = p.release();
+    });
+}
+
+int mgx_matrix_galerkin_hierarchy(int32_t nx, int32_t ny, int32_t nz, double E, double nu, const double *dens,
+                                  int32_t nlevels, mgx_matrix *levels)
+{
+    return guard([&] {
+        MGX_REQUIRE(levels && nlevels >= 1 && nx > 0 && ny > 0 && nz > 0, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        MGX_REQUIRE(nx % (1 << (nlevels - 1)) == 0 && ny % (1 << (nlevels - 1)) == 0 && nz % (1 << (nlevels - 1)) == 0,
+                    MGX_ERR_INVALID_ARGUMENT, "element counts must be divisible by 2^(nlevels-1)");
+        hipStream_t s = nullptr;
+        double Ke[576];
+        ke_hex8(E, nu, Ke);
+        DBuf<double> d;
+        if (dens) d.upload(dens, (size_t)nx * ny * nz, s);
+        std::vector<std::unique_ptr<mgx_matrix_s>> out(nlevels);
+        out[0] = std::make_unique<mgx_matrix_s>();
+        assemble_into(out[0]->A, nx, ny, nz, Ke, dens ? d.p : nullptr, nullptr, s);
+        DBuf<double> Kprev, Kcur;
+        int fx = nx, fy = ny, fz = nz;
+        DBuf<double> ones;
+        if (!dens) {
+            std::vector<double> h((size_t)nx * ny * nz, 1.0);
+            ones.upload(h.data(), h.size(), s);
+        }
+        for (int l = 1; l < nlevels; ++l) {
+            const int cx = fx / 2, cy = fy / 2, cz = fz / 2;
+            Kcur.alloc((size_t)cx * cy * cz * 576);
+            if (l == 1) launch_galerkin_elements(cx, cy, cz, Ke, dens ? d.p : ones.p, nullptr, Kcur.p, s);
+            else launch_galerkin_elements(cx, cy, cz, nullptr, nullptr, Kprev.p, Kcur.p, s);
+            out[l] = std::make_unique<mgx_matrix_s>();
+            assemble_into(out[l]->A, cx, cy, cz, nullptr, nullptr, Kcur.p, s);
+            std::swap(Kprev.p, Kcur.p); std::swap(Kprev.n, Kcur.n);
+            fx = cx; fy = cy; fz = cz;
+        }
+        for (int l = 0; l < nlevels; ++l) levels[l] = out[l].release();
+    });
+}
+
+int mgx_matrix_from_host(int32_t nrows, int32_t ncols, const int32_t *rowptr, const int32_t *col, const double *val,
+                         mgx_matrix *out)
+{
+    return guard([&] {
+        MGX_REQUIRE(out, MGX_ERR_INVALID_ARGUMENT, "out must not be NULL");
+        mgx_csr_host h{nrows, ncols, rowptr ? (int64_t)rowptr[nrows] : 0, rowptr, col, val};
+        check_csr_host(&h, "matrix");
+        auto m = std::make_unique<mgx_matrix_s>();
+        m->A.upload(h, nullptr);
+        MGX_HIP(hipStreamSynchronize(nullptr));
+        *out = m.release();
+    });
+}
+
+int mgx_matrix_view(mgx_matrix m, mgx_csr_view *view)
+{
+    return guard([&] {
+        MGX_REQUIRE(m && view, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        view->nrows = m->A.nrows; view->ncols = m->A.ncols; view->nnz = m->A.nnz;
+        view->rowptr = m->A.rp; view->col = m->A.ci; view->val = m->A.v;
+    });
+}
+
+int mgx_matrix_download(mgx_matrix m, int32_t *rowptr, int32_t *col, double *val)
+{
+    return guard([&] {
+        MGX_REQUIRE(m && rowptr && col && val, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_HIP(hipMemcpy(rowptr, m->A.rp, sizeof(int32_t) * ((size_t)m->A.nrows + 1), hipMemcpyDeviceToHost));
+        MGX_HIP(hipMemcpy(col, m->A.ci, sizeof(int32_t) * (size_t)m->A.nnz, hipMemcpyDeviceToHost));
+        MGX_HIP(hipMemcpy(val, m->A.v, sizeof(double) * (size_t)m->A.nnz, hipMemcpyDeviceToHost));
+    });
+}
+
+int mgx_matrix_destroy(mgx_matrix m)
+{
+    return guard([&] { delete m; });
+}
+
+// ------------------------------------------------------------------ solver
+void mgx_options_default(mgx_options *o)
+{
+    if (!o) return;
+    std::memset(o, 0, sizeof *o);
+    o->method = MGX_METHOD_MGPCG;
+    o->format = MGX_FORMAT_CSR;
+    o->smoother = MGX_SMOOTHER_JACOBI;
+    o->coarse_relaxation = 1;
+    o->omega = 0.6;
+    o->max_iterations = 10000;
+    o->tolerance = 1e-8;
+    o->stop = MGX_STOP_REL_L2;
+    o->depth = 2;
+    o->level_iterations = 2;
+    o->precision = MGX_PRECISION_FP64;
+    o->nrhs = 1;
+    o->duvi_tolerance = 1e-10;
+    o->duvi_mode = 0;
+    o->inner_iterations = 50;
+    o->inner_tolerance = 1e-4;
+    o->profile = 0;
+    o->check_every = 4;
+}
+
+int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
+{
+    return guard([&] {
+        MGX_REQUIRE(opt && out, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(opt->method >= 0 && opt->method <= 2, MGX_ERR_INVALID_ARGUMENT, "unknown method");
+        MGX_REQUIRE(opt->format == MGX_FORMAT_CSR || opt->format == MGX_FORMAT_DUVI, MGX_ERR_INVALID_ARGUMENT,
+                    "unknown format");
+        MGX_REQUIRE(opt->smoother == MGX_SMOOTHER_JACOBI || opt->smoother == MGX_SMOOTHER_GS, MGX_ERR_INVALID_ARGUMENT,
+                    "unknown smoother");
+        MGX_REQUIRE(opt->level_iterations >= 1, MGX_ERR_INVALID_ARGUMENT, "level_iterations must be >= 1");
+        MGX_REQUIRE(opt->nrhs >= 1, MGX_ERR_INVALID_ARGUMENT, "nrhs must be >= 1");
+        auto s = std::make_unique<mgx_solver_s>();
+        s->opt = *opt;
+        s->nrhs = opt->nrhs;
+        MGX_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
+        MGX_HIP(hipHostMalloc((void **)&s->ctl_host, sizeof(Ctl), hipHostMallocDefault));
+        MGX_HIP(hipHostMalloc((void **)&s->flag_host, sizeof(int32_t), hipHostMallocDefault));
+        MGX_HIP(hipEventCreate(&s->ev0));
+        MGX_HIP(hipEventCreate(&s->ev1));
+        if (opt->check_every > 0) s->check_every = opt->check_every;
+        if (opt->profile) {
+            const int n = std::min(std::max(opt->max_iterations, 1), 4096) + 8;
+            s->prof_events.resize(2 * (size_t)n);
+            for (auto &e : s->prof_events) MGX_HIP(hipEventCreate(&e));
+        }
+        *out = s.release();
+    });
+}
+
+int mgx_solver_set_level(mgx_solver s, int32_t level, const mgx_csr_host *A, const mgx_csr_host *R,
+                         const mgx_csr_host *P)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && level >= 0 && level < 32, MGX_ERR_INVALID_ARGUMENT, "invalid solver or level");
+        check_csr_host(A, "A");
+        if (R) check_csr_host(R, "R");
+        if (P) check_csr_host(P, "P");
+        MGX_REQUIRE(A->nrows == A->ncols, MGX_ERR_INVALID_ARGUMENT, "A must be square");
+        if (R) MGX_REQUIRE(R->ncols == A->nrows, MGX_ERR_INVALID_ARGUMENT, "R columns must equal A rows");
+        if (P) MGX_REQUIRE(P->nrows == A->nrows && (!R || P->ncols == R->nrows), MGX_ERR_INVALID_ARGUMENT,
+                           "P must map the coarse level onto this level");
+        if ((int)s->lv.size() <= level) s->lv.resize(level + 1);
+        Level &V = s->lv[level];
+        stage_host(V.hArp, V.hAci, V.hAv, A);
+        if (R) stage_host(V.hRrp, V.hRci, V.hRv, R); else { V.hRrp.clear(); V.hRci.clear(); V.hRv.clear(); }
+        if (P) stage_host(V.hPrp, V.hPci, V.hPv, P); else { V.hPrp.clear(); V.hPci.clear(); V.hPv.clear(); }
+        V.extA = V.extR = V.extP = nullptr;
+        V.staged = true;
+        s->initialized = false;
+    });
+}
+
+int mgx_solver_set_level_device(mgx_solver s, int32_t level, mgx_matrix A, mgx_matrix R, mgx_matrix P)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && A && level >= 0 && level < 32, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        if ((int)s->lv.size() <= level) s->lv.resize(level + 1);
+        Level &V = s->lv[level];
+        V.extA = A; V.extR = R; V.extP = P;
+        V.staged = false;
+        s->initialized = false;
+    });
+}
+
+int mgx_solver_set_rhs(mgx_solver s, const double *b, int32_t n, int32_t nrhs)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && b && n > 0 && nrhs >= 1, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        MGX_REQUIRE(nrhs == s->nrhs, MGX_ERR_INVALID_ARGUMENT, "nrhs differs from the options");
+        s->b_host.assign(b, b + (size_t)n * nrhs);
+        if (s->initialized) {
+            MGX_REQUIRE(n == s->n, MGX_ERR_INVALID_ARGUMENT, "rhs size differs from the level-0 size");
+            s->bdev.upload(s->b_host.data(), s->b_host.size(), s->stream);
+            MGX_HIP(hipStreamSynchronize(s->stream));
+        }
+    });
+}
+
+int mgx_solver_initialize(mgx_solver s)
+{
+    return guard([&] {
+        MGX_REQUIRE(s, MGX_ERR_INVALID_ARGUMENT, "null solver");
+        initialize(s);
+    });
+}
+
+int mgx_solver_solve(mgx_solver s, double *x, int32_t has_x0, mgx_stats *stats)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && x, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(s->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
+        const size_t total = (size_t)s->n * s->nrhs;
+        if (has_x0) {
+            MGX_HIP(hipMemcpyAsync(s->xdev.p, x, sizeof(double) * total, hipMemcpyHostToDevice, s->stream));
+        }
+        solve_device(s, s->bdev.p, s->xdev.p, has_x0 != 0, stats);
+        MGX_HIP(hipMemcpyAsync(x, s->xdev.p, sizeof(double) * total, hipMemcpyDeviceToHost, s->stream));
+        MGX_HIP(hipStreamSynchronize(s->stream));
+    });
+}
+
+int mgx_solver_solve_device(mgx_solver s, const double *b_dev, double *x_dev, int32_t has_x0, mgx_stats *stats)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && b_dev && x_dev, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        solve_device(s, b_dev, x_dev, has_x0 != 0, stats);
+    });
+}
+
+int mgx_solver_release(mgx_solver s)
+{
+    return guard([&] {
+        if (!s) return;
+        if (s->stream) (void)hipStreamSynchronize(s->stream);
+        if (s->ctl_host) (void)hipHostFree(s->ctl_host);
+        if (s->flag_host) (void)hipHostFree(s->flag_host);
+        if (s->ev0) (void)hipEventDestroy(s->ev0);
+        if (s->ev1) (void)hipEventDestroy(s->ev1);
+        for (auto &e : s->prof_events) (void)hipEventDestroy(e);
+        hipStream_t st = s->stream;
+        delete s;
+        if (st) (void)hipStreamDestroy(st);
+    });
+}
+
+// ------------------------------------------------------------------ plumbing
+int mgx_device_malloc(void **ptr, uint64_t bytes)
+{
+    return guard([&] {
+        MGX_REQUIRE(ptr, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        *ptr = nullptr;
+        MGX_HIP(hipMalloc(ptr, bytes ? bytes : 1));
+    });
+}
+int mgx_device_free(void *ptr)
+{
+    return guard([&] { if (ptr) MGX_HIP(hipFree(ptr)); });
+}
+int mgx_memcpy_htod(void *dst, const void *src, uint64_t bytes)
+{
+    return guard([&] { if (bytes) MGX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice)); });
+}
+int mgx_memcpy_dtoh(void *dst, const void *src, uint64_t bytes)
+{
+    return guard([&] { if (bytes) MGX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost)); });
+}
+int mgx_memcpy_dtod(void *dst, const void *src, uint64_t bytes)
+{
+    return guard([&] { if (bytes) MGX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice)); });
+}
+int mgx_memset(void *dst, int32_t value, uint64_t bytes)
+{
+    return guard([&] { if (bytes) MGX_HIP(hipMemset(dst, value, bytes)); });
+}
+int mgx_device_synchronize(void)
+{
+    return guard([&] { MGX_HIP(hipDeviceSynchronize()); });
+}
+
+int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
+                        const void *x, const void *b, const void *pre, void *y, int32_t warmup, int32_t repeats,
+                        double *avg_ms)
+{
+    return guard([&] {
+        MGX_CHECK_ROWS(nrows, rowptr);
+        MGX_REQUIRE(kind >= 0 && kind <= 3 && repeats >= 1 && warmup >= 0 && avg_ms, MGX_ERR_INVALID_ARGUMENT,
+                    "invalid arguments");
+        MGX_REQUIRE(kind != 1 || (b && pre), MGX_ERR_INVALID_ARGUMENT, "Jacobi needs b and pre");
+        MGX_REQUIRE(kind != 2 || b, MGX_ERR_INVALID_ARGUMENT, "residual needs b");
+        hipStream_t s;
+        MGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t e0, e1;
+        MGX_HIP(hipEventCreate(&e0));
+        MGX_HIP(hipEventCreate(&e1));
+        const int lanes = lanes_for(read_nnz(nrows, rowptr, s), nrows);
+        auto one = [&]() {
+            if (kind == 0) launch_spmv(nrows, rowptr, col, (const double *)val, (const double *)x, (double *)y, true, lanes, nullptr, s);
+            else if (kind == 1) launch_jacobi(nrows, (const double *)pre, rowptr, col, (const double *)val, (const double *)b,
+                                              (const double *)x, (double *)y, lanes, nullptr, s);
+            else if (kind == 2) launch_residual(nrows, rowptr, col, (const double *)val, (const double *)b, (const double *)x,
+                                                (double *)y, lanes, nullptr, s);
+            else launch_spmv_f32(nrows, rowptr, col, (const float *)val, (const float *)x, (float *)y, true, lanes, nullptr, s);
+        };
+        for (int i = 0; i < warmup; ++i) one();
+        MGX_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < repeats; ++i) one();
+        MGX_HIP(hipEventRecord(e1, s));
+        MGX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = ms / repeats;
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    });
+}
+
+int mgx_time_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t warmup, int32_t repeats, double *avg_ms)
+{
+    return guard([&] {
+        MGX_REQUIRE(A && repeats >= 1 && warmup >= 0 && avg_ms, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        hipStream_t s;
+        MGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t e0, e1;
+        MGX_HIP(hipEventCreate(&e0));
+        MGX_HIP(hipEventCreate(&e1));
+        for (int i = 0; i < warmup; ++i) launch_duvi_spmv(*A, x, y, true, nullptr, s);
+        MGX_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < repeats; ++i) launch_duvi_spmv(*A, x, y, true, nullptr, s);
+        MGX_HIP(hipEventRecord(e1, s));
+        MGX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = ms / repeats;
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    });
+}
+
+int mgx_rng_uniform(uint64_t seed, int64_t n, double *out)
+{
+    return guard([&] {
+        MGX_REQUIRE(n >= 0 && (n == 0 || out), MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        rng_uniform(seed, n, out);
+    });
+}
+
+int mgx_rng_normal(uint64_t seed, int64_t n, double *out)
+{
+    return guard([&] {
+        MGX_REQUIRE(n >= 0 && (n == 0 || out), MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        rng_normal(seed, n, out);
+    });
+}
+
+}  // extern "C"

@@ -1,0 +1,127 @@
+// mgx_internal.h -- declarations shared by the HIP kernels, the C-ABI layer and the
+// host-side setup code of libmgx.so.  Not part of the public ABI (include/mgx.h).
+#pragma once
+#include <hip/hip_runtime.h>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "../../include/mgx.h"
+
+namespace mgx {
+
+// ------------------------------------------------------------------ errors
+void set_error(const std::string &msg);
+struct Error {
+    int code;
+    std::string msg;
+};
+#define MGX_HIP(call)                                                                        \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess)                                                                \
+            throw ::mgx::Error{e_ == hipErrorOutOfMemory ? MGX_ERR_OUT_OF_MEMORY : MGX_ERR_DEVICE, \
+                               std::string(#call) + ": " + hipGetErrorString(e_)};            \
+    } while (0)
+#define MGX_REQUIRE(cond, code, msg)                                                         \
+    do { if (!(cond)) throw ::mgx::Error{(code), (msg)}; } while (0)
+
+// ------------------------------------------------------------------ device control block
+// Scalars of one solve that stay on the device (no host round trip per iteration).
+struct Ctl {
+    double rz;        // r . z (current)
+    double pq;        // p . A p
+    double alpha, beta;
+    double rnorm2;    // ||r||^2
+    double rmax;      // max |r_i| (NaN-propagating)
+    double bnorm2;    // ||b||^2
+    double scratch[9];
+    int32_t iter;     // A*p products done
+    int32_t done;     // 1 once the stop test passed or failed; kernels of later iterations return
+    int32_t status;   // 1 converged, -1 failed
+    int32_t stop;     // MGX_STOP_*
+    double tol;
+    int32_t max_it;
+    uint32_t counter[16];   // last-block tickets of grid reductions
+};
+
+// number of partial-sum slots a grid reduction may use
+constexpr int kMaxReduceBlocks = 4096;
+constexpr int kBlock = 256;
+
+// ------------------------------------------------------------------ kernel launchers (mgx_kernels.hip)
+// lanes-per-row chosen from the mean row length
+int lanes_for(int64_t nnz, int32_t nrows);
+
+void launch_spmv(int32_t nrows, const int32_t *rp, const int32_t *ci, const double *v,
+                 const double *x, double *y, bool clear, int lanes, const int32_t *done, hipStream_t s);
+void launch_spmv_f32(int32_t nrows, const int32_t *rp, const int32_t *ci, const float *v,
+                     const float *x, float *y, bool clear, int lanes, const int32_t *done, hipStream_t s);
+void launch_residual(int32_t nrows, const int32_t *rp, const int32_t *ci, const double *v,
+                     const double *b, const double *x, double *r, int lanes, const int32_t *done, hipStream_t s);
+void launch_residual_f32(int32_t nrows, const int32_t *rp, const int32_t *ci, const float *v,
+                         const float *b, const float *x, float *r, int lanes, const int32_t *done, hipStream_t s);
+void launch_residual_flag(int32_t nrows, const int32_t *rp, const int32_t *ci, const double *v,
+                          const double *b, const double *x, double *r, double tol, int32_t *flag,
+                          int lanes, hipStream_t s);
+void launch_jacobi(int32_t nrows, const double *pre, const int32_t *rp, const int32_t *ci, const double *v,
+                   const double *b, const double *x, double *y, int lanes, const int32_t *done, hipStream_t s);
+void launch_jacobi_f32(int32_t nrows, const float *pre, const int32_t *rp, const int32_t *ci, const float *v,
+                       const float *b, const float *x, float *y, int lanes, const int32_t *done, hipStream_t s);
+void launch_jacobi_initial(int32_t n, const double *pre, const double *b, double *x, const int32_t *done, hipStream_t s);
+void launch_jacobi_initial_f32(int32_t n, const float *pre, const float *b, float *x, const int32_t *done, hipStream_t s);
+void launch_gs_color(int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const double *v,
+                     const double *b, double *x, bool backward, const int32_t *done, hipStream_t s);
+void launch_gs_color_f32(int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const float *v,
+                         const float *b, float *x, bool backward, const int32_t *done, hipStream_t s);
+void launch_dot(int32_t n, const double *a, const double *b, double *result, double *partials,
+                uint32_t *counter, hipStream_t s);
+void launch_dense_gemv(int32_t n, const double *M, const double *x, double *y, bool accumulate_into_y,
+                       const int32_t *done, hipStream_t s);
+void launch_duvi_spmv(const mgx_duvi &A, const double *x, double *y, bool clear, const int32_t *done, hipStream_t s);
+void launch_simp_gradient(int32_t nx, int32_t ny, int32_t nz, const double *Ke0, const double *rho, double penal,
+                          double Emin, double E0, const double *u, double *grad, hipStream_t s);
+
+// PCG pieces working on Ctl (all honour ctl->done)
+void launch_pcg_init(int32_t n, const double *b, double *r, double *x, Ctl *ctl, double *partials, hipStream_t s);
+void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, const double *v,
+                           const double *p, double *q, int lanes, Ctl *ctl, double *partials, hipStream_t s);
+void launch_pcg_update(int32_t n, const double *p, const double *q, double *x, double *r,
+                       Ctl *ctl, double *partials, hipStream_t s);
+void launch_dot_beta(int32_t n, const double *r, const double *z, Ctl *ctl, double *partials, bool first, hipStream_t s);
+void launch_update_p(int32_t n, const double *z, double *p, const Ctl *ctl, bool first, hipStream_t s);
+// precision helpers
+void launch_convert_f64_f32(int64_t n, const double *a, float *b, const int32_t *done, hipStream_t s);
+void launch_convert_f32_f64(int64_t n, const float *a, double *b, const int32_t *done, hipStream_t s);
+void launch_fill_zero_f32(int64_t n, float *a, const int32_t *done, hipStream_t s);
+void launch_fill_zero(int64_t n, double *a, const int32_t *done, hipStream_t s);
+void launch_axpy_f32_into_f64(int64_t n, const float *a, double *x, const int32_t *done, hipStream_t s);
+
+// structured assembly
+void launch_assemble_count(int nx, int ny, int nz, int32_t *rowlen, hipStream_t s);
+void launch_assemble_fill(int nx, int ny, int nz, const double *Ke, const double *dens, const double *Kel,
+                          const int32_t *rowptr, int32_t *col, double *val, hipStream_t s);
+void launch_galerkin_elements(int ncx, int ncy, int ncz, const double *Ke0, const double *dens,
+                              const double *Kfine, double *Kcoarse, hipStream_t s);
+void exclusive_scan_i32(const int32_t *in, int32_t *out, int32_t n, hipStream_t s);
+
+// ------------------------------------------------------------------ host setup (mgx_host.cpp)
+void ke_hex8(double E, double nu, double *Ke);
+int64_t cantilever_nnz(int nx, int ny, int nz);
+struct HostCsr {
+    int32_t nrows = 0, ncols = 0;
+    std::vector<int32_t> rowptr, col;
+    std::vector<double> val;
+};
+void transfer_reference(int nx, int ny, int nz, HostCsr &R, HostCsr &P);
+void transfer_trilinear(int nx, int ny, int nz, HostCsr &R, HostCsr &P);
+void csr_transpose(const HostCsr &A, HostCsr &T);
+int block_coloring(int32_t nblocks, int bs, const int32_t *rp, const int32_t *ci, std::vector<int32_t> &color);
+// dense SPD inverse via Cholesky; returns false if not SPD
+bool dense_spd_inverse(int32_t n, std::vector<double> &A);
+int duvi_encode(int32_t nrows, const int32_t *rp, const int32_t *ci, const double *v, double tol, int mode,
+                mgx_duvi_host *out);
+void rng_uniform(uint64_t seed, int64_t n, double *out);
+void rng_normal(uint64_t seed, int64_t n, double *out);
+
+}  // namespace mgx

@@ -1,0 +1,891 @@
+// mgx_kernels.hip -- hand-written gfx950 kernels of the MG-PCG / SpMV path.
+//
+// All kernels are HBM-bound integer/FP64 streaming work (no MFMA).  Design points:
+//  * CSR rows are processed by a group of G lanes of a 64-wide wavefront (G = 4..32 from
+//    the mean row length), so the col/val streams of a row are read as contiguous
+//    G-element segments; the matrix streams use non-temporal loads so that L2 keeps the
+//    gathered vector.
+//  * blockIdx is remapped so that each of the 8 XCDs walks one contiguous slab of rows:
+//    consecutive blocks are dispatched round-robin over the XCDs, and without the remap every
+//    XCD's L2 would fetch the same x window.
+//  * Grid reductions are deterministic: per-block partials, then the last block (agent-scope
+//    acq_rel ticket) sums them in a fixed order and resets the ticket.
+//  * Kernels of the PCG loop take `done` (device flag): once the stop test has fired they
+//    return at once, so the host can queue several iterations per synchronisation.
+#include "mgx_internal.h"
+
+namespace mgx {
+
+// ------------------------------------------------------------------ helpers
+__device__ __forceinline__ int xcd_remap(int b, int nb)
+{
+    // nb is a multiple of 8 for every launch that uses this (see grid_rows)
+    const int per = nb >> 3;
+    return (b & 7) * per + (b >> 3);
+}
+
+static inline int grid_rows(int64_t nrows, int rows_per_block)
+{
+    int64_t nb = (nrows + rows_per_block - 1) / rows_per_block;
+    if (nb < 1) nb = 1;
+    nb = (nb + 7) & ~int64_t(7);
+    return (int)nb;
+}
+
+template <typename T>
+__device__ __forceinline__ T ntload(const T *p) { return __builtin_nontemporal_load(p); }
+
+__device__ __forceinline__ double nanmax(double a, double b)
+{
+    return (a != a) ? a : ((b != b) ? b : (a > b ? a : b));
+}
+
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
+    return v;
+}
+
+// Block reduction of (sum a, sum b, max m); totals valid in thread 0.
+__device__ __forceinline__ void block_reduce3(double &a, double &b, double &m)
+{
+    __shared__ double sh[3][kBlock / 64];
+    a = wave_sum(a); b = wave_sum(b); m = wave_max(m);
+    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
+    __syncthreads();
+    if (l == 0) { sh[0][w] = a; sh[1][w] = b; sh[2][w] = m; }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        double sa = 0, sb = 0, sm = 0;
+        for (int i = 0; i < nw; ++i) { sa += sh[0][i]; sb += sh[1][i]; sm = nanmax(sm, sh[2][i]); }
+        a = sa; b = sb; m = sm;
+    }
+}
+
+// Grid reduction; returns true in thread 0 of the last block, which holds the totals.
+__device__ bool grid_finish3(double &a, double &b, double &m, double *partials, uint32_t *counter)
+{
+    __shared__ int is_last;
+    block_reduce3(a, b, m);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(&partials[3 * blockIdx.x + 0], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[3 * blockIdx.x + 1], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&partials[3 * blockIdx.x + 2], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        is_last = (t == gridDim.x - 1);
+    }
+    __syncthreads();
+    if (!is_last) return false;
+    double sa = 0, sb = 0, sm = 0;
+    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
+        sa += __hip_atomic_load(&partials[3 * i + 0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        sb += __hip_atomic_load(&partials[3 * i + 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        sm = nanmax(sm, __hip_atomic_load(&partials[3 * i + 2], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+    }
+    block_reduce3(sa, sb, sm);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a = sa; b = sb; m = sm;
+        return true;
+    }
+    return false;
+}
+
+// Sum of one CSR row by a group of G lanes; every lane of the group gets the total.
+template <int G, typename T>
+__device__ __forceinline__ T row_dot(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                     const T *__restrict__ v, const T *__restrict__ x, int row, int lane)
+{
+    const int s = rp[row], e = rp[row + 1];
+    T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
+    int k = s + lane;
+    for (; k + 3 * G < e; k += 4 * G) {
+        const int c0 = ntload(ci + k), c1 = ntload(ci + k + G), c2 = ntload(ci + k + 2 * G), c3 = ntload(ci + k + 3 * G);
+        const T v0 = ntload(v + k), v1 = ntload(v + k + G), v2 = ntload(v + k + 2 * G), v3 = ntload(v + k + 3 * G);
+        a0 += v0 * x[c0]; a1 += v1 * x[c1]; a2 += v2 * x[c2]; a3 += v3 * x[c3];
+    }
+    for (; k < e; k += G) a0 += ntload(v + k) * x[ntload(ci + k)];
+    T acc = (a0 + a1) + (a2 + a3);
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
+    return acc;
+}
+
+// ------------------------------------------------------------------ SpMV family
+enum : int { EPI_SPMV = 0, EPI_SPMV_ADD = 1, EPI_RESID = 2, EPI_JACOBI = 3 };
+
+template <int G, int EPI, typename T>
+__global__ __launch_bounds__(kBlock) void k_csr_rows(int32_t nrows, const int32_t *__restrict__ rp,
+                                                     const int32_t *__restrict__ ci, const T *__restrict__ v,
+                                                     const T *__restrict__ x, const T *__restrict__ b,
+                                                     const T *__restrict__ pre, T *__restrict__ y,
+                                                     const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    constexpr int RPB = kBlock / G;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int row = blk * RPB + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    if (row >= nrows) return;   // whole group leaves together
+    const T s = row_dot<G, T>(rp, ci, v, x, row, lane);
+    if (lane == 0) {
+        if (EPI == EPI_SPMV) y[row] = s;
+        else if (EPI == EPI_SPMV_ADD) y[row] = y[row] + s;
+        else if (EPI == EPI_RESID) y[row] = b[row] - s;
+        else y[row] = x[row] + pre[row] * (b[row] - s);
+    }
+}
+
+template <int EPI, typename T>
+static void dispatch_rows(int32_t nrows, const int32_t *rp, const int32_t *ci, const T *v, const T *x,
+                          const T *b, const T *pre, T *y, int lanes, const int32_t *done, hipStream_t s)
+{
+    if (nrows <= 0) return;
+#define MGX_ROWS(G)                                                                                      \
+    hipLaunchKernelGGL((k_csr_rows<G, EPI, T>), dim3(grid_rows(nrows, kBlock / G)), dim3(kBlock), 0, s, \
+                       nrows, rp, ci, v, x, b, pre, y, done)
+    switch (lanes) {
+    case 2: MGX_ROWS(2); break;
+    case 4: MGX_ROWS(4); break;
+    case 8: MGX_ROWS(8); break;
+    case 16: MGX_ROWS(16); break;
+    case 32: MGX_ROWS(32); break;
+    default: MGX_ROWS(64); break;
+    }
+#undef MGX_ROWS
+    MGX_HIP(hipGetLastError());
+}
+
+int lanes_for(int64_t nnz, int32_t nrows)
+{
+    double avg = nrows > 0 ? (double)nnz / nrows : 0;
+    if (avg <= 3) return 2;
+    if (avg <= 6) return 4;
+    if (avg <= 14) return 8;
+    if (avg <= 120) return 16;
+    if (avg <= 240) return 32;
+    return 64;
+}
+
+void launch_spmv(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *x, double *y,
+                 bool clear, int lanes, const int32_t *done, hipStream_t s)
+{
+    if (clear) dispatch_rows<EPI_SPMV, double>(n, rp, ci, v, x, nullptr, nullptr, y, lanes, done, s);
+    else dispatch_rows<EPI_SPMV_ADD, double>(n, rp, ci, v, x, nullptr, nullptr, y, lanes, done, s);
+}
+void launch_spmv_f32(int32_t n, const int32_t *rp, const int32_t *ci, const float *v, const float *x, float *y,
+                     bool clear, int lanes, const int32_t *done, hipStream_t s)
+{
+    if (clear) dispatch_rows<EPI_SPMV, float>(n, rp, ci, v, x, nullptr, nullptr, y, lanes, done, s);
+    else dispatch_rows<EPI_SPMV_ADD, float>(n, rp, ci, v, x, nullptr, nullptr, y, lanes, done, s);
+}
+void launch_residual(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b,
+                     const double *x, double *r, int lanes, const int32_t *done, hipStream_t s)
+{
+    dispatch_rows<EPI_RESID, double>(n, rp, ci, v, x, b, nullptr, r, lanes, done, s);
+}
+void launch_residual_f32(int32_t n, const int32_t *rp, const int32_t *ci, const float *v, const float *b,
+                         const float *x, float *r, int lanes, const int32_t *done, hipStream_t s)
+{
+    dispatch_rows<EPI_RESID, float>(n, rp, ci, v, x, b, nullptr, r, lanes, done, s);
+}
+void launch_jacobi(int32_t n, const double *pre, const int32_t *rp, const int32_t *ci, const double *v,
+                   const double *b, const double *x, double *y, int lanes, const int32_t *done, hipStream_t s)
+{
+    dispatch_rows<EPI_JACOBI, double>(n, rp, ci, v, x, b, pre, y, lanes, done, s);
+}
+void launch_jacobi_f32(int32_t n, const float *pre, const int32_t *rp, const int32_t *ci, const float *v,
+                       const float *b, const float *x, float *y, int lanes, const int32_t *done, hipStream_t s)
+{
+    dispatch_rows<EPI_JACOBI, float>(n, rp, ci, v, x, b, pre, y, lanes, done, s);
+}
+
+// residual_with_check: same flag protocol as CsrGeometricMultigrid.cl:121-141
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_residual_flag(int32_t nrows, const int32_t *__restrict__ rp,
+                                                          const int32_t *__restrict__ ci, const double *__restrict__ v,
+                                                          const double *__restrict__ b, const double *__restrict__ x,
+                                                          double *__restrict__ r, double tol, int32_t *flag)
+{
+    constexpr int RPB = kBlock / G;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int row = blk * RPB + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    if (row >= nrows) return;
+    const double s = row_dot<G, double>(rp, ci, v, x, row, lane);
+    if (lane == 0) {
+        const double ri = b[row] - s;
+        r[row] = ri;
+        const double c = fabs(ri);
+        if (ri != ri || c > 1e50) atomicOr(flag, 2);
+        else if (c > tol) atomicAnd(flag, 2);
+    }
+}
+
+void launch_residual_flag(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b,
+                          const double *x, double *r, double tol, int32_t *flag, int lanes, hipStream_t s)
+{
+    if (n <= 0) return;
+#define MGX_RF(G) hipLaunchKernelGGL((k_residual_flag<G>), dim3(grid_rows(n, kBlock / G)), dim3(kBlock), 0, s, \
+                                     n, rp, ci, v, b, x, r, tol, flag)
+    switch (lanes) {
+    case 2: MGX_RF(2); break;
+    case 4: MGX_RF(4); break;
+    case 8: MGX_RF(8); break;
+    case 16: MGX_RF(16); break;
+    case 32: MGX_RF(32); break;
+    default: MGX_RF(64); break;
+    }
+#undef MGX_RF
+    MGX_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ streaming vector kernels
+static inline int grid_stream(int64_t n)
+{
+    int64_t nb = (n + kBlock * 4 - 1) / (kBlock * 4);
+    if (nb < 1) nb = 1;
+    if (nb > 2048) nb = 2048;
+    return (int)nb;
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_jacobi_initial(int32_t n, const T *__restrict__ pre, const T *__restrict__ b,
+                                                           T *__restrict__ x, const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        x[i] = pre[i] * b[i];
+}
+
+void launch_jacobi_initial(int32_t n, const double *pre, const double *b, double *x, const int32_t *done, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_jacobi_initial<double>, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, pre, b, x, done);
+    MGX_HIP(hipGetLastError());
+}
+void launch_jacobi_initial_f32(int32_t n, const float *pre, const float *b, float *x, const int32_t *done, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_jacobi_initial<float>, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, pre, b, x, done);
+    MGX_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kBlock) void k_dot(int32_t n, const double *__restrict__ a, const double *__restrict__ b,
+                                                double *result, double *partials, uint32_t *counter)
+{
+    double s = 0, z = 0, m = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) s += a[i] * b[i];
+    if (grid_finish3(s, z, m, partials, counter)) *result = s;
+}
+
+void launch_dot(int32_t n, const double *a, const double *b, double *result, double *partials, uint32_t *counter,
+                hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dot, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, a, b, result, partials, counter);
+    MGX_HIP(hipGetLastError());
+}
+
+// PCG: r = b, x = 0, ||b||^2, zero the iteration state
+__global__ __launch_bounds__(kBlock) void k_pcg_init(int32_t n, const double *__restrict__ b, double *__restrict__ r,
+                                                     double *__restrict__ x, Ctl *ctl, double *partials)
+{
+    double s = 0, z = 0, m = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double bi = b[i];
+        r[i] = bi;
+        if (x) x[i] = 0.0;
+        s += bi * bi;
+        m = nanmax(m, fabs(bi));
+    }
+    if (grid_finish3(s, z, m, partials, &ctl->counter[0])) {
+        ctl->bnorm2 = s;
+        ctl->rnorm2 = s;
+        ctl->rmax = m;
+        ctl->iter = 0;
+        int done = 0, status = 0;
+        if (!(m == m) || m > 1e50) { done = 1; status = -1; }
+        else if (ctl->stop == MGX_STOP_REL_L2 ? s == 0.0 : m <= ctl->tol) { done = 1; status = 1; }
+        else if (ctl->max_it <= 0) { done = 1; }
+        ctl->status = status;
+        ctl->done = done;
+    }
+}
+
+void launch_pcg_init(int32_t n, const double *b, double *r, double *x, Ctl *ctl, double *partials, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_init, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, b, r, x, ctl, partials);
+    MGX_HIP(hipGetLastError());
+}
+
+// q = A p and p.q -> alpha = rz / pq
+template <int G>
+__global__ __launch_bounds__(kBlock) void k_spmv_dot_alpha(int32_t nrows, const int32_t *__restrict__ rp,
+                                                           const int32_t *__restrict__ ci, const double *__restrict__ v,
+                                                           const double *__restrict__ p, double *__restrict__ q,
+                                                           Ctl *ctl, double *partials)
+{
+    if (ctl->done) return;
+    constexpr int RPB = kBlock / G;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int row = blk * RPB + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    double contrib = 0, z = 0, m = 0;
+    if (row < nrows) {
+        const double s = row_dot<G, double>(rp, ci, v, p, row, lane);
+        if (lane == 0) { q[row] = s; contrib = p[row] * s; }
+    }
+    if (grid_finish3(contrib, z, m, partials, &ctl->counter[1])) {
+        ctl->pq = contrib;
+        ctl->alpha = ctl->rz / contrib;
+    }
+}
+
+void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *p,
+                           double *q, int lanes, Ctl *ctl, double *partials, hipStream_t s)
+{
+#define MGX_SDA(G) hipLaunchKernelGGL((k_spmv_dot_alpha<G>), dim3(grid_rows(n, kBlock / G)), dim3(kBlock), 0, s, \
+                                      n, rp, ci, v, p, q, ctl, partials)
+    switch (lanes) {
+    case 2: MGX_SDA(2); break;
+    case 4: MGX_SDA(4); break;
+    case 8: MGX_SDA(8); break;
+    case 16: MGX_SDA(16); break;
+    case 32: MGX_SDA(32); break;
+    default: MGX_SDA(64); break;
+    }
+#undef MGX_SDA
+    MGX_HIP(hipGetLastError());
+}
+
+// x += alpha p; r -= alpha q; ||r||^2, max|r|; stop test
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int32_t n, const double *__restrict__ p,
+                                                       const double *__restrict__ q, double *__restrict__ x,
+                                                       double *__restrict__ r, Ctl *ctl, double *partials)
+{
+    if (ctl->done) return;
+    const double alpha = ctl->alpha;
+    double s = 0, z = 0, m = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        x[i] = x[i] + alpha * p[i];
+        const double ri = r[i] - alpha * q[i];
+        r[i] = ri;
+        s += ri * ri;
+        m = nanmax(m, fabs(ri));
+    }
+    if (grid_finish3(s, z, m, partials, &ctl->counter[2])) {
+        ctl->rnorm2 = s;
+        ctl->rmax = m;
+        const int it = ctl->iter + 1;
+        ctl->iter = it;
+        int done = 0, status = 0;
+        if (!(m == m) || m > 1e50 || !(s == s)) { done = 1; status = -1; }
+        else if (ctl->stop == MGX_STOP_REL_L2 ? (s <= ctl->tol * ctl->tol * ctl->bnorm2) : (m <= ctl->tol)) {
+            done = 1; status = 1;
+        } else if (it >= ctl->max_it) done = 1;
+        ctl->status = status;
+        ctl->done = done;
+    }
+}
+
+void launch_pcg_update(int32_t n, const double *p, const double *q, double *x, double *r, Ctl *ctl,
+                       double *partials, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_update, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, p, q, x, r, ctl, partials);
+    MGX_HIP(hipGetLastError());
+}
+
+// rz_new = r . z; beta = rz_new / rz (first: only store rz)
+__global__ __launch_bounds__(kBlock) void k_dot_beta(int32_t n, const double *__restrict__ r, const double *__restrict__ z,
+                                                     Ctl *ctl, double *partials, int first)
+{
+    if (ctl->done) return;
+    double s = 0, u = 0, m = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) s += r[i] * z[i];
+    if (grid_finish3(s, u, m, partials, &ctl->counter[3])) {
+        if (!first) ctl->beta = s / ctl->rz;
+        ctl->rz = s;
+    }
+}
+
+void launch_dot_beta(int32_t n, const double *r, const double *z, Ctl *ctl, double *partials, bool first, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_dot_beta, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, r, z, ctl, partials, first ? 1 : 0);
+    MGX_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kBlock) void k_update_p(int32_t n, const double *__restrict__ z, double *__restrict__ p,
+                                                     const Ctl *ctl, int first)
+{
+    if (ctl->done) return;
+    const double beta = first ? 0.0 : ctl->beta;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        p[i] = first ? z[i] : z[i] + beta * p[i];
+}
+
+void launch_update_p(int32_t n, const double *z, double *p, const Ctl *ctl, bool first, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_update_p, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, z, p, ctl, first ? 1 : 0);
+    MGX_HIP(hipGetLastError());
+}
+
+template <typename A, typename B>
+__global__ __launch_bounds__(kBlock) void k_convert(int64_t n, const A *__restrict__ a, B *__restrict__ b,
+                                                    const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) b[i] = (B)a[i];
+}
+void launch_convert_f64_f32(int64_t n, const double *a, float *b, const int32_t *done, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_convert<double, float>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, a, b, done);
+    MGX_HIP(hipGetLastError());
+}
+void launch_convert_f32_f64(int64_t n, const float *a, double *b, const int32_t *done, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_convert<float, double>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, a, b, done);
+    MGX_HIP(hipGetLastError());
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_fill_zero(int64_t n, T *__restrict__ a, const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) a[i] = T(0);
+}
+void launch_fill_zero_f32(int64_t n, float *a, const int32_t *done, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_zero<float>, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, a, done);
+    MGX_HIP(hipGetLastError());
+}
+void launch_fill_zero(int64_t n, double *a, const int32_t *done, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_fill_zero<double>, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, a, done);
+    MGX_HIP(hipGetLastError());
+}
+
+__global__ __launch_bounds__(kBlock) void k_axpy_f32_into_f64(int64_t n, const float *__restrict__ a,
+                                                              double *__restrict__ x, const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        x[i] = x[i] + (double)a[i];
+}
+void launch_axpy_f32_into_f64(int64_t n, const float *a, double *x, const int32_t *done, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_axpy_f32_into_f64, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, a, x, done);
+    MGX_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ multicolour Gauss-Seidel
+// One node (3 rows) per group of 16 lanes.  s_a = sum over columns outside the node,
+// d_ab = the node's own 3x3 block; lane 0 then relaxes the three dofs in order.
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_gs_color(int32_t nnodes, const int32_t *__restrict__ nodes,
+                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                     const T *__restrict__ v, const T *__restrict__ b, T *__restrict__ x,
+                                                     int backward, const int32_t *__restrict__ done)
+{
+    constexpr int G = 16;
+    if (done && *done) return;
+    const int grp = blockIdx.x * (kBlock / G) + threadIdx.x / G;
+    const int lane = threadIdx.x % G;
+    if (grp >= nnodes) return;
+    const int node = nodes[grp];
+    const int r0 = 3 * node;
+    T s[3], d[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        s[a] = 0; d[a][0] = d[a][1] = d[a][2] = 0;
+        const int e = rp[r0 + a + 1];
+        for (int k = rp[r0 + a] + lane; k < e; k += G) {
+            const int c = ci[k];
+            const T val = v[k];
+            const unsigned off = (unsigned)(c - r0);
+            if (off < 3u) {
+                if (off == 0) d[a][0] += val; else if (off == 1) d[a][1] += val; else d[a][2] += val;
+            } else s[a] += val * x[c];
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) {
+            s[a] += __shfl_xor(s[a], o, G);
+            d[a][0] += __shfl_xor(d[a][0], o, G);
+            d[a][1] += __shfl_xor(d[a][1], o, G);
+            d[a][2] += __shfl_xor(d[a][2], o, G);
+        }
+    }
+    if (lane == 0) {
+        T x0 = x[r0], x1 = x[r0 + 1], x2 = x[r0 + 2];
+        const T b0 = b[r0] - s[0], b1 = b[r0 + 1] - s[1], b2 = b[r0 + 2] - s[2];
+        if (!backward) {
+            x0 = (b0 - d[0][1] * x1 - d[0][2] * x2) / d[0][0];
+            x1 = (b1 - d[1][0] * x0 - d[1][2] * x2) / d[1][1];
+            x2 = (b2 - d[2][0] * x0 - d[2][1] * x1) / d[2][2];
+        } else {
+            x2 = (b2 - d[2][0] * x0 - d[2][1] * x1) / d[2][2];
+            x1 = (b1 - d[1][0] * x0 - d[1][2] * x2) / d[1][1];
+            x0 = (b0 - d[0][1] * x1 - d[0][2] * x2) / d[0][0];
+        }
+        x[r0] = x0; x[r0 + 1] = x1; x[r0 + 2] = x2;
+    }
+}
+
+void launch_gs_color(int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const double *v,
+                     const double *b, double *x, bool backward, const int32_t *done, hipStream_t s)
+{
+    if (nnodes <= 0) return;
+    const int nb = (nnodes + (kBlock / 16) - 1) / (kBlock / 16);
+    hipLaunchKernelGGL(k_gs_color<double>, dim3(nb), dim3(kBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
+                       backward ? 1 : 0, done);
+    MGX_HIP(hipGetLastError());
+}
+void launch_gs_color_f32(int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const float *v,
+                         const float *b, float *x, bool backward, const int32_t *done, hipStream_t s)
+{
+    if (nnodes <= 0) return;
+    const int nb = (nnodes + (kBlock / 16) - 1) / (kBlock / 16);
+    hipLaunchKernelGGL(k_gs_color<float>, dim3(nb), dim3(kBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
+                       backward ? 1 : 0, done);
+    MGX_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ dense coarse solve: y (+)= M x
+__global__ __launch_bounds__(kBlock) void k_dense_gemv(int32_t n, const double *__restrict__ M,
+                                                       const double *__restrict__ x, double *__restrict__ y,
+                                                       int accumulate, const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double *Mr = M + (int64_t)row * n;
+    double s = 0;
+    for (int j = lane; j < n; j += 64) s += Mr[j] * x[j];
+    s = wave_sum(s);
+    if (lane == 0) y[row] = accumulate ? y[row] + s : s;
+}
+
+void launch_dense_gemv(int32_t n, const double *M, const double *x, double *y, bool acc, const int32_t *done,
+                       hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_dense_gemv, dim3((n + 3) / 4), dim3(kBlock), 0, s, n, M, x, y, acc ? 1 : 0, done);
+    MGX_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ DuVi block layout SpMV
+// Decode as DuViGeometricMultigrid.cl:57-99; one row per lane.
+template <int VIB>
+__device__ __forceinline__ double duvi_value(const void *vi, const double *values, int64_t d)
+{
+    if (VIB == 1) return values[((const uint8_t *)vi)[d]];
+    if (VIB == 2) return values[((const uint16_t *)vi)[d]];
+    if (VIB == 4) return values[((const uint32_t *)vi)[d]];
+    return values[d];
+}
+
+template <int VIB>
+__global__ __launch_bounds__(kBlock) void k_duvi_spmv(int32_t nrows, const int32_t *__restrict__ row_blocks,
+                                                      const int32_t *__restrict__ row_dist,
+                                                      const int32_t *__restrict__ block_col,
+                                                      const uint8_t *__restrict__ dist, const void *__restrict__ vi,
+                                                      const double *__restrict__ values, const double *__restrict__ x,
+                                                      double *__restrict__ y, int clear, const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    const int blk = xcd_remap(blockIdx.x, gridDim.x);
+    const int row = blk * kBlock + threadIdx.x;
+    if (row >= nrows) return;
+    double acc = clear ? 0.0 : y[row];
+    int64_t d = row_dist[row];
+    const int b1 = row_blocks[row + 1];
+    for (int b = row_blocks[row]; b < b1; ++b) {
+        int col = block_col[b];
+        const int64_t dend = d + dist[d] + 1;
+        for (;;) {
+            acc += duvi_value<VIB>(vi, values, d) * x[col];
+            ++d;
+            if (d == dend) break;
+            col += dist[d] + 1;
+        }
+    }
+    y[row] = acc;
+}
+
+void launch_duvi_spmv(const mgx_duvi &A, const double *x, double *y, bool clear, const int32_t *done, hipStream_t s)
+{
+    if (A.nrows <= 0) return;
+    const dim3 g(grid_rows(A.nrows, kBlock)), bl(kBlock);
+#define MGX_DV(V) hipLaunchKernelGGL(k_duvi_spmv<V>, g, bl, 0, s, A.nrows, A.row_blocks, A.row_dist, A.block_col, \
+                                     A.dist, A.vi, A.values, x, y, clear ? 1 : 0, done)
+    switch (A.vi_bytes) {
+    case 1: MGX_DV(1); break;
+    case 2: MGX_DV(2); break;
+    case 4: MGX_DV(4); break;
+    default: MGX_DV(0); break;
+    }
+#undef MGX_DV
+    MGX_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ SIMP adjoint
+__constant__ double c_Ke0[576];
+
+__global__ __launch_bounds__(kBlock) void k_simp_gradient(int nx, int ny, int nz, const double *__restrict__ rho,
+                                                          double penal, double Emin, double E0,
+                                                          const double *__restrict__ u, double *__restrict__ grad)
+{
+    const int64_t e = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const int64_t ne = (int64_t)nx * ny * nz;
+    if (e >= ne) return;
+    const int ex = (int)(e % nx), ey = (int)((e / nx) % ny), ez = (int)(e / ((int64_t)nx * ny));
+    const int ox[8] = {0, 1, 1, 0, 0, 1, 1, 0}, oy[8] = {0, 0, 1, 1, 0, 0, 1, 1}, oz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    double ue[24];
+#pragma unroll
+    for (int l = 0; l < 8; ++l) {
+        const int jx = ex + ox[l], jy = ey + oy[l], jz = ez + oz[l];
+        const int64_t fn = ((int64_t)jz * (ny + 1) + jy) * nx + (jx - 1);
+#pragma unroll
+        for (int b = 0; b < 3; ++b) ue[3 * l + b] = jx == 0 ? 0.0 : u[3 * fn + b];
+    }
+    double ce = 0;
+    for (int i = 0; i < 24; ++i) {
+        double s = 0;
+#pragma unroll
+        for (int j = 0; j < 24; ++j) s += c_Ke0[i * 24 + j] * ue[j];
+        ce += ue[i] * s;
+    }
+    grad[e] = -penal * pow(rho[e], penal - 1) * (E0 - Emin) * ce;
+}
+
+void launch_simp_gradient(int32_t nx, int32_t ny, int32_t nz, const double *Ke0, const double *rho, double penal,
+                          double Emin, double E0, const double *u, double *grad, hipStream_t s)
+{
+    MGX_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_Ke0), Ke0, sizeof(double) * 576, 0, hipMemcpyHostToDevice, s));
+    const int64_t ne = (int64_t)nx * ny * nz;
+    hipLaunchKernelGGL(k_simp_gradient, dim3((ne + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nx, ny, nz, rho, penal,
+                       Emin, E0, u, grad);
+    MGX_HIP(hipGetLastError());
+}
+
+// ------------------------------------------------------------------ structured assembly
+// Row (node (ix,iy,iz) with ix >= 1, dof a) of K_ff in sorted CSR; value of column (node j,
+// dof b) = sum over the elements holding both nodes, in increasing element index, of
+// rho_e * Ke[3 li + a][3 lj + b] (or of Kel_e[...]); sums start at 0.0 and add each
+// contribution as `c + acc`, as DokRowMajor.AddSubmatrix does (FemCantileverBase.cs:95-101).
+__constant__ double c_Ke[576];
+
+__device__ __forceinline__ int loc_index(int ox, int oy, int oz)
+{
+    // (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),(1,0,1),(1,1,1),(0,1,1)
+    return oz * 4 + (oy ? (ox ? 2 : 3) : (ox ? 1 : 0));
+}
+
+__device__ __forceinline__ int axis_count(int i, int lo, int hi)
+{
+    int c = 0;
+    for (int d = -1; d <= 1; ++d) c += (i + d >= lo && i + d <= hi);
+    return c;
+}
+
+__global__ __launch_bounds__(kBlock) void k_assemble_count(int nx, int ny, int nz, int32_t *rowlen)
+{
+    const int64_t row = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const int64_t nrows = 3LL * nx * (ny + 1) * (nz + 1);
+    if (row >= nrows) return;
+    const int64_t fn = row / 3;
+    const int ix = (int)(fn % nx) + 1;
+    const int iy = (int)((fn / nx) % (ny + 1));
+    const int iz = (int)(fn / ((int64_t)nx * (ny + 1)));
+    rowlen[row] = 3 * axis_count(ix, 1, nx) * axis_count(iy, 0, ny) * axis_count(iz, 0, nz);
+}
+
+__global__ __launch_bounds__(kBlock) void k_assemble_fill(int nx, int ny, int nz, const double *__restrict__ dens,
+                                                          const double *__restrict__ Kel,
+                                                          const int32_t *__restrict__ rowptr, int32_t *__restrict__ col,
+                                                          double *__restrict__ val)
+{
+#pragma clang fp contract(off)
+    const int64_t row = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    const int64_t nrows = 3LL * nx * (ny + 1) * (nz + 1);
+    if (row >= nrows) return;
+    const int a = (int)(row % 3);
+    const int64_t fn = row / 3;
+    const int ix = (int)(fn % nx) + 1;
+    const int iy = (int)((fn / nx) % (ny + 1));
+    const int iz = (int)(fn / ((int64_t)nx * (ny + 1)));
+    int64_t pos = rowptr[row];
+    for (int dz = -1; dz <= 1; ++dz) {
+        const int jz = iz + dz;
+        if (jz < 0 || jz > nz) continue;
+        for (int dy = -1; dy <= 1; ++dy) {
+            const int jy = iy + dy;
+            if (jy < 0 || jy > ny) continue;
+            for (int dx = -1; dx <= 1; ++dx) {
+                const int jx = ix + dx;
+                if (jx < 1 || jx > nx) continue;
+                const int64_t fj = ((int64_t)jz * (ny + 1) + jy) * nx + (jx - 1);
+                int ex0, ey0, ez0, cx, cy, cz;
+                if (dx == 0) { ex0 = ix - 1 < 0 ? 0 : ix - 1; cx = (ix > nx - 1 ? nx - 1 : ix) - ex0 + 1; } else { ex0 = dx > 0 ? ix : ix - 1; cx = 1; }
+                if (dy == 0) { ey0 = iy - 1 < 0 ? 0 : iy - 1; cy = (iy > ny - 1 ? ny - 1 : iy) - ey0 + 1; } else { ey0 = dy > 0 ? iy : iy - 1; cy = 1; }
+                if (dz == 0) { ez0 = iz - 1 < 0 ? 0 : iz - 1; cz = (iz > nz - 1 ? nz - 1 : iz) - ez0 + 1; } else { ez0 = dz > 0 ? iz : iz - 1; cz = 1; }
+                for (int b = 0; b < 3; ++b) {
+                    double acc = 0.0;
+                    for (int ez = ez0; ez < ez0 + cz; ++ez)
+                        for (int ey = ey0; ey < ey0 + cy; ++ey)
+                            for (int ex = ex0; ex < ex0 + cx; ++ex) {
+                                const int64_t e = ex + (int64_t)nx * (ey + (int64_t)ny * ez);
+                                const int li = loc_index(ix - ex, iy - ey, iz - ez);
+                                const int lj = loc_index(jx - ex, jy - ey, jz - ez);
+                                const int kk = (3 * li + a) * 24 + 3 * lj + b;
+                                double c;
+                                if (Kel) c = Kel[e * 576 + kk];
+                                else c = (dens ? dens[e] : 1.0) * c_Ke[kk];
+                                acc = c + acc;
+                            }
+                    col[pos] = (int32_t)(3 * fj + b);
+                    val[pos] = acc;
+                    ++pos;
+                }
+            }
+        }
+    }
+}
+
+void launch_assemble_count(int nx, int ny, int nz, int32_t *rowlen, hipStream_t s)
+{
+    const int64_t nrows = 3LL * nx * (ny + 1) * (nz + 1);
+    hipLaunchKernelGGL(k_assemble_count, dim3((nrows + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nx, ny, nz, rowlen);
+    MGX_HIP(hipGetLastError());
+}
+
+void launch_assemble_fill(int nx, int ny, int nz, const double *Ke, const double *dens, const double *Kel,
+                          const int32_t *rowptr, int32_t *col, double *val, hipStream_t s)
+{
+    if (Ke) MGX_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_Ke), Ke, sizeof(double) * 576, 0, hipMemcpyHostToDevice, s));
+    const int64_t nrows = 3LL * nx * (ny + 1) * (nz + 1);
+    hipLaunchKernelGGL(k_assemble_fill, dim3((nrows + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nx, ny, nz, dens, Kel,
+                       rowptr, col, val);
+    MGX_HIP(hipGetLastError());
+}
+
+// exclusive scan of int32 (single-block chunked scan: chunk sums, then offsets)
+__global__ __launch_bounds__(1024) void k_scan_chunks(const int32_t *__restrict__ in, int32_t *__restrict__ out,
+                                                      int32_t n, int32_t chunk, int64_t *__restrict__ sums)
+{
+    // each thread scans one chunk sequentially
+    const int t = blockIdx.x * 1024 + threadIdx.x;
+    const int64_t s = (int64_t)t * chunk;
+    if (s >= n) return;
+    int64_t e = s + chunk; if (e > n) e = n;
+    int64_t acc = 0;
+    for (int64_t i = s; i < e; ++i) { out[i] = (int32_t)acc; acc += in[i]; }
+    sums[t] = acc;
+}
+__global__ void k_scan_sums(int64_t *sums, int nchunks, int32_t *total_out, int32_t n)
+{
+    if (threadIdx.x || blockIdx.x) return;
+    int64_t acc = 0;
+    for (int i = 0; i < nchunks; ++i) { int64_t v = sums[i]; sums[i] = acc; acc += v; }
+    total_out[n] = (int32_t)acc;
+}
+__global__ __launch_bounds__(kBlock) void k_scan_add(int32_t *out, int32_t n, int32_t chunk, const int64_t *sums)
+{
+    const int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    if (i >= n) return;
+    out[i] = (int32_t)(out[i] + sums[i / chunk]);
+}
+
+void exclusive_scan_i32(const int32_t *in, int32_t *out, int32_t n, hipStream_t s)
+{
+    // out has n+1 entries; out[n] = total
+    const int32_t chunk = 256;
+    const int nchunks = (n + chunk - 1) / chunk;
+    int64_t *sums = nullptr;
+    MGX_HIP(hipMalloc((void **)&sums, sizeof(int64_t) * (nchunks > 0 ? nchunks : 1)));
+    if (n > 0) {
+        hipLaunchKernelGGL(k_scan_chunks, dim3((nchunks + 1023) / 1024), dim3(1024), 0, s, in, out, n, chunk, sums);
+        MGX_HIP(hipGetLastError());
+    }
+    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, sums, nchunks, out, n);
+    MGX_HIP(hipGetLastError());
+    if (n > 0) {
+        hipLaunchKernelGGL(k_scan_add, dim3((n + kBlock - 1) / kBlock), dim3(kBlock), 0, s, out, n, chunk, sums);
+        MGX_HIP(hipGetLastError());
+    }
+    MGX_HIP(hipStreamSynchronize(s));
+    MGX_HIP(hipFree(sums));
+}
+
+// ------------------------------------------------------------------ Galerkin element coarsening
+// Coarse element E (ncx x ncy x ncz) holds children c = (cx,cy,cz) in {0,1}^3, fine element
+// (2X+cx, 2Y+cy, 2Z+cz).  Fine corner l of child c sits at (c + off(l)) / 2 in the coarse
+// element; its trilinear weight from coarse corner L is t_c[l][L] = prod_axis (1 - |p - L|).
+// Kc = sum_c (t_c x I3)^T K_c (t_c x I3), K_c = rho_c Ke0 (level 0) or Kfine[c].
+__global__ __launch_bounds__(kBlock) void k_galerkin_elements(int ncx, int ncy, int ncz,
+                                                              const double *__restrict__ dens,
+                                                              const double *__restrict__ Kfine,
+                                                              double *__restrict__ Kcoarse)
+{
+    __shared__ double Kc[576];
+    __shared__ double W[576];
+    __shared__ double acc[576];
+    const int64_t E = blockIdx.x;
+    const int X = (int)(E % ncx), Y = (int)((E / ncx) % ncy), Z = (int)(E / ((int64_t)ncx * ncy));
+    const int nfx = 2 * ncx, nfy = 2 * ncy;
+    const int ox[8] = {0, 1, 1, 0, 0, 1, 1, 0}, oy[8] = {0, 0, 1, 1, 0, 0, 1, 1}, oz[8] = {0, 0, 0, 0, 1, 1, 1, 1};
+    for (int i = threadIdx.x; i < 576; i += kBlock) acc[i] = 0;
+    for (int c = 0; c < 8; ++c) {
+        const int fx = 2 * X + ox[c], fy = 2 * Y + oy[c], fz = 2 * Z + oz[c];
+        const int64_t fe = fx + (int64_t)nfx * (fy + (int64_t)nfy * fz);
+        __syncthreads();
+        for (int i = threadIdx.x; i < 576; i += kBlock)
+            Kc[i] = Kfine ? Kfine[fe * 576 + i] : dens[fe] * c_Ke[i];
+        __syncthreads();
+        // W[3l+a][3M+b] = sum_m K[3l+a][3m+b] t[m][M]
+        for (int i = threadIdx.x; i < 576; i += kBlock) {
+            const int r = i / 24, q = i % 24, M = q / 3, bb = q % 3;
+            double s = 0;
+            for (int m = 0; m < 8; ++m) {
+                const double px = 0.5 * (ox[c] + ox[m]), py = 0.5 * (oy[c] + oy[m]), pz = 0.5 * (oz[c] + oz[m]);
+                const double t = (1 - fabs(px - ox[M])) * (1 - fabs(py - oy[M])) * (1 - fabs(pz - oz[M]));
+                if (t != 0) s += Kc[r * 24 + 3 * m + bb] * t;
+            }
+            W[i] = s;
+        }
+        __syncthreads();
+        // acc[3L+a][3M+b] += sum_l t[l][L] W[3l+a][3M+b]
+        for (int i = threadIdx.x; i < 576; i += kBlock) {
+            const int r = i / 24, q = i % 24, L = r / 3, aa = r % 3;
+            double s = 0;
+            for (int l = 0; l < 8; ++l) {
+                const double px = 0.5 * (ox[c] + ox[l]), py = 0.5 * (oy[c] + oy[l]), pz = 0.5 * (oz[c] + oz[l]);
+                const double t = (1 - fabs(px - ox[L])) * (1 - fabs(py - oy[L])) * (1 - fabs(pz - oz[L]));
+                if (t != 0) s += t * W[(3 * l + aa) * 24 + q];
+            }
+            acc[i] += s;
+        }
+    }
+    __syncthreads();
+    for (int i = threadIdx.x; i < 576; i += kBlock) Kcoarse[E * 576 + i] = acc[i];
+}
+
+void launch_galerkin_elements(int ncx, int ncy, int ncz, const double *Ke0, const double *dens,
+                              const double *Kfine, double *Kcoarse, hipStream_t s)
+{
+    if (Ke0) MGX_HIP(hipMemcpyToSymbolAsync(HIP_SYMBOL(c_Ke), Ke0, sizeof(double) * 576, 0, hipMemcpyHostToDevice, s));
+    const int64_t ne = (int64_t)ncx * ncy * ncz;
+    hipLaunchKernelGGL(k_galerkin_elements, dim3(ne), dim3(kBlock), 0, s, ncx, ncy, ncz, dens, Kfine, Kcoarse);
+    MGX_HIP(hipGetLastError());
+}
+
+}  // namespace mgx

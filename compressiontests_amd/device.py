@@ -1,0 +1,213 @@
+"""Device buffers and device-resident matrices for the Python side of the C ABI."""
+import ctypes as C
+from collections import namedtuple
+
+import numpy as np
+
+from . import _lib as L
+
+CsrMatrix = namedtuple("CsrMatrix", "rowptr col val nrows ncols")
+"""Host CSR (int32 row offsets / columns, float64 values), the layout of
+DokRowMajor.BuildCsrArrays(sorted) (DokRowMajor.cs:286-324)."""
+
+
+class DeviceArray:
+    """A device allocation owned by Python (mgx_device_malloc / mgx_device_free)."""
+
+    def __init__(self, n, dtype=np.float64):
+        self.dtype = np.dtype(dtype)
+        self.n = int(n)
+        self.nbytes = self.n * self.dtype.itemsize
+        p = C.c_void_p()
+        L.call("mgx_device_malloc", C.byref(p), C.c_uint64(self.nbytes))
+        self.ptr = p.value
+
+    @classmethod
+    def from_numpy(cls, a, dtype=None):
+        a = np.ascontiguousarray(a, dtype=dtype)
+        d = cls(a.size, a.dtype)
+        if a.size:
+            L.call("mgx_memcpy_htod", C.c_void_p(d.ptr), a.ctypes.data_as(C.c_void_p), C.c_uint64(a.nbytes))
+        return d
+
+    @classmethod
+    def zeros(cls, n, dtype=np.float64):
+        d = cls(n, dtype)
+        L.call("mgx_memset", C.c_void_p(d.ptr), 0, C.c_uint64(d.nbytes))
+        return d
+
+    def numpy(self):
+        out = np.empty(self.n, self.dtype)
+        if self.n:
+            L.call("mgx_memcpy_dtoh", out.ctypes.data_as(C.c_void_p), C.c_void_p(self.ptr), C.c_uint64(self.nbytes))
+        return out
+
+    def copy_from(self, a):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        assert a.size == self.n
+        L.call("mgx_memcpy_htod", C.c_void_p(self.ptr), a.ctypes.data_as(C.c_void_p), C.c_uint64(a.nbytes))
+
+    @property
+    def vp(self):
+        return C.c_void_p(self.ptr)
+
+    def free(self):
+        if getattr(self, "ptr", None):
+            L.lib().mgx_device_free(C.c_void_p(self.ptr))
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+class DeviceCsr:
+    """CSR arrays on the device, either uploaded by Python (arrays owned here) or owned by
+    the library (an mgx_matrix handle from the structured generators)."""
+
+    def __init__(self, handle=None, arrays=None, nrows=0, ncols=0, nnz=0):
+        self.handle = handle
+        self._arrays = arrays
+        if handle is not None:
+            v = L.CsrView()
+            L.call("mgx_matrix_view", C.c_void_p(handle), C.byref(v))
+            self.nrows, self.ncols, self.nnz = v.nrows, v.ncols, v.nnz
+            self.rowptr, self.col, self.val = v.rowptr, v.col, v.val
+        else:
+            self.nrows, self.ncols, self.nnz = nrows, ncols, nnz
+            self.rowptr, self.col, self.val = (a.ptr for a in arrays)
+
+    @classmethod
+    def from_host(cls, A):
+        """Library-owned copy of a host CSR (usable by mgx_solver_set_level_device)."""
+        h = C.c_void_p()
+        rp = np.ascontiguousarray(A.rowptr, np.int32)
+        ci = np.ascontiguousarray(A.col, np.int32)
+        v = np.ascontiguousarray(A.val, np.float64)
+        L.call("mgx_matrix_from_host", A.nrows, A.ncols, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+               v.ctypes.data_as(C.c_void_p), C.byref(h))
+        return cls(handle=h.value)
+
+    @classmethod
+    def arrays_from_host(cls, A, val_dtype=np.float64):
+        """Python-owned device arrays (for the device-level kernels)."""
+        arrs = (DeviceArray.from_numpy(A.rowptr, np.int32), DeviceArray.from_numpy(A.col, np.int32),
+                DeviceArray.from_numpy(A.val, val_dtype))
+        return cls(arrays=arrs, nrows=A.nrows, ncols=A.ncols, nnz=len(A.col))
+
+    def to_host(self):
+        rp = np.empty(self.nrows + 1, np.int32)
+        ci = np.empty(self.nnz, np.int32)
+        v = np.empty(self.nnz, np.float64)
+        if self.handle is not None:
+            L.call("mgx_matrix_download", C.c_void_p(self.handle), rp.ctypes.data_as(C.c_void_p),
+                   ci.ctypes.data_as(C.c_void_p), v.ctypes.data_as(C.c_void_p))
+        else:
+            rp, ci, v = (a.numpy() for a in self._arrays)
+        return CsrMatrix(rp, ci, v, self.nrows, self.ncols)
+
+    def free(self):
+        if self.handle is not None:
+            L.lib().mgx_matrix_destroy(C.c_void_p(self.handle))
+            self.handle = None
+        self._arrays = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+# ------------------------------------------------------------------ structured generators
+def cantilever_matrix(nx, ny, nz, E=1.0, nu=0.3, dens=None):
+    """K_ff of the clamped cantilever, assembled on the device (bit-identical to the
+    element-order DOK sums of FemCantileverBase.cs:87-104)."""
+    h = C.c_void_p()
+    d = None if dens is None else np.ascontiguousarray(dens, np.float64)
+    L.call("mgx_matrix_cantilever", nx, ny, nz, C.c_double(E), C.c_double(nu),
+           d.ctypes.data_as(C.c_void_p) if d is not None else None, C.byref(h))
+    return DeviceCsr(handle=h.value)
+
+
+def transfer_reference(nx, ny, nz):
+    """(R, P) of IGeometricMultigridModel.CreateRestrictionAndInterpolationMatrix."""
+    r, p = C.c_void_p(), C.c_void_p()
+    L.call("mgx_matrix_transfer_reference", nx, ny, nz, C.byref(r), C.byref(p))
+    return DeviceCsr(handle=r.value), DeviceCsr(handle=p.value)
+
+
+def transfer_trilinear(nx, ny, nz):
+    """(R = P^T, P) with exact trilinear weights."""
+    r, p = C.c_void_p(), C.c_void_p()
+    L.call("mgx_matrix_transfer_trilinear", nx, ny, nz, C.byref(r), C.byref(p))
+    return DeviceCsr(handle=r.value), DeviceCsr(handle=p.value)
+
+
+def galerkin_hierarchy(nx, ny, nz, nlevels, E=1.0, nu=0.3, dens=None):
+    """[K_0, P_0^T K_0 P_0, ...] assembled element by element on the device."""
+    hs = (C.c_void_p * nlevels)()
+    d = None if dens is None else np.ascontiguousarray(dens, np.float64)
+    L.call("mgx_matrix_galerkin_hierarchy", nx, ny, nz, C.c_double(E), C.c_double(nu),
+           d.ctypes.data_as(C.c_void_p) if d is not None else None, nlevels, hs)
+    return [DeviceCsr(handle=hs[i]) for i in range(nlevels)]
+
+
+def rng_uniform(seed, n):
+    out = np.empty(n, np.float64)
+    L.call("mgx_rng_uniform", C.c_uint64(seed), C.c_int64(n), out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def rng_normal(seed, n):
+    out = np.empty(n, np.float64)
+    L.call("mgx_rng_normal", C.c_uint64(seed), C.c_int64(n), out.ctypes.data_as(C.c_void_p))
+    return out
+
+
+def duvi_encode(A, tol=1e-10, mode=0):
+    """Host DuVi encoder of the library (OpenCLDuViGeometricMultigridSolver.cs:753-851)."""
+    h = L.DuviHost()
+    rp = np.ascontiguousarray(A.rowptr, np.int32)
+    ci = np.ascontiguousarray(A.col, np.int32)
+    v = np.ascontiguousarray(A.val, np.float64)
+    L.call("mgx_duvi_encode", A.nrows, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
+           v.ctypes.data_as(C.c_void_p), C.c_double(tol), mode, C.byref(h))
+    nnz, nb, nu = h.nnz, h.nblocks, h.nuniq
+    out = dict(nrows=h.nrows, nnz=nnz, nblocks=nb, nuniq=nu, vi_bytes=h.vi_bytes,
+               row_blocks=np.ctypeslib.as_array(h.row_blocks, (h.nrows + 1,)).copy(),
+               row_dist=np.ctypeslib.as_array(h.row_dist, (h.nrows + 1,)).copy(),
+               block_col=np.ctypeslib.as_array(h.block_col, (max(nb, 1),))[:nb].copy(),
+               dist=np.ctypeslib.as_array(h.dist, (max(nnz, 1),))[:nnz].copy(),
+               values=np.ctypeslib.as_array(h.values, (max(nu, 1),))[:nu].copy())
+    if h.vi_bytes:
+        dt = {1: np.uint8, 2: np.uint16, 4: np.uint32}[h.vi_bytes]
+        buf = (C.c_uint8 * (h.vi_bytes * max(nnz, 1))).from_address(h.vi)
+        out["vi"] = np.frombuffer(buf, dtype=dt, count=nnz).copy()
+    else:
+        out["vi"] = None
+    L.lib().mgx_duvi_host_free(C.byref(h))
+    return out
+
+
+class DeviceDuvi:
+    """A DuVi block-layout matrix uploaded to the device (mgx_duvi)."""
+
+    def __init__(self, d):
+        self.keep = [DeviceArray.from_numpy(d["row_blocks"], np.int32), DeviceArray.from_numpy(d["row_dist"], np.int32),
+                     DeviceArray.from_numpy(d["block_col"], np.int32), DeviceArray.from_numpy(d["dist"], np.uint8),
+                     DeviceArray.from_numpy(d["values"], np.float64)]
+        vi = None
+        if d["vi"] is not None:
+            vi = DeviceArray.from_numpy(d["vi"])
+            self.keep.append(vi)
+        m = L.Duvi()
+        m.nrows, m.vi_bytes, m.nnz, m.nblocks, m.nuniq = d["nrows"], d["vi_bytes"], d["nnz"], d["nblocks"], d["nuniq"]
+        m.row_blocks, m.row_dist, m.block_col, m.dist, m.values = (a.ptr for a in self.keep[:5])
+        m.vi = vi.ptr if vi is not None else None
+        self.m = m
+
+    def spmv(self, x, y, clear=True):
+        L.call("mgx_duvi_spmv", C.byref(self.m), x.vp, y.vp, int(clear), None)

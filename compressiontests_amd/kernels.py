@@ -1,0 +1,76 @@
+"""Device-level kernels of libmgx, one per OpenCL kernel of the reference
+(src/MGroup.OCL/CsrGeometricMultigrid.cl, CsrCG.cl, DuViGeometricMultigrid.cl), same
+argument meaning.  Arguments are DeviceCsr / DeviceArray objects."""
+import ctypes as C
+
+import numpy as np
+
+from . import _lib as L
+from .device import DeviceArray
+
+
+def matrix_vector_product(A, x, y, clear=True, stream=None):
+    """y = A x (clear) or y += A x.  CsrGeometricMultigrid.cl:49-62."""
+    name = "mgx_csr_spmv_f32" if x.dtype == np.float32 else "mgx_csr_spmv"
+    L.call(name, A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), x.vp, y.vp, int(clear), stream)
+
+
+def residual(A, b, x, r, stream=None):
+    """r = b - A x.  CsrGeometricMultigrid.cl:96-110."""
+    L.call("mgx_csr_residual", A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), b.vp, x.vp, r.vp,
+           stream)
+
+
+def residual_with_check(A, b, x, r, tolerance, stream=None):
+    """r = b - A x and the reference flag (1 converged, bit 1 failed).  CsrGeometricMultigrid.cl:121-141."""
+    flag = DeviceArray.from_numpy(np.array([1], np.int32))
+    L.call("mgx_csr_residual_check", A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), b.vp, x.vp,
+           r.vp, C.c_double(tolerance), flag.vp, stream)
+    return int(flag.numpy()[0])
+
+
+def jacobi_initial_iteration(pre, b, x, stream=None):
+    """x = pre .* b.  CsrGeometricMultigrid.cl:153-164."""
+    L.call("mgx_jacobi_initial", pre.n, pre.vp, b.vp, x.vp, stream)
+
+
+def jacobi_iteration(pre, A, b, x, y, stream=None):
+    """y = x + pre .* (b - A x).  CsrGeometricMultigrid.cl:176-194."""
+    L.call("mgx_csr_jacobi", A.nrows, pre.vp, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), b.vp, x.vp,
+           y.vp, stream)
+
+
+def gauss_seidel_color(nodes, A, b, x, backward=False, stream=None):
+    """One colour of the multicolour Gauss-Seidel sweep (replaces the racy
+    gauss_seidel_iteration, CsrGeometricMultigrid.cl:207-225)."""
+    L.call("mgx_csr_gs_color", nodes.n, nodes.vp, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), b.vp,
+           x.vp, int(backward), stream)
+
+
+def dot(a, b, stream=None):
+    """a . b.  CsrCG.cl:43-85 (dot_partial + dot_finalize)."""
+    res = DeviceArray(1)
+    L.call("mgx_dot", a.n, a.vp, b.vp, res.vp, stream)
+    return float(res.numpy()[0])
+
+
+def time_csr_kernel(kind, A, x, y, b=None, pre=None, warmup=5, repeats=100):
+    """Average device ms per launch (HIP events) of kind 0 SpMV / 1 Jacobi / 2 residual / 3 FP32 SpMV."""
+    out = C.c_double()
+    L.call("mgx_time_csr_kernel", kind, A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), x.vp,
+           b.vp if b is not None else None, pre.vp if pre is not None else None, y.vp, warmup, repeats, C.byref(out))
+    return out.value
+
+
+def time_duvi_spmv(D, x, y, warmup=5, repeats=100):
+    out = C.c_double()
+    L.call("mgx_time_duvi_spmv", C.byref(D.m), x.vp, y.vp, warmup, repeats, C.byref(out))
+    return out.value
+
+
+def simp_gradient(nx, ny, nz, rho, u, penal=3.0, Emin=1e-9, E0=1.0, nu=0.3):
+    """dC/drho_e = -p rho_e^(p-1) (E0 - Emin) u_e^T Ke0 u_e on the device."""
+    g = DeviceArray(nx * ny * nz)
+    L.call("mgx_simp_gradient", nx, ny, nz, C.c_double(nu), rho.vp, C.c_double(penal), C.c_double(Emin),
+           C.c_double(E0), u.vp, g.vp, None)
+    return g

@@ -1,0 +1,112 @@
+"""The FE cantilever model of the reference (IGeometricMultigridModel + FemCantileverBase +
+FemCantilever3D), producing its matrices with the library's device-side generators.
+
+Mesh conventions (the reference takes them from the unvendored UniformCartesianMesh3D):
+node id = ix + (nx+1)(iy + (ny+1) iz); element e = ex + nx (ey + ny ez); element corners in
+VTK hexahedron order; 3 dofs (ux, uy, uz) per node; nodes with ix == 0 clamped
+(FemCantileverBase.cs:137-158).
+"""
+import numpy as np
+
+from .device import DeviceArray, cantilever_matrix, transfer_reference
+from . import kernels
+
+
+class FemCantilever3D:
+    """FemCantilever3D (FemCantilever3D.cs:27-131) with the base class behaviour of
+    FemCantileverBase (FemCantileverBase.cs:11-193).
+
+    rhs: "manufactured" -> b = K_ff U_f with U_f from Euler-Bernoulli beam formulas
+         (FemCantileverBase.cs:53-82,160-182), the reference's own right-hand side;
+         "tip" -> total force `TipLoad` along z split equally over the x = Lx face nodes.
+    """
+
+    NumDofsPerNode = 3
+
+    def __init__(self, numElementsPerAxis, lengthPerAxis=(1.0, 1.0, 1.0), ElementDensities=None,
+                 ElasticityModulus=1.0, PoissonRatio=0.3, DistributedLoad=1.0, rhs="manufactured", TipLoad=-1.0):
+        self.NumElements = tuple(int(n) for n in numElementsPerAxis)
+        if len(self.NumElements) != 3:
+            raise ValueError("FemCantilever3D needs 3 element counts")
+        self.Lengths = tuple(float(v) for v in lengthPerAxis)
+        self.ElementDensities = None if ElementDensities is None else np.ascontiguousarray(ElementDensities, np.float64)
+        self.ElasticityModulus = ElasticityModulus
+        self.PoissonRatio = PoissonRatio
+        self.DistributedLoad = DistributedLoad
+        self.rhs_kind = rhs
+        self.TipLoad = TipLoad
+        nx, ny, nz = self.NumElements
+        self.NumNodes = (nx + 1, ny + 1, nz + 1)
+        self.NumDofsAll = 3 * (nx + 1) * (ny + 1) * (nz + 1)
+        self.NumDofsFree = self.NumDofsAll - self.NumDofsAll // (nx + 1)
+        L1, L2 = self.Lengths[1], self.Lengths[2]
+        self.MomentOfInertia = L1 * L2 / 12 * L2 * L2            # FemCantilever3D.cs:28
+        self._K = None
+
+    # ---------------------------------------------------------------- matrices
+    def device_matrix(self):
+        """K_ff on the device (library-owned), element moduli = ElementDensities x E."""
+        if self._K is None:
+            nx, ny, nz = self.NumElements
+            self._K = cantilever_matrix(nx, ny, nz, self.ElasticityModulus, self.PoissonRatio, self.ElementDensities)
+        return self._K
+
+    def CalcFreeDisplacements(self):
+        """U_f of FemCantileverBase.cs:160-182 with FemCantilever3D.cs:122-128."""
+        nx, ny, nz = self.NumElements
+        Lx, Ly, Lz = self.Lengths
+        ix = np.arange(1, nx + 1, dtype=np.float64)
+        iz = np.arange(0, nz + 1, dtype=np.float64)
+        x = ix * (Lx / nx)
+        z = iz * (Lz / nz) - Lz / 2
+        q, E, I, L = self.DistributedLoad, self.ElasticityModulus, self.MomentOfInertia, Lx
+        w = -q / (24 * E * I) * (np.power(x, 4) - 4 * L * np.power(x, 3) + 6 * L * L * x * x)
+        rot = -q / (6 * E * I) * (np.power(x, 3) - 3 * L * x * x + 3 * L * L * x)
+        u = -rot[None, None, :] * z[:, None, None]                 # [iz, iy, ix]
+        U = np.zeros((nz + 1, ny + 1, nx, 3))
+        U[..., 0] = np.broadcast_to(u, (nz + 1, ny + 1, nx))
+        U[..., 2] = np.broadcast_to(w[None, None, :], (nz + 1, ny + 1, nx))
+        return U.reshape(-1)
+
+    def tip_load(self):
+        nx, ny, nz = self.NumElements
+        f = np.zeros((nz + 1, ny + 1, nx, 3))
+        f[:, :, nx - 1, 2] = self.TipLoad / ((ny + 1) * (nz + 1))
+        return f.reshape(-1)
+
+    def rhs(self):
+        if self.rhs_kind == "tip":
+            return self.tip_load()
+        K = self.device_matrix()
+        U = DeviceArray.from_numpy(self.CalcFreeDisplacements())
+        b = DeviceArray(self.NumDofsFree)
+        kernels.matrix_vector_product(K, U, b)
+        return b.numpy()
+
+    def CreateLinearSystem(self):
+        """(A, b): K_ff as host CSR and the right-hand side (IGeometricMultigridModel.cs:39)."""
+        return self.device_matrix().to_host(), self.rhs()
+
+    def CreateAndSolveLinearSystem(self):
+        A, b = self.CreateLinearSystem()
+        return A, self.CalcFreeDisplacements(), b
+
+    def FindFreeDofs(self):
+        nx1, ny1, nz1 = self.NumNodes
+        n = np.arange(nx1 * ny1 * nz1)
+        free_nodes = n[(n % nx1) != 0]
+        return (3 * free_nodes[:, None] + np.arange(3)[None, :]).reshape(-1).astype(np.int32)
+
+    def CreateCoarserModel(self):
+        """FemCantileverBase.cs:186-191: halve every element count > 1, same lengths.  Like the
+        reference, the coarser model has default densities and the unit-cube Ke."""
+        c = tuple(n >> 1 if n > 1 else n for n in self.NumElements)
+        return FemCantilever3D(c, self.Lengths, None, self.ElasticityModulus, self.PoissonRatio, self.DistributedLoad,
+                               self.rhs_kind, self.TipLoad)
+
+
+def CreateCoarserModelAndSmoothenerMatrices(model):
+    """IGeometricMultigridModel.cs:60-65: (coarser model, R, P) with R, P on the device."""
+    coarser = model.CreateCoarserModel()
+    R, P = transfer_reference(*model.NumElements)
+    return coarser, R, P

@@ -1,0 +1,315 @@
+"""Python mirror of the reference's solver interfaces over the C ABI of libmgx.
+
+  IGeometricMultigridSolver.Solve(xInitialGuess) -> (x, IterativeStatistics, levelTimes)
+      (src/MGroup.Solvers/Multigrid/IGeometricMultigridSolver.cs:6-9)
+  HipCsrGeometricMultigridSolver   <- OpenCLCsrGeometricMultigridSolver (CreateSimpleV/CreateDeepV,
+                                      Initialize, Solve, ReleaseOpenCLResources)
+  HipDuViGeometricMultigridSolver  <- OpenCLDuViGeometricMultigridSolver
+  HipCsrPcgSolver                  <- OpenCLCsrPcgSolver (OpenCLCsrPcgSolver.cs:40-331)
+  HipMgPcgSolver                   new: PCG preconditioned by one symmetric V-cycle
+
+Same argument names and meaning, same error behaviour: Solve before Initialize raises
+MgxInvalidOperation (InvalidOperationException), a first level other than 0 raises
+MgxNotImplemented, non-convergence is reported in the statistics, not raised.
+"""
+import ctypes as C
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import _lib as L
+from .device import DeviceArray, DeviceCsr, galerkin_hierarchy, transfer_trilinear
+from .model import CreateCoarserModelAndSmoothenerMatrices
+
+MAX_CIRCLE_ITERATIONS = 10000
+
+
+@dataclass
+class IterativeStatistics:
+    NumIterationsRequired: int = 0
+    HasConverged: bool = False
+    Failed: bool = False
+    ConvergenceCriterion: tuple = ("", 0.0)
+    ResidualNorm: float = 0.0
+    RelativeResidual: float = 0.0
+    SolveMs: float = 0.0
+    SpmvMs: float = 0.0
+    SpmvLaunches: int = 0
+    OuterIterations: int = 0
+
+    @classmethod
+    def from_c(cls, s, criterion_name, criterion_value):
+        return cls(NumIterationsRequired=s.iterations, HasConverged=s.converged == 1, Failed=s.converged == -1,
+                   ConvergenceCriterion=(criterion_name, criterion_value), ResidualNorm=s.residual_norm,
+                   RelativeResidual=s.rel_residual, SolveMs=s.solve_ms, SpmvMs=s.spmv_ms,
+                   SpmvLaunches=s.spmv_launches, OuterIterations=s.outer_iterations)
+
+
+def make_path(level_down):
+    """GeometricMultigridSolver.MakePath (GeometricMultigridSolver.cs:178-201)."""
+    level_down = list(level_down)
+    current, total, first = 0, 1, 0
+    for d in level_down:
+        if d:
+            current += 1
+            if current == total:
+                total += 1
+        elif current == 0:
+            first += 1
+            total += 1
+        else:
+            current -= 1
+    if current > first:
+        level_down += [False] * (current - first)
+    elif current < first:
+        level_down += [True] * (first - current)
+    return total, level_down, first
+
+
+class _Handle:
+    """Owns one mgx_solver."""
+
+    def __init__(self, opt):
+        h = C.c_void_p()
+        L.call("mgx_solver_create", C.byref(opt), C.byref(h))
+        self.h = h.value
+        self.opt = opt
+        self.keep = []
+        self.n = 0
+
+    def set_level_device(self, level, A, R=None, P=None):
+        self.keep += [A, R, P]
+        L.call("mgx_solver_set_level_device", C.c_void_p(self.h), level, C.c_void_p(A.handle),
+               C.c_void_p(R.handle) if R is not None else None, C.c_void_p(P.handle) if P is not None else None)
+        if level == 0:
+            self.n = A.nrows
+
+    def set_level_host(self, level, A, R=None, P=None):
+        def host(M):
+            if M is None:
+                return None, None
+            arrs = (np.ascontiguousarray(M.rowptr, np.int32), np.ascontiguousarray(M.col, np.int32),
+                    np.ascontiguousarray(M.val, np.float64))
+            h = L.CsrHost(M.nrows, M.ncols, len(arrs[1]), *(a.ctypes.data for a in arrs))
+            return h, arrs
+        hA, kA = host(A)
+        hR, kR = host(R)
+        hP, kP = host(P)
+        L.call("mgx_solver_set_level", C.c_void_p(self.h), level, C.byref(hA), C.byref(hR) if hR else None,
+               C.byref(hP) if hP else None)
+        if level == 0:
+            self.n = A.nrows
+
+    def set_rhs(self, b, nrhs=1):
+        b = np.ascontiguousarray(b, np.float64).reshape(-1)
+        n = b.size // nrhs
+        L.call("mgx_solver_set_rhs", C.c_void_p(self.h), b.ctypes.data_as(C.c_void_p), n, nrhs)
+
+    def initialize(self):
+        L.call("mgx_solver_initialize", C.c_void_p(self.h))
+
+    def solve(self, x0=None, nrhs=1):
+        n = self.n
+        x = np.zeros(n * nrhs) if x0 is None else np.array(x0, np.float64).reshape(-1).copy()
+        st = (L.Stats * nrhs)()
+        L.call("mgx_solver_solve", C.c_void_p(self.h), x.ctypes.data_as(C.c_void_p), int(x0 is not None), st)
+        return x, list(st)
+
+    def solve_device(self, b_dev, x_dev, has_x0=False, nrhs=1):
+        st = (L.Stats * nrhs)()
+        L.call("mgx_solver_solve_device", C.c_void_p(self.h), b_dev.vp, x_dev.vp, int(has_x0), st)
+        return list(st)
+
+    def release(self):
+        if self.h:
+            L.call("mgx_solver_release", C.c_void_p(self.h))
+            self.h = None
+        self.keep = []
+
+    def __del__(self):
+        try:
+            self.release()
+        except Exception:
+            pass
+
+
+class HipCsrGeometricMultigridSolver:
+    """Stationary V-cycle multigrid (OpenCLCsrGeometricMultigridSolver.cs:14-557)."""
+
+    FORMAT = L.FORMAT_CSR
+
+    def __init__(self, maxCircleIterations, convergenceTolerance, levelDown, levelIterations, GaussSeidel=True):
+        self.MaxIterations = maxCircleIterations
+        self.ConvergenceTolerance = convergenceTolerance
+        self.LevelDown = list(levelDown)
+        self.LevelIterations = list(levelIterations) if levelIterations is not None else [4]
+        self.GaussSeidel = GaussSeidel
+        self._h = None
+        self.LevelDoFs = None
+
+    @classmethod
+    def CreateSimpleV(cls, model, GaussSeidel=True, maxCircleIterations=MAX_CIRCLE_ITERATIONS, coarseRelaxation=True,
+                      convergenceTolerance=1e-6, levelIterations=4):
+        a = cls(maxCircleIterations, convergenceTolerance, [True, False], [levelIterations], GaussSeidel)
+        a.Initialize(model, coarseRelaxation)
+        return a
+
+    @classmethod
+    def CreateDeepV(cls, model, GaussSeidel=True, maxCircleIterations=MAX_CIRCLE_ITERATIONS, coarseRelaxation=True,
+                    convergenceTolerance=1e-6, depth=2, levelIterations=4):
+        a = cls(maxCircleIterations, convergenceTolerance, [True] * depth + [False] * depth, [levelIterations],
+                GaussSeidel)
+        a.Initialize(model, coarseRelaxation)
+        return a
+
+    def Initialize(self, model, coarseRelaxation=True):
+        total, level_down, first = make_path(self.LevelDown)
+        self.LevelDown = level_down
+        depth = total - 1
+        if first != 0:
+            raise L.MgxNotImplemented("firstLevel != 0 is not implementent yet")
+        if level_down != [True] * depth + [False] * depth:
+            raise L.MgxNotImplemented("only V paths [down x depth, up x depth] are supported")
+        if len(set(self.LevelIterations)) != 1:
+            raise L.MgxNotImplemented("one smoother iteration count for every level")
+        opt = L.default_options(method=L.METHOD_GMG, format=self.FORMAT,
+                                smoother=L.SMOOTHER_GS if self.GaussSeidel else L.SMOOTHER_JACOBI,
+                                coarse_relaxation=int(bool(coarseRelaxation)), omega=0.0,
+                                max_iterations=self.MaxIterations, tolerance=self.ConvergenceTolerance,
+                                stop=L.STOP_MAXNORM, depth=depth, level_iterations=self.LevelIterations[0])
+        h = _Handle(opt)
+        A = model.device_matrix()
+        b = model.rhs()
+        self.LevelDoFs = [A.nrows]
+        m = model
+        for level in range(depth):
+            coarser, R, P = CreateCoarserModelAndSmoothenerMatrices(m)
+            h.set_level_device(level, m.device_matrix(), R, P)
+            self.LevelDoFs.append(R.nrows)
+            m = coarser
+        h.set_level_device(depth, m.device_matrix())
+        h.set_rhs(b)
+        h.initialize()
+        self._h = h
+
+    def Solve(self, xInitialGuess=None):
+        if self._h is None:
+            raise L.MgxInvalidOperation("You must call Initialize(model) first")
+        x, st = self._h.solve(xInitialGuess)
+        stats = IterativeStatistics.from_c(st[0], "dumb text", self.ConvergenceTolerance)
+        return x, stats, np.zeros(len(self.LevelDoFs))
+
+    def ReleaseOpenCLResources(self):
+        if self._h is not None:
+            self._h.release()
+            self._h = None
+
+    ReleaseResources = ReleaseOpenCLResources
+
+
+class HipDuViGeometricMultigridSolver(HipCsrGeometricMultigridSolver):
+    """Same V-cycle over the DuVi block layout (OpenCLDuViGeometricMultigridSolver.cs:14-861)."""
+
+    FORMAT = L.FORMAT_DUVI
+
+
+class HipCsrPcgSolver:
+    """Jacobi-preconditioned CG (OpenCLCsrPcgSolver.cs:40-331): stop when every |r_i| <= tol,
+    iteration count is the 0-based index of the last iteration, as the reference reports."""
+
+    def __init__(self, maxIterations, convergenceTolerance):
+        self.MaxIterations = maxIterations
+        self.ConvergenceTolerance = convergenceTolerance
+        self._h = None
+
+    def Initialize(self, model):
+        opt = L.default_options(method=L.METHOD_PCG_JACOBI, max_iterations=self.MaxIterations,
+                                tolerance=self.ConvergenceTolerance, stop=L.STOP_MAXNORM)
+        h = _Handle(opt)
+        h.set_level_device(0, model.device_matrix())
+        h.set_rhs(model.rhs())
+        h.initialize()
+        self._h = h
+
+    def Solve(self, xInitialGuess=None):
+        if self._h is None:
+            raise L.MgxInvalidOperation("You must call Initialize(model) first")
+        x, st = self._h.solve(xInitialGuess)
+        return x, IterativeStatistics.from_c(st[0], "dumb text", self.ConvergenceTolerance)
+
+    def ReleaseOpenCLResources(self):
+        if self._h is not None:
+            self._h.release()
+            self._h = None
+
+    ReleaseResources = ReleaseOpenCLResources
+
+
+class HipMgPcgSolver:
+    """PCG with one V-cycle as preconditioner (the new composition of OpenCLCsrPcgSolver and the
+    V-cycle of OpenCLCsrGeometricMultigridSolver).  hierarchy:
+      "galerkin":  K_{l+1} = P^T K_l P with exact trilinear P and R = P^T (symmetric, handles
+                   heterogeneous moduli);
+      "reference": the reference's rediscretised coarse models with R = P^T of the
+                   reference interpolation (its cone-weight R is not symmetric to P).
+    smoother "jacobi" (omega, nu pre and nu post sweeps) or "gs" (nu forward / nu backward
+    multicolour Gauss-Seidel sweeps)."""
+
+    def __init__(self, model, levels=3, smoother="jacobi", omega=0.6, nu=2, tolerance=1e-8, stop="rel",
+                 max_iterations=1000, hierarchy="galerkin", format="csr", profile=False, check_every=4, nrhs=1):
+        self.model = model
+        self.levels = levels
+        opt = L.default_options(method=L.METHOD_MGPCG,
+                                format=L.FORMAT_DUVI if format == "duvi" else L.FORMAT_CSR,
+                                smoother=L.SMOOTHER_GS if smoother == "gs" else L.SMOOTHER_JACOBI,
+                                omega=omega, max_iterations=max_iterations, tolerance=tolerance,
+                                stop=L.STOP_REL_L2 if stop == "rel" else L.STOP_MAXNORM, level_iterations=nu,
+                                profile=int(profile), check_every=check_every, nrhs=nrhs)
+        self.opt = opt
+        self.nrhs = nrhs
+        h = _Handle(opt)
+        nx, ny, nz = model.NumElements
+        if hierarchy == "galerkin":
+            Ks = galerkin_hierarchy(nx, ny, nz, levels, model.ElasticityModulus, model.PoissonRatio,
+                                    model.ElementDensities)
+            model._K = Ks[0]
+            dims = (nx, ny, nz)
+            for l in range(levels - 1):
+                R, P = transfer_trilinear(*dims)
+                h.set_level_device(l, Ks[l], R, P)
+                dims = tuple(d // 2 for d in dims)
+            h.set_level_device(levels - 1, Ks[-1])
+        elif hierarchy == "reference":
+            m = model
+            for l in range(levels - 1):
+                coarser, _, P = CreateCoarserModelAndSmoothenerMatrices(m)
+                Ph = P.to_host()
+                import scipy.sparse as sp   # host transpose of the reference P (setup only)
+                PT = sp.csr_matrix((Ph.val, Ph.col, Ph.rowptr), shape=(Ph.nrows, Ph.ncols)).T.tocsr()
+                PT.sort_indices()
+                from .device import CsrMatrix
+                R = DeviceCsr.from_host(CsrMatrix(PT.indptr.astype(np.int32), PT.indices.astype(np.int32),
+                                                  PT.data, PT.shape[0], PT.shape[1]))
+                h.set_level_device(l, m.device_matrix(), R, P)
+                m = coarser
+            h.set_level_device(levels - 1, m.device_matrix())
+        else:
+            raise ValueError("hierarchy must be 'galerkin' or 'reference'")
+        self._h = h
+
+    def Initialize(self, b=None):
+        if b is None:
+            b = self.model.rhs()
+        self._h.set_rhs(b, self.nrhs)
+        self._h.initialize()
+
+    def Solve(self, xInitialGuess=None):
+        x, st = self._h.solve(xInitialGuess, self.nrhs)
+        stats = [IterativeStatistics.from_c(s, "relative residual", self.opt.tolerance) for s in st]
+        return x, (stats[0] if self.nrhs == 1 else stats)
+
+    def solve_device(self, b_dev, x_dev):
+        return self._h.solve_device(b_dev, x_dev, False, self.nrhs)
+
+    def ReleaseResources(self):
+        self._h.release()

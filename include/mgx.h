@@ -1,0 +1,286 @@
+/*
+ * mgx.h -- C ABI of the MI355X (gfx950) MG-PCG / SpMV solver backend (libmgx.so).
+ *
+ * Plain C: pointers, sizes, status codes.  No torch or HIP types in any signature;
+ * a stream is passed as `void*` (a hipStream_t, NULL = the library's own stream).
+ * Every function returns MGX_OK (0) or an error status; mgx_last_error() returns the
+ * message of the calling thread's last failure.
+ *
+ * Two groups of entry points:
+ *   1. Device-level kernels (device pointers in and out).  One per OpenCL kernel of the
+ *      reference, same argument meaning; the reference kernel each replaces is cited.
+ *   2. The solver handle, replacing the reference's OpenCL solver classes
+ *      (Initialize / Solve / ReleaseOpenCLResources) with host arrays in and out.
+ *
+ * Reference paths are relative to nickreserved/CompressionTests.
+ */
+#ifndef MGX_H
+#define MGX_H
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------ status codes
+ * The reference throws; a C# binding maps these back to its exceptions:
+ *   MGX_ERR_INVALID_OPERATION  <- InvalidOperationException ("You must call Initialize(model) first",
+ *                                 OpenCLCsrGeometricMultigridSolver.cs:336, OpenCLCsrPcgSolver.cs:252)
+ *   MGX_ERR_NOT_IMPLEMENTED    <- NotImplementedException (firstLevel != 0,
+ *                                 OpenCLCsrGeometricMultigridSolver.cs:345; varint range,
+ *                                 DuViCompressedSparseMatrix.cs:22)
+ *   MGX_ERR_DEVICE             <- OpenCLException (OpenCLDuViGeometricMultigridSolver.cs:859)
+ *   MGX_ERR_INVALID_ARGUMENT   <- ArgumentException / Debug.Assert on sizes
+ * Non-convergence is not an error: it is reported in mgx_stats.converged. */
+#define MGX_OK                     0
+#define MGX_ERR_INVALID_ARGUMENT   1
+#define MGX_ERR_INVALID_OPERATION  2
+#define MGX_ERR_NOT_IMPLEMENTED    3
+#define MGX_ERR_DEVICE             4
+#define MGX_ERR_OUT_OF_MEMORY      5
+#define MGX_ERR_NOT_SPD            6
+
+const char *mgx_last_error(void);
+/* library version, e.g. 0x000100 */
+int32_t mgx_version(void);
+/* number of visible devices (0 on a host without a GPU; never an error) */
+int32_t mgx_device_count(void);
+
+/* ================================================================== 1. device-level kernels */
+
+/* y = A x (clear = 1) or y += A x (clear = 0); CSR with int32 row offsets / columns.
+ * Replaces matrix_vector_product, src/MGroup.OCL/CsrGeometricMultigrid.cl:49-62 (= CsrCG.cl:137-150). */
+int mgx_csr_spmv(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val,
+                 const double *x, double *y, int32_t clear, void *stream);
+
+/* FP32 values and vectors, int32 indices (precision form; no reference counterpart). */
+int mgx_csr_spmv_f32(int32_t nrows, const int32_t *rowptr, const int32_t *col, const float *val,
+                     const float *x, float *y, int32_t clear, void *stream);
+
+/* r = b - A x.  Replaces residual, CsrGeometricMultigrid.cl:96-110. */
+int mgx_csr_residual(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val,
+                     const double *b, const double *x, double *r, void *stream);
+
+/* r = b - A x and the convergence flag of residual_with_check, CsrGeometricMultigrid.cl:121-141:
+ * *flag must hold 1 on entry; bit 1 is set if any r_i is NaN or |r_i| > 1e50, bit 0 is cleared
+ * if any other |r_i| > tol.  flag is a device pointer. */
+int mgx_csr_residual_check(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val,
+                           const double *b, const double *x, double *r, double tol,
+                           int32_t *flag, void *stream);
+
+/* x = pre .* b.  Replaces jacobi_initial_iteration, CsrGeometricMultigrid.cl:153-164. */
+int mgx_jacobi_initial(int32_t n, const double *pre, const double *b, double *x, void *stream);
+
+/* y = x + pre .* (b - A x).  Replaces jacobi_iteration, CsrGeometricMultigrid.cl:176-194. */
+int mgx_csr_jacobi(int32_t n, const double *pre, const int32_t *rowptr, const int32_t *col,
+                   const double *val, const double *b, const double *x, double *y, void *stream);
+
+/* One colour of a multicolour Gauss-Seidel sweep over node blocks of 3 rows: for each node
+ * k in nodes[0..nnodes), rows 3k..3k+2 are relaxed in order (reverse order when
+ * backward != 0) using the current x.  Nodes of one colour must not be coupled.
+ * Replaces gauss_seidel_iteration, CsrGeometricMultigrid.cl:207-225, whose in-place update
+ * races; the multicolour order is the documented deviation. */
+int mgx_csr_gs_color(int32_t nnodes, const int32_t *nodes, const int32_t *rowptr, const int32_t *col,
+                     const double *val, const double *b, double *x, int32_t backward, void *stream);
+
+/* *result = a . b (device pointer result).  Replaces dot_partial + dot_finalize, CsrCG.cl:43-85. */
+int mgx_dot(int32_t n, const double *a, const double *b, double *result, void *stream);
+
+/* DuVi GPU block layout (OpenCLDuViGeometricMultigridSolver.cs:16-31) on the device.
+ * vi_bytes: 1 (u8), 2 (u16), 4 (u32) value indices into values[nuniq], or 0 = "DU + raw
+ * values" (values[nnz] in CSR order, no index stream). */
+typedef struct {
+    int32_t nrows;
+    int32_t vi_bytes;
+    int64_t nnz, nblocks, nuniq;
+    const int32_t *row_blocks;   /* [nrows+1] RowToColumnIndices */
+    const int32_t *row_dist;     /* [nrows+1] RowToDistances     */
+    const int32_t *block_col;    /* [nblocks] ColumnIndices       */
+    const uint8_t *dist;         /* [nnz]     Distances           */
+    const void    *vi;           /* [nnz]     ValueIndices or NULL */
+    const double  *values;
+} mgx_duvi;
+
+/* y = A x / y += A x decoding the DuVi block layout.
+ * Replaces matrix_vector_product, src/MGroup.OCL/DuViGeometricMultigrid.cl:122-159. */
+int mgx_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t clear, void *stream);
+
+/* Host-side DuVi encoder: CSR -> block layout.  Restates FromDokRowMajor,
+ * OpenCLDuViGeometricMultigridSolver.cs:753-851 (tolerance dedup of ToleranceComparer,
+ * DuViCompressedSparseMatrix.cs:168-174).  mode 0: u8 if <= 256 unique values, u16 if
+ * <= 65536, else DU + raw values (the intent of the reference's defective 16-bit switch
+ * :818-825 and its >65536 fallback :814); mode 4: u32 indices; mode 8: raw values.
+ * Output arrays are allocated by the library (host memory) and freed by mgx_duvi_host_free. */
+typedef struct {
+    int32_t nrows, vi_bytes;
+    int64_t nnz, nblocks, nuniq;
+    int32_t *row_blocks, *row_dist, *block_col;
+    uint8_t *dist;
+    void *vi;
+    double *values;
+} mgx_duvi_host;
+int mgx_duvi_encode(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val,
+                    double tol, int32_t mode, mgx_duvi_host *out);
+void mgx_duvi_host_free(mgx_duvi_host *m);
+
+/* SIMP compliance sensitivity, one thread per element (no reference counterpart; the hook
+ * is ElementDensities x Ke, FemCantileverBase.cs:99-100):
+ *   grad_e = -penal * rho_e^(penal-1) * (E0 - Emin) * u_e^T Ke0 u_e
+ * u is the free-dof displacement (device), rho and grad per element (device), on the
+ * structured cantilever mesh nx x ny x nz with x = 0 clamped. */
+int mgx_simp_gradient(int32_t nx, int32_t ny, int32_t nz, double nu, const double *rho, double penal,
+                      double Emin, double E0, const double *u, double *grad, void *stream);
+
+/* ================================================================== structured generator
+ * The reference builds K_ff, R and P on the host from DokRowMajor dictionaries
+ * (FemCantileverBase.cs:87-104, IGeometricMultigridModel.cs:76-194), which cannot reach
+ * 128^3.  These build them on the device, in the same sorted-CSR layout and the same
+ * element-order summation (bit-identical K_ff).  Matrices are owned by the library. */
+typedef struct mgx_matrix_s *mgx_matrix;
+
+/* device view of a matrix owned by the library */
+typedef struct {
+    int32_t nrows, ncols;
+    int64_t nnz;
+    int32_t *rowptr, *col;
+    double *val;
+} mgx_csr_view;
+
+/* K_ff of the x = 0 clamped cantilever on nx*ny*nz unit-cube-Ke hexahedra
+ * (FemCantilever3D.cs:32-120); dens (host, nx*ny*nz, may be NULL = 1) scales each element. */
+int mgx_matrix_cantilever(int32_t nx, int32_t ny, int32_t nz, double E, double nu, const double *dens,
+                          mgx_matrix *out);
+/* Reference restriction / interpolation pair of the level nx*ny*nz -> coarser
+ * (IGeometricMultigridModel.cs:76-194, cone weights, rows normalised, free-dof submatrix). */
+int mgx_matrix_transfer_reference(int32_t nx, int32_t ny, int32_t nz, mgx_matrix *R, mgx_matrix *P);
+/* Exact trilinear interpolation P (fine free x coarse free) and R = P^T. */
+int mgx_matrix_transfer_trilinear(int32_t nx, int32_t ny, int32_t nz, mgx_matrix *R, mgx_matrix *P);
+/* Galerkin hierarchy: K_0 from dens, K_{l+1} = P_l^T K_l P_l for trilinear P_l, computed
+ * element by element on the device; levels[l] receives K_l (nlevels entries). */
+int mgx_matrix_galerkin_hierarchy(int32_t nx, int32_t ny, int32_t nz, double E, double nu,
+                                  const double *dens, int32_t nlevels, mgx_matrix *levels);
+/* Upload a host CSR into a library-owned device matrix. */
+int mgx_matrix_from_host(int32_t nrows, int32_t ncols, const int32_t *rowptr, const int32_t *col,
+                         const double *val, mgx_matrix *out);
+int mgx_matrix_view(mgx_matrix m, mgx_csr_view *view);
+/* copy to host arrays sized from the view (rowptr[nrows+1], col[nnz], val[nnz]) */
+int mgx_matrix_download(mgx_matrix m, int32_t *rowptr, int32_t *col, double *val);
+int mgx_matrix_destroy(mgx_matrix m);
+
+/* ================================================================== 2. solver handle
+ * Replaces OpenCLCsrGeometricMultigridSolver / OpenCLDuViGeometricMultigridSolver
+ * (IGeometricMultigridSolver.cs:6-9, IOpenCLGeometricMultigridSolver.cs:3-6) and
+ * OpenCLCsrPcgSolver (OpenCLCsrPcgSolver.cs:40-331), plus the new MG-PCG composition. */
+
+#define MGX_METHOD_GMG        0   /* stationary V-cycle solver (OpenCL*GeometricMultigridSolver.Solve) */
+#define MGX_METHOD_PCG_JACOBI 1   /* OpenCLCsrPcgSolver.Solve */
+#define MGX_METHOD_MGPCG      2   /* PCG preconditioned by one symmetric V-cycle (new) */
+
+#define MGX_FORMAT_CSR   0
+#define MGX_FORMAT_DUVI  1
+
+#define MGX_SMOOTHER_JACOBI      0   /* damped Jacobi, omega * D^-1 */
+#define MGX_SMOOTHER_GS          1   /* multicolour Gauss-Seidel (symmetric in MG-PCG) */
+
+#define MGX_STOP_MAXNORM 0   /* reference: every |r_i| <= tol (CsrGeometricMultigrid.cl:138-140) */
+#define MGX_STOP_REL_L2  1   /* ||r||_2 <= tol * ||b||_2 */
+
+#define MGX_PRECISION_FP64  0
+#define MGX_PRECISION_MIXED 1   /* FP32 V-cycle and SpMV inside FP64 iterative refinement */
+
+typedef struct {
+    int32_t method;
+    int32_t format;
+    int32_t smoother;
+    int32_t coarse_relaxation;  /* reference flag: Jacobi weight 2/lambda from the coarsest matrix
+                                   (GeometricMultigridSolver.cs:144-151,209-214); used when omega <= 0 */
+    double  omega;              /* Jacobi damping; > 0 overrides the reference rule */
+    int32_t max_iterations;     /* cycles (GMG) or iterations (PCG) */
+    double  tolerance;
+    int32_t stop;
+    int32_t depth;              /* GMG: levelDown = [T x depth, F x depth] (CreateDeepV) */
+    int32_t level_iterations;   /* smoother sweeps per step (GMG) / pre and post sweeps (MG-PCG) */
+    int32_t precision;
+    int32_t nrhs;               /* right-hand sides solved together (batched block-PCG) */
+    double  duvi_tolerance;     /* value dedup tolerance (1e-10 in the reference) */
+    int32_t duvi_mode;          /* see mgx_duvi_encode */
+    int32_t inner_iterations;   /* mixed precision: max inner PCG iterations per refinement step */
+    double  inner_tolerance;    /* mixed precision: inner relative reduction */
+    int32_t profile;            /* 1: time every level-0 A*p SpMV with HIP events (mgx_stats.spmv_ms) */
+    int32_t check_every;        /* PCG iterations queued per host check of the device stop flag (default 4) */
+} mgx_options;
+
+typedef struct {
+    int32_t iterations;      /* cycles (GMG), 0-based iteration index (PCG_JACOBI, as the
+                                reference), or A*p products (MG-PCG) */
+    int32_t converged;       /* 1 converged, 0 iteration cap, -1 failed (NaN or > 1e50) */
+    double  residual_norm;   /* ||b - A x||_2 at exit */
+    double  rel_residual;    /* ||b - A x||_2 / ||b||_2 */
+    double  solve_ms;        /* device time of the solve */
+    int32_t outer_iterations;/* mixed precision: refinement steps */
+    int32_t spmv_launches;   /* level-0 A*p SpMV launches timed (options.profile) */
+    double  spmv_ms;         /* their summed device time (HIP events on the solver stream) */
+} mgx_stats;
+
+typedef struct mgx_solver_s *mgx_solver;
+
+void mgx_options_default(mgx_options *opt);
+int mgx_solver_create(const mgx_options *opt, mgx_solver *out);
+
+/* Level data, host arrays (copied at initialize, like the reference's blocking WriteBuffer,
+ * OpenCLCsrGeometricMultigridSolver.cs:194-220).  Level 0 is the finest.  R maps level ->
+ * level+1 and P maps level+1 -> level; both NULL on the coarsest level.  For MG-PCG, pass
+ * R = P^T.  For PCG_JACOBI set level 0 only. */
+typedef struct {
+    int32_t nrows, ncols;
+    int64_t nnz;
+    const int32_t *rowptr, *col;
+    const double *val;
+} mgx_csr_host;
+int mgx_solver_set_level(mgx_solver s, int32_t level, const mgx_csr_host *A,
+                         const mgx_csr_host *R, const mgx_csr_host *P);
+/* Same, from library-owned device matrices (no copy; they must outlive the solver). */
+int mgx_solver_set_level_device(mgx_solver s, int32_t level, mgx_matrix A, mgx_matrix R, mgx_matrix P);
+/* b: n x nrhs, column-major (one right-hand side after the other), host memory. */
+int mgx_solver_set_rhs(mgx_solver s, const double *b, int32_t n, int32_t nrhs);
+/* Builds D^-1, the Jacobi weights, the coarse factor, colourings, DuVi encodings and
+ * FP32 copies; uploads everything.  Replaces Initialize (OpenCLCsrGeometricMultigridSolver.cs:145-285). */
+int mgx_solver_initialize(mgx_solver s);
+/* x: n x nrhs host array; in: initial guess when has_x0, out: solution (the reference writes
+ * into xInitialGuess.RawData, OpenCLCsrGeometricMultigridSolver.cs:480).  stats: one per rhs. */
+int mgx_solver_solve(mgx_solver s, double *x, int32_t has_x0, mgx_stats *stats);
+/* Same with device-resident b and x (n x nrhs each, column-major); no host transfer. */
+int mgx_solver_solve_device(mgx_solver s, const double *b_dev, double *x_dev, int32_t has_x0, mgx_stats *stats);
+/* Replaces ReleaseOpenCLResources (OpenCLCsrGeometricMultigridSolver.cs:106-138). */
+int mgx_solver_release(mgx_solver s);
+
+/* ================================================================== plumbing
+ * Device memory and timing helpers so that a host program (the C# binding, the Python test
+ * harness) needs no other GPU runtime binding.  Plain hipMalloc / hipMemcpy wrappers. */
+int mgx_device_malloc(void **ptr, uint64_t bytes);
+int mgx_device_free(void *ptr);
+int mgx_memcpy_htod(void *dst, const void *src, uint64_t bytes);
+int mgx_memcpy_dtoh(void *dst, const void *src, uint64_t bytes);
+int mgx_memcpy_dtod(void *dst, const void *src, uint64_t bytes);
+int mgx_memset(void *dst, int32_t value, uint64_t bytes);
+int mgx_device_synchronize(void);
+/* Average device time (HIP events on one stream, after `warmup` untimed launches) of
+ * `repeats` back-to-back launches of a CSR row kernel on device arrays:
+ * kind 0: y = A x; 1: y = x + pre .* (b - A x); 2: y = b - A x; 3: the same as 0 with FP32
+ * values and vectors (val, x, y point to float). */
+int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
+                        const void *x, const void *b, const void *pre, void *y, int32_t warmup, int32_t repeats,
+                        double *avg_ms);
+/* Same for the DuVi SpMV (kind 0 only). */
+int mgx_time_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t warmup, int32_t repeats, double *avg_ms);
+
+/* ================================================================== workload helpers
+ * SplitMix64-seeded xoshiro256** (U[0,1) from the top 53 bits; N(0,1) by Box-Muller,
+ * cosine branch), used to draw element moduli / load cases reproducibly. */
+int mgx_rng_uniform(uint64_t seed, int64_t n, double *out);
+int mgx_rng_normal(uint64_t seed, int64_t n, double *out);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

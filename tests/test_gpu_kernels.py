@@ -1,0 +1,185 @@
+"""Device-level kernels of libmgx against the oracle, through the C ABI.
+FP64 results agree within 1e-12 relative (summation order differs: lanes of a wavefront
+reduce a row in a tree); index work and copies are bit-exact."""
+import json
+import os
+
+import numpy as np
+import pytest
+
+from compressiontests_amd import kernels as K
+from compressiontests_amd.device import DeviceArray, DeviceCsr, DeviceDuvi, duvi_encode
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+RTOL = 1e-12
+
+
+def close(a, b, rtol=RTOL):
+    scale = max(np.abs(b).max(), 1e-300)
+    return np.abs(a - b).max() <= rtol * scale
+
+
+@pytest.fixture(scope="module")
+def k16(gpu):
+    n = 16
+    dens = 10 ** (4 * O.rng_uniform(20261205, n ** 3) - 2)
+    A = O.cantilever(n, n, n, dens=dens)
+    return A, DeviceCsr.arrays_from_host(A)
+
+
+def test_opencl_diagonal_kat(gpu):
+    """GMCantileverOpenCLTests.cs:16-52: y_i == v_i^2 exactly."""
+    idx = np.arange(17, dtype=np.int32)
+    vals = np.arange(16, 0, -1).astype(np.float64)
+    A = DeviceCsr.arrays_from_host(O.Csr(idx, idx[:16].copy(), vals, 16, 16))
+    x = DeviceArray.from_numpy(vals)
+    y = DeviceArray(16)
+    K.matrix_vector_product(A, x, y)
+    assert np.array_equal(y.numpy(), vals * vals)
+
+
+def test_spmv_clear_and_accumulate(k16):
+    A, dA = k16
+    rng = np.random.default_rng(0)
+    x = rng.normal(size=A.nrows)
+    y0 = rng.normal(size=A.nrows)
+    dx, dy = DeviceArray.from_numpy(x), DeviceArray.from_numpy(y0)
+    K.matrix_vector_product(dA, dx, dy, clear=True)
+    assert close(dy.numpy(), O.spmv(A, x))
+    dy.copy_from(y0)
+    K.matrix_vector_product(dA, dx, dy, clear=False)
+    assert close(dy.numpy(), O.spmv(A, x, y0.copy(), clear=False))
+
+
+def test_residual_jacobi(k16):
+    A, dA = k16
+    rng = np.random.default_rng(1)
+    x, b = rng.normal(size=A.nrows), rng.normal(size=A.nrows)
+    pre = 0.6 * O.jacobi_preconditioner(A)
+    dx, db, dpre, dy = (DeviceArray.from_numpy(v) for v in (x, b, pre, np.zeros(A.nrows)))
+    K.residual(dA, db, dx, dy)
+    assert close(dy.numpy(), O.residual(A, b, x))
+    K.jacobi_iteration(dpre, dA, db, dx, dy)
+    assert close(dy.numpy(), O.jacobi(A, pre, b, x))
+    K.jacobi_initial_iteration(dpre, db, dy)
+    assert np.array_equal(dy.numpy(), O.jacobi_initial(pre, b))
+
+
+def test_residual_check_flags(k16):
+    A, dA = k16
+    x = np.zeros(A.nrows)
+    dx = DeviceArray.from_numpy(x)
+    r = DeviceArray(A.nrows)
+    for b, tol in ((np.full(A.nrows, 1e-9), 1e-8), (np.full(A.nrows, 1e-3), 1e-8)):
+        _, ref = O.residual_check(A, b, x, tol)
+        assert K.residual_with_check(dA, DeviceArray.from_numpy(b), dx, r, tol) == ref
+    b = np.full(A.nrows, 1e-3); b[17] = np.nan
+    assert K.residual_with_check(dA, DeviceArray.from_numpy(b), dx, r, 1e-8) & 2
+
+
+def test_dot(k16):
+    A, _ = k16
+    rng = np.random.default_rng(2)
+    for n in (1, 255, 256, 257, A.nrows, 3_000_001):
+        a, b = rng.normal(size=n), rng.normal(size=n)
+        got = K.dot(DeviceArray.from_numpy(a), DeviceArray.from_numpy(b))
+        assert abs(got - a @ b) <= 1e-12 * np.abs(a * b).sum()
+
+
+def test_edge_shapes(gpu):
+    """Empty rows, a single row, a row longer than a wavefront, nrows = 0."""
+    rows = [[], [0, 3, 7], list(range(0, 300)), [], [299]]
+    rp, ci = [0], []
+    for r in rows:
+        ci += r
+        rp.append(len(ci))
+    val = np.random.default_rng(3).normal(size=len(ci))
+    A = O.Csr(np.array(rp, np.int32), np.array(ci, np.int32), val, len(rows), 300)
+    dA = DeviceCsr.arrays_from_host(A)
+    x = np.random.default_rng(4).normal(size=300)
+    y = DeviceArray(len(rows))
+    K.matrix_vector_product(dA, DeviceArray.from_numpy(x), y)
+    assert close(y.numpy(), O.spmv(A, x))
+    E = DeviceCsr.arrays_from_host(O.Csr(np.zeros(1, np.int32), np.zeros(0, np.int32), np.zeros(0), 0, 0))
+    K.matrix_vector_product(E, DeviceArray(1), DeviceArray(1))
+
+
+def test_spmv_f32(k16):
+    A, _ = k16
+    dA32 = DeviceCsr.arrays_from_host(A, val_dtype=np.float32)
+    x = np.random.default_rng(5).normal(size=A.nrows)
+    dx = DeviceArray.from_numpy(x.astype(np.float32))
+    dy = DeviceArray(A.nrows, np.float32)
+    K.matrix_vector_product(dA32, dx, dy)
+    A32 = O.Csr(A.rowptr, A.col, A.val.astype(np.float32).astype(np.float64), A.nrows, A.ncols)
+    ref = O.spmv(A32, x.astype(np.float32).astype(np.float64))
+    scale = np.abs(A32.val).max() * np.abs(x).max() * 81
+    assert np.abs(dy.numpy() - ref).max() <= 1e-5 * scale
+
+
+def test_gs_color_matches_oracle(gpu):
+    n = 8
+    A = O.cantilever(n, n, n, dens=10 ** (O.rng_uniform(3, n ** 3) - 0.5))
+    dA = DeviceCsr.arrays_from_host(A)
+    ncol, color = O.block_coloring(A)
+    rng = np.random.default_rng(6)
+    b, x = rng.normal(size=A.nrows), rng.normal(size=A.nrows)
+    for backward in (False, True):
+        dx, db = DeviceArray.from_numpy(x), DeviceArray.from_numpy(b)
+        order = range(ncol - 1, -1, -1) if backward else range(ncol)
+        for c in order:
+            nodes = DeviceArray.from_numpy(np.nonzero(color == c)[0].astype(np.int32))
+            K.gauss_seidel_color(nodes, dA, db, dx, backward)
+        ref = O.gs_multicolor(A, ncol, color, b, x, backward)
+        assert close(dx.numpy(), ref, 1e-11)
+
+
+@pytest.mark.parametrize("mode", [0, 4, 8])
+def test_duvi_spmv_matches_oracle(gpu, mode):
+    n = 8
+    dens = None if mode == 0 else 10 ** (4 * O.rng_uniform(9, n ** 3) - 2)
+    A = O.cantilever(n, n, n, dens=dens)
+    d = duvi_encode(A, 1e-10, mode)
+    Dd = DeviceDuvi(d)
+    x = np.random.default_rng(7).normal(size=A.nrows)
+    y = DeviceArray(A.nrows)
+    Dd.spmv(DeviceArray.from_numpy(x), y)
+    ref = O.duvi_block_spmv(O.duvi_block_encode(A, 1e-10, mode), x)
+    assert close(y.numpy(), ref)
+
+
+def test_duvi_kat(gpu):
+    g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "duvi_kat.json")))
+    A = O.Csr(np.array(g["rowptr"], np.int32), np.array(g["col"], np.int32), np.array(g["val"], float), 2, 1024)
+    Dd = DeviceDuvi(duvi_encode(A))
+    x = np.zeros(1024)
+    for k, v in g["x_nonzeros"].items():
+        x[int(k)] = v
+    y = DeviceArray(2)
+    Dd.spmv(DeviceArray.from_numpy(x), y)
+    assert y.numpy().tolist() == g["y"]
+
+
+def test_simp_gradient_matches_oracle(gpu):
+    n = 6
+    rho = 0.1 + 0.9 * O.rng_uniform(11, n ** 3)
+    u = np.random.default_rng(8).normal(size=O.n_free(n, n, n))
+    g = K.simp_gradient(n, n, n, DeviceArray.from_numpy(rho), DeviceArray.from_numpy(u))
+    ref = O.simp_gradient(n, n, n, rho, u)
+    assert close(g.numpy(), ref)
+
+
+@pytest.mark.slow
+def test_spmv_64_parity(gpu):
+    """Config 2 size: 811,200 rows, 63.7M nnz, log-uniform moduli in [1e-2, 1e2]."""
+    n = 64
+    dens = 10 ** (4 * O.rng_uniform(20261205, n ** 3) - 2)
+    A = O.cantilever(n, n, n, dens=dens)
+    dA = DeviceCsr.arrays_from_host(A)
+    x = O.rng_normal(20261206, A.nrows)
+    y = DeviceArray(A.nrows)
+    K.matrix_vector_product(dA, DeviceArray.from_numpy(x), y)
+    ref = O.spmv(A, x)
+    err = np.abs(y.numpy() - ref) / (np.abs(A.val).max() * np.abs(x).max())
+    assert err.max() <= 1e-12

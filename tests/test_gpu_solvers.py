@@ -1,0 +1,174 @@
+"""Solver handle (C ABI) against the oracle drivers: the stationary V-cycle (CSR and DuVi,
+Jacobi and multicolour GS), Jacobi PCG, and MG-PCG.  Parity = same iteration count (+-1,
+the stop test sits on a tolerance boundary and sums round differently) and solutions that
+agree to the solve tolerance; MG-PCG additionally reproduces the manufactured exact solution."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from compressiontests_amd import _lib as L
+from compressiontests_amd.model import FemCantilever3D
+from compressiontests_amd.solvers import (HipCsrGeometricMultigridSolver, HipCsrPcgSolver,
+                                          HipDuViGeometricMultigridSolver, HipMgPcgSolver)
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def S(M):
+    return sp.csr_matrix((M.val, M.col, M.rowptr), shape=(M.nrows, M.ncols))
+
+
+def energy_err(A, x, xs):
+    e = x - xs
+    return np.sqrt(e @ (S(A) @ e)) / np.sqrt(xs @ (S(A) @ xs))
+
+
+def oracle_reference_levels(n, depth):
+    lv, m, A = [], n, O.cantilever(n, n, n)
+    for _ in range(depth):
+        R, P = O.rp_reference(m, m, m)
+        lv.append(dict(A=A, R=R, P=P))
+        m //= 2
+        A = O.cantilever(m, m, m)
+    lv.append(dict(A=A))
+    return lv
+
+
+def oracle_galerkin_levels(n, nlev, dens=None):
+    lv, m, A = [], n, O.cantilever(n, n, n, dens=dens)
+    for _ in range(nlev - 1):
+        P = O.p_trilinear(m, m, m)
+        lv.append(dict(A=A, R=O.transpose(P), P=P))
+        A = O.galerkin(A, P)
+        m //= 2
+    lv.append(dict(A=A))
+    return lv
+
+
+@pytest.fixture(scope="module")
+def m16(gpu):
+    m = FemCantilever3D((16, 16, 16))
+    A, U, b = m.CreateAndSolveLinearSystem()
+    return m, A, U, b
+
+
+@pytest.mark.parametrize("gs", [False, True])
+@pytest.mark.parametrize("cls", [HipCsrGeometricMultigridSolver, HipDuViGeometricMultigridSolver])
+def test_gmg_parity(m16, gs, cls):
+    if gs and cls is HipDuViGeometricMultigridSolver:
+        pytest.skip("multicolour GS runs on CSR; the DuVi GS of the reference is the racy variant")
+    m, A, U, b = m16
+    depth, it, tol = 2, 4, 1e-5
+    solver = cls.CreateDeepV(m, GaussSeidel=gs, maxCircleIterations=20000, coarseRelaxation=True,
+                             convergenceTolerance=tol, depth=depth, levelIterations=it)
+    x, st, times = solver.Solve(None)
+    solver.ReleaseOpenCLResources()
+    lv = oracle_reference_levels(16, depth)
+    lam = O.eigen_upper_bound(lv[-1]["A"])
+    pre = [O.jacobi_preconditioner(l["A"]) * (2 / lam) for l in lv[:-1]]
+    xo, so = O.gmg_solve(lv, b, pre=pre, smoother=1 if gs else 0, level_iterations=it, max_cycles=20000, tol=tol)
+    assert st.HasConverged and so.converged == 1
+    assert abs(st.NumIterationsRequired - so.iterations) <= 1
+    assert np.abs(b - O.spmv(A, x)).max() <= tol
+    assert energy_err(A, x, xo) < 1e-5
+    assert len(times) == depth + 1
+
+
+def test_pcg_jacobi_parity(m16):
+    m, A, U, b = m16
+    s = HipCsrPcgSolver(20000, 1e-5)
+    s.Initialize(m)
+    x, st = s.Solve(None)
+    xo, so = O.pcg_jacobi(A, b, max_it=20000, tol=1e-5)
+    assert st.HasConverged and so.converged == 1
+    assert abs(st.NumIterationsRequired - so.iterations) <= 1
+    assert np.abs(b - O.spmv(A, x)).max() <= 1e-5
+    assert energy_err(A, x, xo) < 1e-6
+
+
+@pytest.mark.parametrize("smoother", ["jacobi", "gs"])
+def test_mgpcg_manufactured_and_parity(m16, smoother):
+    m, A, U, b = m16
+    s = HipMgPcgSolver(m, levels=3, smoother=smoother, omega=0.6, nu=2, tolerance=1e-10)
+    s.Initialize(b)
+    x, st = s.Solve()
+    xo, so = O.mgpcg(oracle_galerkin_levels(16, 3), b, smoother=1 if smoother == "gs" else 0, omega=0.6, nu=2,
+                     tol=1e-10)
+    assert st.HasConverged and so.converged == 1
+    assert abs(st.NumIterationsRequired - so.iterations) <= max(1, so.iterations // 10)
+    assert st.RelativeResidual <= 1e-10 * 1.01
+    assert energy_err(A, x, U) < 1e-8
+    assert energy_err(A, x, xo) < 1e-9
+
+
+def test_mgpcg_heterogeneous_tip_load(gpu):
+    n = 16
+    dens = 10 ** (4 * O.rng_uniform(20261205, n ** 3) - 2)
+    m = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
+    f = m.rhs()
+    A = O.cantilever(n, n, n, dens=dens)
+    xo, so = O.mgpcg(oracle_galerkin_levels(n, 3, dens), f, smoother=1, nu=2, tol=1e-12)
+    for smoother in ("jacobi", "gs"):
+        s = HipMgPcgSolver(m, levels=3, smoother=smoother, nu=2, tolerance=1e-8)
+        s.Initialize(f)
+        x, st = s.Solve()
+        assert st.HasConverged
+        assert energy_err(A, x, xo) < 1e-8
+
+
+def test_warm_start_and_caps(m16):
+    m, A, U, b = m16
+    s = HipMgPcgSolver(m, levels=3, smoother="gs", nu=1, tolerance=1e-8)
+    s.Initialize(b)
+    x, st = s.Solve()
+    assert st.HasConverged
+    x2, st2 = s.Solve(x)          # warm start from the solution: stop test passes at once
+    assert st2.HasConverged and st2.NumIterationsRequired == 0
+    assert np.array_equal(x2, x)
+    s2 = HipMgPcgSolver(m, levels=3, smoother="gs", nu=1, tolerance=0.0, max_iterations=3)
+    s2.Initialize(b)
+    x3, st3 = s2.Solve()
+    assert not st3.HasConverged and not st3.Failed and st3.NumIterationsRequired == 3
+
+
+def test_nan_rhs_reports_failure(m16):
+    m, A, U, b = m16
+    bb = b.copy(); bb[100] = np.nan
+    s = HipMgPcgSolver(m, levels=3, smoother="jacobi", nu=2, tolerance=1e-8)
+    s.Initialize(bb)
+    x, st = s.Solve()
+    assert st.Failed and not st.HasConverged
+    g = HipCsrGeometricMultigridSolver(100, 1e-5, [True, False], [2], GaussSeidel=False)
+    with pytest.raises(L.MgxInvalidOperation):
+        g.Solve(None)
+    with pytest.raises(L.MgxNotImplemented):
+        HipCsrGeometricMultigridSolver(100, 1e-5, [False, True], [2]).Initialize(m)
+
+
+def test_duvi_mgpcg(m16):
+    m, A, U, b = m16
+    s = HipMgPcgSolver(m, levels=3, smoother="jacobi", nu=2, tolerance=1e-9, format="duvi")
+    s.Initialize(b)
+    x, st = s.Solve()
+    assert st.HasConverged
+    assert energy_err(A, x, U) < 1e-7
+
+
+@pytest.mark.slow
+def test_mgpcg_64_heterogeneous(gpu):
+    """Config 2 moduli at 64^3 (811,200 dofs): the solve reaches 1e-8 relative residual and
+    agrees with a 1e-12 solve of the same system to 1e-8 in the energy norm."""
+    n = 64
+    dens = 10 ** (4 * O.rng_uniform(20261205, n ** 3) - 2)
+    m = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
+    f = m.rhs()
+    s = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-8)
+    s.Initialize(f)
+    x, st = s.Solve()
+    assert st.HasConverged and st.RelativeResidual <= 1.01e-8
+    t = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-12)
+    t.Initialize(f)
+    xt, stt = t.Solve()
+    A = O.cantilever(n, n, n, dens=dens)
+    assert energy_err(A, x, xt) < 1e-8

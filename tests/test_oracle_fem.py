@@ -1,0 +1,225 @@
+"""Pins the oracle's FE generator and transfer operators against mathematics that does not
+come from the oracle itself: textbook Gauss quadrature, closed-form stencil counts (the
+sizes the reference's free-dof systems have), a brute-force element scatter, and the
+closed-form cone weights of IGeometricMultigridModel.cs:121-185."""
+import itertools
+import math
+
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+
+CORNERS = [(0, 0, 0), (1, 0, 0), (1, 1, 0), (0, 1, 0), (0, 0, 1), (1, 0, 1), (1, 1, 1), (0, 1, 1)]
+
+
+def gauss_ke(E=1.0, nu=0.3, side=2.0):
+    """Trilinear hexahedron on a cube of edge `side`, 2x2x2 Gauss points, corner order CORNERS.
+    The top3d closed form (FemCantilever3D.cs:32-120) integrates over the reference cube
+    [-1, 1]^3 with an identity Jacobian, i.e. an element of edge 2."""
+    lam = E * nu / ((1 + nu) * (1 - 2 * nu))
+    mu = E / (2 * (1 + nu))
+    D = np.zeros((6, 6))
+    D[:3, :3] = lam
+    D[np.arange(3), np.arange(3)] += 2 * mu
+    D[3:, 3:] = np.eye(3) * mu
+    g = 1 / math.sqrt(3)
+    K = np.zeros((24, 24))
+    for xi in itertools.product((-g, g), repeat=3):
+        dN = np.zeros((8, 3))
+        for a, c in enumerate(CORNERS):
+            s = [2 * ci - 1 for ci in c]
+            for d in range(3):
+                t = 0.125 * s[d]
+                for e in range(3):
+                    if e != d:
+                        t *= 1 + s[e] * xi[e]
+                dN[a, d] = t * (2.0 / side)      # d/dx = (2 / side) d/dxi
+        B = np.zeros((6, 24))
+        for a in range(8):
+            B[0, 3 * a] = dN[a, 0]; B[1, 3 * a + 1] = dN[a, 1]; B[2, 3 * a + 2] = dN[a, 2]
+            B[3, 3 * a] = dN[a, 1]; B[3, 3 * a + 1] = dN[a, 0]
+            B[4, 3 * a + 1] = dN[a, 2]; B[4, 3 * a + 2] = dN[a, 1]
+            B[5, 3 * a] = dN[a, 2]; B[5, 3 * a + 2] = dN[a, 0]
+        K += B.T @ D @ B * (side / 2) ** 3
+    return K
+
+
+@pytest.mark.parametrize("nu", [0.3, 0.0, 0.45])
+def test_ke_matches_gauss_quadrature(nu):
+    Ke = O.ke_hex8(1.0, nu)
+    Kg = gauss_ke(1.0, nu)
+    assert np.abs(Ke - Kg).max() <= 1e-13 * np.abs(Kg).max()
+
+
+def test_ke_invariants():
+    Ke = O.ke_hex8(2.5, 0.3)
+    assert np.array_equal(Ke, Ke.T)
+    w = np.linalg.eigvalsh(Ke)
+    assert np.sum(np.abs(w) < 1e-12 * w.max()) == 6
+    # rigid rotations about each axis are in the null space for this corner order
+    X = np.array(CORNERS, float)
+    for axis in range(3):
+        u = np.zeros((8, 3))
+        e = np.zeros(3); e[axis] = 1
+        u[:] = np.cross(e, X)
+        assert np.abs(Ke @ u.reshape(-1)).max() < 1e-12
+    # E scales linearly
+    assert np.allclose(O.ke_hex8(2.0, 0.3), 2 * O.ke_hex8(1.0, 0.3), rtol=0, atol=1e-15)
+
+
+@pytest.mark.parametrize("n,free,nnz", [(16, 13872, 994014), (64, 811200, 63695790),
+                                         (96, 2709792, 214983054), (128, 6390144, 509597550)])
+def test_stencil_sizes(n, free, nnz):
+    """Free-dof count and nnz of K_ff: the 27-node stencil restricted to ix >= 1."""
+    assert O.n_free(n, n, n) == free
+    assert O.lib().orc_cantilever_nnz(n, n, n) == nnz
+
+
+def brute_assemble(nx, ny, nz, dens, Ke):
+    """Dense scatter-add of rho_e * Ke in element order (FemCantileverBase.cs:95-101),
+    then the free-dof submatrix (ix >= 1)."""
+    nn = (nx + 1) * (ny + 1) * (nz + 1)
+    K = np.zeros((3 * nn, 3 * nn))
+    e = 0
+    for ez in range(nz):
+        for ey in range(ny):
+            for ex in range(nx):
+                nodes = [(ex + c[0]) + (nx + 1) * ((ey + c[1]) + (ny + 1) * (ez + c[2])) for c in CORNERS]
+                dofs = [3 * n + d for n in nodes for d in range(3)]
+                for i, gi in enumerate(dofs):
+                    for j, gj in enumerate(dofs):
+                        K[gi, gj] = dens[e] * Ke[i, j] + K[gi, gj]
+                e += 1
+    free = [3 * n + d for n in range(nn) if n % (nx + 1) != 0 for d in range(3)]
+    return K[np.ix_(free, free)]
+
+
+@pytest.mark.parametrize("dims", [(2, 2, 2), (3, 2, 4), (4, 3, 2)])
+def test_assembly_bit_exact_against_scatter(dims):
+    nx, ny, nz = dims
+    rng = np.random.default_rng(7)
+    dens = 10 ** rng.uniform(-2, 2, nx * ny * nz)
+    A = O.cantilever(nx, ny, nz, dens=dens)
+    Kd = brute_assemble(nx, ny, nz, dens, O.ke_hex8())
+    D = np.zeros_like(Kd)
+    for i in range(A.nrows):
+        cols = A.col[A.rowptr[i]:A.rowptr[i + 1]]
+        assert np.all(np.diff(cols) > 0), "columns must be sorted (BuildCsrArrays(true))"
+        D[i, cols] = A.val[A.rowptr[i]:A.rowptr[i + 1]]
+    assert np.array_equal(D, Kd)
+    assert np.array_equal(D, D.T)
+
+
+def test_rigid_translation_free_rows():
+    """Rows of nodes away from the clamp sum to zero against a rigid translation."""
+    n = 4
+    A = O.cantilever(n, n, n)
+    for d in range(3):
+        u = np.zeros(A.nrows); u[d::3] = 1.0
+        y = O.spmv(A, u)
+        # rows whose stencil does not reach the clamped plane (ix >= 2)
+        ix = (np.arange(A.nrows) // 3) % n + 1
+        assert np.abs(y[ix >= 2]).max() < 1e-13
+
+
+def test_manufactured_solution_closed_form():
+    """Tip deflection of U_f equals the Euler-Bernoulli closed form -q L^4 / (8 E I)."""
+    n = 8
+    L = (1.0, 1.0, 1.0)
+    U = O.manufactured(n, n, n, L)
+    I = L[1] * L[2] ** 3 / 12
+    w_tip = U.reshape(n + 1, n + 1, n, 3)[:, :, n - 1, 2]
+    assert np.allclose(w_tip, -1.0 / (8 * I), rtol=1e-14, atol=0)
+    assert np.all(U.reshape(-1, 3)[:, 1] == 0)
+
+
+def test_tip_load_total():
+    f = O.tip_load(8, 8, 8, -1.0)
+    assert math.isclose(f.sum(), -1.0, rel_tol=1e-14)
+    assert np.count_nonzero(f) == 81
+
+
+def test_dok_order_multiply_close_to_sorted():
+    n = 6
+    A = O.cantilever(n, n, n)
+    U = O.manufactured(n, n, n)
+    b1 = O.multiply_dok_order(n, n, n, A, U)
+    b2 = O.spmv(A, U)
+    assert np.abs(b1 - b2).max() <= 1e-13 * np.abs(b2).max()
+
+
+def cone_weight(d):
+    return 1 - math.sqrt(sum((t / 2) ** 2 for t in d))
+
+
+def test_reference_restriction_interior_weights():
+    n = 8
+    R, P = O.rp_reference(n, n, n)
+    s = sum(cone_weight(d) for d in itertools.product((-1, 0, 1), repeat=3))
+    # an interior coarse node: (2, 2, 2) of the 5^3 coarse grid (free index along x: 2-1)
+    cn = (2 * 5 + 2) * 4 + (2 - 1)
+    row = 3 * cn
+    vals = R.val[R.rowptr[row]:R.rowptr[row + 1]]
+    assert len(vals) == 27
+    assert math.isclose(vals.sum(), 1.0, rel_tol=1e-14)
+    expected = sorted(cone_weight(d) / s for d in itertools.product((-1, 0, 1), repeat=3))
+    assert np.allclose(sorted(vals), expected, rtol=1e-15, atol=0)
+    # the three dofs of a node carry the same weights
+    assert np.array_equal(R.val[R.rowptr[row]:R.rowptr[row + 1]], R.val[R.rowptr[row + 1]:R.rowptr[row + 2]])
+
+
+def test_reference_interpolation_is_trilinear():
+    for n in (4, 8):
+        R, P = O.rp_reference(n, n, n)
+        T = O.p_trilinear(n, n, n)
+        assert np.array_equal(P.rowptr, T.rowptr) and np.array_equal(P.col, T.col)
+        assert np.abs(P.val - T.val).max() <= 2 * np.finfo(float).eps
+
+
+def brute_rp(n):
+    """Dense restatement of IGeometricMultigridModel.cs:76-194 with the clamped dofs dropped."""
+    nf, nc, s = n + 1, n // 2 + 1, 2.0
+    Rf = np.zeros((nc ** 3, nf ** 3))
+    sumP = np.zeros(nf ** 3)
+    for a0 in range(nc):
+        for a2 in range(nc):
+            for a1 in range(nc):
+                a = (a0, a1, a2)
+                cur = [ai * s for ai in a]
+                lo = [max(0, math.ceil(math.nextafter(c - s, math.inf))) for c in cur]
+                hi = [min(nf, math.floor(math.nextafter(c + s, -math.inf)) + 1) for c in cur]
+                cid = a[0] + nc * (a[1] + nc * a[2])
+                tot = 0.0
+                for w0 in range(lo[0], hi[0]):
+                    for w2 in range(lo[2], hi[2]):
+                        for w1 in range(lo[1], hi[1]):
+                            w = (w0, w1, w2)
+                            q = 0.0
+                            for i in range(3):
+                                t = (w[i] - cur[i]) / s
+                                q += t * t
+                            f = 1 - math.sqrt(q)
+                            fid = w[0] + nf * (w[1] + nf * w[2])
+                            tot += f
+                            sumP[fid] += f
+                            Rf[cid, fid] = f
+                Rf[cid] /= tot
+    return Rf, sumP
+
+
+def test_reference_rp_against_dense_restatement():
+    n = 4
+    R, P = O.rp_reference(n, n, n)
+    Rd, sumP = brute_rp(n)
+    nf, nc = n + 1, n // 2 + 1
+    ffree = [i for i in range(nf ** 3) if i % nf != 0]
+    cfree = [i for i in range(nc ** 3) if i % nc != 0]
+    for dof in range(3):
+        for ri, cid in enumerate(cfree):
+            row = 3 * ri + dof
+            cols = R.col[R.rowptr[row]:R.rowptr[row + 1]]
+            vals = R.val[R.rowptr[row]:R.rowptr[row + 1]]
+            dense = np.zeros(len(ffree) * 3)
+            dense[cols] = vals
+            assert np.array_equal(dense[dof::3], Rd[cid, ffree])
