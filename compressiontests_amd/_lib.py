@@ -147,6 +147,7 @@ _SIGS = {
     "mgx_device_synchronize": (C.c_int, []),
     "mgx_time_csr_kernel": (C.c_int, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, C.POINTER(_D)]),
     "mgx_time_duvi_spmv": (C.c_int, [C.POINTER(Duvi), _P, _P, _I32, _I32, C.POINTER(_D)]),
+    "mgx_time_copy_kernel": (C.c_int, [_P, _P, _U64, _I32, _I32, C.POINTER(_D)]),
     "mgx_rng_uniform": (C.c_int, [_U64, _I64, _P]),
     "mgx_rng_normal": (C.c_int, [_U64, _I64, _P]),
 }

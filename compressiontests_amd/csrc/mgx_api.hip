@@ -221,19 +221,21 @@ __global__ __launch_bounds__(kBlock) void k_pcg_norms(int32_t n, const double *_
             const double t = sh[2][i];
             mm = (mm != mm) ? mm : ((t != t) ? t : (t > mm ? t : mm));
         }
-        __hip_atomic_store(&partials[3 * blockIdx.x + 0], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[3 * blockIdx.x + 1], c, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[3 * blockIdx.x + 2], mm, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        partials[3 * blockIdx.x + 0] = a;
+        partials[3 * blockIdx.x + 1] = c;
+        partials[3 * blockIdx.x + 2] = mm;
         const uint32_t t = __hip_atomic_fetch_add(&ctl->counter[4], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         is_last = (t == gridDim.x - 1);
     }
     __syncthreads();
     if (!is_last || threadIdx.x != 0) return;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    const volatile double *vp = partials;
     double a = 0, c = 0, mm = 0;
     for (int i = 0; i < (int)gridDim.x; ++i) {
-        a += __hip_atomic_load(&partials[3 * i + 0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        c += __hip_atomic_load(&partials[3 * i + 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        const double t = __hip_atomic_load(&partials[3 * i + 2], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
+        a += vp[3 * i + 0];
+        c += vp[3 * i + 1];
+        const double t = vp[3 * i + 2];
         mm = (mm != mm) ? mm : ((t != t) ? t : (t > mm ? t : mm));
     }
     __hip_atomic_store(&ctl->counter[4], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -462,7 +464,16 @@ static void initialize(mgx_solver_s *S)
         }
     }
     if (S->opt.method != MGX_METHOD_PCG_JACOBI && S->opt.smoother == MGX_SMOOTHER_JACOBI) {
-        if (S->opt.omega > 0) {
+        if (S->opt.method == MGX_METHOD_MGPCG && S->opt.omega <= 0) {
+            // per level omega = 2 / (1.2 lambda), lambda from the power method of
+            // RelaxedJacobiPreconditioner (GeometricMultigridSolver.cs:80-121): a convergent
+            // smoother whatever the modulus contrast
+            for (int l = 0; l < nsmooth; ++l) {
+                const double lam = 1.2 * power_bound(hrp[l], hci[l], hv[l], invd[l]);
+                const double w = 2 / lam;
+                for (auto &t : invd[l]) t *= w;
+            }
+        } else if (S->opt.omega > 0) {
             for (auto &d : invd) for (auto &t : d) t *= S->opt.omega;
         } else if (S->opt.coarse_relaxation) {
             // RelaxateJacobiPreconditioners: 2 / lambda of the coarsest matrix, GeometricMultigridSolver.cs:144-167,209-214
@@ -1288,6 +1299,39 @@ int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, cons
         MGX_HIP(hipEventRecord(e0, s));
         for (int i = 0; i < repeats; ++i) one();
         MGX_HIP(hipEventRecord(e1, s));
+        MGX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = ms / repeats;
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    });
+}
+
+__global__ __launch_bounds__(kBlock) void k_copy16(const uint4 *__restrict__ src, uint4 *__restrict__ dst, int64_t n)
+{
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) dst[i] = src[i];
+}
+
+int mgx_time_copy_kernel(const void *src, void *dst, uint64_t bytes, int32_t warmup, int32_t repeats, double *avg_ms)
+{
+    return guard([&] {
+        MGX_REQUIRE(src && dst && avg_ms && bytes % 16 == 0 && repeats >= 1 && warmup >= 0, MGX_ERR_INVALID_ARGUMENT,
+                    "invalid arguments");
+        hipStream_t s;
+        MGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t e0, e1;
+        MGX_HIP(hipEventCreate(&e0));
+        MGX_HIP(hipEventCreate(&e1));
+        const int64_t n = (int64_t)(bytes / 16);
+        const int grid = 256 * 16;
+        for (int i = 0; i < warmup; ++i)
+            hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(kBlock), 0, s, (const uint4 *)src, (uint4 *)dst, n);
+        MGX_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < repeats; ++i)
+            hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(kBlock), 0, s, (const uint4 *)src, (uint4 *)dst, n);
+        MGX_HIP(hipEventRecord(e1, s));
+        MGX_HIP(hipGetLastError());
         MGX_HIP(hipEventSynchronize(e1));
         float ms = 0;
         MGX_HIP(hipEventElapsedTime(&ms, e0, e1));

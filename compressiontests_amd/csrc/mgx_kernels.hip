@@ -72,24 +72,29 @@ __device__ __forceinline__ void block_reduce3(double &a, double &b, double &m)
 }
 
 // Grid reduction; returns true in thread 0 of the last block, which holds the totals.
+// Each block publishes its partials and takes a ticket (agent-scope acq_rel RMW: release of
+// the partials); the last block issues one agent-scope acquire fence per thread and then
+// reads all partials with plain loads in a fixed order (deterministic sum).
 __device__ bool grid_finish3(double &a, double &b, double &m, double *partials, uint32_t *counter)
 {
     __shared__ int is_last;
     block_reduce3(a, b, m);
     if (threadIdx.x == 0) {
-        __hip_atomic_store(&partials[3 * blockIdx.x + 0], a, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[3 * blockIdx.x + 1], b, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        __hip_atomic_store(&partials[3 * blockIdx.x + 2], m, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        partials[3 * blockIdx.x + 0] = a;
+        partials[3 * blockIdx.x + 1] = b;
+        partials[3 * blockIdx.x + 2] = m;
         uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         is_last = (t == gridDim.x - 1);
     }
     __syncthreads();
     if (!is_last) return false;
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
     double sa = 0, sb = 0, sm = 0;
+    const volatile double *vp = partials;
     for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-        sa += __hip_atomic_load(&partials[3 * i + 0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        sb += __hip_atomic_load(&partials[3 * i + 1], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT);
-        sm = nanmax(sm, __hip_atomic_load(&partials[3 * i + 2], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT));
+        sa += vp[3 * i + 0];
+        sb += vp[3 * i + 1];
+        sm = nanmax(sm, vp[3 * i + 2]);
     }
     block_reduce3(sa, sb, sm);
     if (threadIdx.x == 0) {
@@ -327,7 +332,17 @@ void launch_pcg_init(int32_t n, const double *b, double *r, double *x, Ctl *ctl,
     MGX_HIP(hipGetLastError());
 }
 
-// q = A p and p.q -> alpha = rz / pq
+// q = A p and p.q -> alpha = rz / pq.  A bounded grid (<= 2048 blocks, each a contiguous range
+// of rows, XCD-remapped) keeps the reduction to a few thousand partials.
+static inline int grid_rows_bounded(int64_t nrows, int rows_per_pass, int max_blocks)
+{
+    int64_t nb = (nrows + rows_per_pass - 1) / rows_per_pass;
+    if (nb > max_blocks) nb = max_blocks;
+    if (nb < 1) nb = 1;
+    nb = (nb + 7) & ~int64_t(7);
+    return (int)nb;
+}
+
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot_alpha(int32_t nrows, const int32_t *__restrict__ rp,
                                                            const int32_t *__restrict__ ci, const double *__restrict__ v,
@@ -337,12 +352,19 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot_alpha(int32_t nrows, const 
     if (ctl->done) return;
     constexpr int RPB = kBlock / G;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
-    const int row = blk * RPB + threadIdx.x / G;
+    const int64_t passes = ((int64_t)nrows + RPB - 1) / RPB;
+    const int64_t per = (passes + gridDim.x - 1) / gridDim.x;
+    const int64_t r0 = (int64_t)blk * per * RPB;
+    int64_t r1 = r0 + per * RPB;
+    if (r1 > nrows) r1 = nrows;
     const int lane = threadIdx.x % G;
     double contrib = 0, z = 0, m = 0;
-    if (row < nrows) {
-        const double s = row_dot<G, double>(rp, ci, v, p, row, lane);
-        if (lane == 0) { q[row] = s; contrib = p[row] * s; }
+    for (int64_t base = r0; base < r1; base += RPB) {
+        const int64_t row = base + threadIdx.x / G;
+        if (row < r1) {
+            const double s = row_dot<G, double>(rp, ci, v, p, (int)row, lane);
+            if (lane == 0) { q[row] = s; contrib += p[row] * s; }
+        }
     }
     if (grid_finish3(contrib, z, m, partials, &ctl->counter[1])) {
         ctl->pq = contrib;
@@ -353,8 +375,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot_alpha(int32_t nrows, const 
 void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *p,
                            double *q, int lanes, Ctl *ctl, double *partials, hipStream_t s)
 {
-#define MGX_SDA(G) hipLaunchKernelGGL((k_spmv_dot_alpha<G>), dim3(grid_rows(n, kBlock / G)), dim3(kBlock), 0, s, \
-                                      n, rp, ci, v, p, q, ctl, partials)
+#define MGX_SDA(G) hipLaunchKernelGGL((k_spmv_dot_alpha<G>), dim3(grid_rows_bounded(n, kBlock / G, 2048)), dim3(kBlock), \
+                                      0, s, n, rp, ci, v, p, q, ctl, partials)
     switch (lanes) {
     case 2: MGX_SDA(2); break;
     case 4: MGX_SDA(4); break;

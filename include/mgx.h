@@ -271,6 +271,9 @@ int mgx_device_synchronize(void);
 int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
                         const void *x, const void *b, const void *pre, void *y, int32_t warmup, int32_t repeats,
                         double *avg_ms);
+/* Average device time of a plain copy kernel (16-byte vector loads and stores) moving `bytes`
+ * (a multiple of 16) from src to dst: the calibration of HBM byte counters. */
+int mgx_time_copy_kernel(const void *src, void *dst, uint64_t bytes, int32_t warmup, int32_t repeats, double *avg_ms);
 /* Same for the DuVi SpMV (kind 0 only). */
 int mgx_time_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t warmup, int32_t repeats, double *avg_ms);
 

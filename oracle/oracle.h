@@ -76,6 +76,7 @@ void orc_csr_jacobi(int32_t n, const double *pre, const int32_t *rp, const int32
 void orc_jacobi_preconditioner(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, double *d);
 double orc_eigen_upper_bound(int32_t n, const int32_t *rp, const double *v);
 double orc_dot(int32_t n, const double *a, const double *b);
+double orc_power_bound(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *invd);
 void orc_gs_forward(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b, double *x);
 /* node-block greedy colouring (block = 3 dofs); returns number of colours */
 int orc_block_coloring(int32_t nblocks, int bs, const int32_t *rp, const int32_t *ci, int32_t *color);
@@ -149,7 +150,8 @@ int orc_gmg_solve(int nlevels, const orc_level *lv, const double *const *pre, in
 int orc_pcg_jacobi(int32_t n, const int32_t *rp, const int32_t *ci, const double *v,
                    const double *b, double *x, int has_x0, int max_it, double tol, orc_stats *st);
 /* MG-preconditioned CG (new composition): symmetric V-cycle, R = P^T,
-   nu pre/post sweeps; smoother 0 = Jacobi(omega), 1 = symmetric multicolour GS.
+   nu pre/post sweeps; smoother 0 = Jacobi(omega; omega <= 0: per level 2 / (1.2 lambda) with
+   lambda from orc_power_bound), 1 = symmetric multicolour GS.
    stop: 0 = max-norm (reference), 1 = ||r||/||b|| <= tol. */
 int orc_vcycle_apply(int nlevels, const orc_level *lv, int smoother, double omega, int nu,
                      const double *r, double *z);

@@ -62,6 +62,7 @@ def lib():
         L.orc_dedup.restype = C.c_int64
         L.orc_eigen_upper_bound.restype = C.c_double
         L.orc_dot.restype = C.c_double
+        L.orc_power_bound.restype = C.c_double
         L.orc_splitmix64_next.restype = C.c_uint64
     return _LIB
 
@@ -225,6 +226,11 @@ def jacobi_preconditioner(A):
 
 def eigen_upper_bound(A):
     return lib().orc_eigen_upper_bound(A.nrows, _p(A.rowptr), _p(A.val))
+
+
+def power_bound(A, invd):
+    invd = np.ascontiguousarray(invd, np.float64)
+    return lib().orc_power_bound(A.nrows, _p(A.rowptr), _p(A.col), _p(A.val), _p(invd))
 
 
 def gs_forward(A, b, x):

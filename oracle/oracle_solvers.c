@@ -226,7 +226,9 @@ static int vcyc_setup(vcyc *Vp, int nlevels, const orc_level *lv, int smoother, 
             } else {
                 V.pre[l] = dalloc(lv[l].n);
                 orc_jacobi_preconditioner(lv[l].n, lv[l].Arp, lv[l].Aci, lv[l].Av, V.pre[l]);
-                for (int32_t i = 0; i < lv[l].n; ++i) V.pre[l][i] *= omega;
+                double w = omega;
+                if (omega <= 0) w = 2 / (1.2 * orc_power_bound(lv[l].n, lv[l].Arp, lv[l].Aci, lv[l].Av, V.pre[l]));
+                for (int32_t i = 0; i < lv[l].n; ++i) V.pre[l][i] *= w;
             }
         }
     }

@@ -96,6 +96,36 @@ double orc_eigen_upper_bound(int32_t n, const int32_t *rp, const double *v)
     return l;
 }
 
+/* EigenValueUpperBound (power method), GeometricMultigridSolver.cs:80-103:
+   x = 1/sqrt(n); repeat y = D^-1 A x, l2 = ||y||, x = y / l2, ratio = l2 / l until
+   0.95 < ratio < 1.05, then l = ratio > 1 ? l2 ratio^(1/sqrt(0.05)) : (9 l2 + l) / 10. */
+double orc_power_bound(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *invd)
+{
+    const double percent = 0.05;
+    double *x = (double *)malloc(sizeof(double) * n), *y = (double *)malloc(sizeof(double) * n);
+    for (int32_t i = 0; i < n; ++i) x[i] = 1 / sqrt((double)n);
+    double l = 1;
+    for (int it = 0; it < 100000; ++it) {
+        for (int32_t i = 0; i < n; ++i) {
+            double s = 0;
+            for (int32_t k = rp[i]; k < rp[i + 1]; ++k) s += v[k] * x[ci[k]];
+            y[i] = s * invd[i];
+        }
+        double l2 = 0;
+        for (int32_t i = 0; i < n; ++i) l2 += y[i] * y[i];
+        l2 = sqrt(l2);
+        for (int32_t i = 0; i < n; ++i) x[i] = y[i] * (1 / l2);
+        double ratio = l2 / l;
+        if (ratio > 1 - percent && ratio < 1 + percent) {
+            l = ratio > 1 ? l2 * pow(ratio, 1 / sqrt(percent)) : (l2 * 9 + l) / 10;
+            break;
+        }
+        l = l2;
+    }
+    free(x); free(y);
+    return l;
+}
+
 double orc_dot(int32_t n, const double *a, const double *b)
 {
     double s = 0.0;

@@ -84,7 +84,9 @@ def test_pcg_jacobi_parity(m16):
     assert st.HasConverged and so.converged == 1
     assert abs(st.NumIterationsRequired - so.iterations) <= 1
     assert np.abs(b - O.spmv(A, x)).max() <= 1e-5
-    assert energy_err(A, x, xo) < 1e-6
+    # both stop on max|r| <= 1e-5: they agree to that level, and both approach U
+    assert energy_err(A, x, xo) < 1e-5
+    assert abs(energy_err(A, x, U) - energy_err(A, xo, U)) < 1e-5
 
 
 @pytest.mark.parametrize("smoother", ["jacobi", "gs"])
@@ -109,12 +111,17 @@ def test_mgpcg_heterogeneous_tip_load(gpu):
     f = m.rhs()
     A = O.cantilever(n, n, n, dens=dens)
     xo, so = O.mgpcg(oracle_galerkin_levels(n, 3, dens), f, smoother=1, nu=2, tol=1e-12)
-    for smoother in ("jacobi", "gs"):
-        s = HipMgPcgSolver(m, levels=3, smoother=smoother, nu=2, tolerance=1e-8)
+    # omega = 0: per-level weight 2 / (1.2 lambda) (power method); a fixed 0.6 diverges here
+    # because lambda_max(D^-1 K) = 5.1 at 1e4 modulus contrast
+    for smoother, sm in (("jacobi", 0), ("gs", 1)):
+        s = HipMgPcgSolver(m, levels=3, smoother=smoother, omega=0.0, nu=2, tolerance=1e-10)
         s.Initialize(f)
         x, st = s.Solve()
         assert st.HasConverged
         assert energy_err(A, x, xo) < 1e-8
+        xs, ss = O.mgpcg(oracle_galerkin_levels(n, 3, dens), f, smoother=sm, omega=0.0, nu=2, tol=1e-10)
+        assert ss.converged == 1
+        assert abs(st.NumIterationsRequired - ss.iterations) <= max(1, ss.iterations // 10)
 
 
 def test_warm_start_and_caps(m16):
