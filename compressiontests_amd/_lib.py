@@ -65,7 +65,8 @@ class Options(C.Structure):
                 ("tolerance", C.c_double), ("stop", C.c_int32), ("depth", C.c_int32),
                 ("level_iterations", C.c_int32), ("precision", C.c_int32), ("nrhs", C.c_int32),
                 ("duvi_tolerance", C.c_double), ("duvi_mode", C.c_int32), ("inner_iterations", C.c_int32),
-                ("inner_tolerance", C.c_double), ("profile", C.c_int32), ("check_every", C.c_int32)]
+                ("inner_tolerance", C.c_double), ("profile", C.c_int32), ("check_every", C.c_int32),
+                ("use_graph", C.c_int32)]
 
 
 class Stats(C.Structure):

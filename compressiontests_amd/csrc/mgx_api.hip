@@ -297,6 +297,9 @@ struct mgx_solver_s {
     size_t partials_tail = 0;
     std::vector<hipEvent_t> prof_events;   // pairs around level-0 A*p launches
     int prof_used = 0;
+    hipGraphExec_t gexec = nullptr;        // captured chunk of PCG iterations
+    const double *graph_x = nullptr;
+    int graph_chunk = 0;
 };
 
 namespace mgx {
@@ -550,13 +553,15 @@ static void initialize(mgx_solver_s *S)
     S->p.alloc(S->n); S->q.alloc(S->n); S->z.alloc(S->n); S->r.alloc(S->n);
     // partial sums: enough for the largest row-kernel grid (2 rows per block minimum)
     size_t maxgrid = 2048;
-    for (int l = 0; l < L; ++l) maxgrid = std::max<size_t>(maxgrid, (size_t)S->lv[l].n / 2 + 64);
+    for (int l = 0; l < L; ++l) maxgrid = std::max<size_t>(maxgrid, (size_t)reduce_slots_for_rows(S->lv[l].n, S->lv[l].A.lanes) + 8);
     S->partials.alloc(3 * maxgrid + 16);
     S->partials_tail = 3 * maxgrid;
     S->ctl.alloc(1);
     MGX_HIP(hipMemsetAsync(S->ctl.p, 0, sizeof(Ctl), st));
     S->flag.alloc(1);
     MGX_HIP(hipStreamSynchronize(st));
+    if (S->gexec) { MGX_HIP(hipGraphExecDestroy(S->gexec)); S->gexec = nullptr; }
+    S->graph_x = nullptr;
     S->initialized = true;
 }
 
@@ -720,8 +725,7 @@ static void solve_pcg(mgx_solver_s *S, const double *b, double *x, bool has_x0, 
     launch_update_p(n, S->z.p, S->p.p, ctl, true, s);
     const int chunk = S->check_every;
     S->prof_used = 0;
-    for (int it = 0;; ++it) {
-        const bool prof = S->opt.profile && (size_t)(2 * S->prof_used + 2) <= S->prof_events.size();
+    auto one_iteration = [&](bool prof) {
         if (prof) MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used], s));
         launch_spmv_dot_alpha(n, V.A.rp, V.A.ci, V.A.v, S->p.p, S->q.p, V.A.lanes, ctl, S->partials.p, s);
         if (prof) { MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used + 1], s)); ++S->prof_used; }
@@ -729,12 +733,35 @@ static void solve_pcg(mgx_solver_s *S, const double *b, double *x, bool has_x0, 
         apply_prec(S, S->r.p, S->z.p, done);
         launch_dot_beta(n, S->r.p, S->z.p, ctl, S->partials.p, false, s);
         launch_update_p(n, S->z.p, S->p.p, ctl, false, s);
-        if ((it + 1) % chunk == 0 || it + 1 >= init.max_it) {
-            MGX_HIP(hipMemcpyAsync(S->ctl_host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
-            MGX_HIP(hipStreamSynchronize(s));
-            if (S->ctl_host->done) break;
-            if (it + 1 >= init.max_it + chunk) break;   // safety: the device test stops at max_it
+    };
+    const bool graph = S->opt.use_graph && !S->opt.profile;
+    if (graph && (!S->gexec || S->graph_x != x || S->graph_chunk != chunk)) {
+        if (S->gexec) { MGX_HIP(hipGraphExecDestroy(S->gexec)); S->gexec = nullptr; }
+        hipGraph_t g = nullptr;
+        MGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int i = 0; i < chunk; ++i) one_iteration(false);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
         }
+        MGX_HIP(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&S->gexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        MGX_HIP(e);
+        S->graph_x = x;
+        S->graph_chunk = chunk;
+    }
+    for (int it = 0;; it += chunk) {
+        if (graph) MGX_HIP(hipGraphLaunch(S->gexec, s));
+        else
+            for (int i = 0; i < chunk; ++i)
+                one_iteration(S->opt.profile && (size_t)(2 * S->prof_used + 2) <= S->prof_events.size());
+        MGX_HIP(hipMemcpyAsync(S->ctl_host, ctl, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        MGX_HIP(hipStreamSynchronize(s));
+        if (S->ctl_host->done) break;
+        if (it + chunk >= init.max_it + chunk) break;   // safety: the device test stops at max_it
     }
     const Ctl &c = *S->ctl_host;
     st->converged = c.status;
@@ -1110,6 +1137,7 @@ void mgx_options_default(mgx_options *o)
     o->inner_tolerance = 1e-4;
     o->profile = 0;
     o->check_every = 4;
+    o->use_graph = 1;
 }
 
 int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
@@ -1231,6 +1259,7 @@ int mgx_solver_release(mgx_solver s)
         if (s->ev0) (void)hipEventDestroy(s->ev0);
         if (s->ev1) (void)hipEventDestroy(s->ev1);
         for (auto &e : s->prof_events) (void)hipEventDestroy(e);
+        if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
         hipStream_t st = s->stream;
         delete s;
         if (st) (void)hipStreamDestroy(st);

@@ -86,6 +86,7 @@ void launch_simp_gradient(int32_t nx, int32_t ny, int32_t nz, const double *Ke0,
 void launch_pcg_init(int32_t n, const double *b, double *r, double *x, Ctl *ctl, double *partials, hipStream_t s);
 void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, const double *v,
                            const double *p, double *q, int lanes, Ctl *ctl, double *partials, hipStream_t s);
+int reduce_slots_for_rows(int64_t nrows, int lanes);
 void launch_pcg_update(int32_t n, const double *p, const double *q, double *x, double *r,
                        Ctl *ctl, double *partials, hipStream_t s);
 void launch_dot_beta(int32_t n, const double *r, const double *z, Ctl *ctl, double *partials, bool first, hipStream_t s);

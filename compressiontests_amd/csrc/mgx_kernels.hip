@@ -105,21 +105,40 @@ __device__ bool grid_finish3(double &a, double &b, double &m, double *partials, 
     return false;
 }
 
+// Partial sum of one CSR row held by one lane of a G-lane group: the lane takes entries
+// s + lane, s + lane + G, ...  Every trip issues U predicated (col, val) loads and then U
+// gathers before any FMA, so a K_ff row (<= 81 entries, G = 16, U = 6) costs one round trip.
+template <int G, int U, typename T>
+__device__ __forceinline__ T row_partial(const int32_t *__restrict__ ci, const T *__restrict__ v,
+                                         const T *__restrict__ x, int s, int e, int lane)
+{
+    T a0 = 0, a1 = 0;
+    for (int k = s + lane; k < e; k += U * G) {
+        int c[U];
+        T w[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int kk = k + u * G;
+            if (kk < e) { c[u] = ntload(ci + kk); w[u] = ntload(v + kk); }
+            else { c[u] = -1; w[u] = 0; }
+        }
+        T xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) xv[u] = c[u] >= 0 ? x[c[u]] : T(0);
+#pragma unroll
+        for (int u = 0; u < U; ++u) { if (u & 1) a1 += w[u] * xv[u]; else a0 += w[u] * xv[u]; }
+    }
+    return a0 + a1;
+}
+
+template <int G> struct Unroll { static constexpr int value = G >= 16 ? 6 : 4; };
+
 // Sum of one CSR row by a group of G lanes; every lane of the group gets the total.
 template <int G, typename T>
 __device__ __forceinline__ T row_dot(const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                      const T *__restrict__ v, const T *__restrict__ x, int row, int lane)
 {
-    const int s = rp[row], e = rp[row + 1];
-    T a0 = 0, a1 = 0, a2 = 0, a3 = 0;
-    int k = s + lane;
-    for (; k + 3 * G < e; k += 4 * G) {
-        const int c0 = ntload(ci + k), c1 = ntload(ci + k + G), c2 = ntload(ci + k + 2 * G), c3 = ntload(ci + k + 3 * G);
-        const T v0 = ntload(v + k), v1 = ntload(v + k + G), v2 = ntload(v + k + 2 * G), v3 = ntload(v + k + 3 * G);
-        a0 += v0 * x[c0]; a1 += v1 * x[c1]; a2 += v2 * x[c2]; a3 += v3 * x[c3];
-    }
-    for (; k < e; k += G) a0 += ntload(v + k) * x[ntload(ci + k)];
-    T acc = (a0 + a1) + (a2 + a3);
+    T acc = row_partial<G, Unroll<G>::value, T>(ci, v, x, rp[row], rp[row + 1], lane);
 #pragma unroll
     for (int o = G / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, G);
     return acc;
@@ -332,17 +351,8 @@ void launch_pcg_init(int32_t n, const double *b, double *r, double *x, Ctl *ctl,
     MGX_HIP(hipGetLastError());
 }
 
-// q = A p and p.q -> alpha = rz / pq.  A bounded grid (<= 2048 blocks, each a contiguous range
-// of rows, XCD-remapped) keeps the reduction to a few thousand partials.
-static inline int grid_rows_bounded(int64_t nrows, int rows_per_pass, int max_blocks)
-{
-    int64_t nb = (nrows + rows_per_pass - 1) / rows_per_pass;
-    if (nb > max_blocks) nb = max_blocks;
-    if (nb < 1) nb = 1;
-    nb = (nb + 7) & ~int64_t(7);
-    return (int)nb;
-}
-
+// q = A p and p.q -> alpha = rz / pq.  Same row mapping as k_csr_rows (one G-lane group per
+// row, XCD-remapped blocks); the per-block partial dots are summed by the last block.
 template <int G>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot_alpha(int32_t nrows, const int32_t *__restrict__ rp,
                                                            const int32_t *__restrict__ ci, const double *__restrict__ v,
@@ -352,19 +362,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot_alpha(int32_t nrows, const 
     if (ctl->done) return;
     constexpr int RPB = kBlock / G;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
-    const int64_t passes = ((int64_t)nrows + RPB - 1) / RPB;
-    const int64_t per = (passes + gridDim.x - 1) / gridDim.x;
-    const int64_t r0 = (int64_t)blk * per * RPB;
-    int64_t r1 = r0 + per * RPB;
-    if (r1 > nrows) r1 = nrows;
+    const int row = blk * RPB + threadIdx.x / G;
     const int lane = threadIdx.x % G;
     double contrib = 0, z = 0, m = 0;
-    for (int64_t base = r0; base < r1; base += RPB) {
-        const int64_t row = base + threadIdx.x / G;
-        if (row < r1) {
-            const double s = row_dot<G, double>(rp, ci, v, p, (int)row, lane);
-            if (lane == 0) { q[row] = s; contrib += p[row] * s; }
-        }
+    if (row < nrows) {
+        const double s = row_dot<G, double>(rp, ci, v, p, row, lane);
+        if (lane == 0) { q[row] = s; contrib = p[row] * s; }
     }
     if (grid_finish3(contrib, z, m, partials, &ctl->counter[1])) {
         ctl->pq = contrib;
@@ -375,8 +378,8 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot_alpha(int32_t nrows, const 
 void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *p,
                            double *q, int lanes, Ctl *ctl, double *partials, hipStream_t s)
 {
-#define MGX_SDA(G) hipLaunchKernelGGL((k_spmv_dot_alpha<G>), dim3(grid_rows_bounded(n, kBlock / G, 2048)), dim3(kBlock), \
-                                      0, s, n, rp, ci, v, p, q, ctl, partials)
+#define MGX_SDA(G) hipLaunchKernelGGL((k_spmv_dot_alpha<G>), dim3(grid_rows(n, kBlock / G)), dim3(kBlock), 0, s, \
+                                      n, rp, ci, v, p, q, ctl, partials)
     switch (lanes) {
     case 2: MGX_SDA(2); break;
     case 4: MGX_SDA(4); break;
@@ -388,6 +391,8 @@ void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, cons
 #undef MGX_SDA
     MGX_HIP(hipGetLastError());
 }
+
+int reduce_slots_for_rows(int64_t nrows, int lanes) { return grid_rows(nrows, kBlock / lanes); }
 
 // x += alpha p; r -= alpha q; ||r||^2, max|r|; stop test
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int32_t n, const double *__restrict__ p,
@@ -509,53 +514,65 @@ void launch_axpy_f32_into_f64(int64_t n, const float *a, double *x, const int32_
 }
 
 // ------------------------------------------------------------------ multicolour Gauss-Seidel
-// One node (3 rows) per group of 16 lanes.  s_a = sum over columns outside the node,
-// d_ab = the node's own 3x3 block; lane 0 then relaxes the three dofs in order.
+// One node (3 rows) per 64-lane wavefront: lanes 16a..16a+15 reduce row a (a = 0, 1, 2) into
+// s_a = sum over columns outside the node and d_ab = the node's own 3x3 block, all loads of a
+// row issued up front; lane 0 then relaxes the three dofs in order (reverse when backward).
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_gs_color(int32_t nnodes, const int32_t *__restrict__ nodes,
                                                      const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                      const T *__restrict__ v, const T *__restrict__ b, T *__restrict__ x,
                                                      int backward, const int32_t *__restrict__ done)
 {
-    constexpr int G = 16;
+    constexpr int G = 16, U = 6;
     if (done && *done) return;
-    const int grp = blockIdx.x * (kBlock / G) + threadIdx.x / G;
-    const int lane = threadIdx.x % G;
-    if (grp >= nnodes) return;
+    const int grp = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    if (grp >= nnodes) return;                     // whole wavefront leaves together
+    const int wl = threadIdx.x & 63;
+    const int a = wl >> 4;                         // row of the node handled by this quarter
+    const int lane = wl & 15;
     const int node = nodes[grp];
     const int r0 = 3 * node;
-    T s[3], d[3][3];
+    T sa = 0, d0 = 0, d1 = 0, d2 = 0;
+    if (a < 3) {
+        const int s = rp[r0 + a], e = rp[r0 + a + 1];
+        for (int k = s + lane; k < e; k += U * G) {
+            int c[U];
+            T w[U];
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        s[a] = 0; d[a][0] = d[a][1] = d[a][2] = 0;
-        const int e = rp[r0 + a + 1];
-        for (int k = rp[r0 + a] + lane; k < e; k += G) {
-            const int c = ci[k];
-            const T val = v[k];
-            const unsigned off = (unsigned)(c - r0);
-            if (off < 3u) {
-                if (off == 0) d[a][0] += val; else if (off == 1) d[a][1] += val; else d[a][2] += val;
-            } else s[a] += val * x[c];
-        }
+            for (int u = 0; u < U; ++u) {
+                const int kk = k + u * G;
+                if (kk < e) { c[u] = ci[kk]; w[u] = v[kk]; } else { c[u] = -1; w[u] = 0; }
+            }
 #pragma unroll
-        for (int o = G / 2; o > 0; o >>= 1) {
-            s[a] += __shfl_xor(s[a], o, G);
-            d[a][0] += __shfl_xor(d[a][0], o, G);
-            d[a][1] += __shfl_xor(d[a][1], o, G);
-            d[a][2] += __shfl_xor(d[a][2], o, G);
+            for (int u = 0; u < U; ++u) {
+                const unsigned off = (unsigned)(c[u] - r0);
+                if (c[u] < 0) continue;
+                if (off < 3u) { if (off == 0) d0 += w[u]; else if (off == 1) d1 += w[u]; else d2 += w[u]; }
+                else sa += w[u] * x[c[u]];
+            }
         }
     }
-    if (lane == 0) {
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) {
+        sa += __shfl_xor(sa, o, G);
+        d0 += __shfl_xor(d0, o, G);
+        d1 += __shfl_xor(d1, o, G);
+        d2 += __shfl_xor(d2, o, G);
+    }
+    // lane 0 gathers rows 1 and 2 from lanes 16 and 32
+    const T s1 = __shfl(sa, 16, 64), e10 = __shfl(d0, 16, 64), e11 = __shfl(d1, 16, 64), e12 = __shfl(d2, 16, 64);
+    const T s2 = __shfl(sa, 32, 64), e20 = __shfl(d0, 32, 64), e21 = __shfl(d1, 32, 64), e22 = __shfl(d2, 32, 64);
+    if (wl == 0) {
         T x0 = x[r0], x1 = x[r0 + 1], x2 = x[r0 + 2];
-        const T b0 = b[r0] - s[0], b1 = b[r0 + 1] - s[1], b2 = b[r0 + 2] - s[2];
+        const T b0 = b[r0] - sa, b1 = b[r0 + 1] - s1, b2 = b[r0 + 2] - s2;
         if (!backward) {
-            x0 = (b0 - d[0][1] * x1 - d[0][2] * x2) / d[0][0];
-            x1 = (b1 - d[1][0] * x0 - d[1][2] * x2) / d[1][1];
-            x2 = (b2 - d[2][0] * x0 - d[2][1] * x1) / d[2][2];
+            x0 = (b0 - d1 * x1 - d2 * x2) / d0;
+            x1 = (b1 - e10 * x0 - e12 * x2) / e11;
+            x2 = (b2 - e20 * x0 - e21 * x1) / e22;
         } else {
-            x2 = (b2 - d[2][0] * x0 - d[2][1] * x1) / d[2][2];
-            x1 = (b1 - d[1][0] * x0 - d[1][2] * x2) / d[1][1];
-            x0 = (b0 - d[0][1] * x1 - d[0][2] * x2) / d[0][0];
+            x2 = (b2 - e20 * x0 - e21 * x1) / e22;
+            x1 = (b1 - e10 * x0 - e12 * x2) / e11;
+            x0 = (b0 - d1 * x1 - d2 * x2) / d0;
         }
         x[r0] = x0; x[r0 + 1] = x1; x[r0 + 2] = x2;
     }
@@ -565,7 +582,7 @@ void launch_gs_color(int32_t nnodes, const int32_t *nodes, const int32_t *rp, co
                      const double *b, double *x, bool backward, const int32_t *done, hipStream_t s)
 {
     if (nnodes <= 0) return;
-    const int nb = (nnodes + (kBlock / 16) - 1) / (kBlock / 16);
+    const int nb = (nnodes + (kBlock / 64) - 1) / (kBlock / 64);
     hipLaunchKernelGGL(k_gs_color<double>, dim3(nb), dim3(kBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
                        backward ? 1 : 0, done);
     MGX_HIP(hipGetLastError());
@@ -574,7 +591,7 @@ void launch_gs_color_f32(int32_t nnodes, const int32_t *nodes, const int32_t *rp
                          const float *b, float *x, bool backward, const int32_t *done, hipStream_t s)
 {
     if (nnodes <= 0) return;
-    const int nb = (nnodes + (kBlock / 16) - 1) / (kBlock / 16);
+    const int nb = (nnodes + (kBlock / 64) - 1) / (kBlock / 64);
     hipLaunchKernelGGL(k_gs_color<float>, dim3(nb), dim3(kBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
                        backward ? 1 : 0, done);
     MGX_HIP(hipGetLastError());

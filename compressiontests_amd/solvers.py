@@ -256,7 +256,8 @@ class HipMgPcgSolver:
     multicolour Gauss-Seidel sweeps)."""
 
     def __init__(self, model, levels=3, smoother="jacobi", omega=0.6, nu=2, tolerance=1e-8, stop="rel",
-                 max_iterations=1000, hierarchy="galerkin", format="csr", profile=False, check_every=4, nrhs=1):
+                 max_iterations=1000, hierarchy="galerkin", format="csr", profile=False, check_every=4, nrhs=1,
+                 use_graph=True):
         self.model = model
         self.levels = levels
         opt = L.default_options(method=L.METHOD_MGPCG,
@@ -264,7 +265,8 @@ class HipMgPcgSolver:
                                 smoother=L.SMOOTHER_GS if smoother == "gs" else L.SMOOTHER_JACOBI,
                                 omega=omega, max_iterations=max_iterations, tolerance=tolerance,
                                 stop=L.STOP_REL_L2 if stop == "rel" else L.STOP_MAXNORM, level_iterations=nu,
-                                profile=int(profile), check_every=check_every, nrhs=nrhs)
+                                profile=int(profile), check_every=check_every, nrhs=nrhs,
+                                use_graph=int(bool(use_graph)))
         self.opt = opt
         self.nrhs = nrhs
         h = _Handle(opt)

@@ -208,6 +208,7 @@ typedef struct {
     double  inner_tolerance;    /* mixed precision: inner relative reduction */
     int32_t profile;            /* 1: time every level-0 A*p SpMV with HIP events (mgx_stats.spmv_ms) */
     int32_t check_every;        /* PCG iterations queued per host check of the device stop flag (default 4) */
+    int32_t use_graph;          /* 1: replay the queued PCG iterations as one captured hipGraph (default) */
 } mgx_options;
 
 typedef struct {

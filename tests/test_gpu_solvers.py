@@ -162,6 +162,31 @@ def test_duvi_mgpcg(m16):
     assert energy_err(A, x, U) < 1e-7
 
 
+def test_graph_replays_match_eager(gpu):
+    """The captured hipGraph of PCG iterations, replayed over several solves with eager work and
+    synchronisations in between, gives the eager result bit for bit (deterministic reductions)."""
+    n = 8
+    dens = 10 ** (4 * O.rng_uniform(20261205, n ** 3) - 2)
+    m = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
+    f = m.rhs()
+    eager = HipMgPcgSolver(m, levels=3, smoother="gs", nu=1, tolerance=1e-10, use_graph=False)
+    eager.Initialize(f)
+    xe, se = eager.Solve()
+    g = HipMgPcgSolver(m, levels=3, smoother="gs", nu=1, tolerance=1e-10, use_graph=True, check_every=3)
+    g.Initialize(f)
+    for rep in range(4):
+        xg, sg = g.Solve()
+        assert sg.NumIterationsRequired == se.NumIterationsRequired
+        assert np.array_equal(xg, xe), rep
+        eager.Solve()                   # eager work on another solver between replays
+    gj = HipMgPcgSolver(m, levels=3, smoother="jacobi", omega=0.0, nu=2, tolerance=1e-10, use_graph=True)
+    gj.Initialize(f)
+    ej = HipMgPcgSolver(m, levels=3, smoother="jacobi", omega=0.0, nu=2, tolerance=1e-10, use_graph=False)
+    ej.Initialize(f)
+    assert np.array_equal(gj.Solve()[0], ej.Solve()[0])
+    assert np.array_equal(gj.Solve()[0], ej.Solve()[0])
+
+
 @pytest.mark.slow
 def test_mgpcg_64_heterogeneous(gpu):
     """Config 2 moduli at 64^3 (811,200 dofs): the solve reaches 1e-8 relative residual and
