@@ -194,63 +194,6 @@ __global__ __launch_bounds__(kBlock) void k_scale_into(int32_t n, const double *
         z[i] = d[i] * r[i];
 }
 
-__global__ __launch_bounds__(kBlock) void k_pcg_norms(int32_t n, const double *__restrict__ b,
-                                                      const double *__restrict__ r, Ctl *ctl, double *partials)
-{
-    double sb = 0, sr = 0, m = 0;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const double bi = b[i], ri = r[i];
-        sb += bi * bi; sr += ri * ri;
-        m = (ri != ri) ? ri : (m != m ? m : (fabs(ri) > m ? fabs(ri) : m));
-    }
-    // grid reduction (shared helper lives in mgx_kernels.hip; restated here for this kernel)
-    __shared__ double sh[3][kBlock / 64];
-    __shared__ int is_last;
-    for (int o = 32; o > 0; o >>= 1) {
-        sb += __shfl_xor(sb, o, 64); sr += __shfl_xor(sr, o, 64);
-        const double t = __shfl_xor(m, o, 64);
-        m = (m != m) ? m : ((t != t) ? t : (t > m ? t : m));
-    }
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    if (l == 0) { sh[0][w] = sb; sh[1][w] = sr; sh[2][w] = m; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        double a = 0, c = 0, mm = 0;
-        for (int i = 0; i < kBlock / 64; ++i) {
-            a += sh[0][i]; c += sh[1][i];
-            const double t = sh[2][i];
-            mm = (mm != mm) ? mm : ((t != t) ? t : (t > mm ? t : mm));
-        }
-        partials[3 * blockIdx.x + 0] = a;
-        partials[3 * blockIdx.x + 1] = c;
-        partials[3 * blockIdx.x + 2] = mm;
-        const uint32_t t = __hip_atomic_fetch_add(&ctl->counter[4], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last || threadIdx.x != 0) return;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    const volatile double *vp = partials;
-    double a = 0, c = 0, mm = 0;
-    for (int i = 0; i < (int)gridDim.x; ++i) {
-        a += vp[3 * i + 0];
-        c += vp[3 * i + 1];
-        const double t = vp[3 * i + 2];
-        mm = (mm != mm) ? mm : ((t != t) ? t : (t > mm ? t : mm));
-    }
-    __hip_atomic_store(&ctl->counter[4], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    ctl->bnorm2 = a;
-    ctl->rnorm2 = c;
-    ctl->rmax = mm;
-    ctl->iter = 0;
-    int done = 0, status = 0;
-    if (!(mm == mm) || mm > 1e50) { done = 1; status = -1; }
-    else if (ctl->stop == MGX_STOP_REL_L2 ? (c <= ctl->tol * ctl->tol * a) : (mm <= ctl->tol)) { done = 1; status = 1; }
-    else if (ctl->max_it <= 0) done = 1;
-    ctl->status = status;
-    ctl->done = done;
-}
-
 static inline int grid_vec(int64_t n)
 {
     int64_t nb = (n + kBlock * 4 - 1) / (kBlock * 4);
@@ -718,8 +661,7 @@ static void solve_pcg(mgx_solver_s *S, const double *b, double *x, bool has_x0, 
         MGX_HIP(hipMemcpyAsync(S->r.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         launch_fill_zero(n, x, nullptr, s);
     }
-    hipLaunchKernelGGL(k_pcg_norms, dim3(grid_vec(n)), dim3(kBlock), 0, s, n, b, S->r.p, ctl, S->partials.p);
-    MGX_HIP(hipGetLastError());
+    launch_pcg_norms(n, b, S->r.p, ctl, S->partials.p, s);
     apply_prec(S, S->r.p, S->z.p, done);
     launch_dot_beta(n, S->r.p, S->z.p, ctl, S->partials.p, true, s);
     launch_update_p(n, S->z.p, S->p.p, ctl, true, s);

@@ -83,6 +83,7 @@ void launch_simp_gradient(int32_t nx, int32_t ny, int32_t nz, const double *Ke0,
                           double Emin, double E0, const double *u, double *grad, hipStream_t s);
 
 // PCG pieces working on Ctl (all honour ctl->done)
+void launch_pcg_norms(int32_t n, const double *b, const double *r, Ctl *ctl, double *partials, hipStream_t s);
 void launch_pcg_init(int32_t n, const double *b, double *r, double *x, Ctl *ctl, double *partials, hipStream_t s);
 void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, const double *v,
                            const double *p, double *q, int lanes, Ctl *ctl, double *partials, hipStream_t s);

@@ -71,35 +71,67 @@ __device__ __forceinline__ void block_reduce3(double &a, double &b, double &m)
     }
 }
 
-// Grid reduction; returns true in thread 0 of the last block, which holds the totals.
-// Each block publishes its partials and takes a ticket (agent-scope acq_rel RMW: release of
-// the partials); the last block issues one agent-scope acquire fence per thread and then
-// reads all partials with plain loads in a fixed order (deterministic sum).
-__device__ bool grid_finish3(double &a, double &b, double &m, double *partials, uint32_t *counter)
+// Grid reductions.  Each block publishes its partial(s) and takes a ticket (agent-scope acq_rel
+// RMW, the release of the partials); the last block issues one agent-scope acquire fence and
+// then reads every partial with plain, pipelined loads in a fixed order, so the sum is
+// deterministic.  The ticket is reset for the next launch.  Return: true in thread 0 of the
+// last block, which holds the totals.
+__device__ __forceinline__ bool take_ticket(uint32_t *counter)
 {
     __shared__ int is_last;
-    block_reduce3(a, b, m);
     if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x + 0] = a;
-        partials[3 * blockIdx.x + 1] = b;
-        partials[3 * blockIdx.x + 2] = m;
-        uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
+        const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
         is_last = (t == gridDim.x - 1);
     }
     __syncthreads();
     if (!is_last) return false;
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    return true;
+}
+
+__device__ bool grid_finish3(double &a, double &b, double &m, double *partials, uint32_t *counter)
+{
+    block_reduce3(a, b, m);
+    if (threadIdx.x == 0) {
+        partials[3 * blockIdx.x + 0] = a;
+        partials[3 * blockIdx.x + 1] = b;
+        partials[3 * blockIdx.x + 2] = m;
+    }
+    if (!take_ticket(counter)) return false;
     double sa = 0, sb = 0, sm = 0;
-    const volatile double *vp = partials;
     for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-        sa += vp[3 * i + 0];
-        sb += vp[3 * i + 1];
-        sm = nanmax(sm, vp[3 * i + 2]);
+        sa += partials[3 * i + 0];
+        sb += partials[3 * i + 1];
+        sm = nanmax(sm, partials[3 * i + 2]);
     }
     block_reduce3(sa, sb, sm);
     if (threadIdx.x == 0) {
         __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         a = sa; b = sb; m = sm;
+        return true;
+    }
+    return false;
+}
+
+// single sum (dots): one partial per block
+__device__ bool grid_finish1(double &a, double *partials, uint32_t *counter)
+{
+    double b = 0, m = 0;
+    block_reduce3(a, b, m);
+    if (threadIdx.x == 0) partials[blockIdx.x] = a;
+    if (!take_ticket(counter)) return false;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    int i = threadIdx.x;
+    for (; i + 3 * (int)blockDim.x < (int)gridDim.x; i += 4 * blockDim.x) {
+        s0 += partials[i]; s1 += partials[i + blockDim.x];
+        s2 += partials[i + 2 * blockDim.x]; s3 += partials[i + 3 * blockDim.x];
+    }
+    for (; i < (int)gridDim.x; i += blockDim.x) s0 += partials[i];
+    double sa = (s0 + s1) + (s2 + s3), z = 0, mm = 0;
+    block_reduce3(sa, z, mm);
+    if (threadIdx.x == 0) {
+        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        a = sa;
         return true;
     }
     return false;
@@ -307,9 +339,9 @@ void launch_jacobi_initial_f32(int32_t n, const float *pre, const float *b, floa
 __global__ __launch_bounds__(kBlock) void k_dot(int32_t n, const double *__restrict__ a, const double *__restrict__ b,
                                                 double *result, double *partials, uint32_t *counter)
 {
-    double s = 0, z = 0, m = 0;
+    double s = 0;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) s += a[i] * b[i];
-    if (grid_finish3(s, z, m, partials, counter)) *result = s;
+    if (grid_finish1(s, partials, counter)) *result = s;
 }
 
 void launch_dot(int32_t n, const double *a, const double *b, double *result, double *partials, uint32_t *counter,
@@ -345,6 +377,37 @@ __global__ __launch_bounds__(kBlock) void k_pcg_init(int32_t n, const double *__
     }
 }
 
+// ||b||^2, ||r||^2, max|r| and the stop test before the first iteration (r = b - A x0 or b)
+__global__ __launch_bounds__(kBlock) void k_pcg_norms(int32_t n, const double *__restrict__ b,
+                                                      const double *__restrict__ r, Ctl *ctl, double *partials)
+{
+    double sb = 0, sr = 0, m = 0;
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
+        const double bi = b[i], ri = r[i];
+        sb += bi * bi;
+        sr += ri * ri;
+        m = nanmax(m, fabs(ri));
+    }
+    if (grid_finish3(sb, sr, m, partials, &ctl->counter[4])) {
+        ctl->bnorm2 = sb;
+        ctl->rnorm2 = sr;
+        ctl->rmax = m;
+        ctl->iter = 0;
+        int done = 0, status = 0;
+        if (!(m == m) || m > 1e50) { done = 1; status = -1; }
+        else if (ctl->stop == MGX_STOP_REL_L2 ? (sr <= ctl->tol * ctl->tol * sb) : (m <= ctl->tol)) { done = 1; status = 1; }
+        else if (ctl->max_it <= 0) done = 1;
+        ctl->status = status;
+        ctl->done = done;
+    }
+}
+
+void launch_pcg_norms(int32_t n, const double *b, const double *r, Ctl *ctl, double *partials, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_pcg_norms, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, b, r, ctl, partials);
+    MGX_HIP(hipGetLastError());
+}
+
 void launch_pcg_init(int32_t n, const double *b, double *r, double *x, Ctl *ctl, double *partials, hipStream_t s)
 {
     hipLaunchKernelGGL(k_pcg_init, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, b, r, x, ctl, partials);
@@ -364,12 +427,12 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot_alpha(int32_t nrows, const 
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int row = blk * RPB + threadIdx.x / G;
     const int lane = threadIdx.x % G;
-    double contrib = 0, z = 0, m = 0;
+    double contrib = 0;
     if (row < nrows) {
         const double s = row_dot<G, double>(rp, ci, v, p, row, lane);
         if (lane == 0) { q[row] = s; contrib = p[row] * s; }
     }
-    if (grid_finish3(contrib, z, m, partials, &ctl->counter[1])) {
+    if (grid_finish1(contrib, partials, &ctl->counter[1])) {
         ctl->pq = contrib;
         ctl->alpha = ctl->rz / contrib;
     }
@@ -436,9 +499,9 @@ __global__ __launch_bounds__(kBlock) void k_dot_beta(int32_t n, const double *__
                                                      Ctl *ctl, double *partials, int first)
 {
     if (ctl->done) return;
-    double s = 0, u = 0, m = 0;
+    double s = 0;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) s += r[i] * z[i];
-    if (grid_finish3(s, u, m, partials, &ctl->counter[3])) {
+    if (grid_finish1(s, partials, &ctl->counter[3])) {
         if (!first) ctl->beta = s / ctl->rz;
         ctl->rz = s;
     }

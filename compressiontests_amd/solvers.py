@@ -252,10 +252,11 @@ class HipMgPcgSolver:
                    heterogeneous moduli);
       "reference": the reference's rediscretised coarse models with R = P^T of the
                    reference interpolation (its cone-weight R is not symmetric to P).
-    smoother "jacobi" (omega, nu pre and nu post sweeps) or "gs" (nu forward / nu backward
-    multicolour Gauss-Seidel sweeps)."""
+    smoother "jacobi" (nu pre and nu post sweeps; omega > 0 fixed, omega <= 0 per level
+    2 / (1.2 lambda_max(D^-1 K)) by the reference's power method) or "gs" (nu forward / nu
+    backward multicolour Gauss-Seidel sweeps)."""
 
-    def __init__(self, model, levels=3, smoother="jacobi", omega=0.6, nu=2, tolerance=1e-8, stop="rel",
+    def __init__(self, model, levels=3, smoother="jacobi", omega=0.0, nu=2, tolerance=1e-8, stop="rel",
                  max_iterations=1000, hierarchy="galerkin", format="csr", profile=False, check_every=4, nrhs=1,
                  use_graph=True):
         self.model = model
