@@ -495,7 +495,7 @@ static void initialize(mgx_solver_s *S)
     S->xdev.alloc((size_t)S->n * S->nrhs);
     S->p.alloc(S->n); S->q.alloc(S->n); S->z.alloc(S->n); S->r.alloc(S->n);
     // partial sums: enough for the largest row-kernel grid (2 rows per block minimum)
-    size_t maxgrid = 2048;
+    size_t maxgrid = kPartialStride;
     for (int l = 0; l < L; ++l) maxgrid = std::max<size_t>(maxgrid, (size_t)reduce_slots_for_rows(S->lv[l].n, S->lv[l].A.lanes) + 8);
     S->partials.alloc(3 * maxgrid + 16);
     S->partials_tail = 3 * maxgrid;
@@ -661,7 +661,7 @@ static void solve_pcg(mgx_solver_s *S, const double *b, double *x, bool has_x0, 
         MGX_HIP(hipMemcpyAsync(S->r.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         launch_fill_zero(n, x, nullptr, s);
     }
-    launch_pcg_norms(n, b, S->r.p, ctl, S->partials.p, s);
+    launch_pcg_norms(n, b, S->r.p, ctl, S->partials.p, kPartialStride, s);
     apply_prec(S, S->r.p, S->z.p, done);
     launch_dot_beta(n, S->r.p, S->z.p, ctl, S->partials.p, true, s);
     launch_update_p(n, S->z.p, S->p.p, ctl, true, s);
@@ -866,10 +866,8 @@ int mgx_dot(int32_t n, const double *a, const double *b, double *result, void *s
     return guard([&] {
         MGX_REQUIRE(n >= 0 && result, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
         hipStream_t s = (hipStream_t)stream;
-        DBuf<double> partials; partials.alloc(3 * 2048);
-        DBuf<uint32_t> counter; counter.alloc(1);
-        MGX_HIP(hipMemsetAsync(counter.p, 0, sizeof(uint32_t), s));
-        launch_dot(n, a, b, result, partials.p, counter.p, s);
+        DBuf<double> partials; partials.alloc(3 * kPartialStride);
+        launch_dot(n, a, b, result, partials.p, nullptr, s);
         MGX_HIP(hipStreamSynchronize(s));
     });
 }

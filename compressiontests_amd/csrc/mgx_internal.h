@@ -45,8 +45,6 @@ struct Ctl {
     uint32_t counter[16];   // last-block tickets of grid reductions
 };
 
-// number of partial-sum slots a grid reduction may use
-constexpr int kMaxReduceBlocks = 4096;
 constexpr int kBlock = 256;
 
 // ------------------------------------------------------------------ kernel launchers (mgx_kernels.hip)
@@ -83,8 +81,11 @@ void launch_simp_gradient(int32_t nx, int32_t ny, int32_t nz, const double *Ke0,
                           double Emin, double E0, const double *u, double *grad, hipStream_t s);
 
 // PCG pieces working on Ctl (all honour ctl->done)
-void launch_pcg_norms(int32_t n, const double *b, const double *r, Ctl *ctl, double *partials, hipStream_t s);
-void launch_pcg_init(int32_t n, const double *b, double *r, double *x, Ctl *ctl, double *partials, hipStream_t s);
+// two-phase reductions: producers store per-block partials, k_finalize (one block) combines them
+enum FinMode : int { FIN_ALPHA = 0, FIN_UPDATE = 1, FIN_BETA = 2, FIN_BETA_FIRST = 3, FIN_NORMS = 4, FIN_DOT = 5 };
+constexpr int kPartialStride = 4096;   // slot stride of the 3-slot reductions (grid_stream <= 2048 blocks)
+void launch_finalize(int nparts, const double *partials, int stride, Ctl *ctl, int mode, double *result, hipStream_t s);
+void launch_pcg_norms(int32_t n, const double *b, const double *r, Ctl *ctl, double *partials, int stride, hipStream_t s);
 void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, const double *v,
                            const double *p, double *q, int lanes, Ctl *ctl, double *partials, hipStream_t s);
 int reduce_slots_for_rows(int64_t nrows, int lanes);

@@ -54,10 +54,11 @@ __device__ __forceinline__ double wave_max(double v)
     return v;
 }
 
-// Block reduction of (sum a, sum b, max m); totals valid in thread 0.
+// Block reduction of (sum a, sum b, max m) for blocks of up to 1024 threads; totals valid in
+// thread 0.  Fixed tree order: deterministic.
 __device__ __forceinline__ void block_reduce3(double &a, double &b, double &m)
 {
-    __shared__ double sh[3][kBlock / 64];
+    __shared__ double sh[3][16];
     a = wave_sum(a); b = wave_sum(b); m = wave_max(m);
     const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
     __syncthreads();
@@ -71,70 +72,19 @@ __device__ __forceinline__ void block_reduce3(double &a, double &b, double &m)
     }
 }
 
-// Grid reductions.  Each block publishes its partial(s) and takes a ticket (agent-scope acq_rel
-// RMW, the release of the partials); the last block issues one agent-scope acquire fence and
-// then reads every partial with plain, pipelined loads in a fixed order, so the sum is
-// deterministic.  The ticket is reset for the next launch.  Return: true in thread 0 of the
-// last block, which holds the totals.
-__device__ __forceinline__ bool take_ticket(uint32_t *counter)
-{
-    __shared__ int is_last;
-    if (threadIdx.x == 0) {
-        const uint32_t t = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        is_last = (t == gridDim.x - 1);
-    }
-    __syncthreads();
-    if (!is_last) return false;
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
-    return true;
-}
-
-__device__ bool grid_finish3(double &a, double &b, double &m, double *partials, uint32_t *counter)
+// Two-phase grid reductions.  Phase 1: every block of the producing kernel reduces its
+// contributions and thread 0 stores them with plain stores into partials[slot * stride + block].
+// Phase 2: k_finalize (one block, next launch on the same stream, so the stores are visible)
+// sums the partials in a fixed order and updates the solver scalars.  No per-block atomics or
+// cache-wide release fences on the hot path.
+__device__ __forceinline__ void block_publish(double a, double b, double m, double *partials, int stride, int nslots)
 {
     block_reduce3(a, b, m);
     if (threadIdx.x == 0) {
-        partials[3 * blockIdx.x + 0] = a;
-        partials[3 * blockIdx.x + 1] = b;
-        partials[3 * blockIdx.x + 2] = m;
+        partials[blockIdx.x] = a;
+        if (nslots > 1) partials[stride + blockIdx.x] = b;
+        if (nslots > 2) partials[2 * stride + blockIdx.x] = m;
     }
-    if (!take_ticket(counter)) return false;
-    double sa = 0, sb = 0, sm = 0;
-    for (int i = threadIdx.x; i < (int)gridDim.x; i += blockDim.x) {
-        sa += partials[3 * i + 0];
-        sb += partials[3 * i + 1];
-        sm = nanmax(sm, partials[3 * i + 2]);
-    }
-    block_reduce3(sa, sb, sm);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        a = sa; b = sb; m = sm;
-        return true;
-    }
-    return false;
-}
-
-// single sum (dots): one partial per block
-__device__ bool grid_finish1(double &a, double *partials, uint32_t *counter)
-{
-    double b = 0, m = 0;
-    block_reduce3(a, b, m);
-    if (threadIdx.x == 0) partials[blockIdx.x] = a;
-    if (!take_ticket(counter)) return false;
-    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
-    int i = threadIdx.x;
-    for (; i + 3 * (int)blockDim.x < (int)gridDim.x; i += 4 * blockDim.x) {
-        s0 += partials[i]; s1 += partials[i + blockDim.x];
-        s2 += partials[i + 2 * blockDim.x]; s3 += partials[i + 3 * blockDim.x];
-    }
-    for (; i < (int)gridDim.x; i += blockDim.x) s0 += partials[i];
-    double sa = (s0 + s1) + (s2 + s3), z = 0, mm = 0;
-    block_reduce3(sa, z, mm);
-    if (threadIdx.x == 0) {
-        __hip_atomic_store(counter, 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-        a = sa;
-        return true;
-    }
-    return false;
 }
 
 // Partial sum of one CSR row held by one lane of a G-lane group: the lane takes entries
@@ -336,50 +286,86 @@ void launch_jacobi_initial_f32(int32_t n, const float *pre, const float *b, floa
     MGX_HIP(hipGetLastError());
 }
 
-__global__ __launch_bounds__(kBlock) void k_dot(int32_t n, const double *__restrict__ a, const double *__restrict__ b,
-                                                double *result, double *partials, uint32_t *counter)
+// phase 1 of a dot product
+__global__ __launch_bounds__(kBlock) void k_dot_partial(int32_t n, const double *__restrict__ a,
+                                                        const double *__restrict__ b, const int32_t *__restrict__ done,
+                                                        double *partials)
 {
+    if (done && *done) return;
     double s = 0;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) s += a[i] * b[i];
-    if (grid_finish1(s, partials, counter)) *result = s;
+    block_publish(s, 0, 0, partials, 0, 1);
+}
+
+// phase 2: one block sums the partials in a fixed order and applies `mode`
+__global__ __launch_bounds__(1024) void k_finalize(int nparts, const double *__restrict__ partials, int stride,
+                                                   Ctl *ctl, int mode, double *result)
+{
+    if (mode != FIN_NORMS && mode != FIN_DOT && ctl->done) return;
+    const int nslots = (mode == FIN_UPDATE || mode == FIN_NORMS) ? 3 : 1;
+    double a = 0, b = 0, m = 0;
+    for (int i = threadIdx.x; i < nparts; i += blockDim.x) {
+        a += partials[i];
+        if (nslots > 1) { b += partials[stride + i]; m = nanmax(m, partials[2 * stride + i]); }
+    }
+    block_reduce3(a, b, m);
+    if (threadIdx.x != 0) return;
+    switch (mode) {
+    case FIN_DOT: *result = a; break;
+    case FIN_ALPHA: ctl->pq = a; ctl->alpha = ctl->rz / a; break;
+    case FIN_BETA_FIRST: ctl->rz = a; break;
+    case FIN_BETA: ctl->beta = a / ctl->rz; ctl->rz = a; break;
+    case FIN_UPDATE: {
+        // a = ||r||^2 (slot 0 holds it for FIN_UPDATE), m = max |r|
+        ctl->rnorm2 = a;
+        ctl->rmax = m;
+        const int it = ctl->iter + 1;
+        ctl->iter = it;
+        int done = 0, status = 0;
+        if (!(m == m) || m > 1e50 || !(a == a)) { done = 1; status = -1; }
+        else if (ctl->stop == MGX_STOP_REL_L2 ? (a <= ctl->tol * ctl->tol * ctl->bnorm2) : (m <= ctl->tol)) {
+            done = 1; status = 1;
+        } else if (it >= ctl->max_it) done = 1;
+        ctl->status = status;
+        ctl->done = done;
+        break;
+    }
+    case FIN_NORMS: {
+        // a = ||b||^2, b = ||r||^2, m = max |r|
+        ctl->bnorm2 = a;
+        ctl->rnorm2 = b;
+        ctl->rmax = m;
+        ctl->iter = 0;
+        int done = 0, status = 0;
+        if (!(m == m) || m > 1e50) { done = 1; status = -1; }
+        else if (ctl->stop == MGX_STOP_REL_L2 ? (b <= ctl->tol * ctl->tol * a) : (m <= ctl->tol)) { done = 1; status = 1; }
+        else if (ctl->max_it <= 0) done = 1;
+        ctl->status = status;
+        ctl->done = done;
+        break;
+    }
+    }
+}
+
+void launch_finalize(int nparts, const double *partials, int stride, Ctl *ctl, int mode, double *result, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, nparts, partials, stride, ctl, mode, result);
+    MGX_HIP(hipGetLastError());
 }
 
 void launch_dot(int32_t n, const double *a, const double *b, double *result, double *partials, uint32_t *counter,
                 hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dot, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, a, b, result, partials, counter);
+    (void)counter;
+    const int g = grid_stream(n);
+    hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kBlock), 0, s, n, a, b, nullptr, partials);
     MGX_HIP(hipGetLastError());
+    launch_finalize(g, partials, 0, nullptr, FIN_DOT, result, s);
 }
 
-// PCG: r = b, x = 0, ||b||^2, zero the iteration state
-__global__ __launch_bounds__(kBlock) void k_pcg_init(int32_t n, const double *__restrict__ b, double *__restrict__ r,
-                                                     double *__restrict__ x, Ctl *ctl, double *partials)
-{
-    double s = 0, z = 0, m = 0;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const double bi = b[i];
-        r[i] = bi;
-        if (x) x[i] = 0.0;
-        s += bi * bi;
-        m = nanmax(m, fabs(bi));
-    }
-    if (grid_finish3(s, z, m, partials, &ctl->counter[0])) {
-        ctl->bnorm2 = s;
-        ctl->rnorm2 = s;
-        ctl->rmax = m;
-        ctl->iter = 0;
-        int done = 0, status = 0;
-        if (!(m == m) || m > 1e50) { done = 1; status = -1; }
-        else if (ctl->stop == MGX_STOP_REL_L2 ? s == 0.0 : m <= ctl->tol) { done = 1; status = 1; }
-        else if (ctl->max_it <= 0) { done = 1; }
-        ctl->status = status;
-        ctl->done = done;
-    }
-}
-
-// ||b||^2, ||r||^2, max|r| and the stop test before the first iteration (r = b - A x0 or b)
+// ||b||^2, ||r||^2, max|r| before the first iteration (r = b - A x0 or b)
 __global__ __launch_bounds__(kBlock) void k_pcg_norms(int32_t n, const double *__restrict__ b,
-                                                      const double *__restrict__ r, Ctl *ctl, double *partials)
+                                                      const double *__restrict__ r, double *partials, int stride)
 {
     double sb = 0, sr = 0, m = 0;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
@@ -388,41 +374,26 @@ __global__ __launch_bounds__(kBlock) void k_pcg_norms(int32_t n, const double *_
         sr += ri * ri;
         m = nanmax(m, fabs(ri));
     }
-    if (grid_finish3(sb, sr, m, partials, &ctl->counter[4])) {
-        ctl->bnorm2 = sb;
-        ctl->rnorm2 = sr;
-        ctl->rmax = m;
-        ctl->iter = 0;
-        int done = 0, status = 0;
-        if (!(m == m) || m > 1e50) { done = 1; status = -1; }
-        else if (ctl->stop == MGX_STOP_REL_L2 ? (sr <= ctl->tol * ctl->tol * sb) : (m <= ctl->tol)) { done = 1; status = 1; }
-        else if (ctl->max_it <= 0) done = 1;
-        ctl->status = status;
-        ctl->done = done;
-    }
+    block_publish(sb, sr, m, partials, stride, 3);
 }
 
-void launch_pcg_norms(int32_t n, const double *b, const double *r, Ctl *ctl, double *partials, hipStream_t s)
+void launch_pcg_norms(int32_t n, const double *b, const double *r, Ctl *ctl, double *partials, int stride, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pcg_norms, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, b, r, ctl, partials);
+    const int g = grid_stream(n);
+    hipLaunchKernelGGL(k_pcg_norms, dim3(g), dim3(kBlock), 0, s, n, b, r, partials, stride);
     MGX_HIP(hipGetLastError());
+    launch_finalize(g, partials, stride, ctl, FIN_NORMS, nullptr, s);
 }
 
-void launch_pcg_init(int32_t n, const double *b, double *r, double *x, Ctl *ctl, double *partials, hipStream_t s)
-{
-    hipLaunchKernelGGL(k_pcg_init, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, b, r, x, ctl, partials);
-    MGX_HIP(hipGetLastError());
-}
-
-// q = A p and p.q -> alpha = rz / pq.  Same row mapping as k_csr_rows (one G-lane group per
-// row, XCD-remapped blocks); the per-block partial dots are summed by the last block.
+// q = A p and the per-block partials of p.q (phase 1 of alpha = rz / pq).  Same row mapping as
+// k_csr_rows (one G-lane group per row, XCD-remapped blocks).
 template <int G>
-__global__ __launch_bounds__(kBlock) void k_spmv_dot_alpha(int32_t nrows, const int32_t *__restrict__ rp,
-                                                           const int32_t *__restrict__ ci, const double *__restrict__ v,
-                                                           const double *__restrict__ p, double *__restrict__ q,
-                                                           Ctl *ctl, double *partials)
+__global__ __launch_bounds__(kBlock) void k_spmv_dot(int32_t nrows, const int32_t *__restrict__ rp,
+                                                     const int32_t *__restrict__ ci, const double *__restrict__ v,
+                                                     const double *__restrict__ p, double *__restrict__ q,
+                                                     const int32_t *__restrict__ done, double *partials)
 {
-    if (ctl->done) return;
+    if (*done) return;
     constexpr int RPB = kBlock / G;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int row = blk * RPB + threadIdx.x / G;
@@ -432,17 +403,15 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot_alpha(int32_t nrows, const 
         const double s = row_dot<G, double>(rp, ci, v, p, row, lane);
         if (lane == 0) { q[row] = s; contrib = p[row] * s; }
     }
-    if (grid_finish1(contrib, partials, &ctl->counter[1])) {
-        ctl->pq = contrib;
-        ctl->alpha = ctl->rz / contrib;
-    }
+    block_publish(contrib, 0, 0, partials, 0, 1);
 }
 
-void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *p,
-                           double *q, int lanes, Ctl *ctl, double *partials, hipStream_t s)
+int launch_spmv_dot(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *p, double *q,
+                    int lanes, const int32_t *done, double *partials, hipStream_t s)
 {
-#define MGX_SDA(G) hipLaunchKernelGGL((k_spmv_dot_alpha<G>), dim3(grid_rows(n, kBlock / G)), dim3(kBlock), 0, s, \
-                                      n, rp, ci, v, p, q, ctl, partials)
+    int grid = 0;
+#define MGX_SDA(G) grid = grid_rows(n, kBlock / G); \
+    hipLaunchKernelGGL((k_spmv_dot<G>), dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, p, q, done, partials)
     switch (lanes) {
     case 2: MGX_SDA(2); break;
     case 4: MGX_SDA(4); break;
@@ -453,18 +422,26 @@ void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, cons
     }
 #undef MGX_SDA
     MGX_HIP(hipGetLastError());
+    return grid;
 }
 
 int reduce_slots_for_rows(int64_t nrows, int lanes) { return grid_rows(nrows, kBlock / lanes); }
 
-// x += alpha p; r -= alpha q; ||r||^2, max|r|; stop test
+void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *p,
+                           double *q, int lanes, Ctl *ctl, double *partials, hipStream_t s)
+{
+    const int g = launch_spmv_dot(n, rp, ci, v, p, q, lanes, &ctl->done, partials, s);
+    launch_finalize(g, partials, 0, ctl, FIN_ALPHA, nullptr, s);
+}
+
+// x += alpha p; r -= alpha q; partials of ||r||^2 and max|r| (phase 1 of the stop test)
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int32_t n, const double *__restrict__ p,
                                                        const double *__restrict__ q, double *__restrict__ x,
-                                                       double *__restrict__ r, Ctl *ctl, double *partials)
+                                                       double *__restrict__ r, const Ctl *ctl, double *partials, int stride)
 {
     if (ctl->done) return;
     const double alpha = ctl->alpha;
-    double s = 0, z = 0, m = 0;
+    double s = 0, m = 0;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
         x[i] = x[i] + alpha * p[i];
         const double ri = r[i] - alpha * q[i];
@@ -472,45 +449,25 @@ __global__ __launch_bounds__(kBlock) void k_pcg_update(int32_t n, const double *
         s += ri * ri;
         m = nanmax(m, fabs(ri));
     }
-    if (grid_finish3(s, z, m, partials, &ctl->counter[2])) {
-        ctl->rnorm2 = s;
-        ctl->rmax = m;
-        const int it = ctl->iter + 1;
-        ctl->iter = it;
-        int done = 0, status = 0;
-        if (!(m == m) || m > 1e50 || !(s == s)) { done = 1; status = -1; }
-        else if (ctl->stop == MGX_STOP_REL_L2 ? (s <= ctl->tol * ctl->tol * ctl->bnorm2) : (m <= ctl->tol)) {
-            done = 1; status = 1;
-        } else if (it >= ctl->max_it) done = 1;
-        ctl->status = status;
-        ctl->done = done;
-    }
+    block_publish(s, 0, m, partials, stride, 3);
 }
 
 void launch_pcg_update(int32_t n, const double *p, const double *q, double *x, double *r, Ctl *ctl,
                        double *partials, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_pcg_update, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, p, q, x, r, ctl, partials);
+    const int g = grid_stream(n);
+    const int stride = kPartialStride;
+    hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, p, q, x, r, ctl, partials, stride);
     MGX_HIP(hipGetLastError());
-}
-
-// rz_new = r . z; beta = rz_new / rz (first: only store rz)
-__global__ __launch_bounds__(kBlock) void k_dot_beta(int32_t n, const double *__restrict__ r, const double *__restrict__ z,
-                                                     Ctl *ctl, double *partials, int first)
-{
-    if (ctl->done) return;
-    double s = 0;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) s += r[i] * z[i];
-    if (grid_finish1(s, partials, &ctl->counter[3])) {
-        if (!first) ctl->beta = s / ctl->rz;
-        ctl->rz = s;
-    }
+    launch_finalize(g, partials, stride, ctl, FIN_UPDATE, nullptr, s);
 }
 
 void launch_dot_beta(int32_t n, const double *r, const double *z, Ctl *ctl, double *partials, bool first, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_dot_beta, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, r, z, ctl, partials, first ? 1 : 0);
+    const int g = grid_stream(n);
+    hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kBlock), 0, s, n, r, z, &ctl->done, partials);
     MGX_HIP(hipGetLastError());
+    launch_finalize(g, partials, 0, ctl, first ? FIN_BETA_FIRST : FIN_BETA, nullptr, s);
 }
 
 __global__ __launch_bounds__(kBlock) void k_update_p(int32_t n, const double *__restrict__ z, double *__restrict__ p,
@@ -588,7 +545,7 @@ __global__ __launch_bounds__(kBlock) void k_gs_color(int32_t nnodes, const int32
 {
     constexpr int G = 16, U = 6;
     if (done && *done) return;
-    const int grp = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int grp = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6);
     if (grp >= nnodes) return;                     // whole wavefront leaves together
     const int wl = threadIdx.x & 63;
     const int a = wl >> 4;                         // row of the node handled by this quarter
@@ -645,7 +602,7 @@ void launch_gs_color(int32_t nnodes, const int32_t *nodes, const int32_t *rp, co
                      const double *b, double *x, bool backward, const int32_t *done, hipStream_t s)
 {
     if (nnodes <= 0) return;
-    const int nb = (nnodes + (kBlock / 64) - 1) / (kBlock / 64);
+    const int nb = grid_rows(nnodes, kBlock / 64);
     hipLaunchKernelGGL(k_gs_color<double>, dim3(nb), dim3(kBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
                        backward ? 1 : 0, done);
     MGX_HIP(hipGetLastError());
@@ -654,7 +611,7 @@ void launch_gs_color_f32(int32_t nnodes, const int32_t *nodes, const int32_t *rp
                          const float *b, float *x, bool backward, const int32_t *done, hipStream_t s)
 {
     if (nnodes <= 0) return;
-    const int nb = (nnodes + (kBlock / 64) - 1) / (kBlock / 64);
+    const int nb = grid_rows(nnodes, kBlock / 64);
     hipLaunchKernelGGL(k_gs_color<float>, dim3(nb), dim3(kBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
                        backward ? 1 : 0, done);
     MGX_HIP(hipGetLastError());
