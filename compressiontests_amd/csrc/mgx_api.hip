@@ -211,7 +211,7 @@ struct Level {
     std::vector<double> hAv, hRv, hPv;
     bool staged = false;
     DBuf<double> pre, x, b, r, t, w;       // omega D^-1 and work vectors (w: DuVi scratch)
-    DBuf<float> pre32, x32, b32, r32, t32;
+    DBuf<float> pre32, x32, b32, r32, t32, w32;   // precision form
     int ncolors = 0;
     std::vector<DBuf<int32_t>> color_nodes;
     std::vector<int32_t> color_count;
@@ -230,6 +230,9 @@ struct mgx_solver_s {
     mgx::DBuf<double> coarse_inv;
     mgx::DBuf<double> bdev, xdev;          // level-0 rhs / solution (nrhs columns)
     mgx::DBuf<double> p, q, z, r;          // PCG vectors
+    mgx::DBuf<float> p32, q32, z32, r32, d32;   // inner PCG of the precision form
+    mgx::DBuf<mgx::Ctl> ctl_in;
+    mgx::Ctl *ctl_in_host = nullptr;       // pinned
     mgx::DBuf<double> partials;
     mgx::DBuf<mgx::Ctl> ctl;
     mgx::Ctl *ctl_host = nullptr;          // pinned
@@ -356,7 +359,9 @@ static void initialize(mgx_solver_s *S)
     if (S->opt.method == MGX_METHOD_MGPCG) MGX_REQUIRE(L >= 2, MGX_ERR_INVALID_ARGUMENT, "MG-PCG needs >= 2 levels");
     MGX_REQUIRE(!(S->opt.format == MGX_FORMAT_DUVI && S->opt.smoother == MGX_SMOOTHER_GS), MGX_ERR_NOT_IMPLEMENTED,
                 "multicolour Gauss-Seidel runs on the CSR format only");
-    MGX_REQUIRE(S->opt.precision == MGX_PRECISION_FP64, MGX_ERR_NOT_IMPLEMENTED, "precision form not available");
+    MGX_REQUIRE(S->opt.precision == MGX_PRECISION_FP64 ||
+                    (S->opt.method == MGX_METHOD_MGPCG && S->opt.format == MGX_FORMAT_CSR),
+                MGX_ERR_NOT_IMPLEMENTED, "the precision form is MG-PCG on CSR");
     // upload / attach level matrices
     for (int l = 0; l < L; ++l) {
         Level &V = S->lv[l];
@@ -490,6 +495,25 @@ static void initialize(mgx_solver_s *S)
     for (int l = 0; l < L; ++l) {
         Level &V = S->lv[l];
         V.x.alloc(V.n); V.b.alloc(V.n); V.r.alloc(V.n); V.t.alloc(V.n); V.w.alloc(V.n);
+    }
+    if (S->opt.precision == MGX_PRECISION_MIXED) {
+        for (int l = 0; l < L; ++l) {
+            Level &V = S->lv[l];
+            V.A.make_f32(st);
+            if (l < L - 1) {
+                V.R.make_f32(st);
+                V.P.make_f32(st);
+                if (V.pre.p && S->opt.smoother == MGX_SMOOTHER_JACOBI) {
+                    V.pre32.alloc(V.n);
+                    launch_convert_f64_f32(V.n, V.pre.p, V.pre32.p, nullptr, st);
+                }
+            }
+            V.x32.alloc(V.n); V.b32.alloc(V.n); V.r32.alloc(V.n); V.t32.alloc(V.n); V.w32.alloc(V.n);
+        }
+        const int32_t n0 = S->lv[0].n;
+        S->p32.alloc(n0); S->q32.alloc(n0); S->z32.alloc(n0); S->r32.alloc(n0); S->d32.alloc(n0);
+        S->ctl_in.alloc(1);
+        MGX_HIP(hipMemsetAsync(S->ctl_in.p, 0, sizeof(Ctl), st));
     }
     S->bdev.upload(S->b_host.data(), S->b_host.size(), st);
     S->xdev.alloc((size_t)S->n * S->nrhs);
@@ -738,6 +762,118 @@ static void finish_stats(mgx_solver_s *S, const double *b, const double *x, mgx_
     st->rel_residual = h[1] > 0 ? std::sqrt(h[0] / h[1]) : std::sqrt(h[0]);
 }
 
+// ---------------------------------------------------------------- precision form
+static void level_gs32(mgx_solver_s *S, Level &L, const float *b, float *x, bool backward, const int32_t *done)
+{
+    if (!backward)
+        for (int c = 0; c < L.ncolors; ++c)
+            launch_gs_color_f32(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v32, b, x, false, done, S->stream);
+    else
+        for (int c = L.ncolors - 1; c >= 0; --c)
+            launch_gs_color_f32(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v32, b, x, true, done, S->stream);
+}
+
+// FP32 twin of vcycle(): FP32 values and vectors, FP64 coarse solve.
+static void vcycle32(mgx_solver_s *S, int l, const float *b, float *x, const int32_t *done)
+{
+    const int L = (int)S->lv.size();
+    Level &V = S->lv[l];
+    hipStream_t s = S->stream;
+    if (l == L - 1) { launch_dense_gemv_f32io(V.n, S->coarse_inv.p, b, x, done, s); return; }
+    const int nu = S->opt.level_iterations;
+    float *cur;
+    if (S->opt.smoother == MGX_SMOOTHER_GS) {
+        launch_fill_zero_f32(V.n, x, done, s);
+        for (int i = 0; i < nu; ++i) level_gs32(S, V, b, x, false, done);
+        cur = x;
+    } else {
+        launch_jacobi_initial_f32(V.n, V.pre32.p, b, V.t32.p, done, s);
+        cur = V.t32.p;
+        for (int i = 1; i < nu; ++i) {
+            float *o = (cur == x) ? V.t32.p : x;
+            launch_jacobi_f32(V.n, V.pre32.p, V.A.rp, V.A.ci, V.A.v32, b, cur, o, V.A.lanes, done, s);
+            cur = o;
+        }
+    }
+    launch_residual_f32(V.n, V.A.rp, V.A.ci, V.A.v32, b, cur, V.r32.p, V.A.lanes, done, s);
+    Level &C = S->lv[l + 1];
+    launch_spmv_f32(V.R.nrows, V.R.rp, V.R.ci, V.R.v32, V.r32.p, C.b32.p, true, V.R.lanes, done, s);
+    vcycle32(S, l + 1, C.b32.p, C.x32.p, done);
+    launch_spmv_f32(V.P.nrows, V.P.rp, V.P.ci, V.P.v32, C.x32.p, cur, false, V.P.lanes, done, s);
+    if (S->opt.smoother == MGX_SMOOTHER_GS) {
+        for (int i = 0; i < nu; ++i) level_gs32(S, V, b, x, true, done);
+    } else {
+        for (int i = 0; i < nu; ++i) {
+            float *o = (cur == x) ? V.t32.p : x;
+            launch_jacobi_f32(V.n, V.pre32.p, V.A.rp, V.A.ci, V.A.v32, b, cur, o, V.A.lanes, done, s);
+            cur = o;
+        }
+        if (cur != x) MGX_HIP(hipMemcpyAsync(x, cur, sizeof(float) * V.n, hipMemcpyDeviceToDevice, s));
+    }
+}
+
+// FP64 iterative refinement around an FP32 MG-PCG: per outer step the true residual
+// r = b - A x in FP64, an inner PCG on A d = r with FP32 SpMV / V-cycle / vectors and FP64
+// dots, alpha and beta (relative reduction inner_tolerance, at most inner_iterations), then
+// x += d in FP64.
+static void solve_mixed(mgx_solver_s *S, const double *b, double *x, bool has_x0, mgx_stats *st)
+{
+    hipStream_t s = S->stream;
+    Level &V = S->lv[0];
+    const int32_t n = S->n;
+    Ctl *co = S->ctl.p, *ci = S->ctl_in.p;
+    Ctl outer{};
+    outer.stop = S->opt.stop;
+    outer.tol = S->opt.tolerance;
+    outer.max_it = 1;
+    Ctl inner{};
+    inner.stop = MGX_STOP_REL_L2;
+    inner.tol = S->opt.inner_tolerance;
+    inner.max_it = S->opt.inner_iterations;
+    if (!has_x0) launch_fill_zero(n, x, nullptr, s);
+    const int chunk = S->check_every;
+    int total_inner = 0, outer_steps = 0, status = 0;
+    for (;;) {
+        *S->ctl_host = outer;
+        MGX_HIP(hipMemcpyAsync(co, S->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, s));
+        launch_residual(n, V.A.rp, V.A.ci, V.A.v, b, x, S->r.p, V.A.lanes, nullptr, s);
+        launch_pcg_norms(n, b, S->r.p, co, S->partials.p, kPartialStride, s);
+        MGX_HIP(hipMemcpyAsync(S->ctl_host, co, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+        MGX_HIP(hipStreamSynchronize(s));
+        if (S->ctl_host->done) { status = S->ctl_host->status; break; }
+        if (outer_steps >= S->opt.max_iterations) { status = 0; break; }
+        // inner FP32 PCG on A d = r
+        const int32_t *done = &ci->done;
+        launch_convert_f64_f32(n, S->r.p, S->r32.p, nullptr, s);
+        launch_fill_zero_f32(n, S->d32.p, nullptr, s);
+        *S->ctl_in_host = inner;
+        MGX_HIP(hipMemcpyAsync(ci, S->ctl_in_host, sizeof(Ctl), hipMemcpyHostToDevice, s));
+        launch_pcg_norms_f32(n, S->r32.p, S->r32.p, ci, S->partials.p, kPartialStride, s);
+        vcycle32(S, 0, S->r32.p, S->z32.p, done);
+        launch_dot_beta_f32(n, S->r32.p, S->z32.p, ci, S->partials.p, true, s);
+        launch_update_p_f32(n, S->z32.p, S->p32.p, ci, true, s);
+        for (int it = 0;; it += chunk) {
+            for (int i = 0; i < chunk; ++i) {
+                launch_spmv_dot_alpha_f32(n, V.A.rp, V.A.ci, V.A.v32, S->p32.p, S->q32.p, V.A.lanes, ci, S->partials.p, s);
+                launch_pcg_update_f32(n, S->p32.p, S->q32.p, S->d32.p, S->r32.p, ci, S->partials.p, s);
+                vcycle32(S, 0, S->r32.p, S->z32.p, done);
+                launch_dot_beta_f32(n, S->r32.p, S->z32.p, ci, S->partials.p, false, s);
+                launch_update_p_f32(n, S->z32.p, S->p32.p, ci, false, s);
+            }
+            MGX_HIP(hipMemcpyAsync(S->ctl_in_host, ci, sizeof(Ctl), hipMemcpyDeviceToHost, s));
+            MGX_HIP(hipStreamSynchronize(s));
+            if (S->ctl_in_host->done || it + chunk >= inner.max_it) break;
+        }
+        total_inner += S->ctl_in_host->iter;
+        launch_axpy_f32_into_f64(n, S->d32.p, x, nullptr, s);
+        ++outer_steps;
+        if (S->ctl_in_host->status == -1) { status = -1; break; }
+    }
+    st->converged = status;
+    st->iterations = total_inner;
+    st->outer_iterations = outer_steps;
+}
+
 static void solve_device(mgx_solver_s *S, const double *b, double *x, bool has_x0, mgx_stats *stats)
 {
     MGX_REQUIRE(S->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
@@ -749,6 +885,7 @@ static void solve_device(mgx_solver_s *S, const double *b, double *x, bool has_x
         mgx_stats st{};
         MGX_HIP(hipEventRecord(S->ev0, s));
         if (S->opt.method == MGX_METHOD_GMG) solve_gmg(S, bk, xk, has_x0, &st);
+        else if (S->opt.precision == MGX_PRECISION_MIXED) solve_mixed(S, bk, xk, has_x0, &st);
         else solve_pcg(S, bk, xk, has_x0, &st);
         MGX_HIP(hipEventRecord(S->ev1, s));
         MGX_HIP(hipEventSynchronize(S->ev1));
@@ -1097,6 +1234,7 @@ int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
         MGX_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         MGX_HIP(hipHostMalloc((void **)&s->ctl_host, sizeof(Ctl), hipHostMallocDefault));
         MGX_HIP(hipHostMalloc((void **)&s->flag_host, sizeof(int32_t), hipHostMallocDefault));
+        MGX_HIP(hipHostMalloc((void **)&s->ctl_in_host, sizeof(Ctl), hipHostMallocDefault));
         MGX_HIP(hipEventCreate(&s->ev0));
         MGX_HIP(hipEventCreate(&s->ev1));
         if (opt->check_every > 0) s->check_every = opt->check_every;
@@ -1196,6 +1334,7 @@ int mgx_solver_release(mgx_solver s)
         if (s->stream) (void)hipStreamSynchronize(s->stream);
         if (s->ctl_host) (void)hipHostFree(s->ctl_host);
         if (s->flag_host) (void)hipHostFree(s->flag_host);
+        if (s->ctl_in_host) (void)hipHostFree(s->ctl_in_host);
         if (s->ev0) (void)hipEventDestroy(s->ev0);
         if (s->ev1) (void)hipEventDestroy(s->ev1);
         for (auto &e : s->prof_events) (void)hipEventDestroy(e);

@@ -93,6 +93,15 @@ void launch_pcg_update(int32_t n, const double *p, const double *q, double *x, d
                        Ctl *ctl, double *partials, hipStream_t s);
 void launch_dot_beta(int32_t n, const double *r, const double *z, Ctl *ctl, double *partials, bool first, hipStream_t s);
 void launch_update_p(int32_t n, const double *z, double *p, const Ctl *ctl, bool first, hipStream_t s);
+// precision form: FP32 vectors, FP64 scalars / accumulation
+void launch_pcg_norms_f32(int32_t n, const float *b, const float *r, Ctl *ctl, double *partials, int stride, hipStream_t s);
+void launch_spmv_dot_alpha_f32(int32_t n, const int32_t *rp, const int32_t *ci, const float *v, const float *p,
+                               float *q, int lanes, Ctl *ctl, double *partials, hipStream_t s);
+void launch_pcg_update_f32(int32_t n, const float *p, const float *q, float *x, float *r, Ctl *ctl,
+                           double *partials, hipStream_t s);
+void launch_dot_beta_f32(int32_t n, const float *r, const float *z, Ctl *ctl, double *partials, bool first, hipStream_t s);
+void launch_update_p_f32(int32_t n, const float *z, float *p, const Ctl *ctl, bool first, hipStream_t s);
+void launch_dense_gemv_f32io(int32_t n, const double *M, const float *x, float *y, const int32_t *done, hipStream_t s);
 // precision helpers
 void launch_convert_f64_f32(int64_t n, const double *a, float *b, const int32_t *done, hipStream_t s);
 void launch_convert_f32_f64(int64_t n, const float *a, double *b, const int32_t *done, hipStream_t s);

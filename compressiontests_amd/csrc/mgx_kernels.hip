@@ -286,14 +286,16 @@ void launch_jacobi_initial_f32(int32_t n, const float *pre, const float *b, floa
     MGX_HIP(hipGetLastError());
 }
 
-// phase 1 of a dot product
-__global__ __launch_bounds__(kBlock) void k_dot_partial(int32_t n, const double *__restrict__ a,
-                                                        const double *__restrict__ b, const int32_t *__restrict__ done,
+// phase 1 of a dot product (FP64 accumulation for FP64 and FP32 vectors)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_dot_partial(int32_t n, const T *__restrict__ a,
+                                                        const T *__restrict__ b, const int32_t *__restrict__ done,
                                                         double *partials)
 {
     if (done && *done) return;
     double s = 0;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) s += a[i] * b[i];
+    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
+        s += (double)a[i] * (double)b[i];
     block_publish(s, 0, 0, partials, 0, 1);
 }
 
@@ -358,14 +360,15 @@ void launch_dot(int32_t n, const double *a, const double *b, double *result, dou
 {
     (void)counter;
     const int g = grid_stream(n);
-    hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kBlock), 0, s, n, a, b, nullptr, partials);
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(g), dim3(kBlock), 0, s, n, a, b, nullptr, partials);
     MGX_HIP(hipGetLastError());
     launch_finalize(g, partials, 0, nullptr, FIN_DOT, result, s);
 }
 
 // ||b||^2, ||r||^2, max|r| before the first iteration (r = b - A x0 or b)
-__global__ __launch_bounds__(kBlock) void k_pcg_norms(int32_t n, const double *__restrict__ b,
-                                                      const double *__restrict__ r, double *partials, int stride)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_norms(int32_t n, const T *__restrict__ b,
+                                                      const T *__restrict__ r, double *partials, int stride)
 {
     double sb = 0, sr = 0, m = 0;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
@@ -380,17 +383,25 @@ __global__ __launch_bounds__(kBlock) void k_pcg_norms(int32_t n, const double *_
 void launch_pcg_norms(int32_t n, const double *b, const double *r, Ctl *ctl, double *partials, int stride, hipStream_t s)
 {
     const int g = grid_stream(n);
-    hipLaunchKernelGGL(k_pcg_norms, dim3(g), dim3(kBlock), 0, s, n, b, r, partials, stride);
+    hipLaunchKernelGGL(k_pcg_norms<double>, dim3(g), dim3(kBlock), 0, s, n, b, r, partials, stride);
+    MGX_HIP(hipGetLastError());
+    launch_finalize(g, partials, stride, ctl, FIN_NORMS, nullptr, s);
+}
+
+void launch_pcg_norms_f32(int32_t n, const float *b, const float *r, Ctl *ctl, double *partials, int stride, hipStream_t s)
+{
+    const int g = grid_stream(n);
+    hipLaunchKernelGGL(k_pcg_norms<float>, dim3(g), dim3(kBlock), 0, s, n, b, r, partials, stride);
     MGX_HIP(hipGetLastError());
     launch_finalize(g, partials, stride, ctl, FIN_NORMS, nullptr, s);
 }
 
 // q = A p and the per-block partials of p.q (phase 1 of alpha = rz / pq).  Same row mapping as
 // k_csr_rows (one G-lane group per row, XCD-remapped blocks).
-template <int G>
+template <int G, typename T>
 __global__ __launch_bounds__(kBlock) void k_spmv_dot(int32_t nrows, const int32_t *__restrict__ rp,
-                                                     const int32_t *__restrict__ ci, const double *__restrict__ v,
-                                                     const double *__restrict__ p, double *__restrict__ q,
+                                                     const int32_t *__restrict__ ci, const T *__restrict__ v,
+                                                     const T *__restrict__ p, T *__restrict__ q,
                                                      const int32_t *__restrict__ done, double *partials)
 {
     if (*done) return;
@@ -400,18 +411,19 @@ __global__ __launch_bounds__(kBlock) void k_spmv_dot(int32_t nrows, const int32_
     const int lane = threadIdx.x % G;
     double contrib = 0;
     if (row < nrows) {
-        const double s = row_dot<G, double>(rp, ci, v, p, row, lane);
-        if (lane == 0) { q[row] = s; contrib = p[row] * s; }
+        const T s = row_dot<G, T>(rp, ci, v, p, row, lane);
+        if (lane == 0) { q[row] = s; contrib = (double)p[row] * (double)s; }
     }
     block_publish(contrib, 0, 0, partials, 0, 1);
 }
 
-int launch_spmv_dot(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *p, double *q,
-                    int lanes, const int32_t *done, double *partials, hipStream_t s)
+template <typename T>
+int launch_spmv_dot_t(int32_t n, const int32_t *rp, const int32_t *ci, const T *v, const T *p, T *q,
+                      int lanes, const int32_t *done, double *partials, hipStream_t s)
 {
     int grid = 0;
 #define MGX_SDA(G) grid = grid_rows(n, kBlock / G); \
-    hipLaunchKernelGGL((k_spmv_dot<G>), dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, p, q, done, partials)
+    hipLaunchKernelGGL((k_spmv_dot<G, T>), dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, p, q, done, partials)
     switch (lanes) {
     case 2: MGX_SDA(2); break;
     case 4: MGX_SDA(4); break;
@@ -430,24 +442,32 @@ int reduce_slots_for_rows(int64_t nrows, int lanes) { return grid_rows(nrows, kB
 void launch_spmv_dot_alpha(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *p,
                            double *q, int lanes, Ctl *ctl, double *partials, hipStream_t s)
 {
-    const int g = launch_spmv_dot(n, rp, ci, v, p, q, lanes, &ctl->done, partials, s);
+    const int g = launch_spmv_dot_t<double>(n, rp, ci, v, p, q, lanes, &ctl->done, partials, s);
+    launch_finalize(g, partials, 0, ctl, FIN_ALPHA, nullptr, s);
+}
+
+void launch_spmv_dot_alpha_f32(int32_t n, const int32_t *rp, const int32_t *ci, const float *v, const float *p,
+                               float *q, int lanes, Ctl *ctl, double *partials, hipStream_t s)
+{
+    const int g = launch_spmv_dot_t<float>(n, rp, ci, v, p, q, lanes, &ctl->done, partials, s);
     launch_finalize(g, partials, 0, ctl, FIN_ALPHA, nullptr, s);
 }
 
 // x += alpha p; r -= alpha q; partials of ||r||^2 and max|r| (phase 1 of the stop test)
-__global__ __launch_bounds__(kBlock) void k_pcg_update(int32_t n, const double *__restrict__ p,
-                                                       const double *__restrict__ q, double *__restrict__ x,
-                                                       double *__restrict__ r, const Ctl *ctl, double *partials, int stride)
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_pcg_update(int32_t n, const T *__restrict__ p,
+                                                       const T *__restrict__ q, T *__restrict__ x,
+                                                       T *__restrict__ r, const Ctl *ctl, double *partials, int stride)
 {
     if (ctl->done) return;
     const double alpha = ctl->alpha;
     double s = 0, m = 0;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        x[i] = x[i] + alpha * p[i];
-        const double ri = r[i] - alpha * q[i];
+        x[i] = (T)((double)x[i] + alpha * (double)p[i]);
+        const T ri = (T)((double)r[i] - alpha * (double)q[i]);
         r[i] = ri;
-        s += ri * ri;
-        m = nanmax(m, fabs(ri));
+        s += (double)ri * (double)ri;
+        m = nanmax(m, fabs((double)ri));
     }
     block_publish(s, 0, m, partials, stride, 3);
 }
@@ -457,7 +477,17 @@ void launch_pcg_update(int32_t n, const double *p, const double *q, double *x, d
 {
     const int g = grid_stream(n);
     const int stride = kPartialStride;
-    hipLaunchKernelGGL(k_pcg_update, dim3(g), dim3(kBlock), 0, s, n, p, q, x, r, ctl, partials, stride);
+    hipLaunchKernelGGL(k_pcg_update<double>, dim3(g), dim3(kBlock), 0, s, n, p, q, x, r, ctl, partials, stride);
+    MGX_HIP(hipGetLastError());
+    launch_finalize(g, partials, stride, ctl, FIN_UPDATE, nullptr, s);
+}
+
+void launch_pcg_update_f32(int32_t n, const float *p, const float *q, float *x, float *r, Ctl *ctl,
+                           double *partials, hipStream_t s)
+{
+    const int g = grid_stream(n);
+    const int stride = kPartialStride;
+    hipLaunchKernelGGL(k_pcg_update<float>, dim3(g), dim3(kBlock), 0, s, n, p, q, x, r, ctl, partials, stride);
     MGX_HIP(hipGetLastError());
     launch_finalize(g, partials, stride, ctl, FIN_UPDATE, nullptr, s);
 }
@@ -465,23 +495,61 @@ void launch_pcg_update(int32_t n, const double *p, const double *q, double *x, d
 void launch_dot_beta(int32_t n, const double *r, const double *z, Ctl *ctl, double *partials, bool first, hipStream_t s)
 {
     const int g = grid_stream(n);
-    hipLaunchKernelGGL(k_dot_partial, dim3(g), dim3(kBlock), 0, s, n, r, z, &ctl->done, partials);
+    hipLaunchKernelGGL(k_dot_partial<double>, dim3(g), dim3(kBlock), 0, s, n, r, z, &ctl->done, partials);
     MGX_HIP(hipGetLastError());
     launch_finalize(g, partials, 0, ctl, first ? FIN_BETA_FIRST : FIN_BETA, nullptr, s);
 }
 
-__global__ __launch_bounds__(kBlock) void k_update_p(int32_t n, const double *__restrict__ z, double *__restrict__ p,
+void launch_dot_beta_f32(int32_t n, const float *r, const float *z, Ctl *ctl, double *partials, bool first, hipStream_t s)
+{
+    const int g = grid_stream(n);
+    hipLaunchKernelGGL(k_dot_partial<float>, dim3(g), dim3(kBlock), 0, s, n, r, z, &ctl->done, partials);
+    MGX_HIP(hipGetLastError());
+    launch_finalize(g, partials, 0, ctl, first ? FIN_BETA_FIRST : FIN_BETA, nullptr, s);
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_update_p(int32_t n, const T *__restrict__ z, T *__restrict__ p,
                                                      const Ctl *ctl, int first)
 {
     if (ctl->done) return;
     const double beta = first ? 0.0 : ctl->beta;
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-        p[i] = first ? z[i] : z[i] + beta * p[i];
+        p[i] = first ? z[i] : (T)((double)z[i] + beta * (double)p[i]);
 }
 
 void launch_update_p(int32_t n, const double *z, double *p, const Ctl *ctl, bool first, hipStream_t s)
 {
-    hipLaunchKernelGGL(k_update_p, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, z, p, ctl, first ? 1 : 0);
+    hipLaunchKernelGGL(k_update_p<double>, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, z, p, ctl, first ? 1 : 0);
+    MGX_HIP(hipGetLastError());
+}
+
+void launch_update_p_f32(int32_t n, const float *z, float *p, const Ctl *ctl, bool first, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_update_p<float>, dim3(grid_stream(n)), dim3(kBlock), 0, s, n, z, p, ctl, first ? 1 : 0);
+    MGX_HIP(hipGetLastError());
+}
+
+// coarse solve of the precision form: y (float) = M (FP64) x (float), FP64 accumulation
+__global__ __launch_bounds__(kBlock) void k_dense_gemv_f32io(int32_t n, const double *__restrict__ M,
+                                                             const float *__restrict__ x, float *__restrict__ y,
+                                                             const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    const int row = blockIdx.x * (kBlock / 64) + (threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+    if (row >= n) return;
+    const double *Mr = M + (int64_t)row * n;
+    double s = 0;
+    for (int j = lane; j < n; j += 64) s += Mr[j] * (double)x[j];
+    s = wave_sum(s);
+    if (lane == 0) y[row] = (float)s;
+}
+
+void launch_dense_gemv_f32io(int32_t n, const double *M, const float *x, float *y, const int32_t *done, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL(k_dense_gemv_f32io, dim3((n + 3) / 4), dim3(kBlock), 0, s, n, M, x, y, done);
     MGX_HIP(hipGetLastError());
 }
 
@@ -534,26 +602,31 @@ void launch_axpy_f32_into_f64(int64_t n, const float *a, double *x, const int32_
 }
 
 // ------------------------------------------------------------------ multicolour Gauss-Seidel
-// One node (3 rows) per 64-lane wavefront: lanes 16a..16a+15 reduce row a (a = 0, 1, 2) into
-// s_a = sum over columns outside the node and d_ab = the node's own 3x3 block, all loads of a
-// row issued up front; lane 0 then relaxes the three dofs in order (reverse when backward).
+// A block of 192 threads relaxes 4 nodes (12 rows): each row is reduced by 16 lanes into
+// s_a = sum over columns outside the node and d_ab = the node's own 3x3 block, with all loads of
+// the row issued up front; the per-row results go through LDS and one thread per node relaxes
+// the three dofs in order (reverse order when backward).  Nodes of one colour are uncoupled, so
+// the x values read here are not written by this launch except the node's own.
+constexpr int kGsBlock = 192, kGsNodes = 4;
+
 template <typename T>
-__global__ __launch_bounds__(kBlock) void k_gs_color(int32_t nnodes, const int32_t *__restrict__ nodes,
-                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                     const T *__restrict__ v, const T *__restrict__ b, T *__restrict__ x,
-                                                     int backward, const int32_t *__restrict__ done)
+__global__ __launch_bounds__(kGsBlock) void k_gs_color(int32_t nnodes, const int32_t *__restrict__ nodes,
+                                                       const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                       const T *__restrict__ v, const T *__restrict__ b, T *__restrict__ x,
+                                                       int backward, const int32_t *__restrict__ done)
 {
     constexpr int G = 16, U = 6;
+    __shared__ T sh[kGsNodes][3][4];
     if (done && *done) return;
-    const int grp = xcd_remap(blockIdx.x, gridDim.x) * (kBlock / 64) + (threadIdx.x >> 6);
-    if (grp >= nnodes) return;                     // whole wavefront leaves together
-    const int wl = threadIdx.x & 63;
-    const int a = wl >> 4;                         // row of the node handled by this quarter
-    const int lane = wl & 15;
-    const int node = nodes[grp];
-    const int r0 = 3 * node;
+    const int base = xcd_remap(blockIdx.x, gridDim.x) * kGsNodes;
+    const int rowslot = threadIdx.x / G;           // 0..11
+    const int ln = rowslot / 3, a = rowslot % 3;   // node within block, row within node
+    const int lane = threadIdx.x % G;
+    const int grp = base + ln;
     T sa = 0, d0 = 0, d1 = 0, d2 = 0;
-    if (a < 3) {
+    int r0 = 0;
+    if (grp < nnodes) {
+        r0 = 3 * nodes[grp];
         const int s = rp[r0 + a], e = rp[r0 + a + 1];
         for (int k = s + lane; k < e; k += U * G) {
             int c[U];
@@ -563,12 +636,18 @@ __global__ __launch_bounds__(kBlock) void k_gs_color(int32_t nnodes, const int32
                 const int kk = k + u * G;
                 if (kk < e) { c[u] = ci[kk]; w[u] = v[kk]; } else { c[u] = -1; w[u] = 0; }
             }
+            T xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned off = (unsigned)(c[u] - r0);
+                xv[u] = (c[u] >= 0 && off >= 3u) ? x[c[u]] : T(0);
+            }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const unsigned off = (unsigned)(c[u] - r0);
                 if (c[u] < 0) continue;
                 if (off < 3u) { if (off == 0) d0 += w[u]; else if (off == 1) d1 += w[u]; else d2 += w[u]; }
-                else sa += w[u] * x[c[u]];
+                else sa += w[u] * xv[u];
             }
         }
     }
@@ -579,22 +658,24 @@ __global__ __launch_bounds__(kBlock) void k_gs_color(int32_t nnodes, const int32
         d1 += __shfl_xor(d1, o, G);
         d2 += __shfl_xor(d2, o, G);
     }
-    // lane 0 gathers rows 1 and 2 from lanes 16 and 32
-    const T s1 = __shfl(sa, 16, 64), e10 = __shfl(d0, 16, 64), e11 = __shfl(d1, 16, 64), e12 = __shfl(d2, 16, 64);
-    const T s2 = __shfl(sa, 32, 64), e20 = __shfl(d0, 32, 64), e21 = __shfl(d1, 32, 64), e22 = __shfl(d2, 32, 64);
-    if (wl == 0) {
-        T x0 = x[r0], x1 = x[r0 + 1], x2 = x[r0 + 2];
-        const T b0 = b[r0] - sa, b1 = b[r0 + 1] - s1, b2 = b[r0 + 2] - s2;
+    if (lane == 0) { sh[ln][a][0] = sa; sh[ln][a][1] = d0; sh[ln][a][2] = d1; sh[ln][a][3] = d2; }
+    __syncthreads();
+    if (threadIdx.x < kGsNodes && base + (int)threadIdx.x < nnodes) {
+        const int m = threadIdx.x;
+        const int q0 = 3 * nodes[base + m];
+        const T(*R)[4] = sh[m];
+        T x0 = x[q0], x1 = x[q0 + 1], x2 = x[q0 + 2];
+        const T b0 = b[q0] - R[0][0], b1 = b[q0 + 1] - R[1][0], b2 = b[q0 + 2] - R[2][0];
         if (!backward) {
-            x0 = (b0 - d1 * x1 - d2 * x2) / d0;
-            x1 = (b1 - e10 * x0 - e12 * x2) / e11;
-            x2 = (b2 - e20 * x0 - e21 * x1) / e22;
+            x0 = (b0 - R[0][2] * x1 - R[0][3] * x2) / R[0][1];
+            x1 = (b1 - R[1][1] * x0 - R[1][3] * x2) / R[1][2];
+            x2 = (b2 - R[2][1] * x0 - R[2][2] * x1) / R[2][3];
         } else {
-            x2 = (b2 - e20 * x0 - e21 * x1) / e22;
-            x1 = (b1 - e10 * x0 - e12 * x2) / e11;
-            x0 = (b0 - d1 * x1 - d2 * x2) / d0;
+            x2 = (b2 - R[2][1] * x0 - R[2][2] * x1) / R[2][3];
+            x1 = (b1 - R[1][1] * x0 - R[1][3] * x2) / R[1][2];
+            x0 = (b0 - R[0][2] * x1 - R[0][3] * x2) / R[0][1];
         }
-        x[r0] = x0; x[r0 + 1] = x1; x[r0 + 2] = x2;
+        x[q0] = x0; x[q0 + 1] = x1; x[q0 + 2] = x2;
     }
 }
 
@@ -602,8 +683,8 @@ void launch_gs_color(int32_t nnodes, const int32_t *nodes, const int32_t *rp, co
                      const double *b, double *x, bool backward, const int32_t *done, hipStream_t s)
 {
     if (nnodes <= 0) return;
-    const int nb = grid_rows(nnodes, kBlock / 64);
-    hipLaunchKernelGGL(k_gs_color<double>, dim3(nb), dim3(kBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
+    const int nb = grid_rows(nnodes, kGsNodes);
+    hipLaunchKernelGGL(k_gs_color<double>, dim3(nb), dim3(kGsBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
                        backward ? 1 : 0, done);
     MGX_HIP(hipGetLastError());
 }
@@ -611,8 +692,8 @@ void launch_gs_color_f32(int32_t nnodes, const int32_t *nodes, const int32_t *rp
                          const float *b, float *x, bool backward, const int32_t *done, hipStream_t s)
 {
     if (nnodes <= 0) return;
-    const int nb = grid_rows(nnodes, kBlock / 64);
-    hipLaunchKernelGGL(k_gs_color<float>, dim3(nb), dim3(kBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
+    const int nb = grid_rows(nnodes, kGsNodes);
+    hipLaunchKernelGGL(k_gs_color<float>, dim3(nb), dim3(kGsBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
                        backward ? 1 : 0, done);
     MGX_HIP(hipGetLastError());
 }

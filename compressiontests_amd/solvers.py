@@ -258,7 +258,7 @@ class HipMgPcgSolver:
 
     def __init__(self, model, levels=3, smoother="jacobi", omega=0.0, nu=2, tolerance=1e-8, stop="rel",
                  max_iterations=1000, hierarchy="galerkin", format="csr", profile=False, check_every=4, nrhs=1,
-                 use_graph=True):
+                 use_graph=True, precision="fp64", inner_tolerance=1e-4, inner_iterations=200):
         self.model = model
         self.levels = levels
         opt = L.default_options(method=L.METHOD_MGPCG,
@@ -267,7 +267,9 @@ class HipMgPcgSolver:
                                 omega=omega, max_iterations=max_iterations, tolerance=tolerance,
                                 stop=L.STOP_REL_L2 if stop == "rel" else L.STOP_MAXNORM, level_iterations=nu,
                                 profile=int(profile), check_every=check_every, nrhs=nrhs,
-                                use_graph=int(bool(use_graph)))
+                                use_graph=int(bool(use_graph)),
+                                precision=L.PRECISION_MIXED if precision == "mixed" else L.PRECISION_FP64,
+                                inner_tolerance=inner_tolerance, inner_iterations=inner_iterations)
         self.opt = opt
         self.nrhs = nrhs
         h = _Handle(opt)

@@ -204,3 +204,64 @@ def test_mgpcg_64_heterogeneous(gpu):
     xt, stt = t.Solve()
     A = O.cantilever(n, n, n, dens=dens)
     assert energy_err(A, x, xt) < 1e-8
+
+
+@pytest.mark.parametrize("smoother", ["gs", "jacobi"])
+def test_mixed_precision_matches_fp64(gpu, smoother):
+    """Precision form: FP32 SpMV / V-cycle / vectors inside FP64 iterative refinement (FP64 true
+    residual, FP64 dots and alpha/beta, FP64 coarse inverse).  The refined solution meets the
+    FP64 stop test on the true residual and agrees with the oracle's FP64 MG-PCG solution of the
+    same system to 1e-8 in the energy norm (16^3, 1e4 modulus contrast, tip load)."""
+    n = 16
+    dens = 10 ** (4 * O.rng_uniform(20261205, n ** 3) - 2)
+    m = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
+    f = m.rhs()
+    A = O.cantilever(n, n, n, dens=dens)
+    xo, so = O.mgpcg(oracle_galerkin_levels(n, 3, dens), f, smoother=1, nu=2, tol=1e-12)
+    assert so.converged == 1
+    s = HipMgPcgSolver(m, levels=3, smoother=smoother, omega=0.0, nu=2, tolerance=1e-10, precision="mixed",
+                       inner_tolerance=1e-4)
+    s.Initialize(f)
+    for _ in range(2):                       # the second solve reuses every buffer
+        x, st = s.Solve()
+        assert st.HasConverged and not st.Failed
+        assert st.OuterIterations >= 2       # one FP32 inner solve cannot reach 1e-10
+        assert st.RelativeResidual <= 1.01e-10
+        r = f - S(A) @ x
+        assert np.linalg.norm(r) <= 1.01e-10 * np.linalg.norm(f)
+        assert energy_err(A, x, xo) < 1e-8
+
+
+def test_mixed_precision_maxnorm_and_caps(m16):
+    m, A, U, b = m16
+    s = HipMgPcgSolver(m, levels=3, smoother="gs", nu=1, tolerance=1e-9, stop="maxnorm", precision="mixed")
+    s.Initialize(b)
+    x, st = s.Solve()
+    assert st.HasConverged
+    assert np.max(np.abs(b - S(A) @ x)) <= 1e-9
+    assert energy_err(A, x, U) < 1e-8
+    s2 = HipMgPcgSolver(m, levels=3, smoother="gs", nu=1, tolerance=0.0, max_iterations=2, precision="mixed")
+    s2.Initialize(b)
+    x2, st2 = s2.Solve()
+    assert not st2.HasConverged and not st2.Failed and st2.OuterIterations == 2
+    with pytest.raises(L.MgxNotImplemented):
+        HipMgPcgSolver(m, levels=3, smoother="jacobi", nu=2, format="duvi", precision="mixed").Initialize(b)
+
+
+@pytest.mark.slow
+def test_mixed_precision_64_heterogeneous(gpu):
+    """64^3 with 1e4 modulus contrast: the precision form reaches 1e-9 on the FP64 true residual
+    and agrees with a 1e-12 FP64 solve to 1e-8 in the energy norm."""
+    n = 64
+    dens = 10 ** (4 * O.rng_uniform(20261205, n ** 3) - 2)
+    m = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
+    f = m.rhs()
+    s = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-9, precision="mixed")
+    s.Initialize(f)
+    x, st = s.Solve()
+    assert st.HasConverged and st.RelativeResidual <= 1.01e-9
+    t = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-12)
+    t.Initialize(f)
+    xt, stt = t.Solve()
+    A = O.cantilever(n, n, n, dens=dens)
+    assert energy_err(A, x, xt) < 1e-8
