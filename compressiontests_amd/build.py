@@ -6,6 +6,7 @@ travels with the source tree; it is listed in .gitignore.
 """
 import os
 import shutil
+import concurrent.futures
 import subprocess
 import sys
 
@@ -41,17 +42,21 @@ def _run(cmd):
 def build_lib(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     hipcc = _hipcc()
-    objs = []
-    for src in ("mgx_kernels.hip", "mgx_api.hip"):
+    jobs = []
+    for src in ("mgx_kernels.hip", "mgx_block.hip", "mgx_api.hip"):
         obj = os.path.join(BUILD, src + ".o")
-        out = _run([hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj])
-        if verbose and out:
-            print(out)
-        objs.append(obj)
+        jobs.append((obj, [hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]))
     # host-only setup code: no device code, no FMA contraction (bit-exact weights)
     obj = os.path.join(BUILD, "mgx_host.cpp.o")
-    _run([hipcc, *HOST_FLAGS, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-c", os.path.join(CSRC, "mgx_host.cpp"), "-o", obj])
-    objs.append(obj)
+    jobs.append((obj, [hipcc, *HOST_FLAGS, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-c",
+                       os.path.join(CSRC, "mgx_host.cpp"), "-o", obj]))
+    with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
+        outs = list(ex.map(lambda j: _run(j[1]), jobs))
+    if verbose:
+        for o in outs:
+            if o:
+                print(o)
+    objs = [j[0] for j in jobs]
     tmp = LIB + ".tmp"
     _run([hipcc, f"--offload-arch={ARCH}", "-shared", "-fPIC", "-o", tmp, *objs, "-lpthread"])
     os.replace(tmp, LIB)

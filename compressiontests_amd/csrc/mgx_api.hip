@@ -14,6 +14,7 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <type_traits>
 
 #include "mgx_internal.h"
 
@@ -66,6 +67,10 @@ struct DBuf {
         if (count == 0) count = 1;
         MGX_HIP(hipMalloc((void **)&p, sizeof(T) * count));
         n = count;
+    }
+    void ensure(size_t count)
+    {
+        if (!p || n < count) alloc(count);
     }
     void upload(const T *h, size_t count, hipStream_t s)
     {
@@ -212,6 +217,8 @@ struct Level {
     bool staged = false;
     DBuf<double> pre, x, b, r, t, w;       // omega D^-1 and work vectors (w: DuVi scratch)
     DBuf<float> pre32, x32, b32, r32, t32, w32;   // precision form
+    DBuf<double> bx, bb, br, bt;           // batching form: n x K block vectors
+    DBuf<float> bx32, bb32, br32, bt32;
     int ncolors = 0;
     std::vector<DBuf<int32_t>> color_nodes;
     std::vector<int32_t> color_count;
@@ -246,6 +253,19 @@ struct mgx_solver_s {
     hipGraphExec_t gexec = nullptr;        // captured chunk of PCG iterations
     const double *graph_x = nullptr;
     int graph_chunk = 0;
+    // batching form
+    int bk = 1;                            // block width in use (1: one right-hand side at a time)
+    int balloc = 0;                        // block width the buffers are sized for
+    std::vector<double> tune_ms;           // autotuner: ms per PCG iteration for K = 4, 8, 16, 32
+    mgx::DBuf<double> bB64, bX64, bR64, bP64, bQ64, bZ64;
+    mgx::DBuf<float> bD32, bR32, bP32, bQ32, bZ32;
+    mgx::DBuf<mgx::BCtl> bctl, bctl_in, bctl_stat;
+    mgx::BCtl *bctl_host = nullptr;        // pinned
+    mgx::DBuf<double> bpartials;
+    hipGraphExec_t bgexec = nullptr;       // captured chunk of block PCG iterations
+    int64_t bgraph_key = -1;
+    double bprof_ms = 0;                   // options.profile: level-0 SpMM time of the current block
+    int bprof_n = 0;
 };
 
 namespace mgx {
@@ -344,6 +364,8 @@ static double power_bound(const std::vector<int32_t> &rp, const std::vector<int3
     }
     return l;
 }
+
+static void choose_block_width(mgx_solver_s *S);
 
 static void initialize(mgx_solver_s *S)
 {
@@ -529,6 +551,9 @@ static void initialize(mgx_solver_s *S)
     MGX_HIP(hipStreamSynchronize(st));
     if (S->gexec) { MGX_HIP(hipGraphExecDestroy(S->gexec)); S->gexec = nullptr; }
     S->graph_x = nullptr;
+    S->balloc = 0;
+    choose_block_width(S);
+    MGX_HIP(hipStreamSynchronize(st));
     S->initialized = true;
 }
 
@@ -874,10 +899,396 @@ static void solve_mixed(mgx_solver_s *S, const double *b, double *x, bool has_x0
     st->outer_iterations = outer_steps;
 }
 
+
+// ---------------------------------------------------------------- batching form
+// K right-hand sides advance through one PCG with per-column alpha/beta (and per-column stop
+// tests); every SpMM, smoother sweep and transfer reads its matrix once for all K columns.
+template <typename T> struct BlkSel;
+template <> struct BlkSel<double> {
+    static double *x(Level &V) { return V.bx.p; }
+    static double *b(Level &V) { return V.bb.p; }
+    static double *r(Level &V) { return V.br.p; }
+    static double *t(Level &V) { return V.bt.p; }
+    static const double *val(const DevCsr &A) { return A.v; }
+    static const double *pre(Level &V) { return V.pre.p; }
+};
+template <> struct BlkSel<float> {
+    static float *x(Level &V) { return V.bx32.p; }
+    static float *b(Level &V) { return V.bb32.p; }
+    static float *r(Level &V) { return V.br32.p; }
+    static float *t(Level &V) { return V.bt32.p; }
+    static const float *val(const DevCsr &A) { return A.v32; }
+    static const float *pre(Level &V) { return V.pre32.p; }
+};
+static inline void fill_zero_t(int64_t n, double *a, const int32_t *d, hipStream_t s) { launch_fill_zero(n, a, d, s); }
+static inline void fill_zero_t(int64_t n, float *a, const int32_t *d, hipStream_t s) { launch_fill_zero_f32(n, a, d, s); }
+
+template <typename T>
+static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, bool backward, const int32_t *done)
+{
+    const T *Av = BlkSel<T>::val(V.A);
+    for (int i = 0; i < V.ncolors; ++i) {
+        const int c = backward ? V.ncolors - 1 - i : i;
+        launch_gs_block<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done, S->stream);
+    }
+}
+
+// the V-cycle of vcycle() applied to K columns at once
+template <typename T>
+static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const int32_t *done)
+{
+    using B = BlkSel<T>;
+    const int L = (int)S->lv.size();
+    Level &V = S->lv[l];
+    hipStream_t s = S->stream;
+    if (l == L - 1) { launch_dense_block<T>(K, V.n, S->coarse_inv.p, b, x, done, s); return; }
+    const int nu = S->opt.level_iterations;
+    const T *Av = B::val(V.A);
+    T *cur;
+    if (S->opt.smoother == MGX_SMOOTHER_GS) {
+        fill_zero_t((int64_t)V.n * K, x, done, s);
+        for (int i = 0; i < nu; ++i) gs_block_sweep<T>(S, V, K, b, x, false, done);
+        cur = x;
+    } else {
+        T *tt = B::t(V);
+        launch_bjacobi_initial<T>(K, V.n, B::pre(V), b, tt, done, s);
+        cur = tt;
+        for (int i = 1; i < nu; ++i) {
+            T *o = (cur == x) ? tt : x;
+            launch_spmm<T, T>(K, BEPI_JACOBI, V.n, V.A.rp, V.A.ci, Av, cur, b, B::pre(V), o, done, nullptr, 0, s);
+            cur = o;
+        }
+    }
+    launch_spmm<T, T>(K, BEPI_RESID, V.n, V.A.rp, V.A.ci, Av, cur, b, nullptr, B::r(V), done, nullptr, 0, s);
+    Level &C = S->lv[l + 1];
+    launch_spmm<T, T>(K, BEPI_SPMM, V.R.nrows, V.R.rp, V.R.ci, B::val(V.R), B::r(V), nullptr, nullptr, B::b(C), done,
+                      nullptr, 0, s);
+    vcycle_block<T>(S, K, l + 1, B::b(C), B::x(C), done);
+    launch_spmm<T, T>(K, BEPI_ADD, V.P.nrows, V.P.rp, V.P.ci, B::val(V.P), B::x(C), nullptr, nullptr, cur, done,
+                      nullptr, 0, s);
+    if (S->opt.smoother == MGX_SMOOTHER_GS) {
+        for (int i = 0; i < nu; ++i) gs_block_sweep<T>(S, V, K, b, x, true, done);
+    } else {
+        T *tt = B::t(V);
+        for (int i = 0; i < nu; ++i) {
+            T *o = (cur == x) ? tt : x;
+            launch_spmm<T, T>(K, BEPI_JACOBI, V.n, V.A.rp, V.A.ci, Av, cur, b, B::pre(V), o, done, nullptr, 0, s);
+            cur = o;
+        }
+        if (cur != x) launch_bcopy<T>((int64_t)V.n * K, cur, x, done, s);
+    }
+}
+
+template <typename T>
+static void block_start(mgx_solver_s *S, int K, BCtl *ctl, T *R, T *P, T *Z)
+{
+    const int32_t n = S->lv[0].n;
+    vcycle_block<T>(S, K, 0, R, Z, &ctl->done);
+    launch_bdot_beta<T>(K, n, R, Z, ctl, S->bpartials.p, true, S->stream);
+    launch_bupdate_p<T>(K, n, Z, P, ctl, true, S->stream);
+}
+
+template <typename T>
+static void block_iteration(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P, T *Q, T *Z, bool prof = false)
+{
+    Level &V = S->lv[0];
+    const int32_t n = V.n;
+    hipStream_t s = S->stream;
+    if (prof) MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used], s));
+    launch_spmm<T, T>(K, BEPI_DOT, n, V.A.rp, V.A.ci, BlkSel<T>::val(V.A), P, nullptr, nullptr, Q, &ctl->done,
+                      S->bpartials.p, kPartialStride, s);
+    if (prof) { MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used + 1], s)); ++S->prof_used; }
+    launch_bfinalize(spmm_grid(K, n), S->bpartials.p, ctl, FIN_ALPHA, s);
+    launch_bupdate<T>(K, n, P, Q, X, R, ctl, S->bpartials.p, s);
+    vcycle_block<T>(S, K, 0, R, Z, &ctl->done);
+    launch_bdot_beta<T>(K, n, R, Z, ctl, S->bpartials.p, false, s);
+    launch_bupdate_p<T>(K, n, Z, P, ctl, false, s);
+}
+
+// Runs block PCG iterations until every column is done; `check_every` iterations are queued
+// (or replayed as one captured hipGraph) per host check of the device flag.
+template <typename T>
+static void block_loop(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P, T *Q, T *Z, int tag, int max_it)
+{
+    hipStream_t s = S->stream;
+    const int chunk = S->check_every;
+    const bool graph = S->opt.use_graph != 0 && !S->opt.profile;
+    const int64_t key = ((int64_t)tag * 64 + K) * 65536 + chunk;
+    const int prof0 = S->prof_used;
+    if (graph && (!S->bgexec || S->bgraph_key != key)) {
+        if (S->bgexec) { MGX_HIP(hipGraphExecDestroy(S->bgexec)); S->bgexec = nullptr; }
+        hipGraph_t g = nullptr;
+        MGX_HIP(hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal));
+        try {
+            for (int i = 0; i < chunk; ++i) block_iteration<T>(S, K, ctl, X, R, P, Q, Z);
+        } catch (...) {
+            (void)hipStreamEndCapture(s, &g);
+            if (g) (void)hipGraphDestroy(g);
+            throw;
+        }
+        MGX_HIP(hipStreamEndCapture(s, &g));
+        const hipError_t e = hipGraphInstantiate(&S->bgexec, g, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(g);
+        MGX_HIP(e);
+        S->bgraph_key = key;
+    }
+    for (int it = 0;; it += chunk) {
+        if (graph) MGX_HIP(hipGraphLaunch(S->bgexec, s));
+        else
+            for (int i = 0; i < chunk; ++i)
+                block_iteration<T>(S, K, ctl, X, R, P, Q, Z,
+                                   S->opt.profile && (size_t)(2 * S->prof_used + 2) <= S->prof_events.size());
+        MGX_HIP(hipMemcpyAsync(S->bctl_host, ctl, sizeof(BCtl), hipMemcpyDeviceToHost, s));
+        MGX_HIP(hipStreamSynchronize(s));
+        if (S->bctl_host->done || it + chunk >= max_it) break;
+    }
+    if (S->opt.profile) {
+        // launches queued after every column was done return at once: count only the real ones
+        int real = 0;
+        for (int c = 0; c < K; ++c) real = std::max(real, S->bctl_host->iter[c]);
+        real = std::min(real, S->prof_used - prof0);
+        for (int i = prof0; i < prof0 + real; ++i) {
+            float ms = 0;
+            MGX_HIP(hipEventElapsedTime(&ms, S->prof_events[2 * i], S->prof_events[2 * i + 1]));
+            S->bprof_ms += ms;
+        }
+        S->bprof_n += real;
+        S->prof_used = prof0 + real;
+    }
+}
+
+static BCtl block_ctl_init(int K, int kc, int stop, double tol, int max_it)
+{
+    BCtl c;
+    std::memset(&c, 0, sizeof c);
+    c.k = K;
+    c.stop = stop;
+    c.tol = tol;
+    c.max_it = max_it;
+    for (int j = kc; j < K; ++j) c.frozen[j] = 1;   // padding columns of the last block
+    return c;
+}
+
+static void put_ctl(mgx_solver_s *S, BCtl *dev, const BCtl &h)
+{
+    *S->bctl_host = h;
+    MGX_HIP(hipMemcpyAsync(dev, S->bctl_host, sizeof(BCtl), hipMemcpyHostToDevice, S->stream));
+}
+
+// FP64 block PCG on columns j0 .. j0 + kc of b (column-major, leading dimension n)
+static void solve_block_fp64(mgx_solver_s *S, int K, int j0, int kc, const double *b, double *x, bool has_x0,
+                             mgx_stats *st)
+{
+    hipStream_t s = S->stream;
+    Level &V = S->lv[0];
+    const int32_t n = V.n;
+    const int64_t nk = (int64_t)n * K;
+    BCtl *ctl = S->bctl.p;
+    const BCtl init = block_ctl_init(K, kc, S->opt.stop, S->opt.tolerance, S->opt.max_iterations);
+    put_ctl(S, ctl, init);
+    launch_pack_cols(n, K, kc, b + (size_t)j0 * n, n, S->bB64.p, s);
+    if (has_x0) {
+        launch_pack_cols(n, K, kc, x + (size_t)j0 * n, n, S->bX64.p, s);
+        launch_spmm<double, double>(K, BEPI_RESID, n, V.A.rp, V.A.ci, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bR64.p,
+                                    nullptr, nullptr, 0, s);
+    } else {
+        launch_bcopy<double>(nk, S->bB64.p, S->bR64.p, nullptr, s);
+        launch_fill_zero(nk, S->bX64.p, nullptr, s);
+    }
+    launch_bnorms<double>(K, n, S->bB64.p, S->bR64.p, ctl, S->bpartials.p, s);
+    block_start<double>(S, K, ctl, S->bR64.p, S->bP64.p, S->bZ64.p);
+    block_loop<double>(S, K, ctl, S->bX64.p, S->bR64.p, S->bP64.p, S->bQ64.p, S->bZ64.p, 0, init.max_it);
+    const BCtl &c = *S->bctl_host;
+    for (int j = 0; j < kc; ++j) { st[j].iterations = c.iter[j]; st[j].converged = c.status[j]; }
+}
+
+// Precision form of the block solve: FP64 iterative refinement, each step an FP32 block PCG on
+// A D = R (R = B - A X in FP64) with per-column relative reduction inner_tolerance.
+static void solve_block_mixed(mgx_solver_s *S, int K, int j0, int kc, const double *b, double *x, bool has_x0,
+                              mgx_stats *st)
+{
+    hipStream_t s = S->stream;
+    Level &V = S->lv[0];
+    const int32_t n = V.n;
+    const int64_t nk = (int64_t)n * K;
+    launch_pack_cols(n, K, kc, b + (size_t)j0 * n, n, S->bB64.p, s);
+    if (has_x0) launch_pack_cols(n, K, kc, x + (size_t)j0 * n, n, S->bX64.p, s);
+    else launch_fill_zero(nk, S->bX64.p, nullptr, s);
+    std::vector<int> iters(K, 0), outer_n(K, 0), status(K, 0), failed(K, 0);
+    int steps = 0;
+    for (;;) {
+        BCtl outer = block_ctl_init(K, kc, S->opt.stop, S->opt.tolerance, 1);
+        for (int j = 0; j < kc; ++j) outer.frozen[j] = failed[j];
+        put_ctl(S, S->bctl.p, outer);
+        launch_spmm<double, double>(K, BEPI_RESID, n, V.A.rp, V.A.ci, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bR64.p,
+                                    nullptr, nullptr, 0, s);
+        launch_bnorms<double>(K, n, S->bB64.p, S->bR64.p, S->bctl.p, S->bpartials.p, s);
+        MGX_HIP(hipMemcpyAsync(S->bctl_host, S->bctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, s));
+        MGX_HIP(hipStreamSynchronize(s));
+        const BCtl o = *S->bctl_host;
+        for (int j = 0; j < kc; ++j) if (!failed[j]) status[j] = o.status[j];
+        if (o.done || steps >= S->opt.max_iterations) break;
+        BCtl in = block_ctl_init(K, kc, MGX_STOP_REL_L2, S->opt.inner_tolerance, S->opt.inner_iterations);
+        for (int j = 0; j < K; ++j) in.frozen[j] = o.cdone[j];
+        put_ctl(S, S->bctl_in.p, in);
+        launch_convert_f64_f32(nk, S->bR64.p, S->bR32.p, nullptr, s);
+        launch_fill_zero_f32(nk, S->bD32.p, nullptr, s);
+        launch_bnorms<float>(K, n, S->bR32.p, S->bR32.p, S->bctl_in.p, S->bpartials.p, s);
+        block_start<float>(S, K, S->bctl_in.p, S->bR32.p, S->bP32.p, S->bZ32.p);
+        block_loop<float>(S, K, S->bctl_in.p, S->bD32.p, S->bR32.p, S->bP32.p, S->bQ32.p, S->bZ32.p, 1, in.max_it);
+        const BCtl &ci = *S->bctl_host;
+        for (int j = 0; j < kc; ++j) {
+            if (o.cdone[j]) continue;
+            iters[j] += ci.iter[j];
+            ++outer_n[j];
+            if (ci.status[j] == -1) { failed[j] = 1; status[j] = -1; }
+        }
+        launch_axpy_f32_into_f64(nk, S->bD32.p, S->bX64.p, nullptr, s);
+        ++steps;
+    }
+    for (int j = 0; j < kc; ++j) { st[j].iterations = iters[j]; st[j].outer_iterations = outer_n[j]; st[j].converged = status[j]; }
+}
+
+// true FP64 residual of every column of the block
+static void finish_stats_block(mgx_solver_s *S, int K, int kc, mgx_stats *st)
+{
+    hipStream_t s = S->stream;
+    Level &V = S->lv[0];
+    const int32_t n = V.n;
+    launch_spmm<double, double>(K, BEPI_RESID, n, V.A.rp, V.A.ci, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bQ64.p,
+                                nullptr, nullptr, 0, s);
+    put_ctl(S, S->bctl_stat.p, block_ctl_init(K, kc, MGX_STOP_REL_L2, 0.0, 1));
+    launch_bnorms<double>(K, n, S->bB64.p, S->bQ64.p, S->bctl_stat.p, S->bpartials.p, s);
+    MGX_HIP(hipMemcpyAsync(S->bctl_host, S->bctl_stat.p, sizeof(BCtl), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    const BCtl &c = *S->bctl_host;
+    for (int j = 0; j < kc; ++j) {
+        st[j].residual_norm = std::sqrt(c.rnorm2[j]);
+        st[j].rel_residual = c.bnorm2[j] > 0 ? std::sqrt(c.rnorm2[j] / c.bnorm2[j]) : std::sqrt(c.rnorm2[j]);
+    }
+}
+
+static void solve_blocks(mgx_solver_s *S, const double *b, double *x, bool has_x0, mgx_stats *stats)
+{
+    hipStream_t s = S->stream;
+    const int K = S->bk;
+    const int32_t n = S->n;
+    for (int j0 = 0; j0 < S->nrhs; j0 += K) {
+        const int kc = std::min(K, S->nrhs - j0);
+        std::vector<mgx_stats> st(kc);
+        for (auto &e : st) { std::memset(&e, 0, sizeof e); e.block_width = K; }
+        S->prof_used = 0;
+        S->bprof_ms = 0;
+        S->bprof_n = 0;
+        MGX_HIP(hipEventRecord(S->ev0, s));
+        if (S->opt.precision == MGX_PRECISION_MIXED) solve_block_mixed(S, K, j0, kc, b, x, has_x0, st.data());
+        else solve_block_fp64(S, K, j0, kc, b, x, has_x0, st.data());
+        MGX_HIP(hipEventRecord(S->ev1, s));
+        MGX_HIP(hipEventSynchronize(S->ev1));
+        float ms = 0;
+        MGX_HIP(hipEventElapsedTime(&ms, S->ev0, S->ev1));
+        launch_unpack_cols(n, K, kc, S->bX64.p, x + (size_t)j0 * n, n, s);
+        finish_stats_block(S, K, kc, st.data());
+        for (int j = 0; j < kc; ++j) {
+            st[j].solve_ms = ms;
+            st[j].spmv_launches = S->bprof_n;
+            st[j].spmv_ms = S->bprof_ms;
+            stats[j0 + j] = st[j];
+        }
+    }
+}
+
+// buffers of the batching form for block width K
+static void alloc_block(mgx_solver_s *S, int K)
+{
+    if (S->balloc >= K) return;
+    const int L = (int)S->lv.size();
+    const bool mixed = S->opt.precision == MGX_PRECISION_MIXED;
+    for (int l = 0; l < L; ++l) {
+        Level &V = S->lv[l];
+        const size_t nk = (size_t)V.n * K;
+        if (l < L - 1) { if (mixed) { V.br32.alloc(nk); V.bt32.alloc(nk); } else { V.br.alloc(nk); V.bt.alloc(nk); } }
+        if (l > 0) { if (mixed) { V.bx32.alloc(nk); V.bb32.alloc(nk); } else { V.bx.alloc(nk); V.bb.alloc(nk); } }
+    }
+    const size_t nk = (size_t)S->n * K;
+    S->bB64.alloc(nk); S->bX64.alloc(nk); S->bR64.alloc(nk); S->bQ64.alloc(nk);
+    if (mixed) { S->bD32.alloc(nk); S->bR32.alloc(nk); S->bP32.alloc(nk); S->bQ32.alloc(nk); S->bZ32.alloc(nk); }
+    else { S->bP64.alloc(nk); S->bZ64.alloc(nk); }
+    S->bpartials.ensure(kBlockPartials);
+    S->bctl.ensure(1); S->bctl_in.ensure(1); S->bctl_stat.ensure(1);
+    if (S->bgexec) { MGX_HIP(hipGraphExecDestroy(S->bgexec)); S->bgexec = nullptr; }
+    S->bgraph_key = -1;
+    S->balloc = K;
+}
+
+// ms per block PCG iteration at width K on the first right-hand sides (stop test disabled)
+template <typename T>
+static double time_block_iteration(mgx_solver_s *S, int K)
+{
+    hipStream_t s = S->stream;
+    const int32_t n = S->n;
+    const int64_t nk = (int64_t)n * K;
+    const int kc = std::min(K, S->nrhs);
+    launch_pack_cols(n, K, kc, S->bdev.p, n, S->bB64.p, s);
+    T *X, *R, *P, *Q, *Z;
+    BCtl *ctl;
+    if constexpr (std::is_same<T, double>::value) {
+        X = S->bX64.p; R = S->bR64.p; P = S->bP64.p; Q = S->bQ64.p; Z = S->bZ64.p; ctl = S->bctl.p;
+        launch_bcopy<double>(nk, S->bB64.p, R, nullptr, s);
+    } else {
+        X = S->bD32.p; R = S->bR32.p; P = S->bP32.p; Q = S->bQ32.p; Z = S->bZ32.p; ctl = S->bctl_in.p;
+        launch_convert_f64_f32(nk, S->bB64.p, R, nullptr, s);
+    }
+    fill_zero_t(nk, X, nullptr, s);
+    put_ctl(S, ctl, block_ctl_init(K, kc, MGX_STOP_REL_L2, 0.0, 1 << 30));
+    launch_bnorms<T>(K, n, R, R, ctl, S->bpartials.p, s);
+    block_start<T>(S, K, ctl, R, P, Z);
+    for (int i = 0; i < 2; ++i) block_iteration<T>(S, K, ctl, X, R, P, Q, Z);
+    const int reps = 4;
+    MGX_HIP(hipEventRecord(S->ev0, s));
+    for (int i = 0; i < reps; ++i) block_iteration<T>(S, K, ctl, X, R, P, Q, Z);
+    MGX_HIP(hipEventRecord(S->ev1, s));
+    MGX_HIP(hipEventSynchronize(S->ev1));
+    float ms = 0;
+    MGX_HIP(hipEventElapsedTime(&ms, S->ev0, S->ev1));
+    return ms / reps;
+}
+
+// block width: fixed by the options, or the K in {4, 8, 16, 32} (not above the next power of
+// two of nrhs) with the least time per column-iteration
+static void choose_block_width(mgx_solver_s *S)
+{
+    const int bw = S->opt.block_width;
+    const bool able = S->opt.method == MGX_METHOD_MGPCG && S->opt.format == MGX_FORMAT_CSR && S->nrhs > 1;
+    S->tune_ms.assign(4, 0.0);
+    if (bw == 1 || !able) { S->bk = 1; return; }
+    if (bw > 1) {
+        MGX_REQUIRE(bw == 4 || bw == 8 || bw == 16 || bw == 32, MGX_ERR_INVALID_ARGUMENT,
+                    "block_width must be 0, 1, 4, 8, 16 or 32");
+        alloc_block(S, bw);
+        S->bk = bw;
+        return;
+    }
+    int kmax = 4;
+    while (kmax < S->nrhs && kmax < 32) kmax *= 2;
+    alloc_block(S, kmax);
+    const bool mixed = S->opt.precision == MGX_PRECISION_MIXED;
+    double best = 0;
+    int bestK = 4;
+    for (int K = 4, i = 0; K <= kmax; K *= 2, ++i) {
+        const double ms = mixed ? time_block_iteration<float>(S, K) : time_block_iteration<double>(S, K);
+        S->tune_ms[i] = ms;
+        const double per_col = ms / K;
+        if (K == 4 || per_col < best) { best = per_col; bestK = K; }
+    }
+    S->bk = bestK;
+}
+
 static void solve_device(mgx_solver_s *S, const double *b, double *x, bool has_x0, mgx_stats *stats)
 {
     MGX_REQUIRE(S->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
     MGX_REQUIRE(stats, MGX_ERR_INVALID_ARGUMENT, "stats must not be NULL");
+    if (S->bk > 1) { solve_blocks(S, b, x, has_x0, stats); return; }
     hipStream_t s = S->stream;
     for (int k = 0; k < S->nrhs; ++k) {
         const double *bk = b + (size_t)k * S->n;
@@ -893,6 +1304,7 @@ static void solve_device(mgx_solver_s *S, const double *b, double *x, bool has_x
         MGX_HIP(hipEventElapsedTime(&ms, S->ev0, S->ev1));
         st.solve_ms = ms;
         finish_stats(S, bk, xk, &st);
+        st.block_width = 1;
         stats[k] = st;
     }
 }
@@ -1215,6 +1627,7 @@ void mgx_options_default(mgx_options *o)
     o->profile = 0;
     o->check_every = 4;
     o->use_graph = 1;
+    o->block_width = 0;
 }
 
 int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
@@ -1235,6 +1648,7 @@ int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
         MGX_HIP(hipHostMalloc((void **)&s->ctl_host, sizeof(Ctl), hipHostMallocDefault));
         MGX_HIP(hipHostMalloc((void **)&s->flag_host, sizeof(int32_t), hipHostMallocDefault));
         MGX_HIP(hipHostMalloc((void **)&s->ctl_in_host, sizeof(Ctl), hipHostMallocDefault));
+        MGX_HIP(hipHostMalloc((void **)&s->bctl_host, sizeof(BCtl), hipHostMallocDefault));
         MGX_HIP(hipEventCreate(&s->ev0));
         MGX_HIP(hipEventCreate(&s->ev1));
         if (opt->check_every > 0) s->check_every = opt->check_every;
@@ -1327,6 +1741,17 @@ int mgx_solver_solve_device(mgx_solver s, const double *b_dev, double *x_dev, in
     });
 }
 
+int mgx_solver_block_width(mgx_solver s, int32_t *width, double *tune_ms)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && width, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(s->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
+        *width = s->bk;
+        if (tune_ms)
+            for (int i = 0; i < 4; ++i) tune_ms[i] = i < (int)s->tune_ms.size() ? s->tune_ms[i] : 0.0;
+    });
+}
+
 int mgx_solver_release(mgx_solver s)
 {
     return guard([&] {
@@ -1335,6 +1760,8 @@ int mgx_solver_release(mgx_solver s)
         if (s->ctl_host) (void)hipHostFree(s->ctl_host);
         if (s->flag_host) (void)hipHostFree(s->flag_host);
         if (s->ctl_in_host) (void)hipHostFree(s->ctl_in_host);
+        if (s->bctl_host) (void)hipHostFree(s->bctl_host);
+        if (s->bgexec) (void)hipGraphExecDestroy(s->bgexec);
         if (s->ev0) (void)hipEventDestroy(s->ev0);
         if (s->ev1) (void)hipEventDestroy(s->ev1);
         for (auto &e : s->prof_events) (void)hipEventDestroy(e);

@@ -1,0 +1,637 @@
+// mgx_block.hip -- kernels of the batching form: K right-hand sides (K in {4, 8, 16, 32})
+// advance together through one PCG, so every pass over a matrix serves K columns.
+//
+// Block vectors are n x K, row-major (element (i, c) at i*K + c): the K values a matrix entry
+// multiplies are one contiguous 8K- (FP64) or 4K-byte (FP32) segment, read by K adjacent lanes.
+//
+//  * k_spmm: a 256-thread block takes a tile of consecutive rows, stages the tile's contiguous
+//    (col, val) range into LDS with coalesced non-temporal loads, and then G = K*S lanes work
+//    on one row: lane (c, s) handles column c and every S-th entry of the row, reading the
+//    entry from LDS (broadcast) and gathering X[col*K + c].  S = 16/K for K < 16 so that a row
+//    is still split over 16 lanes.  The grid is bounded (<= 2048 blocks): each XCD walks a
+//    contiguous eighth of the tiles, so its L2 sees one window of X.  Epilogues: Y = AX,
+//    Y += AX, Y = B - AX, Jacobi Y = X + pre (B - AX), and Y = AX with per-column X.Y partials.
+//  * k_gs_block: multicolour Gauss-Seidel over 3-row node blocks for K columns at once; the
+//    node's 3x3 diagonal block is common to all columns.
+//  * Vector kernels (norms, fused x/r update, r.z, p update) give every thread a fixed column
+//    (block size and grid stride are multiples of K); per-column partial sums go to
+//    partials[(slot*K + c)*stride + block] and k_bfinalize (one block) combines them in a
+//    fixed order and updates the per-column PCG scalars of a BCtl.  Converged columns are
+//    frozen (alpha = beta = 0) while the others continue; every kernel returns at once when
+//    all columns are done.
+#include "mgx_device.h"
+
+namespace mgx {
+
+template <int K> struct BShape {
+    static constexpr int S = K >= 16 ? 1 : 16 / K;   // lanes sharing one row for one column
+    static constexpr int G = K * S;                  // lanes per row
+    static constexpr int RPB = kBlock / G;           // rows per tile
+    static constexpr int CAP = RPB * 96;             // staged entries per tile
+    static constexpr int U = S == 1 ? 8 : 4;         // entries in flight per lane
+};
+
+static inline int grid_tiles(int64_t ntiles)
+{
+    int64_t nb = std::min<int64_t>(std::max<int64_t>(ntiles, 1), 2048);
+    nb = (nb + 7) & ~int64_t(7);
+    return (int)nb;
+}
+
+// Per-column block reduction.  Thread t holds values of column t % K (blockDim = 256); the
+// K column totals of slot j go to partials[(j*K + c)*stride + blockIdx.x].  Slot MAXSLOT is a
+// NaN-propagating max, the others sums.  Fixed order: deterministic.
+template <int K, int NS, int MAXSLOT>
+__device__ __forceinline__ void publish_cols(double v0, double v1, double v2, double *__restrict__ partials, int stride)
+{
+    __shared__ double red[NS][kBlock];
+    const int t = threadIdx.x;
+    __syncthreads();
+    red[0][t] = v0;
+    if constexpr (NS > 1) red[1][t] = v1;
+    if constexpr (NS > 2) red[2][t] = v2;
+    __syncthreads();
+    if (t < K * NS) {
+        const int slot = t / K, c = t % K;
+        double a = 0;
+        for (int j = c; j < kBlock; j += K) {
+            const double x = red[slot][j];
+            a = (slot == MAXSLOT) ? nanmax(a, x) : a + x;
+        }
+        partials[(size_t)(slot * K + c) * stride + blockIdx.x] = a;
+    }
+}
+
+// ------------------------------------------------------------------ SpMM
+// One lane's share of row [s, e) for column c: entries s + sl, s + sl + S, ...
+template <int K, typename TV, typename TX, typename PI, typename PV>
+__device__ __forceinline__ TV row_cols(PI cis, PV vs, int s, int e, const TX *__restrict__ X, int c, int sl)
+{
+    using Sh = BShape<K>;
+    TV a0 = 0, a1 = 0;
+    for (int k = s + sl; k < e; k += Sh::U * Sh::S) {
+        int cc[Sh::U];
+        TV w[Sh::U];
+#pragma unroll
+        for (int u = 0; u < Sh::U; ++u) {
+            const int kk = k + u * Sh::S;
+            if (kk < e) { cc[u] = cis[kk]; w[u] = vs[kk]; } else { cc[u] = -1; w[u] = 0; }
+        }
+        TX xv[Sh::U];
+#pragma unroll
+        for (int u = 0; u < Sh::U; ++u) xv[u] = cc[u] >= 0 ? X[(size_t)cc[u] * K + c] : TX(0);
+#pragma unroll
+        for (int u = 0; u < Sh::U; ++u) { if (u & 1) a1 += w[u] * (TV)xv[u]; else a0 += w[u] * (TV)xv[u]; }
+    }
+    return a0 + a1;
+}
+
+template <typename TV, typename TX, int K, int EPI>
+__global__ __launch_bounds__(kBlock) void k_spmm(int32_t nrows, const int32_t *__restrict__ rp,
+                                                 const int32_t *__restrict__ ci, const TV *__restrict__ v,
+                                                 const TX *__restrict__ X, const TX *__restrict__ B,
+                                                 const TV *__restrict__ pre, TX *__restrict__ Y,
+                                                 const int32_t *__restrict__ done, double *__restrict__ partials,
+                                                 int stride)
+{
+    using Sh = BShape<K>;
+    __shared__ int32_t s_ci[Sh::CAP];
+    __shared__ TV s_v[Sh::CAP];
+    if (done && *done) return;
+    const int t = threadIdx.x, g = t / Sh::G, l = t % Sh::G, c = l % K, sl = l / K;
+    const int ntiles = (nrows + Sh::RPB - 1) / Sh::RPB;
+    const int per = gridDim.x >> 3, xcd = blockIdx.x & 7;
+    const int hi = (int)((int64_t)ntiles * (xcd + 1) / 8);
+    double dot = 0;
+    for (int tile = (int)((int64_t)ntiles * xcd / 8) + (int)(blockIdx.x >> 3); tile < hi; tile += per) {
+        const int r0 = tile * Sh::RPB, r1 = min(r0 + Sh::RPB, nrows);
+        const int e0 = rp[r0], e1 = rp[r1];
+        const bool staged = e1 - e0 <= Sh::CAP;     // the same for the whole block
+        if (staged) {
+            __syncthreads();                        // the previous tile's readers are done
+            for (int k = t; k < e1 - e0; k += kBlock) { s_ci[k] = ntload(ci + e0 + k); s_v[k] = ntload(v + e0 + k); }
+            __syncthreads();
+        }
+        const int row = r0 + g;
+        if (row < r1) {
+            const int s = rp[row], e = rp[row + 1];
+            TV acc = staged ? row_cols<K, TV, TX>(s_ci, s_v, s - e0, e - e0, X, c, sl)
+                            : row_cols<K, TV, TX>(ci, v, s, e, X, c, sl);
+#pragma unroll
+            for (int o = K; o < Sh::G; o <<= 1) acc += __shfl_xor(acc, o, Sh::G);
+            if (sl == 0) {
+                const size_t i = (size_t)row * K + c;
+                if (EPI == BEPI_SPMM) Y[i] = (TX)acc;
+                else if (EPI == BEPI_ADD) Y[i] = (TX)((TV)Y[i] + acc);
+                else if (EPI == BEPI_RESID) Y[i] = (TX)((TV)B[i] - acc);
+                else if (EPI == BEPI_JACOBI) Y[i] = (TX)((TV)X[i] + pre[row] * ((TV)B[i] - acc));
+                else {
+                    const TX q = (TX)acc;
+                    Y[i] = q;
+                    dot += (double)X[i] * (double)q;
+                }
+            }
+        }
+    }
+    if (EPI == BEPI_DOT) publish_cols<K, 1, -1>(dot, 0, 0, partials, stride);
+}
+
+template <typename TV, typename TX>
+void launch_spmm(int K, int epi, int32_t nrows, const int32_t *rp, const int32_t *ci, const TV *v, const TX *X,
+                 const TX *B, const TV *pre, TX *Y, const int32_t *done, double *partials, int stride, hipStream_t s)
+{
+    if (nrows <= 0) return;
+#define MGX_SPMM(KK, EE)                                                                                       \
+    hipLaunchKernelGGL((k_spmm<TV, TX, KK, EE>), dim3(grid_tiles((nrows + BShape<KK>::RPB - 1) / BShape<KK>::RPB)), \
+                       dim3(kBlock), 0, s, nrows, rp, ci, v, X, B, pre, Y, done, partials, stride)
+#define MGX_SPMM_K(KK)                                        \
+    switch (epi) {                                            \
+    case BEPI_SPMM: MGX_SPMM(KK, BEPI_SPMM); break;           \
+    case BEPI_ADD: MGX_SPMM(KK, BEPI_ADD); break;             \
+    case BEPI_RESID: MGX_SPMM(KK, BEPI_RESID); break;         \
+    case BEPI_JACOBI: MGX_SPMM(KK, BEPI_JACOBI); break;       \
+    default: MGX_SPMM(KK, BEPI_DOT); break;                   \
+    }
+    switch (K) {
+    case 4: MGX_SPMM_K(4); break;
+    case 8: MGX_SPMM_K(8); break;
+    case 16: MGX_SPMM_K(16); break;
+    case 32: MGX_SPMM_K(32); break;
+    default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
+    }
+#undef MGX_SPMM_K
+#undef MGX_SPMM
+    MGX_HIP(hipGetLastError());
+}
+
+template void launch_spmm<double, double>(int, int, int32_t, const int32_t *, const int32_t *, const double *,
+                                          const double *, const double *, const double *, double *, const int32_t *,
+                                          double *, int, hipStream_t);
+template void launch_spmm<float, float>(int, int, int32_t, const int32_t *, const int32_t *, const float *,
+                                        const float *, const float *, const float *, float *, const int32_t *,
+                                        double *, int, hipStream_t);
+template void launch_spmm<double, float>(int, int, int32_t, const int32_t *, const int32_t *, const double *,
+                                         const float *, const float *, const double *, float *, const int32_t *,
+                                         double *, int, hipStream_t);
+
+// ------------------------------------------------------------------ multicolour GS, K columns
+// A block of 192 threads relaxes N nodes: thread (m, a, s, c) reduces row a of node m for
+// column c over the entries s, s + S, ...: sa = sum over columns outside the node, and the
+// node's 3x3 diagonal block (the same for every column).  Then one thread per (node, column)
+// relaxes the three dofs in order (reverse order when backward).
+constexpr int kGsbBlock = 192;
+
+template <typename T, int K>
+__global__ __launch_bounds__(kGsbBlock) void k_gs_block(int32_t nnodes, const int32_t *__restrict__ nodes,
+                                                        const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                        const T *__restrict__ v, const T *__restrict__ B,
+                                                        T *__restrict__ X, int backward, const int32_t *__restrict__ done)
+{
+    constexpr int S = BShape<K>::S, G = BShape<K>::G, N = kGsbBlock / (3 * G), U = S == 1 ? 8 : 4;
+    __shared__ T sh[N][3][K];
+    __shared__ T dg[N][3][3];
+    if (done && *done) return;
+    const int base = xcd_remap(blockIdx.x, gridDim.x) * N;
+    const int t = threadIdx.x, l = t % G, c = l % K, sl = l / K, a = (t / G) % 3, m = t / (3 * G);
+    const int nd = base + m;
+    T sa = 0, d0 = 0, d1 = 0, d2 = 0;
+    int r0 = 0;
+    if (nd < nnodes) {
+        r0 = 3 * nodes[nd];
+        const int s = rp[r0 + a], e = rp[r0 + a + 1];
+        for (int k = s + sl; k < e; k += U * S) {
+            int cc[U];
+            T w[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const int kk = k + u * S;
+                if (kk < e) { cc[u] = ci[kk]; w[u] = v[kk]; } else { cc[u] = -1; w[u] = 0; }
+            }
+            T xv[U];
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                const unsigned off = (unsigned)(cc[u] - r0);
+                xv[u] = (cc[u] >= 0 && off >= 3u) ? X[(size_t)cc[u] * K + c] : T(0);
+            }
+#pragma unroll
+            for (int u = 0; u < U; ++u) {
+                if (cc[u] < 0) continue;
+                const unsigned off = (unsigned)(cc[u] - r0);
+                if (off < 3u) { if (off == 0) d0 += w[u]; else if (off == 1) d1 += w[u]; else d2 += w[u]; }
+                else sa += w[u] * xv[u];
+            }
+        }
+    }
+#pragma unroll
+    for (int o = K; o < G; o <<= 1) {
+        sa += __shfl_xor(sa, o, G);
+        d0 += __shfl_xor(d0, o, G);
+        d1 += __shfl_xor(d1, o, G);
+        d2 += __shfl_xor(d2, o, G);
+    }
+    if (nd < nnodes && sl == 0) {
+        sh[m][a][c] = sa;
+        if (c == 0) { dg[m][a][0] = d0; dg[m][a][1] = d1; dg[m][a][2] = d2; }
+    }
+    __syncthreads();
+    if (t < N * K) {
+        const int m2 = t / K, c2 = t % K, nd2 = base + m2;
+        if (nd2 < nnodes) {
+            const size_t q0 = (size_t)3 * nodes[nd2] * K + c2;
+            const T(*D)[3] = dg[m2];
+            T x0 = X[q0], x1 = X[q0 + K], x2 = X[q0 + 2 * K];
+            const T b0 = B[q0] - sh[m2][0][c2], b1 = B[q0 + K] - sh[m2][1][c2], b2 = B[q0 + 2 * K] - sh[m2][2][c2];
+            if (!backward) {
+                x0 = (b0 - D[0][1] * x1 - D[0][2] * x2) / D[0][0];
+                x1 = (b1 - D[1][0] * x0 - D[1][2] * x2) / D[1][1];
+                x2 = (b2 - D[2][0] * x0 - D[2][1] * x1) / D[2][2];
+            } else {
+                x2 = (b2 - D[2][0] * x0 - D[2][1] * x1) / D[2][2];
+                x1 = (b1 - D[1][0] * x0 - D[1][2] * x2) / D[1][1];
+                x0 = (b0 - D[0][1] * x1 - D[0][2] * x2) / D[0][0];
+            }
+            X[q0] = x0; X[q0 + K] = x1; X[q0 + 2 * K] = x2;
+        }
+    }
+}
+
+template <typename T>
+void launch_gs_block(int K, int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const T *v,
+                     const T *B, T *X, bool backward, const int32_t *done, hipStream_t s)
+{
+    if (nnodes <= 0) return;
+#define MGX_GSB(KK)                                                                                          \
+    hipLaunchKernelGGL((k_gs_block<T, KK>), dim3(grid_rows(nnodes, kGsbBlock / (3 * BShape<KK>::G))),       \
+                       dim3(kGsbBlock), 0, s, nnodes, nodes, rp, ci, v, B, X, backward ? 1 : 0, done)
+    switch (K) {
+    case 4: MGX_GSB(4); break;
+    case 8: MGX_GSB(8); break;
+    case 16: MGX_GSB(16); break;
+    case 32: MGX_GSB(32); break;
+    default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
+    }
+#undef MGX_GSB
+    MGX_HIP(hipGetLastError());
+}
+template void launch_gs_block<double>(int, int32_t, const int32_t *, const int32_t *, const int32_t *, const double *,
+                                      const double *, double *, bool, const int32_t *, hipStream_t);
+template void launch_gs_block<float>(int, int32_t, const int32_t *, const int32_t *, const int32_t *, const float *,
+                                     const float *, float *, bool, const int32_t *, hipStream_t);
+
+// ------------------------------------------------------------------ block vector kernels
+// Every thread keeps one column: blockDim (256) and the grid stride are multiples of K.
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_bnorms(int64_t nk, const T *__restrict__ B, const T *__restrict__ R,
+                                                   double *__restrict__ partials, int stride)
+{
+    double bb = 0, rr = 0, m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nk; i += (int64_t)gridDim.x * kBlock) {
+        const double b = B[i], r = R[i];
+        bb += b * b;
+        rr += r * r;
+        m = nanmax(m, fabs(r));
+    }
+    publish_cols<K, 3, 2>(bb, rr, m, partials, stride);
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_bdot(int64_t nk, const T *__restrict__ A, const T *__restrict__ Bv,
+                                                 const int32_t *__restrict__ done, double *__restrict__ partials,
+                                                 int stride)
+{
+    if (*done) return;
+    double d = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nk; i += (int64_t)gridDim.x * kBlock)
+        d += (double)A[i] * (double)Bv[i];
+    publish_cols<K, 1, -1>(d, 0, 0, partials, stride);
+}
+
+// x += alpha_c p, r -= alpha_c q, with per-column ||r||^2 and max |r|
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_bupdate(int64_t nk, const T *__restrict__ P, const T *__restrict__ Q,
+                                                    T *__restrict__ X, T *__restrict__ R, const BCtl *__restrict__ ctl,
+                                                    double *__restrict__ partials, int stride)
+{
+    if (ctl->done) return;
+    const double a = ctl->alpha[threadIdx.x % K];
+    double rr = 0, m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nk; i += (int64_t)gridDim.x * kBlock) {
+        const T r = (T)((double)R[i] - a * (double)Q[i]);
+        X[i] = (T)((double)X[i] + a * (double)P[i]);
+        R[i] = r;
+        rr += (double)r * (double)r;
+        m = nanmax(m, fabs((double)r));
+    }
+    publish_cols<K, 2, 1>(rr, m, 0, partials, stride);
+}
+
+// p = z + beta_c p (p = z on the first iteration)
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_bupdate_p(int64_t nk, const T *__restrict__ Z, T *__restrict__ P,
+                                                      const BCtl *__restrict__ ctl, int first)
+{
+    if (ctl->done) return;
+    const double b = first ? 0.0 : ctl->beta[threadIdx.x % K];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nk; i += (int64_t)gridDim.x * kBlock)
+        P[i] = first ? Z[i] : (T)((double)Z[i] + b * (double)P[i]);
+}
+
+// x = pre_row * b
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_bjacobi_initial(int64_t nk, const T *__restrict__ pre,
+                                                            const T *__restrict__ B, T *__restrict__ X,
+                                                            const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nk; i += (int64_t)gridDim.x * kBlock)
+        X[i] = pre[i / K] * B[i];
+}
+
+// Per-column scalars from the partials of the producing kernel (one block, 16 waves; wave w
+// handles columns w, w + 16).
+__global__ __launch_bounds__(1024) void k_bfinalize(int nparts, const double *__restrict__ partials, int stride,
+                                                    BCtl *ctl, int mode)
+{
+    __shared__ int s_done[kMaxBlock];
+    if (mode != FIN_NORMS && ctl->done) return;
+    const int K = ctl->k;
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    for (int c = w; c < K; c += 16) {
+        double a = 0, b = 0, m = 0;
+        for (int i = lane; i < nparts; i += 64) {
+            a += partials[(size_t)c * stride + i];
+            if (mode == FIN_UPDATE) m = nanmax(m, partials[(size_t)(K + c) * stride + i]);
+            if (mode == FIN_NORMS) {
+                b += partials[(size_t)(K + c) * stride + i];
+                m = nanmax(m, partials[(size_t)(2 * K + c) * stride + i]);
+            }
+        }
+        a = wave_sum(a); b = wave_sum(b); m = wave_max(m);
+        if (lane == 0) {
+            int cd = ctl->cdone[c];
+            switch (mode) {
+            case FIN_ALPHA: ctl->pq[c] = a; ctl->alpha[c] = cd ? 0.0 : ctl->rz[c] / a; break;
+            case FIN_BETA_FIRST: ctl->rz[c] = a; break;
+            case FIN_BETA: ctl->beta[c] = cd ? 0.0 : a / ctl->rz[c]; ctl->rz[c] = a; break;
+            case FIN_UPDATE:
+                if (!cd) {
+                    // a = ||r||^2, m = max |r|
+                    ctl->rnorm2[c] = a;
+                    ctl->rmax[c] = m;
+                    const int it = ctl->iter[c] + 1;
+                    ctl->iter[c] = it;
+                    if (!(m == m) || m > 1e50 || !(a == a)) { cd = 1; ctl->status[c] = -1; }
+                    else if (ctl->stop == MGX_STOP_REL_L2 ? (a <= ctl->tol * ctl->tol * ctl->bnorm2[c]) : (m <= ctl->tol)) {
+                        cd = 1; ctl->status[c] = 1;
+                    } else if (it >= ctl->max_it) cd = 1;
+                }
+                break;
+            case FIN_NORMS: {
+                // a = ||b||^2, b = ||r||^2, m = max |r|
+                ctl->bnorm2[c] = a;
+                ctl->rnorm2[c] = b;
+                ctl->rmax[c] = m;
+                ctl->iter[c] = 0;
+                int st = 0;
+                cd = 0;
+                if (!(m == m) || m > 1e50) { cd = 1; st = -1; }
+                else if (ctl->stop == MGX_STOP_REL_L2 ? (b <= ctl->tol * ctl->tol * a) : (m <= ctl->tol)) { cd = 1; st = 1; }
+                else if (ctl->frozen[c] || ctl->max_it <= 0) cd = 1;
+                ctl->status[c] = st;
+                break;
+            }
+            default: break;
+            }
+            ctl->cdone[c] = cd;
+            s_done[c] = cd;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        int all = 1;
+        for (int c = 0; c < K; ++c) all &= s_done[c];
+        ctl->done = all;
+    }
+}
+
+void launch_bfinalize(int nparts, const double *partials, BCtl *ctl, int mode, hipStream_t s)
+{
+    hipLaunchKernelGGL(k_bfinalize, dim3(1), dim3(1024), 0, s, nparts, partials, kPartialStride, ctl, mode);
+    MGX_HIP(hipGetLastError());
+}
+
+static inline int grid_cols(int64_t nk) { return grid_stream(nk); }
+
+#define MGX_K_SWITCH(K, LAUNCH)                                                          \
+    switch (K) {                                                                         \
+    case 4: LAUNCH(4); break;                                                            \
+    case 8: LAUNCH(8); break;                                                            \
+    case 16: LAUNCH(16); break;                                                          \
+    case 32: LAUNCH(32); break;                                                          \
+    default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"}; \
+    }
+
+template <typename T>
+void launch_bnorms(int K, int32_t n, const T *B, const T *R, BCtl *ctl, double *partials, hipStream_t s)
+{
+    const int64_t nk = (int64_t)n * K;
+    const int g = grid_cols(nk);
+#define L_(KK) hipLaunchKernelGGL((k_bnorms<T, KK>), dim3(g), dim3(kBlock), 0, s, nk, B, R, partials, kPartialStride)
+    MGX_K_SWITCH(K, L_)
+#undef L_
+    MGX_HIP(hipGetLastError());
+    launch_bfinalize(g, partials, ctl, FIN_NORMS, s);
+}
+
+template <typename T>
+void launch_bdot_beta(int K, int32_t n, const T *R, const T *Z, BCtl *ctl, double *partials, bool first, hipStream_t s)
+{
+    const int64_t nk = (int64_t)n * K;
+    const int g = grid_cols(nk);
+#define L_(KK) hipLaunchKernelGGL((k_bdot<T, KK>), dim3(g), dim3(kBlock), 0, s, nk, R, Z, &ctl->done, partials, kPartialStride)
+    MGX_K_SWITCH(K, L_)
+#undef L_
+    MGX_HIP(hipGetLastError());
+    launch_bfinalize(g, partials, ctl, first ? FIN_BETA_FIRST : FIN_BETA, s);
+}
+
+template <typename T>
+void launch_bupdate(int K, int32_t n, const T *P, const T *Q, T *X, T *R, BCtl *ctl, double *partials, hipStream_t s)
+{
+    const int64_t nk = (int64_t)n * K;
+    const int g = grid_cols(nk);
+#define L_(KK) hipLaunchKernelGGL((k_bupdate<T, KK>), dim3(g), dim3(kBlock), 0, s, nk, P, Q, X, R, ctl, partials, kPartialStride)
+    MGX_K_SWITCH(K, L_)
+#undef L_
+    MGX_HIP(hipGetLastError());
+    launch_bfinalize(g, partials, ctl, FIN_UPDATE, s);
+}
+
+template <typename T>
+void launch_bupdate_p(int K, int32_t n, const T *Z, T *P, const BCtl *ctl, bool first, hipStream_t s)
+{
+    const int64_t nk = (int64_t)n * K;
+    const int g = grid_cols(nk);
+#define L_(KK) hipLaunchKernelGGL((k_bupdate_p<T, KK>), dim3(g), dim3(kBlock), 0, s, nk, Z, P, ctl, first ? 1 : 0)
+    MGX_K_SWITCH(K, L_)
+#undef L_
+    MGX_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_bjacobi_initial(int K, int32_t n, const T *pre, const T *B, T *X, const int32_t *done, hipStream_t s)
+{
+    const int64_t nk = (int64_t)n * K;
+    const int g = grid_cols(nk);
+#define L_(KK) hipLaunchKernelGGL((k_bjacobi_initial<T, KK>), dim3(g), dim3(kBlock), 0, s, nk, pre, B, X, done)
+    MGX_K_SWITCH(K, L_)
+#undef L_
+    MGX_HIP(hipGetLastError());
+}
+
+int spmm_grid(int K, int32_t nrows)
+{
+    const int rpb = K == 32 ? BShape<32>::RPB : BShape<16>::RPB;   // 8 or 16 rows per tile
+    return grid_tiles((nrows + rpb - 1) / rpb);
+}
+
+template <typename TV, typename TX>
+void launch_spmm_dot_alpha(int K, int32_t n, const int32_t *rp, const int32_t *ci, const TV *v, const TX *P, TX *Q,
+                           BCtl *ctl, double *partials, hipStream_t s)
+{
+    launch_spmm<TV, TX>(K, BEPI_DOT, n, rp, ci, v, P, nullptr, nullptr, Q, &ctl->done, partials, kPartialStride, s);
+    launch_bfinalize(spmm_grid(K, n), partials, ctl, FIN_ALPHA, s);
+}
+
+#define MGX_INST_T(T)                                                                                             \
+    template void launch_bnorms<T>(int, int32_t, const T *, const T *, BCtl *, double *, hipStream_t);             \
+    template void launch_bdot_beta<T>(int, int32_t, const T *, const T *, BCtl *, double *, bool, hipStream_t);    \
+    template void launch_bupdate<T>(int, int32_t, const T *, const T *, T *, T *, BCtl *, double *, hipStream_t);  \
+    template void launch_bupdate_p<T>(int, int32_t, const T *, T *, const BCtl *, bool, hipStream_t);              \
+    template void launch_bjacobi_initial<T>(int, int32_t, const T *, const T *, T *, const int32_t *, hipStream_t); \
+    template void launch_spmm_dot_alpha<T, T>(int, int32_t, const int32_t *, const int32_t *, const T *, const T *, \
+                                              T *, BCtl *, double *, hipStream_t);
+MGX_INST_T(double)
+MGX_INST_T(float)
+#undef MGX_INST_T
+
+// ------------------------------------------------------------------ dense coarse solve, K columns
+// Y = M X with M the dense FP64 inverse of the coarsest matrix (nc x nc, row-major) and X, Y
+// nc x K blocks.  A block takes kDenseRows rows; thread (slice, c) accumulates column c over the
+// inner indices j = slice, slice + J, ... for all of the block's rows (one X load feeds
+// kDenseRows FMAs), and the J slices are summed in a fixed order through LDS.
+constexpr int kDenseRows = 8;
+
+template <typename TX, int K>
+__global__ __launch_bounds__(kBlock) void k_dense_block(int32_t nc, const double *__restrict__ M,
+                                                        const TX *__restrict__ X, TX *__restrict__ Y,
+                                                        const int32_t *__restrict__ done)
+{
+    constexpr int J = kBlock / K;
+    __shared__ double red[kDenseRows][J][K];
+    if (done && *done) return;
+    const int r0 = blockIdx.x * kDenseRows, t = threadIdx.x, c = t % K, sl = t / K;
+    const int nr = min(kDenseRows, nc - r0);
+    double acc[kDenseRows];
+#pragma unroll
+    for (int r = 0; r < kDenseRows; ++r) acc[r] = 0;
+    for (int j = sl; j < nc; j += J) {
+        const double xv = (double)X[(size_t)j * K + c];
+#pragma unroll
+        for (int r = 0; r < kDenseRows; ++r)
+            if (r < nr) acc[r] += M[(size_t)(r0 + r) * nc + j] * xv;
+    }
+#pragma unroll
+    for (int r = 0; r < kDenseRows; ++r) red[r][sl][c] = acc[r];
+    __syncthreads();
+    if (t < kDenseRows * K) {
+        const int r = t / K, cc = t % K;
+        if (r < nr) {
+            double a = 0;
+            for (int q = 0; q < J; ++q) a += red[r][q][cc];
+            Y[(size_t)(r0 + r) * K + cc] = (TX)a;
+        }
+    }
+}
+
+template <typename TX>
+void launch_dense_block(int K, int32_t nc, const double *M, const TX *X, TX *Y, const int32_t *done, hipStream_t s)
+{
+    const int nb = (nc + kDenseRows - 1) / kDenseRows;
+#define L_(KK) hipLaunchKernelGGL((k_dense_block<TX, KK>), dim3(nb), dim3(kBlock), 0, s, nc, M, X, Y, done)
+    MGX_K_SWITCH(K, L_)
+#undef L_
+    MGX_HIP(hipGetLastError());
+}
+template void launch_dense_block<double>(int, int32_t, const double *, const double *, double *, const int32_t *,
+                                         hipStream_t);
+template void launch_dense_block<float>(int, int32_t, const double *, const float *, float *, const int32_t *,
+                                        hipStream_t);
+
+// ------------------------------------------------------------------ column-major <-> block layout
+// dst[i*K + c] = src[c*ld + i] for c < kc, 0 for kc <= c < K.  A 64-row tile goes through LDS so
+// that both the column reads and the row-major writes are coalesced.
+template <typename TS, typename TD>
+__global__ __launch_bounds__(kBlock) void k_pack(int32_t n, int K, int kc, const TS *__restrict__ src, int64_t ld,
+                                                 TD *__restrict__ dst)
+{
+    __shared__ TD tile[kMaxBlock][65];
+    const int i0 = blockIdx.x * 64, t = threadIdx.x;
+    for (int c = t / 64; c < K; c += kBlock / 64) {
+        const int i = i0 + (t & 63);
+        tile[c][t & 63] = (c < kc && i < n) ? (TD)src[(size_t)c * ld + i] : TD(0);
+    }
+    __syncthreads();
+    for (int e = t; e < 64 * K; e += kBlock) {
+        const int r = e / K, c = e % K;
+        if (i0 + r < n) dst[(size_t)(i0 + r) * K + c] = tile[c][r];
+    }
+}
+
+// dst[c*ld + i] = src[i*K + c] for c < kc
+template <typename TS, typename TD>
+__global__ __launch_bounds__(kBlock) void k_unpack(int32_t n, int K, int kc, const TS *__restrict__ src,
+                                                   TD *__restrict__ dst, int64_t ld)
+{
+    __shared__ TS tile[kMaxBlock][65];
+    const int i0 = blockIdx.x * 64, t = threadIdx.x;
+    for (int e = t; e < 64 * K; e += kBlock) {
+        const int r = e / K, c = e % K;
+        tile[c][r] = (i0 + r < n) ? src[(size_t)(i0 + r) * K + c] : TS(0);
+    }
+    __syncthreads();
+    for (int c = t / 64; c < kc; c += kBlock / 64) {
+        const int i = i0 + (t & 63);
+        if (i < n) dst[(size_t)c * ld + i] = (TD)tile[c][t & 63];
+    }
+}
+
+template <typename T>
+__global__ __launch_bounds__(kBlock) void k_bcopy(int64_t n, const T *__restrict__ src, T *__restrict__ dst,
+                                                  const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) dst[i] = src[i];
+}
+template <typename T>
+void launch_bcopy(int64_t n, const T *src, T *dst, const int32_t *done, hipStream_t s)
+{
+    if (n <= 0) return;
+    hipLaunchKernelGGL((k_bcopy<T>), dim3(grid_stream(n)), dim3(kBlock), 0, s, n, src, dst, done);
+    MGX_HIP(hipGetLastError());
+}
+template void launch_bcopy<double>(int64_t, const double *, double *, const int32_t *, hipStream_t);
+template void launch_bcopy<float>(int64_t, const float *, float *, const int32_t *, hipStream_t);
+
+void launch_pack_cols(int32_t n, int K, int kc, const double *src, int64_t ld, double *dst, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_pack<double, double>), dim3((n + 63) / 64), dim3(kBlock), 0, s, n, K, kc, src, ld, dst);
+    MGX_HIP(hipGetLastError());
+}
+void launch_unpack_cols(int32_t n, int K, int kc, const double *src, double *dst, int64_t ld, hipStream_t s)
+{
+    hipLaunchKernelGGL((k_unpack<double, double>), dim3((n + 63) / 64), dim3(kBlock), 0, s, n, K, kc, src, dst, ld);
+    MGX_HIP(hipGetLastError());
+}
+
+}  // namespace mgx

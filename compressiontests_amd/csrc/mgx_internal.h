@@ -109,6 +109,51 @@ void launch_fill_zero_f32(int64_t n, float *a, const int32_t *done, hipStream_t 
 void launch_fill_zero(int64_t n, double *a, const int32_t *done, hipStream_t s);
 void launch_axpy_f32_into_f64(int64_t n, const float *a, double *x, const int32_t *done, hipStream_t s);
 
+// batching form: K right-hand sides as n x K row-major block vectors (mgx_block.hip)
+constexpr int kMaxBlock = 32;
+// per-column PCG scalars of a block solve; columns that are done are frozen (alpha = beta = 0)
+struct BCtl {
+    double rz[kMaxBlock], pq[kMaxBlock], alpha[kMaxBlock], beta[kMaxBlock];
+    double rnorm2[kMaxBlock], rmax[kMaxBlock], bnorm2[kMaxBlock];
+    int32_t iter[kMaxBlock], cdone[kMaxBlock], status[kMaxBlock];
+    int32_t frozen[kMaxBlock];   // set by the host: the column takes no part in this solve
+    int32_t done;                // every column done: kernels of later iterations return
+    int32_t k;                   // block width
+    int32_t stop, max_it;
+    double tol;
+};
+enum BlockEpi : int { BEPI_SPMM = 0, BEPI_ADD = 1, BEPI_RESID = 2, BEPI_JACOBI = 3, BEPI_DOT = 4 };
+// partials of the block reductions: 3 slots x kMaxBlock columns x kPartialStride
+constexpr size_t kBlockPartials = (size_t)3 * kMaxBlock * kPartialStride;
+template <typename TV, typename TX>
+void launch_spmm(int K, int epi, int32_t nrows, const int32_t *rp, const int32_t *ci, const TV *v, const TX *X,
+                 const TX *B, const TV *pre, TX *Y, const int32_t *done, double *partials, int stride, hipStream_t s);
+template <typename T>
+void launch_gs_block(int K, int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const T *v,
+                     const T *B, T *X, bool backward, const int32_t *done, hipStream_t s);
+template <typename T>
+void launch_bnorms(int K, int32_t n, const T *B, const T *R, BCtl *ctl, double *partials, hipStream_t s);
+template <typename T>
+void launch_bdot_beta(int K, int32_t n, const T *R, const T *Z, BCtl *ctl, double *partials, bool first, hipStream_t s);
+template <typename T>
+void launch_bupdate(int K, int32_t n, const T *P, const T *Q, T *X, T *R, BCtl *ctl, double *partials, hipStream_t s);
+template <typename T>
+void launch_bupdate_p(int K, int32_t n, const T *Z, T *P, const BCtl *ctl, bool first, hipStream_t s);
+template <typename T>
+void launch_bjacobi_initial(int K, int32_t n, const T *pre, const T *B, T *X, const int32_t *done, hipStream_t s);
+// one-block combine of per-column partials (FinMode) and the partial count of a k_spmm launch
+void launch_bfinalize(int nparts, const double *partials, BCtl *ctl, int mode, hipStream_t s);
+int spmm_grid(int K, int32_t nrows);
+template <typename TV, typename TX>
+void launch_spmm_dot_alpha(int K, int32_t n, const int32_t *rp, const int32_t *ci, const TV *v, const TX *P, TX *Q,
+                           BCtl *ctl, double *partials, hipStream_t s);
+template <typename TX>
+void launch_dense_block(int K, int32_t nc, const double *M, const TX *X, TX *Y, const int32_t *done, hipStream_t s);
+template <typename T>
+void launch_bcopy(int64_t n, const T *src, T *dst, const int32_t *done, hipStream_t s);
+void launch_pack_cols(int32_t n, int K, int kc, const double *src, int64_t ld, double *dst, hipStream_t s);
+void launch_unpack_cols(int32_t n, int K, int kc, const double *src, double *dst, int64_t ld, hipStream_t s);
+
 // structured assembly
 void launch_assemble_count(int nx, int ny, int nz, int32_t *rowlen, hipStream_t s);
 void launch_assemble_fill(int nx, int ny, int nz, const double *Ke, const double *dens, const double *Kel,

@@ -8,69 +8,14 @@
 //  * blockIdx is remapped so that each of the 8 XCDs walks one contiguous slab of rows:
 //    consecutive blocks are dispatched round-robin over the XCDs, and without the remap every
 //    XCD's L2 would fetch the same x window.
-//  * Grid reductions are deterministic: per-block partials, then the last block (agent-scope
-//    acq_rel ticket) sums them in a fixed order and resets the ticket.
+//  * Grid reductions are deterministic and fence-free: every block stores its partial with
+//    a plain store, and a one-block finalize kernel (the next launch on the stream) sums the
+//    partials in a fixed order.
 //  * Kernels of the PCG loop take `done` (device flag): once the stop test has fired they
 //    return at once, so the host can queue several iterations per synchronisation.
-#include "mgx_internal.h"
+#include "mgx_device.h"
 
 namespace mgx {
-
-// ------------------------------------------------------------------ helpers
-__device__ __forceinline__ int xcd_remap(int b, int nb)
-{
-    // nb is a multiple of 8 for every launch that uses this (see grid_rows)
-    const int per = nb >> 3;
-    return (b & 7) * per + (b >> 3);
-}
-
-static inline int grid_rows(int64_t nrows, int rows_per_block)
-{
-    int64_t nb = (nrows + rows_per_block - 1) / rows_per_block;
-    if (nb < 1) nb = 1;
-    nb = (nb + 7) & ~int64_t(7);
-    return (int)nb;
-}
-
-template <typename T>
-__device__ __forceinline__ T ntload(const T *p) { return __builtin_nontemporal_load(p); }
-
-__device__ __forceinline__ double nanmax(double a, double b)
-{
-    return (a != a) ? a : ((b != b) ? b : (a > b ? a : b));
-}
-
-__device__ __forceinline__ double wave_sum(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ double wave_max(double v)
-{
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v = nanmax(v, __shfl_xor(v, o, 64));
-    return v;
-}
-
-// Block reduction of (sum a, sum b, max m) for blocks of up to 1024 threads; totals valid in
-// thread 0.  Fixed tree order: deterministic.
-__device__ __forceinline__ void block_reduce3(double &a, double &b, double &m)
-{
-    __shared__ double sh[3][16];
-    a = wave_sum(a); b = wave_sum(b); m = wave_max(m);
-    const int w = threadIdx.x >> 6, l = threadIdx.x & 63;
-    __syncthreads();
-    if (l == 0) { sh[0][w] = a; sh[1][w] = b; sh[2][w] = m; }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        const int nw = blockDim.x >> 6;
-        double sa = 0, sb = 0, sm = 0;
-        for (int i = 0; i < nw; ++i) { sa += sh[0][i]; sb += sh[1][i]; sm = nanmax(sm, sh[2][i]); }
-        a = sa; b = sb; m = sm;
-    }
-}
 
 // Two-phase grid reductions.  Phase 1: every block of the producing kernel reduces its
 // contributions and thread 0 stores them with plain stores into partials[slot * stride + block].
@@ -256,13 +201,6 @@ void launch_residual_flag(int32_t n, const int32_t *rp, const int32_t *ci, const
 }
 
 // ------------------------------------------------------------------ streaming vector kernels
-static inline int grid_stream(int64_t n)
-{
-    int64_t nb = (n + kBlock * 4 - 1) / (kBlock * 4);
-    if (nb < 1) nb = 1;
-    if (nb > 2048) nb = 2048;
-    return (int)nb;
-}
 
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_jacobi_initial(int32_t n, const T *__restrict__ pre, const T *__restrict__ b,

@@ -8,7 +8,7 @@ VTK hexahedron order; 3 dofs (ux, uy, uz) per node; nodes with ix == 0 clamped
 """
 import numpy as np
 
-from .device import DeviceArray, cantilever_matrix, transfer_reference
+from .device import DeviceArray, cantilever_matrix, rng_normal, transfer_reference
 from . import kernels
 
 
@@ -73,6 +73,18 @@ class FemCantilever3D:
         f = np.zeros((nz + 1, ny + 1, nx, 3))
         f[:, :, nx - 1, 2] = self.TipLoad / ((ny + 1) * (nz + 1))
         return f.reshape(-1)
+
+    def load_cases(self, nrhs, seed):
+        """nrhs load cases, (nrhs, NumDofsFree): iid N(0,1) force components on every node of the
+        top face z = Lz (all three directions; x = 0 nodes are clamped), zero elsewhere, each case
+        scaled to unit 2-norm.  The normals come from one stream of `seed`, case after case, node
+        order within a case."""
+        nx, ny, nz = self.NumElements
+        g = rng_normal(seed, nrhs * (ny + 1) * nx * 3).reshape(nrhs, ny + 1, nx, 3)
+        f = np.zeros((nrhs, nz + 1, ny + 1, nx, 3))
+        f[:, nz] = g
+        f = f.reshape(nrhs, -1)
+        return f / np.linalg.norm(f, axis=1, keepdims=True)
 
     def rhs(self):
         if self.rhs_kind == "tip":

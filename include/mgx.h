@@ -201,7 +201,7 @@ typedef struct {
     int32_t depth;              /* GMG: levelDown = [T x depth, F x depth] (CreateDeepV) */
     int32_t level_iterations;   /* smoother sweeps per step (GMG) / pre and post sweeps (MG-PCG) */
     int32_t precision;
-    int32_t nrhs;               /* right-hand sides solved together (batched block-PCG) */
+    int32_t nrhs;               /* right-hand sides (load cases), columns of b and x */
     double  duvi_tolerance;     /* value dedup tolerance (1e-10 in the reference) */
     int32_t duvi_mode;          /* see mgx_duvi_encode */
     int32_t inner_iterations;   /* mixed precision: max inner PCG iterations per refinement step */
@@ -209,6 +209,10 @@ typedef struct {
     int32_t profile;            /* 1: time every level-0 A*p SpMV with HIP events (mgx_stats.spmv_ms) */
     int32_t check_every;        /* PCG iterations queued per host check of the device stop flag (default 4) */
     int32_t use_graph;          /* 1: replay the queued PCG iterations as one captured hipGraph (default) */
+    int32_t block_width;        /* MG-PCG on CSR with nrhs > 1: right-hand sides advanced together by one
+                                   batched PCG (its SpMM reads each matrix once per block); 4, 8, 16 or 32,
+                                   1 = one right-hand side at a time, 0 = chosen at initialize by timing
+                                   PCG iterations of each width on the first right-hand sides (default) */
 } mgx_options;
 
 typedef struct {
@@ -221,6 +225,8 @@ typedef struct {
     int32_t outer_iterations;/* mixed precision: refinement steps */
     int32_t spmv_launches;   /* level-0 A*p SpMV launches timed (options.profile) */
     double  spmv_ms;         /* their summed device time (HIP events on the solver stream) */
+    int32_t block_width;     /* right-hand sides solved together with this one (1: alone); with
+                                block_width > 1, solve_ms is the device time of that whole block */
 } mgx_stats;
 
 typedef struct mgx_solver_s *mgx_solver;
@@ -252,6 +258,10 @@ int mgx_solver_initialize(mgx_solver s);
 int mgx_solver_solve(mgx_solver s, double *x, int32_t has_x0, mgx_stats *stats);
 /* Same with device-resident b and x (n x nrhs each, column-major); no host transfer. */
 int mgx_solver_solve_device(mgx_solver s, const double *b_dev, double *x_dev, int32_t has_x0, mgx_stats *stats);
+/* Block width chosen at initialize (1 = one right-hand side at a time) and, when the options
+ * asked for the automatic choice, the timed ms per block PCG iteration for K = 4, 8, 16, 32
+ * (0 where not timed); tune_ms may be NULL.  No reference counterpart (batching form). */
+int mgx_solver_block_width(mgx_solver s, int32_t *width, double *tune_ms);
 /* Replaces ReleaseOpenCLResources (OpenCLCsrGeometricMultigridSolver.cs:106-138). */
 int mgx_solver_release(mgx_solver s);
 
