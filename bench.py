@@ -1,20 +1,29 @@
-"""Benchmark of the MG-PCG / SpMV hot path on one MI355X.
+"""Benchmark of the MG-PCG hot path on one MI355X.
 
-One step = one full FP64 MG-PCG solve of the 64^3 hexahedral cantilever (811,200 free dofs,
-63.7M nonzeros; element moduli log-uniform in [1e-2, 1e2]; x = 0 clamped; total load -1 in z
-on the x = 1 face) to relative residual `--tol`, starting from x = 0, with the matrix, the
-multigrid hierarchy and the right-hand side resident in HBM.  The timed region covers
-exactly K solves.
+Default workload (the headline): the 128^3 hexahedral cantilever (6,390,144 free dofs,
+509,597,550 nonzeros; element moduli log-normal, sigma 1.5, clipped to a 1e3 contrast; x = 0
+clamped) with 32 load cases (iid N(0,1) forces on the top-face nodes, each case of unit norm),
+solved by the batching + precision form: the 32 right-hand sides advance together through one
+block PCG (block width chosen by the library's timing at initialize) whose 5-level Galerkin
+V-cycle (multicolour Gauss-Seidel, one forward / one backward sweep), SpMM and vectors are FP32,
+inside FP64 iterative refinement (FP64 true residual, dots, alpha/beta and coarse solve), to
+relative residual 1e-8 on every FP64 true residual.  The matrices, hierarchy and right-hand
+sides are resident in HBM; one step = one solve of all 32 load cases; the timed region covers
+exactly K steps.
 
-Printed JSON line: value = solves/s; roofline of the dominant kernel (the level-0 CSR
-SpMV p -> A p, timed with HIP events on the solver's stream during the timed solves);
-cpu_baseline = the oracle's single-core C restatement of the same FP64 solve (a bounded
-sample of its iterations, scaled to the GPU solve's iteration count).
+Printed JSON line: value = solves/s (load cases solved per second); config carries the plain
+FP64 GPU path (one right-hand side at a time) timed on the same system, the energy-norm error
+of the batched solutions against FP64 solves to 1e-12, and the autotuner's timings; roofline of
+the dominant kernel (the level-0 SpMM q = A p of the block PCG, HIP events on the solver
+stream around each launch of profiled solves of the same batch); cpu_baseline = the oracle's
+single-threaded C MG-PCG on the same hierarchy (a bounded sample of its iterations).
+
+--n 64 --rhs 1 --precision fp64 --moduli loguniform gives the single-right-hand-side FP64
+solve of the 64^3 system (moduli log-uniform in [1e-2, 1e2], tip load).
 """
 import argparse
 import json
 import os
-import subprocess
 import sys
 import time
 
@@ -22,30 +31,51 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-SEED_MODULI = 20261203 + 2          # config 2: log-uniform moduli
+SEED_LOGUNIFORM = 20261203 + 2      # moduli log-uniform in [1e-2, 1e2]
+SEED_LOGNORMAL = 20261203 + 3       # moduli log-normal, sigma 1.5, contrast 1e3
+SEED_LOADS = 20261203 + 4           # top-face load cases
 
 
-def csr_spmv_bytes(nnz, nrows, ncols):
-    """Algorithmic bytes of one FP64 CSR SpMV: values + column indices, row offsets,
-    x read once, y written once."""
-    return 12 * nnz + 4 * (nrows + 1) + 8 * ncols + 8 * nrows
+def csr_spmv_bytes(nnz, nrows, ncols, val_bytes=8):
+    """Algorithmic bytes of one CSR SpMV: values + column indices, row offsets, x read once,
+    y written once."""
+    return (val_bytes + 4) * nnz + 4 * (nrows + 1) + val_bytes * (ncols + nrows)
+
+
+def spmm_bytes(nnz, n, k, val_bytes):
+    """Algorithmic bytes of one CSR SpMM with k right-hand sides: the matrix once, X read once
+    and Y written once (n x k each)."""
+    return (val_bytes + 4) * nnz + 4 * (n + 1) + 2 * val_bytes * k * n
+
+
+def moduli(kind, n):
+    import numpy as np
+    from compressiontests_amd.device import rng_normal, rng_uniform
+    if kind == "loguniform":
+        return 10 ** (4 * rng_uniform(SEED_LOGUNIFORM, n ** 3) - 2)
+    g = np.clip(1.5 * rng_normal(SEED_LOGNORMAL, n ** 3), -0.5 * np.log(1e3), 0.5 * np.log(1e3))
+    return np.exp(g)
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--n", type=int, default=64, help="elements per axis")
-    ap.add_argument("--levels", type=int, default=4)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--n", type=int, default=128, help="elements per axis")
+    ap.add_argument("--levels", type=int, default=0, help="0: coarsest level 8^3")
+    ap.add_argument("--rhs", type=int, default=32, help="load cases (1: the tip load)")
+    ap.add_argument("--moduli", default="lognormal", choices=["lognormal", "loguniform"])
+    ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
+    ap.add_argument("--block-width", type=int, default=0, help="0: chosen by the library's timing")
     ap.add_argument("--smoother", default="gs")
     ap.add_argument("--nu", type=int, default=1)
-    ap.add_argument("--tol", type=float, default=1e-9)
+    ap.add_argument("--tol", type=float, default=1e-8)
     ap.add_argument("--cpu-baseline", type=int, default=1, help="0: skip the CPU leg")
+    ap.add_argument("--fp64-compare", type=int, default=2, help="load cases timed on the plain FP64 path")
+    ap.add_argument("--check", type=int, default=2, help="load cases checked against FP64 1e-12 solves")
     ap.add_argument("--graph", type=int, default=1, help="replay PCG iterations as a captured hipGraph")
     ap.add_argument("--check-every", type=int, default=8)
-    ap.add_argument("--precision", default="fp64", choices=["fp64", "mixed"],
-                    help="mixed: FP32 SpMV / V-cycle inside FP64 iterative refinement")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -54,82 +84,123 @@ def main():
         sys.exit(2)
 
     import numpy as np
-    from compressiontests_amd import _lib
-    from compressiontests_amd.device import DeviceArray, rng_uniform
+    from compressiontests_amd import _lib, kernels
+    from compressiontests_amd.device import DeviceArray
     from compressiontests_amd.model import FemCantilever3D
     from compressiontests_amd.solvers import HipMgPcgSolver
 
     assert _lib.device_count() > 0, "no GPU visible to libmgx.so"
     n = args.n
-    dens = 10 ** (4 * rng_uniform(SEED_MODULI, n ** 3) - 2)
+    levels = args.levels or max(2, int(round(np.log2(n))) - 2)
+    nrhs = args.rhs
+    dens = moduli(args.moduli, n)
     t0 = time.time()
     model = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
-    solver = HipMgPcgSolver(model, levels=args.levels, smoother=args.smoother, nu=args.nu, tolerance=args.tol,
-                            max_iterations=500, use_graph=bool(args.graph), check_every=args.check_every,
-                            precision=args.precision)
-    f = model.rhs()
-    solver.Initialize(f)
-    # the same solve with HIP events around every level-0 A*p SpMV (no graph): the roofline pass
-    prof = HipMgPcgSolver(model, levels=args.levels, smoother=args.smoother, nu=args.nu, tolerance=args.tol,
-                          max_iterations=500, profile=True, check_every=args.check_every)
-    prof.Initialize(f)
+    F = model.rhs()[None, :] if nrhs == 1 else model.load_cases(nrhs, SEED_LOADS)
+    common = dict(levels=levels, smoother=args.smoother, nu=args.nu, check_every=args.check_every)
+    solver = HipMgPcgSolver(model, tolerance=args.tol, max_iterations=500, use_graph=bool(args.graph), nrhs=nrhs,
+                            precision=args.precision, block_width=args.block_width, **common)
+    hier = solver.hierarchy
+    solver.Initialize(F)
+    width, tune_ms = solver.BlockWidth
+    print(f"# setup done in {time.time() - t0:.1f} s, block width {width}", file=sys.stderr, flush=True)
+    # the same solve with HIP events around every level-0 A*p launch (no graph): the roofline pass
+    prof = HipMgPcgSolver(model, tolerance=args.tol, max_iterations=500, profile=True, nrhs=nrhs,
+                          precision=args.precision, block_width=width, hierarchy=hier, **common)
+    prof.Initialize(F)
     setup_s = time.time() - t0
     K = model.device_matrix()
-    nrows, nnz = K.nrows, K.nnz
+    ndof, nnz = K.nrows, K.nnz
 
-    b_dev = DeviceArray.from_numpy(f)
-    x_dev = DeviceArray(nrows)
+    b_dev = DeviceArray.from_numpy(F.reshape(-1))
+    x_dev = DeviceArray(F.size)
     for _ in range(args.warmup):
         solver.solve_device(b_dev, x_dev)
     _lib.call("mgx_device_synchronize")
-    iters, rels, outer = [], [], []
+    iters, outer, rels, conv = [], [], [], True
     t1 = time.perf_counter()
     for _ in range(args.steps):
-        st = solver.solve_device(b_dev, x_dev)[0]
-        iters.append(st.iterations)
-        outer.append(st.outer_iterations)
-        rels.append(st.rel_residual)
-        if st.converged != 1:
-            raise SystemExit(f"solve did not converge: {st.converged}")
+        st = solver.solve_device(b_dev, x_dev)
+        iters += [s.iterations for s in st]
+        outer += [s.outer_iterations for s in st]
+        rels += [s.rel_residual for s in st]
+        conv = conv and all(s.converged == 1 for s in st)
     _lib.call("mgx_device_synchronize")
     elapsed = time.perf_counter() - t1
-    solves_per_s = args.steps / elapsed
-    x = x_dev.numpy()
+    if not conv:
+        raise SystemExit("a load case did not converge")
+    solves_per_s = nrhs * args.steps / elapsed
+    print(f"# timed {args.steps} steps: {solves_per_s:.2f} solves/s", file=sys.stderr, flush=True)
+    X = x_dev.numpy().reshape(nrhs, -1)
+
     spmv_ms, spmv_n = 0.0, 0
-    xp = DeviceArray(nrows)
-    for _ in range(max(1, min(args.steps, 3))):
-        st = prof.solve_device(b_dev, xp)[0]
-        spmv_ms += st.spmv_ms
-        spmv_n += st.spmv_launches
+    xp = DeviceArray(F.size)
+    for _ in range(max(1, min(args.steps, 2))):
+        st = prof.solve_device(b_dev, xp)
+        for j in range(0, nrhs, max(width, 1)):   # one entry per block: the block's launches
+            spmv_ms += st[j].spmv_ms
+            spmv_n += st[j].spmv_launches
 
-    # FP64 check outside the timed region: energy-norm error against a 1e-13 solve
-    tight = HipMgPcgSolver(model, levels=args.levels, smoother=args.smoother, nu=args.nu, tolerance=1e-13,
-                           max_iterations=2000)
-    tight.Initialize(f)
-    xt, stt = tight.Solve()
-    e = DeviceArray.from_numpy(x - xt)
-    Ke = DeviceArray(nrows)
-    from compressiontests_amd import kernels
-    kernels.matrix_vector_product(K, e, Ke)
-    xd = DeviceArray.from_numpy(xt)
-    Kx = DeviceArray(nrows)
-    kernels.matrix_vector_product(K, xd, Kx)
-    energy_err = float(np.sqrt(max((x - xt) @ Ke.numpy(), 0.0) / (xt @ Kx.numpy())))
+    # the plain FP64 GPU path (one right-hand side at a time) on the first load cases
+    fp64 = None
+    if args.fp64_compare > 0 and not (args.precision == "fp64" and width == 1):
+        m = min(args.fp64_compare, nrhs)
+        one = HipMgPcgSolver(model, tolerance=args.tol, max_iterations=500, nrhs=m, block_width=1,
+                             hierarchy=hier, use_graph=bool(args.graph), **common)
+        one.Initialize(F[:m])
+        bd = DeviceArray.from_numpy(F[:m].reshape(-1))
+        xd = DeviceArray(m * ndof)
+        one.solve_device(bd, xd)
+        _lib.call("mgx_device_synchronize")
+        t = time.perf_counter()
+        st1 = one.solve_device(bd, xd)
+        _lib.call("mgx_device_synchronize")
+        dt = time.perf_counter() - t
+        fp64 = {"solves_per_s": round(m / dt, 4), "load_cases": m,
+                "iterations": int(np.median([s.iterations for s in st1]))}
+        one.ReleaseResources()
+        print(f"# plain FP64 path: {fp64['solves_per_s']} solves/s", file=sys.stderr, flush=True)
 
-    bytes_per_launch = csr_spmv_bytes(nnz, nrows, nrows)
+    # FP64 check outside the timed region: energy-norm error against FP64 solves to 1e-12
+    cols = sorted(set([0, nrhs - 1][:max(1, min(args.check, nrhs))]))
+    tight = HipMgPcgSolver(model, tolerance=1e-12, max_iterations=2000, nrhs=len(cols), block_width=1,
+                           hierarchy=hier, **common)
+    tight.Initialize(F[cols])
+    Xt, stt = tight.Solve()
+    Xt = Xt.reshape(len(cols), -1)
+    energy_err = 0.0
+    for i, j in enumerate(cols):
+        e = DeviceArray.from_numpy(X[j] - Xt[i])
+        Ke = DeviceArray(ndof)
+        kernels.matrix_vector_product(K, e, Ke)
+        xd = DeviceArray.from_numpy(Xt[i])
+        Kx = DeviceArray(ndof)
+        kernels.matrix_vector_product(K, xd, Kx)
+        energy_err = max(energy_err, float(np.sqrt(max((X[j] - Xt[i]) @ Ke.numpy(), 0.0) / (Xt[i] @ Kx.numpy()))))
+    tight.ReleaseResources()
+
+    vb = 4 if args.precision == "mixed" else 8
+    if width > 1:
+        bytes_per_launch = spmm_bytes(nnz, ndof, width, vb)
+        kname = (f"k_spmm<{'float' if vb == 4 else 'double'}, K={width}, DOT> (level-0 block SpMM Q = A P with "
+                 f"per-column P.Q)")
+    else:
+        bytes_per_launch = csr_spmv_bytes(nnz, ndof, ndof, vb)
+        kname = f"k_spmv_dot<16, {'float' if vb == 4 else 'double'}> (level-0 CSR SpMV q = A p with p.q)"
     avg_ms = spmv_ms / max(spmv_n, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
-    pmc = os.path.join(ROOT, "profiles", f"spmv_csr_fp64_{n}cubed_pmc.json")
+    tag = f"{'spmm' if width > 1 else 'spmv'}_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed"
+    pmc = os.path.join(ROOT, "profiles", f"{tag}_pmc.json")
     if os.path.exists(pmc):
         try:
             traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
-        except Exception:
+        except (OSError, ValueError):
             traffic = None
 
     cpu = None
     if args.cpu_baseline:
-        cpu = cpu_baseline(n, dens, f, args, int(np.median(iters)))
+        cpu = cpu_baseline(hier, F[0], args, fp64["iterations"] if fp64 else int(np.median(iters)))
 
     line = {
         "metric": "MG-PCG solves/s to 1e-8 energy-norm (FP64-checked); SpMV HBM GB/s vs peak",
@@ -145,24 +216,32 @@ def main():
         "dtype": "f64" if args.precision == "fp64" else "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"config2: {n}^3 hex cantilever, {nrows} free dofs, {nnz} nnz, moduli 10^U(-2,2) "
-                        f"seed {SEED_MODULI}, tip load -1 on x=1, "
-                        + ("FP64 MG-PCG" if args.precision == "fp64" else
-                           "FP64 iterative refinement around FP32 MG-PCG")
-                        + f" ({args.levels} levels Galerkin, {args.smoother} nu={args.nu}) to rel. residual "
-                          f"{args.tol:g} (FP64 true residual) from x=0",
-            "precision": args.precision,
-            "dofs": nrows, "nnz": nnz, "iterations": int(np.median(iters)),
-            "outer_iterations": int(np.median(outer)),
-            "dof_per_s": round(nrows * solves_per_s, 1),
+            "workload": f"{n}^3 hex cantilever, {ndof} free dofs, {nnz} nnz, moduli "
+                        + ("log-normal sigma 1.5 clipped to 1e3 contrast" if args.moduli == "lognormal"
+                           else "log-uniform [1e-2, 1e2]")
+                        + (f", {nrhs} top-face load cases (unit norm)" if nrhs > 1 else ", tip load -1 on x=1")
+                        + f"; {'batched ' if width > 1 else ''}MG-PCG, {levels}-level Galerkin V-cycle, "
+                          f"{args.smoother} nu={args.nu}, "
+                        + ("FP32 V-cycle/SpMM inside FP64 iterative refinement" if args.precision == "mixed"
+                           else "FP64")
+                        + f", to rel. residual {args.tol:g} on every FP64 true residual, from x=0",
+            "dofs": ndof, "nnz": nnz, "load_cases": nrhs, "levels": levels,
+            "precision": args.precision, "block_width": width,
+            "autotune_ms_per_iteration": {f"k{k}": round(v, 4) for k, v in zip((4, 8, 16, 32), tune_ms) if v > 0},
+            "iterations_median": int(np.median(iters)), "iterations_max": int(max(iters)),
+            "outer_iterations_max": int(max(outer)),
+            "dof_per_s": round(ndof * solves_per_s, 1),
             "energy_norm_error": energy_err,
-            "rel_residual": float(max(rels)),
+            "energy_norm_checked_cases": cols,
+            "rel_residual_max": float(max(rels)),
+            "fp64_single_rhs_path": fp64,
+            "speedup_vs_fp64_single_rhs": round(solves_per_s / fp64["solves_per_s"], 3) if fp64 else None,
             "setup_s": round(setup_s, 2),
         },
         "roofline": {
             "bound": "hbm",
-            "kernel": "k_spmv_dot_alpha<16> (level-0 CSR SpMV q = A p with p.q), HIP events on the solver "
-                      "stream around each launch in profiled solves of the same system",
+            "kernel": kname + ", HIP events on the solver stream around each launch in profiled solves of the "
+                              "same batch",
             "achieved": round(achieved, 1),
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
@@ -177,32 +256,37 @@ def main():
     print(json.dumps(line), flush=True)
 
 
-def cpu_baseline(n, dens, f, args, gpu_iterations):
-    """The oracle's single-threaded C restatement of the same FP64 MG-PCG (same matrix, Galerkin
-    hierarchy, smoother and right-hand side).  Bounded sample: the time of 4 PCG iterations
-    (a 6-iteration run minus a 2-iteration run, so the per-solve set-up cancels), scaled to
-    the GPU solve's iteration count."""
+def cpu_baseline(hier, f, args, iterations):
+    """The oracle's single-threaded C restatement of the FP64 MG-PCG (one right-hand side, same
+    Galerkin hierarchy, smoother and first load case).  Bounded sample: the time of 2 PCG
+    iterations (a 3-iteration run minus a 1-iteration run, so the per-solve set-up cancels),
+    scaled to the FP64 GPU solve's iteration count."""
     from oracle import oracle as O
-    A = O.cantilever(n, n, n, dens=dens)
-    lv, m = [], n
-    for _ in range(args.levels - 1):
-        P = O.p_trilinear(m, m, m)
-        lv.append(dict(A=A, R=O.transpose(P), P=P))
-        A = O.galerkin(A, P)
-        m //= 2
-    lv.append(dict(A=A))
+    lv = []
+    for A, R, P in hier:
+        d = {"A": _oracle_csr(O, A)}
+        if R is not None:
+            d["R"], d["P"] = _oracle_csr(O, R), _oracle_csr(O, P)
+        lv.append(d)
     sm = 1 if args.smoother == "gs" else 0
     times = {}
-    for it in (2, 6):
+    for it in (1, 3):
         t = time.perf_counter()
         O.mgpcg(lv, f, smoother=sm, omega=0.0, nu=args.nu, tol=0.0, max_it=it)
         times[it] = time.perf_counter() - t
-    per_it = max(times[6] - times[2], 1e-9) / 4
-    solve_s = per_it * gpu_iterations
-    return {"value": round(1.0 / solve_s, 5), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"4 FP64 MG-PCG iterations of the same {n}^3 system (oracle C restatement, -O2, "
-                      f"1 thread; 6-iteration minus 2-iteration run), {per_it:.3f} s/iteration, scaled to "
-                      f"{gpu_iterations} iterations = {solve_s:.1f} s per solve"}
+        print(f"# cpu baseline: {it} iterations in {times[it]:.1f} s", file=sys.stderr, flush=True)
+    per_it = max(times[3] - times[1], 1e-9) / 2
+    solve_s = per_it * iterations
+    return {"value": round(1.0 / solve_s, 6), "unit": "solves/s", "cores": 1, "kind": "port",
+            "sample": f"2 FP64 MG-PCG iterations on one load case of the same {args.n}^3 system and hierarchy "
+                      f"(oracle C restatement, -O2, 1 thread; 3-iteration minus 1-iteration run), {per_it:.3f} "
+                      f"s/iteration, scaled to the FP64 GPU path's {iterations} iterations = {solve_s:.1f} s "
+                      f"per solve"}
+
+
+def _oracle_csr(O, M):
+    h = M.to_host()
+    return O.Csr(h.rowptr, h.col, h.val, h.nrows, h.ncols)
 
 
 if __name__ == "__main__":

@@ -112,6 +112,10 @@ _SIGS = {
     "mgx_device_count": (_I32, []),
     "mgx_csr_spmv": (C.c_int, [_I32, _P, _P, _P, _P, _P, _I32, _P]),
     "mgx_csr_spmv_f32": (C.c_int, [_I32, _P, _P, _P, _P, _P, _I32, _P]),
+    "mgx_csr_spmm": (C.c_int, [C.c_int32, _P, _P, _P, C.c_int32, _P, _P, C.c_int32, _P]),
+    "mgx_csr_spmm_f32": (C.c_int, [C.c_int32, _P, _P, _P, C.c_int32, _P, _P, C.c_int32, _P]),
+    "mgx_time_spmm": (C.c_int, [C.c_int32, C.c_int32, C.c_int32, _P, _P, _P, _P, _P, C.c_int32, C.c_int32,
+                                C.POINTER(C.c_double)]),
     "mgx_csr_residual": (C.c_int, [_I32, _P, _P, _P, _P, _P, _P, _P]),
     "mgx_csr_residual_check": (C.c_int, [_I32, _P, _P, _P, _P, _P, _P, _D, _P, _P]),
     "mgx_jacobi_initial": (C.c_int, [_I32, _P, _P, _P, _P]),

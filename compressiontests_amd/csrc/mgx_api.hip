@@ -307,12 +307,10 @@ static void level_jacobi(mgx_solver_s *S, Level &L, const double *b, const doubl
 
 static void level_gs(mgx_solver_s *S, Level &L, const double *b, double *x, bool backward, const int32_t *done)
 {
-    if (!backward)
-        for (int c = 0; c < L.ncolors; ++c)
-            launch_gs_color(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v, b, x, false, done, S->stream);
-    else
-        for (int c = L.ncolors - 1; c >= 0; --c)
-            launch_gs_color(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v, b, x, true, done, S->stream);
+    for (int i = 0; i < L.ncolors; ++i) {
+        const int c = backward ? L.ncolors - 1 - i : i;
+        launch_gs_color(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v, b, x, backward, done, S->stream);
+    }
 }
 
 // ---------------------------------------------------------------- initialize
@@ -790,12 +788,11 @@ static void finish_stats(mgx_solver_s *S, const double *b, const double *x, mgx_
 // ---------------------------------------------------------------- precision form
 static void level_gs32(mgx_solver_s *S, Level &L, const float *b, float *x, bool backward, const int32_t *done)
 {
-    if (!backward)
-        for (int c = 0; c < L.ncolors; ++c)
-            launch_gs_color_f32(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v32, b, x, false, done, S->stream);
-    else
-        for (int c = L.ncolors - 1; c >= 0; --c)
-            launch_gs_color_f32(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v32, b, x, true, done, S->stream);
+    for (int i = 0; i < L.ncolors; ++i) {
+        const int c = backward ? L.ncolors - 1 - i : i;
+        launch_gs_color_f32(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v32, b, x, backward, done,
+                            S->stream);
+    }
 }
 
 // FP32 twin of vcycle(): FP32 values and vectors, FP64 coarse solve.
@@ -1359,6 +1356,30 @@ int mgx_csr_spmv_f32(int32_t nrows, const int32_t *rowptr, const int32_t *col, c
     });
 }
 
+int mgx_csr_spmm(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val, int32_t k,
+                 const double *X, double *Y, int32_t clear, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_ROWS(nrows, rowptr);
+        MGX_REQUIRE(col && val && X && Y, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(k == 4 || k == 8 || k == 16 || k == 32, MGX_ERR_INVALID_ARGUMENT, "k must be 4, 8, 16 or 32");
+        launch_spmm<double, double>(k, clear ? BEPI_SPMM : BEPI_ADD, nrows, rowptr, col, val, X, nullptr, nullptr, Y,
+                                    nullptr, nullptr, 0, (hipStream_t)stream);
+    });
+}
+
+int mgx_csr_spmm_f32(int32_t nrows, const int32_t *rowptr, const int32_t *col, const float *val, int32_t k,
+                     const float *X, float *Y, int32_t clear, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_ROWS(nrows, rowptr);
+        MGX_REQUIRE(col && val && X && Y, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(k == 4 || k == 8 || k == 16 || k == 32, MGX_ERR_INVALID_ARGUMENT, "k must be 4, 8, 16 or 32");
+        launch_spmm<float, float>(k, clear ? BEPI_SPMM : BEPI_ADD, nrows, rowptr, col, val, X, nullptr, nullptr, Y,
+                                  nullptr, nullptr, 0, (hipStream_t)stream);
+    });
+}
+
 int mgx_csr_residual(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val, const double *b,
                      const double *x, double *r, void *stream)
 {
@@ -1829,6 +1850,38 @@ int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, cons
             else if (kind == 2) launch_residual(nrows, rowptr, col, (const double *)val, (const double *)b, (const double *)x,
                                                 (double *)y, lanes, nullptr, s);
             else launch_spmv_f32(nrows, rowptr, col, (const float *)val, (const float *)x, (float *)y, true, lanes, nullptr, s);
+        };
+        for (int i = 0; i < warmup; ++i) one();
+        MGX_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < repeats; ++i) one();
+        MGX_HIP(hipEventRecord(e1, s));
+        MGX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = ms / repeats;
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    });
+}
+
+int mgx_time_spmm(int32_t fp32, int32_t k, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
+                  const void *X, void *Y, int32_t warmup, int32_t repeats, double *avg_ms)
+{
+    return guard([&] {
+        MGX_CHECK_ROWS(nrows, rowptr);
+        MGX_REQUIRE(col && val && X && Y && avg_ms && repeats >= 1 && warmup >= 0, MGX_ERR_INVALID_ARGUMENT,
+                    "invalid arguments");
+        MGX_REQUIRE(k == 4 || k == 8 || k == 16 || k == 32, MGX_ERR_INVALID_ARGUMENT, "k must be 4, 8, 16 or 32");
+        hipStream_t s;
+        MGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t e0, e1;
+        MGX_HIP(hipEventCreate(&e0));
+        MGX_HIP(hipEventCreate(&e1));
+        auto one = [&]() {
+            if (fp32) launch_spmm<float, float>(k, BEPI_SPMM, nrows, rowptr, col, (const float *)val, (const float *)X,
+                                                nullptr, nullptr, (float *)Y, nullptr, nullptr, 0, s);
+            else launch_spmm<double, double>(k, BEPI_SPMM, nrows, rowptr, col, (const double *)val, (const double *)X,
+                                             nullptr, nullptr, (double *)Y, nullptr, nullptr, 0, s);
         };
         for (int i = 0; i < warmup; ++i) one();
         MGX_HIP(hipEventRecord(e0, s));

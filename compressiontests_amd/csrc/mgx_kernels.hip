@@ -96,11 +96,102 @@ __global__ __launch_bounds__(kBlock) void k_csr_rows(int32_t nrows, const int32_
     }
 }
 
+
+// CSR-stream SpMV for the stiffness matrices (mean row length >= 16 lanes' worth): a 256-thread
+// block takes a tile of RB consecutive rows whose (col, val) range is contiguous; every thread
+// loads E entries at stride 256 (each wave load is one contiguous, fully used segment), gathers
+// x, and stores the products in LDS; then G = 256 / RB lanes sum each row's products.  Tiles
+// longer than the LDS chunk are processed in chunks, so any row length is handled.  Each XCD
+// walks a contiguous eighth of the tiles; with a capped grid (the DOT form, whose per-block
+// partials feed k_finalize) blocks loop over the tiles of their eighth.
+template <typename T> struct StreamShape {
+    static constexpr int RB = sizeof(T) == 8 ? 64 : 32;                 // rows per tile
+    static constexpr int G = kBlock / RB;                                // lanes per row in the sum
+    static constexpr int E = (RB * 84 + kBlock - 1) / kBlock;           // entries per thread per chunk
+    static constexpr int C = E * kBlock;                                 // entries per chunk
+};
+
+template <typename T, int EPI, bool DOT>
+__global__ __launch_bounds__(kBlock) void k_csr_stream(int32_t nrows, const int32_t *__restrict__ rp,
+                                                       const int32_t *__restrict__ ci, const T *__restrict__ v,
+                                                       const T *__restrict__ x, const T *__restrict__ b,
+                                                       const T *__restrict__ pre, T *__restrict__ y,
+                                                       const int32_t *__restrict__ done, double *__restrict__ partials)
+{
+    using Sh = StreamShape<T>;
+    constexpr int RB = Sh::RB, G = Sh::G, E = Sh::E, C = Sh::C;
+    __shared__ T sp[C];
+    if (done && *done) return;
+    const int t = threadIdx.x, lane = t % G;
+    const int ntiles = (nrows + RB - 1) / RB;
+    const int per = gridDim.x >> 3, xcd = blockIdx.x & 7;
+    const int tend = (int)((int64_t)ntiles * (xcd + 1) / 8);
+    double dot = 0;
+    bool first = true;
+    for (int tile = (int)((int64_t)ntiles * xcd / 8) + (int)(blockIdx.x >> 3); tile < tend; tile += per) {
+        const int r0 = tile * RB, r1 = min(r0 + RB, nrows);
+        const int e0 = rp[r0], cnt = rp[r1] - e0;
+        const int row = r0 + t / G;
+        int s = 0, e = 0;
+        if (row < r1) { s = rp[row] - e0; e = rp[row + 1] - e0; }
+        T a = 0;
+        if (!first) __syncthreads();                  // the previous tile's readers are done
+        first = false;
+        if (cnt <= C) {
+            // the common case: the whole tile in one chunk, all loads issued before any use
+            int c[E];
+            T w[E];
+#pragma unroll
+            for (int j = 0; j < E; ++j) {
+                const int k = t + kBlock * j;
+                if (k < cnt) { c[j] = ntload(ci + e0 + k); w[j] = ntload(v + e0 + k); } else { c[j] = 0; w[j] = 0; }
+            }
+            T xv[E];
+#pragma unroll
+            for (int j = 0; j < E; ++j) xv[j] = x[c[j]];
+#pragma unroll
+            for (int j = 0; j < E; ++j) sp[t + kBlock * j] = w[j] * xv[j];
+            __syncthreads();
+            for (int k = s + lane; k < e; k += G) a += sp[k];
+        } else {
+            for (int base = 0; base < cnt; base += C) {
+                if (base > 0) __syncthreads();
+                for (int j = 0; j < E; ++j) {
+                    const int k = base + t + kBlock * j;
+                    sp[t + kBlock * j] = k < cnt ? ntload(v + e0 + k) * x[ntload(ci + e0 + k)] : T(0);
+                }
+                __syncthreads();
+                const int lo = max(s, base), hi = min(e, base + C);
+                for (int k = lo + lane; k < hi; k += G) a += sp[k - base];
+            }
+        }
+#pragma unroll
+        for (int o = G / 2; o > 0; o >>= 1) a += __shfl_xor(a, o, G);
+        if (lane == 0 && row < r1) {
+            if (DOT) { y[row] = a; dot += (double)x[row] * (double)a; }
+            else if (EPI == EPI_SPMV) y[row] = a;
+            else if (EPI == EPI_SPMV_ADD) y[row] = y[row] + a;
+            else if (EPI == EPI_RESID) y[row] = b[row] - a;
+            else y[row] = x[row] + pre[row] * (b[row] - a);
+        }
+    }
+    if (DOT) block_publish(dot, 0, 0, partials, 0, 1);
+}
+
+static inline int stream_tiles(int64_t nrows, int rb) { return (int)((nrows + rb - 1) / rb); }
+
 template <int EPI, typename T>
 static void dispatch_rows(int32_t nrows, const int32_t *rp, const int32_t *ci, const T *v, const T *x,
                           const T *b, const T *pre, T *y, int lanes, const int32_t *done, hipStream_t s)
 {
     if (nrows <= 0) return;
+    if (lanes >= 16) {
+        const int nb = grid_rows(stream_tiles(nrows, StreamShape<T>::RB), 1);
+        hipLaunchKernelGGL((k_csr_stream<T, EPI, false>), dim3(nb), dim3(kBlock), 0, s, nrows, rp, ci, v, x, b, pre, y,
+                           done, nullptr);
+        MGX_HIP(hipGetLastError());
+        return;
+    }
 #define MGX_ROWS(G)                                                                                      \
     hipLaunchKernelGGL((k_csr_rows<G, EPI, T>), dim3(grid_rows(nrows, kBlock / G)), dim3(kBlock), 0, s, \
                        nrows, rp, ci, v, x, b, pre, y, done)
@@ -287,8 +378,34 @@ __global__ __launch_bounds__(1024) void k_finalize(int nparts, const double *__r
     }
 }
 
+// First stage for long one-slot partial lists (the per-tile partials of the streaming SpMV + dot):
+// block b sums partials[b*2048 .. b*2048 + 2048) in a fixed order into out[b].
+__global__ __launch_bounds__(kBlock) void k_partials_reduce(int nparts, const double *__restrict__ partials,
+                                                            double *__restrict__ out, const Ctl *ctl)
+{
+    if (ctl->done) return;
+    double a = 0, b = 0, m = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int i = blockIdx.x * 2048 + j * kBlock + threadIdx.x;
+        if (i < nparts) a += partials[i];
+    }
+    block_reduce3(a, b, m);
+    if (threadIdx.x == 0) out[blockIdx.x] = a;
+}
+
 void launch_finalize(int nparts, const double *partials, int stride, Ctl *ctl, int mode, double *result, hipStream_t s)
 {
+    const bool one_slot = mode != FIN_UPDATE && mode != FIN_NORMS;
+    if (one_slot && nparts > 4096 && ctl) {
+        // the stage-1 sums go right after the partials (the buffer holds at least 3x as many)
+        double *out = const_cast<double *>(partials) + ((nparts + 255) & ~255);
+        const int g1 = (nparts + 2047) / 2048;
+        hipLaunchKernelGGL(k_partials_reduce, dim3(g1), dim3(kBlock), 0, s, nparts, partials, out, ctl);
+        MGX_HIP(hipGetLastError());
+        partials = out;
+        nparts = g1;
+    }
     hipLaunchKernelGGL(k_finalize, dim3(1), dim3(1024), 0, s, nparts, partials, stride, ctl, mode, result);
     MGX_HIP(hipGetLastError());
 }
@@ -360,6 +477,13 @@ int launch_spmv_dot_t(int32_t n, const int32_t *rp, const int32_t *ci, const T *
                       int lanes, const int32_t *done, double *partials, hipStream_t s)
 {
     int grid = 0;
+    if (lanes >= 16) {
+        grid = grid_rows(stream_tiles(n, StreamShape<T>::RB), 1);
+        hipLaunchKernelGGL((k_csr_stream<T, EPI_SPMV, true>), dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, p, nullptr,
+                           nullptr, q, done, partials);
+        MGX_HIP(hipGetLastError());
+        return grid;
+    }
 #define MGX_SDA(G) grid = grid_rows(n, kBlock / G); \
     hipLaunchKernelGGL((k_spmv_dot<G, T>), dim3(grid), dim3(kBlock), 0, s, n, rp, ci, v, p, q, done, partials)
     switch (lanes) {

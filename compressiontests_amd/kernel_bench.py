@@ -6,7 +6,8 @@ the resulting GB/s against the 8 TB/s HBM peak.
   python -m compressiontests_amd.kernel_bench --n 64 --kinds spmv,jacobi,residual,spmv_f32,copy
 
 `copy` is a plain device-to-device copy kernel of known size, the calibration for rocprofv3's
-FETCH_SIZE / WRITE_SIZE counters.
+FETCH_SIZE / WRITE_SIZE counters.  `spmm64_k32` / `spmm32_k32` (any k in 4, 8, 16, 32): the
+block SpMM Y = A X of the batching form, FP64 or FP32 values and vectors.
 """
 import argparse
 import ctypes as C
@@ -32,9 +33,13 @@ def main():
     ap.add_argument("--repeats", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=20261203 + 2)
+    ap.add_argument("--moduli", default="loguniform", choices=["loguniform", "lognormal"])
     args = ap.parse_args()
     n = args.n
-    dens = 10 ** (4 * rng_uniform(args.seed, n ** 3) - 2)
+    if args.moduli == "loguniform":
+        dens = 10 ** (4 * rng_uniform(args.seed, n ** 3) - 2)
+    else:   # log-normal, sigma 1.5, clipped to a 1e3 contrast
+        dens = np.exp(np.clip(1.5 * rng_normal(20261203 + 3, n ** 3), -0.5 * np.log(1e3), 0.5 * np.log(1e3)))
     A = cantilever_matrix(n, n, n, dens=dens)
     N, nnz = A.nrows, A.nnz
     x = DeviceArray.from_numpy(rng_normal(args.seed + 1, N))
@@ -60,6 +65,17 @@ def main():
             y32 = DeviceArray(N, np.float32)
             ms = kernels.time_csr_kernel(3, A32, x32, y32, warmup=args.warmup, repeats=args.repeats)
             by = spmv_bytes(nnz, N, 4, 4)
+        elif kind.startswith("spmm"):
+            fp32 = kind.startswith("spmm32")
+            k = int(kind.split("_k")[1])
+            dt = np.float32 if fp32 else np.float64
+            from .device import DeviceCsr
+            Ak = DeviceCsr.arrays_from_host(A.to_host(), val_dtype=dt) if fp32 else A
+            X = DeviceArray.from_numpy(rng_normal(args.seed + 3, N * k).astype(dt))
+            Y = DeviceArray(N * k, dt)
+            ms = kernels.time_spmm(Ak, X, Y, k, warmup=args.warmup, repeats=args.repeats)
+            vb = 4 if fp32 else 8
+            by = (vb + 4) * nnz + 4 * (N + 1) + 2 * vb * k * N
         elif kind.startswith("duvi"):
             mode = {"duvi": 0, "duvi_u32": 4, "duvi_raw": 8}[kind]
             d = duvi_encode(A.to_host(), 1e-10, mode)
@@ -80,7 +96,7 @@ def main():
         gbs = by / (ms * 1e-3) / 1e9
         out["kernels"][kind] = {"avg_ms": round(ms, 5), "bytes": int(by), "GBps": round(gbs, 1),
                                 "frac_of_peak": round(gbs / HBM_PEAK_GBS, 4)}
-        print(f"{kind:10s} {ms:9.4f} ms  {by / 1e6:9.1f} MB  {gbs:8.1f} GB/s  {gbs / HBM_PEAK_GBS:6.3f} of peak",
+        print(f"{kind:12s} {ms:9.4f} ms  {by / 1e6:9.1f} MB  {gbs:8.1f} GB/s  {gbs / HBM_PEAK_GBS:6.3f} of peak",
               flush=True)
     print(json.dumps(out), flush=True)
 

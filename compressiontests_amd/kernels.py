@@ -15,6 +15,13 @@ def matrix_vector_product(A, x, y, clear=True, stream=None):
     L.call(name, A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), x.vp, y.vp, int(clear), stream)
 
 
+def spmm(A, X, Y, k, clear=True, stream=None):
+    """Y = A X (clear) or Y += A X for k right-hand sides, X and Y row-major (rows x k) blocks."""
+    name = "mgx_csr_spmm_f32" if X.dtype == np.float32 else "mgx_csr_spmm"
+    L.call(name, A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), k, X.vp, Y.vp, int(clear),
+           stream)
+
+
 def residual(A, b, x, r, stream=None):
     """r = b - A x.  CsrGeometricMultigrid.cl:96-110."""
     L.call("mgx_csr_residual", A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), b.vp, x.vp, r.vp,
@@ -59,6 +66,14 @@ def time_csr_kernel(kind, A, x, y, b=None, pre=None, warmup=5, repeats=100):
     out = C.c_double()
     L.call("mgx_time_csr_kernel", kind, A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), x.vp,
            b.vp if b is not None else None, pre.vp if pre is not None else None, y.vp, warmup, repeats, C.byref(out))
+    return out.value
+
+
+def time_spmm(A, X, Y, k, warmup=5, repeats=100):
+    """Average device ms per launch of Y = A X with k right-hand sides (FP32 if X is float32)."""
+    out = C.c_double()
+    L.call("mgx_time_spmm", int(X.dtype == np.float32), k, A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col),
+           C.c_void_p(A.val), X.vp, Y.vp, warmup, repeats, C.byref(out))
     return out.value
 
 

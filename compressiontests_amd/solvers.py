@@ -287,16 +287,19 @@ class HipMgPcgSolver:
         self.nrhs = nrhs
         h = _Handle(opt)
         nx, ny, nz = model.NumElements
-        if hierarchy == "galerkin":
+        self.hierarchy = []             # (A, R, P) device matrices per level, finest first
+        if isinstance(hierarchy, (list, tuple)):
+            self.hierarchy = list(hierarchy)      # shared with another solver of the same model
+        elif hierarchy == "galerkin":
             Ks = galerkin_hierarchy(nx, ny, nz, levels, model.ElasticityModulus, model.PoissonRatio,
                                     model.ElementDensities)
             model._K = Ks[0]
             dims = (nx, ny, nz)
             for l in range(levels - 1):
                 R, P = transfer_trilinear(*dims)
-                h.set_level_device(l, Ks[l], R, P)
+                self.hierarchy.append((Ks[l], R, P))
                 dims = tuple(d // 2 for d in dims)
-            h.set_level_device(levels - 1, Ks[-1])
+            self.hierarchy.append((Ks[-1], None, None))
         elif hierarchy == "reference":
             m = model
             for l in range(levels - 1):
@@ -308,11 +311,15 @@ class HipMgPcgSolver:
                 from .device import CsrMatrix
                 R = DeviceCsr.from_host(CsrMatrix(PT.indptr.astype(np.int32), PT.indices.astype(np.int32),
                                                   PT.data, PT.shape[0], PT.shape[1]))
-                h.set_level_device(l, m.device_matrix(), R, P)
+                self.hierarchy.append((m.device_matrix(), R, P))
                 m = coarser
-            h.set_level_device(levels - 1, m.device_matrix())
+            self.hierarchy.append((m.device_matrix(), None, None))
         else:
-            raise ValueError("hierarchy must be 'galerkin' or 'reference'")
+            raise ValueError("hierarchy must be 'galerkin', 'reference' or a list of (A, R, P)")
+        if len(self.hierarchy) != levels:
+            raise ValueError("hierarchy depth differs from levels")
+        for l, (A, R, P) in enumerate(self.hierarchy):
+            h.set_level_device(l, A, R, P)
         self._h = h
 
     def Initialize(self, b=None):

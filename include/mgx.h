@@ -57,6 +57,15 @@ int mgx_csr_spmv(int32_t nrows, const int32_t *rowptr, const int32_t *col, const
 int mgx_csr_spmv_f32(int32_t nrows, const int32_t *rowptr, const int32_t *col, const float *val,
                      const float *x, float *y, int32_t clear, void *stream);
 
+/* Y = A X (clear = 1) or Y += A X (clear = 0) for k right-hand sides at once: X (ncols x k) and
+ * Y (nrows x k) are row-major blocks (element (i, c) at i*k + c), k in {4, 8, 16, 32}; the
+ * matrix is read once for all k columns.  Batching form (no reference counterpart; the
+ * reference solves one right-hand side at a time, PcgSolver2.cs:92-136). */
+int mgx_csr_spmm(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val, int32_t k,
+                 const double *X, double *Y, int32_t clear, void *stream);
+int mgx_csr_spmm_f32(int32_t nrows, const int32_t *rowptr, const int32_t *col, const float *val, int32_t k,
+                     const float *X, float *Y, int32_t clear, void *stream);
+
 /* r = b - A x.  Replaces residual, CsrGeometricMultigrid.cl:96-110. */
 int mgx_csr_residual(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val,
                      const double *b, const double *x, double *r, void *stream);
@@ -282,6 +291,9 @@ int mgx_device_synchronize(void);
 int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
                         const void *x, const void *b, const void *pre, void *y, int32_t warmup, int32_t repeats,
                         double *avg_ms);
+/* Same for mgx_csr_spmm (fp32 = 0) / mgx_csr_spmm_f32 (fp32 = 1) with k right-hand sides. */
+int mgx_time_spmm(int32_t fp32, int32_t k, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
+                  const void *X, void *Y, int32_t warmup, int32_t repeats, double *avg_ms);
 /* Average device time of a plain copy kernel (16-byte vector loads and stores) moving `bytes`
  * (a multiple of 16) from src to dst: the calibration of HBM byte counters. */
 int mgx_time_copy_kernel(const void *src, void *dst, uint64_t bytes, int32_t warmup, int32_t repeats, double *avg_ms);

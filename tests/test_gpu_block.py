@@ -46,6 +46,30 @@ def het16(gpu):
     return m, F, lv, O.cantilever(n, n, n, dens=dens)
 
 
+@pytest.mark.parametrize("k", [4, 8, 16, 32])
+def test_spmm_matches_columnwise_products(het16, k):
+    """Block SpMM Y = A X (and Y += A X) against the FP64 CPU product of every column."""
+    from compressiontests_amd import kernels
+    from compressiontests_amd.device import DeviceArray, DeviceCsr
+    m, F, lv, A = het16
+    As = S(A)
+    Xh = O.rng_normal(77 + k, A.nrows * k).reshape(A.nrows, k)
+    ref = As @ Xh
+    D64 = DeviceCsr.arrays_from_host(A)
+    X = DeviceArray.from_numpy(Xh.reshape(-1))
+    Y = DeviceArray(A.nrows * k)
+    kernels.spmm(D64, X, Y, k)
+    got = Y.numpy().reshape(A.nrows, k)
+    assert np.max(np.abs(got - ref)) <= 1e-12 * np.max(np.abs(ref))
+    kernels.spmm(D64, X, Y, k, clear=False)
+    assert np.max(np.abs(Y.numpy().reshape(A.nrows, k) - 2 * ref)) <= 2e-12 * np.max(np.abs(ref))
+    D32 = DeviceCsr.arrays_from_host(A, val_dtype=np.float32)
+    X32 = DeviceArray.from_numpy(Xh.reshape(-1).astype(np.float32))
+    Y32 = DeviceArray(A.nrows * k, np.float32)
+    kernels.spmm(D32, X32, Y32, k)
+    assert np.max(np.abs(Y32.numpy().reshape(A.nrows, k) - ref)) <= 1e-5 * np.max(np.abs(ref))
+
+
 def test_load_cases_shape_and_norm(het16):
     m, F, lv, A = het16
     assert F.shape == (6, m.NumDofsFree)

@@ -105,6 +105,49 @@ def test_edge_shapes(gpu):
     K.matrix_vector_product(E, DeviceArray(1), DeviceArray(1))
 
 
+@pytest.mark.parametrize("f32", [False, True])
+def test_stream_kernel_long_rows_and_chunks(gpu, f32):
+    """Rows long enough that one 64-row tile (32 rows in FP32) spans several LDS chunks, one
+    row longer than a chunk, empty rows and a ragged last tile: SpMV, y += Ax, residual and
+    Jacobi against the FP64 CPU product."""
+    rng = np.random.default_rng(11)
+    ncols = 7000
+    lens = [400] * 70 + [0, 6500, 3, 0] + [90] * 9
+    rp, ci = [0], []
+    for L in lens:
+        ci += sorted(rng.choice(ncols, size=L, replace=False).tolist())
+        rp.append(len(ci))
+    val = rng.normal(size=len(ci))
+    A = O.Csr(np.array(rp, np.int32), np.array(ci, np.int32), val, len(lens), ncols)
+    dt = np.float32 if f32 else np.float64
+    dA = DeviceCsr.arrays_from_host(A, val_dtype=dt)
+    x = rng.normal(size=ncols)
+    Ah = O.Csr(A.rowptr, A.col, val.astype(dt).astype(np.float64), A.nrows, ncols)
+    xh = x.astype(dt).astype(np.float64)
+    ref = O.spmv(Ah, xh)
+    tol = 1e-5 if f32 else 1e-12
+    scale = np.abs(Ah.val).max() * np.abs(xh).max() * np.sqrt(6500)
+    dx = DeviceArray.from_numpy(x.astype(dt))
+    y = DeviceArray(A.nrows, dt)
+    K.matrix_vector_product(dA, dx, y)
+    assert np.abs(y.numpy() - ref).max() <= tol * scale
+    K.matrix_vector_product(dA, dx, y, clear=False)
+    assert np.abs(y.numpy() - 2 * ref).max() <= 2 * tol * scale
+    if not f32:
+        # square part for the residual / Jacobi epilogues: first nrows columns of x
+        n = A.nrows
+        sq = O.Csr(A.rowptr, np.minimum(A.col, n - 1).astype(np.int32), val, n, n)
+        dS = DeviceCsr.arrays_from_host(sq)
+        xs, b = rng.normal(size=n), rng.normal(size=n)
+        pre = np.full(n, 0.3)
+        r = DeviceArray(n)
+        K.residual(dS, DeviceArray.from_numpy(b), DeviceArray.from_numpy(xs), r)
+        assert close(r.numpy(), O.residual(sq, b, xs), 1e-11)
+        yj = DeviceArray(n)
+        K.jacobi_iteration(DeviceArray.from_numpy(pre), dS, DeviceArray.from_numpy(b), DeviceArray.from_numpy(xs), yj)
+        assert close(yj.numpy(), O.jacobi(sq, pre, b, xs), 1e-11)
+
+
 def test_spmv_f32(k16):
     A, _ = k16
     dA32 = DeviceCsr.arrays_from_host(A, val_dtype=np.float32)
