@@ -220,6 +220,7 @@ struct Level {
     DBuf<double> bx, bb, br, bt;           // batching form: n x K block vectors
     DBuf<float> bx32, bb32, br32, bt32;
     int ncolors = 0;
+    bool node3 = false;                    // rows 3q..3q+2 share one column list (node-triple kernels)
     std::vector<DBuf<int32_t>> color_nodes;
     std::vector<int32_t> color_count;
     DevDuvi dA, dR, dP;
@@ -420,6 +421,22 @@ static void initialize(mgx_solver_s *S)
         Level &V = S->lv[l];
         if (V.staged && !V.extA) { hrp[l] = V.hArp; hci[l] = V.hAci; hv[l] = V.hAv; }
         else download_csr(V.A, hrp[l], hci[l], hv[l], st);
+    }
+    // node triples: rows 3q, 3q+1, 3q+2 with identical column lists
+    if (S->opt.method == MGX_METHOD_MGPCG && S->opt.format == MGX_FORMAT_CSR) {
+        for (int l = 0; l < L; ++l) {
+            Level &V = S->lv[l];
+            const auto &rp = hrp[l];
+            const auto &ci = hci[l];
+            bool ok = V.n % 3 == 0;
+            for (int32_t q = 0; ok && q < V.n / 3; ++q) {
+                const int32_t s0 = rp[3 * q], len = rp[3 * q + 1] - s0;
+                if (rp[3 * q + 2] - rp[3 * q + 1] != len || rp[3 * q + 3] - rp[3 * q + 2] != len) { ok = false; break; }
+                for (int32_t j = 0; j < len; ++j)
+                    if (ci[s0 + len + j] != ci[s0 + j] || ci[s0 + 2 * len + j] != ci[s0 + j]) { ok = false; break; }
+            }
+            V.node3 = ok;
+        }
     }
     // Jacobi weights
     const int nsmooth = (S->opt.method == MGX_METHOD_PCG_JACOBI) ? 1 : L - 1;
@@ -926,9 +943,26 @@ static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, b
     const T *Av = BlkSel<T>::val(V.A);
     for (int i = 0; i < V.ncolors; ++i) {
         const int c = backward ? V.ncolors - 1 - i : i;
-        launch_gs_block<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done, S->stream);
+        if (V.node3)
+            launch_gs_node<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done,
+                              S->stream);
+        else
+            launch_gs_block<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done,
+                               S->stream);
     }
 }
+
+// Y (op)= A X with the level matrix: node-triple kernel when the level allows it
+template <typename TV, typename TX>
+static void level_spmm(Level &V, int K, int epi, const TV *val, const TX *X, const TX *B, const TV *pre, TX *Y,
+                       const int32_t *done, double *partials, hipStream_t s)
+{
+    if (V.node3)
+        launch_spmm_node<TV, TX>(K, epi, V.n / 3, V.A.rp, V.A.ci, val, X, B, pre, Y, done, partials, kPartialStride, s);
+    else
+        launch_spmm<TV, TX>(K, epi, V.n, V.A.rp, V.A.ci, val, X, B, pre, Y, done, partials, kPartialStride, s);
+}
+static int level_spmm_parts(const Level &V, int K) { return V.node3 ? spmm_node_grid(K, V.n / 3) : spmm_grid(K, V.n); }
 
 // the V-cycle of vcycle() applied to K columns at once
 template <typename T>
@@ -952,11 +986,11 @@ static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const 
         cur = tt;
         for (int i = 1; i < nu; ++i) {
             T *o = (cur == x) ? tt : x;
-            launch_spmm<T, T>(K, BEPI_JACOBI, V.n, V.A.rp, V.A.ci, Av, cur, b, B::pre(V), o, done, nullptr, 0, s);
+            level_spmm<T, T>(V, K, BEPI_JACOBI, Av, cur, b, B::pre(V), o, done, nullptr, s);
             cur = o;
         }
     }
-    launch_spmm<T, T>(K, BEPI_RESID, V.n, V.A.rp, V.A.ci, Av, cur, b, nullptr, B::r(V), done, nullptr, 0, s);
+    level_spmm<T, T>(V, K, BEPI_RESID, Av, cur, b, nullptr, B::r(V), done, nullptr, s);
     Level &C = S->lv[l + 1];
     launch_spmm<T, T>(K, BEPI_SPMM, V.R.nrows, V.R.rp, V.R.ci, B::val(V.R), B::r(V), nullptr, nullptr, B::b(C), done,
                       nullptr, 0, s);
@@ -969,7 +1003,7 @@ static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const 
         T *tt = B::t(V);
         for (int i = 0; i < nu; ++i) {
             T *o = (cur == x) ? tt : x;
-            launch_spmm<T, T>(K, BEPI_JACOBI, V.n, V.A.rp, V.A.ci, Av, cur, b, B::pre(V), o, done, nullptr, 0, s);
+            level_spmm<T, T>(V, K, BEPI_JACOBI, Av, cur, b, B::pre(V), o, done, nullptr, s);
             cur = o;
         }
         if (cur != x) launch_bcopy<T>((int64_t)V.n * K, cur, x, done, s);
@@ -992,10 +1026,9 @@ static void block_iteration(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P,
     const int32_t n = V.n;
     hipStream_t s = S->stream;
     if (prof) MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used], s));
-    launch_spmm<T, T>(K, BEPI_DOT, n, V.A.rp, V.A.ci, BlkSel<T>::val(V.A), P, nullptr, nullptr, Q, &ctl->done,
-                      S->bpartials.p, kPartialStride, s);
+    level_spmm<T, T>(V, K, BEPI_DOT, BlkSel<T>::val(V.A), P, nullptr, nullptr, Q, &ctl->done, S->bpartials.p, s);
     if (prof) { MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used + 1], s)); ++S->prof_used; }
-    launch_bfinalize(spmm_grid(K, n), S->bpartials.p, ctl, FIN_ALPHA, s);
+    launch_bfinalize(level_spmm_parts(V, K), S->bpartials.p, ctl, FIN_ALPHA, s);
     launch_bupdate<T>(K, n, P, Q, X, R, ctl, S->bpartials.p, s);
     vcycle_block<T>(S, K, 0, R, Z, &ctl->done);
     launch_bdot_beta<T>(K, n, R, Z, ctl, S->bpartials.p, false, s);
@@ -1086,8 +1119,7 @@ static void solve_block_fp64(mgx_solver_s *S, int K, int j0, int kc, const doubl
     launch_pack_cols(n, K, kc, b + (size_t)j0 * n, n, S->bB64.p, s);
     if (has_x0) {
         launch_pack_cols(n, K, kc, x + (size_t)j0 * n, n, S->bX64.p, s);
-        launch_spmm<double, double>(K, BEPI_RESID, n, V.A.rp, V.A.ci, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bR64.p,
-                                    nullptr, nullptr, 0, s);
+        level_spmm<double, double>(V, K, BEPI_RESID, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bR64.p, nullptr, nullptr, s);
     } else {
         launch_bcopy<double>(nk, S->bB64.p, S->bR64.p, nullptr, s);
         launch_fill_zero(nk, S->bX64.p, nullptr, s);
@@ -1117,8 +1149,8 @@ static void solve_block_mixed(mgx_solver_s *S, int K, int j0, int kc, const doub
         BCtl outer = block_ctl_init(K, kc, S->opt.stop, S->opt.tolerance, 1);
         for (int j = 0; j < kc; ++j) outer.frozen[j] = failed[j];
         put_ctl(S, S->bctl.p, outer);
-        launch_spmm<double, double>(K, BEPI_RESID, n, V.A.rp, V.A.ci, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bR64.p,
-                                    nullptr, nullptr, 0, s);
+        level_spmm<double, double>(V, K, BEPI_RESID, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bR64.p, nullptr, nullptr,
+                                   s);
         launch_bnorms<double>(K, n, S->bB64.p, S->bR64.p, S->bctl.p, S->bpartials.p, s);
         MGX_HIP(hipMemcpyAsync(S->bctl_host, S->bctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, s));
         MGX_HIP(hipStreamSynchronize(s));
@@ -1152,8 +1184,7 @@ static void finish_stats_block(mgx_solver_s *S, int K, int kc, mgx_stats *st)
     hipStream_t s = S->stream;
     Level &V = S->lv[0];
     const int32_t n = V.n;
-    launch_spmm<double, double>(K, BEPI_RESID, n, V.A.rp, V.A.ci, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bQ64.p,
-                                nullptr, nullptr, 0, s);
+    level_spmm<double, double>(V, K, BEPI_RESID, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bQ64.p, nullptr, nullptr, s);
     put_ctl(S, S->bctl_stat.p, block_ctl_init(K, kc, MGX_STOP_REL_L2, 0.0, 1));
     launch_bnorms<double>(K, n, S->bB64.p, S->bQ64.p, S->bctl_stat.p, S->bpartials.p, s);
     MGX_HIP(hipMemcpyAsync(S->bctl_host, S->bctl_stat.p, sizeof(BCtl), hipMemcpyDeviceToHost, s));
@@ -1864,7 +1895,7 @@ int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, cons
     });
 }
 
-int mgx_time_spmm(int32_t fp32, int32_t k, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
+int mgx_time_spmm(int32_t flags, int32_t k, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
                   const void *X, void *Y, int32_t warmup, int32_t repeats, double *avg_ms)
 {
     return guard([&] {
@@ -1872,16 +1903,35 @@ int mgx_time_spmm(int32_t fp32, int32_t k, int32_t nrows, const int32_t *rowptr,
         MGX_REQUIRE(col && val && X && Y && avg_ms && repeats >= 1 && warmup >= 0, MGX_ERR_INVALID_ARGUMENT,
                     "invalid arguments");
         MGX_REQUIRE(k == 4 || k == 8 || k == 16 || k == 32, MGX_ERR_INVALID_ARGUMENT, "k must be 4, 8, 16 or 32");
+        const bool fp32 = flags & 1, node = flags & 2;
+        if (node) {
+            // the node-triple kernel reads len(3q) entries of rows 3q+1 and 3q+2: check the lengths
+            MGX_REQUIRE(nrows % 3 == 0, MGX_ERR_INVALID_ARGUMENT, "node-triple SpMM needs nrows % 3 == 0");
+            std::vector<int32_t> rp(nrows + 1);
+            MGX_HIP(hipMemcpy(rp.data(), rowptr, sizeof(int32_t) * rp.size(), hipMemcpyDeviceToHost));
+            for (int32_t q = 0; q < nrows / 3; ++q)
+                MGX_REQUIRE(rp[3 * q + 1] - rp[3 * q] == rp[3 * q + 2] - rp[3 * q + 1] &&
+                                rp[3 * q + 2] - rp[3 * q + 1] == rp[3 * q + 3] - rp[3 * q + 2],
+                            MGX_ERR_INVALID_ARGUMENT, "rows of a node triple differ in length");
+        }
         hipStream_t s;
         MGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         hipEvent_t e0, e1;
         MGX_HIP(hipEventCreate(&e0));
         MGX_HIP(hipEventCreate(&e1));
         auto one = [&]() {
-            if (fp32) launch_spmm<float, float>(k, BEPI_SPMM, nrows, rowptr, col, (const float *)val, (const float *)X,
-                                                nullptr, nullptr, (float *)Y, nullptr, nullptr, 0, s);
-            else launch_spmm<double, double>(k, BEPI_SPMM, nrows, rowptr, col, (const double *)val, (const double *)X,
-                                             nullptr, nullptr, (double *)Y, nullptr, nullptr, 0, s);
+            if (fp32 && node)
+                launch_spmm_node<float, float>(k, BEPI_SPMM, nrows / 3, rowptr, col, (const float *)val, (const float *)X,
+                                               nullptr, nullptr, (float *)Y, nullptr, nullptr, 0, s);
+            else if (node)
+                launch_spmm_node<double, double>(k, BEPI_SPMM, nrows / 3, rowptr, col, (const double *)val,
+                                                 (const double *)X, nullptr, nullptr, (double *)Y, nullptr, nullptr, 0, s);
+            else if (fp32)
+                launch_spmm<float, float>(k, BEPI_SPMM, nrows, rowptr, col, (const float *)val, (const float *)X,
+                                          nullptr, nullptr, (float *)Y, nullptr, nullptr, 0, s);
+            else
+                launch_spmm<double, double>(k, BEPI_SPMM, nrows, rowptr, col, (const double *)val, (const double *)X,
+                                            nullptr, nullptr, (double *)Y, nullptr, nullptr, 0, s);
         };
         for (int i = 0; i < warmup; ++i) one();
         MGX_HIP(hipEventRecord(e0, s));

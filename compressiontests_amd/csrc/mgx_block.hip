@@ -174,6 +174,236 @@ template void launch_spmm<double, float>(int, int, int32_t, const int32_t *, con
                                          const float *, const float *, const double *, float *, const int32_t *,
                                          double *, int, hipStream_t);
 
+
+// ------------------------------------------------------------------ node-triple kernels
+// For matrices whose rows come in node triples (rows 3q, 3q+1, 3q+2 with identical column lists,
+// as every level matrix of the 3-dof hexahedral hierarchy), each gathered segment X[col*K ..]
+// serves all three rows: the X traffic through the caches drops threefold against the row
+// kernels.  Lanes as in k_spmm, with G lanes per node instead of per row.
+template <int K> struct NShape {
+    static constexpr int S = K >= 16 ? 1 : 16 / K;
+    static constexpr int G = K * S;
+    static constexpr int NPB = kBlock / G;          // nodes per tile
+    static constexpr int CAP = NPB * 3 * 84;        // staged entries per tile
+    static constexpr int U = S == 1 ? 8 : 4;
+};
+
+template <int K, typename TV, typename TX, typename PI, typename PV>
+__device__ __forceinline__ void node_cols(PI cis, PV vs, int o0, int o1, int o2, int len, const TX *__restrict__ X,
+                                          int c, int sl, TV &y0, TV &y1, TV &y2)
+{
+    using Sh = NShape<K>;
+    for (int j = sl; j < len; j += Sh::U * Sh::S) {
+        int cc[Sh::U];
+        TV w0[Sh::U], w1[Sh::U], w2[Sh::U];
+#pragma unroll
+        for (int u = 0; u < Sh::U; ++u) {
+            const int jj = j + u * Sh::S;
+            if (jj < len) { cc[u] = cis[o0 + jj]; w0[u] = vs[o0 + jj]; w1[u] = vs[o1 + jj]; w2[u] = vs[o2 + jj]; }
+            else { cc[u] = -1; w0[u] = w1[u] = w2[u] = 0; }
+        }
+        TX xv[Sh::U];
+#pragma unroll
+        for (int u = 0; u < Sh::U; ++u) xv[u] = cc[u] >= 0 ? X[(size_t)cc[u] * K + c] : TX(0);
+#pragma unroll
+        for (int u = 0; u < Sh::U; ++u) {
+            const TV xx = (TV)xv[u];
+            y0 += w0[u] * xx; y1 += w1[u] * xx; y2 += w2[u] * xx;
+        }
+    }
+}
+
+template <typename TV, typename TX, int K, int EPI>
+__global__ __launch_bounds__(kBlock) void k_spmm_node(int32_t nnodes, const int32_t *__restrict__ rp,
+                                                      const int32_t *__restrict__ ci, const TV *__restrict__ v,
+                                                      const TX *__restrict__ X, const TX *__restrict__ B,
+                                                      const TV *__restrict__ pre, TX *__restrict__ Y,
+                                                      const int32_t *__restrict__ done, double *__restrict__ partials,
+                                                      int stride)
+{
+    using Sh = NShape<K>;
+    __shared__ int32_t s_ci[Sh::CAP];
+    __shared__ TV s_v[Sh::CAP];
+    if (done && *done) return;
+    const int t = threadIdx.x, g = t / Sh::G, l = t % Sh::G, c = l % K, sl = l / K;
+    const int ntiles = (nnodes + Sh::NPB - 1) / Sh::NPB;
+    const int per = gridDim.x >> 3, xcd = blockIdx.x & 7;
+    const int hi = (int)((int64_t)ntiles * (xcd + 1) / 8);
+    double dot = 0;
+    for (int tile = (int)((int64_t)ntiles * xcd / 8) + (int)(blockIdx.x >> 3); tile < hi; tile += per) {
+        const int q0 = tile * Sh::NPB, q1 = min(q0 + Sh::NPB, nnodes);
+        const int e0 = rp[3 * q0], e1 = rp[3 * q1];
+        const bool staged = e1 - e0 <= Sh::CAP;     // the same for the whole block
+        if (staged) {
+            __syncthreads();
+            for (int k = t; k < e1 - e0; k += kBlock) { s_ci[k] = ntload(ci + e0 + k); s_v[k] = ntload(v + e0 + k); }
+            __syncthreads();
+        }
+        const int q = q0 + g;
+        if (q < q1) {
+            const int r = 3 * q;
+            const int s0 = rp[r], s1 = rp[r + 1], s2 = rp[r + 2];
+            TV y[3] = {0, 0, 0};
+            if (staged) node_cols<K, TV, TX>(s_ci, s_v, s0 - e0, s1 - e0, s2 - e0, s1 - s0, X, c, sl, y[0], y[1], y[2]);
+            else node_cols<K, TV, TX>(ci, v, s0, s1, s2, s1 - s0, X, c, sl, y[0], y[1], y[2]);
+#pragma unroll
+            for (int o = K; o < Sh::G; o <<= 1) {
+                y[0] += __shfl_xor(y[0], o, Sh::G);
+                y[1] += __shfl_xor(y[1], o, Sh::G);
+                y[2] += __shfl_xor(y[2], o, Sh::G);
+            }
+            if (sl == 0) {
+#pragma unroll
+                for (int a = 0; a < 3; ++a) {
+                    const size_t i = (size_t)(r + a) * K + c;
+                    const TV acc = y[a];
+                    if (EPI == BEPI_SPMM) Y[i] = (TX)acc;
+                    else if (EPI == BEPI_ADD) Y[i] = (TX)((TV)Y[i] + acc);
+                    else if (EPI == BEPI_RESID) Y[i] = (TX)((TV)B[i] - acc);
+                    else if (EPI == BEPI_JACOBI) Y[i] = (TX)((TV)X[i] + pre[r + a] * ((TV)B[i] - acc));
+                    else {
+                        const TX qv = (TX)acc;
+                        Y[i] = qv;
+                        dot += (double)X[i] * (double)qv;
+                    }
+                }
+            }
+        }
+    }
+    if (EPI == BEPI_DOT) publish_cols<K, 1, -1>(dot, 0, 0, partials, stride);
+}
+
+int spmm_node_grid(int K, int32_t nnodes)
+{
+    const int npb = K == 32 ? NShape<32>::NPB : NShape<16>::NPB;   // 8 or 16 nodes per tile
+    return grid_tiles((nnodes + npb - 1) / npb);
+}
+
+template <typename TV, typename TX>
+void launch_spmm_node(int K, int epi, int32_t nnodes, const int32_t *rp, const int32_t *ci, const TV *v, const TX *X,
+                      const TX *B, const TV *pre, TX *Y, const int32_t *done, double *partials, int stride, hipStream_t s)
+{
+    if (nnodes <= 0) return;
+    const int g = spmm_node_grid(K, nnodes);
+#define MGX_SPMMN(KK, EE)                                                                                       \
+    hipLaunchKernelGGL((k_spmm_node<TV, TX, KK, EE>), dim3(g), dim3(kBlock), 0, s, nnodes, rp, ci, v, X, B, pre, Y, \
+                       done, partials, stride)
+#define MGX_SPMMN_K(KK)                                       \
+    switch (epi) {                                            \
+    case BEPI_SPMM: MGX_SPMMN(KK, BEPI_SPMM); break;          \
+    case BEPI_ADD: MGX_SPMMN(KK, BEPI_ADD); break;            \
+    case BEPI_RESID: MGX_SPMMN(KK, BEPI_RESID); break;        \
+    case BEPI_JACOBI: MGX_SPMMN(KK, BEPI_JACOBI); break;      \
+    default: MGX_SPMMN(KK, BEPI_DOT); break;                  \
+    }
+    switch (K) {
+    case 4: MGX_SPMMN_K(4); break;
+    case 8: MGX_SPMMN_K(8); break;
+    case 16: MGX_SPMMN_K(16); break;
+    case 32: MGX_SPMMN_K(32); break;
+    default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
+    }
+#undef MGX_SPMMN_K
+#undef MGX_SPMMN
+    MGX_HIP(hipGetLastError());
+}
+template void launch_spmm_node<double, double>(int, int, int32_t, const int32_t *, const int32_t *, const double *,
+                                               const double *, const double *, const double *, double *,
+                                               const int32_t *, double *, int, hipStream_t);
+template void launch_spmm_node<float, float>(int, int, int32_t, const int32_t *, const int32_t *, const float *,
+                                             const float *, const float *, const float *, float *, const int32_t *,
+                                             double *, int, hipStream_t);
+
+// Multicolour GS step on node triples: thread (node, s, c) walks every S-th entry of the node's
+// three rows for column c, gathering X once for the three rows; the node's 3x3 diagonal block
+// comes from the same pass.  No LDS and no block barrier.
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_gs_node(int32_t nnodes, const int32_t *__restrict__ nodes,
+                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                    const T *__restrict__ v, const T *__restrict__ B,
+                                                    T *__restrict__ X, int backward, const int32_t *__restrict__ done)
+{
+    using Sh = NShape<K>;
+    constexpr int S = Sh::S, G = Sh::G, N = kBlock / G, U = Sh::U;
+    if (done && *done) return;
+    const int t = threadIdx.x, l = t % G, c = l % K, sl = l / K;
+    const int nd = xcd_remap(blockIdx.x, gridDim.x) * N + t / G;
+    if (nd >= nnodes) return;                      // the whole G-lane group leaves together
+    const int q3 = 3 * nodes[nd];
+    const int s0 = rp[q3], s1 = rp[q3 + 1], s2 = rp[q3 + 2], len = s1 - s0;
+    T a0 = 0, a1 = 0, a2 = 0;
+    T d00 = 0, d01 = 0, d02 = 0, d10 = 0, d11 = 0, d12 = 0, d20 = 0, d21 = 0, d22 = 0;
+    for (int j = sl; j < len; j += U * S) {
+        int cc[U];
+        T w0[U], w1[U], w2[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int jj = j + u * S;
+            if (jj < len) { cc[u] = ci[s0 + jj]; w0[u] = v[s0 + jj]; w1[u] = v[s1 + jj]; w2[u] = v[s2 + jj]; }
+            else { cc[u] = -1; w0[u] = w1[u] = w2[u] = 0; }
+        }
+        T xv[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const unsigned off = (unsigned)(cc[u] - q3);
+            xv[u] = (cc[u] >= 0 && off >= 3u) ? X[(size_t)cc[u] * K + c] : T(0);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (cc[u] < 0) continue;
+            const unsigned off = (unsigned)(cc[u] - q3);
+            if (off >= 3u) { a0 += w0[u] * xv[u]; a1 += w1[u] * xv[u]; a2 += w2[u] * xv[u]; }
+            else if (off == 0) { d00 += w0[u]; d10 += w1[u]; d20 += w2[u]; }
+            else if (off == 1) { d01 += w0[u]; d11 += w1[u]; d21 += w2[u]; }
+            else { d02 += w0[u]; d12 += w1[u]; d22 += w2[u]; }
+        }
+    }
+#pragma unroll
+    for (int o = K; o < G; o <<= 1) {
+        a0 += __shfl_xor(a0, o, G); a1 += __shfl_xor(a1, o, G); a2 += __shfl_xor(a2, o, G);
+        d00 += __shfl_xor(d00, o, G); d01 += __shfl_xor(d01, o, G); d02 += __shfl_xor(d02, o, G);
+        d10 += __shfl_xor(d10, o, G); d11 += __shfl_xor(d11, o, G); d12 += __shfl_xor(d12, o, G);
+        d20 += __shfl_xor(d20, o, G); d21 += __shfl_xor(d21, o, G); d22 += __shfl_xor(d22, o, G);
+    }
+    if (sl != 0) return;
+    const size_t i0 = (size_t)q3 * K + c;
+    T x0 = X[i0], x1 = X[i0 + K], x2 = X[i0 + 2 * K];
+    const T b0 = B[i0] - a0, b1 = B[i0 + K] - a1, b2 = B[i0 + 2 * K] - a2;
+    if (!backward) {
+        x0 = (b0 - d01 * x1 - d02 * x2) / d00;
+        x1 = (b1 - d10 * x0 - d12 * x2) / d11;
+        x2 = (b2 - d20 * x0 - d21 * x1) / d22;
+    } else {
+        x2 = (b2 - d20 * x0 - d21 * x1) / d22;
+        x1 = (b1 - d10 * x0 - d12 * x2) / d11;
+        x0 = (b0 - d01 * x1 - d02 * x2) / d00;
+    }
+    X[i0] = x0; X[i0 + K] = x1; X[i0 + 2 * K] = x2;
+}
+
+template <typename T>
+void launch_gs_node(int K, int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const T *v,
+                    const T *B, T *X, bool backward, const int32_t *done, hipStream_t s)
+{
+    if (nnodes <= 0) return;
+#define MGX_GSN(KK)                                                                                             \
+    hipLaunchKernelGGL((k_gs_node<T, KK>), dim3(grid_rows(nnodes, kBlock / NShape<KK>::G)), dim3(kBlock), 0, s, \
+                       nnodes, nodes, rp, ci, v, B, X, backward ? 1 : 0, done)
+    switch (K) {
+    case 4: MGX_GSN(4); break;
+    case 8: MGX_GSN(8); break;
+    case 16: MGX_GSN(16); break;
+    case 32: MGX_GSN(32); break;
+    default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
+    }
+#undef MGX_GSN
+    MGX_HIP(hipGetLastError());
+}
+template void launch_gs_node<double>(int, int32_t, const int32_t *, const int32_t *, const int32_t *, const double *,
+                                     const double *, double *, bool, const int32_t *, hipStream_t);
+template void launch_gs_node<float>(int, int32_t, const int32_t *, const int32_t *, const int32_t *, const float *,
+                                    const float *, float *, bool, const int32_t *, hipStream_t);
+
 // ------------------------------------------------------------------ multicolour GS, K columns
 // A block of 192 threads relaxes N nodes: thread (m, a, s, c) reduces row a of node m for
 // column c over the entries s, s + S, ...: sa = sum over columns outside the node, and the

@@ -128,6 +128,15 @@ constexpr size_t kBlockPartials = (size_t)3 * kMaxBlock * kPartialStride;
 template <typename TV, typename TX>
 void launch_spmm(int K, int epi, int32_t nrows, const int32_t *rp, const int32_t *ci, const TV *v, const TX *X,
                  const TX *B, const TV *pre, TX *Y, const int32_t *done, double *partials, int stride, hipStream_t s);
+// node-triple forms (rows 3q..3q+2 share their column list): nnodes = nrows / 3
+template <typename TV, typename TX>
+void launch_spmm_node(int K, int epi, int32_t nnodes, const int32_t *rp, const int32_t *ci, const TV *v, const TX *X,
+                      const TX *B, const TV *pre, TX *Y, const int32_t *done, double *partials, int stride,
+                      hipStream_t s);
+int spmm_node_grid(int K, int32_t nnodes);
+template <typename T>
+void launch_gs_node(int K, int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const T *v,
+                    const T *B, T *X, bool backward, const int32_t *done, hipStream_t s);
 template <typename T>
 void launch_gs_block(int K, int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const T *v,
                      const T *B, T *X, bool backward, const int32_t *done, hipStream_t s);

@@ -291,8 +291,10 @@ int mgx_device_synchronize(void);
 int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
                         const void *x, const void *b, const void *pre, void *y, int32_t warmup, int32_t repeats,
                         double *avg_ms);
-/* Same for mgx_csr_spmm (fp32 = 0) / mgx_csr_spmm_f32 (fp32 = 1) with k right-hand sides. */
-int mgx_time_spmm(int32_t fp32, int32_t k, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
+/* Same for the block SpMM Y = A X with k right-hand sides.  flags bit 0: FP32 values and vectors
+ * (mgx_csr_spmm_f32) instead of FP64; bit 1: the node-triple kernel of the block solver (rows
+ * 3q..3q+2 share one column list; triples of unequal length are rejected). */
+int mgx_time_spmm(int32_t flags, int32_t k, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
                   const void *X, void *Y, int32_t warmup, int32_t repeats, double *avg_ms);
 /* Average device time of a plain copy kernel (16-byte vector loads and stores) moving `bytes`
  * (a multiple of 16) from src to dst: the calibration of HBM byte counters. */

@@ -68,6 +68,13 @@ def test_spmm_matches_columnwise_products(het16, k):
     Y32 = DeviceArray(A.nrows * k, np.float32)
     kernels.spmm(D32, X32, Y32, k)
     assert np.max(np.abs(Y32.numpy().reshape(A.nrows, k) - ref)) <= 1e-5 * np.max(np.abs(ref))
+    # the node-triple kernel of the block solver (K_ff rows come in triples with equal columns)
+    Yn = DeviceArray(A.nrows * k)
+    kernels.time_spmm(D64, X, Yn, k, warmup=0, repeats=1, node=True)
+    assert np.max(np.abs(Yn.numpy().reshape(A.nrows, k) - ref)) <= 1e-12 * np.max(np.abs(ref))
+    Yn32 = DeviceArray(A.nrows * k, np.float32)
+    kernels.time_spmm(D32, X32, Yn32, k, warmup=0, repeats=1, node=True)
+    assert np.max(np.abs(Yn32.numpy().reshape(A.nrows, k) - ref)) <= 1e-5 * np.max(np.abs(ref))
 
 
 def test_load_cases_shape_and_norm(het16):
