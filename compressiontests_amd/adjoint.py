@@ -23,14 +23,14 @@ def simp_moduli(rho, penal=3.0, Emin=1e-9, E0=1.0):
 
 
 def compliance_and_gradient(dims, rho, penal=3.0, Emin=1e-9, E0=1.0, nu=0.3, levels=3, tolerance=1e-12,
-                            smoother="gs", nu_smooth=1, load=None):
+                            smoother="gs", nu_smooth=1, load=None, precision="fp64"):
     """Returns (C, dC/drho, u, stats) for the x = 0 clamped cantilever with the tip load (or
-    `load`, a free-dof vector)."""
+    `load`, a free-dof vector).  precision "mixed": the solve is the FP64-refined FP32 MG-PCG."""
     nx, ny, nz = dims
     model = FemCantilever3D(dims, ElementDensities=simp_moduli(rho, penal, Emin, E0), PoissonRatio=nu, rhs="tip")
     f = model.rhs() if load is None else np.ascontiguousarray(load, np.float64)
     s = HipMgPcgSolver(model, levels=levels, smoother=smoother, nu=nu_smooth, tolerance=tolerance,
-                       max_iterations=2000)
+                       max_iterations=2000, precision=precision)
     s.Initialize(f)
     u, st = s.Solve()
     C = float(f @ u)

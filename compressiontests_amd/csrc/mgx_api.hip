@@ -263,6 +263,7 @@ struct mgx_solver_s {
     mgx::DBuf<mgx::BCtl> bctl, bctl_in, bctl_stat;
     mgx::BCtl *bctl_host = nullptr;        // pinned
     mgx::DBuf<double> bpartials;
+    mgx::DBuf<double> bdense;              // split sums of the coarse solve
     hipGraphExec_t bgexec = nullptr;       // captured chunk of block PCG iterations
     int64_t bgraph_key = -1;
     double bprof_ms = 0;                   // options.profile: level-0 SpMM time of the current block
@@ -851,6 +852,8 @@ static void vcycle32(mgx_solver_s *S, int l, const float *b, float *x, const int
     }
 }
 
+constexpr int kStagnationSteps = 3;
+
 // FP64 iterative refinement around an FP32 MG-PCG: per outer step the true residual
 // r = b - A x in FP64, an inner PCG on A d = r with FP32 SpMV / V-cycle / vectors and FP64
 // dots, alpha and beta (relative reduction inner_tolerance, at most inner_iterations), then
@@ -871,7 +874,8 @@ static void solve_mixed(mgx_solver_s *S, const double *b, double *x, bool has_x0
     inner.max_it = S->opt.inner_iterations;
     if (!has_x0) launch_fill_zero(n, x, nullptr, s);
     const int chunk = S->check_every;
-    int total_inner = 0, outer_steps = 0, status = 0;
+    int total_inner = 0, outer_steps = 0, status = 0, stalled = 0;
+    double best_r2 = INFINITY;
     for (;;) {
         *S->ctl_host = outer;
         MGX_HIP(hipMemcpyAsync(co, S->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, s));
@@ -881,6 +885,10 @@ static void solve_mixed(mgx_solver_s *S, const double *b, double *x, bool has_x0
         MGX_HIP(hipStreamSynchronize(s));
         if (S->ctl_host->done) { status = S->ctl_host->status; break; }
         if (outer_steps >= S->opt.max_iterations) { status = 0; break; }
+        // stagnation: the FP64 true residual has a floor (rounding of b - A x); stop with
+        // status 0 once kStagnationSteps refinement steps in a row fail to halve its norm
+        if (S->ctl_host->rnorm2 < 0.25 * best_r2) { best_r2 = S->ctl_host->rnorm2; stalled = 0; }
+        else if (++stalled >= kStagnationSteps) { status = 0; break; }
         // inner FP32 PCG on A d = r
         const int32_t *done = &ci->done;
         launch_convert_f64_f32(n, S->r.p, S->r32.p, nullptr, s);
@@ -972,7 +980,7 @@ static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const 
     const int L = (int)S->lv.size();
     Level &V = S->lv[l];
     hipStream_t s = S->stream;
-    if (l == L - 1) { launch_dense_block<T>(K, V.n, S->coarse_inv.p, b, x, done, s); return; }
+    if (l == L - 1) { launch_dense_block<T>(K, V.n, S->coarse_inv.p, b, x, S->bdense.p, done, s); return; }
     const int nu = S->opt.level_iterations;
     const T *Av = B::val(V.A);
     T *cur;
@@ -1143,11 +1151,12 @@ static void solve_block_mixed(mgx_solver_s *S, int K, int j0, int kc, const doub
     launch_pack_cols(n, K, kc, b + (size_t)j0 * n, n, S->bB64.p, s);
     if (has_x0) launch_pack_cols(n, K, kc, x + (size_t)j0 * n, n, S->bX64.p, s);
     else launch_fill_zero(nk, S->bX64.p, nullptr, s);
-    std::vector<int> iters(K, 0), outer_n(K, 0), status(K, 0), failed(K, 0);
+    std::vector<int> iters(K, 0), outer_n(K, 0), status(K, 0), failed(K, 0), stalled(K, 0);
+    std::vector<double> best_r2(K, INFINITY);
     int steps = 0;
     for (;;) {
         BCtl outer = block_ctl_init(K, kc, S->opt.stop, S->opt.tolerance, 1);
-        for (int j = 0; j < kc; ++j) outer.frozen[j] = failed[j];
+        for (int j = 0; j < kc; ++j) outer.frozen[j] = failed[j] != 0;
         put_ctl(S, S->bctl.p, outer);
         level_spmm<double, double>(V, K, BEPI_RESID, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bR64.p, nullptr, nullptr,
                                    s);
@@ -1157,8 +1166,18 @@ static void solve_block_mixed(mgx_solver_s *S, int K, int j0, int kc, const doub
         const BCtl o = *S->bctl_host;
         for (int j = 0; j < kc; ++j) if (!failed[j]) status[j] = o.status[j];
         if (o.done || steps >= S->opt.max_iterations) break;
+        // per-column stagnation of the FP64 true residual (see solve_mixed): such a column stops
+        // with status 0 and is frozen from then on
+        bool active = false;
+        for (int j = 0; j < kc; ++j) {
+            if (o.cdone[j] || failed[j]) continue;
+            if (o.rnorm2[j] < 0.25 * best_r2[j]) { best_r2[j] = o.rnorm2[j]; stalled[j] = 0; }
+            else if (++stalled[j] >= kStagnationSteps) failed[j] = 2;
+            if (failed[j] != 2) active = true;
+        }
+        if (!active) break;
         BCtl in = block_ctl_init(K, kc, MGX_STOP_REL_L2, S->opt.inner_tolerance, S->opt.inner_iterations);
-        for (int j = 0; j < K; ++j) in.frozen[j] = o.cdone[j];
+        for (int j = 0; j < K; ++j) in.frozen[j] = o.cdone[j] || (j < kc && failed[j]);
         put_ctl(S, S->bctl_in.p, in);
         launch_convert_f64_f32(nk, S->bR64.p, S->bR32.p, nullptr, s);
         launch_fill_zero_f32(nk, S->bD32.p, nullptr, s);
@@ -1167,10 +1186,10 @@ static void solve_block_mixed(mgx_solver_s *S, int K, int j0, int kc, const doub
         block_loop<float>(S, K, S->bctl_in.p, S->bD32.p, S->bR32.p, S->bP32.p, S->bQ32.p, S->bZ32.p, 1, in.max_it);
         const BCtl &ci = *S->bctl_host;
         for (int j = 0; j < kc; ++j) {
-            if (o.cdone[j]) continue;
+            if (o.cdone[j] || failed[j]) continue;
             iters[j] += ci.iter[j];
             ++outer_n[j];
-            if (ci.status[j] == -1) { failed[j] = 1; status[j] = -1; }
+            if (ci.status[j] == -1) { failed[j] = 1; status[j] = -1; }   // 1: NaN / overflow, 2: stagnated
         }
         launch_axpy_f32_into_f64(nk, S->bD32.p, S->bX64.p, nullptr, s);
         ++steps;
@@ -1243,6 +1262,7 @@ static void alloc_block(mgx_solver_s *S, int K)
     if (mixed) { S->bD32.alloc(nk); S->bR32.alloc(nk); S->bP32.alloc(nk); S->bQ32.alloc(nk); S->bZ32.alloc(nk); }
     else { S->bP64.alloc(nk); S->bZ64.alloc(nk); }
     S->bpartials.ensure(kBlockPartials);
+    S->bdense.ensure(dense_block_scratch(S->lv[L - 1].n, K));
     S->bctl.ensure(1); S->bctl_in.ensure(1); S->bctl_stat.ensure(1);
     if (S->bgexec) { MGX_HIP(hipGraphExecDestroy(S->bgexec)); S->bgexec = nullptr; }
     S->bgraph_key = -1;

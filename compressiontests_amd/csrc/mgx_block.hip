@@ -314,32 +314,25 @@ template void launch_spmm_node<float, float>(int, int, int32_t, const int32_t *,
                                              const float *, const float *, const float *, float *, const int32_t *,
                                              double *, int, hipStream_t);
 
-// Multicolour GS step on node triples: thread (node, s, c) walks every S-th entry of the node's
-// three rows for column c, gathering X once for the three rows; the node's 3x3 diagonal block
-// comes from the same pass.  No LDS and no block barrier.
-template <typename T, int K>
-__global__ __launch_bounds__(kBlock) void k_gs_node(int32_t nnodes, const int32_t *__restrict__ nodes,
-                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                    const T *__restrict__ v, const T *__restrict__ B,
-                                                    T *__restrict__ X, int backward, const int32_t *__restrict__ done)
-{
-    using Sh = NShape<K>;
-    constexpr int S = Sh::S, G = Sh::G, N = kBlock / G, U = Sh::U;
-    if (done && *done) return;
-    const int t = threadIdx.x, l = t % G, c = l % K, sl = l / K;
-    const int nd = xcd_remap(blockIdx.x, gridDim.x) * N + t / G;
-    if (nd >= nnodes) return;                      // the whole G-lane group leaves together
-    const int q3 = 3 * nodes[nd];
-    const int s0 = rp[q3], s1 = rp[q3 + 1], s2 = rp[q3 + 2], len = s1 - s0;
+template <typename T> struct GsAcc {
     T a0 = 0, a1 = 0, a2 = 0;
     T d00 = 0, d01 = 0, d02 = 0, d10 = 0, d11 = 0, d12 = 0, d20 = 0, d21 = 0, d22 = 0;
+};
+
+// one lane's share of a node triple for GS: entries j = sl, sl + S, ... of the three rows (at
+// offsets 0, o1, o2 of cis / vs) for column c
+template <int K, typename T, typename PI, typename PV>
+__device__ __forceinline__ void gs_node_accum(PI cis, PV vs, int o1, int o2, int len, int q3, const T *__restrict__ X,
+                                              int c, int sl, GsAcc<T> &A)
+{
+    constexpr int S = NShape<K>::S, U = NShape<K>::U;
     for (int j = sl; j < len; j += U * S) {
         int cc[U];
         T w0[U], w1[U], w2[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int jj = j + u * S;
-            if (jj < len) { cc[u] = ci[s0 + jj]; w0[u] = v[s0 + jj]; w1[u] = v[s1 + jj]; w2[u] = v[s2 + jj]; }
+            if (jj < len) { cc[u] = cis[jj]; w0[u] = vs[jj]; w1[u] = vs[o1 + jj]; w2[u] = vs[o2 + jj]; }
             else { cc[u] = -1; w0[u] = w1[u] = w2[u] = 0; }
         }
         T xv[U];
@@ -352,12 +345,37 @@ __global__ __launch_bounds__(kBlock) void k_gs_node(int32_t nnodes, const int32_
         for (int u = 0; u < U; ++u) {
             if (cc[u] < 0) continue;
             const unsigned off = (unsigned)(cc[u] - q3);
-            if (off >= 3u) { a0 += w0[u] * xv[u]; a1 += w1[u] * xv[u]; a2 += w2[u] * xv[u]; }
-            else if (off == 0) { d00 += w0[u]; d10 += w1[u]; d20 += w2[u]; }
-            else if (off == 1) { d01 += w0[u]; d11 += w1[u]; d21 += w2[u]; }
-            else { d02 += w0[u]; d12 += w1[u]; d22 += w2[u]; }
+            if (off >= 3u) { A.a0 += w0[u] * xv[u]; A.a1 += w1[u] * xv[u]; A.a2 += w2[u] * xv[u]; }
+            else if (off == 0) { A.d00 += w0[u]; A.d10 += w1[u]; A.d20 += w2[u]; }
+            else if (off == 1) { A.d01 += w0[u]; A.d11 += w1[u]; A.d21 += w2[u]; }
+            else { A.d02 += w0[u]; A.d12 += w1[u]; A.d22 += w2[u]; }
         }
     }
+}
+
+// Multicolour GS step on node triples: thread (node, s, c) walks every S-th entry of the node's
+// three rows for column c, gathering X once for the three rows; the node's 3x3 diagonal block
+// comes from the same pass.  No LDS and no block barrier (staging the node rows through LDS was
+// measured slower: the kernel waits on the X gathers, not on the matrix loads).
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_gs_node(int32_t nnodes, const int32_t *__restrict__ nodes,
+                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                    const T *__restrict__ v, const T *__restrict__ B,
+                                                    T *__restrict__ X, int backward, const int32_t *__restrict__ done)
+{
+    using Sh = NShape<K>;
+    constexpr int G = Sh::G, N = kBlock / G;
+    if (done && *done) return;
+    const int t = threadIdx.x, l = t % G, c = l % K, sl = l / K;
+    const int nd = xcd_remap(blockIdx.x, gridDim.x) * N + t / G;
+    if (nd >= nnodes) return;                      // the whole G-lane group leaves together
+    const int q3 = 3 * nodes[nd];
+    const int s0 = rp[q3], s1 = rp[q3 + 1], s2 = rp[q3 + 2], len = s1 - s0;
+    GsAcc<T> acc;
+    gs_node_accum<K>(ci + s0, v + s0, s1 - s0, s2 - s0, len, q3, X, c, sl, acc);
+    T a0 = acc.a0, a1 = acc.a1, a2 = acc.a2;
+    T d00 = acc.d00, d01 = acc.d01, d02 = acc.d02, d10 = acc.d10, d11 = acc.d11, d12 = acc.d12;
+    T d20 = acc.d20, d21 = acc.d21, d22 = acc.d22;
 #pragma unroll
     for (int o = K; o < G; o <<= 1) {
         a0 += __shfl_xor(a0, o, G); a1 += __shfl_xor(a1, o, G); a2 += __shfl_xor(a2, o, G);
@@ -747,25 +765,29 @@ MGX_INST_T(float)
 
 // ------------------------------------------------------------------ dense coarse solve, K columns
 // Y = M X with M the dense FP64 inverse of the coarsest matrix (nc x nc, row-major) and X, Y
-// nc x K blocks.  A block takes kDenseRows rows; thread (slice, c) accumulates column c over the
-// inner indices j = slice, slice + J, ... for all of the block's rows (one X load feeds
-// kDenseRows FMAs), and the J slices are summed in a fixed order through LDS.
-constexpr int kDenseRows = 8;
+// nc x K blocks, split over the inner index: block (row tile, split) takes kDenseRows rows and
+// the inner indices of one of kDenseSplit equal slices; thread (slice, c) accumulates column c
+// over j = slice, slice + J, ... of that range for all of the tile's rows (one X load feeds
+// kDenseRows FMAs); the J slices are summed in LDS and the kDenseSplit split sums are added in a
+// fixed order by a second kernel.  Deterministic.
+constexpr int kDenseRows = 8, kDenseSplit = 8;
 
 template <typename TX, int K>
-__global__ __launch_bounds__(kBlock) void k_dense_block(int32_t nc, const double *__restrict__ M,
-                                                        const TX *__restrict__ X, TX *__restrict__ Y,
+__global__ __launch_bounds__(kBlock) void k_dense_split(int32_t nc, const double *__restrict__ M,
+                                                        const TX *__restrict__ X, double *__restrict__ part,
                                                         const int32_t *__restrict__ done)
 {
     constexpr int J = kBlock / K;
     __shared__ double red[kDenseRows][J][K];
     if (done && *done) return;
-    const int r0 = blockIdx.x * kDenseRows, t = threadIdx.x, c = t % K, sl = t / K;
+    const int r0 = blockIdx.x * kDenseRows, sp = blockIdx.y, t = threadIdx.x, c = t % K, sl = t / K;
     const int nr = min(kDenseRows, nc - r0);
+    const int chunk = (nc + kDenseSplit - 1) / kDenseSplit;
+    const int j0 = sp * chunk, j1 = min(nc, j0 + chunk);
     double acc[kDenseRows];
 #pragma unroll
     for (int r = 0; r < kDenseRows; ++r) acc[r] = 0;
-    for (int j = sl; j < nc; j += J) {
+    for (int j = j0 + sl; j < j1; j += J) {
         const double xv = (double)X[(size_t)j * K + c];
 #pragma unroll
         for (int r = 0; r < kDenseRows; ++r)
@@ -779,24 +801,43 @@ __global__ __launch_bounds__(kBlock) void k_dense_block(int32_t nc, const double
         if (r < nr) {
             double a = 0;
             for (int q = 0; q < J; ++q) a += red[r][q][cc];
-            Y[(size_t)(r0 + r) * K + cc] = (TX)a;
+            part[((size_t)sp * nc + r0 + r) * K + cc] = a;
         }
     }
 }
 
 template <typename TX>
-void launch_dense_block(int K, int32_t nc, const double *M, const TX *X, TX *Y, const int32_t *done, hipStream_t s)
+__global__ __launch_bounds__(kBlock) void k_dense_sum(int64_t nk, const double *__restrict__ part,
+                                                      TX *__restrict__ Y, const int32_t *__restrict__ done)
 {
-    const int nb = (nc + kDenseRows - 1) / kDenseRows;
-#define L_(KK) hipLaunchKernelGGL((k_dense_block<TX, KK>), dim3(nb), dim3(kBlock), 0, s, nc, M, X, Y, done)
+    if (done && *done) return;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nk; i += (int64_t)gridDim.x * kBlock) {
+        double a = 0;
+#pragma unroll
+        for (int sp = 0; sp < kDenseSplit; ++sp) a += part[sp * nk + i];
+        Y[i] = (TX)a;
+    }
+}
+
+size_t dense_block_scratch(int32_t nc, int K) { return (size_t)kDenseSplit * nc * K; }
+
+template <typename TX>
+void launch_dense_block(int K, int32_t nc, const double *M, const TX *X, TX *Y, double *scratch, const int32_t *done,
+                        hipStream_t s)
+{
+    const dim3 grid((nc + kDenseRows - 1) / kDenseRows, kDenseSplit);
+#define L_(KK) hipLaunchKernelGGL((k_dense_split<TX, KK>), grid, dim3(kBlock), 0, s, nc, M, X, scratch, done)
     MGX_K_SWITCH(K, L_)
 #undef L_
     MGX_HIP(hipGetLastError());
+    const int64_t nk = (int64_t)nc * K;
+    hipLaunchKernelGGL((k_dense_sum<TX>), dim3(grid_stream(nk)), dim3(kBlock), 0, s, nk, scratch, Y, done);
+    MGX_HIP(hipGetLastError());
 }
-template void launch_dense_block<double>(int, int32_t, const double *, const double *, double *, const int32_t *,
-                                         hipStream_t);
-template void launch_dense_block<float>(int, int32_t, const double *, const float *, float *, const int32_t *,
-                                        hipStream_t);
+template void launch_dense_block<double>(int, int32_t, const double *, const double *, double *, double *,
+                                         const int32_t *, hipStream_t);
+template void launch_dense_block<float>(int, int32_t, const double *, const float *, float *, double *,
+                                        const int32_t *, hipStream_t);
 
 // ------------------------------------------------------------------ column-major <-> block layout
 // dst[i*K + c] = src[c*ld + i] for c < kc, 0 for kc <= c < K.  A 64-row tile goes through LDS so

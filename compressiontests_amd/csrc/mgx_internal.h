@@ -156,8 +156,11 @@ int spmm_grid(int K, int32_t nrows);
 template <typename TV, typename TX>
 void launch_spmm_dot_alpha(int K, int32_t n, const int32_t *rp, const int32_t *ci, const TV *v, const TX *P, TX *Q,
                            BCtl *ctl, double *partials, hipStream_t s);
+// coarse solve Y = M X for K columns; scratch: dense_block_scratch(nc, K) doubles
+size_t dense_block_scratch(int32_t nc, int K);
 template <typename TX>
-void launch_dense_block(int K, int32_t nc, const double *M, const TX *X, TX *Y, const int32_t *done, hipStream_t s);
+void launch_dense_block(int K, int32_t nc, const double *M, const TX *X, TX *Y, double *scratch, const int32_t *done,
+                        hipStream_t s);
 template <typename T>
 void launch_bcopy(int64_t n, const T *src, T *dst, const int32_t *done, hipStream_t s);
 void launch_pack_cols(int32_t n, int K, int kc, const double *src, int64_t ld, double *dst, hipStream_t s);

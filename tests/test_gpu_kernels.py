@@ -192,6 +192,33 @@ def test_duvi_spmv_matches_oracle(gpu, mode):
     assert close(y.numpy(), ref)
 
 
+@pytest.mark.parametrize("mode", [0, 4, 8])
+def test_duvi_ragged_rows(gpu, mode):
+    """Rows with > 256 consecutive columns (blocks split at 255 further entries), gaps of 255/256
+    and more, empty rows, a tile longer than the LDS capacity (one-thread-per-row fallback) and
+    accumulation (y += A x), against the oracle's DuVi SpMV."""
+    rng = np.random.default_rng(21)
+    ncols = 70000
+    rows = [list(range(0, 300)), [], [5, 261, 262, 518, 60000], list(range(1000, 1600, 2))]
+    rows += [sorted(rng.choice(ncols, size=40, replace=False).tolist()) for _ in range(40)]
+    rows += [list(range(2000, 9000))] + [[7, 8]] * 5
+    rp, ci = [0], []
+    for r in rows:
+        ci += r
+        rp.append(len(ci))
+    val = np.round(rng.normal(size=len(ci)), 1) if mode == 0 else rng.normal(size=len(ci))
+    A = O.Csr(np.array(rp, np.int32), np.array(ci, np.int32), val, len(rows), ncols)
+    d = duvi_encode(A, 1e-10, mode)
+    Dd = DeviceDuvi(d)
+    x = rng.normal(size=ncols)
+    ref = O.duvi_block_spmv(O.duvi_block_encode(A, 1e-10, mode), x)
+    y = DeviceArray(A.nrows)
+    Dd.spmv(DeviceArray.from_numpy(x), y)
+    assert close(y.numpy(), ref)
+    Dd.spmv(DeviceArray.from_numpy(x), y, clear=False)
+    assert close(y.numpy(), 2 * ref)
+
+
 def test_duvi_kat(gpu):
     g = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "duvi_kat.json")))
     A = O.Csr(np.array(g["rowptr"], np.int32), np.array(g["col"], np.int32), np.array(g["val"], float), 2, 1024)

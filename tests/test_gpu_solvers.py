@@ -248,6 +248,19 @@ def test_mixed_precision_maxnorm_and_caps(m16):
         HipMgPcgSolver(m, levels=3, smoother="jacobi", nu=2, format="duvi", precision="mixed").Initialize(b)
 
 
+def test_mixed_precision_stagnation_stops(m16):
+    """A tolerance below the FP64 true-residual floor: the refinement stops after three steps
+    that fail to halve the residual instead of running to the iteration cap."""
+    m, A, U, b = m16
+    s = HipMgPcgSolver(m, levels=3, smoother="gs", nu=1, tolerance=1e-30, max_iterations=500, precision="mixed")
+    s.Initialize(b)
+    x, st = s.Solve()
+    assert not st.HasConverged and not st.Failed
+    assert st.OuterIterations < 30
+    assert st.RelativeResidual < 1e-13
+    assert energy_err(A, x, U) < 1e-10
+
+
 @pytest.mark.slow
 def test_mixed_precision_64_heterogeneous(gpu):
     """64^3 with 1e4 modulus contrast: the precision form reaches 1e-9 on the FP64 true residual
