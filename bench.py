@@ -182,11 +182,12 @@ def main():
     vb = 4 if args.precision == "mixed" else 8
     if width > 1:
         bytes_per_launch = spmm_bytes(nnz, ndof, width, vb)
-        kname = (f"k_spmm<{'float' if vb == 4 else 'double'}, K={width}, DOT> (level-0 block SpMM Q = A P with "
-                 f"per-column P.Q)")
+        kname = (f"k_spmm_node<{'float' if vb == 4 else 'double'}, K={width}, DOT> (level-0 node-triple block "
+                 f"SpMM Q = A P with per-column P.Q)")
     else:
         bytes_per_launch = csr_spmv_bytes(nnz, ndof, ndof, vb)
-        kname = f"k_spmv_dot<16, {'float' if vb == 4 else 'double'}> (level-0 CSR SpMV q = A p with p.q)"
+        kname = (f"k_csr_stream<{'float' if vb == 4 else 'double'}, DOT> (level-0 CSR-stream SpMV q = A p with "
+                 f"p.q)")
     avg_ms = spmv_ms / max(spmv_n, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None

@@ -151,6 +151,7 @@ static void download_csr(const DevCsr &A, std::vector<int32_t> &rp, std::vector<
 // ====================================================================== opaque handle types
 struct mgx_matrix_s {
     mgx::DevCsr A;
+    int32_t gx = 0, gy = 0, gz = 0;        // structured mesh elements per axis (0: unknown)
 };
 
 using namespace mgx;
@@ -221,6 +222,9 @@ struct Level {
     DBuf<float> bx32, bb32, br32, bt32;
     int ncolors = 0;
     bool node3 = false;                    // rows 3q..3q+2 share one column list (node-triple kernels)
+    int32_t gx = 0, gy = 0, gz = 0;        // structured mesh of the level (0: unknown)
+    DBuf<int32_t> tile_order;              // node-tile visiting order of the block kernels (empty: natural)
+    int tile_order_npb = 0;                // nodes per tile that tile_order was built for
     std::vector<DBuf<int32_t>> color_nodes;
     std::vector<int32_t> color_count;
     DevDuvi dA, dR, dP;
@@ -388,8 +392,10 @@ static void initialize(mgx_solver_s *S)
     for (int l = 0; l < L; ++l) {
         Level &V = S->lv[l];
         const bool coarsest = (l == L - 1);
+        V.gx = V.gy = V.gz = 0;
         if (V.extA) {
             V.A.view_of(V.extA->A);
+            V.gx = V.extA->gx; V.gy = V.extA->gy; V.gz = V.extA->gz;
             if (!coarsest && S->opt.method != MGX_METHOD_PCG_JACOBI) {
                 MGX_REQUIRE(V.extR && V.extP, MGX_ERR_INVALID_ARGUMENT, "R and P required below the coarsest level");
                 V.R.view_of(V.extR->A); V.P.view_of(V.extP->A);
@@ -966,7 +972,8 @@ static void level_spmm(Level &V, int K, int epi, const TV *val, const TX *X, con
                        const int32_t *done, double *partials, hipStream_t s)
 {
     if (V.node3)
-        launch_spmm_node<TV, TX>(K, epi, V.n / 3, V.A.rp, V.A.ci, val, X, B, pre, Y, done, partials, kPartialStride, s);
+        launch_spmm_node<TV, TX>(K, epi, V.n / 3, V.A.rp, V.A.ci, val, X, B, pre, Y, done, partials, kPartialStride, s,
+                                 V.tile_order_npb == spmm_node_tile(K) ? V.tile_order.p : nullptr);
     else
         launch_spmm<TV, TX>(K, epi, V.n, V.A.rp, V.A.ci, val, X, B, pre, Y, done, partials, kPartialStride, s);
 }
@@ -1245,6 +1252,53 @@ static void solve_blocks(mgx_solver_s *S, const double *b, double *x, bool has_x
     }
 }
 
+// Visiting order of the block kernels on structured levels.  A node's neighbours lie within one
+// plane either side, so walking whole planes needs a window of three planes of X (3 x K values
+// per node) in L2, which at K = 32 is several times the 4 MB L2 of an XCD.  Walking y-strips of
+// SY lines through all planes needs only 3 planes x (SY + 2) lines; SY is the largest strip
+// whose window fits ~3 MB.  Applied to the node tiles of the SpMM and to the node lists of the
+// GS colours (any order is valid within a colour).
+static void order_for_cache(mgx_solver_s *S, int K)
+{
+    const int L = (int)S->lv.size();
+    const size_t tbytes = S->opt.precision == MGX_PRECISION_MIXED ? sizeof(float) : sizeof(double);
+    for (int l = 0; l < L - 1; ++l) {
+        Level &V = S->lv[l];
+        V.tile_order.reset();
+        V.tile_order_npb = 0;
+        if (!V.node3 || V.gx <= 0) continue;
+        const int64_t lines = V.gy + 1, planes = V.gz + 1;
+        if ((int64_t)V.gx * lines * planes * 3 != V.n) continue;   // not the free-node grid of the mesh
+        const double line_bytes = (double)V.gx * 3 * K * tbytes;
+        const int64_t sy = std::max<int64_t>(1, (int64_t)(3.0e6 / (3 * line_bytes)) - 2);
+        if (sy >= lines) continue;                                   // whole planes fit
+        auto key = [&](int64_t q) {
+            const int64_t ix = q % V.gx, iy = (q / V.gx) % lines, iz = q / (V.gx * lines);
+            return (((iy / sy) * planes + iz) * lines + iy) * V.gx + ix;
+        };
+        const int npb = spmm_node_tile(K);
+        if (V.gx % npb == 0) {
+            const int32_t ntiles = (int32_t)(V.n / 3 / npb);
+            std::vector<int32_t> ord(ntiles);
+            for (int32_t i = 0; i < ntiles; ++i) ord[i] = i;
+            std::sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return key((int64_t)a * npb) < key((int64_t)b * npb); });
+            V.tile_order.upload(ord.data(), ord.size(), S->stream);
+            V.tile_order_npb = npb;
+        }
+        for (int c = 0; c < V.ncolors; ++c) {
+            std::vector<int32_t> nodes(V.color_count[c]);
+            if (nodes.empty()) continue;
+            MGX_HIP(hipMemcpyAsync(nodes.data(), V.color_nodes[c].p, sizeof(int32_t) * nodes.size(), hipMemcpyDeviceToHost,
+                                   S->stream));
+            MGX_HIP(hipStreamSynchronize(S->stream));
+            std::sort(nodes.begin(), nodes.end(), [&](int32_t a, int32_t b) { return key(a) < key(b); });
+            MGX_HIP(hipMemcpyAsync(V.color_nodes[c].p, nodes.data(), sizeof(int32_t) * nodes.size(), hipMemcpyHostToDevice,
+                                   S->stream));
+            MGX_HIP(hipStreamSynchronize(S->stream));
+        }
+    }
+}
+
 // buffers of the batching form for block width K
 static void alloc_block(mgx_solver_s *S, int K)
 {
@@ -1263,6 +1317,7 @@ static void alloc_block(mgx_solver_s *S, int K)
     else { S->bP64.alloc(nk); S->bZ64.alloc(nk); }
     S->bpartials.ensure(kBlockPartials);
     S->bdense.ensure(dense_block_scratch(S->lv[L - 1].n, K));
+    order_for_cache(S, K);
     S->bctl.ensure(1); S->bctl_in.ensure(1); S->bctl_stat.ensure(1);
     if (S->bgexec) { MGX_HIP(hipGraphExecDestroy(S->bgexec)); S->bgexec = nullptr; }
     S->bgraph_key = -1;
@@ -1330,6 +1385,11 @@ static void choose_block_width(mgx_solver_s *S)
         if (K == 4 || per_col < best) { best = per_col; bestK = K; }
     }
     S->bk = bestK;
+    if (bestK != kmax) {
+        order_for_cache(S, bestK);
+        if (S->bgexec) { MGX_HIP(hipGraphExecDestroy(S->bgexec)); S->bgexec = nullptr; }
+        S->bgraph_key = -1;
+    }
 }
 
 static void solve_device(mgx_solver_s *S, const double *b, double *x, bool has_x0, mgx_stats *stats)
@@ -1570,6 +1630,7 @@ int mgx_matrix_cantilever(int32_t nx, int32_t ny, int32_t nz, double E, double n
         DBuf<double> d;
         if (dens) d.upload(dens, (size_t)nx * ny * nz, s);
         assemble_into(m->A, nx, ny, nz, Ke, dens ? d.p : nullptr, nullptr, s);
+        m->gx = nx; m->gy = ny; m->gz = nz;
         *out = m.release();
     });
 }
@@ -1616,6 +1677,7 @@ int mgx_matrix_galerkin_hierarchy(int32_t nx, int32_t ny, int32_t nz, double E, 
         std::vector<std::unique_ptr<mgx_matrix_s>> out(nlevels);
         out[0] = std::make_unique<mgx_matrix_s>();
         assemble_into(out[0]->A, nx, ny, nz, Ke, dens ? d.p : nullptr, nullptr, s);
+        out[0]->gx = nx; out[0]->gy = ny; out[0]->gz = nz;
         DBuf<double> Kprev, Kcur;
         int fx = nx, fy = ny, fz = nz;
         DBuf<double> ones;
@@ -1630,6 +1692,7 @@ int mgx_matrix_galerkin_hierarchy(int32_t nx, int32_t ny, int32_t nz, double E, 
             else launch_galerkin_elements(cx, cy, cz, nullptr, nullptr, Kprev.p, Kcur.p, s);
             out[l] = std::make_unique<mgx_matrix_s>();
             assemble_into(out[l]->A, cx, cy, cz, nullptr, nullptr, Kcur.p, s);
+            out[l]->gx = cx; out[l]->gy = cy; out[l]->gz = cz;
             std::swap(Kprev.p, Kcur.p); std::swap(Kprev.n, Kcur.n);
             fx = cx; fy = cy; fz = cz;
         }
@@ -1924,6 +1987,9 @@ int mgx_time_spmm(int32_t flags, int32_t k, int32_t nrows, const int32_t *rowptr
                     "invalid arguments");
         MGX_REQUIRE(k == 4 || k == 8 || k == 16 || k == 32, MGX_ERR_INVALID_ARGUMENT, "k must be 4, 8, 16 or 32");
         const bool fp32 = flags & 1, node = flags & 2;
+        const int epi = (flags & 4) ? BEPI_DOT : BEPI_SPMM;
+        DBuf<double> parts;
+        if (epi == BEPI_DOT) parts.alloc((size_t)k * kPartialStride);
         if (node) {
             // the node-triple kernel reads len(3q) entries of rows 3q+1 and 3q+2: check the lengths
             MGX_REQUIRE(nrows % 3 == 0, MGX_ERR_INVALID_ARGUMENT, "node-triple SpMM needs nrows % 3 == 0");
@@ -1941,17 +2007,18 @@ int mgx_time_spmm(int32_t flags, int32_t k, int32_t nrows, const int32_t *rowptr
         MGX_HIP(hipEventCreate(&e1));
         auto one = [&]() {
             if (fp32 && node)
-                launch_spmm_node<float, float>(k, BEPI_SPMM, nrows / 3, rowptr, col, (const float *)val, (const float *)X,
-                                               nullptr, nullptr, (float *)Y, nullptr, nullptr, 0, s);
+                launch_spmm_node<float, float>(k, epi, nrows / 3, rowptr, col, (const float *)val, (const float *)X,
+                                               nullptr, nullptr, (float *)Y, nullptr, parts.p, kPartialStride, s);
             else if (node)
-                launch_spmm_node<double, double>(k, BEPI_SPMM, nrows / 3, rowptr, col, (const double *)val,
-                                                 (const double *)X, nullptr, nullptr, (double *)Y, nullptr, nullptr, 0, s);
+                launch_spmm_node<double, double>(k, epi, nrows / 3, rowptr, col, (const double *)val,
+                                                 (const double *)X, nullptr, nullptr, (double *)Y, nullptr, parts.p,
+                                                 kPartialStride, s);
             else if (fp32)
-                launch_spmm<float, float>(k, BEPI_SPMM, nrows, rowptr, col, (const float *)val, (const float *)X,
-                                          nullptr, nullptr, (float *)Y, nullptr, nullptr, 0, s);
+                launch_spmm<float, float>(k, epi, nrows, rowptr, col, (const float *)val, (const float *)X,
+                                          nullptr, nullptr, (float *)Y, nullptr, parts.p, kPartialStride, s);
             else
-                launch_spmm<double, double>(k, BEPI_SPMM, nrows, rowptr, col, (const double *)val, (const double *)X,
-                                            nullptr, nullptr, (double *)Y, nullptr, nullptr, 0, s);
+                launch_spmm<double, double>(k, epi, nrows, rowptr, col, (const double *)val, (const double *)X,
+                                            nullptr, nullptr, (double *)Y, nullptr, parts.p, kPartialStride, s);
         };
         for (int i = 0; i < warmup; ++i) one();
         MGX_HIP(hipEventRecord(e0, s));

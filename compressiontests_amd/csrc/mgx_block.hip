@@ -19,6 +19,8 @@
 //    fixed order and updates the per-column PCG scalars of a BCtl.  Converged columns are
 //    frozen (alpha = beta = 0) while the others continue; every kernel returns at once when
 //    all columns are done.
+#include <type_traits>
+
 #include "mgx_device.h"
 
 namespace mgx {
@@ -177,60 +179,101 @@ template void launch_spmm<double, float>(int, int, int32_t, const int32_t *, con
 
 // ------------------------------------------------------------------ node-triple kernels
 // For matrices whose rows come in node triples (rows 3q, 3q+1, 3q+2 with identical column lists,
-// as every level matrix of the 3-dof hexahedral hierarchy), each gathered segment X[col*K ..]
-// serves all three rows: the X traffic through the caches drops threefold against the row
-// kernels.  Lanes as in k_spmm, with G lanes per node instead of per row.
-template <int K> struct NShape {
-    static constexpr int S = K >= 16 ? 1 : 16 / K;
-    static constexpr int G = K * S;
-    static constexpr int NPB = kBlock / G;          // nodes per tile
-    static constexpr int CAP = NPB * 3 * 84;        // staged entries per tile
-    static constexpr int U = S == 1 ? 8 : 4;
+// as every level matrix of the 3-dof hexahedral hierarchy), each gathered X row serves all three
+// rows.  A lane holds W = 16 bytes of columns (4 FP32 or 2 FP64), so the K values of one X row
+// come in with V = K / W lanes of one 16-byte load each; 32 lanes per node split the node's
+// entries S = 32 / V ways.  (With one column per lane the kernels were bound by the latency of
+// many 4-byte gathers, not by bandwidth.)
+template <typename T> struct Vec16;
+template <> struct Vec16<float> {
+    typedef float type __attribute__((ext_vector_type(4)));
+    static constexpr int W = 4;
+};
+template <> struct Vec16<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+    static constexpr int W = 2;
 };
 
-template <int K, typename TV, typename TX, typename PI, typename PV>
-__device__ __forceinline__ void node_cols(PI cis, PV vs, int o0, int o1, int o2, int len, const TX *__restrict__ X,
-                                          int c, int sl, TV &y0, TV &y1, TV &y2)
+template <typename T, int K> struct NShape {
+    static constexpr int W = Vec16<T>::W;     // columns per lane
+    static constexpr int V = K / W;           // lanes covering one X row
+    static constexpr int G = 32;              // lanes per node
+    static constexpr int S = G / V;           // lanes splitting a node's entries
+    static constexpr int NPB = kBlock / G;    // nodes per tile
+    static constexpr int CAP = NPB * 3 * 84;  // staged entries per tile
+    static constexpr int U = 4;               // entries in flight per lane
+};
+
+int spmm_node_tile(int) { return kBlock / 32; }
+
+// K column partials of a block: thread t holds columns c0 .. c0 + W - 1 (c0 = (t % V) W)
+template <typename T, int K>
+__device__ __forceinline__ void publish_vcols(const double *dot, double *__restrict__ partials, int stride)
 {
-    using Sh = NShape<K>;
+    using Sh = NShape<T, K>;
+    __shared__ double red[Sh::W][kBlock];
+    const int t = threadIdx.x;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < Sh::W; ++w) red[w][t] = dot[w];
+    __syncthreads();
+    if (t < K) {
+        const int v = t / Sh::W, w = t % Sh::W;
+        double a = 0;
+        for (int j = v; j < kBlock; j += Sh::V) a += red[w][j];
+        partials[(size_t)t * stride + blockIdx.x] = a;
+    }
+}
+
+template <typename T, int K, typename PI, typename PV>
+__device__ __forceinline__ void node_cols(PI cis, PV vs, int o0, int o1, int o2, int len, const T *__restrict__ X,
+                                          int c0, int sl, typename Vec16<T>::type &y0, typename Vec16<T>::type &y1,
+                                          typename Vec16<T>::type &y2)
+{
+    using Sh = NShape<T, K>;
+    using VT = typename Vec16<T>::type;
     for (int j = sl; j < len; j += Sh::U * Sh::S) {
         int cc[Sh::U];
-        TV w0[Sh::U], w1[Sh::U], w2[Sh::U];
+        T w0[Sh::U], w1[Sh::U], w2[Sh::U];
 #pragma unroll
         for (int u = 0; u < Sh::U; ++u) {
             const int jj = j + u * Sh::S;
             if (jj < len) { cc[u] = cis[o0 + jj]; w0[u] = vs[o0 + jj]; w1[u] = vs[o1 + jj]; w2[u] = vs[o2 + jj]; }
             else { cc[u] = -1; w0[u] = w1[u] = w2[u] = 0; }
         }
-        TX xv[Sh::U];
+        VT xv[Sh::U];
 #pragma unroll
-        for (int u = 0; u < Sh::U; ++u) xv[u] = cc[u] >= 0 ? X[(size_t)cc[u] * K + c] : TX(0);
+        for (int u = 0; u < Sh::U; ++u)
+            xv[u] = cc[u] >= 0 ? *(const VT *)(X + (size_t)cc[u] * K + c0) : VT(0);
 #pragma unroll
-        for (int u = 0; u < Sh::U; ++u) {
-            const TV xx = (TV)xv[u];
-            y0 += w0[u] * xx; y1 += w1[u] * xx; y2 += w2[u] * xx;
-        }
+        for (int u = 0; u < Sh::U; ++u) { y0 += w0[u] * xv[u]; y1 += w1[u] * xv[u]; y2 += w2[u] * xv[u]; }
     }
 }
 
-template <typename TV, typename TX, int K, int EPI>
+template <typename T, int K, int EPI>
 __global__ __launch_bounds__(kBlock) void k_spmm_node(int32_t nnodes, const int32_t *__restrict__ rp,
-                                                      const int32_t *__restrict__ ci, const TV *__restrict__ v,
-                                                      const TX *__restrict__ X, const TX *__restrict__ B,
-                                                      const TV *__restrict__ pre, TX *__restrict__ Y,
+                                                      const int32_t *__restrict__ ci, const T *__restrict__ v,
+                                                      const T *__restrict__ X, const T *__restrict__ B,
+                                                      const T *__restrict__ pre, T *__restrict__ Y,
                                                       const int32_t *__restrict__ done, double *__restrict__ partials,
-                                                      int stride)
+                                                      int stride, const int32_t *__restrict__ order)
 {
-    using Sh = NShape<K>;
+    using Sh = NShape<T, K>;
+    using VT = typename Vec16<T>::type;
+    constexpr int W = Sh::W;
     __shared__ int32_t s_ci[Sh::CAP];
-    __shared__ TV s_v[Sh::CAP];
+    __shared__ T s_v[Sh::CAP];
     if (done && *done) return;
-    const int t = threadIdx.x, g = t / Sh::G, l = t % Sh::G, c = l % K, sl = l / K;
+    const int t = threadIdx.x, g = t / Sh::G, l = t % Sh::G, c0 = (l % Sh::V) * W, sl = l / Sh::V;
     const int ntiles = (nnodes + Sh::NPB - 1) / Sh::NPB;
     const int per = gridDim.x >> 3, xcd = blockIdx.x & 7;
     const int hi = (int)((int64_t)ntiles * (xcd + 1) / 8);
-    double dot = 0;
-    for (int tile = (int)((int64_t)ntiles * xcd / 8) + (int)(blockIdx.x >> 3); tile < hi; tile += per) {
+    double dot[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) dot[w] = 0;
+    for (int pos = (int)((int64_t)ntiles * xcd / 8) + (int)(blockIdx.x >> 3); pos < hi; pos += per) {
+        // order: the tiles in y-strips through z, so that an XCD's window of X fits its L2
+        const int tile = order ? order[pos] : pos;
         const int q0 = tile * Sh::NPB, q1 = min(q0 + Sh::NPB, nnodes);
         const int e0 = rp[3 * q0], e1 = rp[3 * q1];
         const bool staged = e1 - e0 <= Sh::CAP;     // the same for the whole block
@@ -243,51 +286,55 @@ __global__ __launch_bounds__(kBlock) void k_spmm_node(int32_t nnodes, const int3
         if (q < q1) {
             const int r = 3 * q;
             const int s0 = rp[r], s1 = rp[r + 1], s2 = rp[r + 2];
-            TV y[3] = {0, 0, 0};
-            if (staged) node_cols<K, TV, TX>(s_ci, s_v, s0 - e0, s1 - e0, s2 - e0, s1 - s0, X, c, sl, y[0], y[1], y[2]);
-            else node_cols<K, TV, TX>(ci, v, s0, s1, s2, s1 - s0, X, c, sl, y[0], y[1], y[2]);
+            VT y[3] = {VT(0), VT(0), VT(0)};
+            if (staged) node_cols<T, K>(s_ci, s_v, s0 - e0, s1 - e0, s2 - e0, s1 - s0, X, c0, sl, y[0], y[1], y[2]);
+            else node_cols<T, K>(ci, v, s0, s1, s2, s1 - s0, X, c0, sl, y[0], y[1], y[2]);
 #pragma unroll
-            for (int o = K; o < Sh::G; o <<= 1) {
-                y[0] += __shfl_xor(y[0], o, Sh::G);
-                y[1] += __shfl_xor(y[1], o, Sh::G);
-                y[2] += __shfl_xor(y[2], o, Sh::G);
-            }
+            for (int o = Sh::V; o < Sh::G; o <<= 1)
+#pragma unroll
+                for (int a = 0; a < 3; ++a)
+#pragma unroll
+                    for (int w = 0; w < W; ++w) y[a][w] += __shfl_xor(y[a][w], o, Sh::G);
             if (sl == 0) {
 #pragma unroll
                 for (int a = 0; a < 3; ++a) {
-                    const size_t i = (size_t)(r + a) * K + c;
-                    const TV acc = y[a];
-                    if (EPI == BEPI_SPMM) Y[i] = (TX)acc;
-                    else if (EPI == BEPI_ADD) Y[i] = (TX)((TV)Y[i] + acc);
-                    else if (EPI == BEPI_RESID) Y[i] = (TX)((TV)B[i] - acc);
-                    else if (EPI == BEPI_JACOBI) Y[i] = (TX)((TV)X[i] + pre[r + a] * ((TV)B[i] - acc));
+                    const size_t i = (size_t)(r + a) * K + c0;
+                    VT *yp = (VT *)(Y + i);
+                    if (EPI == BEPI_SPMM) *yp = y[a];
+                    else if (EPI == BEPI_ADD) *yp = *yp + y[a];
+                    else if (EPI == BEPI_RESID) *yp = *(const VT *)(B + i) - y[a];
+                    else if (EPI == BEPI_JACOBI)
+                        *yp = *(const VT *)(X + i) + pre[r + a] * (*(const VT *)(B + i) - y[a]);
                     else {
-                        const TX qv = (TX)acc;
-                        Y[i] = qv;
-                        dot += (double)X[i] * (double)qv;
+                        *yp = y[a];
+                        const VT xp = *(const VT *)(X + i);
+#pragma unroll
+                        for (int w = 0; w < W; ++w) dot[w] += (double)xp[w] * (double)y[a][w];
                     }
                 }
             }
         }
     }
-    if (EPI == BEPI_DOT) publish_cols<K, 1, -1>(dot, 0, 0, partials, stride);
+    if (EPI == BEPI_DOT) publish_vcols<T, K>(dot, partials, stride);
 }
 
 int spmm_node_grid(int K, int32_t nnodes)
 {
-    const int npb = K == 32 ? NShape<32>::NPB : NShape<16>::NPB;   // 8 or 16 nodes per tile
+    const int npb = spmm_node_tile(K);
     return grid_tiles((nnodes + npb - 1) / npb);
 }
 
 template <typename TV, typename TX>
 void launch_spmm_node(int K, int epi, int32_t nnodes, const int32_t *rp, const int32_t *ci, const TV *v, const TX *X,
-                      const TX *B, const TV *pre, TX *Y, const int32_t *done, double *partials, int stride, hipStream_t s)
+                      const TX *B, const TV *pre, TX *Y, const int32_t *done, double *partials, int stride,
+                      hipStream_t s, const int32_t *order)
 {
+    static_assert(std::is_same<TV, TX>::value, "node-triple kernels take matrix and vectors of one type");
     if (nnodes <= 0) return;
     const int g = spmm_node_grid(K, nnodes);
-#define MGX_SPMMN(KK, EE)                                                                                       \
-    hipLaunchKernelGGL((k_spmm_node<TV, TX, KK, EE>), dim3(g), dim3(kBlock), 0, s, nnodes, rp, ci, v, X, B, pre, Y, \
-                       done, partials, stride)
+#define MGX_SPMMN(KK, EE)                                                                                      \
+    hipLaunchKernelGGL((k_spmm_node<TX, KK, EE>), dim3(g), dim3(kBlock), 0, s, nnodes, rp, ci, v, X, B, pre, Y, \
+                       done, partials, stride, order)
 #define MGX_SPMMN_K(KK)                                       \
     switch (epi) {                                            \
     case BEPI_SPMM: MGX_SPMMN(KK, BEPI_SPMM); break;          \
@@ -309,23 +356,35 @@ void launch_spmm_node(int K, int epi, int32_t nnodes, const int32_t *rp, const i
 }
 template void launch_spmm_node<double, double>(int, int, int32_t, const int32_t *, const int32_t *, const double *,
                                                const double *, const double *, const double *, double *,
-                                               const int32_t *, double *, int, hipStream_t);
+                                               const int32_t *, double *, int, hipStream_t, const int32_t *);
 template void launch_spmm_node<float, float>(int, int, int32_t, const int32_t *, const int32_t *, const float *,
                                              const float *, const float *, const float *, float *, const int32_t *,
-                                             double *, int, hipStream_t);
+                                             double *, int, hipStream_t, const int32_t *);
 
-template <typename T> struct GsAcc {
-    T a0 = 0, a1 = 0, a2 = 0;
-    T d00 = 0, d01 = 0, d02 = 0, d10 = 0, d11 = 0, d12 = 0, d20 = 0, d21 = 0, d22 = 0;
-};
-
-// one lane's share of a node triple for GS: entries j = sl, sl + S, ... of the three rows (at
-// offsets 0, o1, o2 of cis / vs) for column c
-template <int K, typename T, typename PI, typename PV>
-__device__ __forceinline__ void gs_node_accum(PI cis, PV vs, int o1, int o2, int len, int q3, const T *__restrict__ X,
-                                              int c, int sl, GsAcc<T> &A)
+// Multicolour GS step on node triples: thread (node, s, v) walks every S-th entry of the node's
+// three rows for columns c0 .. c0 + W - 1, gathering X once (16 bytes) for the three rows; the
+// node's 3x3 diagonal block comes from the same pass.  No LDS and no block barrier (staging the
+// node rows through LDS was measured slower: the kernel waits on the X gathers).
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_gs_node(int32_t nnodes, const int32_t *__restrict__ nodes,
+                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                    const T *__restrict__ v, const T *__restrict__ B,
+                                                    T *__restrict__ X, int backward, const int32_t *__restrict__ done)
 {
-    constexpr int S = NShape<K>::S, U = NShape<K>::U;
+    using Sh = NShape<T, K>;
+    using VT = typename Vec16<T>::type;
+    constexpr int G = Sh::G, N = kBlock / G, S = Sh::S, U = Sh::U, W = Sh::W;
+    if (done && *done) return;
+    const int t = threadIdx.x, l = t % G, c0 = (l % Sh::V) * W, sl = l / Sh::V;
+    const int nd = xcd_remap(blockIdx.x, gridDim.x) * N + t / G;
+    if (nd >= nnodes) return;                      // the whole G-lane group leaves together
+    const int q3 = 3 * nodes[nd];
+    const int s0 = rp[q3], s1 = rp[q3 + 1], s2 = rp[q3 + 2], len = s1 - s0;
+    const int o1 = s1 - s0, o2 = s2 - s0;
+    const int32_t *cis = ci + s0;
+    const T *vs = v + s0;
+    VT a0 = VT(0), a1 = VT(0), a2 = VT(0);
+    T d[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
     for (int j = sl; j < len; j += U * S) {
         int cc[U];
         T w0[U], w1[U], w2[U];
@@ -335,68 +394,47 @@ __device__ __forceinline__ void gs_node_accum(PI cis, PV vs, int o1, int o2, int
             if (jj < len) { cc[u] = cis[jj]; w0[u] = vs[jj]; w1[u] = vs[o1 + jj]; w2[u] = vs[o2 + jj]; }
             else { cc[u] = -1; w0[u] = w1[u] = w2[u] = 0; }
         }
-        T xv[U];
+        VT xv[U];
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const unsigned off = (unsigned)(cc[u] - q3);
-            xv[u] = (cc[u] >= 0 && off >= 3u) ? X[(size_t)cc[u] * K + c] : T(0);
+            xv[u] = (cc[u] >= 0 && off >= 3u) ? *(const VT *)(X + (size_t)cc[u] * K + c0) : VT(0);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             if (cc[u] < 0) continue;
             const unsigned off = (unsigned)(cc[u] - q3);
-            if (off >= 3u) { A.a0 += w0[u] * xv[u]; A.a1 += w1[u] * xv[u]; A.a2 += w2[u] * xv[u]; }
-            else if (off == 0) { A.d00 += w0[u]; A.d10 += w1[u]; A.d20 += w2[u]; }
-            else if (off == 1) { A.d01 += w0[u]; A.d11 += w1[u]; A.d21 += w2[u]; }
-            else { A.d02 += w0[u]; A.d12 += w1[u]; A.d22 += w2[u]; }
+            if (off >= 3u) { a0 += w0[u] * xv[u]; a1 += w1[u] * xv[u]; a2 += w2[u] * xv[u]; }
+            else if (off == 0) { d[0] += w0[u]; d[3] += w1[u]; d[6] += w2[u]; }
+            else if (off == 1) { d[1] += w0[u]; d[4] += w1[u]; d[7] += w2[u]; }
+            else { d[2] += w0[u]; d[5] += w1[u]; d[8] += w2[u]; }
         }
     }
-}
-
-// Multicolour GS step on node triples: thread (node, s, c) walks every S-th entry of the node's
-// three rows for column c, gathering X once for the three rows; the node's 3x3 diagonal block
-// comes from the same pass.  No LDS and no block barrier (staging the node rows through LDS was
-// measured slower: the kernel waits on the X gathers, not on the matrix loads).
-template <typename T, int K>
-__global__ __launch_bounds__(kBlock) void k_gs_node(int32_t nnodes, const int32_t *__restrict__ nodes,
-                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                    const T *__restrict__ v, const T *__restrict__ B,
-                                                    T *__restrict__ X, int backward, const int32_t *__restrict__ done)
-{
-    using Sh = NShape<K>;
-    constexpr int G = Sh::G, N = kBlock / G;
-    if (done && *done) return;
-    const int t = threadIdx.x, l = t % G, c = l % K, sl = l / K;
-    const int nd = xcd_remap(blockIdx.x, gridDim.x) * N + t / G;
-    if (nd >= nnodes) return;                      // the whole G-lane group leaves together
-    const int q3 = 3 * nodes[nd];
-    const int s0 = rp[q3], s1 = rp[q3 + 1], s2 = rp[q3 + 2], len = s1 - s0;
-    GsAcc<T> acc;
-    gs_node_accum<K>(ci + s0, v + s0, s1 - s0, s2 - s0, len, q3, X, c, sl, acc);
-    T a0 = acc.a0, a1 = acc.a1, a2 = acc.a2;
-    T d00 = acc.d00, d01 = acc.d01, d02 = acc.d02, d10 = acc.d10, d11 = acc.d11, d12 = acc.d12;
-    T d20 = acc.d20, d21 = acc.d21, d22 = acc.d22;
 #pragma unroll
-    for (int o = K; o < G; o <<= 1) {
-        a0 += __shfl_xor(a0, o, G); a1 += __shfl_xor(a1, o, G); a2 += __shfl_xor(a2, o, G);
-        d00 += __shfl_xor(d00, o, G); d01 += __shfl_xor(d01, o, G); d02 += __shfl_xor(d02, o, G);
-        d10 += __shfl_xor(d10, o, G); d11 += __shfl_xor(d11, o, G); d12 += __shfl_xor(d12, o, G);
-        d20 += __shfl_xor(d20, o, G); d21 += __shfl_xor(d21, o, G); d22 += __shfl_xor(d22, o, G);
+    for (int o = Sh::V; o < G; o <<= 1) {
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            a0[w] += __shfl_xor(a0[w], o, G); a1[w] += __shfl_xor(a1[w], o, G); a2[w] += __shfl_xor(a2[w], o, G);
+        }
+#pragma unroll
+        for (int k = 0; k < 9; ++k) d[k] += __shfl_xor(d[k], o, G);
     }
     if (sl != 0) return;
-    const size_t i0 = (size_t)q3 * K + c;
-    T x0 = X[i0], x1 = X[i0 + K], x2 = X[i0 + 2 * K];
-    const T b0 = B[i0] - a0, b1 = B[i0 + K] - a1, b2 = B[i0 + 2 * K] - a2;
+    const size_t i0 = (size_t)q3 * K + c0;
+    VT x0 = *(const VT *)(X + i0), x1 = *(const VT *)(X + i0 + K), x2 = *(const VT *)(X + i0 + 2 * K);
+    const VT b0 = *(const VT *)(B + i0) - a0, b1 = *(const VT *)(B + i0 + K) - a1, b2 = *(const VT *)(B + i0 + 2 * K) - a2;
     if (!backward) {
-        x0 = (b0 - d01 * x1 - d02 * x2) / d00;
-        x1 = (b1 - d10 * x0 - d12 * x2) / d11;
-        x2 = (b2 - d20 * x0 - d21 * x1) / d22;
+        x0 = (b0 - d[1] * x1 - d[2] * x2) / d[0];
+        x1 = (b1 - d[3] * x0 - d[5] * x2) / d[4];
+        x2 = (b2 - d[6] * x0 - d[7] * x1) / d[8];
     } else {
-        x2 = (b2 - d20 * x0 - d21 * x1) / d22;
-        x1 = (b1 - d10 * x0 - d12 * x2) / d11;
-        x0 = (b0 - d01 * x1 - d02 * x2) / d00;
+        x2 = (b2 - d[6] * x0 - d[7] * x1) / d[8];
+        x1 = (b1 - d[3] * x0 - d[5] * x2) / d[4];
+        x0 = (b0 - d[1] * x1 - d[2] * x2) / d[0];
     }
-    X[i0] = x0; X[i0 + K] = x1; X[i0 + 2 * K] = x2;
+    *(VT *)(X + i0) = x0;
+    *(VT *)(X + i0 + K) = x1;
+    *(VT *)(X + i0 + 2 * K) = x2;
 }
 
 template <typename T>
@@ -404,8 +442,8 @@ void launch_gs_node(int K, int32_t nnodes, const int32_t *nodes, const int32_t *
                     const T *B, T *X, bool backward, const int32_t *done, hipStream_t s)
 {
     if (nnodes <= 0) return;
-#define MGX_GSN(KK)                                                                                             \
-    hipLaunchKernelGGL((k_gs_node<T, KK>), dim3(grid_rows(nnodes, kBlock / NShape<KK>::G)), dim3(kBlock), 0, s, \
+#define MGX_GSN(KK)                                                                                                \
+    hipLaunchKernelGGL((k_gs_node<T, KK>), dim3(grid_rows(nnodes, kBlock / NShape<T, KK>::G)), dim3(kBlock), 0, s, \
                        nnodes, nodes, rp, ci, v, B, X, backward ? 1 : 0, done)
     switch (K) {
     case 4: MGX_GSN(4); break;

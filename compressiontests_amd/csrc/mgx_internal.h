@@ -132,8 +132,9 @@ void launch_spmm(int K, int epi, int32_t nrows, const int32_t *rp, const int32_t
 template <typename TV, typename TX>
 void launch_spmm_node(int K, int epi, int32_t nnodes, const int32_t *rp, const int32_t *ci, const TV *v, const TX *X,
                       const TX *B, const TV *pre, TX *Y, const int32_t *done, double *partials, int stride,
-                      hipStream_t s);
+                      hipStream_t s, const int32_t *order = nullptr);
 int spmm_node_grid(int K, int32_t nnodes);
+int spmm_node_tile(int K);   // nodes per tile of the node-triple SpMM
 template <typename T>
 void launch_gs_node(int K, int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const T *v,
                     const T *B, T *X, bool backward, const int32_t *done, hipStream_t s);

@@ -8,7 +8,8 @@ the resulting GB/s against the 8 TB/s HBM peak.
 `copy` is a plain device-to-device copy kernel of known size, the calibration for rocprofv3's
 FETCH_SIZE / WRITE_SIZE counters.  `spmm64_k32` / `spmm32_k32` (any k in 4, 8, 16, 32): the
 block SpMM Y = A X of the batching form, FP64 or FP32 values and vectors (suffix `_node`: the
-node-triple kernel the block solver uses on the stiffness matrices).
+node-triple kernel the block solver uses on the stiffness matrices; `_node_dot`: its block-PCG
+form q = A p with the per-column p.q partials).
 """
 import argparse
 import ctypes as C
@@ -68,14 +69,15 @@ def main():
             by = spmv_bytes(nnz, N, 4, 4)
         elif kind.startswith("spmm"):
             fp32 = kind.startswith("spmm32")
-            node = kind.endswith("_node")
+            node = "_node" in kind
+            dot = kind.endswith("_dot")
             k = int(kind.split("_k")[1].split("_")[0])
             dt = np.float32 if fp32 else np.float64
             from .device import DeviceCsr
             Ak = DeviceCsr.arrays_from_host(A.to_host(), val_dtype=dt) if fp32 else A
             X = DeviceArray.from_numpy(rng_normal(args.seed + 3, N * k).astype(dt))
             Y = DeviceArray(N * k, dt)
-            ms = kernels.time_spmm(Ak, X, Y, k, warmup=args.warmup, repeats=args.repeats, node=node)
+            ms = kernels.time_spmm(Ak, X, Y, k, warmup=args.warmup, repeats=args.repeats, node=node, dot=dot)
             vb = 4 if fp32 else 8
             by = (vb + 4) * nnz + 4 * (N + 1) + 2 * vb * k * N
         elif kind.startswith("duvi"):

@@ -69,11 +69,11 @@ def time_csr_kernel(kind, A, x, y, b=None, pre=None, warmup=5, repeats=100):
     return out.value
 
 
-def time_spmm(A, X, Y, k, warmup=5, repeats=100, node=False):
+def time_spmm(A, X, Y, k, warmup=5, repeats=100, node=False, dot=False):
     """Average device ms per launch of Y = A X with k right-hand sides (FP32 if X is float32;
-    node: the node-triple kernel)."""
+    node: the node-triple kernel; dot: its block-PCG form with the per-column X.Y partials)."""
     out = C.c_double()
-    L.call("mgx_time_spmm", int(X.dtype == np.float32) | (2 if node else 0), k, A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col),
+    L.call("mgx_time_spmm", int(X.dtype == np.float32) | (2 if node else 0) | (4 if dot else 0), k, A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col),
            C.c_void_p(A.val), X.vp, Y.vp, warmup, repeats, C.byref(out))
     return out.value
 

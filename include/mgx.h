@@ -293,7 +293,8 @@ int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, cons
                         double *avg_ms);
 /* Same for the block SpMM Y = A X with k right-hand sides.  flags bit 0: FP32 values and vectors
  * (mgx_csr_spmm_f32) instead of FP64; bit 1: the node-triple kernel of the block solver (rows
- * 3q..3q+2 share one column list; triples of unequal length are rejected). */
+ * 3q..3q+2 share one column list; triples of unequal length are rejected); bit 2: the block
+ * PCG's form that also produces the per-column partials of X.Y (Y = A X itself unchanged). */
 int mgx_time_spmm(int32_t flags, int32_t k, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
                   const void *X, void *Y, int32_t warmup, int32_t repeats, double *avg_ms);
 /* Average device time of a plain copy kernel (16-byte vector loads and stores) moving `bytes`

@@ -183,6 +183,27 @@ def test_block_width_validation(het16):
     assert one.BlockWidth[0] == 1
 
 
+def test_block_strip_order_48(gpu):
+    """48^3 with 32 load cases at block width 32: large enough that the block kernels walk the
+    node tiles and the GS colours in y-strips (the three-plane window of X exceeds the L2 budget).
+    Every column still matches the single-right-hand-side solve of that column."""
+    n = 48
+    dens = 10 ** (4 * O.rng_uniform(20261205, n ** 3) - 2)
+    m = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
+    F = m.load_cases(32, SEED_LOADS)
+    blk = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-10, nrhs=32, block_width=32)
+    blk.Initialize(F)
+    X, st = blk.Solve()
+    assert all(t.HasConverged for t in st)
+    A = O.cantilever(n, n, n, dens=dens)
+    for j in (0, 17, 31):
+        one = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-10, hierarchy=blk.hierarchy)
+        one.Initialize(F[j])
+        x1, s1 = one.Solve()
+        assert abs(st[j].NumIterationsRequired - s1.NumIterationsRequired) <= 1
+        assert energy_err(A, X[j], x1) < 1e-9
+
+
 @pytest.mark.slow
 def test_block_mixed_64_32_load_cases(gpu):
     """64^3, 1e4 modulus contrast, 32 top-face load cases: the automatically blocked precision
