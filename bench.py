@@ -42,9 +42,13 @@ def csr_spmv_bytes(nnz, nrows, ncols, val_bytes=8):
     return (val_bytes + 4) * nnz + 4 * (nrows + 1) + val_bytes * (ncols + nrows)
 
 
-def spmm_bytes(nnz, n, k, val_bytes):
-    """Algorithmic bytes of one CSR SpMM with k right-hand sides: the matrix once, X read once
-    and Y written once (n x k each)."""
+def spmm_bytes(nnz, n, k, val_bytes, layout="csr"):
+    """Algorithmic bytes of one SpMM with k right-hand sides: the matrix once, X read once and Y
+    written once (n x k each).  CSR: a value and a column index per nonzero and n + 1 row offsets;
+    node blocks: a value per nonzero, one column index per 3x3 block (nnz / 9) and n / 3 + 1 node
+    offsets."""
+    if layout == "node-block":
+        return val_bytes * nnz + 4 * (nnz // 9) + 4 * (n // 3 + 1) + 2 * val_bytes * k * n
     return (val_bytes + 4) * nnz + 4 * (n + 1) + 2 * val_bytes * k * n
 
 
@@ -180,10 +184,14 @@ def main():
     tight.ReleaseResources()
 
     vb = 4 if args.precision == "mixed" else 8
+    layout = solver.LevelLayout(0)
     if width > 1:
-        bytes_per_launch = spmm_bytes(nnz, ndof, width, vb)
-        kname = (f"k_spmm_node<{'float' if vb == 4 else 'double'}, K={width}, DOT> (level-0 node-triple block "
-                 f"SpMM Q = A P with per-column P.Q)")
+        bytes_per_launch = spmm_bytes(nnz, ndof, width, vb, layout)
+        tname = "float" if vb == 4 else "double"
+        kname = {"node-block": f"k_nb_spmm<{tname}, K={width}, DOT> (level-0 node-block (3x3) SpMM Q = A P with "
+                               f"per-column P.Q)",
+                 "node-triple": f"k_spmm_node<{tname}, K={width}, DOT> (level-0 node-triple CSR SpMM Q = A P with "
+                                f"per-column P.Q)"}.get(layout, f"k_spmm<{tname}, K={width}, DOT> (level-0 CSR SpMM)")
     else:
         bytes_per_launch = csr_spmv_bytes(nnz, ndof, ndof, vb)
         kname = (f"k_csr_stream<{'float' if vb == 4 else 'double'}, DOT> (level-0 CSR-stream SpMV q = A p with "
@@ -191,7 +199,8 @@ def main():
     avg_ms = spmv_ms / max(spmv_n, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
     traffic = None
-    tag = f"{'spmm' if width > 1 else 'spmv'}_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed"
+    kind = ("spmm_nodeblock" if layout == "node-block" else "spmm") if width > 1 else "spmv"
+    tag = f"{kind}_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed"
     pmc = os.path.join(ROOT, "profiles", f"{tag}_pmc.json")
     if os.path.exists(pmc):
         try:

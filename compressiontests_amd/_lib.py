@@ -125,6 +125,9 @@ _SIGS = {
     "mgx_duvi_spmv": (C.c_int, [C.POINTER(Duvi), _P, _P, _I32, _P]),
     "mgx_duvi_encode": (C.c_int, [_I32, _P, _P, _P, _D, _I32, C.POINTER(DuviHost)]),
     "mgx_duvi_host_free": (None, [C.POINTER(DuviHost)]),
+    "mgx_nodeblock_from_csr": (C.c_int, [_I32, _P, _P, _P, _I32, C.POINTER(_P)]),
+    "mgx_nodeblock_spmm": (C.c_int, [_P, _I32, _P, _P, _I32, _P]),
+    "mgx_nodeblock_destroy": (C.c_int, [_P]),
     "mgx_simp_gradient": (C.c_int, [_I32, _I32, _I32, _D, _P, _D, _D, _D, _P, _P, _P]),
     "mgx_matrix_cantilever": (C.c_int, [_I32, _I32, _I32, _D, _D, _P, C.POINTER(_P)]),
     "mgx_matrix_transfer_reference": (C.c_int, [_I32, _I32, _I32, C.POINTER(_P), C.POINTER(_P)]),
@@ -143,6 +146,7 @@ _SIGS = {
     "mgx_solver_solve": (C.c_int, [_P, _P, _I32, C.POINTER(Stats)]),
     "mgx_solver_solve_device": (C.c_int, [_P, _P, _P, _I32, C.POINTER(Stats)]),
     "mgx_solver_block_width": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
+    "mgx_solver_level_layout": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
     "mgx_solver_release": (C.c_int, [_P]),
     "mgx_device_malloc": (C.c_int, [C.POINTER(_P), _U64]),
     "mgx_device_free": (C.c_int, [_P]),

@@ -136,6 +136,78 @@ struct DevDuvi {
     }
 };
 
+// Node-block (3x3) copy of a node-triple level matrix for the block kernels (mgx_nodeblock.hip):
+// FP32 or FP64 blocks, positions in node order or in a given node order (the colour order of
+// the Gauss-Seidel copy, with each node's own block first).
+struct DevNb {
+    NbView view;                  // view.npos == 0: not built
+    bool fp32 = false;
+    DBuf<int32_t> nrp, ncol, node;
+    DBuf<double> v64;
+    DBuf<float> v32;
+    bool built() const { return view.npos > 0; }
+    void reset()
+    {
+        view = NbView{};
+        nrp.reset(); ncol.reset(); node.reset(); v64.reset(); v32.reset();
+    }
+};
+
+// Builds nb from the device CSR (rp, ci, FP64 or FP32 values v) of nrows = 3 * npos rows.
+// node_dev: npos node indices
+// giving the position order (nullptr: node order).  Returns false, with nothing built, if the
+// matrix is not made of whole 3x3 node blocks or has a node of more than 27 blocks.
+template <typename TI>
+static bool build_nb(DevNb &nb, int32_t nrows, const int32_t *rp, const int32_t *ci, const TI *v,
+                     const int32_t *node_dev, bool diag_first, bool fp32, hipStream_t s)
+{
+    nb.reset();
+    MGX_REQUIRE(nrows % 3 == 0 && nrows > 0, MGX_ERR_INVALID_ARGUMENT, "node blocks need 3 rows per node");
+    MGX_REQUIRE(!(fp32 == false && std::is_same<TI, float>::value), MGX_ERR_INVALID_ARGUMENT,
+                "FP64 node blocks need FP64 values");
+    const int32_t npos = nrows / 3;
+    DBuf<int32_t> cnt, bad;
+    cnt.alloc(npos);
+    bad.alloc(1);
+    MGX_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), s));
+    nb_count(npos, node_dev, rp, ci, diag_first, cnt.p, bad.p, s);
+    int32_t hbad = 0;
+    MGX_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    if (hbad) return false;
+    nb.nrp.alloc((size_t)npos + 1);
+    exclusive_scan_i32(cnt.p, nb.nrp.p, npos, s);
+    int32_t nblk = 0;
+    MGX_HIP(hipMemcpyAsync(&nblk, nb.nrp.p + npos, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    // 16 bytes of slack after ncol and the values: the kernels' staging loads reach past the end
+    nb.ncol.alloc((size_t)nblk + 4);
+    if (node_dev) {
+        nb.node.alloc(npos);
+        MGX_HIP(hipMemcpyAsync(nb.node.p, node_dev, sizeof(int32_t) * npos, hipMemcpyDeviceToDevice, s));
+    }
+    const void *val;
+    if (fp32) {
+        nb.v32.alloc((size_t)nblk * 9 + 4);
+        nb_fill<TI, float>(npos, node_dev, rp, ci, v, nb.nrp.p, diag_first, nb.ncol.p, nb.v32.p, s);
+        val = nb.v32.p;
+    } else {
+        nb.v64.alloc((size_t)nblk * 9 + 2);
+        if constexpr (std::is_same<TI, double>::value)
+            nb_fill<double, double>(npos, node_dev, rp, ci, v, nb.nrp.p, diag_first, nb.ncol.p, nb.v64.p, s);
+        val = nb.v64.p;
+    }
+    MGX_HIP(hipStreamSynchronize(s));
+    nb.fp32 = fp32;
+    nb.view.npos = npos;
+    nb.view.nblk = nblk;
+    nb.view.nrp = nb.nrp.p;
+    nb.view.ncol = nb.ncol.p;
+    nb.view.node = node_dev ? nb.node.p : nullptr;
+    nb.view.val = val;
+    return true;
+}
+
 static void download_csr(const DevCsr &A, std::vector<int32_t> &rp, std::vector<int32_t> &ci, std::vector<double> &v,
                          hipStream_t s)
 {
@@ -155,6 +227,10 @@ struct mgx_matrix_s {
 };
 
 using namespace mgx;
+
+struct mgx_nodeblock_s {
+    mgx::DevNb nb;
+};
 
 // small fill / combine kernels used by the drivers
 __global__ void k_set_i32(int32_t *p, int32_t v) { *p = v; }
@@ -225,6 +301,10 @@ struct Level {
     int32_t gx = 0, gy = 0, gz = 0;        // structured mesh of the level (0: unknown)
     DBuf<int32_t> tile_order;              // node-tile visiting order of the block kernels (empty: natural)
     int tile_order_npb = 0;                // nodes per tile that tile_order was built for
+    DevNb nb;                              // node-block copy for the block SpMM (node order)
+    DevNb nbg;                             // node-block copy for block Gauss-Seidel (colour order, own block first)
+    bool nb_unfit = false;                 // the level matrix is not a node-block matrix the kernels take
+    std::vector<int32_t> color_off;        // colour c = positions color_off[c] .. color_off[c+1]-1 of nbg
     std::vector<DBuf<int32_t>> color_nodes;
     std::vector<int32_t> color_count;
     DevDuvi dA, dR, dP;
@@ -952,12 +1032,21 @@ static inline void fill_zero_t(int64_t n, double *a, const int32_t *d, hipStream
 static inline void fill_zero_t(int64_t n, float *a, const int32_t *d, hipStream_t s) { launch_fill_zero_f32(n, a, d, s); }
 
 template <typename T>
+static bool nb_ready(const DevNb &nb)
+{
+    return nb.built() && nb.fp32 == std::is_same<T, float>::value;
+}
+
+template <typename T>
 static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, bool backward, const int32_t *done)
 {
     const T *Av = BlkSel<T>::val(V.A);
+    const bool nbk = nb_ready<T>(V.nbg);
     for (int i = 0; i < V.ncolors; ++i) {
         const int c = backward ? V.ncolors - 1 - i : i;
-        if (V.node3)
+        if (nbk)
+            launch_nb_gs<T>(K, V.nbg.view, V.color_off[c], V.color_off[c + 1], b, x, backward, done, S->stream);
+        else if (V.node3)
             launch_gs_node<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done,
                               S->stream);
         else
@@ -966,18 +1055,32 @@ static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, b
     }
 }
 
-// Y (op)= A X with the level matrix: node-triple kernel when the level allows it
+// Y (op)= A X with the level matrix: node-block kernel when the level has its copy, else the
+// node-triple CSR kernel, else the generic CSR SpMM
 template <typename TV, typename TX>
 static void level_spmm(Level &V, int K, int epi, const TV *val, const TX *X, const TX *B, const TV *pre, TX *Y,
                        const int32_t *done, double *partials, hipStream_t s)
 {
+    if constexpr (std::is_same<TV, TX>::value) {
+        if (nb_ready<TX>(V.nb)) {
+            launch_nb_spmm<TX>(K, epi, V.nb.view, X, B, pre, Y, done, partials, kPartialStride, s,
+                               V.tile_order_npb == nb_tile<TX>(K) ? V.tile_order.p : nullptr);
+            return;
+        }
+    }
     if (V.node3)
         launch_spmm_node<TV, TX>(K, epi, V.n / 3, V.A.rp, V.A.ci, val, X, B, pre, Y, done, partials, kPartialStride, s,
                                  V.tile_order_npb == spmm_node_tile(K) ? V.tile_order.p : nullptr);
     else
         launch_spmm<TV, TX>(K, epi, V.n, V.A.rp, V.A.ci, val, X, B, pre, Y, done, partials, kPartialStride, s);
 }
-static int level_spmm_parts(const Level &V, int K) { return V.node3 ? spmm_node_grid(K, V.n / 3) : spmm_grid(K, V.n); }
+// per-block partials written by level_spmm's DOT form
+template <typename T>
+static int level_spmm_parts(const Level &V, int K)
+{
+    if (nb_ready<T>(V.nb)) return nb_spmm_grid<T>(K, V.nb.view.npos);
+    return V.node3 ? spmm_node_grid(K, V.n / 3) : spmm_grid(K, V.n);
+}
 
 // the V-cycle of vcycle() applied to K columns at once
 template <typename T>
@@ -1043,7 +1146,7 @@ static void block_iteration(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P,
     if (prof) MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used], s));
     level_spmm<T, T>(V, K, BEPI_DOT, BlkSel<T>::val(V.A), P, nullptr, nullptr, Q, &ctl->done, S->bpartials.p, s);
     if (prof) { MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used + 1], s)); ++S->prof_used; }
-    launch_bfinalize(level_spmm_parts(V, K), S->bpartials.p, ctl, FIN_ALPHA, s);
+    launch_bfinalize(level_spmm_parts<T>(V, K), S->bpartials.p, ctl, FIN_ALPHA, s);
     launch_bupdate<T>(K, n, P, Q, X, R, ctl, S->bpartials.p, s);
     vcycle_block<T>(S, K, 0, R, Z, &ctl->done);
     launch_bdot_beta<T>(K, n, R, Z, ctl, S->bpartials.p, false, s);
@@ -1261,40 +1364,62 @@ static void solve_blocks(mgx_solver_s *S, const double *b, double *x, bool has_x
 static void order_for_cache(mgx_solver_s *S, int K)
 {
     const int L = (int)S->lv.size();
-    const size_t tbytes = S->opt.precision == MGX_PRECISION_MIXED ? sizeof(float) : sizeof(double);
+    const bool mixed = S->opt.precision == MGX_PRECISION_MIXED;
+    const size_t tbytes = mixed ? sizeof(float) : sizeof(double);
+    hipStream_t st = S->stream;
     for (int l = 0; l < L - 1; ++l) {
         Level &V = S->lv[l];
         V.tile_order.reset();
         V.tile_order_npb = 0;
-        if (!V.node3 || V.gx <= 0) continue;
+        if (!V.node3) continue;
+        // node-block copy in node order (independent of K): built once
+        if (!V.nb.built() && !V.nb_unfit)
+            V.nb_unfit = !build_nb<double>(V.nb, V.n, V.A.rp, V.A.ci, V.A.v, nullptr, false, mixed, st);
+        const int npb = V.nb.built() ? (mixed ? nb_tile<float>(K) : nb_tile<double>(K)) : spmm_node_tile(K);
+        bool strips = V.gx > 0;
         const int64_t lines = V.gy + 1, planes = V.gz + 1;
-        if ((int64_t)V.gx * lines * planes * 3 != V.n) continue;   // not the free-node grid of the mesh
+        if ((int64_t)V.gx * lines * planes * 3 != V.n) strips = false;   // not the free-node grid of the mesh
         const double line_bytes = (double)V.gx * 3 * K * tbytes;
-        const int64_t sy = std::max<int64_t>(1, (int64_t)(3.0e6 / (3 * line_bytes)) - 2);
-        if (sy >= lines) continue;                                   // whole planes fit
+        const int64_t sy = strips ? std::max<int64_t>(1, (int64_t)(3.0e6 / (3 * line_bytes)) - 2) : 0;
+        if (sy >= lines) strips = false;                                 // whole planes fit
         auto key = [&](int64_t q) {
             const int64_t ix = q % V.gx, iy = (q / V.gx) % lines, iz = q / (V.gx * lines);
             return (((iy / sy) * planes + iz) * lines + iy) * V.gx + ix;
         };
-        const int npb = spmm_node_tile(K);
-        if (V.gx % npb == 0) {
+        if (strips && V.gx % npb == 0) {
             const int32_t ntiles = (int32_t)(V.n / 3 / npb);
             std::vector<int32_t> ord(ntiles);
             for (int32_t i = 0; i < ntiles; ++i) ord[i] = i;
             std::sort(ord.begin(), ord.end(), [&](int32_t a, int32_t b) { return key((int64_t)a * npb) < key((int64_t)b * npb); });
-            V.tile_order.upload(ord.data(), ord.size(), S->stream);
+            V.tile_order.upload(ord.data(), ord.size(), st);
             V.tile_order_npb = npb;
         }
+        if (V.ncolors == 0) continue;
+        // colour lists in strip order (any order is valid within a colour), then the colour-ordered
+        // node-block copy of the Gauss-Seidel kernels
+        std::vector<int32_t> all;
+        V.color_off.assign(V.ncolors + 1, 0);
         for (int c = 0; c < V.ncolors; ++c) {
             std::vector<int32_t> nodes(V.color_count[c]);
-            if (nodes.empty()) continue;
-            MGX_HIP(hipMemcpyAsync(nodes.data(), V.color_nodes[c].p, sizeof(int32_t) * nodes.size(), hipMemcpyDeviceToHost,
-                                   S->stream));
-            MGX_HIP(hipStreamSynchronize(S->stream));
-            std::sort(nodes.begin(), nodes.end(), [&](int32_t a, int32_t b) { return key(a) < key(b); });
-            MGX_HIP(hipMemcpyAsync(V.color_nodes[c].p, nodes.data(), sizeof(int32_t) * nodes.size(), hipMemcpyHostToDevice,
-                                   S->stream));
-            MGX_HIP(hipStreamSynchronize(S->stream));
+            if (!nodes.empty()) {
+                MGX_HIP(hipMemcpyAsync(nodes.data(), V.color_nodes[c].p, sizeof(int32_t) * nodes.size(),
+                                       hipMemcpyDeviceToHost, st));
+                MGX_HIP(hipStreamSynchronize(st));
+                if (strips) {
+                    std::sort(nodes.begin(), nodes.end(), [&](int32_t a, int32_t b) { return key(a) < key(b); });
+                    MGX_HIP(hipMemcpyAsync(V.color_nodes[c].p, nodes.data(), sizeof(int32_t) * nodes.size(),
+                                           hipMemcpyHostToDevice, st));
+                    MGX_HIP(hipStreamSynchronize(st));
+                }
+            }
+            all.insert(all.end(), nodes.begin(), nodes.end());
+            V.color_off[c + 1] = (int32_t)all.size();
+        }
+        V.nbg.reset();
+        if (V.nb.built() && (int32_t)all.size() == V.n / 3) {
+            DBuf<int32_t> order;
+            order.upload(all.data(), all.size(), st);
+            if (!build_nb<double>(V.nbg, V.n, V.A.rp, V.A.ci, V.A.v, order.p, true, mixed, st)) V.nbg.reset();
         }
     }
 }
@@ -1594,6 +1719,41 @@ int mgx_simp_gradient(int32_t nx, int32_t ny, int32_t nz, double nu, const doubl
     });
 }
 
+// ------------------------------------------------------------------ node-block copies
+int mgx_nodeblock_from_csr(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val, int32_t fp32,
+                           mgx_nodeblock *out)
+{
+    return guard([&] {
+        MGX_CHECK_ROWS(nrows, rowptr);
+        MGX_REQUIRE(col && val && out, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        std::unique_ptr<mgx_nodeblock_s> m(new mgx_nodeblock_s());
+        MGX_REQUIRE(build_nb<double>(m->nb, nrows, rowptr, col, val, nullptr, false, fp32 != 0, nullptr),
+                    MGX_ERR_INVALID_ARGUMENT,
+                    "not a node-block matrix (node triples sharing whole-node columns, at most 27 blocks per node)");
+        *out = m.release();
+    });
+}
+
+int mgx_nodeblock_spmm(mgx_nodeblock A, int32_t k, const void *X, void *Y, int32_t clear, void *stream)
+{
+    return guard([&] {
+        MGX_REQUIRE(A && X && Y, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        MGX_REQUIRE(k == 4 || k == 8 || k == 16 || k == 32, MGX_ERR_INVALID_ARGUMENT, "k must be 4, 8, 16 or 32");
+        const int epi = clear ? BEPI_SPMM : BEPI_ADD;
+        hipStream_t s = (hipStream_t)stream;
+        if (A->nb.fp32)
+            launch_nb_spmm<float>(k, epi, A->nb.view, (const float *)X, nullptr, nullptr, (float *)Y, nullptr, nullptr, 0, s);
+        else
+            launch_nb_spmm<double>(k, epi, A->nb.view, (const double *)X, nullptr, nullptr, (double *)Y, nullptr, nullptr,
+                                   0, s);
+    });
+}
+
+int mgx_nodeblock_destroy(mgx_nodeblock A)
+{
+    return guard([&] { delete A; });
+}
+
 // ------------------------------------------------------------------ matrices
 static void assemble_into(DevCsr &A, int nx, int ny, int nz, const double *Ke, const double *dens_dev,
                           const double *Kel_dev, hipStream_t s)
@@ -1790,7 +1950,9 @@ int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
         if (opt->profile) {
             const int n = std::min(std::max(opt->max_iterations, 1), 4096) + 8;
             s->prof_events.resize(2 * (size_t)n);
-            for (auto &e : s->prof_events) MGX_HIP(hipEventCreate(&e));
+            // timing-only events: no system-scope fence (its L2 write-back and invalidate would
+            // lengthen the bracketed launch and cool the cache of the next one)
+            for (auto &e : s->prof_events) MGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         }
         *out = s.release();
     });
@@ -1884,6 +2046,20 @@ int mgx_solver_block_width(mgx_solver s, int32_t *width, double *tune_ms)
         *width = s->bk;
         if (tune_ms)
             for (int i = 0; i < 4; ++i) tune_ms[i] = i < (int)s->tune_ms.size() ? s->tune_ms[i] : 0.0;
+    });
+}
+
+int mgx_solver_level_layout(mgx_solver s, int32_t level, int32_t *layout)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && layout, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(s->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
+        MGX_REQUIRE(level >= 0 && level < (int)s->lv.size(), MGX_ERR_INVALID_ARGUMENT, "level out of range");
+        const Level &V = s->lv[level];
+        const bool mixed = s->opt.precision == MGX_PRECISION_MIXED;
+        if (s->bk <= 1) *layout = MGX_LAYOUT_CSR;
+        else if (mixed ? nb_ready<float>(V.nb) : nb_ready<double>(V.nb)) *layout = MGX_LAYOUT_NODEBLOCK;
+        else *layout = V.node3 ? MGX_LAYOUT_NODE_TRIPLE : MGX_LAYOUT_CSR;
     });
 }
 
@@ -1986,10 +2162,18 @@ int mgx_time_spmm(int32_t flags, int32_t k, int32_t nrows, const int32_t *rowptr
         MGX_REQUIRE(col && val && X && Y && avg_ms && repeats >= 1 && warmup >= 0, MGX_ERR_INVALID_ARGUMENT,
                     "invalid arguments");
         MGX_REQUIRE(k == 4 || k == 8 || k == 16 || k == 32, MGX_ERR_INVALID_ARGUMENT, "k must be 4, 8, 16 or 32");
-        const bool fp32 = flags & 1, node = flags & 2;
+        const bool fp32 = flags & 1, node = flags & 2, nbk = flags & 8;
         const int epi = (flags & 4) ? BEPI_DOT : BEPI_SPMM;
         DBuf<double> parts;
         if (epi == BEPI_DOT) parts.alloc((size_t)k * kPartialStride);
+        DevNb nbm;
+        if (nbk) {
+            MGX_REQUIRE(!node, MGX_ERR_INVALID_ARGUMENT, "flags: bits 1 and 3 exclude each other");
+            hipStream_t bs = nullptr;
+            const bool ok = fp32 ? build_nb<float>(nbm, nrows, rowptr, col, (const float *)val, nullptr, false, true, bs)
+                                 : build_nb<double>(nbm, nrows, rowptr, col, (const double *)val, nullptr, false, false, bs);
+            MGX_REQUIRE(ok, MGX_ERR_INVALID_ARGUMENT, "not a node-block matrix of at most 27 blocks per node");
+        }
         if (node) {
             // the node-triple kernel reads len(3q) entries of rows 3q+1 and 3q+2: check the lengths
             MGX_REQUIRE(nrows % 3 == 0, MGX_ERR_INVALID_ARGUMENT, "node-triple SpMM needs nrows % 3 == 0");
@@ -2006,7 +2190,13 @@ int mgx_time_spmm(int32_t flags, int32_t k, int32_t nrows, const int32_t *rowptr
         MGX_HIP(hipEventCreate(&e0));
         MGX_HIP(hipEventCreate(&e1));
         auto one = [&]() {
-            if (fp32 && node)
+            if (nbk && fp32)
+                launch_nb_spmm<float>(k, epi, nbm.view, (const float *)X, nullptr, nullptr, (float *)Y, nullptr, parts.p,
+                                      kPartialStride, s);
+            else if (nbk)
+                launch_nb_spmm<double>(k, epi, nbm.view, (const double *)X, nullptr, nullptr, (double *)Y, nullptr,
+                                       parts.p, kPartialStride, s);
+            else if (fp32 && node)
                 launch_spmm_node<float, float>(k, epi, nrows / 3, rowptr, col, (const float *)val, (const float *)X,
                                                nullptr, nullptr, (float *)Y, nullptr, parts.p, kPartialStride, s);
             else if (node)

@@ -167,6 +167,31 @@ void launch_bcopy(int64_t n, const T *src, T *dst, const int32_t *done, hipStrea
 void launch_pack_cols(int32_t n, int K, int kc, const double *src, int64_t ld, double *dst, hipStream_t s);
 void launch_unpack_cols(int32_t n, int K, int kc, const double *src, double *dst, int64_t ld, hipStream_t s);
 
+// node-block (3x3) layout of node-triple level matrices (mgx_nodeblock.hip); the kernels take
+// nodes of at most 27 blocks (nb_count flags larger ones)
+struct NbView {
+    int32_t npos = 0;          // positions (nodes)
+    int64_t nblk = 0;          // 3x3 blocks
+    const int32_t *nrp = nullptr, *ncol = nullptr;
+    const int32_t *node = nullptr;   // node at each position (nullptr: the identity)
+    const void *val = nullptr;       // float or double [9 * nblk] + 16 bytes of slack
+};
+template <typename T>
+int nb_tile(int K);                  // positions per wave tile
+template <typename T>
+int nb_spmm_grid(int K, int32_t npos);
+template <typename T>
+void launch_nb_spmm(int K, int epi, const NbView &A, const T *X, const T *B, const T *pre, T *Y, const int32_t *done,
+                    double *partials, int stride, hipStream_t s, const int32_t *order = nullptr);
+template <typename T>
+void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T *X, bool backward, const int32_t *done,
+                  hipStream_t s);
+void nb_count(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, bool diag_first, int32_t *cnt,
+              int32_t *bad, hipStream_t s);
+template <typename TI, typename TV>
+void nb_fill(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, const TI *v, const int32_t *nrp,
+             bool diag_first, int32_t *ncol, TV *val, hipStream_t s);
+
 // structured assembly
 void launch_assemble_count(int nx, int ny, int nz, int32_t *rowlen, hipStream_t s);
 void launch_assemble_fill(int nx, int ny, int nz, const double *Ke, const double *dens, const double *Kel,

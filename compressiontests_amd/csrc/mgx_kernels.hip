@@ -802,8 +802,8 @@ __device__ __forceinline__ double duvi_value(const void *vi, const double *value
 // distance bytes are copied to LDS with coalesced loads; (2) lane 0 of each row walks the row's
 // chain of blocks (one step per block: the first slot's count) and records every block's first
 // slot; (3) the 8 lanes decode each block with a prefix sum of the gaps, writing every entry's
-// column to LDS; (4) every thread takes entries t, t + 256, ...: value (through the index stream), gather
-// of x, product to LDS; (5) the 8 lanes sum the row.  A tile longer than the LDS capacity, or
+// column to LDS; (4) every thread takes entries t, t + 256, ...: value (through the index stream,
+// loaded before step 1), gather of x, product to LDS; (5) the 8 lanes sum the row.  A tile longer than the LDS capacity, or
 // with more than kDuviBlocks blocks, falls back to one thread per row.
 constexpr int kDuviRows = 32, kDuviE = 11, kDuviCap = kDuviE * kBlock, kDuviBlocks = 512;
 
@@ -847,6 +847,14 @@ __global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t n
             y[row] = acc;
         }
         return;
+    }
+    // the value stream (through the index stream) does not depend on the column decode: issue
+    // its loads first so that they are in flight while the tile is decoded
+    double w[kDuviE];
+#pragma unroll
+    for (int j = 0; j < kDuviE; ++j) {
+        const int k = t + kBlock * j;
+        w[j] = k < cnt ? duvi_value<VIB>(vi, values, d0 + k) : 0.0;
     }
     {
         // distance bytes as aligned 32-bit words; a word that is not wholly inside the array is
@@ -901,12 +909,6 @@ __global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t n
         }
     }
     __syncthreads();
-    double w[kDuviE];
-#pragma unroll
-    for (int j = 0; j < kDuviE; ++j) {
-        const int k = t + kBlock * j;
-        w[j] = k < cnt ? duvi_value<VIB>(vi, values, d0 + k) : 0.0;
-    }
     double xv[kDuviE];
 #pragma unroll
     for (int j = 0; j < kDuviE; ++j) {

@@ -1,0 +1,430 @@
+// mgx_nodeblock.hip -- node-block (3x3) layout of the level matrices of the batching form, and
+// its SpMM and multicolour Gauss-Seidel kernels.
+//
+// Every level matrix of the 3-dof hexahedral hierarchy has its rows in node triples (3q, 3q+1,
+// 3q+2) sharing one column list made of whole neighbour nodes (columns 3j, 3j+1, 3j+2).  Stored
+// as 3x3 blocks the matrix needs one column index per 9 values instead of one per value (FP32:
+// 4.44 bytes per nonzero instead of 8; FP64: 8.44 instead of 12), and the lane that gathers a
+// neighbour node's 3 x W values of X applies the whole block to them.
+//
+// Layout of one copy: positions p = 0 .. npos-1 (node[p] = the node at position p; no node
+// array = the identity), blocks nrp[p] .. nrp[p+1]-1 of position p, ncol[b] = the neighbour node
+// of block b, val[9b + 3a + c] = entry (row 3q+a, column 3j+c).  The SpMM copy is in node order
+// with the blocks of a node in column order; the Gauss-Seidel copy lists the nodes colour by
+// colour with each node's own (diagonal) block first.  val carries 16 bytes of slack at the end
+// (the staging loads below round the tile's byte range out to 16-byte boundaries).
+//
+// Kernels: a wavefront owns a tile of NW consecutive positions.  It copies the tile's blocks
+// (one contiguous range of ncol and of val) into its own LDS region with coalesced 16-byte
+// non-temporal loads, then G lanes handle one node: lane (s, v) takes the W columns v*W ..
+// v*W+W-1 (one 16-byte vector of X per dof row) and every S-th block of the node.  Waves proceed
+// independently (no block-wide barrier); LDS operations of one wavefront execute in order, so a
+// compiler barrier plus a wait for the wave's own LDS writes separates staging from use.
+#include <type_traits>
+
+#include "mgx_device.h"
+
+namespace mgx {
+
+typedef unsigned int u32x4 __attribute__((ext_vector_type(4)));
+
+template <typename T> struct NbVec;
+template <> struct NbVec<float> { typedef float type __attribute__((ext_vector_type(4))); };
+template <> struct NbVec<double> { typedef double type __attribute__((ext_vector_type(2))); };
+
+constexpr int kNbMaxBlocks = 27;   // neighbour nodes of the 27-node hexahedral stencil
+
+template <typename T, int K> struct NbShape {
+    static constexpr int W = 16 / (int)sizeof(T);           // columns per lane
+    static constexpr int V = K / W;                         // lanes covering the K columns of one row
+    static constexpr int GMIN = 2 * (int)sizeof(T);         // 8 lanes per node (FP32), 16 (FP64)
+    static constexpr int G = V > GMIN ? V : GMIN;           // lanes per node
+    static constexpr int S = G / V;                         // lanes splitting a node's blocks
+    static constexpr int NW = 64 / G;                       // nodes per wave tile
+    static constexpr int MB = kNbMaxBlocks;                 // blocks per node staged
+    // 16-byte LDS slots per wave for the values and the columns of a tile (+ alignment), rounded
+    // up to whole 64-slot rounds of stage16
+    static constexpr int VQ = ((NW * MB * 9 * (int)sizeof(T)) / 16 + 2 + 63) / 64 * 64;
+    static constexpr int CQ = ((NW * MB * 4) / 16 + 2 + 63) / 64 * 64;
+    static constexpr int U = 3;                             // blocks in flight per lane
+};
+static_assert(NbShape<float, 4>::G == 8 && NbShape<float, 32>::S == 1 && NbShape<double, 32>::NW == 4, "shape");
+
+constexpr int kNbWaves = kBlock / 64;
+
+// wait for this wave's LDS writes and keep the compiler from moving LDS accesses across
+__device__ __forceinline__ void wave_lds_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);   // lgkmcnt(0), vmcnt and expcnt unconstrained
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// copy bytes [lo, hi) of src into dst[0 ..) shifted by lo & 15, with 16-byte loads that are all
+// issued before the first LDS store.  Every lane loads and stores NI = NQ/64 slots without
+// predication: slots past the range reload its last 16 bytes (same cache line) into dst slots
+// < 64 * NI that the tile does not use.  The loads reach up to 15 bytes past hi.
+template <int NQ>
+__device__ __forceinline__ void stage16(const char *__restrict__ src, size_t lo, size_t hi, u32x4 *dst, int lane)
+{
+    constexpr int NI = NQ / 64;
+    const size_t a0 = lo & ~(size_t)15;
+    const int last = max((int)((hi - a0 + 15) >> 4), 1) - 1;
+    const u32x4 *s = (const u32x4 *)(src + a0);
+    u32x4 t[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) t[i] = __builtin_nontemporal_load(s + min(lane + 64 * i, last));
+#pragma unroll
+    for (int i = 0; i < NI; ++i) dst[lane + 64 * i] = t[i];
+}
+
+// Sum over the blocks m = m0, m0 + S, ... < m1 of (block m) x (X rows of its neighbour node), for
+// columns c0 .. c0+W-1: y[a] += sum_c A_m[a][c] X[3 j_m + c].  cols/vals: the wave's staged tile
+// in LDS, block indices relative to the tile.  Groups of U blocks are loaded before use; the
+// remainder goes one block at a time (no predicated loads).
+template <typename T, int K>
+__device__ __forceinline__ void nb_block(const T *a, const typename NbVec<T>::type (&x)[3], typename NbVec<T>::type (&y)[3])
+{
+    // one FMA per entry and column, accumulated straight into y
+#pragma unroll
+    for (int c = 0; c < 3; ++c)
+#pragma unroll
+        for (int r = 0; r < 3; ++r) y[r] += a[3 * r + c] * x[c];
+}
+
+template <typename T, int K>
+__device__ __forceinline__ void nb_accumulate(const int32_t *cols, const T *vals, int m0, int m1, const T *__restrict__ X,
+                                              int c0, typename NbVec<T>::type (&y)[3])
+{
+    using Sh = NbShape<T, K>;
+    using VT = typename NbVec<T>::type;
+    int m = m0;
+    for (; m + (Sh::U - 1) * Sh::S < m1; m += Sh::U * Sh::S) {
+        VT x[Sh::U][3];
+#pragma unroll
+        for (int u = 0; u < Sh::U; ++u) {
+            const T *xr = X + (size_t)3 * cols[m + u * Sh::S] * K + c0;
+#pragma unroll
+            for (int c = 0; c < 3; ++c) x[u][c] = *(const VT *)(xr + c * K);
+        }
+#pragma unroll
+        for (int u = 0; u < Sh::U; ++u) nb_block<T, K>(vals + (size_t)(m + u * Sh::S) * 9, x[u], y);
+    }
+    for (; m < m1; m += Sh::S) {
+        VT x[3];
+        const T *xr = X + (size_t)3 * cols[m] * K + c0;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) x[c] = *(const VT *)(xr + c * K);
+        nb_block<T, K>(vals + (size_t)m * 9, x, y);
+    }
+}
+
+// reduce the S partial sums of a node (lanes v, v + V, ... of its G-lane group)
+template <typename T, int K>
+__device__ __forceinline__ void nb_reduce(typename NbVec<T>::type (&y)[3])
+{
+    using Sh = NbShape<T, K>;
+#pragma unroll
+    for (int o = Sh::V; o < Sh::G; o <<= 1)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int w = 0; w < Sh::W; ++w) y[r][w] += __shfl_xor(y[r][w], o, Sh::G);
+}
+
+// K column partials of a block: thread t holds columns (t % V) * W .. + W - 1
+// (red: W * kBlock doubles of LDS, here the staging area, free once every wave is past its tiles)
+template <typename T, int K>
+__device__ __forceinline__ void nb_publish(const double *dot, double *red, double *__restrict__ partials, int stride)
+{
+    using Sh = NbShape<T, K>;
+    const int t = threadIdx.x;
+    __syncthreads();
+#pragma unroll
+    for (int w = 0; w < Sh::W; ++w) red[w * kBlock + t] = dot[w];
+    __syncthreads();
+    if (t < K) {
+        const int v = t / Sh::W, w = t % Sh::W;
+        double a = 0;
+        for (int i = v; i < kBlock; i += Sh::V) a += red[w * kBlock + i];
+        partials[(size_t)t * stride + blockIdx.x] = a;
+    }
+}
+
+// ------------------------------------------------------------------ SpMM  Y (op)= A X
+template <typename T, int K, int EPI>
+__global__ __launch_bounds__(kBlock) void k_nb_spmm(int32_t npos, const int32_t *__restrict__ nrp,
+                                                    const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
+                                                    const T *__restrict__ val, const T *__restrict__ X,
+                                                    const T *__restrict__ B, const T *__restrict__ pre,
+                                                    T *__restrict__ Y, const int32_t *__restrict__ done,
+                                                    double *__restrict__ partials, int stride,
+                                                    const int32_t *__restrict__ order)
+{
+    using Sh = NbShape<T, K>;
+    using VT = typename NbVec<T>::type;
+    constexpr int W = Sh::W;
+    __shared__ u32x4 s_val[kNbWaves][Sh::VQ];
+    __shared__ u32x4 s_col[kNbWaves][Sh::CQ];
+    if (done && *done) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane / Sh::G, l = lane % Sh::G, s = l / Sh::V, c0 = (l % Sh::V) * W;
+    const int ntiles = (npos + Sh::NW - 1) / Sh::NW;
+    // XCD x (blocks b with b % 8 == x) walks the x-th eighth of the tiles
+    const int xcd = blockIdx.x & 7;
+    const int per = (gridDim.x >> 3) * kNbWaves;
+    const int hi = (int)((int64_t)ntiles * (xcd + 1) / 8);
+    double dot[W];
+#pragma unroll
+    for (int w = 0; w < W; ++w) dot[w] = 0;
+    for (int pos = (int)((int64_t)ntiles * xcd / 8) + (int)(blockIdx.x >> 3) * kNbWaves + wv; pos < hi; pos += per) {
+        const int tile = order ? order[pos] : pos;
+        const int p0 = tile * Sh::NW, p1 = min(p0 + Sh::NW, npos);
+        const int b0 = nrp[p0], b1 = nrp[p1];
+        const int p = p0 + g;
+        VT y[3] = {VT(0), VT(0), VT(0)};
+        wave_lds_fence();                                        // the previous tile's LDS reads are issued
+        stage16<Sh::VQ>((const char *)val, (size_t)b0 * 9 * sizeof(T), (size_t)b1 * 9 * sizeof(T), s_val[wv], lane);
+        stage16<Sh::CQ>((const char *)ncol, (size_t)b0 * 4, (size_t)b1 * 4, s_col[wv], lane);
+        wave_lds_fence();
+        const T *lv = (const T *)((const char *)s_val[wv] + (((size_t)b0 * 9 * sizeof(T)) & 15));
+        const int32_t *lc = (const int32_t *)((const char *)s_col[wv] + (((size_t)b0 * 4) & 15));
+        if (p < p1) nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + s, nrp[p + 1] - b0, X, c0, y);
+        nb_reduce<T, K>(y);
+        if (p < p1 && s == 0) {
+            const int q = node ? node[p] : p;
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                const size_t i = ((size_t)3 * q + a) * K + c0;
+                VT *yp = (VT *)(Y + i);
+                if (EPI == BEPI_SPMM) *yp = y[a];
+                else if (EPI == BEPI_ADD) *yp = *yp + y[a];
+                else if (EPI == BEPI_RESID) *yp = *(const VT *)(B + i) - y[a];
+                else if (EPI == BEPI_JACOBI) *yp = *(const VT *)(X + i) + pre[3 * q + a] * (*(const VT *)(B + i) - y[a]);
+                else {
+                    *yp = y[a];
+                    const VT xp = *(const VT *)(X + i);
+#pragma unroll
+                    for (int w = 0; w < W; ++w) dot[w] += (double)xp[w] * (double)y[a][w];
+                }
+            }
+        }
+    }
+    if (EPI == BEPI_DOT) nb_publish<T, K>(dot, (double *)&s_val[0][0], partials, stride);
+}
+
+template <typename T>
+int nb_tile(int K)
+{
+    switch (K) {
+    case 4: return NbShape<T, 4>::NW;
+    case 8: return NbShape<T, 8>::NW;
+    case 16: return NbShape<T, 16>::NW;
+    default: return NbShape<T, 32>::NW;
+    }
+}
+template int nb_tile<float>(int);
+template int nb_tile<double>(int);
+
+// waves of the bounded SpMM grid: at most 1024 blocks (4 waves each), a multiple of 8
+template <typename T>
+int nb_spmm_grid(int K, int32_t npos)
+{
+    const int64_t ntiles = (npos + nb_tile<T>(K) - 1) / nb_tile<T>(K);
+    int64_t nb = (ntiles + kNbWaves - 1) / kNbWaves;
+    nb = std::min<int64_t>(std::max<int64_t>(nb, 1), 1024);
+    return (int)((nb + 7) & ~int64_t(7));
+}
+template int nb_spmm_grid<float>(int, int32_t);
+template int nb_spmm_grid<double>(int, int32_t);
+
+template <typename T>
+void launch_nb_spmm(int K, int epi, const NbView &A, const T *X, const T *B, const T *pre, T *Y, const int32_t *done,
+                    double *partials, int stride, hipStream_t s, const int32_t *order)
+{
+    if (A.npos <= 0) return;
+    const int g = nb_spmm_grid<T>(K, A.npos);
+#define MGX_NBS(KK, EE)                                                                                          \
+    hipLaunchKernelGGL((k_nb_spmm<T, KK, EE>), dim3(g), dim3(kBlock), 0, s, A.npos, A.nrp, A.ncol, A.node,     \
+                       (const T *)A.val, X, B, pre, Y, done, partials, stride, order)
+#define MGX_NBS_K(KK)                                       \
+    switch (epi) {                                          \
+    case BEPI_SPMM: MGX_NBS(KK, BEPI_SPMM); break;          \
+    case BEPI_ADD: MGX_NBS(KK, BEPI_ADD); break;            \
+    case BEPI_RESID: MGX_NBS(KK, BEPI_RESID); break;        \
+    case BEPI_JACOBI: MGX_NBS(KK, BEPI_JACOBI); break;      \
+    default: MGX_NBS(KK, BEPI_DOT); break;                  \
+    }
+    switch (K) {
+    case 4: MGX_NBS_K(4); break;
+    case 8: MGX_NBS_K(8); break;
+    case 16: MGX_NBS_K(16); break;
+    case 32: MGX_NBS_K(32); break;
+    default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
+    }
+#undef MGX_NBS_K
+#undef MGX_NBS
+    MGX_HIP(hipGetLastError());
+}
+template void launch_nb_spmm<float>(int, int, const NbView &, const float *, const float *, const float *, float *,
+                                    const int32_t *, double *, int, hipStream_t, const int32_t *);
+template void launch_nb_spmm<double>(int, int, const NbView &, const double *, const double *, const double *, double *,
+                                     const int32_t *, double *, int, hipStream_t, const int32_t *);
+
+// ------------------------------------------------------------------ multicolour Gauss-Seidel
+// Positions [p0, p1) of the colour-ordered copy (one colour: no two of its nodes are coupled).
+// Each node: a[r] = sum over its off-diagonal blocks (slots 1 ..) of A X, then the three dofs are
+// relaxed in order with the node's own block D (slot 0): x_0 = (b_0 - a_0 - D01 x_1 - D02 x_2) /
+// D00, then x_1 with the new x_0, then x_2 (reverse order when backward).
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, const int32_t *__restrict__ nrp,
+                                                  const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
+                                                  const T *__restrict__ val, const T *__restrict__ B, T *__restrict__ X,
+                                                  int backward, const int32_t *__restrict__ done)
+{
+    using Sh = NbShape<T, K>;
+    using VT = typename NbVec<T>::type;
+    __shared__ u32x4 s_val[kNbWaves][Sh::VQ];
+    __shared__ u32x4 s_col[kNbWaves][Sh::CQ];
+    if (done && *done) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = lane / Sh::G, l = lane % Sh::G, s = l / Sh::V, c0 = (l % Sh::V) * Sh::W;
+    const int ntiles = (p1c - p0c + Sh::NW - 1) / Sh::NW;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x) * kNbWaves + wv;
+    if (tile >= ntiles) return;                                  // the whole wave leaves together
+    const int pa = p0c + tile * Sh::NW, pb = min(pa + Sh::NW, p1c);
+    const int b0 = nrp[pa], b1 = nrp[pb];
+    const int p = pa + g;
+    VT y[3] = {VT(0), VT(0), VT(0)};
+    stage16<Sh::VQ>((const char *)val, (size_t)b0 * 9 * sizeof(T), (size_t)b1 * 9 * sizeof(T), s_val[wv], lane);
+    stage16<Sh::CQ>((const char *)ncol, (size_t)b0 * 4, (size_t)b1 * 4, s_col[wv], lane);
+    wave_lds_fence();
+    const T *lv = (const T *)((const char *)s_val[wv] + (((size_t)b0 * 9 * sizeof(T)) & 15));
+    const int32_t *lc = (const int32_t *)((const char *)s_col[wv] + (((size_t)b0 * 4) & 15));
+    if (p < pb) nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + 1 + s, nrp[p + 1] - b0, X, c0, y);
+    nb_reduce<T, K>(y);
+    if (p >= pb || s != 0) return;
+    const int q = node ? node[p] : p;
+    const T *d = lv + (size_t)(nrp[p] - b0) * 9;                 // the node's own block (slot 0)
+    const size_t i0 = (size_t)3 * q * K + c0;
+    VT x0 = *(const VT *)(X + i0), x1 = *(const VT *)(X + i0 + K), x2 = *(const VT *)(X + i0 + 2 * K);
+    const VT r0 = *(const VT *)(B + i0) - y[0], r1 = *(const VT *)(B + i0 + K) - y[1],
+             r2 = *(const VT *)(B + i0 + 2 * K) - y[2];
+    if (!backward) {
+        x0 = (r0 - d[1] * x1 - d[2] * x2) / d[0];
+        x1 = (r1 - d[3] * x0 - d[5] * x2) / d[4];
+        x2 = (r2 - d[6] * x0 - d[7] * x1) / d[8];
+    } else {
+        x2 = (r2 - d[6] * x0 - d[7] * x1) / d[8];
+        x1 = (r1 - d[3] * x0 - d[5] * x2) / d[4];
+        x0 = (r0 - d[1] * x1 - d[2] * x2) / d[0];
+    }
+    *(VT *)(X + i0) = x0;
+    *(VT *)(X + i0 + K) = x1;
+    *(VT *)(X + i0 + 2 * K) = x2;
+}
+
+template <typename T>
+void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T *X, bool backward, const int32_t *done,
+                  hipStream_t s)
+{
+    if (p1 <= p0) return;
+    const int nw = nb_tile<T>(K);
+    const int g = grid_rows((p1 - p0 + nw - 1) / nw, kNbWaves);
+#define MGX_NBG(KK)                                                                                              \
+    hipLaunchKernelGGL((k_nb_gs<T, KK>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.ncol, A.node,           \
+                       (const T *)A.val, B, X, backward ? 1 : 0, done)
+    switch (K) {
+    case 4: MGX_NBG(4); break;
+    case 8: MGX_NBG(8); break;
+    case 16: MGX_NBG(16); break;
+    case 32: MGX_NBG(32); break;
+    default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
+    }
+#undef MGX_NBG
+    MGX_HIP(hipGetLastError());
+}
+template void launch_nb_gs<float>(int, const NbView &, int32_t, int32_t, const float *, float *, bool, const int32_t *,
+                                  hipStream_t);
+template void launch_nb_gs<double>(int, const NbView &, int32_t, int32_t, const double *, double *, bool,
+                                   const int32_t *, hipStream_t);
+
+// ------------------------------------------------------------------ building a copy from CSR
+// Blocks per position, and the checks that make the CSR a node-block matrix: the three rows of
+// the node have one column list, its length a multiple of 3, and every column triple is a whole
+// node (3j, 3j+1, 3j+2).  Bit 0 of *bad
+// on any violation (with diag_first, also when a node has no block of its own); bit 1 when a node
+// has more blocks than the kernels stage per node (kNbMaxBlocks).
+__global__ __launch_bounds__(kBlock) void k_nb_count(int32_t npos, const int32_t *__restrict__ node,
+                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                     int diag_first, int32_t *__restrict__ cnt, int32_t *__restrict__ bad)
+{
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npos) return;
+    const int q = node ? node[p] : p;
+    const int s0 = rp[3 * q], len = rp[3 * q + 1] - s0;
+    bool ok = len % 3 == 0 && rp[3 * q + 2] - rp[3 * q + 1] == len && rp[3 * q + 3] - rp[3 * q + 2] == len;
+    bool own = false;
+    for (int m = 0; ok && m < len / 3; ++m) {
+        const int c = ci[s0 + 3 * m];
+        ok = c % 3 == 0 && ci[s0 + 3 * m + 1] == c + 1 && ci[s0 + 3 * m + 2] == c + 2;
+        own = own || c == 3 * q;
+    }
+    for (int k = 0; ok && k < len; ++k) ok = ci[s0 + len + k] == ci[s0 + k] && ci[s0 + 2 * len + k] == ci[s0 + k];
+    if (!ok || (diag_first && !own)) atomicOr(bad, 1);
+    if (len / 3 > kNbMaxBlocks) atomicOr(bad, 2);
+    cnt[p] = ok ? len / 3 : 0;
+}
+
+template <typename TI, typename TV>
+__global__ __launch_bounds__(kBlock) void k_nb_fill(int32_t npos, const int32_t *__restrict__ node,
+                                                    const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
+                                                    const TI *__restrict__ v, const int32_t *__restrict__ nrp,
+                                                    int diag_first, int32_t *__restrict__ ncol, TV *__restrict__ val)
+{
+    const int p = blockIdx.x * kBlock + threadIdx.x;
+    if (p >= npos) return;
+    const int q = node ? node[p] : p;
+    const int nb = nrp[p + 1] - nrp[p];
+    if (nb == 0) return;
+    const int r0 = rp[3 * q];
+    int own = -1;
+    if (diag_first)
+        for (int m = 0; m < nb; ++m) if (ci[r0 + 3 * m] == 3 * q) own = m;
+    for (int m = 0; m < nb; ++m) {
+        const int slot = nrp[p] + (own < 0 ? m : (m == own ? 0 : (m < own ? m + 1 : m)));
+        ncol[slot] = ci[r0 + 3 * m] / 3;
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) val[(size_t)slot * 9 + 3 * a + c] = (TV)v[rp[3 * q + a] + 3 * m + c];
+    }
+}
+
+void nb_count(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, bool diag_first, int32_t *cnt,
+              int32_t *bad, hipStream_t s)
+{
+    if (npos <= 0) return;
+    hipLaunchKernelGGL(k_nb_count, dim3((npos + kBlock - 1) / kBlock), dim3(kBlock), 0, s, npos, node, rp, ci,
+                       diag_first ? 1 : 0, cnt, bad);
+    MGX_HIP(hipGetLastError());
+}
+
+template <typename TI, typename TV>
+void nb_fill(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, const TI *v, const int32_t *nrp,
+             bool diag_first, int32_t *ncol, TV *val, hipStream_t s)
+{
+    if (npos <= 0) return;
+    hipLaunchKernelGGL((k_nb_fill<TI, TV>), dim3((npos + kBlock - 1) / kBlock), dim3(kBlock), 0, s, npos, node, rp, ci,
+                       v, nrp, diag_first ? 1 : 0, ncol, val);
+    MGX_HIP(hipGetLastError());
+}
+template void nb_fill<double, double>(int32_t, const int32_t *, const int32_t *, const int32_t *, const double *,
+                                      const int32_t *, bool, int32_t *, double *, hipStream_t);
+template void nb_fill<double, float>(int32_t, const int32_t *, const int32_t *, const int32_t *, const double *,
+                                     const int32_t *, bool, int32_t *, float *, hipStream_t);
+template void nb_fill<float, float>(int32_t, const int32_t *, const int32_t *, const int32_t *, const float *,
+                                    const int32_t *, bool, int32_t *, float *, hipStream_t);
+
+}  // namespace mgx

@@ -8,8 +8,9 @@ the resulting GB/s against the 8 TB/s HBM peak.
 `copy` is a plain device-to-device copy kernel of known size, the calibration for rocprofv3's
 FETCH_SIZE / WRITE_SIZE counters.  `spmm64_k32` / `spmm32_k32` (any k in 4, 8, 16, 32): the
 block SpMM Y = A X of the batching form, FP64 or FP32 values and vectors (suffix `_node`: the
-node-triple kernel the block solver uses on the stiffness matrices; `_node_dot`: its block-PCG
-form q = A p with the per-column p.q partials).
+node-triple CSR kernel; `_nb`: the node-block kernel the block solver uses on the level
+matrices, on a 3x3 block copy built from the CSR; suffix `_dot`: the block-PCG form q = A p with the
+per-column p.q partials).
 """
 import argparse
 import ctypes as C
@@ -70,6 +71,7 @@ def main():
         elif kind.startswith("spmm"):
             fp32 = kind.startswith("spmm32")
             node = "_node" in kind
+            nodeblock = "_nb" in kind
             dot = kind.endswith("_dot")
             k = int(kind.split("_k")[1].split("_")[0])
             dt = np.float32 if fp32 else np.float64
@@ -77,9 +79,13 @@ def main():
             Ak = DeviceCsr.arrays_from_host(A.to_host(), val_dtype=dt) if fp32 else A
             X = DeviceArray.from_numpy(rng_normal(args.seed + 3, N * k).astype(dt))
             Y = DeviceArray(N * k, dt)
-            ms = kernels.time_spmm(Ak, X, Y, k, warmup=args.warmup, repeats=args.repeats, node=node, dot=dot)
+            ms = kernels.time_spmm(Ak, X, Y, k, warmup=args.warmup, repeats=args.repeats, node=node, dot=dot,
+                                   nodeblock=nodeblock)
             vb = 4 if fp32 else 8
-            by = (vb + 4) * nnz + 4 * (N + 1) + 2 * vb * k * N
+            if nodeblock:   # 3x3 blocks: one column index and one block offset per 9 values / node
+                by = vb * nnz + 4 * (nnz // 9) + 4 * (N // 3 + 1) + 2 * vb * k * N
+            else:
+                by = (vb + 4) * nnz + 4 * (N + 1) + 2 * vb * k * N
         elif kind.startswith("duvi"):
             mode = {"duvi": 0, "duvi_u32": 4, "duvi_raw": 8}[kind]
             d = duvi_encode(A.to_host(), 1e-10, mode)

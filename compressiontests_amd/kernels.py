@@ -22,6 +22,34 @@ def spmm(A, X, Y, k, clear=True, stream=None):
            stream)
 
 
+class NodeBlockMatrix:
+    """Node-block (3x3 block CSR) copy of a device CSR whose rows come in node triples sharing
+    whole-node columns: the layout of the batching form's SpMM (FP32 blocks if fp32)."""
+
+    def __init__(self, A, fp32=False):
+        h = C.c_void_p()
+        L.call("mgx_nodeblock_from_csr", A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val),
+               int(fp32), C.byref(h))
+        self.handle, self.fp32, self.nrows = h.value, fp32, A.nrows
+
+    def spmm(self, X, Y, k, clear=True, stream=None):
+        """Y = A X (clear) or Y += A X; X, Y row-major (rows x k), float32 for an FP32 copy."""
+        assert X.dtype == (np.float32 if self.fp32 else np.float64) and Y.dtype == X.dtype
+        assert X.n == self.nrows * k and Y.n == self.nrows * k
+        L.call("mgx_nodeblock_spmm", C.c_void_p(self.handle), k, X.vp, Y.vp, int(clear), stream)
+
+    def free(self):
+        if self.handle is not None:
+            L.lib().mgx_nodeblock_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def residual(A, b, x, r, stream=None):
     """r = b - A x.  CsrGeometricMultigrid.cl:96-110."""
     L.call("mgx_csr_residual", A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), b.vp, x.vp, r.vp,
@@ -69,12 +97,14 @@ def time_csr_kernel(kind, A, x, y, b=None, pre=None, warmup=5, repeats=100):
     return out.value
 
 
-def time_spmm(A, X, Y, k, warmup=5, repeats=100, node=False, dot=False):
+def time_spmm(A, X, Y, k, warmup=5, repeats=100, node=False, dot=False, nodeblock=False):
     """Average device ms per launch of Y = A X with k right-hand sides (FP32 if X is float32;
-    node: the node-triple kernel; dot: its block-PCG form with the per-column X.Y partials)."""
+    node: the node-triple CSR kernel; nodeblock: the node-block kernel on a copy built from A;
+    dot: the block-PCG form with the per-column X.Y partials)."""
     out = C.c_double()
-    L.call("mgx_time_spmm", int(X.dtype == np.float32) | (2 if node else 0) | (4 if dot else 0), k, A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col),
-           C.c_void_p(A.val), X.vp, Y.vp, warmup, repeats, C.byref(out))
+    flags = int(X.dtype == np.float32) | (2 if node else 0) | (4 if dot else 0) | (8 if nodeblock else 0)
+    L.call("mgx_time_spmm", flags, k, A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), X.vp, Y.vp,
+           warmup, repeats, C.byref(out))
     return out.value
 
 

@@ -127,6 +127,11 @@ class _Handle:
         L.call("mgx_solver_block_width", C.c_void_p(self.h), C.byref(w), ms)
         return w.value, list(ms)
 
+    def level_layout(self, level):
+        v = C.c_int32()
+        L.call("mgx_solver_level_layout", C.c_void_p(self.h), level, C.byref(v))
+        return v.value
+
     def release(self):
         if self.h:
             L.call("mgx_solver_release", C.c_void_p(self.h))
@@ -339,6 +344,10 @@ class HipMgPcgSolver:
     def BlockWidth(self):
         """(block width in use, ms per block PCG iteration timed for K = 4, 8, 16, 32)"""
         return self._h.block_width()
+
+    def LevelLayout(self, level=0):
+        """Layout the block kernels read on a level: "csr", "node-triple" or "node-block"."""
+        return ("csr", "node-triple", "node-block")[self._h.level_layout(level)]
 
     def solve_device(self, b_dev, x_dev):
         return self._h.solve_device(b_dev, x_dev, False, self.nrhs)

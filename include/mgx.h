@@ -132,6 +132,20 @@ int mgx_duvi_encode(int32_t nrows, const int32_t *rowptr, const int32_t *col, co
                     double tol, int32_t mode, mgx_duvi_host *out);
 void mgx_duvi_host_free(mgx_duvi_host *m);
 
+/* Node-block (3x3 block CSR) copy of a matrix whose rows come in node triples (3q, 3q+1, 3q+2)
+ * sharing one column list made of whole nodes (3j, 3j+1, 3j+2) -- every level matrix of the
+ * 3-dof hexahedral hierarchy.  This is the layout the batching form's SpMM and Gauss-Seidel read
+ * (one column index per 9 values).  Built on the device from a device CSR with FP64 values;
+ * fp32 = 1 stores FP32 blocks.  MGX_ERR_INVALID_ARGUMENT if the matrix is not of that form or a
+ * node has more than 27 neighbour nodes.  No reference counterpart (batching form). */
+typedef struct mgx_nodeblock_s *mgx_nodeblock;
+int mgx_nodeblock_from_csr(int32_t nrows, const int32_t *rowptr, const int32_t *col, const double *val,
+                           int32_t fp32, mgx_nodeblock *out);
+/* Y = A X (clear = 1) or Y += A X (clear = 0); X, Y: n x k row-major device blocks of the copy's
+ * type (float for an FP32 copy, double otherwise), k in {4, 8, 16, 32}. */
+int mgx_nodeblock_spmm(mgx_nodeblock A, int32_t k, const void *X, void *Y, int32_t clear, void *stream);
+int mgx_nodeblock_destroy(mgx_nodeblock A);
+
 /* SIMP compliance sensitivity, one thread per element (no reference counterpart; the hook
  * is ElementDensities x Ke, FemCantileverBase.cs:99-100):
  *   grad_e = -penal * rho_e^(penal-1) * (E0 - Emin) * u_e^T Ke0 u_e
@@ -271,6 +285,12 @@ int mgx_solver_solve_device(mgx_solver s, const double *b_dev, double *x_dev, in
  * asked for the automatic choice, the timed ms per block PCG iteration for K = 4, 8, 16, 32
  * (0 where not timed); tune_ms may be NULL.  No reference counterpart (batching form). */
 int mgx_solver_block_width(mgx_solver s, int32_t *width, double *tune_ms);
+/* Matrix layout the block kernels read on a level (batching form; MGX_LAYOUT_CSR whenever the
+ * right-hand sides are solved one at a time).  No reference counterpart. */
+#define MGX_LAYOUT_CSR          0   /* CSR (row kernels / generic SpMM) */
+#define MGX_LAYOUT_NODE_TRIPLE  1   /* CSR read by node triples (rows 3q..3q+2 share their columns) */
+#define MGX_LAYOUT_NODEBLOCK    2   /* 3x3 node blocks (mgx_nodeblock_from_csr) */
+int mgx_solver_level_layout(mgx_solver s, int32_t level, int32_t *layout);
 /* Replaces ReleaseOpenCLResources (OpenCLCsrGeometricMultigridSolver.cs:106-138). */
 int mgx_solver_release(mgx_solver s);
 
@@ -294,7 +314,9 @@ int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, cons
 /* Same for the block SpMM Y = A X with k right-hand sides.  flags bit 0: FP32 values and vectors
  * (mgx_csr_spmm_f32) instead of FP64; bit 1: the node-triple kernel of the block solver (rows
  * 3q..3q+2 share one column list; triples of unequal length are rejected); bit 2: the block
- * PCG's form that also produces the per-column partials of X.Y (Y = A X itself unchanged). */
+ * PCG's form that also produces the per-column partials of X.Y (Y = A X itself unchanged); bit 3: the
+ * node-block kernel (mgx_nodeblock_spmm; the copy is built from the CSR before timing, FP32 blocks
+ * with bit 0; takes bit 2, not bit 1). */
 int mgx_time_spmm(int32_t flags, int32_t k, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
                   const void *X, void *Y, int32_t warmup, int32_t repeats, double *avg_ms);
 /* Average device time of a plain copy kernel (16-byte vector loads and stores) moving `bytes`

@@ -77,6 +77,70 @@ def test_spmm_matches_columnwise_products(het16, k):
     assert np.max(np.abs(Yn32.numpy().reshape(A.nrows, k) - ref)) <= 1e-5 * np.max(np.abs(ref))
 
 
+def random_node_block_csr(nnodes, seed, max_nbrs=27):
+    """A random matrix made of 3x3 node blocks: node q couples to a random sorted set of up to
+    max_nbrs nodes (some nodes none at all), every block dense with N(0,1) entries."""
+    rng = np.random.default_rng(seed)
+    rp, ci, v = [0], [], []
+    for q in range(nnodes):
+        k = int(rng.integers(0, max_nbrs + 1)) if q % 7 else 0
+        nb = np.sort(rng.choice(nnodes, size=min(k, nnodes), replace=False))
+        blocks = rng.standard_normal((len(nb), 3, 3))
+        for a in range(3):
+            for m, j in enumerate(nb):
+                ci += [3 * j, 3 * j + 1, 3 * j + 2]
+                v += list(blocks[m, a])
+            rp.append(len(ci))
+    return O.Csr(np.array(rp, np.int32), np.array(ci, np.int32), np.array(v), 3 * nnodes, 3 * nnodes)
+
+
+@pytest.mark.parametrize("k", [4, 8, 16, 32])
+def test_nodeblock_spmm_matches_columnwise_products(het16, k):
+    """Node-block SpMM (the block solver's layout) against the FP64 CPU product of every column:
+    the 16^3 stiffness matrix, its Galerkin coarse operators, and random block patterns with empty
+    and ragged nodes and a partial last tile; FP64 to 1e-12 of max|y|, FP32 blocks and vectors to
+    1e-5."""
+    from compressiontests_amd import kernels
+    from compressiontests_amd.device import DeviceArray, DeviceCsr
+    m, F, lv, A = het16
+    mats = [A, lv[1]["A"], lv[2]["A"], random_node_block_csr(50, 3 + k), random_node_block_csr(8, 5 + k, 8)]
+    for M in mats:
+        ref_x = O.rng_normal(91 + k, M.nrows * k).reshape(M.nrows, k)
+        ref = S(M) @ ref_x
+        scale = max(np.max(np.abs(ref)), 1e-300)
+        Dm = DeviceCsr.arrays_from_host(M)
+        for fp32, tol in ((False, 1e-12), (True, 1e-5)):
+            dt = np.float32 if fp32 else np.float64
+            nbm = kernels.NodeBlockMatrix(Dm, fp32=fp32)
+            X = DeviceArray.from_numpy(ref_x.reshape(-1).astype(dt))
+            Y = DeviceArray(M.nrows * k, dt)
+            nbm.spmm(X, Y, k)
+            assert np.max(np.abs(Y.numpy().reshape(M.nrows, k) - ref)) <= tol * scale, (M.nrows, fp32)
+            nbm.spmm(X, Y, k, clear=False)
+            assert np.max(np.abs(Y.numpy().reshape(M.nrows, k) - 2 * ref)) <= 2 * tol * scale, (M.nrows, fp32)
+            # the timed block-PCG form (per-column partials of X.Y) leaves Y = A X unchanged
+            Yt = DeviceArray(M.nrows * k, dt)
+            kernels.time_spmm(Dm if not fp32 else DeviceCsr.arrays_from_host(M, val_dtype=np.float32), X, Yt, k,
+                              warmup=0, repeats=1, nodeblock=True, dot=True)
+            assert np.max(np.abs(Yt.numpy().reshape(M.nrows, k) - ref)) <= tol * scale, (M.nrows, fp32)
+            nbm.free()
+
+
+def test_nodeblock_rejects_other_matrices(het16):
+    """R = P^T (rows of a node triple with different columns) and nodes of more than 27 blocks are
+    not node-block matrices the kernels take: the constructor raises."""
+    from compressiontests_amd import kernels
+    from compressiontests_amd.device import DeviceCsr
+    m, F, lv, A = het16
+    with pytest.raises(L.MgxInvalidArgument):
+        kernels.NodeBlockMatrix(DeviceCsr.arrays_from_host(lv[0]["R"]))
+    big = random_node_block_csr(40, 11, max_nbrs=40)
+    counts = np.diff(big.rowptr)[::3] // 3
+    assert counts.max() > 27
+    with pytest.raises(L.MgxInvalidArgument):
+        kernels.NodeBlockMatrix(DeviceCsr.arrays_from_host(big))
+
+
 def test_load_cases_shape_and_norm(het16):
     m, F, lv, A = het16
     assert F.shape == (6, m.NumDofsFree)
