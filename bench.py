@@ -52,6 +52,37 @@ def spmm_bytes(nnz, n, k, val_bytes, layout="csr"):
     return (val_bytes + 4) * nnz + 4 * (n + 1) + 2 * val_bytes * k * n
 
 
+def gs_sweep_bytes(nnz, n, k, val_bytes, ncolors):
+    """Algorithmic bytes of the ncolors launches of one node-block multicolour Gauss-Seidel
+    sweep with k right-hand sides: the blocks (a value per nonzero, a column per block, the node
+    offsets and node map) once over the sweep; per colour launch all of X (the colour's nodes
+    have neighbours of every other colour, and their own rows are read for the update); B read
+    and X written once over the sweep (each colour its own rows)."""
+    matrix = val_bytes * nnz + 4 * (nnz // 9) + 8 * (n // 3) + 4
+    vec = val_bytes * k * n
+    return matrix + ncolors * vec + 2 * vec
+
+
+def roofline(name, bytes_per_launch, total_ms, launches, pmc_tag):
+    """HBM roofline object of one kernel: algorithmic bytes per launch / average launch time (HIP
+    events on the solver stream around each launch), and the rocprofv3 PMC traffic per launch if
+    profiles/<pmc_tag>_pmc.json holds it."""
+    avg_ms = total_ms / max(launches, 1)
+    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
+    traffic = None
+    pmc = os.path.join(ROOT, "profiles", f"{pmc_tag}_pmc.json")
+    if os.path.exists(pmc):
+        try:
+            traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
+        except (OSError, ValueError):
+            traffic = None
+    return {"bound": "hbm",
+            "kernel": name + ", HIP events on the solver stream around each launch in profiled solves of the same batch",
+            "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_launch": int(bytes_per_launch),
+            "avg_launch_ms": round(avg_ms, 5), "launches": launches, "total_ms": round(total_ms, 3)}
+
+
 def moduli(kind, n):
     import numpy as np
     from compressiontests_amd.device import rng_normal, rng_uniform
@@ -107,10 +138,12 @@ def main():
     hier = solver.hierarchy
     solver.Initialize(F)
     width, tune_ms = solver.BlockWidth
+    strip = solver.StripWindow
     print(f"# setup done in {time.time() - t0:.1f} s, block width {width}", file=sys.stderr, flush=True)
     # the same solve with HIP events around every level-0 A*p launch (no graph): the roofline pass
     prof = HipMgPcgSolver(model, tolerance=args.tol, max_iterations=500, profile=True, nrhs=nrhs,
-                          precision=args.precision, block_width=width, hierarchy=hier, **common)
+                          precision=args.precision, block_width=width, strip_window=strip[0], hierarchy=hier,
+                          **common)
     prof.Initialize(F)
     setup_s = time.time() - t0
     K = model.device_matrix()
@@ -137,13 +170,15 @@ def main():
     print(f"# timed {args.steps} steps: {solves_per_s:.2f} solves/s", file=sys.stderr, flush=True)
     X = x_dev.numpy().reshape(nrhs, -1)
 
-    spmv_ms, spmv_n = 0.0, 0
+    spmv_ms, spmv_n, gs_ms, gs_n = 0.0, 0, 0.0, 0
     xp = DeviceArray(F.size)
     for _ in range(max(1, min(args.steps, 2))):
         st = prof.solve_device(b_dev, xp)
         for j in range(0, nrhs, max(width, 1)):   # one entry per block: the block's launches
             spmv_ms += st[j].spmv_ms
             spmv_n += st[j].spmv_launches
+            gs_ms += st[j].gs_ms
+            gs_n += st[j].gs_launches
 
     # the plain FP64 GPU path (one right-hand side at a time) on the first load cases
     fp64 = None
@@ -185,28 +220,28 @@ def main():
 
     vb = 4 if args.precision == "mixed" else 8
     layout = solver.LevelLayout(0)
+    tname = "float" if vb == 4 else "double"
     if width > 1:
-        bytes_per_launch = spmm_bytes(nnz, ndof, width, vb, layout)
-        tname = "float" if vb == 4 else "double"
-        kname = {"node-block": f"k_nb_spmm<{tname}, K={width}, DOT> (level-0 node-block (3x3) SpMM Q = A P with "
-                               f"per-column P.Q)",
-                 "node-triple": f"k_spmm_node<{tname}, K={width}, DOT> (level-0 node-triple CSR SpMM Q = A P with "
-                                f"per-column P.Q)"}.get(layout, f"k_spmm<{tname}, K={width}, DOT> (level-0 CSR SpMM)")
+        spmm_per_launch = spmm_bytes(nnz, ndof, width, vb, layout)
+        spmm_name = {"node-block": f"k_nb_spmm<{tname}, K={width}, DOT> (level-0 node-block (3x3) SpMM Q = A P with "
+                                   f"per-column P.Q)",
+                     "node-triple": f"k_spmm_node<{tname}, K={width}, DOT> (level-0 node-triple CSR SpMM Q = A P "
+                                    f"with per-column P.Q)"}.get(layout, f"k_spmm<{tname}, K={width}, DOT> (level-0 "
+                                                                         f"CSR SpMM)")
     else:
-        bytes_per_launch = csr_spmv_bytes(nnz, ndof, ndof, vb)
-        kname = (f"k_csr_stream<{'float' if vb == 4 else 'double'}, DOT> (level-0 CSR-stream SpMV q = A p with "
-                 f"p.q)")
-    avg_ms = spmv_ms / max(spmv_n, 1)
-    achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9
-    traffic = None
+        spmm_per_launch = csr_spmv_bytes(nnz, ndof, ndof, vb)
+        spmm_name = f"k_csr_stream<{tname}, DOT> (level-0 CSR-stream SpMV q = A p with p.q)"
     kind = ("spmm_nodeblock" if layout == "node-block" else "spmm") if width > 1 else "spmv"
-    tag = f"{kind}_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed"
-    pmc = os.path.join(ROOT, "profiles", f"{tag}_pmc.json")
-    if os.path.exists(pmc):
-        try:
-            traffic = json.load(open(pmc)).get("hbm_bytes_per_launch")
-        except (OSError, ValueError):
-            traffic = None
+    rooflines = [roofline(spmm_name, spmm_per_launch, spmv_ms, spmv_n,
+                          f"{kind}_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed")]
+    ncol = solver.LevelColors(0)
+    if gs_n > 0 and ncol > 0:
+        rooflines.append(roofline(f"k_nb_gs<{tname}, K={width}> (one colour of the level-0 node-block multicolour "
+                                  f"Gauss-Seidel; bytes averaged over the {ncol} colours of a sweep)",
+                                  gs_sweep_bytes(nnz, ndof, width, vb, ncol) / ncol, gs_ms, gs_n,
+                                  f"gs_nodeblock_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed"))
+    # the dominant kernel: the most device time in the profiled solves
+    rooflines.sort(key=lambda r: -r["total_ms"])
 
     cpu = None
     if args.cpu_baseline:
@@ -238,6 +273,9 @@ def main():
             "dofs": ndof, "nnz": nnz, "load_cases": nrhs, "levels": levels,
             "precision": args.precision, "block_width": width,
             "autotune_ms_per_iteration": {f"k{k}": round(v, 4) for k, v in zip((4, 8, 16, 32), tune_ms) if v > 0},
+            "strip_window_bytes": strip[0],
+            "autotune_strip_ms_per_iteration": {f"{w:g}": round(v, 4) for w, v in zip((3e6, 1.5e6, 0.75e6, 0), strip[1])
+                                                if v > 0},
             "iterations_median": int(np.median(iters)), "iterations_max": int(max(iters)),
             "outer_iterations_max": int(max(outer)),
             "dof_per_s": round(ndof * solves_per_s, 1),
@@ -248,19 +286,8 @@ def main():
             "speedup_vs_fp64_single_rhs": round(solves_per_s / fp64["solves_per_s"], 3) if fp64 else None,
             "setup_s": round(setup_s, 2),
         },
-        "roofline": {
-            "bound": "hbm",
-            "kernel": kname + ", HIP events on the solver stream around each launch in profiled solves of the "
-                              "same batch",
-            "achieved": round(achieved, 1),
-            "peak": HBM_PEAK_GBS,
-            "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4),
-            "traffic": traffic,
-            "bytes_per_launch": bytes_per_launch,
-            "avg_launch_ms": round(avg_ms, 5),
-            "launches": spmv_n,
-        },
+        "roofline": rooflines[0],
+        "roofline_next": rooflines[1] if len(rooflines) > 1 else None,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -66,13 +66,14 @@ class Options(C.Structure):
                 ("level_iterations", C.c_int32), ("precision", C.c_int32), ("nrhs", C.c_int32),
                 ("duvi_tolerance", C.c_double), ("duvi_mode", C.c_int32), ("inner_iterations", C.c_int32),
                 ("inner_tolerance", C.c_double), ("profile", C.c_int32), ("check_every", C.c_int32),
-                ("use_graph", C.c_int32), ("block_width", C.c_int32)]
+                ("use_graph", C.c_int32), ("block_width", C.c_int32), ("strip_window", C.c_double)]
 
 
 class Stats(C.Structure):
     _fields_ = [("iterations", C.c_int32), ("converged", C.c_int32), ("residual_norm", C.c_double),
                 ("rel_residual", C.c_double), ("solve_ms", C.c_double), ("outer_iterations", C.c_int32),
-                ("spmv_launches", C.c_int32), ("spmv_ms", C.c_double), ("block_width", C.c_int32)]
+                ("spmv_launches", C.c_int32), ("spmv_ms", C.c_double), ("block_width", C.c_int32),
+                ("gs_launches", C.c_int32), ("gs_ms", C.c_double)]
 
 
 class CsrHost(C.Structure):
@@ -147,6 +148,8 @@ _SIGS = {
     "mgx_solver_solve_device": (C.c_int, [_P, _P, _P, _I32, C.POINTER(Stats)]),
     "mgx_solver_block_width": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "mgx_solver_level_layout": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
+    "mgx_solver_level_colors": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
+    "mgx_solver_strip_window": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mgx_solver_release": (C.c_int, [_P]),
     "mgx_device_malloc": (C.c_int, [C.POINTER(_P), _U64]),
     "mgx_device_free": (C.c_int, [_P]),

@@ -43,7 +43,7 @@ def build_lib(verbose=False):
     os.makedirs(BUILD, exist_ok=True)
     hipcc = _hipcc()
     jobs = []
-    for src in ("mgx_kernels.hip", "mgx_block.hip", "mgx_nodeblock.hip", "mgx_api.hip"):
+    for src in ("mgx_kernels.hip", "mgx_block.hip", "mgx_nodeblock.hip", "mgx_duvi.hip", "mgx_api.hip"):
         obj = os.path.join(BUILD, src + ".o")
         jobs.append((obj, [hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]))
     # host-only setup code: no device code, no FMA contraction (bit-exact weights)

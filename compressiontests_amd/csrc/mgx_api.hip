@@ -342,6 +342,8 @@ struct mgx_solver_s {
     int bk = 1;                            // block width in use (1: one right-hand side at a time)
     int balloc = 0;                        // block width the buffers are sized for
     std::vector<double> tune_ms;           // autotuner: ms per PCG iteration for K = 4, 8, 16, 32
+    double strip_bytes = 3.0e6;            // X window of the strip order of the block kernels (<= 0: no strips)
+    std::vector<double> strip_ms;          // autotuner: ms per PCG iteration for each kStripCandidates window
     mgx::DBuf<double> bB64, bX64, bR64, bP64, bQ64, bZ64;
     mgx::DBuf<float> bD32, bR32, bP32, bQ32, bZ32;
     mgx::DBuf<mgx::BCtl> bctl, bctl_in, bctl_stat;
@@ -352,6 +354,11 @@ struct mgx_solver_s {
     int64_t bgraph_key = -1;
     double bprof_ms = 0;                   // options.profile: level-0 SpMM time of the current block
     int bprof_n = 0;
+    std::vector<hipEvent_t> gs_events;     // options.profile, batching form: pairs around level-0 GS colour launches
+    int gs_used = 0;
+    bool prof_gs = false;                  // record GS events (inside a profiled block PCG iteration)
+    double bgs_ms = 0;                     // their summed time in the current block
+    int bgs_n = 0;
 };
 
 namespace mgx {
@@ -1044,14 +1051,18 @@ static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, b
     const bool nbk = nb_ready<T>(V.nbg);
     for (int i = 0; i < V.ncolors; ++i) {
         const int c = backward ? V.ncolors - 1 - i : i;
+        const bool rec = S->prof_gs && &V == &S->lv[0] && (size_t)(2 * S->gs_used + 2) <= S->gs_events.size();
+        if (rec) MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used], S->stream));
         if (nbk)
-            launch_nb_gs<T>(K, V.nbg.view, V.color_off[c], V.color_off[c + 1], b, x, backward, done, S->stream);
+            launch_nb_gs<T>(K, V.nbg.view, V.color_off[c], V.color_off[c + 1], b, x, backward, (i & 1) != 0, done,
+                            S->stream);
         else if (V.node3)
             launch_gs_node<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done,
                               S->stream);
         else
             launch_gs_block<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done,
                                S->stream);
+        if (rec) { MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used + 1], S->stream)); ++S->gs_used; }
     }
 }
 
@@ -1148,7 +1159,9 @@ static void block_iteration(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P,
     if (prof) { MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used + 1], s)); ++S->prof_used; }
     launch_bfinalize(level_spmm_parts<T>(V, K), S->bpartials.p, ctl, FIN_ALPHA, s);
     launch_bupdate<T>(K, n, P, Q, X, R, ctl, S->bpartials.p, s);
+    S->prof_gs = prof;
     vcycle_block<T>(S, K, 0, R, Z, &ctl->done);
+    S->prof_gs = false;
     launch_bdot_beta<T>(K, n, R, Z, ctl, S->bpartials.p, false, s);
     launch_bupdate_p<T>(K, n, Z, P, ctl, false, s);
 }
@@ -1162,7 +1175,7 @@ static void block_loop(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P, T *Q
     const int chunk = S->check_every;
     const bool graph = S->opt.use_graph != 0 && !S->opt.profile;
     const int64_t key = ((int64_t)tag * 64 + K) * 65536 + chunk;
-    const int prof0 = S->prof_used;
+    const int prof0 = S->prof_used, gs0 = S->gs_used;
     if (graph && (!S->bgexec || S->bgraph_key != key)) {
         if (S->bgexec) { MGX_HIP(hipGraphExecDestroy(S->bgexec)); S->bgexec = nullptr; }
         hipGraph_t g = nullptr;
@@ -1202,6 +1215,17 @@ static void block_loop(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P, T *Q
         }
         S->bprof_n += real;
         S->prof_used = prof0 + real;
+        // level-0 GS launches of the real iterations (a fixed number per V-cycle)
+        const Level &V0 = S->lv[0];
+        const int per = S->opt.smoother == MGX_SMOOTHER_GS ? 2 * S->opt.level_iterations * V0.ncolors : 0;
+        const int rg = std::min(S->gs_used - gs0, per * real);
+        for (int i = gs0; i < gs0 + rg; ++i) {
+            float ms = 0;
+            MGX_HIP(hipEventElapsedTime(&ms, S->gs_events[2 * i], S->gs_events[2 * i + 1]));
+            S->bgs_ms += ms;
+        }
+        S->bgs_n += rg;
+        S->gs_used = gs0 + rg;
     }
 }
 
@@ -1337,6 +1361,9 @@ static void solve_blocks(mgx_solver_s *S, const double *b, double *x, bool has_x
         S->prof_used = 0;
         S->bprof_ms = 0;
         S->bprof_n = 0;
+        S->gs_used = 0;
+        S->bgs_ms = 0;
+        S->bgs_n = 0;
         MGX_HIP(hipEventRecord(S->ev0, s));
         if (S->opt.precision == MGX_PRECISION_MIXED) solve_block_mixed(S, K, j0, kc, b, x, has_x0, st.data());
         else solve_block_fp64(S, K, j0, kc, b, x, has_x0, st.data());
@@ -1350,6 +1377,8 @@ static void solve_blocks(mgx_solver_s *S, const double *b, double *x, bool has_x
             st[j].solve_ms = ms;
             st[j].spmv_launches = S->bprof_n;
             st[j].spmv_ms = S->bprof_ms;
+            st[j].gs_launches = S->bgs_n;
+            st[j].gs_ms = S->bgs_ms;
             stats[j0 + j] = st[j];
         }
     }
@@ -1359,7 +1388,7 @@ static void solve_blocks(mgx_solver_s *S, const double *b, double *x, bool has_x
 // plane either side, so walking whole planes needs a window of three planes of X (3 x K values
 // per node) in L2, which at K = 32 is several times the 4 MB L2 of an XCD.  Walking y-strips of
 // SY lines through all planes needs only 3 planes x (SY + 2) lines; SY is the largest strip
-// whose window fits ~3 MB.  Applied to the node tiles of the SpMM and to the node lists of the
+// whose window fits strip_bytes (3 MB unless the autotuner chose another window).  Applied to the node tiles of the SpMM and to the node lists of the
 // GS colours (any order is valid within a colour).
 static void order_for_cache(mgx_solver_s *S, int K)
 {
@@ -1380,7 +1409,8 @@ static void order_for_cache(mgx_solver_s *S, int K)
         const int64_t lines = V.gy + 1, planes = V.gz + 1;
         if ((int64_t)V.gx * lines * planes * 3 != V.n) strips = false;   // not the free-node grid of the mesh
         const double line_bytes = (double)V.gx * 3 * K * tbytes;
-        const int64_t sy = strips ? std::max<int64_t>(1, (int64_t)(3.0e6 / (3 * line_bytes)) - 2) : 0;
+        if (S->strip_bytes <= 0) strips = false;
+        const int64_t sy = strips ? std::max<int64_t>(1, (int64_t)(S->strip_bytes / (3 * line_bytes)) - 2) : 0;
         if (sy >= lines) strips = false;                                 // whole planes fit
         auto key = [&](int64_t q) {
             const int64_t ix = q % V.gx, iy = (q / V.gx) % lines, iz = q / (V.gx * lines);
@@ -1482,39 +1512,55 @@ static double time_block_iteration(mgx_solver_s *S, int K)
     return ms / reps;
 }
 
+// X windows of the strip order tried by the autotuner (0: node order, no strips)
+static const std::vector<double> kStripCandidates = {3.0e6, 1.5e6, 0.75e6, 0.0};
+
 // block width: fixed by the options, or the K in {4, 8, 16, 32} (not above the next power of
-// two of nrhs) with the least time per column-iteration
+// two of nrhs) with the least time per column-iteration; then the X window of the strip order:
+// fixed by the options, or the candidate with the least time per iteration at that width
 static void choose_block_width(mgx_solver_s *S)
 {
     const int bw = S->opt.block_width;
     const bool able = S->opt.method == MGX_METHOD_MGPCG && S->opt.format == MGX_FORMAT_CSR && S->nrhs > 1;
+    const bool mixed = S->opt.precision == MGX_PRECISION_MIXED;
     S->tune_ms.assign(4, 0.0);
+    S->strip_ms.assign(kStripCandidates.size(), 0.0);
+    S->strip_bytes = S->opt.strip_window >= 0 ? S->opt.strip_window : kStripCandidates[0];
     if (bw == 1 || !able) { S->bk = 1; return; }
     if (bw > 1) {
         MGX_REQUIRE(bw == 4 || bw == 8 || bw == 16 || bw == 32, MGX_ERR_INVALID_ARGUMENT,
                     "block_width must be 0, 1, 4, 8, 16 or 32");
         alloc_block(S, bw);
         S->bk = bw;
-        return;
+    } else {
+        int kmax = 4;
+        while (kmax < S->nrhs && kmax < 32) kmax *= 2;
+        alloc_block(S, kmax);
+        double best = 0;
+        int bestK = 4;
+        for (int K = 4, i = 0; K <= kmax; K *= 2, ++i) {
+            const double ms = mixed ? time_block_iteration<float>(S, K) : time_block_iteration<double>(S, K);
+            S->tune_ms[i] = ms;
+            const double per_col = ms / K;
+            if (K == 4 || per_col < best) { best = per_col; bestK = K; }
+        }
+        S->bk = bestK;
     }
-    int kmax = 4;
-    while (kmax < S->nrhs && kmax < 32) kmax *= 2;
-    alloc_block(S, kmax);
-    const bool mixed = S->opt.precision == MGX_PRECISION_MIXED;
-    double best = 0;
-    int bestK = 4;
-    for (int K = 4, i = 0; K <= kmax; K *= 2, ++i) {
-        const double ms = mixed ? time_block_iteration<float>(S, K) : time_block_iteration<double>(S, K);
-        S->tune_ms[i] = ms;
-        const double per_col = ms / K;
-        if (K == 4 || per_col < best) { best = per_col; bestK = K; }
+    // then the X window of the strip order at that width (structured levels only), unless fixed
+    if (S->opt.strip_window < 0 && S->lv[0].gx > 0 && S->lv[0].node3) {
+        double bestw = S->strip_bytes, bestms = 0;
+        for (size_t i = 0; i < kStripCandidates.size(); ++i) {
+            S->strip_bytes = kStripCandidates[i];
+            order_for_cache(S, S->bk);
+            const double ms = mixed ? time_block_iteration<float>(S, S->bk) : time_block_iteration<double>(S, S->bk);
+            S->strip_ms[i] = ms;
+            if (i == 0 || ms < bestms) { bestms = ms; bestw = kStripCandidates[i]; }
+        }
+        S->strip_bytes = bestw;
     }
-    S->bk = bestK;
-    if (bestK != kmax) {
-        order_for_cache(S, bestK);
-        if (S->bgexec) { MGX_HIP(hipGraphExecDestroy(S->bgexec)); S->bgexec = nullptr; }
-        S->bgraph_key = -1;
-    }
+    order_for_cache(S, S->bk);
+    if (S->bgexec) { MGX_HIP(hipGraphExecDestroy(S->bgexec)); S->bgexec = nullptr; }
+    S->bgraph_key = -1;
 }
 
 static void solve_device(mgx_solver_s *S, const double *b, double *x, bool has_x0, mgx_stats *stats)
@@ -1923,6 +1969,7 @@ void mgx_options_default(mgx_options *o)
     o->check_every = 4;
     o->use_graph = 1;
     o->block_width = 0;
+    o->strip_window = -1;
 }
 
 int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
@@ -1953,6 +2000,8 @@ int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
             // timing-only events: no system-scope fence (its L2 write-back and invalidate would
             // lengthen the bracketed launch and cool the cache of the next one)
             for (auto &e : s->prof_events) MGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
+            s->gs_events.resize(2 * 16 * 64);     // 64 V-cycles of 16 colour launches
+            for (auto &e : s->gs_events) MGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         }
         *out = s.release();
     });
@@ -2063,6 +2112,28 @@ int mgx_solver_level_layout(mgx_solver s, int32_t level, int32_t *layout)
     });
 }
 
+int mgx_solver_strip_window(mgx_solver s, double *bytes, double *tune_ms)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && bytes, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(s->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
+        *bytes = s->strip_bytes;
+        if (tune_ms)
+            for (size_t i = 0; i < kStripCandidates.size(); ++i)
+                tune_ms[i] = i < s->strip_ms.size() ? s->strip_ms[i] : 0.0;
+    });
+}
+
+int mgx_solver_level_colors(mgx_solver s, int32_t level, int32_t *ncolors)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && ncolors, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(s->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
+        MGX_REQUIRE(level >= 0 && level < (int)s->lv.size(), MGX_ERR_INVALID_ARGUMENT, "level out of range");
+        *ncolors = s->lv[level].ncolors;
+    });
+}
+
 int mgx_solver_release(mgx_solver s)
 {
     return guard([&] {
@@ -2076,6 +2147,7 @@ int mgx_solver_release(mgx_solver s)
         if (s->ev0) (void)hipEventDestroy(s->ev0);
         if (s->ev1) (void)hipEventDestroy(s->ev1);
         for (auto &e : s->prof_events) (void)hipEventDestroy(e);
+        for (auto &e : s->gs_events) (void)hipEventDestroy(e);
         if (s->gexec) (void)hipGraphExecDestroy(s->gexec);
         hipStream_t st = s->stream;
         delete s;

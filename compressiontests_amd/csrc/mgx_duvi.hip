@@ -1,0 +1,196 @@
+// mgx_duvi.hip -- SpMV over the DuVi GPU block layout (OpenCLDuViGeometricMultigridSolver.cs:16-31).
+//
+// Decode rule of DuViGeometricMultigrid.cl:57-99: a row's blocks are consecutive in
+// block_col; a block's entries are consecutive slots of dist/vi starting at the row's offset
+// row_dist[r]; at a block's first slot the distance byte is the count of the block's further
+// entries and the column is block_col[b]; at a later slot the column is the previous one plus
+// the distance byte plus 1.  The value of slot d is values[vi[d]] (u8/u16/u32 indices) or
+// values[d] ("DU + raw values").
+//
+// A wavefront owns a tile of 8 consecutive rows (8 lanes per row) and needs no block-wide
+// barrier: (1) every lane issues the value loads of its first segment; (2) the tile's distance
+// bytes (one contiguous range) and block columns go to the wave's LDS region; (3) lane 0 of each
+// row walks the row's block chain (one step per block: the count byte) and marks each block's
+// first slot with its column; (4) in rounds of 96 entries per row, lane l takes the 12 slots
+// 12l .. 12l+11 of the round: it folds its slots into (has a block start, column or column
+// offset), a segmented scan over the row's 8 lanes gives every lane the column before its
+// first slot, and the lane then decodes its slots again, gathering x and accumulating the
+// products; (5) the 8 lanes sum the row.  A tile of more than kDwCap entries falls back to one
+// lane per row decoding straight from global memory.
+#include "mgx_device.h"
+
+namespace mgx {
+
+typedef unsigned int dw_u32x4 __attribute__((ext_vector_type(4)));
+
+template <int VIB>
+__device__ __forceinline__ double duvi_value(const void *vi, const double *values, int64_t d)
+{
+    if (VIB == 1) return values[((const uint8_t *)vi)[d]];
+    if (VIB == 2) return values[((const uint16_t *)vi)[d]];
+    if (VIB == 4) return values[((const uint32_t *)vi)[d]];
+    return values[d];
+}
+
+constexpr int kDwRows = 8;                     // rows per wave tile
+constexpr int kDwSeg = 12;                     // slots per lane per round (8 lanes: 96 per row and round)
+constexpr int kDwCap = 768;                    // slots staged per wave tile
+constexpr int kDwWaves = kBlock / 64;
+
+__device__ __forceinline__ void dw_fence()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_s_waitcnt(0xc07f);        // lgkmcnt(0): this wave's LDS writes are done
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+template <int VIB>
+__global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz, const int32_t *__restrict__ row_blocks,
+                                                      const int32_t *__restrict__ row_dist,
+                                                      const int32_t *__restrict__ block_col,
+                                                      const uint8_t *__restrict__ dist, const void *__restrict__ vi,
+                                                      const double *__restrict__ values, const double *__restrict__ x,
+                                                      double *__restrict__ y, int clear, const int32_t *__restrict__ done)
+{
+    __shared__ dw_u32x4 s_dq[kDwWaves][(kDwCap + 32) / 16];   // distance bytes, shifted by d0 & 15
+    __shared__ int32_t s_h[kDwWaves][kDwCap];                  // column + 1 at a block's first slot, else 0
+    __shared__ int32_t s_bc[kDwWaves][kDwCap];                 // the tile's block columns
+    if (done && *done) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane >> 3, l = lane & 7;
+    const int ntiles = (nrows + kDwRows - 1) / kDwRows;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x) * kDwWaves + wv;
+    if (tile >= ntiles) return;                                // the whole wave leaves together
+    const int r0 = tile * kDwRows, r1 = min(r0 + kDwRows, nrows);
+    const int row = r0 + r;
+    const int64_t d0 = row_dist[r0];
+    const int cnt = (int)(row_dist[r1] - d0);
+    if (cnt > kDwCap) {
+        // long tile: one lane per row, decoding straight from global memory
+        if (l == 0 && row < r1) {
+            double acc = clear ? 0.0 : y[row];
+            int64_t d = row_dist[row];
+            for (int b = row_blocks[row]; b < row_blocks[row + 1]; ++b) {
+                int col = block_col[b];
+                const int64_t dend = d + dist[d] + 1;
+                for (;;) {
+                    acc += duvi_value<VIB>(vi, values, d) * x[col];
+                    if (++d == dend) break;
+                    col += dist[d] + 1;
+                }
+            }
+            y[row] = acc;
+        }
+        return;
+    }
+    int s = 0, e = 0, rb0 = 0, rb1 = 0;
+    if (row < r1) {
+        s = (int)(row_dist[row] - d0); e = (int)(row_dist[row + 1] - d0);
+        rb0 = row_blocks[row]; rb1 = row_blocks[row + 1];
+    }
+    // (1) values of this lane's slots in round 0: they do not depend on the decode
+    double w[kDwSeg];
+    {
+        const int a = s + l * kDwSeg;
+#pragma unroll
+        for (int j = 0; j < kDwSeg; ++j) w[j] = a + j < e ? duvi_value<VIB>(vi, values, d0 + a + j) : 0.0;
+    }
+    // (2) distance bytes (16-byte loads inside the array, byte loads at its end), block columns,
+    //     cleared start marks
+    const int64_t g0 = d0 & ~(int64_t)15;
+    const int sh = (int)(d0 - g0);
+    const uint8_t *sd = (const uint8_t *)s_dq[wv] + sh;          // sd[k]: distance byte of slot d0 + k
+    {
+        const int nq = (sh + cnt + 15) >> 4;
+        for (int q = lane; q < nq; q += 64) {
+            const int64_t g = g0 + 16 * q;
+            if (g + 16 <= nnz) {
+                s_dq[wv][q] = __builtin_nontemporal_load((const dw_u32x4 *)(dist + g));
+            } else {
+                uint8_t *o = (uint8_t *)&s_dq[wv][q];
+                for (int i = 0; i < 16; ++i) o[i] = g + i < nnz ? dist[g + i] : 0;
+            }
+        }
+        const int b0t = row_blocks[r0], nbt = row_blocks[r1] - b0t;
+        for (int i = lane; i < nbt; i += 64) s_bc[wv][i] = block_col[b0t + i];
+        for (int k = lane; k < cnt; k += 64) s_h[wv][k] = 0;
+    }
+    dw_fence();
+    // (3) the block chain of each row: mark every block's first slot with its column + 1
+    if (l == 0 && row < r1) {
+        const int b0t = row_blocks[r0];
+        int d = s;
+        for (int b = rb0; b < rb1 && d < e; ++b) {
+            s_h[wv][d] = s_bc[wv][b - b0t] + 1;
+            d += sd[d] + 1;
+        }
+    }
+    dw_fence();
+    // (4) rounds of 96 slots per row
+    double acc = 0;
+    int carry = 0;                                             // column before the round's first slot
+    for (int base = s, round = 0; base < e; base += 8 * kDwSeg, ++round) {
+        const int a = base + l * kDwSeg, bnd = min(a + kDwSeg, e);
+        // fold this lane's slots: h = a block starts among them; v = column of the last slot
+        // (h) or its offset from the column before the first slot (!h)
+        int h = 0, v = 0;
+#pragma unroll
+        for (int j = 0; j < kDwSeg; ++j) {
+            const int k = a + j;
+            if (k < bnd) {
+                const int hc = s_h[wv][k];
+                if (hc) { h = 1; v = hc - 1; } else v += sd[k] + 1;
+            }
+        }
+        // inclusive segmented scan over the row's 8 lanes
+#pragma unroll
+        for (int o = 1; o < 8; o <<= 1) {
+            const int ph = __shfl_up(h, o, 8), pv = __shfl_up(v, o, 8);
+            if (l >= o && !h) { v += pv; h = ph; }
+        }
+        // column before this lane's first slot
+        const int ph = __shfl_up(h, 1, 8), pv = __shfl_up(v, 1, 8);
+        int col = (l == 0) ? carry : (ph ? pv : carry + pv);
+        // the round's last column, for the next round
+        const int th = __shfl(h, 7, 8), tv = __shfl(v, 7, 8);
+        carry = th ? tv : carry + tv;
+        // decode again, gathering x
+        double xv[kDwSeg], wv2[kDwSeg];
+#pragma unroll
+        for (int j = 0; j < kDwSeg; ++j) {
+            const int k = a + j;
+            int c = 0;
+            if (k < bnd) {
+                const int hc = s_h[wv][k];
+                col = hc ? hc - 1 : col + sd[k] + 1;
+                c = col;
+            }
+            xv[j] = k < bnd ? x[c] : 0.0;
+            wv2[j] = round == 0 ? w[j] : (k < bnd ? duvi_value<VIB>(vi, values, d0 + k) : 0.0);
+        }
+#pragma unroll
+        for (int j = 0; j < kDwSeg; ++j) acc += wv2[j] * xv[j];
+    }
+    // (5) row sum over the 8 lanes
+#pragma unroll
+    for (int o = 4; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 8);
+    if (l == 0 && row < r1) y[row] = clear ? acc : y[row] + acc;
+}
+
+void launch_duvi_spmv(const mgx_duvi &A, const double *x, double *y, bool clear, const int32_t *done, hipStream_t s)
+{
+    if (A.nrows <= 0) return;
+    const dim3 g(grid_rows((A.nrows + kDwRows - 1) / kDwRows, kDwWaves)), bl(kBlock);
+#define MGX_DV(V) hipLaunchKernelGGL(k_duvi_wave<V>, g, bl, 0, s, A.nrows, A.nnz, A.row_blocks, A.row_dist, A.block_col, \
+                                     A.dist, A.vi, A.values, x, y, clear ? 1 : 0, done)
+    switch (A.vi_bytes) {
+    case 1: MGX_DV(1); break;
+    case 2: MGX_DV(2); break;
+    case 4: MGX_DV(4); break;
+    default: MGX_DV(0); break;
+    }
+#undef MGX_DV
+    MGX_HIP(hipGetLastError());
+}
+
+}  // namespace mgx

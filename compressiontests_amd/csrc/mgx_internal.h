@@ -184,8 +184,8 @@ template <typename T>
 void launch_nb_spmm(int K, int epi, const NbView &A, const T *X, const T *B, const T *pre, T *Y, const int32_t *done,
                     double *partials, int stride, hipStream_t s, const int32_t *order = nullptr);
 template <typename T>
-void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T *X, bool backward, const int32_t *done,
-                  hipStream_t s);
+void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T *X, bool backward, bool reverse,
+                  const int32_t *done, hipStream_t s);
 void nb_count(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, bool diag_first, int32_t *cnt,
               int32_t *bad, hipStream_t s);
 template <typename TI, typename TV>

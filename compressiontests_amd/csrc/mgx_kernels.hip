@@ -784,163 +784,6 @@ void launch_dense_gemv(int32_t n, const double *M, const double *x, double *y, b
     MGX_HIP(hipGetLastError());
 }
 
-// ------------------------------------------------------------------ DuVi block layout SpMV
-// Decode rule of DuViGeometricMultigrid.cl:57-99: a row's blocks start at block_col[b]; at a
-// block's first slot the distance byte is the count of further entries, at later slots it is
-// the column gap minus one.
-template <int VIB>
-__device__ __forceinline__ double duvi_value(const void *vi, const double *values, int64_t d)
-{
-    if (VIB == 1) return values[((const uint8_t *)vi)[d]];
-    if (VIB == 2) return values[((const uint16_t *)vi)[d]];
-    if (VIB == 4) return values[((const uint32_t *)vi)[d]];
-    return values[d];
-}
-
-// Streamed DuVi SpMV: a 256-thread block takes a tile of 32 rows whose distance bytes and value
-// indices are contiguous (row_dist is the CSR row offset), 8 lanes per row: (1) the tile's
-// distance bytes are copied to LDS with coalesced loads; (2) lane 0 of each row walks the row's
-// chain of blocks (one step per block: the first slot's count) and records every block's first
-// slot; (3) the 8 lanes decode each block with a prefix sum of the gaps, writing every entry's
-// column to LDS; (4) every thread takes entries t, t + 256, ...: value (through the index stream,
-// loaded before step 1), gather of x, product to LDS; (5) the 8 lanes sum the row.  A tile longer than the LDS capacity, or
-// with more than kDuviBlocks blocks, falls back to one thread per row.
-constexpr int kDuviRows = 32, kDuviE = 11, kDuviCap = kDuviE * kBlock, kDuviBlocks = 512;
-
-template <int VIB>
-__global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t nnz, const int32_t *__restrict__ row_blocks,
-                                                        const int32_t *__restrict__ row_dist,
-                                                        const int32_t *__restrict__ block_col,
-                                                        const uint8_t *__restrict__ dist, const void *__restrict__ vi,
-                                                        const double *__restrict__ values, const double *__restrict__ x,
-                                                        double *__restrict__ y, int clear, const int32_t *__restrict__ done)
-{
-    constexpr int G = kBlock / kDuviRows;
-    __shared__ uint8_t s_d[kDuviCap + 4];
-    __shared__ int32_t s_col[kDuviCap];
-    __shared__ double s_p[kDuviCap];
-    __shared__ int32_t s_bs[kDuviBlocks];
-    if (done && *done) return;
-    const int t = threadIdx.x;
-    const int ntiles = (nrows + kDuviRows - 1) / kDuviRows;
-    const int tile = xcd_remap(blockIdx.x, gridDim.x);
-    if (tile >= ntiles) return;
-    const int r0 = tile * kDuviRows, r1 = min(r0 + kDuviRows, nrows);
-    const int64_t d0 = row_dist[r0];
-    const int cnt = (int)(row_dist[r1] - d0);
-    const int b0 = row_blocks[r0], nb = row_blocks[r1] - b0;
-    if (cnt > kDuviCap || nb > kDuviBlocks) {
-        // long tile: one thread per row, decoding straight from global memory
-        if (t < r1 - r0) {
-            const int row = r0 + t;
-            double acc = clear ? 0.0 : y[row];
-            int64_t d = row_dist[row];
-            for (int b = row_blocks[row]; b < row_blocks[row + 1]; ++b) {
-                int col = block_col[b];
-                const int64_t dend = d + dist[d] + 1;
-                for (;;) {
-                    acc += duvi_value<VIB>(vi, values, d) * x[col];
-                    if (++d == dend) break;
-                    col += dist[d] + 1;
-                }
-            }
-            y[row] = acc;
-        }
-        return;
-    }
-    // the value stream (through the index stream) does not depend on the column decode: issue
-    // its loads first so that they are in flight while the tile is decoded
-    double w[kDuviE];
-#pragma unroll
-    for (int j = 0; j < kDuviE; ++j) {
-        const int k = t + kBlock * j;
-        w[j] = k < cnt ? duvi_value<VIB>(vi, values, d0 + k) : 0.0;
-    }
-    {
-        // distance bytes as aligned 32-bit words; a word that is not wholly inside the array is
-        // read byte by byte
-        const int lead = (int)(((uintptr_t)(dist + d0)) & 3);
-        const int64_t w0 = d0 - lead;
-        const int nw = (lead + cnt + 3) >> 2;
-        for (int k = t; k < nw; k += kBlock) {
-            const int64_t g = w0 + 4 * k;
-            if (g >= 0 && g + 4 <= nnz) {
-                const uint32_t w = ntload((const uint32_t *)(dist + g));
-#pragma unroll
-                for (int q = 0; q < 4; ++q) {
-                    const int i = 4 * k + q - lead;
-                    if (i >= 0 && i < cnt) s_d[i] = (uint8_t)(w >> (8 * q));
-                }
-            } else {
-                for (int q = 0; q < 4; ++q) {
-                    const int i = 4 * k + q - lead;
-                    if (i >= 0 && i < cnt) s_d[i] = dist[d0 + i];
-                }
-            }
-        }
-    }
-    __syncthreads();
-    const int row = r0 + t / G, lane = t % G;
-    int rb0 = 0, rb1 = 0, s0 = 0, e0 = 0;
-    if (row < r1) {
-        rb0 = row_blocks[row] - b0; rb1 = row_blocks[row + 1] - b0;
-        s0 = (int)(row_dist[row] - d0); e0 = (int)(row_dist[row + 1] - d0);
-        if (lane == 0) {
-            int d = s0;
-            for (int b = rb0; b < rb1; ++b) { s_bs[b] = d; d += s_d[d] + 1; }
-        }
-    }
-    __syncthreads();
-    // the row's blocks one after the other, each decoded by the 8 lanes in chunks of 8 entries:
-    // column = block start + inclusive scan of (distance + 1) over the block's later slots
-    for (int b = rb0; b < rb1; ++b) {
-        const int d = s_bs[b], n = s_d[d];
-        int carry = block_col[b0 + b];
-        for (int e0 = 0; e0 <= n; e0 += G) {
-            const int e = e0 + lane;
-            int inc = (e > 0 && e <= n) ? s_d[d + e] + 1 : 0;
-#pragma unroll
-            for (int o = 1; o < G; o <<= 1) {
-                const int up = __shfl_up(inc, o, G);
-                if (lane >= o) inc += up;
-            }
-            if (e <= n) s_col[d + e] = carry + inc;
-            carry += __shfl(inc, G - 1, G);
-        }
-    }
-    __syncthreads();
-    double xv[kDuviE];
-#pragma unroll
-    for (int j = 0; j < kDuviE; ++j) {
-        const int k = t + kBlock * j;
-        xv[j] = k < cnt ? x[s_col[k]] : 0.0;
-    }
-#pragma unroll
-    for (int j = 0; j < kDuviE; ++j) s_p[t + kBlock * j] = w[j] * xv[j];
-    __syncthreads();
-    double a = 0;
-    for (int k = s0 + lane; k < e0; k += G) a += s_p[k];
-#pragma unroll
-    for (int o = G / 2; o > 0; o >>= 1) a += __shfl_xor(a, o, G);
-    if (lane == 0 && row < r1) y[row] = clear ? a : y[row] + a;
-}
-
-void launch_duvi_spmv(const mgx_duvi &A, const double *x, double *y, bool clear, const int32_t *done, hipStream_t s)
-{
-    if (A.nrows <= 0) return;
-    const dim3 g(grid_rows((A.nrows + kDuviRows - 1) / kDuviRows, 1)), bl(kBlock);
-#define MGX_DV(V) hipLaunchKernelGGL(k_duvi_stream<V>, g, bl, 0, s, A.nrows, A.nnz, A.row_blocks, A.row_dist, A.block_col, \
-                                     A.dist, A.vi, A.values, x, y, clear ? 1 : 0, done)
-    switch (A.vi_bytes) {
-    case 1: MGX_DV(1); break;
-    case 2: MGX_DV(2); break;
-    case 4: MGX_DV(4); break;
-    default: MGX_DV(0); break;
-    }
-#undef MGX_DV
-    MGX_HIP(hipGetLastError());
-}
-
 // ------------------------------------------------------------------ SIMP adjoint
 __constant__ double c_Ke0[576];
 

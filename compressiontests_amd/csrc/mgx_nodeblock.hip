@@ -273,7 +273,11 @@ template void launch_nb_spmm<double>(int, int, const NbView &, const double *, c
                                      const int32_t *, double *, int, hipStream_t, const int32_t *);
 
 // ------------------------------------------------------------------ multicolour Gauss-Seidel
-// Positions [p0, p1) of the colour-ordered copy (one colour: no two of its nodes are coupled).
+// Positions [p0, p1) of the colour-ordered copy (one colour: no two of its nodes are coupled),
+// tiles visited in ascending or (reverse) descending order: the nodes of one colour are
+// independent, so the order changes no result, and alternating it between the colour launches
+// of a sweep lets each launch start on the X rows the previous one touched last (still in the
+// memory-side cache).
 // Each node: a[r] = sum over its off-diagonal blocks (slots 1 ..) of A X, then the three dofs are
 // relaxed in order with the node's own block D (slot 0): x_0 = (b_0 - a_0 - D01 x_1 - D02 x_2) /
 // D00, then x_1 with the new x_0, then x_2 (reverse order when backward).
@@ -281,7 +285,7 @@ template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, const int32_t *__restrict__ nrp,
                                                   const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
                                                   const T *__restrict__ val, const T *__restrict__ B, T *__restrict__ X,
-                                                  int backward, const int32_t *__restrict__ done)
+                                                  int backward, int reverse, const int32_t *__restrict__ done)
 {
     using Sh = NbShape<T, K>;
     using VT = typename NbVec<T>::type;
@@ -291,8 +295,9 @@ __global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, cons
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane / Sh::G, l = lane % Sh::G, s = l / Sh::V, c0 = (l % Sh::V) * Sh::W;
     const int ntiles = (p1c - p0c + Sh::NW - 1) / Sh::NW;
-    const int tile = xcd_remap(blockIdx.x, gridDim.x) * kNbWaves + wv;
-    if (tile >= ntiles) return;                                  // the whole wave leaves together
+    const int t0 = xcd_remap(blockIdx.x, gridDim.x) * kNbWaves + wv;
+    if (t0 >= ntiles) return;                                    // the whole wave leaves together
+    const int tile = reverse ? ntiles - 1 - t0 : t0;
     const int pa = p0c + tile * Sh::NW, pb = min(pa + Sh::NW, p1c);
     const int b0 = nrp[pa], b1 = nrp[pb];
     const int p = pa + g;
@@ -326,15 +331,15 @@ __global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, cons
 }
 
 template <typename T>
-void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T *X, bool backward, const int32_t *done,
-                  hipStream_t s)
+void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T *X, bool backward, bool reverse,
+                  const int32_t *done, hipStream_t s)
 {
     if (p1 <= p0) return;
     const int nw = nb_tile<T>(K);
     const int g = grid_rows((p1 - p0 + nw - 1) / nw, kNbWaves);
 #define MGX_NBG(KK)                                                                                              \
     hipLaunchKernelGGL((k_nb_gs<T, KK>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.ncol, A.node,           \
-                       (const T *)A.val, B, X, backward ? 1 : 0, done)
+                       (const T *)A.val, B, X, backward ? 1 : 0, reverse ? 1 : 0, done)
     switch (K) {
     case 4: MGX_NBG(4); break;
     case 8: MGX_NBG(8); break;
@@ -345,9 +350,9 @@ void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T 
 #undef MGX_NBG
     MGX_HIP(hipGetLastError());
 }
-template void launch_nb_gs<float>(int, const NbView &, int32_t, int32_t, const float *, float *, bool, const int32_t *,
-                                  hipStream_t);
-template void launch_nb_gs<double>(int, const NbView &, int32_t, int32_t, const double *, double *, bool,
+template void launch_nb_gs<float>(int, const NbView &, int32_t, int32_t, const float *, float *, bool, bool,
+                                  const int32_t *, hipStream_t);
+template void launch_nb_gs<double>(int, const NbView &, int32_t, int32_t, const double *, double *, bool, bool,
                                    const int32_t *, hipStream_t);
 
 // ------------------------------------------------------------------ building a copy from CSR

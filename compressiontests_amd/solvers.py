@@ -37,13 +37,16 @@ class IterativeStatistics:
     SpmvLaunches: int = 0
     OuterIterations: int = 0
     BlockWidth: int = 1
+    GsMs: float = 0.0
+    GsLaunches: int = 0
 
     @classmethod
     def from_c(cls, s, criterion_name, criterion_value):
         return cls(NumIterationsRequired=s.iterations, HasConverged=s.converged == 1, Failed=s.converged == -1,
                    ConvergenceCriterion=(criterion_name, criterion_value), ResidualNorm=s.residual_norm,
                    RelativeResidual=s.rel_residual, SolveMs=s.solve_ms, SpmvMs=s.spmv_ms,
-                   SpmvLaunches=s.spmv_launches, OuterIterations=s.outer_iterations, BlockWidth=s.block_width)
+                   SpmvLaunches=s.spmv_launches, OuterIterations=s.outer_iterations, BlockWidth=s.block_width,
+                   GsMs=s.gs_ms, GsLaunches=s.gs_launches)
 
 
 def make_path(level_down):
@@ -130,6 +133,17 @@ class _Handle:
     def level_layout(self, level):
         v = C.c_int32()
         L.call("mgx_solver_level_layout", C.c_void_p(self.h), level, C.byref(v))
+        return v.value
+
+    def strip_window(self):
+        b = C.c_double()
+        ms = (C.c_double * 4)()
+        L.call("mgx_solver_strip_window", C.c_void_p(self.h), C.byref(b), ms)
+        return b.value, list(ms)
+
+    def level_colors(self, level):
+        v = C.c_int32()
+        L.call("mgx_solver_level_colors", C.c_void_p(self.h), level, C.byref(v))
         return v.value
 
     def release(self):
@@ -271,11 +285,14 @@ class HipMgPcgSolver:
     nrhs > 1: b is (nrhs, n), one load case per row, and Solve returns x of the same shape and
     a list of statistics.  block_width: right-hand sides advanced together by one batched PCG
     (4, 8, 16, 32), 1 = one at a time, 0 = chosen at Initialize by timing each width.
+    strip_window: X window (bytes) of the strip order of the block kernels on structured levels,
+    0 = node order, < 0 = chosen at Initialize by timing the candidates.
     precision "mixed": FP32 SpMV / V-cycle inside FP64 iterative refinement."""
 
     def __init__(self, model, levels=3, smoother="jacobi", omega=0.0, nu=2, tolerance=1e-8, stop="rel",
                  max_iterations=1000, hierarchy="galerkin", format="csr", profile=False, check_every=4, nrhs=1,
-                 use_graph=True, precision="fp64", inner_tolerance=1e-4, inner_iterations=200, block_width=0):
+                 use_graph=True, precision="fp64", inner_tolerance=1e-4, inner_iterations=200, block_width=0,
+                 strip_window=-1.0):
         self.model = model
         self.levels = levels
         opt = L.default_options(method=L.METHOD_MGPCG,
@@ -287,7 +304,7 @@ class HipMgPcgSolver:
                                 use_graph=int(bool(use_graph)),
                                 precision=L.PRECISION_MIXED if precision == "mixed" else L.PRECISION_FP64,
                                 inner_tolerance=inner_tolerance, inner_iterations=inner_iterations,
-                                block_width=block_width)
+                                block_width=block_width, strip_window=float(strip_window))
         self.opt = opt
         self.nrhs = nrhs
         h = _Handle(opt)
@@ -348,6 +365,16 @@ class HipMgPcgSolver:
     def LevelLayout(self, level=0):
         """Layout the block kernels read on a level: "csr", "node-triple" or "node-block"."""
         return ("csr", "node-triple", "node-block")[self._h.level_layout(level)]
+
+    @property
+    def StripWindow(self):
+        """(X window in bytes of the block kernels' strip order, 0 = node order; ms per block PCG
+        iteration the autotuner timed for the windows 3e6, 1.5e6, 0.75e6 and 0)"""
+        return self._h.strip_window()
+
+    def LevelColors(self, level=0):
+        """Colours of the level's multicolour Gauss-Seidel (0 without that smoother)."""
+        return self._h.level_colors(level)
 
     def solve_device(self, b_dev, x_dev):
         return self._h.solve_device(b_dev, x_dev, False, self.nrhs)

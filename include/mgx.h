@@ -236,6 +236,9 @@ typedef struct {
                                    batched PCG (its SpMM reads each matrix once per block); 4, 8, 16 or 32,
                                    1 = one right-hand side at a time, 0 = chosen at initialize by timing
                                    PCG iterations of each width on the first right-hand sides (default) */
+    double  strip_window;       /* batching form on structured levels: X window (bytes) of the strip order in
+                                   which the block kernels visit the nodes, 0 = node order, < 0 = chosen at
+                                   initialize by timing 3e6, 1.5e6, 0.75e6 and 0 (default) */
 } mgx_options;
 
 typedef struct {
@@ -250,6 +253,8 @@ typedef struct {
     double  spmv_ms;         /* their summed device time (HIP events on the solver stream) */
     int32_t block_width;     /* right-hand sides solved together with this one (1: alone); with
                                 block_width > 1, solve_ms is the device time of that whole block */
+    int32_t gs_launches;     /* batching form with options.profile: level-0 Gauss-Seidel colour launches timed */
+    double  gs_ms;           /* their summed device time (HIP events on the solver stream) */
 } mgx_stats;
 
 typedef struct mgx_solver_s *mgx_solver;
@@ -291,6 +296,13 @@ int mgx_solver_block_width(mgx_solver s, int32_t *width, double *tune_ms);
 #define MGX_LAYOUT_NODE_TRIPLE  1   /* CSR read by node triples (rows 3q..3q+2 share their columns) */
 #define MGX_LAYOUT_NODEBLOCK    2   /* 3x3 node blocks (mgx_nodeblock_from_csr) */
 int mgx_solver_level_layout(mgx_solver s, int32_t level, int32_t *layout);
+/* X window (bytes) of the strip order in which the block kernels visit the nodes of structured
+ * levels (0: node order) and, when the autotuner chose it, the ms per block PCG iteration it timed
+ * for the windows 3e6, 1.5e6, 0.75e6 and 0 (tune_ms: 4 entries, may be NULL).  No reference
+ * counterpart (batching form). */
+int mgx_solver_strip_window(mgx_solver s, double *bytes, double *tune_ms);
+/* Colours of the multicolour Gauss-Seidel node colouring of a level (0 without that smoother). */
+int mgx_solver_level_colors(mgx_solver s, int32_t level, int32_t *ncolors);
 /* Replaces ReleaseOpenCLResources (OpenCLCsrGeometricMultigridSolver.cs:106-138). */
 int mgx_solver_release(mgx_solver s);
 
