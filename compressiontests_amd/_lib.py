@@ -149,6 +149,7 @@ _SIGS = {
     "mgx_solver_block_width": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "mgx_solver_level_layout": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
     "mgx_solver_level_colors": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
+    "mgx_solver_gs_order": (C.c_int, [_P, _I32, _P]),
     "mgx_solver_strip_window": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mgx_solver_release": (C.c_int, [_P]),
     "mgx_device_malloc": (C.c_int, [C.POINTER(_P), _U64]),

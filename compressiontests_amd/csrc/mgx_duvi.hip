@@ -8,15 +8,16 @@
 // values[d] ("DU + raw values").
 //
 // A wavefront owns a tile of 8 consecutive rows (8 lanes per row) and needs no block-wide
-// barrier: (1) every lane issues the value loads of its first segment; (2) the tile's distance
-// bytes (one contiguous range) and block columns go to the wave's LDS region; (3) lane 0 of each
-// row walks the row's block chain (one step per block: the count byte) and marks each block's
-// first slot with its column; (4) in rounds of 96 entries per row, lane l takes the 12 slots
-// 12l .. 12l+11 of the round: it folds its slots into (has a block start, column or column
-// offset), a segmented scan over the row's 8 lanes gives every lane the column before its
-// first slot, and the lane then decodes its slots again, gathering x and accumulating the
-// products; (5) the 8 lanes sum the row.  A tile of more than kDwCap entries falls back to one
-// lane per row decoding straight from global memory.
+// barrier: (1) every lane issues the loads of the raw values of its first slots (raw mode); the
+// tile's distance bytes and, with value indices, its index stream are one contiguous range each
+// and go to the wave's LDS region with coalesced 16-byte loads; (2) each row's lanes walk the
+// row's block chain together (one step per block: the count byte) and mark each block's first
+// slot with its column; (3) in rounds of 96 slots per row, lane l takes the 12 slots 12l ..
+// 12l+11 of the round: it folds its slots into (a block starts among them, column or column
+// offset), a segmented scan over the row's 8 lanes gives every lane the column before its first
+// slot, and the lane decodes its slots again, gathering x and accumulating the products; (4) the
+// 8 lanes sum the row.  A tile of more than kDwCap slots falls back to one lane per row decoding
+// straight from global memory.
 #include "mgx_device.h"
 
 namespace mgx {
@@ -33,7 +34,8 @@ __device__ __forceinline__ double duvi_value(const void *vi, const double *value
 }
 
 constexpr int kDwRows = 8;                     // rows per wave tile
-constexpr int kDwSeg = 12;                     // slots per lane per round (8 lanes: 96 per row and round)
+constexpr int kDwLanes = 8;                    // lanes per row
+constexpr int kDwSeg = 12;                     // slots per lane per round (96 per row and round)
 constexpr int kDwCap = 768;                    // slots staged per wave tile
 constexpr int kDwWaves = kBlock / 64;
 
@@ -45,6 +47,43 @@ __device__ __forceinline__ void dw_fence()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
+// Copy bytes [lo, hi) of src (tot bytes long) into dst[0 ..) shifted by lo & 15.  The 16-byte
+// slots wholly inside src are loaded as vectors, NI rounds of 64 slots issued before the first
+// store (slots past the range reload the last whole slot into dst slots < 64 NI that the tile
+// does not use); slots reaching past the end of src are copied byte by byte.
+template <int NI>
+__device__ __forceinline__ void dw_stage(const char *__restrict__ src, int64_t lo, int64_t hi, int64_t tot,
+                                         dw_u32x4 *dst, int lane)
+{
+    const int64_t a0 = lo & ~(int64_t)15;
+    const int nq = (int)((hi - a0 + 15) >> 4);
+    const int nin = (int)min<int64_t>((int64_t)nq, (tot - a0) >> 4);
+    if (nin > 0) {
+        const dw_u32x4 *s = (const dw_u32x4 *)(src + a0);
+        dw_u32x4 t[NI];
+#pragma unroll
+        for (int i = 0; i < NI; ++i) t[i] = __builtin_nontemporal_load(s + min(lane + 64 * i, nin - 1));
+#pragma unroll
+        for (int i = 0; i < NI; ++i) dst[lane + 64 * i] = t[i];
+    }
+    for (int q = max(nin, 0) + lane; q < nq; q += 64) {
+        uint8_t *o = (uint8_t *)&dst[q];
+        for (int i = 0; i < 16; ++i) o[i] = a0 + 16 * q + i < tot ? (uint8_t)src[a0 + 16 * q + i] : 0;
+    }
+}
+
+// staged index stream: 16-byte rounds of 64 slots per wave (none in raw mode)
+template <int VIB> struct DwIdx { static constexpr int NI = VIB == 0 ? 1 : (kDwCap * VIB / 16 + 2 + 63) / 64; };
+
+template <int VIB>
+__device__ __forceinline__ double dw_value(const uint8_t *si, const double *__restrict__ values, int64_t d0, int k)
+{
+    if (VIB == 0) return values[d0 + k];
+    if (VIB == 1) return values[si[k]];
+    if (VIB == 2) return values[((const uint16_t *)si)[k]];
+    return values[((const uint32_t *)si)[k]];
+}
+
 template <int VIB>
 __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz, const int32_t *__restrict__ row_blocks,
                                                       const int32_t *__restrict__ row_dist,
@@ -53,11 +92,12 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
                                                       const double *__restrict__ values, const double *__restrict__ x,
                                                       double *__restrict__ y, int clear, const int32_t *__restrict__ done)
 {
-    __shared__ dw_u32x4 s_dq[kDwWaves][(kDwCap + 32) / 16];   // distance bytes, shifted by d0 & 15
-    __shared__ int32_t s_h[kDwWaves][kDwCap];                  // column + 1 at a block's first slot, else 0
-    __shared__ int32_t s_bc[kDwWaves][kDwCap];                 // the tile's block columns
+    __shared__ dw_u32x4 s_iq[kDwWaves][64 * DwIdx<VIB>::NI];  // value indices (not used in raw mode)
+    __shared__ dw_u32x4 s_dq[kDwWaves][64];                   // distance bytes
+    __shared__ int32_t s_h[kDwWaves][kDwCap];                 // column + 1 at a block's first slot, else 0
+    static_assert(64 * 16 >= kDwCap + 32, "distance staging");
     if (done && *done) return;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane >> 3, l = lane & 7;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane / kDwLanes, l = lane % kDwLanes;
     const int ntiles = (nrows + kDwRows - 1) / kDwRows;
     const int tile = xcd_remap(blockIdx.x, gridDim.x) * kDwWaves + wv;
     if (tile >= ntiles) return;                                // the whole wave leaves together
@@ -88,48 +128,39 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
         s = (int)(row_dist[row] - d0); e = (int)(row_dist[row + 1] - d0);
         rb0 = row_blocks[row]; rb1 = row_blocks[row + 1];
     }
-    // (1) values of this lane's slots in round 0: they do not depend on the decode
-    double w[kDwSeg];
-    {
+    // (1) raw values of this lane's slots in round 0 (they do not depend on the decode); the
+    //     index stream and the distance bytes to LDS; cleared start marks
+    double w0[kDwSeg];
+    if (VIB == 0) {
         const int a = s + l * kDwSeg;
 #pragma unroll
-        for (int j = 0; j < kDwSeg; ++j) w[j] = a + j < e ? duvi_value<VIB>(vi, values, d0 + a + j) : 0.0;
+        for (int j = 0; j < kDwSeg; ++j) w0[j] = a + j < e ? values[d0 + a + j] : 0.0;
     }
-    // (2) distance bytes (16-byte loads inside the array, byte loads at its end), block columns,
-    //     cleared start marks
-    const int64_t g0 = d0 & ~(int64_t)15;
-    const int sh = (int)(d0 - g0);
-    const uint8_t *sd = (const uint8_t *)s_dq[wv] + sh;          // sd[k]: distance byte of slot d0 + k
+    if (VIB != 0) dw_stage<DwIdx<VIB>::NI>((const char *)vi, d0 * VIB, (d0 + cnt) * VIB, nnz * VIB, s_iq[wv], lane);
+    dw_stage<1>((const char *)dist, d0, d0 + cnt, nnz, s_dq[wv], lane);
+    for (int k = lane; k < cnt; k += 64) s_h[wv][k] = 0;
+    const uint8_t *sd = (const uint8_t *)s_dq[wv] + (d0 & 15);               // sd[k]: distance byte of slot d0 + k
+    const uint8_t *si = (const uint8_t *)s_iq[wv] + ((d0 * (VIB ? VIB : 1)) & 15);
+    dw_fence();
+    // (2) the block chain of each row, walked by its 8 lanes together (the same d in every
+    //     lane); the block columns come 8 at a time, one per lane
     {
-        const int nq = (sh + cnt + 15) >> 4;
-        for (int q = lane; q < nq; q += 64) {
-            const int64_t g = g0 + 16 * q;
-            if (g + 16 <= nnz) {
-                s_dq[wv][q] = __builtin_nontemporal_load((const dw_u32x4 *)(dist + g));
-            } else {
-                uint8_t *o = (uint8_t *)&s_dq[wv][q];
-                for (int i = 0; i < 16; ++i) o[i] = g + i < nnz ? dist[g + i] : 0;
+        int d = s;
+        for (int bw = rb0; bw < rb1; bw += kDwLanes) {
+            const int bc = bw + l < rb1 ? block_col[bw + l] : 0;
+            const int nb = min(kDwLanes, rb1 - bw);
+            for (int j = 0; j < nb; ++j) {
+                const int c = __shfl(bc, j, kDwLanes);
+                if (l == 0 && d < e) s_h[wv][d] = c + 1;
+                d += sd[d] + 1;
             }
         }
-        const int b0t = row_blocks[r0], nbt = row_blocks[r1] - b0t;
-        for (int i = lane; i < nbt; i += 64) s_bc[wv][i] = block_col[b0t + i];
-        for (int k = lane; k < cnt; k += 64) s_h[wv][k] = 0;
     }
     dw_fence();
-    // (3) the block chain of each row: mark every block's first slot with its column + 1
-    if (l == 0 && row < r1) {
-        const int b0t = row_blocks[r0];
-        int d = s;
-        for (int b = rb0; b < rb1 && d < e; ++b) {
-            s_h[wv][d] = s_bc[wv][b - b0t] + 1;
-            d += sd[d] + 1;
-        }
-    }
-    dw_fence();
-    // (4) rounds of 96 slots per row
+    // (3) rounds of 96 slots per row
     double acc = 0;
     int carry = 0;                                             // column before the round's first slot
-    for (int base = s, round = 0; base < e; base += 8 * kDwSeg, ++round) {
+    for (int base = s, round = 0; base < e; base += kDwLanes * kDwSeg, ++round) {
         const int a = base + l * kDwSeg, bnd = min(a + kDwSeg, e);
         // fold this lane's slots: h = a block starts among them; v = column of the last slot
         // (h) or its offset from the column before the first slot (!h)
@@ -144,18 +175,18 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
         }
         // inclusive segmented scan over the row's 8 lanes
 #pragma unroll
-        for (int o = 1; o < 8; o <<= 1) {
-            const int ph = __shfl_up(h, o, 8), pv = __shfl_up(v, o, 8);
+        for (int o = 1; o < kDwLanes; o <<= 1) {
+            const int ph = __shfl_up(h, o, kDwLanes), pv = __shfl_up(v, o, kDwLanes);
             if (l >= o && !h) { v += pv; h = ph; }
         }
         // column before this lane's first slot
-        const int ph = __shfl_up(h, 1, 8), pv = __shfl_up(v, 1, 8);
+        const int ph = __shfl_up(h, 1, kDwLanes), pv = __shfl_up(v, 1, kDwLanes);
         int col = (l == 0) ? carry : (ph ? pv : carry + pv);
         // the round's last column, for the next round
-        const int th = __shfl(h, 7, 8), tv = __shfl(v, 7, 8);
+        const int th = __shfl(h, kDwLanes - 1, kDwLanes), tv = __shfl(v, kDwLanes - 1, kDwLanes);
         carry = th ? tv : carry + tv;
         // decode again, gathering x
-        double xv[kDwSeg], wv2[kDwSeg];
+        double xv[kDwSeg], w[kDwSeg];
 #pragma unroll
         for (int j = 0; j < kDwSeg; ++j) {
             const int k = a + j;
@@ -166,14 +197,15 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
                 c = col;
             }
             xv[j] = k < bnd ? x[c] : 0.0;
-            wv2[j] = round == 0 ? w[j] : (k < bnd ? duvi_value<VIB>(vi, values, d0 + k) : 0.0);
+            if (VIB == 0 && round == 0) w[j] = w0[j];
+            else w[j] = k < bnd ? dw_value<VIB>(si, values, d0, k) : 0.0;
         }
 #pragma unroll
-        for (int j = 0; j < kDwSeg; ++j) acc += wv2[j] * xv[j];
+        for (int j = 0; j < kDwSeg; ++j) acc += w[j] * xv[j];
     }
-    // (5) row sum over the 8 lanes
+    // (4) row sum over the 8 lanes
 #pragma unroll
-    for (int o = 4; o > 0; o >>= 1) acc += __shfl_xor(acc, o, 8);
+    for (int o = kDwLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kDwLanes);
     if (l == 0 && row < r1) y[row] = clear ? acc : y[row] + acc;
 }
 

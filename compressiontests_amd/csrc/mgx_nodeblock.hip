@@ -281,32 +281,25 @@ template void launch_nb_spmm<double>(int, int, const NbView &, const double *, c
 // Each node: a[r] = sum over its off-diagonal blocks (slots 1 ..) of A X, then the three dofs are
 // relaxed in order with the node's own block D (slot 0): x_0 = (b_0 - a_0 - D01 x_1 - D02 x_2) /
 // D00, then x_1 with the new x_0, then x_2 (reverse order when backward).
+// One wave tile [pa, pb) of the colour-ordered copy (pb - pa <= NW): stage its blocks, sum the
+// off-diagonal blocks, relax each node.  sv / sc: the wave's LDS staging region.
 template <typename T, int K>
-__global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, const int32_t *__restrict__ nrp,
-                                                  const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
-                                                  const T *__restrict__ val, const T *__restrict__ B, T *__restrict__ X,
-                                                  int backward, int reverse, const int32_t *__restrict__ done)
+__device__ __forceinline__ void nb_gs_tile(int pa, int pb, const int32_t *__restrict__ nrp,
+                                           const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
+                                           const T *__restrict__ val, const T *__restrict__ B, T *__restrict__ X,
+                                           int backward, u32x4 *sv, u32x4 *sc, int lane)
 {
     using Sh = NbShape<T, K>;
     using VT = typename NbVec<T>::type;
-    __shared__ u32x4 s_val[kNbWaves][Sh::VQ];
-    __shared__ u32x4 s_col[kNbWaves][Sh::CQ];
-    if (done && *done) return;
-    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = lane / Sh::G, l = lane % Sh::G, s = l / Sh::V, c0 = (l % Sh::V) * Sh::W;
-    const int ntiles = (p1c - p0c + Sh::NW - 1) / Sh::NW;
-    const int t0 = xcd_remap(blockIdx.x, gridDim.x) * kNbWaves + wv;
-    if (t0 >= ntiles) return;                                    // the whole wave leaves together
-    const int tile = reverse ? ntiles - 1 - t0 : t0;
-    const int pa = p0c + tile * Sh::NW, pb = min(pa + Sh::NW, p1c);
     const int b0 = nrp[pa], b1 = nrp[pb];
     const int p = pa + g;
     VT y[3] = {VT(0), VT(0), VT(0)};
-    stage16<Sh::VQ>((const char *)val, (size_t)b0 * 9 * sizeof(T), (size_t)b1 * 9 * sizeof(T), s_val[wv], lane);
-    stage16<Sh::CQ>((const char *)ncol, (size_t)b0 * 4, (size_t)b1 * 4, s_col[wv], lane);
+    stage16<Sh::VQ>((const char *)val, (size_t)b0 * 9 * sizeof(T), (size_t)b1 * 9 * sizeof(T), sv, lane);
+    stage16<Sh::CQ>((const char *)ncol, (size_t)b0 * 4, (size_t)b1 * 4, sc, lane);
     wave_lds_fence();
-    const T *lv = (const T *)((const char *)s_val[wv] + (((size_t)b0 * 9 * sizeof(T)) & 15));
-    const int32_t *lc = (const int32_t *)((const char *)s_col[wv] + (((size_t)b0 * 4) & 15));
+    const T *lv = (const T *)((const char *)sv + (((size_t)b0 * 9 * sizeof(T)) & 15));
+    const int32_t *lc = (const int32_t *)((const char *)sc + (((size_t)b0 * 4) & 15));
     if (p < pb) nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + 1 + s, nrp[p + 1] - b0, X, c0, y);
     nb_reduce<T, K>(y);
     if (p >= pb || s != 0) return;
@@ -328,6 +321,25 @@ __global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, cons
     *(VT *)(X + i0) = x0;
     *(VT *)(X + i0 + K) = x1;
     *(VT *)(X + i0 + 2 * K) = x2;
+}
+
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, const int32_t *__restrict__ nrp,
+                                                  const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
+                                                  const T *__restrict__ val, const T *__restrict__ B, T *__restrict__ X,
+                                                  int backward, int reverse, const int32_t *__restrict__ done)
+{
+    using Sh = NbShape<T, K>;
+    __shared__ u32x4 s_val[kNbWaves][Sh::VQ];
+    __shared__ u32x4 s_col[kNbWaves][Sh::CQ];
+    if (done && *done) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int ntiles = (p1c - p0c + Sh::NW - 1) / Sh::NW;
+    const int t0 = xcd_remap(blockIdx.x, gridDim.x) * kNbWaves + wv;
+    if (t0 >= ntiles) return;                                    // the whole wave leaves together
+    const int tile = reverse ? ntiles - 1 - t0 : t0;
+    const int pa = p0c + tile * Sh::NW;
+    nb_gs_tile<T, K>(pa, min(pa + Sh::NW, p1c), nrp, ncol, node, val, B, X, backward, s_val[wv], s_col[wv], lane);
 }
 
 template <typename T>

@@ -141,6 +141,11 @@ class _Handle:
         L.call("mgx_solver_strip_window", C.c_void_p(self.h), C.byref(b), ms)
         return b.value, list(ms)
 
+    def gs_order(self, level, nnodes):
+        o = np.empty(nnodes, np.int32)
+        L.call("mgx_solver_gs_order", C.c_void_p(self.h), level, o.ctypes.data_as(C.c_void_p))
+        return o
+
     def level_colors(self, level):
         v = C.c_int32()
         L.call("mgx_solver_level_colors", C.c_void_p(self.h), level, C.byref(v))
@@ -286,7 +291,7 @@ class HipMgPcgSolver:
     a list of statistics.  block_width: right-hand sides advanced together by one batched PCG
     (4, 8, 16, 32), 1 = one at a time, 0 = chosen at Initialize by timing each width.
     strip_window: X window (bytes) of the strip order of the block kernels on structured levels,
-    0 = node order, < 0 = chosen at Initialize by timing the candidates.
+    0 = whole planes, < 0 = chosen at Initialize by timing the candidates.
     precision "mixed": FP32 SpMV / V-cycle inside FP64 iterative refinement."""
 
     def __init__(self, model, levels=3, smoother="jacobi", omega=0.0, nu=2, tolerance=1e-8, stop="rel",
@@ -368,9 +373,14 @@ class HipMgPcgSolver:
 
     @property
     def StripWindow(self):
-        """(X window in bytes of the block kernels' strip order, 0 = node order; ms per block PCG
+        """(X window in bytes of the block kernels' strip order, 0 = whole planes; ms per block PCG
         iteration the autotuner timed for the windows 3e6, 1.5e6, 0.75e6 and 0)"""
         return self._h.strip_window()
+
+    def GsOrder(self, level=0):
+        """Node order of the level's forward Gauss-Seidel sweep as the solver runs it."""
+        A = self.hierarchy[level][0]
+        return self._h.gs_order(level, A.nrows // 3)
 
     def LevelColors(self, level=0):
         """Colours of the level's multicolour Gauss-Seidel (0 without that smoother)."""

@@ -237,7 +237,7 @@ typedef struct {
                                    1 = one right-hand side at a time, 0 = chosen at initialize by timing
                                    PCG iterations of each width on the first right-hand sides (default) */
     double  strip_window;       /* batching form on structured levels: X window (bytes) of the strip order in
-                                   which the block kernels visit the nodes, 0 = node order, < 0 = chosen at
+                                   which the block kernels visit the nodes, 0 = whole planes, < 0 = chosen at
                                    initialize by timing 3e6, 1.5e6, 0.75e6 and 0 (default) */
 } mgx_options;
 
@@ -297,12 +297,19 @@ int mgx_solver_block_width(mgx_solver s, int32_t *width, double *tune_ms);
 #define MGX_LAYOUT_NODEBLOCK    2   /* 3x3 node blocks (mgx_nodeblock_from_csr) */
 int mgx_solver_level_layout(mgx_solver s, int32_t level, int32_t *layout);
 /* X window (bytes) of the strip order in which the block kernels visit the nodes of structured
- * levels (0: node order) and, when the autotuner chose it, the ms per block PCG iteration it timed
+ * levels (0: whole planes) and, when the autotuner chose it, the ms per block PCG iteration it timed
  * for the windows 3e6, 1.5e6, 0.75e6 and 0 (tune_ms: 4 entries, may be NULL).  No reference
  * counterpart (batching form). */
 int mgx_solver_strip_window(mgx_solver s, double *bytes, double *tune_ms);
 /* Colours of the multicolour Gauss-Seidel node colouring of a level (0 without that smoother). */
 int mgx_solver_level_colors(mgx_solver s, int32_t level, int32_t *ncolors);
+/* The node order (n/3 nodes) of the level's forward Gauss-Seidel sweep as the solver runs it: the
+ * sweep relaxes node order[0], then order[1], ... (rows 3q, 3q+1, 3q+2 of node q in turn), the
+ * backward sweep the exact reverse; nodes that the kernels relax together (one colour) are
+ * uncoupled, so the parallel sweep equals this sequential one: colour 0's nodes, then colour
+ * 1's, ...  The reference's GPU "Gauss-Seidel" (CsrGeometricMultigrid.cl:207-225) updates in
+ * place with racy reads; this ordering is the documented deviation. */
+int mgx_solver_gs_order(mgx_solver s, int32_t level, int32_t *order);
 /* Replaces ReleaseOpenCLResources (OpenCLCsrGeometricMultigridSolver.cs:106-138). */
 int mgx_solver_release(mgx_solver s);
 

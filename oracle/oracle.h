@@ -82,6 +82,10 @@ void orc_gs_forward(int32_t n, const int32_t *rp, const int32_t *ci, const doubl
 int orc_block_coloring(int32_t nblocks, int bs, const int32_t *rp, const int32_t *ci, int32_t *color);
 void orc_gs_multicolor(int32_t n, int bs, const int32_t *rp, const int32_t *ci, const double *v,
                        int ncolors, const int32_t *color, const double *b, double *x, int backward);
+/* Gauss-Seidel sweep over node blocks of bs rows in a given node order: forward = nodes
+   order[0 .. nn), rows in order; backward = the exact reverse (nodes and rows). */
+void orc_gs_ordered(int32_t n, int bs, const int32_t *rp, const int32_t *ci, const double *v,
+                    const int32_t *order, const double *b, double *x, int backward);
 
 /* ---------------- dense direct solve (LDL^T / Cholesky of the coarsest level) */
 int orc_dense_cholesky(int32_t n, double *A /*in: SPD, out: L lower*/);
@@ -132,6 +136,8 @@ typedef struct {
     const int32_t *Arp, *Aci; const double *Av;   /* level matrix */
     const int32_t *Rrp, *Rci; const double *Rv;   /* restriction to level+1 (NULL on coarsest) */
     const int32_t *Prp, *Pci; const double *Pv;   /* interpolation from level+1 (NULL on coarsest) */
+    const int32_t *gs_order;        /* Gauss-Seidel node order of the forward sweep (n/3 nodes; NULL:
+                                       multicolour order of orc_block_coloring) */
 } orc_level;
 
 typedef struct {

@@ -31,7 +31,8 @@ class OrcLevel(C.Structure):
     _fields_ = [("n", C.c_int32),
                 ("Arp", C.c_void_p), ("Aci", C.c_void_p), ("Av", C.c_void_p),
                 ("Rrp", C.c_void_p), ("Rci", C.c_void_p), ("Rv", C.c_void_p),
-                ("Prp", C.c_void_p), ("Pci", C.c_void_p), ("Pv", C.c_void_p)]
+                ("Prp", C.c_void_p), ("Pci", C.c_void_p), ("Pv", C.c_void_p),
+                ("gs_order", C.c_void_p)]
 
 
 class OrcDuviBlock(C.Structure):
@@ -245,6 +246,15 @@ def block_coloring(A, bs=3):
     return nc, col
 
 
+def gs_ordered(A, order, b, x, backward=False, bs=3):
+    """Gauss-Seidel sweep over node blocks in the given node order (backward: exact reverse)."""
+    x = np.array(x, np.float64)
+    o = np.ascontiguousarray(order, np.int32)
+    lib().orc_gs_ordered(A.nrows, bs, _p(A.rowptr), _p(A.col), _p(A.val), _p(o), _p(np.ascontiguousarray(b)), _p(x),
+                         int(backward))
+    return x
+
+
 def gs_multicolor(A, ncol, color, b, x, backward=False, bs=3):
     x = np.array(x, np.float64)
     lib().orc_gs_multicolor(A.nrows, bs, _p(A.rowptr), _p(A.col), _p(A.val), ncol, _p(color),
@@ -396,6 +406,11 @@ def _levels(levels):
             arr[i].Rrp, arr[i].Rci, arr[i].Rv = _p(R.rowptr), _p(R.col), _p(R.val)
             arr[i].Prp, arr[i].Pci, arr[i].Pv = _p(P.rowptr), _p(P.col), _p(P.val)
             keep += [R, P]
+        if lv.get("gs_order") is not None:
+            o = np.ascontiguousarray(lv["gs_order"], np.int32)
+            assert o.size == A.nrows // 3
+            arr[i].gs_order = _p(o)
+            keep.append(o)
     return arr, keep
 
 

@@ -177,7 +177,8 @@ static void vcycle(vcyc *V, int l)
     if (l == V->L - 1) { orc_dense_cholesky_solve(q->n, V->Lc, b, x); return; }
     memset(x, 0, sizeof(double) * q->n);
     for (int s = 0; s < V->nu; ++s) {
-        if (V->smoother == 1) orc_gs_multicolor(q->n, 3, q->Arp, q->Aci, q->Av, V->ncol[l], V->color[l], b, x, 0);
+        if (V->smoother == 1 && q->gs_order) orc_gs_ordered(q->n, 3, q->Arp, q->Aci, q->Av, q->gs_order, b, x, 0);
+        else if (V->smoother == 1) orc_gs_multicolor(q->n, 3, q->Arp, q->Aci, q->Av, V->ncol[l], V->color[l], b, x, 0);
         else if (s == 0) orc_jacobi_initial(q->n, V->pre[l], b, x);
         else { orc_csr_jacobi(q->n, V->pre[l], q->Arp, q->Aci, q->Av, b, x, r); memcpy(x, r, sizeof(double) * q->n); }
     }
@@ -186,7 +187,8 @@ static void vcycle(vcyc *V, int l)
     vcycle(V, l + 1);
     orc_csr_spmv(q->n, q->Prp, q->Pci, q->Pv, V->x[l + 1], x, 0);
     for (int s = 0; s < V->nu; ++s) {
-        if (V->smoother == 1) orc_gs_multicolor(q->n, 3, q->Arp, q->Aci, q->Av, V->ncol[l], V->color[l], b, x, 1);
+        if (V->smoother == 1 && q->gs_order) orc_gs_ordered(q->n, 3, q->Arp, q->Aci, q->Av, q->gs_order, b, x, 1);
+        else if (V->smoother == 1) orc_gs_multicolor(q->n, 3, q->Arp, q->Aci, q->Av, V->ncol[l], V->color[l], b, x, 1);
         else { orc_csr_jacobi(q->n, V->pre[l], q->Arp, q->Aci, q->Av, b, x, r); memcpy(x, r, sizeof(double) * q->n); }
     }
 }

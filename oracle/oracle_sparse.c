@@ -171,6 +171,23 @@ int orc_block_coloring(int32_t nblocks, int bs, const int32_t *rp, const int32_t
     return ncol;
 }
 
+/* Any fixed node order gives a Gauss-Seidel sweep (each node relaxed once with the latest values
+   of its neighbours); the multicolour sweep above is the order colour 0, colour 1, ...  A
+   parallel sweep that relaxes pairwise uncoupled nodes together equals the sequential sweep in
+   any order that keeps its groups in sequence. */
+void orc_gs_ordered(int32_t n, int bs, const int32_t *rp, const int32_t *ci, const double *v,
+                    const int32_t *order, const double *b, double *x, int backward)
+{
+    int32_t nb = n / bs;
+    if (!backward) {
+        for (int32_t i = 0; i < nb; ++i)
+            for (int a = 0; a < bs; ++a) gs_row(order[i] * bs + a, rp, ci, v, b, x);
+    } else {
+        for (int32_t i = nb - 1; i >= 0; --i)
+            for (int a = bs - 1; a >= 0; --a) gs_row(order[i] * bs + a, rp, ci, v, b, x);
+    }
+}
+
 void orc_gs_multicolor(int32_t n, int bs, const int32_t *rp, const int32_t *ci, const double *v,
                        int ncolors, const int32_t *color, const double *b, double *x, int backward)
 {

@@ -160,6 +160,9 @@ def test_block_fp64_matches_oracle_per_column(het16, smoother, sm, bw):
     assert s.BlockWidth[0] == bw
     X, st = s.Solve()
     assert X.shape == F.shape
+    if smoother == "gs":
+        # the oracle relaxes the nodes in the order the solver's sweep does
+        lv = [dict(d, gs_order=s.GsOrder(l)) if "R" in d else d for l, d in enumerate(lv)]
     for j in range(6):
         xo, so = O.mgpcg(lv, F[j], smoother=sm, omega=0.0, nu=2, tol=1e-10)
         assert so.converged == 1
@@ -171,15 +174,22 @@ def test_block_fp64_matches_oracle_per_column(het16, smoother, sm, bw):
 
 def test_block_columns_match_single_rhs_solves(het16):
     """A column of a block solve is the single-right-hand-side solve of that column up to
-    rounding: same iteration count (+-1), solutions within 1e-9 in the energy norm."""
+    rounding: each path matches the oracle run in its own Gauss-Seidel node order (iterations
+    +-1, 1e-9 in the energy norm), and the two agree to the solve tolerance."""
     m, F, lv, A = het16
     blk = HipMgPcgSolver(m, levels=3, smoother="gs", nu=1, tolerance=1e-10, nrhs=6, block_width=8)
     blk.Initialize(F)
     X, st = blk.Solve()
+    lvb = [dict(d, gs_order=blk.GsOrder(l)) if "R" in d else d for l, d in enumerate(lv)]
     for j in (0, 3, 5):
         one = HipMgPcgSolver(m, levels=3, smoother="gs", nu=1, tolerance=1e-10)
         one.Initialize(F[j])
         x1, s1 = one.Solve()
+        lv1 = [dict(d, gs_order=one.GsOrder(l)) if "R" in d else d for l, d in enumerate(lv)]
+        for (x, it), lvo in (((X[j], st[j].NumIterationsRequired), lvb), ((x1, s1.NumIterationsRequired), lv1)):
+            xo, so = O.mgpcg(lvo, F[j], smoother=1, nu=1, tol=1e-10)
+            assert abs(it - so.iterations) <= 1, j
+            assert energy_err(A, x, xo) < 1e-9, j
         assert abs(st[j].NumIterationsRequired - s1.NumIterationsRequired) <= 1
         assert energy_err(A, X[j], x1) < 1e-9
 
@@ -268,19 +278,44 @@ def test_block_strip_order_48(gpu):
         assert energy_err(A, X[j], x1) < 1e-9
 
 
+def test_block_visiting_order_invariance(gpu):
+    """The block kernels' node visiting order (strip window 0 = whole planes, or strips of 2 and 4
+    lines, all in cube order) changes no node's arithmetic, only the grouping of the per-column
+    dot-product partials: the iterations agree and the solutions to rounding."""
+    n = 24
+    dens = 10 ** (4 * O.rng_uniform(20261206, n ** 3) - 2)
+    m = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
+    F = m.load_cases(8, SEED_LOADS)
+    res = []
+    for w in (0.0, 5e4, 1e5):
+        s = HipMgPcgSolver(m, levels=3, smoother="gs", nu=1, tolerance=1e-10, nrhs=8, block_width=8, strip_window=w)
+        s.Initialize(F)
+        assert s.StripWindow[0] == w
+        assert s.LevelLayout(0) == "node-block"
+        X, st = s.Solve()
+        assert all(t.HasConverged for t in st)
+        res.append((X, [t.NumIterationsRequired for t in st]))
+        s.ReleaseResources()
+    for X, it in res[1:]:
+        assert it == res[0][1]
+        assert np.max(np.abs(X - res[0][0])) <= 1e-11 * np.max(np.abs(res[0][0]))
+
+
 @pytest.mark.slow
 def test_block_mixed_64_32_load_cases(gpu):
     """64^3, 1e4 modulus contrast, 32 top-face load cases: the automatically blocked precision
-    form reaches 1e-8 on every FP64 true residual and agrees with FP64 block solves to 1e-12
-    within 1e-8 in the energy norm."""
+    form reaches 5e-9 on every FP64 true residual and agrees with FP64 block solves to 1e-12
+    within 1e-8 in the energy norm (the relative residual bounds the relative energy-norm error
+    only up to the conditioning: at residual 1e-8 one column of this 1e4-contrast system sits
+    at 1.15e-8, so the energy-norm target takes the tighter residual)."""
     n = 64
     dens = 10 ** (4 * O.rng_uniform(20261205, n ** 3) - 2)
     m = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
     F = m.load_cases(32, SEED_LOADS)
-    s = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-8, nrhs=32, precision="mixed")
+    s = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=5e-9, nrhs=32, precision="mixed")
     s.Initialize(F)
     X, st = s.Solve()
-    assert all(t.HasConverged and t.RelativeResidual <= 1.01e-8 for t in st)
+    assert all(t.HasConverged and t.RelativeResidual <= 5.05e-9 for t in st)
     t = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-12, nrhs=32, block_width=32)
     t.Initialize(F)
     Xt, stt = t.Solve()

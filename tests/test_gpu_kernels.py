@@ -195,13 +195,15 @@ def test_duvi_spmv_matches_oracle(gpu, mode):
 @pytest.mark.parametrize("mode", [0, 4, 8])
 def test_duvi_ragged_rows(gpu, mode):
     """Rows with > 256 consecutive columns (blocks split at 255 further entries), gaps of 255/256
-    and more, empty rows, a tile longer than the LDS capacity (one-thread-per-row fallback) and
-    accumulation (y += A x), against the oracle's DuVi SpMV."""
+    and more, empty rows, a tile longer than the LDS capacity (one-thread-per-row fallback), a
+    row decoded over several rounds, and accumulation (y += A x), against the oracle's DuVi SpMV."""
     rng = np.random.default_rng(21)
     ncols = 70000
     rows = [list(range(0, 300)), [], [5, 261, 262, 518, 60000], list(range(1000, 1600, 2))]
     rows += [sorted(rng.choice(ncols, size=40, replace=False).tolist()) for _ in range(40)]
     rows += [list(range(2000, 9000))] + [[7, 8]] * 5
+    # a tile of four rows holding one 300-entry row (several 96-slot decode rounds) and three empty rows
+    rows += [[], []] + [list(range(3000, 3600, 2))] + [[], [], []]
     rp, ci = [0], []
     for r in rows:
         ci += r

@@ -106,6 +106,30 @@ def test_multicolor_gs_is_gs_in_colour_order():
     assert np.allclose(y3[rev], y4, rtol=1e-12, atol=1e-12)
 
 
+def test_ordered_gs_is_gs_in_that_order():
+    """orc_gs_ordered: the forward sweep in any node order equals the textbook forward GS on the
+    matrix permuted to that order (a closed-form lower-triangular solve), the backward sweep the
+    forward GS of the reversed order; with the colour-list order it is the multicolour sweep."""
+    n = 5
+    A = O.cantilever(n, n, n, dens=10 ** (2 * O.rng_uniform(4, n ** 3) - 1))
+    S = sp.csr_matrix((A.val, A.col, A.rowptr), shape=(A.nrows, A.nrows))
+    rng = np.random.default_rng(8)
+    b, x = rng.normal(size=A.nrows), rng.normal(size=A.nrows)
+    order = rng.permutation(A.nrows // 3).astype(np.int32)
+    perm = (3 * order[:, None] + np.arange(3)[None, :]).reshape(-1)
+    Sp = S[perm][:, perm].tocsr()
+    ref = spl.spsolve_triangular(sp.tril(Sp, 0).tocsr(), b[perm] - sp.triu(Sp, 1).tocsr() @ x[perm], lower=True)
+    assert np.allclose(O.gs_ordered(A, order, b, x)[perm], ref, rtol=1e-12, atol=1e-12)
+    rev = perm[::-1]
+    Sr = S[rev][:, rev].tocsr()
+    ref = spl.spsolve_triangular(sp.tril(Sr, 0).tocsr(), b[rev] - sp.triu(Sr, 1).tocsr() @ x[rev], lower=True)
+    assert np.allclose(O.gs_ordered(A, order, b, x, backward=True)[rev], ref, rtol=1e-12, atol=1e-12)
+    ncol, color = O.block_coloring(A)
+    corder = np.concatenate([np.nonzero(color == c)[0] for c in range(ncol)]).astype(np.int32)
+    assert np.array_equal(O.gs_ordered(A, corder, b, x), O.gs_multicolor(A, ncol, color, b, x))
+    assert np.array_equal(O.gs_ordered(A, corder, b, x, True), O.gs_multicolor(A, ncol, color, b, x, True))
+
+
 def test_cholesky_solve_against_numpy():
     A, S = spd_csr(120, 9)
     b = np.random.default_rng(2).normal(size=120)
