@@ -105,6 +105,8 @@ def main():
     ap.add_argument("--moduli", default="lognormal", choices=["lognormal", "loguniform"])
     ap.add_argument("--precision", default="mixed", choices=["fp64", "mixed"])
     ap.add_argument("--block-width", type=int, default=0, help="0: chosen by the library's timing")
+    ap.add_argument("--strip-window", type=float, default=-1,
+                    help="X bytes per strip of the level-0 tile order (-1: chosen by the library's timing)")
     ap.add_argument("--smoother", default="gs")
     ap.add_argument("--nu", type=int, default=1)
     ap.add_argument("--tol", type=float, default=1e-8, help="relative residual of every FP64 true residual")
@@ -136,7 +138,8 @@ def main():
     F = model.rhs()[None, :] if nrhs == 1 else model.load_cases(nrhs, SEED_LOADS)
     common = dict(levels=levels, smoother=args.smoother, nu=args.nu, check_every=args.check_every)
     solver = HipMgPcgSolver(model, tolerance=args.tol, max_iterations=500, use_graph=bool(args.graph), nrhs=nrhs,
-                            precision=args.precision, block_width=args.block_width, **common)
+                            precision=args.precision, block_width=args.block_width,
+                            strip_window=args.strip_window, **common)
     hier = solver.hierarchy
     solver.Initialize(F)
     width, tune_ms = solver.BlockWidth

@@ -303,6 +303,7 @@ struct Level {
     int tile_order_npb = 0;                // nodes per tile that tile_order was built for
     DevNb nb;                              // node-block copy for the block SpMM (node order)
     DevNb nbg;                             // node-block copy for block Gauss-Seidel (colour order, own block first)
+    DevNb nb64;                            // precision form, level 0: FP64 node-block copy for the true residual
     bool nb_unfit = false;                 // the level matrix is not a node-block matrix the kernels take
     std::vector<int32_t> color_off;        // colour c = positions color_off[c] .. color_off[c+1]-1 of nbg
     std::vector<DBuf<int32_t>> color_nodes;
@@ -1077,6 +1078,10 @@ static void level_spmm(Level &V, int K, int epi, const TV *val, const TX *X, con
                                V.tile_order_npb == nb_tile<TX>(K) ? V.tile_order.p : nullptr);
             return;
         }
+        if (nb_ready<TX>(V.nb64)) {
+            launch_nb_spmm<TX>(K, epi, V.nb64.view, X, B, pre, Y, done, partials, kPartialStride, s, nullptr);
+            return;
+        }
     }
     if (V.node3)
         launch_spmm_node<TV, TX>(K, epi, V.n / 3, V.A.rp, V.A.ci, val, X, B, pre, Y, done, partials, kPartialStride, s,
@@ -1403,6 +1408,9 @@ static void order_for_cache(mgx_solver_s *S, int K)
         // node-block copy in node order (independent of K): built once
         if (!V.nb.built() && !V.nb_unfit)
             V.nb_unfit = !build_nb<double>(V.nb, V.n, V.A.rp, V.A.ci, V.A.v, nullptr, false, mixed, st);
+        // the precision form's FP64 true residual on level 0 reads an FP64 copy
+        if (l == 0 && mixed && V.nb.built() && !V.nb64.built())
+            build_nb<double>(V.nb64, V.n, V.A.rp, V.A.ci, V.A.v, nullptr, false, false, st);
         const int npb = V.nb.built() ? (mixed ? nb_tile<float>(K) : nb_tile<double>(K)) : spmm_node_tile(K);
         bool strips = V.gx > 0;
         const int64_t lines = V.gy + 1, planes = V.gz + 1;

@@ -37,8 +37,9 @@ constexpr int kNbMaxBlocks = 27;   // neighbour nodes of the 27-node hexahedral 
 template <typename T, int K> struct NbShape {
     static constexpr int W = 16 / (int)sizeof(T);           // columns per lane
     static constexpr int V = K / W;                         // lanes covering the K columns of one row
-    static constexpr int GMIN = 2 * (int)sizeof(T);         // 8 lanes per node (FP32), 16 (FP64)
-    static constexpr int G = V > GMIN ? V : GMIN;           // lanes per node
+    static constexpr int SMIN = 2;                          // at least 2 lanes split a node's blocks
+    static constexpr int GMIN = 2 * (int)sizeof(T);         // and at least 8 (FP32) / 16 (FP64) lanes per node
+    static constexpr int G = SMIN * V > GMIN ? SMIN * V : GMIN;   // lanes per node
     static constexpr int S = G / V;                         // lanes splitting a node's blocks
     static constexpr int NW = 64 / G;                       // nodes per wave tile
     static constexpr int MB = kNbMaxBlocks;                 // blocks per node staged
@@ -48,7 +49,9 @@ template <typename T, int K> struct NbShape {
     static constexpr int CQ = ((NW * MB * 4) / 16 + 2 + 63) / 64 * 64;
     static constexpr int U = 3;                             // blocks in flight per lane
 };
-static_assert(NbShape<float, 4>::G == 8 && NbShape<float, 32>::S == 1 && NbShape<double, 32>::NW == 4, "shape");
+static_assert(NbShape<float, 4>::G == 8 && NbShape<float, 16>::S == 2 && NbShape<float, 32>::G == 16 &&
+                  NbShape<double, 32>::NW == 2,
+              "shape");
 
 constexpr int kNbWaves = kBlock / 64;
 
