@@ -16,7 +16,8 @@ Printed JSON line: value = solves/s (load cases solved per second); config carri
 FP64 GPU path (one right-hand side at a time) timed on the same system, the largest energy-norm
 error of the batched solutions against FP64 solves to 1e-12 of every load case, and the
 autotuner's timings; roofline = the kernel with the most device time in profiled solves of the
-same batch (HIP events on the solver stream around each launch), roofline_next = the other of
+same batch (HIP events on the solver stream around each SpMM launch and around each level-0
+Gauss-Seidel sweep of colour launches), roofline_next = the other of
 the level-0 Gauss-Seidel and SpMM; cpu_baseline = the oracle's single-threaded C MG-PCG on the
 same hierarchy (a bounded sample of its iterations).
 
@@ -65,10 +66,10 @@ def gs_sweep_bytes(nnz, n, k, val_bytes, ncolors):
     return matrix + ncolors * vec + 2 * vec
 
 
-def roofline(name, bytes_per_launch, total_ms, launches, pmc_tag):
+def roofline(name, bytes_per_launch, total_ms, launches, pmc_tag, timing):
     """HBM roofline object of one kernel: algorithmic bytes per launch / average launch time (HIP
-    events on the solver stream around each launch), and the rocprofv3 PMC traffic per launch if
-    profiles/<pmc_tag>_pmc.json holds it."""
+    events on the solver stream, placed as `timing` says), and the rocprofv3 PMC traffic per launch
+    if profiles/<pmc_tag>_pmc.json holds it."""
     avg_ms = total_ms / max(launches, 1)
     achieved = bytes_per_launch / (avg_ms * 1e-3) / 1e9 if avg_ms > 0 else 0.0
     traffic = None
@@ -79,7 +80,7 @@ def roofline(name, bytes_per_launch, total_ms, launches, pmc_tag):
         except (OSError, ValueError):
             traffic = None
     return {"bound": "hbm",
-            "kernel": name + ", HIP events on the solver stream around each launch in profiled solves of the same batch",
+            "kernel": name + ", HIP events on the solver stream " + timing + " in profiled solves of the same batch",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_launch": int(bytes_per_launch),
             "avg_launch_ms": round(avg_ms, 5), "launches": launches, "total_ms": round(total_ms, 3)}
@@ -239,13 +240,14 @@ def main():
         spmm_name = f"k_csr_stream<{tname}, DOT> (level-0 CSR-stream SpMV q = A p with p.q)"
     kind = ("spmm_nodeblock" if layout == "node-block" else "spmm") if width > 1 else "spmv"
     rooflines = [roofline(spmm_name, spmm_per_launch, spmv_ms, spmv_n,
-                          f"{kind}_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed")]
+                          f"{kind}_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed", "around each launch")]
     ncol = solver.LevelColors(0)
     if gs_n > 0 and ncol > 0:
         rooflines.append(roofline(f"k_nb_gs<{tname}, K={width}> (one colour of the level-0 node-block multicolour "
                                   f"Gauss-Seidel; bytes averaged over the {ncol} colours of a sweep)",
                                   gs_sweep_bytes(nnz, ndof, width, vb, ncol) / ncol, gs_ms, gs_n,
-                                  f"gs_nodeblock_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed"))
+                                  f"gs_nodeblock_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed",
+                                  f"around each sweep of {ncol} back-to-back colour launches"))
     # the dominant kernel: the most device time in the profiled solves
     rooflines.sort(key=lambda r: -r["total_ms"])
 
@@ -280,7 +282,7 @@ def main():
             "precision": args.precision, "block_width": width,
             "autotune_ms_per_iteration": {f"k{k}": round(v, 4) for k, v in zip((4, 8, 16, 32), tune_ms) if v > 0},
             "strip_window_bytes": strip[0],
-            "autotune_strip_ms_per_iteration": {f"{w:g}": round(v, 4) for w, v in zip((3e6, 1.5e6, 0.75e6, 0), strip[1])
+            "autotune_strip_ms_per_iteration": {f"{w:g}": round(v, 4) for w, v in zip((6e6, 3e6, 1.5e6, 0), strip[1])
                                                 if v > 0},
             "iterations_median": int(np.median(iters)), "iterations_max": int(max(iters)),
             "outer_iterations_max": int(max(outer)),

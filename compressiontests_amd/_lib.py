@@ -129,6 +129,9 @@ _SIGS = {
     "mgx_nodeblock_from_csr": (C.c_int, [_I32, _P, _P, _P, _I32, C.POINTER(_P)]),
     "mgx_nodeblock_spmm": (C.c_int, [_P, _I32, _P, _P, _I32, _P]),
     "mgx_nodeblock_destroy": (C.c_int, [_P]),
+    "mgx_nodetransfer_from_csr": (C.c_int, [_I32, _I32, _P, _P, _P, _I32, C.POINTER(_P)]),
+    "mgx_nodetransfer_apply": (C.c_int, [_P, _I32, _P, _P, _I32, _P]),
+    "mgx_nodetransfer_destroy": (C.c_int, [_P]),
     "mgx_simp_gradient": (C.c_int, [_I32, _I32, _I32, _D, _P, _D, _D, _D, _P, _P, _P]),
     "mgx_matrix_cantilever": (C.c_int, [_I32, _I32, _I32, _D, _D, _P, C.POINTER(_P)]),
     "mgx_matrix_transfer_reference": (C.c_int, [_I32, _I32, _I32, C.POINTER(_P), C.POINTER(_P)]),
@@ -148,6 +151,7 @@ _SIGS = {
     "mgx_solver_solve_device": (C.c_int, [_P, _P, _P, _I32, C.POINTER(Stats)]),
     "mgx_solver_block_width": (C.c_int, [_P, C.POINTER(C.c_int32), C.POINTER(C.c_double)]),
     "mgx_solver_level_layout": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
+    "mgx_solver_transfer_layout": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
     "mgx_solver_level_colors": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
     "mgx_solver_gs_order": (C.c_int, [_P, _I32, _P]),
     "mgx_solver_strip_window": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -162,6 +166,7 @@ _SIGS = {
     "mgx_time_csr_kernel": (C.c_int, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, C.POINTER(_D)]),
     "mgx_time_duvi_spmv": (C.c_int, [C.POINTER(Duvi), _P, _P, _I32, _I32, C.POINTER(_D)]),
     "mgx_time_copy_kernel": (C.c_int, [_P, _P, _U64, _I32, _I32, C.POINTER(_D)]),
+    "mgx_time_read_kernel": (C.c_int, [_P, _U64, _P, _I32, _I32, C.POINTER(_D)]),
     "mgx_rng_uniform": (C.c_int, [_U64, _I64, _P]),
     "mgx_rng_normal": (C.c_int, [_U64, _I64, _P]),
 }

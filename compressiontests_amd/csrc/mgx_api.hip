@@ -208,6 +208,65 @@ static bool build_nb(DevNb &nb, int32_t nrows, const int32_t *rp, const int32_t 
     return true;
 }
 
+struct DevNt {
+    NtView view;                  // view.nout == 0: not built
+    bool fp32 = false;
+    DBuf<int32_t> rp, ci;
+    DBuf<double> w64;
+    DBuf<float> w32;
+    bool built() const { return view.nout > 0; }
+    void reset()
+    {
+        view = NtView{};
+        rp.reset(); ci.reset(); w64.reset(); w32.reset();
+    }
+};
+
+// Builds nt from the device CSR (rp, ci, FP64 values v) of a transfer matrix with nrows rows and
+// ncols columns, both multiples of 3.  Returns false, with nothing built, if the matrix does not
+// act on the three dofs of a node alike (see k_nt_count).
+static bool build_nt(DevNt &nt, int32_t nrows, int32_t ncols, const int32_t *rp, const int32_t *ci, const double *v,
+                     bool fp32, hipStream_t s)
+{
+    nt.reset();
+    if (nrows <= 0 || nrows % 3 != 0 || ncols % 3 != 0) return false;
+    const int32_t nout = nrows / 3;
+    DBuf<int32_t> cnt, bad;
+    cnt.alloc(nout);
+    bad.alloc(1);
+    MGX_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), s));
+    nt_count(nout, rp, ci, v, cnt.p, bad.p, s);
+    int32_t hbad = 0;
+    MGX_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    if (hbad) return false;
+    nt.rp.alloc((size_t)nout + 1);
+    exclusive_scan_i32(cnt.p, nt.rp.p, nout, s);
+    int32_t nnz = 0;
+    MGX_HIP(hipMemcpyAsync(&nnz, nt.rp.p + nout, sizeof(int32_t), hipMemcpyDeviceToHost, s));
+    MGX_HIP(hipStreamSynchronize(s));
+    nt.ci.alloc((size_t)nnz);
+    const void *w;
+    if (fp32) {
+        nt.w32.alloc((size_t)nnz);
+        nt_fill<float>(nout, rp, ci, v, nt.rp.p, nt.ci.p, nt.w32.p, s);
+        w = nt.w32.p;
+    } else {
+        nt.w64.alloc((size_t)nnz);
+        nt_fill<double>(nout, rp, ci, v, nt.rp.p, nt.ci.p, nt.w64.p, s);
+        w = nt.w64.p;
+    }
+    MGX_HIP(hipStreamSynchronize(s));
+    nt.fp32 = fp32;
+    nt.view.nout = nout;
+    nt.view.nin = ncols / 3;
+    nt.view.nnz = nnz;
+    nt.view.rp = nt.rp.p;
+    nt.view.ci = nt.ci.p;
+    nt.view.w = w;
+    return true;
+}
+
 static void download_csr(const DevCsr &A, std::vector<int32_t> &rp, std::vector<int32_t> &ci, std::vector<double> &v,
                          hipStream_t s)
 {
@@ -230,6 +289,10 @@ using namespace mgx;
 
 struct mgx_nodeblock_s {
     mgx::DevNb nb;
+};
+
+struct mgx_nodetransfer_s {
+    mgx::DevNt nt;
 };
 
 // small fill / combine kernels used by the drivers
@@ -305,6 +368,8 @@ struct Level {
     DevNb nbg;                             // node-block copy for block Gauss-Seidel (colour order, own block first)
     DevNb nb64;                            // precision form, level 0: FP64 node-block copy for the true residual
     bool nb_unfit = false;                 // the level matrix is not a node-block matrix the kernels take
+    DevNt Rn, Pn;                          // node transfers of R and P (block V-cycle)
+    bool nt_unfit = false;                 // R or P does not act on the dofs of a node alike
     std::vector<int32_t> color_off;        // colour c = positions color_off[c] .. color_off[c+1]-1 of nbg
     std::vector<DBuf<int32_t>> color_nodes;
     std::vector<int32_t> color_count;
@@ -355,7 +420,7 @@ struct mgx_solver_s {
     int64_t bgraph_key = -1;
     double bprof_ms = 0;                   // options.profile: level-0 SpMM time of the current block
     int bprof_n = 0;
-    std::vector<hipEvent_t> gs_events;     // options.profile, batching form: pairs around level-0 GS colour launches
+    std::vector<hipEvent_t> gs_events;     // options.profile, batching form: pairs around level-0 GS sweeps
     int gs_used = 0;
     bool prof_gs = false;                  // record GS events (inside a profiled block PCG iteration)
     double bgs_ms = 0;                     // their summed time in the current block
@@ -1045,14 +1110,22 @@ static bool nb_ready(const DevNb &nb)
 }
 
 template <typename T>
+static bool nt_ready(const DevNt &nt)
+{
+    return nt.built() && nt.fp32 == std::is_same<T, float>::value;
+}
+
+template <typename T>
 static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, bool backward, const int32_t *done)
 {
     const T *Av = BlkSel<T>::val(V.A);
     const bool nbk = nb_ready<T>(V.nbg);
+    // profiled level-0 sweeps: one event pair around the whole sweep (its colour launches run
+    // back to back, as in an unprofiled solve)
+    const bool rec = S->prof_gs && &V == &S->lv[0] && (size_t)(2 * S->gs_used + 2) <= S->gs_events.size();
+    if (rec) MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used], S->stream));
     for (int i = 0; i < V.ncolors; ++i) {
         const int c = backward ? V.ncolors - 1 - i : i;
-        const bool rec = S->prof_gs && &V == &S->lv[0] && (size_t)(2 * S->gs_used + 2) <= S->gs_events.size();
-        if (rec) MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used], S->stream));
         if (nbk)
             launch_nb_gs<T>(K, V.nbg.view, V.color_off[c], V.color_off[c + 1], b, x, backward, (i & 1) != 0, done,
                             S->stream);
@@ -1062,8 +1135,8 @@ static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, b
         else
             launch_gs_block<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done,
                                S->stream);
-        if (rec) { MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used + 1], S->stream)); ++S->gs_used; }
     }
+    if (rec) { MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used + 1], S->stream)); ++S->gs_used; }
 }
 
 // Y (op)= A X with the level matrix: node-block kernel when the level has its copy, else the
@@ -1125,11 +1198,15 @@ static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const 
     }
     level_spmm<T, T>(V, K, BEPI_RESID, Av, cur, b, nullptr, B::r(V), done, nullptr, s);
     Level &C = S->lv[l + 1];
-    launch_spmm<T, T>(K, BEPI_SPMM, V.R.nrows, V.R.rp, V.R.ci, B::val(V.R), B::r(V), nullptr, nullptr, B::b(C), done,
-                      nullptr, 0, s);
+    if (nt_ready<T>(V.Rn)) launch_nt_apply<T>(K, false, V.Rn.view, B::r(V), B::b(C), done, s);
+    else
+        launch_spmm<T, T>(K, BEPI_SPMM, V.R.nrows, V.R.rp, V.R.ci, B::val(V.R), B::r(V), nullptr, nullptr, B::b(C),
+                          done, nullptr, 0, s);
     vcycle_block<T>(S, K, l + 1, B::b(C), B::x(C), done);
-    launch_spmm<T, T>(K, BEPI_ADD, V.P.nrows, V.P.rp, V.P.ci, B::val(V.P), B::x(C), nullptr, nullptr, cur, done,
-                      nullptr, 0, s);
+    if (nt_ready<T>(V.Pn)) launch_nt_apply<T>(K, true, V.Pn.view, B::x(C), cur, done, s);
+    else
+        launch_spmm<T, T>(K, BEPI_ADD, V.P.nrows, V.P.rp, V.P.ci, B::val(V.P), B::x(C), nullptr, nullptr, cur, done,
+                          nullptr, 0, s);
     if (S->opt.smoother == MGX_SMOOTHER_GS) {
         for (int i = 0; i < nu; ++i) gs_block_sweep<T>(S, V, K, b, x, true, done);
     } else {
@@ -1219,16 +1296,17 @@ static void block_loop(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P, T *Q
         }
         S->bprof_n += real;
         S->prof_used = prof0 + real;
-        // level-0 GS launches of the real iterations (a fixed number per V-cycle)
+        // level-0 GS sweeps of the real iterations (a fixed number per V-cycle), counted in
+        // colour launches
         const Level &V0 = S->lv[0];
-        const int per = S->opt.smoother == MGX_SMOOTHER_GS ? 2 * S->opt.level_iterations * V0.ncolors : 0;
+        const int per = S->opt.smoother == MGX_SMOOTHER_GS ? 2 * S->opt.level_iterations : 0;
         const int rg = std::min(S->gs_used - gs0, per * real);
         for (int i = gs0; i < gs0 + rg; ++i) {
             float ms = 0;
             MGX_HIP(hipEventElapsedTime(&ms, S->gs_events[2 * i], S->gs_events[2 * i + 1]));
             S->bgs_ms += ms;
         }
-        S->bgs_n += rg;
+        S->bgs_n += rg * V0.ncolors;
         S->gs_used = gs0 + rg;
     }
 }
@@ -1404,6 +1482,12 @@ static void order_for_cache(mgx_solver_s *S, int K)
         Level &V = S->lv[l];
         V.tile_order.reset();
         V.tile_order_npb = 0;
+        // node transfers (independent of K): built once
+        if (!V.Rn.built() && !V.nt_unfit && V.R.v && V.P.v) {
+            const bool ok = build_nt(V.Rn, V.R.nrows, V.R.ncols, V.R.rp, V.R.ci, V.R.v, mixed, st) &&
+                            build_nt(V.Pn, V.P.nrows, V.P.ncols, V.P.rp, V.P.ci, V.P.v, mixed, st);
+            if (!ok) { V.Rn.reset(); V.Pn.reset(); V.nt_unfit = true; }
+        }
         if (!V.node3) continue;
         // node-block copy in node order (independent of K): built once
         if (!V.nb.built() && !V.nb_unfit)
@@ -1520,7 +1604,7 @@ static double time_block_iteration(mgx_solver_s *S, int K)
 }
 
 // X windows of the strip order tried by the autotuner (0: no strips, whole planes)
-static const std::vector<double> kStripCandidates = {3.0e6, 1.5e6, 0.75e6, 0.0};
+static const std::vector<double> kStripCandidates = {6.0e6, 3.0e6, 1.5e6, 0.0};
 
 // block width: fixed by the options, or the K in {4, 8, 16, 32} (not above the next power of
 // two of nrhs) with the least time per column-iteration (a wider K must be 3 % faster per
@@ -1809,6 +1893,35 @@ int mgx_nodeblock_destroy(mgx_nodeblock A)
     return guard([&] { delete A; });
 }
 
+int mgx_nodetransfer_from_csr(int32_t nrows, int32_t ncols, const int32_t *rowptr, const int32_t *col,
+                              const double *val, int32_t fp32, mgx_nodetransfer *out)
+{
+    return guard([&] {
+        MGX_CHECK_ROWS(nrows, rowptr);
+        MGX_REQUIRE(col && val && out && ncols > 0, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        std::unique_ptr<mgx_nodetransfer_s> m(new mgx_nodetransfer_s());
+        MGX_REQUIRE(build_nt(m->nt, nrows, ncols, rowptr, col, val, fp32 != 0, nullptr), MGX_ERR_INVALID_ARGUMENT,
+                    "not a node transfer (node triples of rows and columns, one weight per node pair)");
+        *out = m.release();
+    });
+}
+
+int mgx_nodetransfer_apply(mgx_nodetransfer M, int32_t k, const void *X, void *Y, int32_t add, void *stream)
+{
+    return guard([&] {
+        MGX_REQUIRE(M && X && Y, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        MGX_REQUIRE(k == 4 || k == 8 || k == 16 || k == 32, MGX_ERR_INVALID_ARGUMENT, "k must be 4, 8, 16 or 32");
+        hipStream_t s = (hipStream_t)stream;
+        if (M->nt.fp32) launch_nt_apply<float>(k, add != 0, M->nt.view, (const float *)X, (float *)Y, nullptr, s);
+        else launch_nt_apply<double>(k, add != 0, M->nt.view, (const double *)X, (double *)Y, nullptr, s);
+    });
+}
+
+int mgx_nodetransfer_destroy(mgx_nodetransfer M)
+{
+    return guard([&] { delete M; });
+}
+
 // ------------------------------------------------------------------ matrices
 static void assemble_into(DevCsr &A, int nx, int ny, int nz, const double *Ke, const double *dens_dev,
                           const double *Kel_dev, hipStream_t s)
@@ -2009,7 +2122,7 @@ int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
             // timing-only events: no system-scope fence (its L2 write-back and invalidate would
             // lengthen the bracketed launch and cool the cache of the next one)
             for (auto &e : s->prof_events) MGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
-            s->gs_events.resize(2 * 16 * 64);     // 64 V-cycles of 16 colour launches
+            s->gs_events.resize(2 * 2 * 256);     // 256 V-cycles of 2 level-0 sweeps
             for (auto &e : s->gs_events) MGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         }
         *out = s.release();
@@ -2118,6 +2231,20 @@ int mgx_solver_level_layout(mgx_solver s, int32_t level, int32_t *layout)
         if (s->bk <= 1) *layout = MGX_LAYOUT_CSR;
         else if (mixed ? nb_ready<float>(V.nb) : nb_ready<double>(V.nb)) *layout = MGX_LAYOUT_NODEBLOCK;
         else *layout = V.node3 ? MGX_LAYOUT_NODE_TRIPLE : MGX_LAYOUT_CSR;
+    });
+}
+
+int mgx_solver_transfer_layout(mgx_solver s, int32_t level, int32_t *layout)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && layout, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(s->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
+        MGX_REQUIRE(level >= 0 && level < (int)s->lv.size(), MGX_ERR_INVALID_ARGUMENT, "level out of range");
+        const Level &V = s->lv[level];
+        const bool mixed = s->opt.precision == MGX_PRECISION_MIXED;
+        const bool nt = mixed ? nt_ready<float>(V.Rn) && nt_ready<float>(V.Pn)
+                              : nt_ready<double>(V.Rn) && nt_ready<double>(V.Pn);
+        *layout = (s->bk > 1 && nt) ? MGX_LAYOUT_NODE_TRANSFER : MGX_LAYOUT_CSR;
     });
 }
 
@@ -2344,6 +2471,55 @@ int mgx_time_copy_kernel(const void *src, void *dst, uint64_t bytes, int32_t war
         MGX_HIP(hipEventRecord(e0, s));
         for (int i = 0; i < repeats; ++i)
             hipLaunchKernelGGL(k_copy16, dim3(grid), dim3(kBlock), 0, s, (const uint4 *)src, (uint4 *)dst, n);
+        MGX_HIP(hipEventRecord(e1, s));
+        MGX_HIP(hipGetLastError());
+        MGX_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        MGX_HIP(hipEventElapsedTime(&ms, e0, e1));
+        *avg_ms = ms / repeats;
+        (void)hipEventDestroy(e0); (void)hipEventDestroy(e1);
+        (void)hipStreamDestroy(s);
+    });
+}
+
+// streaming read of n 16-byte words, 4 loads in flight per lane; the XOR of everything read goes
+// to sink only when it equals a value no input of the timing produces, so the loads stay live
+// and nothing is written
+typedef unsigned int rd_u32x4 __attribute__((ext_vector_type(4)));
+__global__ __launch_bounds__(kBlock) void k_read16(const rd_u32x4 *__restrict__ src, int64_t n,
+                                                   rd_u32x4 *__restrict__ sink)
+{
+    rd_u32x4 acc = {0, 0, 0, 0};
+    const int64_t step = (int64_t)gridDim.x * kBlock;
+    int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x;
+    for (; i + 3 * step < n; i += 4 * step) {
+        rd_u32x4 v[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) v[u] = __builtin_nontemporal_load(src + i + u * step);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc ^= v[u];
+    }
+    for (; i < n; i += step) acc ^= src[i];
+    if (acc.x == 0x9e3779b9u && acc.y == 0x7f4a7c15u && acc.z == 0xf39cc060u && acc.w == 0x5cedc834u) *sink = acc;
+}
+
+int mgx_time_read_kernel(const void *src, uint64_t bytes, void *sink, int32_t warmup, int32_t repeats, double *avg_ms)
+{
+    return guard([&] {
+        MGX_REQUIRE(src && sink && avg_ms && bytes % 16 == 0 && repeats >= 1 && warmup >= 0,
+                    MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        hipStream_t s;
+        MGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+        hipEvent_t e0, e1;
+        MGX_HIP(hipEventCreate(&e0));
+        MGX_HIP(hipEventCreate(&e1));
+        const int64_t n = (int64_t)(bytes / 16);
+        const int grid = 256 * 8;
+        for (int i = 0; i < warmup; ++i)
+            hipLaunchKernelGGL(k_read16, dim3(grid), dim3(kBlock), 0, s, (const rd_u32x4 *)src, n, (rd_u32x4 *)sink);
+        MGX_HIP(hipEventRecord(e0, s));
+        for (int i = 0; i < repeats; ++i)
+            hipLaunchKernelGGL(k_read16, dim3(grid), dim3(kBlock), 0, s, (const rd_u32x4 *)src, n, (rd_u32x4 *)sink);
         MGX_HIP(hipEventRecord(e1, s));
         MGX_HIP(hipGetLastError());
         MGX_HIP(hipEventSynchronize(e1));

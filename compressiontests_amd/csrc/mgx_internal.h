@@ -192,6 +192,21 @@ template <typename TI, typename TV>
 void nb_fill(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, const TI *v, const int32_t *nrp,
              bool diag_first, int32_t *ncol, TV *val, hipStream_t s);
 
+// node transfers: output node i, input node j, one weight for the three dofs of the node
+struct NtView {
+    int32_t nout = 0, nin = 0;       // output / input nodes
+    int64_t nnz = 0;
+    const int32_t *rp = nullptr, *ci = nullptr;
+    const void *w = nullptr;         // float or double [nnz]
+};
+template <typename T>
+void launch_nt_apply(int K, bool add, const NtView &M, const T *X, T *Y, const int32_t *done, hipStream_t s);
+void nt_count(int32_t nout, const int32_t *rp, const int32_t *ci, const double *v, int32_t *cnt, int32_t *bad,
+              hipStream_t s);
+template <typename TV>
+void nt_fill(int32_t nout, const int32_t *rp, const int32_t *ci, const double *v, const int32_t *nrp, int32_t *ncol,
+             TV *w, hipStream_t s);
+
 // structured assembly
 void launch_assemble_count(int nx, int ny, int nz, int32_t *rowlen, hipStream_t s);
 void launch_assemble_fill(int nx, int ny, int nz, const double *Ke, const double *dens, const double *Kel,

@@ -370,6 +370,133 @@ template void launch_nb_gs<float>(int, const NbView &, int32_t, int32_t, const f
 template void launch_nb_gs<double>(int, const NbView &, int32_t, int32_t, const double *, double *, bool, bool,
                                    const int32_t *, hipStream_t);
 
+// ------------------------------------------------------------------ node transfers
+// A transfer matrix (prolongation P or restriction R) between two levels whose dofs come in node
+// triples, when it acts on the three dofs of a node alike: entry (3i+a, 3j+b) = w_ij if a == b,
+// else 0.  Stored per node: rows i of the output level, columns j of the input level, one weight
+// each (a third of the CSR's entries, and no dof column indices).  Y[3i+a][:] (+)= sum_j w_ij
+// X[3j+a][:]: the 3K values of a node are contiguous in the row-major n x K blocks, so thread
+// (i, q) owns the 16-byte vector q of output node i and reads vector q of each input node.
+template <typename T, int K, bool ADD>
+__global__ __launch_bounds__(kBlock) void k_nt_apply(int32_t nout, const int32_t *__restrict__ rp,
+                                                     const int32_t *__restrict__ ci, const T *__restrict__ w,
+                                                     const T *__restrict__ X, T *__restrict__ Y,
+                                                     const int32_t *__restrict__ done)
+{
+    using VT = typename NbVec<T>::type;
+    constexpr int W = 16 / (int)sizeof(T);
+    constexpr int Q = 3 * K / W;                                 // vectors per node
+    if (done && *done) return;
+    const int64_t t = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int64_t i = t / Q;
+    if (i >= nout) return;
+    const int q = (int)(t - i * Q);
+    const int b1 = rp[i + 1];
+    int b = rp[i];
+    VT acc = VT(0);
+    for (; b + 3 < b1; b += 4) {
+        int j[4];
+        T wv[4];
+        VT x[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) { j[u] = ci[b + u]; wv[u] = w[b + u]; }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) x[u] = *(const VT *)(X + (size_t)j[u] * (3 * K) + q * W);
+#pragma unroll
+        for (int u = 0; u < 4; ++u) acc += wv[u] * x[u];
+    }
+    for (; b < b1; ++b) acc += w[b] * *(const VT *)(X + (size_t)ci[b] * (3 * K) + q * W);
+    VT *yp = (VT *)(Y + (size_t)i * (3 * K) + q * W);
+    if (ADD) *yp = *yp + acc;
+    else *yp = acc;
+}
+
+template <typename T>
+void launch_nt_apply(int K, bool add, const NtView &M, const T *X, T *Y, const int32_t *done, hipStream_t s)
+{
+    if (M.nout <= 0) return;
+    const int W = 16 / (int)sizeof(T);
+    const int g = grid_rows((int64_t)M.nout * (3 * K / W), kBlock);
+#define MGX_NTA(KK, AA)                                                                                          \
+    hipLaunchKernelGGL((k_nt_apply<T, KK, AA>), dim3(g), dim3(kBlock), 0, s, M.nout, M.rp, M.ci, (const T *)M.w, \
+                       X, Y, done)
+#define MGX_NTA_K(KK)                    \
+    if (add) MGX_NTA(KK, true);          \
+    else MGX_NTA(KK, false)
+    switch (K) {
+    case 4: MGX_NTA_K(4); break;
+    case 8: MGX_NTA_K(8); break;
+    case 16: MGX_NTA_K(16); break;
+    case 32: MGX_NTA_K(32); break;
+    default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
+    }
+#undef MGX_NTA_K
+#undef MGX_NTA
+    MGX_HIP(hipGetLastError());
+}
+template void launch_nt_apply<float>(int, bool, const NtView &, const float *, float *, const int32_t *, hipStream_t);
+template void launch_nt_apply<double>(int, bool, const NtView &, const double *, double *, const int32_t *,
+                                      hipStream_t);
+
+// Entries per output node, and the check that the CSR is a node transfer: the three rows of node
+// i have one length, row 3i+a holds only columns 3j+a, and the three rows list the same input
+// nodes j in the same order with bit-identical values.  *bad = 1 on any violation.
+__global__ __launch_bounds__(kBlock) void k_nt_count(int32_t nout, const int32_t *__restrict__ rp,
+                                                     const int32_t *__restrict__ ci, const double *__restrict__ v,
+                                                     int32_t *__restrict__ cnt, int32_t *__restrict__ bad)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nout) return;
+    const int r0 = rp[3 * i], len = rp[3 * i + 1] - r0;
+    bool ok = rp[3 * i + 2] - rp[3 * i + 1] == len && rp[3 * i + 3] - rp[3 * i + 2] == len;
+    for (int a = 0; ok && a < 3; ++a) {
+        const int ra = r0 + a * len;
+        for (int m = 0; ok && m < len; ++m) {
+            const int c = ci[ra + m];
+            ok = c % 3 == a && c / 3 == ci[r0 + m] / 3 && v[ra + m] == v[r0 + m];
+        }
+    }
+    if (!ok) atomicOr(bad, 1);
+    cnt[i] = ok ? len : 0;
+}
+
+template <typename TV>
+__global__ __launch_bounds__(kBlock) void k_nt_fill(int32_t nout, const int32_t *__restrict__ rp,
+                                                    const int32_t *__restrict__ ci, const double *__restrict__ v,
+                                                    const int32_t *__restrict__ nrp, int32_t *__restrict__ ncol,
+                                                    TV *__restrict__ w)
+{
+    const int i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= nout) return;
+    const int r0 = rp[3 * i], o = nrp[i], len = nrp[i + 1] - o;
+    for (int m = 0; m < len; ++m) {
+        ncol[o + m] = ci[r0 + m] / 3;
+        w[o + m] = (TV)v[r0 + m];
+    }
+}
+
+void nt_count(int32_t nout, const int32_t *rp, const int32_t *ci, const double *v, int32_t *cnt, int32_t *bad,
+              hipStream_t s)
+{
+    if (nout <= 0) return;
+    hipLaunchKernelGGL(k_nt_count, dim3((nout + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nout, rp, ci, v, cnt, bad);
+    MGX_HIP(hipGetLastError());
+}
+
+template <typename TV>
+void nt_fill(int32_t nout, const int32_t *rp, const int32_t *ci, const double *v, const int32_t *nrp, int32_t *ncol,
+             TV *w, hipStream_t s)
+{
+    if (nout <= 0) return;
+    hipLaunchKernelGGL((k_nt_fill<TV>), dim3((nout + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nout, rp, ci, v, nrp,
+                       ncol, w);
+    MGX_HIP(hipGetLastError());
+}
+template void nt_fill<float>(int32_t, const int32_t *, const int32_t *, const double *, const int32_t *, int32_t *,
+                             float *, hipStream_t);
+template void nt_fill<double>(int32_t, const int32_t *, const int32_t *, const double *, const int32_t *, int32_t *,
+                              double *, hipStream_t);
+
 // ------------------------------------------------------------------ building a copy from CSR
 // Blocks per position, and the checks that make the CSR a node-block matrix: the three rows of
 // the node have one column list, its length a multiple of 3, and every column triple is a whole

@@ -6,7 +6,8 @@ the resulting GB/s against the 8 TB/s HBM peak.
   python -m compressiontests_amd.kernel_bench --n 64 --kinds spmv,jacobi,residual,spmv_f32,copy
 
 `copy` is a plain device-to-device copy kernel of known size, the calibration for rocprofv3's
-FETCH_SIZE / WRITE_SIZE counters.  `spmm64_k32` / `spmm32_k32` (any k in 4, 8, 16, 32): the
+FETCH_SIZE / WRITE_SIZE counters; `read` a streaming read of the same byte count (16-byte
+non-temporal loads, nothing written), the practical HBM read ceiling.  `spmm64_k32` / `spmm32_k32` (any k in 4, 8, 16, 32): the
 block SpMM Y = A X of the batching form, FP64 or FP32 values and vectors (suffix `_node`: the
 node-triple CSR kernel; `_nb`: the node-block kernel the block solver uses on the level
 matrices, on a 3x3 block copy built from the CSR; suffix `_dot`: the block-PCG form q = A p with the
@@ -99,6 +100,14 @@ def main():
             dst = DeviceArray(nnz)
             res = C.c_double()
             L.call("mgx_time_copy_kernel", src.vp, dst.vp, C.c_uint64(8 * nnz), args.warmup, args.repeats, C.byref(res))
+            ms = res.value
+            by = 16 * nnz
+        elif kind == "read":
+            src = DeviceArray(2 * nnz)
+            sink = DeviceArray(2)
+            res = C.c_double()
+            L.call("mgx_time_read_kernel", src.vp, C.c_uint64(16 * nnz), sink.vp, args.warmup, args.repeats,
+                   C.byref(res))
             ms = res.value
             by = 16 * nnz
         else:

@@ -50,6 +50,35 @@ class NodeBlockMatrix:
             pass
 
 
+class NodeTransfer:
+    """Node-transfer copy of a device CSR prolongation or restriction that acts on the three dofs
+    of a node alike (one weight per output/input node pair): the layout of the batching form's
+    R and P (FP32 weights if fp32)."""
+
+    def __init__(self, M, fp32=False):
+        h = C.c_void_p()
+        L.call("mgx_nodetransfer_from_csr", M.nrows, M.ncols, C.c_void_p(M.rowptr), C.c_void_p(M.col),
+               C.c_void_p(M.val), int(fp32), C.byref(h))
+        self.handle, self.fp32, self.nrows, self.ncols = h.value, fp32, M.nrows, M.ncols
+
+    def apply(self, X, Y, k, add=False, stream=None):
+        """Y = M X or (add) Y += M X; X (ncols x k), Y (nrows x k) row-major, float32 for an FP32 copy."""
+        assert X.dtype == (np.float32 if self.fp32 else np.float64) and Y.dtype == X.dtype
+        assert X.n == self.ncols * k and Y.n == self.nrows * k
+        L.call("mgx_nodetransfer_apply", C.c_void_p(self.handle), k, X.vp, Y.vp, int(add), stream)
+
+    def free(self):
+        if self.handle is not None:
+            L.lib().mgx_nodetransfer_destroy(C.c_void_p(self.handle))
+            self.handle = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
 def residual(A, b, x, r, stream=None):
     """r = b - A x.  CsrGeometricMultigrid.cl:96-110."""
     L.call("mgx_csr_residual", A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), b.vp, x.vp, r.vp,

@@ -135,6 +135,11 @@ class _Handle:
         L.call("mgx_solver_level_layout", C.c_void_p(self.h), level, C.byref(v))
         return v.value
 
+    def transfer_layout(self, level):
+        v = C.c_int32()
+        L.call("mgx_solver_transfer_layout", C.c_void_p(self.h), level, C.byref(v))
+        return v.value
+
     def strip_window(self):
         b = C.c_double()
         ms = (C.c_double * 4)()
@@ -371,10 +376,14 @@ class HipMgPcgSolver:
         """Layout the block kernels read on a level: "csr", "node-triple" or "node-block"."""
         return ("csr", "node-triple", "node-block")[self._h.level_layout(level)]
 
+    def TransferLayout(self, level=0):
+        """Layout of the level's R and P in the block V-cycle: "csr" or "node-transfer"."""
+        return {0: "csr", 3: "node-transfer"}[self._h.transfer_layout(level)]
+
     @property
     def StripWindow(self):
         """(X window in bytes of the block kernels' strip order, 0 = whole planes; ms per block PCG
-        iteration the autotuner timed for the windows 3e6, 1.5e6, 0.75e6 and 0)"""
+        iteration the autotuner timed for the windows 6e6, 3e6, 1.5e6 and 0)"""
         return self._h.strip_window()
 
     def GsOrder(self, level=0):

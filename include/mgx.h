@@ -146,6 +146,21 @@ int mgx_nodeblock_from_csr(int32_t nrows, const int32_t *rowptr, const int32_t *
 int mgx_nodeblock_spmm(mgx_nodeblock A, int32_t k, const void *X, void *Y, int32_t clear, void *stream);
 int mgx_nodeblock_destroy(mgx_nodeblock A);
 
+/* Node transfer: a prolongation or restriction between two levels whose dofs come in node triples,
+ * acting on the three dofs of a node alike (entry (3i+a, 3j+b) = w_ij when a == b, else 0), stored
+ * as one weight per (output node, input node) pair.  It replaces the CSR R / P products of the
+ * V-cycle in the batching form (the reference applies R and P as CSR SpMVs:
+ * CsrGeometricMultigrid.cl, driven from OpenCLCsrGeometricMultigridSolver.cs:334-536).  Built
+ * on the device from a device CSR with FP64 values (nrows x ncols, both multiples of 3); fp32 = 1
+ * stores FP32 weights.  MGX_ERR_INVALID_ARGUMENT if the matrix is not of that form. */
+typedef struct mgx_nodetransfer_s *mgx_nodetransfer;
+int mgx_nodetransfer_from_csr(int32_t nrows, int32_t ncols, const int32_t *rowptr, const int32_t *col,
+                              const double *val, int32_t fp32, mgx_nodetransfer *out);
+/* Y = M X (add = 0) or Y += M X (add = 1); X: ncols x k and Y: nrows x k row-major device blocks of
+ * the copy's type, k in {4, 8, 16, 32}. */
+int mgx_nodetransfer_apply(mgx_nodetransfer M, int32_t k, const void *X, void *Y, int32_t add, void *stream);
+int mgx_nodetransfer_destroy(mgx_nodetransfer M);
+
 /* SIMP compliance sensitivity, one thread per element (no reference counterpart; the hook
  * is ElementDensities x Ke, FemCantileverBase.cs:99-100):
  *   grad_e = -penal * rho_e^(penal-1) * (E0 - Emin) * u_e^T Ke0 u_e
@@ -238,7 +253,7 @@ typedef struct {
                                    PCG iterations of each width on the first right-hand sides (default) */
     double  strip_window;       /* batching form on structured levels: X window (bytes) of the strip order in
                                    which the block kernels visit the nodes, 0 = whole planes, < 0 = chosen at
-                                   initialize by timing 3e6, 1.5e6, 0.75e6 and 0 (default) */
+                                   initialize by timing 6e6, 3e6, 1.5e6 and 0 (default) */
 } mgx_options;
 
 typedef struct {
@@ -254,7 +269,7 @@ typedef struct {
     int32_t block_width;     /* right-hand sides solved together with this one (1: alone); with
                                 block_width > 1, solve_ms is the device time of that whole block */
     int32_t gs_launches;     /* batching form with options.profile: level-0 Gauss-Seidel colour launches timed */
-    double  gs_ms;           /* their summed device time (HIP events on the solver stream) */
+    double  gs_ms;           /* their summed device time (HIP events on the solver stream around each sweep) */
 } mgx_stats;
 
 typedef struct mgx_solver_s *mgx_solver;
@@ -296,9 +311,13 @@ int mgx_solver_block_width(mgx_solver s, int32_t *width, double *tune_ms);
 #define MGX_LAYOUT_NODE_TRIPLE  1   /* CSR read by node triples (rows 3q..3q+2 share their columns) */
 #define MGX_LAYOUT_NODEBLOCK    2   /* 3x3 node blocks (mgx_nodeblock_from_csr) */
 int mgx_solver_level_layout(mgx_solver s, int32_t level, int32_t *layout);
+/* Layout of the level's R and P in the batching form's V-cycle: MGX_LAYOUT_NODE_TRANSFER when both
+ * are node transfers (mgx_nodetransfer_from_csr), else MGX_LAYOUT_CSR.  No reference counterpart. */
+#define MGX_LAYOUT_NODE_TRANSFER 3
+int mgx_solver_transfer_layout(mgx_solver s, int32_t level, int32_t *layout);
 /* X window (bytes) of the strip order in which the block kernels visit the nodes of structured
  * levels (0: whole planes) and, when the autotuner chose it, the ms per block PCG iteration it timed
- * for the windows 3e6, 1.5e6, 0.75e6 and 0 (tune_ms: 4 entries, may be NULL).  No reference
+ * for the windows 6e6, 3e6, 1.5e6 and 0 (tune_ms: 4 entries, may be NULL).  No reference
  * counterpart (batching form). */
 int mgx_solver_strip_window(mgx_solver s, double *bytes, double *tune_ms);
 /* Colours of the multicolour Gauss-Seidel node colouring of a level (0 without that smoother). */
@@ -341,6 +360,10 @@ int mgx_time_spmm(int32_t flags, int32_t k, int32_t nrows, const int32_t *rowptr
 /* Average device time of a plain copy kernel (16-byte vector loads and stores) moving `bytes`
  * (a multiple of 16) from src to dst: the calibration of HBM byte counters. */
 int mgx_time_copy_kernel(const void *src, void *dst, uint64_t bytes, int32_t warmup, int32_t repeats, double *avg_ms);
+/* Average device time of a streaming read of `bytes` (a multiple of 16) from src (16-byte
+ * non-temporal loads, nothing written; sink: 16 bytes of device memory the kernel may write): the
+ * practical HBM read ceiling that the gather kernels are compared with.  No reference counterpart. */
+int mgx_time_read_kernel(const void *src, uint64_t bytes, void *sink, int32_t warmup, int32_t repeats, double *avg_ms);
 /* Same for the DuVi SpMV (kind 0 only). */
 int mgx_time_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t warmup, int32_t repeats, double *avg_ms);
 

@@ -141,6 +141,53 @@ def test_nodeblock_rejects_other_matrices(het16):
         kernels.NodeBlockMatrix(DeviceCsr.arrays_from_host(big))
 
 
+@pytest.mark.parametrize("k", [4, 8, 16, 32])
+def test_nodetransfer_matches_columnwise_products(het16, k):
+    """R and P of both transfer levels of the 16^3 hierarchy as node transfers (one weight per
+    node pair) against the FP64 CPU product of every column: FP64 to 1e-13 of max|y|, FP32 weights
+    and vectors to 1e-6; Y += M X adds the same product once more."""
+    from compressiontests_amd import kernels
+    from compressiontests_amd.device import DeviceArray, DeviceCsr
+    m, F, lv, A = het16
+    for M in (lv[0]["P"], lv[0]["R"], lv[1]["P"], lv[1]["R"]):
+        xh = O.rng_normal(31 + k, M.ncols * k).reshape(M.ncols, k)
+        ref = S(M) @ xh
+        scale = max(np.max(np.abs(ref)), 1e-300)
+        Dm = DeviceCsr.arrays_from_host(M)
+        for fp32, tol in ((False, 1e-13), (True, 1e-6)):
+            dt = np.float32 if fp32 else np.float64
+            nt = kernels.NodeTransfer(Dm, fp32=fp32)
+            X = DeviceArray.from_numpy(xh.reshape(-1).astype(dt))
+            Y = DeviceArray(M.nrows * k, dt)
+            nt.apply(X, Y, k)
+            assert np.max(np.abs(Y.numpy().reshape(M.nrows, k) - ref)) <= tol * scale, (M.nrows, M.ncols, fp32)
+            nt.apply(X, Y, k, add=True)
+            assert np.max(np.abs(Y.numpy().reshape(M.nrows, k) - 2 * ref)) <= 2 * tol * scale, (M.nrows, fp32)
+            nt.free()
+
+
+def test_nodetransfer_rejects_other_matrices(het16):
+    """Matrices that couple the dofs of a node (the stiffness matrix), give the three dofs of a node
+    different weights, or do not come in node triples are not node transfers: the constructor
+    raises."""
+    from compressiontests_amd import kernels
+    from compressiontests_amd.device import DeviceCsr
+    from compressiontests_amd.device import CsrMatrix
+    m, F, lv, A = het16
+    with pytest.raises(L.MgxInvalidArgument):
+        kernels.NodeTransfer(DeviceCsr.arrays_from_host(A))
+    P = lv[0]["P"]
+    v = P.val.copy()
+    r = 3 * 5 + 1                                   # dof 1 of fine node 5: one weight changed
+    v[P.rowptr[r]] *= 1.5
+    with pytest.raises(L.MgxInvalidArgument):
+        kernels.NodeTransfer(DeviceCsr.arrays_from_host(CsrMatrix(P.rowptr, P.col, v, P.nrows, P.ncols)))
+    cut = P.rowptr[P.nrows - 1]                     # drop the last row: not whole node triples
+    short = CsrMatrix(P.rowptr[:P.nrows].copy(), P.col[:cut].copy(), P.val[:cut].copy(), P.nrows - 1, P.ncols)
+    with pytest.raises(L.MgxInvalidArgument):
+        kernels.NodeTransfer(DeviceCsr.arrays_from_host(short))
+
+
 def test_load_cases_shape_and_norm(het16):
     m, F, lv, A = het16
     assert F.shape == (6, m.NumDofsFree)
@@ -158,6 +205,7 @@ def test_block_fp64_matches_oracle_per_column(het16, smoother, sm, bw):
     s = HipMgPcgSolver(m, levels=3, smoother=smoother, omega=0.0, nu=2, tolerance=1e-10, nrhs=6, block_width=bw)
     s.Initialize(F)
     assert s.BlockWidth[0] == bw
+    assert [s.TransferLayout(l) for l in range(2)] == ["node-transfer"] * 2
     X, st = s.Solve()
     assert X.shape == F.shape
     if smoother == "gs":
@@ -200,6 +248,7 @@ def test_block_mixed_precision(het16):
     m, F, lv, A = het16
     s = HipMgPcgSolver(m, levels=3, smoother="gs", nu=2, tolerance=1e-10, nrhs=6, block_width=4, precision="mixed")
     s.Initialize(F)
+    assert [s.TransferLayout(l) for l in range(2)] == ["node-transfer"] * 2
     for _ in range(2):
         X, st = s.Solve()
         for j in range(6):
