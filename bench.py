@@ -244,10 +244,12 @@ def main():
     ncol = solver.LevelColors(0)
     if gs_n > 0 and ncol > 0:
         rooflines.append(roofline(f"k_nb_gs<{tname}, K={width}> (one colour of the level-0 node-block multicolour "
-                                  f"Gauss-Seidel; bytes averaged over the {ncol} colours of a sweep)",
+                                  f"Gauss-Seidel; bytes averaged over the {ncol} colours of a sweep; the first "
+                                  f"launch of a sweep from x = 0, which reads no neighbours, is a separate kernel "
+                                  f"and not counted)",
                                   gs_sweep_bytes(nnz, ndof, width, vb, ncol) / ncol, gs_ms, gs_n,
                                   f"gs_nodeblock_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed",
-                                  f"around each sweep of {ncol} back-to-back colour launches"))
+                                  "around the back-to-back colour launches of each sweep"))
     # the dominant kernel: the most device time in the profiled solves
     rooflines.sort(key=lambda r: -r["total_ms"])
 

@@ -142,24 +142,27 @@ struct DevDuvi {
 struct DevNb {
     NbView view;                  // view.npos == 0: not built
     bool fp32 = false;
-    DBuf<int32_t> nrp, ncol, node;
+    DBuf<int32_t> nrp, ncol, node, split;
     DBuf<double> v64;
     DBuf<float> v32;
     bool built() const { return view.npos > 0; }
     void reset()
     {
         view = NbView{};
-        nrp.reset(); ncol.reset(); node.reset(); v64.reset(); v32.reset();
+        nrp.reset(); ncol.reset(); node.reset(); split.reset(); v64.reset(); v32.reset();
     }
 };
 
 // Builds nb from the device CSR (rp, ci, FP64 or FP32 values v) of nrows = 3 * npos rows.
 // node_dev: npos node indices
-// giving the position order (nullptr: node order).  Returns false, with nothing built, if the
+// giving the position order (nullptr: node order).  colour_dev (with diag_first; may be nullptr):
+// the colour of every node, to order each position's other blocks by neighbour colour and record
+// the split.  Returns false, with nothing built, if the
 // matrix is not made of whole 3x3 node blocks or has a node of more than 27 blocks.
 template <typename TI>
 static bool build_nb(DevNb &nb, int32_t nrows, const int32_t *rp, const int32_t *ci, const TI *v,
-                     const int32_t *node_dev, bool diag_first, bool fp32, hipStream_t s)
+                     const int32_t *node_dev, bool diag_first, bool fp32, hipStream_t s,
+                     const int32_t *colour_dev = nullptr)
 {
     nb.reset();
     MGX_REQUIRE(nrows % 3 == 0 && nrows > 0, MGX_ERR_INVALID_ARGUMENT, "node blocks need 3 rows per node");
@@ -186,15 +189,20 @@ static bool build_nb(DevNb &nb, int32_t nrows, const int32_t *rp, const int32_t 
         nb.node.alloc(npos);
         MGX_HIP(hipMemcpyAsync(nb.node.p, node_dev, sizeof(int32_t) * npos, hipMemcpyDeviceToDevice, s));
     }
+    const bool bycol = colour_dev && diag_first;
+    if (bycol) nb.split.alloc(npos);
+    int32_t *split = bycol ? nb.split.p : nullptr;
     const void *val;
     if (fp32) {
         nb.v32.alloc((size_t)nblk * 9 + 4);
-        nb_fill<TI, float>(npos, node_dev, rp, ci, v, nb.nrp.p, diag_first, nb.ncol.p, nb.v32.p, s);
+        nb_fill<TI, float>(npos, node_dev, rp, ci, v, nb.nrp.p, diag_first, nb.ncol.p, nb.v32.p, s,
+                           bycol ? colour_dev : nullptr, split);
         val = nb.v32.p;
     } else {
         nb.v64.alloc((size_t)nblk * 9 + 2);
         if constexpr (std::is_same<TI, double>::value)
-            nb_fill<double, double>(npos, node_dev, rp, ci, v, nb.nrp.p, diag_first, nb.ncol.p, nb.v64.p, s);
+            nb_fill<double, double>(npos, node_dev, rp, ci, v, nb.nrp.p, diag_first, nb.ncol.p, nb.v64.p, s,
+                                    bycol ? colour_dev : nullptr, split);
         val = nb.v64.p;
     }
     MGX_HIP(hipStreamSynchronize(s));
@@ -204,6 +212,7 @@ static bool build_nb(DevNb &nb, int32_t nrows, const int32_t *rp, const int32_t 
     nb.view.nrp = nb.nrp.p;
     nb.view.ncol = nb.ncol.p;
     nb.view.node = node_dev ? nb.node.p : nullptr;
+    nb.view.split = split;
     nb.view.val = val;
     return true;
 }
@@ -421,6 +430,7 @@ struct mgx_solver_s {
     double bprof_ms = 0;                   // options.profile: level-0 SpMM time of the current block
     int bprof_n = 0;
     std::vector<hipEvent_t> gs_events;     // options.profile, batching form: pairs around level-0 GS sweeps
+    std::vector<int> gs_launches;          // colour launches between each pair
     int gs_used = 0;
     bool prof_gs = false;                  // record GS events (inside a profiled block PCG iteration)
     double bgs_ms = 0;                     // their summed time in the current block
@@ -1115,20 +1125,28 @@ static bool nt_ready(const DevNt &nt)
     return nt.built() && nt.fp32 == std::is_same<T, float>::value;
 }
 
+// start: NB_GS_GENERAL; NB_GS_ZERO (x cleared before a forward sweep: its first colour reads only
+// the nodes' own blocks); NB_GS_LOWER (x not cleared; a forward sweep that treats x as 0 on entry,
+// node-block copies with the colour split only)
 template <typename T>
-static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, bool backward, const int32_t *done)
+static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, bool backward, const int32_t *done,
+                           int start = NB_GS_GENERAL)
 {
     const T *Av = BlkSel<T>::val(V.A);
     const bool nbk = nb_ready<T>(V.nbg);
-    // profiled level-0 sweeps: one event pair around the whole sweep (its colour launches run
-    // back to back, as in an unprofiled solve)
-    const bool rec = S->prof_gs && &V == &S->lv[0] && (size_t)(2 * S->gs_used + 2) <= S->gs_events.size();
+    // profiled level-0 sweeps: one event pair around the colour launches of each general sweep
+    // (the general kernel only), which run back to back as in an unprofiled solve
+    const bool rec = start == NB_GS_GENERAL && S->prof_gs && &V == &S->lv[0] &&
+                     (size_t)(2 * S->gs_used + 2) <= S->gs_events.size();
     if (rec) MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used], S->stream));
     for (int i = 0; i < V.ncolors; ++i) {
         const int c = backward ? V.ncolors - 1 - i : i;
+        const int mode = start == NB_GS_GENERAL ? NB_GS_GENERAL
+                         : i == 0              ? NB_GS_ZERO
+                                               : (start == NB_GS_LOWER ? NB_GS_LOWER : NB_GS_GENERAL);
         if (nbk)
             launch_nb_gs<T>(K, V.nbg.view, V.color_off[c], V.color_off[c + 1], b, x, backward, (i & 1) != 0, done,
-                            S->stream);
+                            S->stream, mode);
         else if (V.node3)
             launch_gs_node<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done,
                               S->stream);
@@ -1136,7 +1154,10 @@ static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, b
             launch_gs_block<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done,
                                S->stream);
     }
-    if (rec) { MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used + 1], S->stream)); ++S->gs_used; }
+    if (rec) {
+        MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used + 1], S->stream));
+        S->gs_launches[S->gs_used++] = V.ncolors;
+    }
 }
 
 // Y (op)= A X with the level matrix: node-block kernel when the level has its copy, else the
@@ -1182,9 +1203,16 @@ static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const 
     const int nu = S->opt.level_iterations;
     const T *Av = B::val(V.A);
     T *cur;
-    if (S->opt.smoother == MGX_SMOOTHER_GS) {
+    // one forward sweep from x = 0 on a colour-sorted copy: the sweep reads the earlier colours
+    // only (x need not be cleared), and the residual is the pull of the later colours
+    const bool tri = S->opt.smoother == MGX_SMOOTHER_GS && nu == 1 && nb_ready<T>(V.nbg) && V.nbg.view.split;
+    if (tri) {
+        gs_block_sweep<T>(S, V, K, b, x, false, done, NB_GS_LOWER);
+        cur = x;
+    } else if (S->opt.smoother == MGX_SMOOTHER_GS) {
         fill_zero_t((int64_t)V.n * K, x, done, s);
-        for (int i = 0; i < nu; ++i) gs_block_sweep<T>(S, V, K, b, x, false, done);
+        for (int i = 0; i < nu; ++i)
+            gs_block_sweep<T>(S, V, K, b, x, false, done, i == 0 ? NB_GS_ZERO : NB_GS_GENERAL);
         cur = x;
     } else {
         T *tt = B::t(V);
@@ -1196,7 +1224,8 @@ static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const 
             cur = o;
         }
     }
-    level_spmm<T, T>(V, K, BEPI_RESID, Av, cur, b, nullptr, B::r(V), done, nullptr, s);
+    if (tri) launch_nb_spmm<T>(K, BEPI_UPRESID, V.nbg.view, cur, nullptr, nullptr, B::r(V), done, nullptr, 0, s);
+    else level_spmm<T, T>(V, K, BEPI_RESID, Av, cur, b, nullptr, B::r(V), done, nullptr, s);
     Level &C = S->lv[l + 1];
     if (nt_ready<T>(V.Rn)) launch_nt_apply<T>(K, false, V.Rn.view, B::r(V), B::b(C), done, s);
     else
@@ -1298,7 +1327,6 @@ static void block_loop(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P, T *Q
         S->prof_used = prof0 + real;
         // level-0 GS sweeps of the real iterations (a fixed number per V-cycle), counted in
         // colour launches
-        const Level &V0 = S->lv[0];
         const int per = S->opt.smoother == MGX_SMOOTHER_GS ? 2 * S->opt.level_iterations : 0;
         const int rg = std::min(S->gs_used - gs0, per * real);
         for (int i = gs0; i < gs0 + rg; ++i) {
@@ -1306,7 +1334,7 @@ static void block_loop(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P, T *Q
             MGX_HIP(hipEventElapsedTime(&ms, S->gs_events[2 * i], S->gs_events[2 * i + 1]));
             S->bgs_ms += ms;
         }
-        S->bgs_n += rg * V0.ncolors;
+        for (int i = gs0; i < gs0 + rg; ++i) S->bgs_n += S->gs_launches[i];
         S->gs_used = gs0 + rg;
     }
 }
@@ -1538,9 +1566,14 @@ static void order_for_cache(mgx_solver_s *S, int K)
         }
         V.nbg.reset();
         if (V.nb.built() && (int32_t)all.size() == V.n / 3) {
-            DBuf<int32_t> order;
+            DBuf<int32_t> order, colour;
             order.upload(all.data(), all.size(), st);
-            if (!build_nb<double>(V.nbg, V.n, V.A.rp, V.A.ci, V.A.v, order.p, true, mixed, st)) V.nbg.reset();
+            std::vector<int32_t> col(all.size());
+            for (int c = 0; c < V.ncolors; ++c)
+                for (int32_t i = V.color_off[c]; i < V.color_off[c + 1]; ++i) col[all[i]] = c;
+            colour.upload(col.data(), col.size(), st);
+            if (!build_nb<double>(V.nbg, V.n, V.A.rp, V.A.ci, V.A.v, order.p, true, mixed, st, colour.p))
+                V.nbg.reset();
         }
     }
 }
@@ -2123,6 +2156,7 @@ int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
             // lengthen the bracketed launch and cool the cache of the next one)
             for (auto &e : s->prof_events) MGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
             s->gs_events.resize(2 * 2 * 256);     // 256 V-cycles of 2 level-0 sweeps
+            s->gs_launches.assign(2 * 256, 0);
             for (auto &e : s->gs_events) MGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         }
         *out = s.release();

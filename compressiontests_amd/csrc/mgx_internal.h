@@ -122,7 +122,8 @@ struct BCtl {
     int32_t stop, max_it;
     double tol;
 };
-enum BlockEpi : int { BEPI_SPMM = 0, BEPI_ADD = 1, BEPI_RESID = 2, BEPI_JACOBI = 3, BEPI_DOT = 4 };
+enum BlockEpi : int { BEPI_SPMM = 0, BEPI_ADD = 1, BEPI_RESID = 2, BEPI_JACOBI = 3, BEPI_DOT = 4,
+                      BEPI_UPRESID = 5 };   // node-block only: residual after a zero-start forward sweep
 // partials of the block reductions: 3 slots x kMaxBlock columns x kPartialStride
 constexpr size_t kBlockPartials = (size_t)3 * kMaxBlock * kPartialStride;
 template <typename TV, typename TX>
@@ -174,6 +175,7 @@ struct NbView {
     int64_t nblk = 0;          // 3x3 blocks
     const int32_t *nrp = nullptr, *ncol = nullptr;
     const int32_t *node = nullptr;   // node at each position (nullptr: the identity)
+    const int32_t *split = nullptr;  // colour-sorted copies: first block of a later-coloured neighbour
     const void *val = nullptr;       // float or double [9 * nblk] + 16 bytes of slack
 };
 template <typename T>
@@ -183,14 +185,22 @@ int nb_spmm_grid(int K, int32_t npos);
 template <typename T>
 void launch_nb_spmm(int K, int epi, const NbView &A, const T *X, const T *B, const T *pre, T *Y, const int32_t *done,
                     double *partials, int stride, hipStream_t s, const int32_t *order = nullptr);
+// start: NB_GS_GENERAL, NB_GS_ZERO (X is 0 on entry and this is the sweep's first colour: only the
+// node's own block is read), NB_GS_LOWER (a zero-start forward sweep past its first colour, on a
+// colour-sorted copy: only the blocks of earlier colours are read, and the node's own X is taken
+// as 0, so X need not be cleared first)
+enum NbGsStart : int { NB_GS_GENERAL = 0, NB_GS_ZERO = 1, NB_GS_LOWER = 2 };
 template <typename T>
 void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T *X, bool backward, bool reverse,
-                  const int32_t *done, hipStream_t s);
+                  const int32_t *done, hipStream_t s, int start = NB_GS_GENERAL);
 void nb_count(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, bool diag_first, int32_t *cnt,
               int32_t *bad, hipStream_t s);
+// colour (per node, may be nullptr): sort each position's other blocks by the colour of their
+// neighbour node and write split (per position)
 template <typename TI, typename TV>
 void nb_fill(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, const TI *v, const int32_t *nrp,
-             bool diag_first, int32_t *ncol, TV *val, hipStream_t s);
+             bool diag_first, int32_t *ncol, TV *val, hipStream_t s, const int32_t *colour = nullptr,
+             int32_t *split = nullptr);
 
 // node transfers: output node i, input node j, one weight for the three dofs of the node
 struct NtView {

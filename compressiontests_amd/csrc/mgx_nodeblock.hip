@@ -159,6 +159,7 @@ __device__ __forceinline__ void nb_publish(const double *dot, double *red, doubl
 template <typename T, int K, int EPI>
 __global__ __launch_bounds__(kBlock) void k_nb_spmm(int32_t npos, const int32_t *__restrict__ nrp,
                                                     const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
+                                                    const int32_t *__restrict__ split,
                                                     const T *__restrict__ val, const T *__restrict__ X,
                                                     const T *__restrict__ B, const T *__restrict__ pre,
                                                     T *__restrict__ Y, const int32_t *__restrict__ done,
@@ -193,15 +194,29 @@ __global__ __launch_bounds__(kBlock) void k_nb_spmm(int32_t npos, const int32_t 
         wave_lds_fence();
         const T *lv = (const T *)((const char *)s_val[wv] + (((size_t)b0 * 9 * sizeof(T)) & 15));
         const int32_t *lc = (const int32_t *)((const char *)s_col[wv] + (((size_t)b0 * 4) & 15));
-        if (p < p1) nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + s, nrp[p + 1] - b0, X, c0, y);
+        // UPRESID: the blocks of later colours only
+        if (p < p1)
+            nb_accumulate<T, K>(lc, lv, (EPI == BEPI_UPRESID ? split[p] : nrp[p]) - b0 + s, nrp[p + 1] - b0, X, c0, y);
         nb_reduce<T, K>(y);
         if (p < p1 && s == 0) {
             const int q = node ? node[p] : p;
+            if (EPI == BEPI_UPRESID) {
+                // after a forward sweep from X = 0 (colours in order, each node's dofs 0, 1, 2 in
+                // order) rows are left with the pull of what was relaxed after them: the later
+                // colours and, inside the node, the strictly upper part of its own block
+                const T *d = lv + (size_t)(nrp[p] - b0) * 9;
+                const size_t i0 = (size_t)3 * q * K + c0;
+                const VT x1 = *(const VT *)(X + i0 + K), x2 = *(const VT *)(X + i0 + 2 * K);
+                y[0] += d[1] * x1;
+                y[0] += d[2] * x2;
+                y[1] += d[5] * x2;
+            }
 #pragma unroll
             for (int a = 0; a < 3; ++a) {
                 const size_t i = ((size_t)3 * q + a) * K + c0;
                 VT *yp = (VT *)(Y + i);
                 if (EPI == BEPI_SPMM) *yp = y[a];
+                else if (EPI == BEPI_UPRESID) *yp = -y[a];
                 else if (EPI == BEPI_ADD) *yp = *yp + y[a];
                 else if (EPI == BEPI_RESID) *yp = *(const VT *)(B + i) - y[a];
                 else if (EPI == BEPI_JACOBI) *yp = *(const VT *)(X + i) + pre[3 * q + a] * (*(const VT *)(B + i) - y[a]);
@@ -247,16 +262,19 @@ void launch_nb_spmm(int K, int epi, const NbView &A, const T *X, const T *B, con
                     double *partials, int stride, hipStream_t s, const int32_t *order)
 {
     if (A.npos <= 0) return;
+    if (epi == BEPI_UPRESID && !A.split)
+        throw Error{MGX_ERR_INVALID_OPERATION, "the residual of a zero-start sweep needs a colour-sorted copy"};
     const int g = nb_spmm_grid<T>(K, A.npos);
 #define MGX_NBS(KK, EE)                                                                                          \
     hipLaunchKernelGGL((k_nb_spmm<T, KK, EE>), dim3(g), dim3(kBlock), 0, s, A.npos, A.nrp, A.ncol, A.node,     \
-                       (const T *)A.val, X, B, pre, Y, done, partials, stride, order)
+                       A.split, (const T *)A.val, X, B, pre, Y, done, partials, stride, order)
 #define MGX_NBS_K(KK)                                       \
     switch (epi) {                                          \
     case BEPI_SPMM: MGX_NBS(KK, BEPI_SPMM); break;          \
     case BEPI_ADD: MGX_NBS(KK, BEPI_ADD); break;            \
     case BEPI_RESID: MGX_NBS(KK, BEPI_RESID); break;        \
     case BEPI_JACOBI: MGX_NBS(KK, BEPI_JACOBI); break;      \
+    case BEPI_UPRESID: MGX_NBS(KK, BEPI_UPRESID); break;    \
     default: MGX_NBS(KK, BEPI_DOT); break;                  \
     }
     switch (K) {
@@ -286,11 +304,31 @@ template void launch_nb_spmm<double>(int, int, const NbView &, const double *, c
 // D00, then x_1 with the new x_0, then x_2 (reverse order when backward).
 // One wave tile [pa, pb) of the colour-ordered copy (pb - pa <= NW): stage its blocks, sum the
 // off-diagonal blocks, relax each node.  sv / sc: the wave's LDS staging region.
-template <typename T, int K>
+// the node's 3x3 solve: x_0 from row 0 with the current x_1, x_2, then x_1, then x_2 (reverse
+// order when backward)
+template <typename T>
+__device__ __forceinline__ void nb_relax(const T *d, const typename NbVec<T>::type &r0,
+                                         const typename NbVec<T>::type &r1, const typename NbVec<T>::type &r2,
+                                         typename NbVec<T>::type &x0, typename NbVec<T>::type &x1,
+                                         typename NbVec<T>::type &x2, int backward)
+{
+    if (!backward) {
+        x0 = (r0 - d[1] * x1 - d[2] * x2) / d[0];
+        x1 = (r1 - d[3] * x0 - d[5] * x2) / d[4];
+        x2 = (r2 - d[6] * x0 - d[7] * x1) / d[8];
+    } else {
+        x2 = (r2 - d[6] * x0 - d[7] * x1) / d[8];
+        x1 = (r1 - d[3] * x0 - d[5] * x2) / d[4];
+        x0 = (r0 - d[1] * x1 - d[2] * x2) / d[0];
+    }
+}
+
+template <typename T, int K, bool LOWER>
 __device__ __forceinline__ void nb_gs_tile(int pa, int pb, const int32_t *__restrict__ nrp,
                                            const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
-                                           const T *__restrict__ val, const T *__restrict__ B, T *__restrict__ X,
-                                           int backward, u32x4 *sv, u32x4 *sc, int lane)
+                                           const int32_t *__restrict__ split, const T *__restrict__ val,
+                                           const T *__restrict__ B, T *__restrict__ X, int backward, u32x4 *sv,
+                                           u32x4 *sc, int lane)
 {
     using Sh = NbShape<T, K>;
     using VT = typename NbVec<T>::type;
@@ -303,34 +341,60 @@ __device__ __forceinline__ void nb_gs_tile(int pa, int pb, const int32_t *__rest
     wave_lds_fence();
     const T *lv = (const T *)((const char *)sv + (((size_t)b0 * 9 * sizeof(T)) & 15));
     const int32_t *lc = (const int32_t *)((const char *)sc + (((size_t)b0 * 4) & 15));
-    if (p < pb) nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + 1 + s, nrp[p + 1] - b0, X, c0, y);
+    // LOWER: the blocks of earlier colours only (the later ones multiply zeros)
+    if (p < pb) nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + 1 + s, (LOWER ? split[p] : nrp[p + 1]) - b0, X, c0, y);
     nb_reduce<T, K>(y);
     if (p >= pb || s != 0) return;
     const int q = node ? node[p] : p;
     const T *d = lv + (size_t)(nrp[p] - b0) * 9;                 // the node's own block (slot 0)
     const size_t i0 = (size_t)3 * q * K + c0;
-    VT x0 = *(const VT *)(X + i0), x1 = *(const VT *)(X + i0 + K), x2 = *(const VT *)(X + i0 + 2 * K);
+    VT x0 = VT(0), x1 = VT(0), x2 = VT(0);
+    if (!LOWER) x0 = *(const VT *)(X + i0), x1 = *(const VT *)(X + i0 + K), x2 = *(const VT *)(X + i0 + 2 * K);
     const VT r0 = *(const VT *)(B + i0) - y[0], r1 = *(const VT *)(B + i0 + K) - y[1],
              r2 = *(const VT *)(B + i0 + 2 * K) - y[2];
-    if (!backward) {
-        x0 = (r0 - d[1] * x1 - d[2] * x2) / d[0];
-        x1 = (r1 - d[3] * x0 - d[5] * x2) / d[4];
-        x2 = (r2 - d[6] * x0 - d[7] * x1) / d[8];
-    } else {
-        x2 = (r2 - d[6] * x0 - d[7] * x1) / d[8];
-        x1 = (r1 - d[3] * x0 - d[5] * x2) / d[4];
-        x0 = (r0 - d[1] * x1 - d[2] * x2) / d[0];
-    }
+    nb_relax<T>(d, r0, r1, r2, x0, x1, x2, backward);
     *(VT *)(X + i0) = x0;
     *(VT *)(X + i0 + K) = x1;
     *(VT *)(X + i0 + 2 * K) = x2;
 }
 
+// The first colour launch of a sweep that starts from X = 0: every neighbour value is zero, so a
+// node's relaxation reads only its own block and its B rows (the off-diagonal sums are 0 and the
+// node's own X values 0, exactly what the general kernel would read).  Thread (p, v) handles the
+// W columns v*W .. v*W+W-1 of position p.
 template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_nb_gs_zero(int32_t p0c, int32_t p1c, const int32_t *__restrict__ nrp,
+                                                       const int32_t *__restrict__ node, const T *__restrict__ val,
+                                                       const T *__restrict__ B, T *__restrict__ X, int backward,
+                                                       const int32_t *__restrict__ done)
+{
+    using VT = typename NbVec<T>::type;
+    constexpr int W = 16 / (int)sizeof(T);
+    constexpr int V = K / W;
+    if (done && *done) return;
+    const int64_t t = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
+    const int64_t pp = t / V;
+    if (pp >= p1c - p0c) return;
+    const int p = p0c + (int)pp, c0 = (int)(t - pp * V) * W;
+    const int q = node ? node[p] : p;
+    T d[9];
+#pragma unroll
+    for (int e = 0; e < 9; ++e) d[e] = val[(size_t)nrp[p] * 9 + e];
+    const size_t i0 = (size_t)3 * q * K + c0;
+    VT x0 = VT(0), x1 = VT(0), x2 = VT(0);
+    const VT r0 = *(const VT *)(B + i0), r1 = *(const VT *)(B + i0 + K), r2 = *(const VT *)(B + i0 + 2 * K);
+    nb_relax<T>(d, r0, r1, r2, x0, x1, x2, backward);
+    *(VT *)(X + i0) = x0;
+    *(VT *)(X + i0 + K) = x1;
+    *(VT *)(X + i0 + 2 * K) = x2;
+}
+
+template <typename T, int K, bool LOWER>
 __global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, const int32_t *__restrict__ nrp,
                                                   const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
-                                                  const T *__restrict__ val, const T *__restrict__ B, T *__restrict__ X,
-                                                  int backward, int reverse, const int32_t *__restrict__ done)
+                                                  const int32_t *__restrict__ split, const T *__restrict__ val,
+                                                  const T *__restrict__ B, T *__restrict__ X, int backward,
+                                                  int reverse, const int32_t *__restrict__ done)
 {
     using Sh = NbShape<T, K>;
     __shared__ u32x4 s_val[kNbWaves][Sh::VQ];
@@ -342,19 +406,42 @@ __global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, cons
     if (t0 >= ntiles) return;                                    // the whole wave leaves together
     const int tile = reverse ? ntiles - 1 - t0 : t0;
     const int pa = p0c + tile * Sh::NW;
-    nb_gs_tile<T, K>(pa, min(pa + Sh::NW, p1c), nrp, ncol, node, val, B, X, backward, s_val[wv], s_col[wv], lane);
+    nb_gs_tile<T, K, LOWER>(pa, min(pa + Sh::NW, p1c), nrp, ncol, node, split, val, B, X, backward, s_val[wv],
+                            s_col[wv], lane);
 }
 
 template <typename T>
 void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T *X, bool backward, bool reverse,
-                  const int32_t *done, hipStream_t s)
+                  const int32_t *done, hipStream_t s, int start)
 {
     if (p1 <= p0) return;
+    if (start == NB_GS_LOWER && (!A.split || backward))
+        throw Error{MGX_ERR_INVALID_OPERATION, "lower-colour sweep needs a colour-sorted copy and a forward sweep"};
+    if (start == NB_GS_ZERO) {
+        const int g = grid_rows((int64_t)(p1 - p0) * (K / (16 / (int)sizeof(T))), kBlock);
+#define MGX_NBZ(KK)                                                                                              \
+    hipLaunchKernelGGL((k_nb_gs_zero<T, KK>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.node,              \
+                       (const T *)A.val, B, X, backward ? 1 : 0, done)
+        switch (K) {
+        case 4: MGX_NBZ(4); break;
+        case 8: MGX_NBZ(8); break;
+        case 16: MGX_NBZ(16); break;
+        case 32: MGX_NBZ(32); break;
+        default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
+        }
+#undef MGX_NBZ
+        MGX_HIP(hipGetLastError());
+        return;
+    }
     const int nw = nb_tile<T>(K);
     const int g = grid_rows((p1 - p0 + nw - 1) / nw, kNbWaves);
 #define MGX_NBG(KK)                                                                                              \
-    hipLaunchKernelGGL((k_nb_gs<T, KK>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.ncol, A.node,           \
-                       (const T *)A.val, B, X, backward ? 1 : 0, reverse ? 1 : 0, done)
+    if (start == NB_GS_LOWER)                                                                                    \
+        hipLaunchKernelGGL((k_nb_gs<T, KK, true>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.ncol, A.node,  \
+                           A.split, (const T *)A.val, B, X, 0, reverse ? 1 : 0, done);                           \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_nb_gs<T, KK, false>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.ncol, A.node, \
+                           A.split, (const T *)A.val, B, X, backward ? 1 : 0, reverse ? 1 : 0, done)
     switch (K) {
     case 4: MGX_NBG(4); break;
     case 8: MGX_NBG(8); break;
@@ -366,9 +453,9 @@ void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T 
     MGX_HIP(hipGetLastError());
 }
 template void launch_nb_gs<float>(int, const NbView &, int32_t, int32_t, const float *, float *, bool, bool,
-                                  const int32_t *, hipStream_t);
+                                  const int32_t *, hipStream_t, int);
 template void launch_nb_gs<double>(int, const NbView &, int32_t, int32_t, const double *, double *, bool, bool,
-                                   const int32_t *, hipStream_t);
+                                   const int32_t *, hipStream_t, int);
 
 // ------------------------------------------------------------------ node transfers
 // A transfer matrix (prolongation P or restriction R) between two levels whose dofs come in node
@@ -524,23 +611,54 @@ __global__ __launch_bounds__(kBlock) void k_nb_count(int32_t npos, const int32_t
     cnt[p] = ok ? len / 3 : 0;
 }
 
+// Slots of a position's blocks: with diag_first its own block first; with colour (diag_first
+// only) the other blocks ordered by the colour of their neighbour node, ties in column order, and
+// split[p] = the first slot of a neighbour coloured after the node itself (neighbours never share
+// the node's colour).  Without colour the column order is kept.
 template <typename TI, typename TV>
 __global__ __launch_bounds__(kBlock) void k_nb_fill(int32_t npos, const int32_t *__restrict__ node,
                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                     const TI *__restrict__ v, const int32_t *__restrict__ nrp,
-                                                    int diag_first, int32_t *__restrict__ ncol, TV *__restrict__ val)
+                                                    int diag_first, int32_t *__restrict__ ncol, TV *__restrict__ val,
+                                                    const int32_t *__restrict__ colour, int32_t *__restrict__ split)
 {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= npos) return;
     const int q = node ? node[p] : p;
     const int nb = nrp[p + 1] - nrp[p];
-    if (nb == 0) return;
+    if (nb == 0) {
+        if (split) split[p] = nrp[p];
+        return;
+    }
     const int r0 = rp[3 * q];
     int own = -1;
     if (diag_first)
         for (int m = 0; m < nb; ++m) if (ci[r0 + 3 * m] == 3 * q) own = m;
+    const bool bycol = colour && own >= 0;
+    if (bycol) {
+        const int cq = colour[q];
+        int before = 0;
+        for (int m = 0; m < nb; ++m)
+            if (m != own && colour[ci[r0 + 3 * m] / 3] < cq) ++before;
+        split[p] = nrp[p] + 1 + before;
+    } else if (split) {
+        split[p] = nrp[p + 1];
+    }
     for (int m = 0; m < nb; ++m) {
-        const int slot = nrp[p] + (own < 0 ? m : (m == own ? 0 : (m < own ? m + 1 : m)));
+        int slot;
+        if (bycol) {
+            // rank among the other blocks by (neighbour colour, position in the column list)
+            const int cm = colour[ci[r0 + 3 * m] / 3];
+            int rank = 0;
+            for (int o = 0; o < nb; ++o) {
+                if (o == own || o == m) continue;
+                const int co = colour[ci[r0 + 3 * o] / 3];
+                rank += (co < cm || (co == cm && o < m)) ? 1 : 0;
+            }
+            slot = nrp[p] + (m == own ? 0 : 1 + rank);
+        } else {
+            slot = nrp[p] + (own < 0 ? m : (m == own ? 0 : (m < own ? m + 1 : m)));
+        }
         ncol[slot] = ci[r0 + 3 * m] / 3;
 #pragma unroll
         for (int a = 0; a < 3; ++a)
@@ -560,18 +678,21 @@ void nb_count(int32_t npos, const int32_t *node, const int32_t *rp, const int32_
 
 template <typename TI, typename TV>
 void nb_fill(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, const TI *v, const int32_t *nrp,
-             bool diag_first, int32_t *ncol, TV *val, hipStream_t s)
+             bool diag_first, int32_t *ncol, TV *val, hipStream_t s, const int32_t *colour, int32_t *split)
 {
     if (npos <= 0) return;
     hipLaunchKernelGGL((k_nb_fill<TI, TV>), dim3((npos + kBlock - 1) / kBlock), dim3(kBlock), 0, s, npos, node, rp, ci,
-                       v, nrp, diag_first ? 1 : 0, ncol, val);
+                       v, nrp, diag_first ? 1 : 0, ncol, val, colour, split);
     MGX_HIP(hipGetLastError());
 }
 template void nb_fill<double, double>(int32_t, const int32_t *, const int32_t *, const int32_t *, const double *,
-                                      const int32_t *, bool, int32_t *, double *, hipStream_t);
+                                      const int32_t *, bool, int32_t *, double *, hipStream_t, const int32_t *,
+                                      int32_t *);
 template void nb_fill<double, float>(int32_t, const int32_t *, const int32_t *, const int32_t *, const double *,
-                                     const int32_t *, bool, int32_t *, float *, hipStream_t);
+                                     const int32_t *, bool, int32_t *, float *, hipStream_t, const int32_t *,
+                                     int32_t *);
 template void nb_fill<float, float>(int32_t, const int32_t *, const int32_t *, const int32_t *, const float *,
-                                    const int32_t *, bool, int32_t *, float *, hipStream_t);
+                                    const int32_t *, bool, int32_t *, float *, hipStream_t, const int32_t *,
+                                    int32_t *);
 
 }  // namespace mgx

@@ -268,7 +268,9 @@ typedef struct {
     double  spmv_ms;         /* their summed device time (HIP events on the solver stream) */
     int32_t block_width;     /* right-hand sides solved together with this one (1: alone); with
                                 block_width > 1, solve_ms is the device time of that whole block */
-    int32_t gs_launches;     /* batching form with options.profile: level-0 Gauss-Seidel colour launches timed */
+    int32_t gs_launches;     /* batching form with options.profile: level-0 Gauss-Seidel colour launches timed
+                                (the general kernel; the first launch of a sweep from x = 0 reads no
+                                neighbours and is not counted) */
     double  gs_ms;           /* their summed device time (HIP events on the solver stream around each sweep) */
 } mgx_stats;
 
