@@ -431,6 +431,7 @@ struct mgx_solver_s {
     int bprof_n = 0;
     std::vector<hipEvent_t> gs_events;     // options.profile, batching form: pairs around level-0 GS sweeps
     std::vector<int> gs_launches;          // colour launches between each pair
+    std::vector<int> gs_iter;              // index of the profiled iteration each pair belongs to
     int gs_used = 0;
     bool prof_gs = false;                  // record GS events (inside a profiled block PCG iteration)
     double bgs_ms = 0;                     // their summed time in the current block
@@ -1156,7 +1157,8 @@ static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, b
     }
     if (rec) {
         MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used + 1], S->stream));
-        S->gs_launches[S->gs_used++] = V.ncolors;
+        S->gs_launches[S->gs_used] = V.ncolors;
+        S->gs_iter[S->gs_used++] = S->prof_used - 1;     // the profiled iteration (its A*p event pair)
     }
 }
 
@@ -1205,7 +1207,8 @@ static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const 
     T *cur;
     // one forward sweep from x = 0 on a colour-sorted copy: the sweep reads the earlier colours
     // only (x need not be cleared), and the residual is the pull of the later colours
-    const bool tri = S->opt.smoother == MGX_SMOOTHER_GS && nu == 1 && nb_ready<T>(V.nbg) && V.nbg.view.split;
+    const bool tri = S->opt.smoother == MGX_SMOOTHER_GS && nu == 1 && nb_ready<T>(V.nbg) && V.nbg.view.split &&
+                     nb_ready<T>(V.nb) && V.nb.view.split;
     if (tri) {
         gs_block_sweep<T>(S, V, K, b, x, false, done, NB_GS_LOWER);
         cur = x;
@@ -1224,7 +1227,9 @@ static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const 
             cur = o;
         }
     }
-    if (tri) launch_nb_spmm<T>(K, BEPI_UPRESID, V.nbg.view, cur, nullptr, nullptr, B::r(V), done, nullptr, 0, s);
+    if (tri)
+        launch_nb_spmm<T>(K, BEPI_UPRESID, V.nb.view, cur, nullptr, nullptr, B::r(V), done, nullptr, 0, s,
+                          V.tile_order_npb == nb_tile<T>(K) ? V.tile_order.p : nullptr);
     else level_spmm<T, T>(V, K, BEPI_RESID, Av, cur, b, nullptr, B::r(V), done, nullptr, s);
     Level &C = S->lv[l + 1];
     if (nt_ready<T>(V.Rn)) launch_nt_apply<T>(K, false, V.Rn.view, B::r(V), B::b(C), done, s);
@@ -1325,16 +1330,15 @@ static void block_loop(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P, T *Q
         }
         S->bprof_n += real;
         S->prof_used = prof0 + real;
-        // level-0 GS sweeps of the real iterations (a fixed number per V-cycle), counted in
-        // colour launches
-        const int per = S->opt.smoother == MGX_SMOOTHER_GS ? 2 * S->opt.level_iterations : 0;
-        const int rg = std::min(S->gs_used - gs0, per * real);
-        for (int i = gs0; i < gs0 + rg; ++i) {
+        // level-0 GS sweeps of the real iterations, counted in colour launches; the V-cycle of
+        // the last one returns at once (every column was done at its update)
+        int rg = 0;
+        for (int i = gs0; i < S->gs_used && S->gs_iter[i] < prof0 + real - 1; ++i, ++rg) {
             float ms = 0;
             MGX_HIP(hipEventElapsedTime(&ms, S->gs_events[2 * i], S->gs_events[2 * i + 1]));
             S->bgs_ms += ms;
+            S->bgs_n += S->gs_launches[i];
         }
-        for (int i = gs0; i < gs0 + rg; ++i) S->bgs_n += S->gs_launches[i];
         S->gs_used = gs0 + rg;
     }
 }
@@ -1517,9 +1521,28 @@ static void order_for_cache(mgx_solver_s *S, int K)
             if (!ok) { V.Rn.reset(); V.Pn.reset(); V.nt_unfit = true; }
         }
         if (!V.node3) continue;
-        // node-block copy in node order (independent of K): built once
-        if (!V.nb.built() && !V.nb_unfit)
-            V.nb_unfit = !build_nb<double>(V.nb, V.n, V.A.rp, V.A.ci, V.A.v, nullptr, false, mixed, st);
+        // node-block copy in node order (independent of K): built once; with a colouring, each
+        // node's own block first and the others by neighbour colour (the split serves the residual
+        // of a zero-start sweep)
+        if (!V.nb.built() && !V.nb_unfit) {
+            DBuf<int32_t> colour;
+            if (V.ncolors > 0) {
+                std::vector<int32_t> col((size_t)V.n / 3, 0);
+                for (int c = 0; c < V.ncolors; ++c) {
+                    std::vector<int32_t> nodes(V.color_count[c]);
+                    if (nodes.empty()) continue;
+                    MGX_HIP(hipMemcpyAsync(nodes.data(), V.color_nodes[c].p, sizeof(int32_t) * nodes.size(),
+                                           hipMemcpyDeviceToHost, st));
+                    MGX_HIP(hipStreamSynchronize(st));
+                    for (int32_t q : nodes) col[q] = c;
+                }
+                colour.upload(col.data(), col.size(), st);
+            }
+            V.nb_unfit = !build_nb<double>(V.nb, V.n, V.A.rp, V.A.ci, V.A.v, nullptr, colour.p != nullptr, mixed, st,
+                                           colour.p);
+            if (V.nb_unfit && colour.p)   // a node without its own block: plain column order
+                V.nb_unfit = !build_nb<double>(V.nb, V.n, V.A.rp, V.A.ci, V.A.v, nullptr, false, mixed, st);
+        }
         // the precision form's FP64 true residual on level 0 reads an FP64 copy
         if (l == 0 && mixed && V.nb.built() && !V.nb64.built())
             build_nb<double>(V.nb64, V.n, V.A.rp, V.A.ci, V.A.v, nullptr, false, false, st);
@@ -2157,6 +2180,7 @@ int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
             for (auto &e : s->prof_events) MGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
             s->gs_events.resize(2 * 2 * 256);     // 256 V-cycles of 2 level-0 sweeps
             s->gs_launches.assign(2 * 256, 0);
+            s->gs_iter.assign(2 * 256, 0);
             for (auto &e : s->gs_events) MGX_HIP(hipEventCreateWithFlags(&e, hipEventDisableSystemFence));
         }
         *out = s.release();
