@@ -1273,12 +1273,12 @@ static void block_iteration(mgx_solver_s *S, int K, BCtl *ctl, T *X, T *R, T *P,
     level_spmm<T, T>(V, K, BEPI_DOT, BlkSel<T>::val(V.A), P, nullptr, nullptr, Q, &ctl->done, S->bpartials.p, s);
     if (prof) { MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used + 1], s)); ++S->prof_used; }
     launch_bfinalize(level_spmm_parts<T>(V, K), S->bpartials.p, ctl, FIN_ALPHA, s);
-    launch_bupdate<T>(K, n, P, Q, X, R, ctl, S->bpartials.p, s);
+    launch_bupdate_r<T>(K, n, Q, R, ctl, S->bpartials.p, s);
     S->prof_gs = prof;
     vcycle_block<T>(S, K, 0, R, Z, &ctl->done);
     S->prof_gs = false;
     launch_bdot_beta<T>(K, n, R, Z, ctl, S->bpartials.p, false, s);
-    launch_bupdate_p<T>(K, n, Z, P, ctl, false, s);
+    launch_bupdate_xp<T>(K, n, Z, P, X, ctl, s);
 }
 
 // Runs block PCG iterations until every column is done; `check_every` iterations are queued

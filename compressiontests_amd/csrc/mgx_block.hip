@@ -611,6 +611,40 @@ __global__ __launch_bounds__(kBlock) void k_bupdate(int64_t nk, const T *__restr
     publish_cols<K, 2, 1>(rr, m, 0, partials, stride);
 }
 
+// r -= alpha_c q, with per-column ||r||^2 and max |r|
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_bupdate_r(int64_t nk, const T *__restrict__ Q, T *__restrict__ R,
+                                                      const BCtl *__restrict__ ctl, double *__restrict__ partials,
+                                                      int stride)
+{
+    if (ctl->done) return;
+    const double a = ctl->alpha[threadIdx.x % K];
+    double rr = 0, m = 0;
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nk; i += (int64_t)gridDim.x * kBlock) {
+        const T r = (T)((double)R[i] - a * (double)Q[i]);
+        R[i] = r;
+        rr += (double)r * (double)r;
+        m = nanmax(m, fabs((double)r));
+    }
+    publish_cols<K, 2, 1>(rr, m, 0, partials, stride);
+}
+
+// x += alpha_c p with the p that gave alpha, then p = z + beta_c p; once every column is done
+// only the pending x update is applied (xflushed guards against applying it twice)
+template <typename T, int K>
+__global__ __launch_bounds__(kBlock) void k_bupdate_xp(int64_t nk, const T *__restrict__ Z, T *__restrict__ P,
+                                                       T *__restrict__ X, const BCtl *__restrict__ ctl)
+{
+    if (ctl->xflushed) return;
+    const bool more = !ctl->done;
+    const double a = ctl->alpha[threadIdx.x % K], b = ctl->beta[threadIdx.x % K];
+    for (int64_t i = (int64_t)blockIdx.x * kBlock + threadIdx.x; i < nk; i += (int64_t)gridDim.x * kBlock) {
+        const double p = (double)P[i];
+        X[i] = (T)((double)X[i] + a * p);
+        if (more) P[i] = (T)((double)Z[i] + b * p);
+    }
+}
+
 // p = z + beta_c p (p = z on the first iteration)
 template <typename T, int K>
 __global__ __launch_bounds__(kBlock) void k_bupdate_p(int64_t nk, const T *__restrict__ Z, T *__restrict__ P,
@@ -639,7 +673,14 @@ __global__ __launch_bounds__(1024) void k_bfinalize(int nparts, const double *__
                                                     BCtl *ctl, int mode)
 {
     __shared__ int s_done[kMaxBlock];
-    if (mode != FIN_NORMS && ctl->done) return;
+    if (mode != FIN_NORMS && ctl->done) {
+        if (mode == FIN_ALPHA && threadIdx.x == 0) ctl->xflushed = 1;   // the last x update is applied
+        return;
+    }
+    if (threadIdx.x == 0) {
+        if (mode == FIN_ALPHA) ctl->xflushed = 0;      // x += alpha p now pending
+        if (mode == FIN_NORMS) ctl->xflushed = 1;
+    }
     const int K = ctl->k;
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     for (int c = w; c < K; c += 16) {
@@ -754,6 +795,29 @@ void launch_bupdate(int K, int32_t n, const T *P, const T *Q, T *X, T *R, BCtl *
 }
 
 template <typename T>
+void launch_bupdate_r(int K, int32_t n, const T *Q, T *R, BCtl *ctl, double *partials, hipStream_t s)
+{
+    const int64_t nk = (int64_t)n * K;
+    const int g = grid_cols(nk);
+#define L_(KK) hipLaunchKernelGGL((k_bupdate_r<T, KK>), dim3(g), dim3(kBlock), 0, s, nk, Q, R, ctl, partials, kPartialStride)
+    MGX_K_SWITCH(K, L_)
+#undef L_
+    MGX_HIP(hipGetLastError());
+    launch_bfinalize(g, partials, ctl, FIN_UPDATE, s);
+}
+
+template <typename T>
+void launch_bupdate_xp(int K, int32_t n, const T *Z, T *P, T *X, const BCtl *ctl, hipStream_t s)
+{
+    const int64_t nk = (int64_t)n * K;
+    const int g = grid_cols(nk);
+#define L_(KK) hipLaunchKernelGGL((k_bupdate_xp<T, KK>), dim3(g), dim3(kBlock), 0, s, nk, Z, P, X, ctl)
+    MGX_K_SWITCH(K, L_)
+#undef L_
+    MGX_HIP(hipGetLastError());
+}
+
+template <typename T>
 void launch_bupdate_p(int K, int32_t n, const T *Z, T *P, const BCtl *ctl, bool first, hipStream_t s)
 {
     const int64_t nk = (int64_t)n * K;
@@ -793,6 +857,8 @@ void launch_spmm_dot_alpha(int K, int32_t n, const int32_t *rp, const int32_t *c
     template void launch_bnorms<T>(int, int32_t, const T *, const T *, BCtl *, double *, hipStream_t);             \
     template void launch_bdot_beta<T>(int, int32_t, const T *, const T *, BCtl *, double *, bool, hipStream_t);    \
     template void launch_bupdate<T>(int, int32_t, const T *, const T *, T *, T *, BCtl *, double *, hipStream_t);  \
+    template void launch_bupdate_r<T>(int, int32_t, const T *, T *, BCtl *, double *, hipStream_t);                 \
+    template void launch_bupdate_xp<T>(int, int32_t, const T *, T *, T *, const BCtl *, hipStream_t);               \
     template void launch_bupdate_p<T>(int, int32_t, const T *, T *, const BCtl *, bool, hipStream_t);              \
     template void launch_bjacobi_initial<T>(int, int32_t, const T *, const T *, T *, const int32_t *, hipStream_t); \
     template void launch_spmm_dot_alpha<T, T>(int, int32_t, const int32_t *, const int32_t *, const T *, const T *, \

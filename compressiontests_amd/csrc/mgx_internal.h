@@ -118,6 +118,7 @@ struct BCtl {
     int32_t iter[kMaxBlock], cdone[kMaxBlock], status[kMaxBlock];
     int32_t frozen[kMaxBlock];   // set by the host: the column takes no part in this solve
     int32_t done;                // every column done: kernels of later iterations return
+    int32_t xflushed;            // no x += alpha p pending (set at the start and once done, cleared by each alpha)
     int32_t k;                   // block width
     int32_t stop, max_it;
     double tol;
@@ -148,6 +149,12 @@ template <typename T>
 void launch_bdot_beta(int K, int32_t n, const T *R, const T *Z, BCtl *ctl, double *partials, bool first, hipStream_t s);
 template <typename T>
 void launch_bupdate(int K, int32_t n, const T *P, const T *Q, T *X, T *R, BCtl *ctl, double *partials, hipStream_t s);
+// the iteration's updates split around the preconditioner: r -= alpha q (with the norms and the
+// stop test) right after alpha; x += alpha p fused with p = z + beta p after beta
+template <typename T>
+void launch_bupdate_r(int K, int32_t n, const T *Q, T *R, BCtl *ctl, double *partials, hipStream_t s);
+template <typename T>
+void launch_bupdate_xp(int K, int32_t n, const T *Z, T *P, T *X, const BCtl *ctl, hipStream_t s);
 template <typename T>
 void launch_bupdate_p(int K, int32_t n, const T *Z, T *P, const BCtl *ctl, bool first, hipStream_t s);
 template <typename T>

@@ -48,6 +48,11 @@ template <typename T, int K> struct NbShape {
     static constexpr int VQ = ((NW * MB * 9 * (int)sizeof(T)) / 16 + 2 + 63) / 64 * 64;
     static constexpr int CQ = ((NW * MB * 4) / 16 + 2 + 63) / 64 * 64;
     static constexpr int U = 3;                             // blocks in flight per lane
+    // per-node staging (kernels that read part of each node's blocks): a region of RQ value slots
+    // and RC column slots per node, whole rounds of the node's G lanes
+    static constexpr int RQ = ((MB * 9 * (int)sizeof(T)) / 16 + 2 + G - 1) / G * G;
+    static constexpr int RC = ((MB * 4) / 16 + 2 + G - 1) / G * G;
+    static_assert(NW * RQ <= VQ && NW * RC <= CQ, "per-node regions fit the wave's staging area");
 };
 static_assert(NbShape<float, 4>::G == 8 && NbShape<float, 16>::S == 2 && NbShape<float, 32>::G == 16 &&
                   NbShape<double, 32>::NW == 2,
@@ -80,6 +85,38 @@ __device__ __forceinline__ void stage16(const char *__restrict__ src, size_t lo,
     for (int i = 0; i < NI; ++i) t[i] = __builtin_nontemporal_load(s + min(lane + 64 * i, last));
 #pragma unroll
     for (int i = 0; i < NI; ++i) dst[lane + 64 * i] = t[i];
+}
+
+// stage16 for the G lanes of one node group (l = lane in the group): bytes [lo, hi) into
+// dst[0 .. NQ), NQ a multiple of G; the same clamping (loads reach up to 15 bytes past hi)
+template <int NQ, int G>
+__device__ __forceinline__ void stage16_group(const char *__restrict__ src, size_t lo, size_t hi, u32x4 *dst, int l)
+{
+    constexpr int NI = NQ / G;
+    const size_t a0 = lo & ~(size_t)15;
+    const int last = max((int)((hi - a0 + 15) >> 4), 1) - 1;
+    const u32x4 *s = (const u32x4 *)(src + a0);
+    u32x4 t[NI];
+#pragma unroll
+    for (int i = 0; i < NI; ++i) t[i] = __builtin_nontemporal_load(s + min(l + G * i, last));
+#pragma unroll
+    for (int i = 0; i < NI; ++i) dst[l + G * i] = t[i];
+}
+
+// Blocks [m0, m1) of one node into its group's regions of the wave's staging area; returns the
+// LDS addresses of block m0's values and column (alignment shift applied).  Call between two
+// wave_lds_fence()s.
+template <typename T, int K>
+__device__ __forceinline__ void nb_stage_node(const T *__restrict__ val, const int32_t *__restrict__ ncol, int m0,
+                                              int m1, u32x4 *sv, u32x4 *sc, int g, int l, const T *&lv,
+                                              const int32_t *&lc)
+{
+    using Sh = NbShape<T, K>;
+    const size_t vlo = (size_t)m0 * 9 * sizeof(T), clo = (size_t)m0 * 4;
+    stage16_group<Sh::RQ, Sh::G>((const char *)val, vlo, (size_t)m1 * 9 * sizeof(T), sv + g * Sh::RQ, l);
+    stage16_group<Sh::RC, Sh::G>((const char *)ncol, clo, (size_t)m1 * 4, sc + g * Sh::RC, l);
+    lv = (const T *)((const char *)(sv + g * Sh::RQ) + (vlo & 15));
+    lc = (const int32_t *)((const char *)(sc + g * Sh::RC) + (clo & 15));
 }
 
 // Sum over the blocks m = m0, m0 + S, ... < m1 of (block m) x (X rows of its neighbour node), for
@@ -189,22 +226,31 @@ __global__ __launch_bounds__(kBlock) void k_nb_spmm(int32_t npos, const int32_t 
         const int p = p0 + g;
         VT y[3] = {VT(0), VT(0), VT(0)};
         wave_lds_fence();                                        // the previous tile's LDS reads are issued
-        stage16<Sh::VQ>((const char *)val, (size_t)b0 * 9 * sizeof(T), (size_t)b1 * 9 * sizeof(T), s_val[wv], lane);
-        stage16<Sh::CQ>((const char *)ncol, (size_t)b0 * 4, (size_t)b1 * 4, s_col[wv], lane);
-        wave_lds_fence();
-        const T *lv = (const T *)((const char *)s_val[wv] + (((size_t)b0 * 9 * sizeof(T)) & 15));
-        const int32_t *lc = (const int32_t *)((const char *)s_col[wv] + (((size_t)b0 * 4) & 15));
-        // UPRESID: the blocks of later colours only
-        if (p < p1)
-            nb_accumulate<T, K>(lc, lv, (EPI == BEPI_UPRESID ? split[p] : nrp[p]) - b0 + s, nrp[p + 1] - b0, X, c0, y);
+        if (EPI == BEPI_UPRESID) {
+            // the blocks of later colours only, staged node by node
+            const int m0 = p < p1 ? split[p] : b1, m1 = p < p1 ? nrp[p + 1] : b1;
+            const T *lv;
+            const int32_t *lc;
+            nb_stage_node<T, K>(val, ncol, m0, m1, s_val[wv], s_col[wv], g, l, lv, lc);
+            wave_lds_fence();
+            if (p < p1) nb_accumulate<T, K>(lc, lv, s, m1 - m0, X, c0, y);
+        } else {
+            stage16<Sh::VQ>((const char *)val, (size_t)b0 * 9 * sizeof(T), (size_t)b1 * 9 * sizeof(T), s_val[wv],
+                            lane);
+            stage16<Sh::CQ>((const char *)ncol, (size_t)b0 * 4, (size_t)b1 * 4, s_col[wv], lane);
+            wave_lds_fence();
+            const T *lv = (const T *)((const char *)s_val[wv] + (((size_t)b0 * 9 * sizeof(T)) & 15));
+            const int32_t *lc = (const int32_t *)((const char *)s_col[wv] + (((size_t)b0 * 4) & 15));
+            if (p < p1) nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + s, nrp[p + 1] - b0, X, c0, y);
+        }
         nb_reduce<T, K>(y);
         if (p < p1 && s == 0) {
             const int q = node ? node[p] : p;
             if (EPI == BEPI_UPRESID) {
                 // after a forward sweep from X = 0 (colours in order, each node's dofs 0, 1, 2 in
                 // order) rows are left with the pull of what was relaxed after them: the later
-                // colours and, inside the node, the strictly upper part of its own block
-                const T *d = lv + (size_t)(nrp[p] - b0) * 9;
+                // colours and, inside the node, the strictly upper part of its own block (slot 0)
+                const T *d = val + (size_t)nrp[p] * 9;
                 const size_t i0 = (size_t)3 * q * K + c0;
                 const VT x1 = *(const VT *)(X + i0 + K), x2 = *(const VT *)(X + i0 + 2 * K);
                 y[0] += d[1] * x1;
@@ -336,17 +382,29 @@ __device__ __forceinline__ void nb_gs_tile(int pa, int pb, const int32_t *__rest
     const int b0 = nrp[pa], b1 = nrp[pb];
     const int p = pa + g;
     VT y[3] = {VT(0), VT(0), VT(0)};
-    stage16<Sh::VQ>((const char *)val, (size_t)b0 * 9 * sizeof(T), (size_t)b1 * 9 * sizeof(T), sv, lane);
-    stage16<Sh::CQ>((const char *)ncol, (size_t)b0 * 4, (size_t)b1 * 4, sc, lane);
-    wave_lds_fence();
-    const T *lv = (const T *)((const char *)sv + (((size_t)b0 * 9 * sizeof(T)) & 15));
-    const int32_t *lc = (const int32_t *)((const char *)sc + (((size_t)b0 * 4) & 15));
-    // LOWER: the blocks of earlier colours only (the later ones multiply zeros)
-    if (p < pb) nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + 1 + s, (LOWER ? split[p] : nrp[p + 1]) - b0, X, c0, y);
+    const T *d;                                                  // the node's own block (slot 0) in LDS
+    if (LOWER) {
+        // the node's own block and the blocks of earlier colours only (the later ones multiply
+        // zeros), staged node by node
+        const int m0 = p < pb ? nrp[p] : b1, m1 = p < pb ? split[p] : b1;
+        const T *lv;
+        const int32_t *lc;
+        nb_stage_node<T, K>(val, ncol, m0, m1, sv, sc, g, l, lv, lc);
+        wave_lds_fence();
+        if (p < pb) nb_accumulate<T, K>(lc, lv, 1 + s, m1 - m0, X, c0, y);
+        d = lv;
+    } else {
+        stage16<Sh::VQ>((const char *)val, (size_t)b0 * 9 * sizeof(T), (size_t)b1 * 9 * sizeof(T), sv, lane);
+        stage16<Sh::CQ>((const char *)ncol, (size_t)b0 * 4, (size_t)b1 * 4, sc, lane);
+        wave_lds_fence();
+        const T *lv = (const T *)((const char *)sv + (((size_t)b0 * 9 * sizeof(T)) & 15));
+        const int32_t *lc = (const int32_t *)((const char *)sc + (((size_t)b0 * 4) & 15));
+        if (p < pb) nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + 1 + s, nrp[p + 1] - b0, X, c0, y);
+        d = lv + (size_t)(p < pb ? nrp[p] - b0 : 0) * 9;
+    }
     nb_reduce<T, K>(y);
     if (p >= pb || s != 0) return;
     const int q = node ? node[p] : p;
-    const T *d = lv + (size_t)(nrp[p] - b0) * 9;                 // the node's own block (slot 0)
     const size_t i0 = (size_t)3 * q * K + c0;
     VT x0 = VT(0), x1 = VT(0), x2 = VT(0);
     if (!LOWER) x0 = *(const VT *)(X + i0), x1 = *(const VT *)(X + i0 + K), x2 = *(const VT *)(X + i0 + 2 * K);
