@@ -243,10 +243,10 @@ def main():
                           f"{kind}_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed", "around each launch")]
     ncol = solver.LevelColors(0)
     if gs_n > 0 and ncol > 0:
-        rooflines.append(roofline(f"k_nb_gs<{tname}, K={width}> (one colour of the level-0 node-block multicolour "
-                                  f"Gauss-Seidel; bytes averaged over the {ncol} colours of a sweep; the first "
-                                  f"launch of a sweep from x = 0, which reads no neighbours, is a separate kernel "
-                                  f"and not counted)",
+        rooflines.append(roofline(f"k_nb_gs<{tname}, K={width}> (one colour launch of the level-0 node-block "
+                                  f"multicolour Gauss-Seidel post-smoothing sweep; bytes averaged over the {ncol} "
+                                  f"colours of a sweep; the "
+                                  f"pre-smoothing sweep from x = 0 reads only earlier colours and is not counted)",
                                   gs_sweep_bytes(nnz, ndof, width, vb, ncol) / ncol, gs_ms, gs_n,
                                   f"gs_nodeblock_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed",
                                   "around the back-to-back colour launches of each sweep"))
