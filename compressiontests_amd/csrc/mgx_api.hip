@@ -159,10 +159,11 @@ struct DevNb {
 // the colour of every node, to order each position's other blocks by neighbour colour and record
 // the split.  Returns false, with nothing built, if the
 // matrix is not made of whole 3x3 node blocks or has a node of more than 27 blocks.
+// upper (with colour_dev and diag_first): only the own block and the later-coloured neighbours.
 template <typename TI>
 static bool build_nb(DevNb &nb, int32_t nrows, const int32_t *rp, const int32_t *ci, const TI *v,
                      const int32_t *node_dev, bool diag_first, bool fp32, hipStream_t s,
-                     const int32_t *colour_dev = nullptr)
+                     const int32_t *colour_dev = nullptr, bool upper = false)
 {
     nb.reset();
     MGX_REQUIRE(nrows % 3 == 0 && nrows > 0, MGX_ERR_INVALID_ARGUMENT, "node blocks need 3 rows per node");
@@ -173,7 +174,8 @@ static bool build_nb(DevNb &nb, int32_t nrows, const int32_t *rp, const int32_t 
     cnt.alloc(npos);
     bad.alloc(1);
     MGX_HIP(hipMemsetAsync(bad.p, 0, sizeof(int32_t), s));
-    nb_count(npos, node_dev, rp, ci, diag_first, cnt.p, bad.p, s);
+    upper = upper && colour_dev && diag_first;
+    nb_count(npos, node_dev, rp, ci, diag_first, cnt.p, bad.p, s, colour_dev, upper);
     int32_t hbad = 0;
     MGX_HIP(hipMemcpyAsync(&hbad, bad.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
     MGX_HIP(hipStreamSynchronize(s));
@@ -189,20 +191,20 @@ static bool build_nb(DevNb &nb, int32_t nrows, const int32_t *rp, const int32_t 
         nb.node.alloc(npos);
         MGX_HIP(hipMemcpyAsync(nb.node.p, node_dev, sizeof(int32_t) * npos, hipMemcpyDeviceToDevice, s));
     }
-    const bool bycol = colour_dev && diag_first;
+    const bool bycol = colour_dev && diag_first && !upper;
     if (bycol) nb.split.alloc(npos);
     int32_t *split = bycol ? nb.split.p : nullptr;
     const void *val;
     if (fp32) {
         nb.v32.alloc((size_t)nblk * 9 + 4);
         nb_fill<TI, float>(npos, node_dev, rp, ci, v, nb.nrp.p, diag_first, nb.ncol.p, nb.v32.p, s,
-                           bycol ? colour_dev : nullptr, split);
+                           (bycol || upper) ? colour_dev : nullptr, split, upper);
         val = nb.v32.p;
     } else {
         nb.v64.alloc((size_t)nblk * 9 + 2);
         if constexpr (std::is_same<TI, double>::value)
             nb_fill<double, double>(npos, node_dev, rp, ci, v, nb.nrp.p, diag_first, nb.ncol.p, nb.v64.p, s,
-                                    bycol ? colour_dev : nullptr, split);
+                                    (bycol || upper) ? colour_dev : nullptr, split, upper);
         val = nb.v64.p;
     }
     MGX_HIP(hipStreamSynchronize(s));
@@ -376,6 +378,7 @@ struct Level {
     DevNb nb;                              // node-block copy for the block SpMM (node order)
     DevNb nbg;                             // node-block copy for block Gauss-Seidel (colour order, own block first)
     DevNb nb64;                            // precision form, level 0: FP64 node-block copy for the true residual
+    DevNb nbu;                             // node order, own block + later-coloured neighbours (zero-start residual)
     bool nb_unfit = false;                 // the level matrix is not a node-block matrix the kernels take
     DevNt Rn, Pn;                          // node transfers of R and P (block V-cycle)
     bool nt_unfit = false;                 // R or P does not act on the dofs of a node alike
@@ -1208,7 +1211,7 @@ static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const 
     // one forward sweep from x = 0 on a colour-sorted copy: the sweep reads the earlier colours
     // only (x need not be cleared), and the residual is the pull of the later colours
     const bool tri = S->opt.smoother == MGX_SMOOTHER_GS && nu == 1 && nb_ready<T>(V.nbg) && V.nbg.view.split &&
-                     nb_ready<T>(V.nb) && V.nb.view.split;
+                     nb_ready<T>(V.nbu) && V.nbu.view.npos == V.nb.view.npos;
     if (tri) {
         gs_block_sweep<T>(S, V, K, b, x, false, done, NB_GS_LOWER);
         cur = x;
@@ -1228,7 +1231,7 @@ static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const 
         }
     }
     if (tri)
-        launch_nb_spmm<T>(K, BEPI_UPRESID, V.nb.view, cur, nullptr, nullptr, B::r(V), done, nullptr, 0, s,
+        launch_nb_spmm<T>(K, BEPI_UPRESID, V.nbu.view, cur, nullptr, nullptr, B::r(V), done, nullptr, 0, s,
                           V.tile_order_npb == nb_tile<T>(K) ? V.tile_order.p : nullptr);
     else level_spmm<T, T>(V, K, BEPI_RESID, Av, cur, b, nullptr, B::r(V), done, nullptr, s);
     Level &C = S->lv[l + 1];
@@ -1521,12 +1524,11 @@ static void order_for_cache(mgx_solver_s *S, int K)
             if (!ok) { V.Rn.reset(); V.Pn.reset(); V.nt_unfit = true; }
         }
         if (!V.node3) continue;
-        // node-block copy in node order (independent of K): built once; with a colouring, each
-        // node's own block first and the others by neighbour colour (the split serves the residual
-        // of a zero-start sweep)
+        // node-block copy in node order (independent of K): built once; with a colouring also the
+        // own-and-later-colours copy the residual of a zero-start sweep reads
         if (!V.nb.built() && !V.nb_unfit) {
-            DBuf<int32_t> colour;
-            if (V.ncolors > 0) {
+            V.nb_unfit = !build_nb<double>(V.nb, V.n, V.A.rp, V.A.ci, V.A.v, nullptr, false, mixed, st);
+            if (!V.nb_unfit && V.ncolors > 0) {
                 std::vector<int32_t> col((size_t)V.n / 3, 0);
                 for (int c = 0; c < V.ncolors; ++c) {
                     std::vector<int32_t> nodes(V.color_count[c]);
@@ -1536,12 +1538,11 @@ static void order_for_cache(mgx_solver_s *S, int K)
                     MGX_HIP(hipStreamSynchronize(st));
                     for (int32_t q : nodes) col[q] = c;
                 }
+                DBuf<int32_t> colour;
                 colour.upload(col.data(), col.size(), st);
+                if (!build_nb<double>(V.nbu, V.n, V.A.rp, V.A.ci, V.A.v, nullptr, true, mixed, st, colour.p, true))
+                    V.nbu.reset();
             }
-            V.nb_unfit = !build_nb<double>(V.nb, V.n, V.A.rp, V.A.ci, V.A.v, nullptr, colour.p != nullptr, mixed, st,
-                                           colour.p);
-            if (V.nb_unfit && colour.p)   // a node without its own block: plain column order
-                V.nb_unfit = !build_nb<double>(V.nb, V.n, V.A.rp, V.A.ci, V.A.v, nullptr, false, mixed, st);
         }
         // the precision form's FP64 true residual on level 0 reads an FP64 copy
         if (l == 0 && mixed && V.nb.built() && !V.nb64.built())

@@ -124,7 +124,8 @@ struct BCtl {
     double tol;
 };
 enum BlockEpi : int { BEPI_SPMM = 0, BEPI_ADD = 1, BEPI_RESID = 2, BEPI_JACOBI = 3, BEPI_DOT = 4,
-                      BEPI_UPRESID = 5 };   // node-block only: residual after a zero-start forward sweep
+                      BEPI_UPRESID = 5 };   // node-block only, on an own-and-later-colours copy: residual after a
+                           // zero-start forward sweep
 // partials of the block reductions: 3 slots x kMaxBlock columns x kPartialStride
 constexpr size_t kBlockPartials = (size_t)3 * kMaxBlock * kPartialStride;
 template <typename TV, typename TX>
@@ -200,14 +201,16 @@ enum NbGsStart : int { NB_GS_GENERAL = 0, NB_GS_ZERO = 1, NB_GS_LOWER = 2 };
 template <typename T>
 void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T *X, bool backward, bool reverse,
                   const int32_t *done, hipStream_t s, int start = NB_GS_GENERAL);
+// colour + upper (with diag_first): count / fill only the node's own block (slot 0) and the blocks
+// of later-coloured neighbours, in column order (the copy the zero-start sweep's residual reads)
 void nb_count(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, bool diag_first, int32_t *cnt,
-              int32_t *bad, hipStream_t s);
+              int32_t *bad, hipStream_t s, const int32_t *colour = nullptr, bool upper = false);
 // colour (per node, may be nullptr): sort each position's other blocks by the colour of their
 // neighbour node and write split (per position)
 template <typename TI, typename TV>
 void nb_fill(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, const TI *v, const int32_t *nrp,
              bool diag_first, int32_t *ncol, TV *val, hipStream_t s, const int32_t *colour = nullptr,
-             int32_t *split = nullptr);
+             int32_t *split = nullptr, bool upper = false);
 
 // node transfers: output node i, input node j, one weight for the three dofs of the node
 struct NtView {

@@ -196,7 +196,6 @@ __device__ __forceinline__ void nb_publish(const double *dot, double *red, doubl
 template <typename T, int K, int EPI>
 __global__ __launch_bounds__(kBlock) void k_nb_spmm(int32_t npos, const int32_t *__restrict__ nrp,
                                                     const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
-                                                    const int32_t *__restrict__ split,
                                                     const T *__restrict__ val, const T *__restrict__ X,
                                                     const T *__restrict__ B, const T *__restrict__ pre,
                                                     T *__restrict__ Y, const int32_t *__restrict__ done,
@@ -226,23 +225,14 @@ __global__ __launch_bounds__(kBlock) void k_nb_spmm(int32_t npos, const int32_t 
         const int p = p0 + g;
         VT y[3] = {VT(0), VT(0), VT(0)};
         wave_lds_fence();                                        // the previous tile's LDS reads are issued
-        if (EPI == BEPI_UPRESID) {
-            // the blocks of later colours only, staged node by node
-            const int m0 = p < p1 ? split[p] : b1, m1 = p < p1 ? nrp[p + 1] : b1;
-            const T *lv;
-            const int32_t *lc;
-            nb_stage_node<T, K>(val, ncol, m0, m1, s_val[wv], s_col[wv], g, l, lv, lc);
-            wave_lds_fence();
-            if (p < p1) nb_accumulate<T, K>(lc, lv, s, m1 - m0, X, c0, y);
-        } else {
-            stage16<Sh::VQ>((const char *)val, (size_t)b0 * 9 * sizeof(T), (size_t)b1 * 9 * sizeof(T), s_val[wv],
-                            lane);
-            stage16<Sh::CQ>((const char *)ncol, (size_t)b0 * 4, (size_t)b1 * 4, s_col[wv], lane);
-            wave_lds_fence();
-            const T *lv = (const T *)((const char *)s_val[wv] + (((size_t)b0 * 9 * sizeof(T)) & 15));
-            const int32_t *lc = (const int32_t *)((const char *)s_col[wv] + (((size_t)b0 * 4) & 15));
-            if (p < p1) nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + s, nrp[p + 1] - b0, X, c0, y);
-        }
+        stage16<Sh::VQ>((const char *)val, (size_t)b0 * 9 * sizeof(T), (size_t)b1 * 9 * sizeof(T), s_val[wv], lane);
+        stage16<Sh::CQ>((const char *)ncol, (size_t)b0 * 4, (size_t)b1 * 4, s_col[wv], lane);
+        wave_lds_fence();
+        const T *lv = (const T *)((const char *)s_val[wv] + (((size_t)b0 * 9 * sizeof(T)) & 15));
+        const int32_t *lc = (const int32_t *)((const char *)s_col[wv] + (((size_t)b0 * 4) & 15));
+        // UPRESID (own-and-later-colours copy): slot 0 is the node's own block, not part of the sum
+        if (p < p1)
+            nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + (EPI == BEPI_UPRESID ? 1 : 0) + s, nrp[p + 1] - b0, X, c0, y);
         nb_reduce<T, K>(y);
         if (p < p1 && s == 0) {
             const int q = node ? node[p] : p;
@@ -250,7 +240,7 @@ __global__ __launch_bounds__(kBlock) void k_nb_spmm(int32_t npos, const int32_t 
                 // after a forward sweep from X = 0 (colours in order, each node's dofs 0, 1, 2 in
                 // order) rows are left with the pull of what was relaxed after them: the later
                 // colours and, inside the node, the strictly upper part of its own block (slot 0)
-                const T *d = val + (size_t)nrp[p] * 9;
+                const T *d = lv + (size_t)(nrp[p] - b0) * 9;
                 const size_t i0 = (size_t)3 * q * K + c0;
                 const VT x1 = *(const VT *)(X + i0 + K), x2 = *(const VT *)(X + i0 + 2 * K);
                 y[0] += d[1] * x1;
@@ -308,12 +298,10 @@ void launch_nb_spmm(int K, int epi, const NbView &A, const T *X, const T *B, con
                     double *partials, int stride, hipStream_t s, const int32_t *order)
 {
     if (A.npos <= 0) return;
-    if (epi == BEPI_UPRESID && !A.split)
-        throw Error{MGX_ERR_INVALID_OPERATION, "the residual of a zero-start sweep needs a colour-sorted copy"};
     const int g = nb_spmm_grid<T>(K, A.npos);
 #define MGX_NBS(KK, EE)                                                                                          \
     hipLaunchKernelGGL((k_nb_spmm<T, KK, EE>), dim3(g), dim3(kBlock), 0, s, A.npos, A.nrp, A.ncol, A.node,     \
-                       A.split, (const T *)A.val, X, B, pre, Y, done, partials, stride, order)
+                       (const T *)A.val, X, B, pre, Y, done, partials, stride, order)
 #define MGX_NBS_K(KK)                                       \
     switch (epi) {                                          \
     case BEPI_SPMM: MGX_NBS(KK, BEPI_SPMM); break;          \
@@ -520,8 +508,10 @@ template void launch_nb_gs<double>(int, const NbView &, int32_t, int32_t, const 
 // triples, when it acts on the three dofs of a node alike: entry (3i+a, 3j+b) = w_ij if a == b,
 // else 0.  Stored per node: rows i of the output level, columns j of the input level, one weight
 // each (a third of the CSR's entries, and no dof column indices).  Y[3i+a][:] (+)= sum_j w_ij
-// X[3j+a][:]: the 3K values of a node are contiguous in the row-major n x K blocks, so thread
-// (i, q) owns the 16-byte vector q of output node i and reads vector q of each input node.
+// X[3j+a][:].  Thread (i, v) owns the 16-byte column vector v (columns v*W .. v*W+W-1) of the three
+// rows of output node i and reads the same vector of the three rows of each input node, 4 entries
+// in flight: V = K / W threads per node, so a wave covers 64 / V consecutive output nodes and each
+// row segment it reads or writes is V * 16 contiguous bytes.
 template <typename T, int K, bool ADD>
 __global__ __launch_bounds__(kBlock) void k_nt_apply(int32_t nout, const int32_t *__restrict__ rp,
                                                      const int32_t *__restrict__ ci, const T *__restrict__ w,
@@ -530,30 +520,42 @@ __global__ __launch_bounds__(kBlock) void k_nt_apply(int32_t nout, const int32_t
 {
     using VT = typename NbVec<T>::type;
     constexpr int W = 16 / (int)sizeof(T);
-    constexpr int Q = 3 * K / W;                                 // vectors per node
+    constexpr int V = K / W;                                     // threads per node
     if (done && *done) return;
     const int64_t t = (int64_t)xcd_remap(blockIdx.x, gridDim.x) * kBlock + threadIdx.x;
-    const int64_t i = t / Q;
+    const int64_t i = t / V;
     if (i >= nout) return;
-    const int q = (int)(t - i * Q);
+    const int c0 = (int)(t - i * V) * W;
     const int b1 = rp[i + 1];
     int b = rp[i];
-    VT acc = VT(0);
+    VT acc[3] = {VT(0), VT(0), VT(0)};
     for (; b + 3 < b1; b += 4) {
         int j[4];
         T wv[4];
-        VT x[4];
+        VT x[4][3];
 #pragma unroll
         for (int u = 0; u < 4; ++u) { j[u] = ci[b + u]; wv[u] = w[b + u]; }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) x[u] = *(const VT *)(X + (size_t)j[u] * (3 * K) + q * W);
+        for (int u = 0; u < 4; ++u)
 #pragma unroll
-        for (int u = 0; u < 4; ++u) acc += wv[u] * x[u];
+            for (int a = 0; a < 3; ++a) x[u][a] = *(const VT *)(X + ((size_t)3 * j[u] + a) * K + c0);
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int a = 0; a < 3; ++a) acc[a] += wv[u] * x[u][a];
     }
-    for (; b < b1; ++b) acc += w[b] * *(const VT *)(X + (size_t)ci[b] * (3 * K) + q * W);
-    VT *yp = (VT *)(Y + (size_t)i * (3 * K) + q * W);
-    if (ADD) *yp = *yp + acc;
-    else *yp = acc;
+    for (; b < b1; ++b) {
+        const int j = ci[b];
+        const T wv = w[b];
+#pragma unroll
+        for (int a = 0; a < 3; ++a) acc[a] += wv * *(const VT *)(X + ((size_t)3 * j + a) * K + c0);
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        VT *yp = (VT *)(Y + ((size_t)3 * i + a) * K + c0);
+        if (ADD) *yp = *yp + acc[a];
+        else *yp = acc[a];
+    }
 }
 
 template <typename T>
@@ -561,7 +563,7 @@ void launch_nt_apply(int K, bool add, const NtView &M, const T *X, T *Y, const i
 {
     if (M.nout <= 0) return;
     const int W = 16 / (int)sizeof(T);
-    const int g = grid_rows((int64_t)M.nout * (3 * K / W), kBlock);
+    const int g = grid_rows((int64_t)M.nout * (K / W), kBlock);
 #define MGX_NTA(KK, AA)                                                                                          \
     hipLaunchKernelGGL((k_nt_apply<T, KK, AA>), dim3(g), dim3(kBlock), 0, s, M.nout, M.rp, M.ci, (const T *)M.w, \
                        X, Y, done)
@@ -648,9 +650,12 @@ template void nt_fill<double>(int32_t, const int32_t *, const int32_t *, const d
 // node (3j, 3j+1, 3j+2).  Bit 0 of *bad
 // on any violation (with diag_first, also when a node has no block of its own); bit 1 when a node
 // has more blocks than the kernels stage per node (kNbMaxBlocks).
+// upper (with diag_first and colour): count only the node's own block and the blocks of
+// neighbours coloured after it.
 __global__ __launch_bounds__(kBlock) void k_nb_count(int32_t npos, const int32_t *__restrict__ node,
                                                      const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
-                                                     int diag_first, int32_t *__restrict__ cnt, int32_t *__restrict__ bad)
+                                                     int diag_first, int32_t *__restrict__ cnt, int32_t *__restrict__ bad,
+                                                     const int32_t *__restrict__ colour, int upper)
 {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= npos) return;
@@ -666,7 +671,12 @@ __global__ __launch_bounds__(kBlock) void k_nb_count(int32_t npos, const int32_t
     for (int k = 0; ok && k < len; ++k) ok = ci[s0 + len + k] == ci[s0 + k] && ci[s0 + 2 * len + k] == ci[s0 + k];
     if (!ok || (diag_first && !own)) atomicOr(bad, 1);
     if (len / 3 > kNbMaxBlocks) atomicOr(bad, 2);
-    cnt[p] = ok ? len / 3 : 0;
+    int n = ok ? len / 3 : 0;
+    if (upper && ok) {
+        n = 1;
+        for (int m = 0; m < len / 3; ++m) n += colour[ci[s0 + 3 * m] / 3] > colour[q] ? 1 : 0;
+    }
+    cnt[p] = n;
 }
 
 // Slots of a position's blocks: with diag_first its own block first; with colour (diag_first
@@ -678,7 +688,8 @@ __global__ __launch_bounds__(kBlock) void k_nb_fill(int32_t npos, const int32_t 
                                                     const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                     const TI *__restrict__ v, const int32_t *__restrict__ nrp,
                                                     int diag_first, int32_t *__restrict__ ncol, TV *__restrict__ val,
-                                                    const int32_t *__restrict__ colour, int32_t *__restrict__ split)
+                                                    const int32_t *__restrict__ colour, int32_t *__restrict__ split,
+                                                    int upper)
 {
     const int p = blockIdx.x * kBlock + threadIdx.x;
     if (p >= npos) return;
@@ -691,7 +702,22 @@ __global__ __launch_bounds__(kBlock) void k_nb_fill(int32_t npos, const int32_t 
     const int r0 = rp[3 * q];
     int own = -1;
     if (diag_first)
-        for (int m = 0; m < nb; ++m) if (ci[r0 + 3 * m] == 3 * q) own = m;
+        for (int m = 0; m < (upper ? (rp[3 * q + 1] - r0) / 3 : nb); ++m) if (ci[r0 + 3 * m] == 3 * q) own = m;
+    if (upper) {
+        // the own block, then the later-coloured neighbours in column order
+        const int len = (rp[3 * q + 1] - r0) / 3, cq = colour[q];
+        int slot = nrp[p];
+        for (int m = 0; m < len; ++m) {
+            if (m != own && colour[ci[r0 + 3 * m] / 3] <= cq) continue;
+            const int sl = m == own ? nrp[p] : ++slot;
+            ncol[sl] = ci[r0 + 3 * m] / 3;
+#pragma unroll
+            for (int a = 0; a < 3; ++a)
+#pragma unroll
+                for (int c = 0; c < 3; ++c) val[(size_t)sl * 9 + 3 * a + c] = (TV)v[rp[3 * q + a] + 3 * m + c];
+        }
+        return;
+    }
     const bool bycol = colour && own >= 0;
     if (bycol) {
         const int cq = colour[q];
@@ -726,31 +752,32 @@ __global__ __launch_bounds__(kBlock) void k_nb_fill(int32_t npos, const int32_t 
 }
 
 void nb_count(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, bool diag_first, int32_t *cnt,
-              int32_t *bad, hipStream_t s)
+              int32_t *bad, hipStream_t s, const int32_t *colour, bool upper)
 {
     if (npos <= 0) return;
     hipLaunchKernelGGL(k_nb_count, dim3((npos + kBlock - 1) / kBlock), dim3(kBlock), 0, s, npos, node, rp, ci,
-                       diag_first ? 1 : 0, cnt, bad);
+                       diag_first ? 1 : 0, cnt, bad, colour, upper ? 1 : 0);
     MGX_HIP(hipGetLastError());
 }
 
 template <typename TI, typename TV>
 void nb_fill(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, const TI *v, const int32_t *nrp,
-             bool diag_first, int32_t *ncol, TV *val, hipStream_t s, const int32_t *colour, int32_t *split)
+             bool diag_first, int32_t *ncol, TV *val, hipStream_t s, const int32_t *colour, int32_t *split,
+             bool upper)
 {
     if (npos <= 0) return;
     hipLaunchKernelGGL((k_nb_fill<TI, TV>), dim3((npos + kBlock - 1) / kBlock), dim3(kBlock), 0, s, npos, node, rp, ci,
-                       v, nrp, diag_first ? 1 : 0, ncol, val, colour, split);
+                       v, nrp, diag_first ? 1 : 0, ncol, val, colour, split, upper ? 1 : 0);
     MGX_HIP(hipGetLastError());
 }
 template void nb_fill<double, double>(int32_t, const int32_t *, const int32_t *, const int32_t *, const double *,
                                       const int32_t *, bool, int32_t *, double *, hipStream_t, const int32_t *,
-                                      int32_t *);
+                                      int32_t *, bool);
 template void nb_fill<double, float>(int32_t, const int32_t *, const int32_t *, const int32_t *, const double *,
                                      const int32_t *, bool, int32_t *, float *, hipStream_t, const int32_t *,
-                                     int32_t *);
+                                     int32_t *, bool);
 template void nb_fill<float, float>(int32_t, const int32_t *, const int32_t *, const int32_t *, const float *,
                                     const int32_t *, bool, int32_t *, float *, hipStream_t, const int32_t *,
-                                    int32_t *);
+                                    int32_t *, bool);
 
 }  // namespace mgx
