@@ -2513,6 +2513,20 @@ __global__ __launch_bounds__(kBlock) void k_copy16(const uint4 *__restrict__ src
     for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) dst[i] = src[i];
 }
 
+int mgx_dense_block_apply(int32_t nc, const double *M, int32_t k, const void *X, void *Y, int32_t fp32, void *stream)
+{
+    return guard([&] {
+        MGX_REQUIRE(nc > 0 && M && X && Y, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        MGX_REQUIRE(k == 4 || k == 8 || k == 16 || k == 32, MGX_ERR_INVALID_ARGUMENT, "k must be 4, 8, 16 or 32");
+        hipStream_t s = (hipStream_t)stream;
+        DBuf<double> scratch;
+        scratch.alloc(dense_block_scratch(nc, k));
+        if (fp32) launch_dense_block<float>(k, nc, M, (const float *)X, (float *)Y, scratch.p, nullptr, s);
+        else launch_dense_block<double>(k, nc, M, (const double *)X, (double *)Y, scratch.p, nullptr, s);
+        MGX_HIP(hipStreamSynchronize(s));
+    });
+}
+
 int mgx_time_copy_kernel(const void *src, void *dst, uint64_t bytes, int32_t warmup, int32_t repeats, double *avg_ms)
 {
     return guard([&] {

@@ -923,16 +923,57 @@ __global__ __launch_bounds__(kBlock) void k_dense_sum(int64_t nk, const double *
     }
 }
 
+// The same product on the FP64 matrix cores: v_mfma_f64_16x16x4f64 (A 16 x 4, B 4 x 16, C 16 x 16;
+// lane l holds A[l % 16][l / 16], B[l / 16][l % 16] and C[l / 16 + 4 i][l % 16], i < 4).  One
+// wave per 16 x 16 output tile (rows of M x columns of X) and inner slice of kDenseSplit; every
+// load is guarded against the matrix edge (zeros past it), and the split sums go to the same
+// partial layout k_dense_sum adds.  X is widened to FP64 as it is loaded.
+typedef double dbl4 __attribute__((ext_vector_type(4)));
+template <typename TX, int K>
+__global__ __launch_bounds__(64) void k_dense_mfma(int32_t nc, const double *__restrict__ M,
+                                                   const TX *__restrict__ X, double *__restrict__ part,
+                                                   const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    const int l = threadIdx.x;
+    const int r0 = blockIdx.x * 16, c0 = blockIdx.y * 16, sp = blockIdx.z;
+    const int chunk = ((nc + kDenseSplit - 1) / kDenseSplit + 3) & ~3;
+    const int j0 = sp * chunk, j1 = min(nc, j0 + chunk);
+    const int ra = r0 + (l & 15), ka = l >> 4, cb = c0 + (l & 15);
+    const bool rowok = ra < nc;
+    const double *mrow = M + (size_t)(rowok ? ra : 0) * nc;
+    dbl4 acc = {0, 0, 0, 0};
+    for (int j = j0; j < j1; j += 4) {
+        const int k = j + ka;
+        const double a = (rowok && k < j1) ? mrow[k] : 0.0;
+        const double b = (k < j1) ? (double)X[(size_t)k * K + cb] : 0.0;
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(a, b, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = r0 + (l >> 4) + 4 * i;
+        if (r < nc) part[((size_t)sp * nc + r) * K + cb] = acc[i];
+    }
+}
+
 size_t dense_block_scratch(int32_t nc, int K) { return (size_t)kDenseSplit * nc * K; }
 
 template <typename TX>
 void launch_dense_block(int K, int32_t nc, const double *M, const TX *X, TX *Y, double *scratch, const int32_t *done,
                         hipStream_t s)
 {
-    const dim3 grid((nc + kDenseRows - 1) / kDenseRows, kDenseSplit);
-#define L_(KK) hipLaunchKernelGGL((k_dense_split<TX, KK>), grid, dim3(kBlock), 0, s, nc, M, X, scratch, done)
-    MGX_K_SWITCH(K, L_)
+    if (K >= 16) {
+        const dim3 grid((nc + 15) / 16, K / 16, kDenseSplit);
+#define L_(KK) hipLaunchKernelGGL((k_dense_mfma<TX, KK>), grid, dim3(64), 0, s, nc, M, X, scratch, done)
+        if (K == 16) L_(16);
+        else L_(32);
 #undef L_
+    } else {
+        const dim3 grid((nc + kDenseRows - 1) / kDenseRows, kDenseSplit);
+#define L_(KK) hipLaunchKernelGGL((k_dense_split<TX, KK>), grid, dim3(kBlock), 0, s, nc, M, X, scratch, done)
+        MGX_K_SWITCH(K, L_)
+#undef L_
+    }
     MGX_HIP(hipGetLastError());
     const int64_t nk = (int64_t)nc * K;
     hipLaunchKernelGGL((k_dense_sum<TX>), dim3(grid_stream(nk)), dim3(kBlock), 0, s, nk, scratch, Y, done);

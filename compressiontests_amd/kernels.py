@@ -79,6 +79,14 @@ class NodeTransfer:
             pass
 
 
+def dense_block_apply(M, X, Y, k, stream=None):
+    """Y = M X for the dense FP64 coarse inverse M (nc x nc DeviceArray, row-major) and nc x k
+    row-major blocks X, Y (float32 or float64): the batching form's coarse solve."""
+    nc = int(round(np.sqrt(M.n)))
+    assert M.dtype == np.float64 and nc * nc == M.n and X.dtype == Y.dtype and X.n == nc * k and Y.n == nc * k
+    L.call("mgx_dense_block_apply", nc, M.vp, k, X.vp, Y.vp, int(X.dtype == np.float32), stream)
+
+
 def residual(A, b, x, r, stream=None):
     """r = b - A x.  CsrGeometricMultigrid.cl:96-110."""
     L.call("mgx_csr_residual", A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), b.vp, x.vp, r.vp,

@@ -359,6 +359,11 @@ int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, cons
  * with bit 0; takes bit 2, not bit 1). */
 int mgx_time_spmm(int32_t flags, int32_t k, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
                   const void *X, void *Y, int32_t warmup, int32_t repeats, double *avg_ms);
+/* Coarse direct solve applied to k columns at once: Y = M X with M the dense FP64 inverse of the
+ * coarsest matrix (nc x nc, row-major, device) and X, Y nc x k row-major device blocks (FP32 with
+ * fp32 = 1, else FP64); the batching form's coarse step (the reference solves the coarsest system
+ * directly for one right-hand side, OpenCLCsrGeometricMultigridSolver.cs:364-367).  Synchronous. */
+int mgx_dense_block_apply(int32_t nc, const double *M, int32_t k, const void *X, void *Y, int32_t fp32, void *stream);
 /* Average device time of a plain copy kernel (16-byte vector loads and stores) moving `bytes`
  * (a multiple of 16) from src to dst: the calibration of HBM byte counters. */
 int mgx_time_copy_kernel(const void *src, void *dst, uint64_t bytes, int32_t warmup, int32_t repeats, double *avg_ms);

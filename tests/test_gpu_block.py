@@ -188,6 +188,27 @@ def test_nodetransfer_rejects_other_matrices(het16):
         kernels.NodeTransfer(DeviceCsr.arrays_from_host(short))
 
 
+@pytest.mark.parametrize("nc", [6, 257, 1944])
+def test_dense_block_apply_matches_numpy(nc):
+    """The batching form's coarse solve Y = M X (FP64 matrix cores from K = 16 on, a split-K FMA
+    kernel below): against numpy for every K, FP64 blocks to 1e-13 of max|y|, FP32 blocks to 1e-6;
+    nc = 6 and 257 leave partial 16-row tiles and inner slices."""
+    from compressiontests_amd import kernels
+    from compressiontests_amd.device import DeviceArray
+    Mh = O.rng_normal(5 + nc, nc * nc).reshape(nc, nc)
+    M = DeviceArray.from_numpy(Mh.reshape(-1))
+    for k in (4, 8, 16, 32):
+        xh = O.rng_normal(9 + k, nc * k).reshape(nc, k)
+        ref = Mh @ xh
+        for dt, tol in ((np.float64, 1e-13), (np.float32, 1e-6)):
+            X = DeviceArray.from_numpy(xh.reshape(-1).astype(dt))
+            Y = DeviceArray(nc * k, dt)
+            kernels.dense_block_apply(M, X, Y, k)
+            ref_t = Mh @ xh.astype(dt).astype(np.float64)
+            err = np.max(np.abs(Y.numpy().reshape(nc, k).astype(np.float64) - ref_t))
+            assert err <= tol * np.max(np.abs(ref)), (nc, k, dt, err)
+
+
 def test_load_cases_shape_and_norm(het16):
     m, F, lv, A = het16
     assert F.shape == (6, m.NumDofsFree)
