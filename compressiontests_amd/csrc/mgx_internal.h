@@ -189,7 +189,10 @@ struct NbView {
 template <typename T>
 int nb_tile(int K);                  // positions per wave tile
 template <typename T>
-int nb_spmm_grid(int K, int32_t npos);
+int nb_spmm_grid(int K, int32_t npos);       // bounded grid (waves walk several tiles)
+template <typename T>
+int nb_spmm_full_grid(int K, int32_t npos);  // one tile per wave
+
 template <typename T>
 void launch_nb_spmm(int K, int epi, const NbView &A, const T *X, const T *B, const T *pre, T *Y, const int32_t *done,
                     double *partials, int stride, hipStream_t s, const int32_t *order = nullptr);

@@ -293,12 +293,27 @@ int nb_spmm_grid(int K, int32_t npos)
 template int nb_spmm_grid<float>(int, int32_t);
 template int nb_spmm_grid<double>(int, int32_t);
 
+// one tile per wave, blocks dispatched in tile order
+template <typename T>
+int nb_spmm_full_grid(int K, int32_t npos)
+{
+    const int64_t ntiles = (npos + nb_tile<T>(K) - 1) / nb_tile<T>(K);
+    return grid_rows(ntiles, kNbWaves);
+}
+template int nb_spmm_full_grid<float>(int, int32_t);
+template int nb_spmm_full_grid<double>(int, int32_t);
+
+// One tile per wave (blocks dispatched in tile order), except the DOT form: it runs the bounded
+// grid of nb_spmm_grid (waves walk several tiles), whose block count is the number of partials
+// the caller's finalize reads.  (One tile per wave was faster for the other forms, 0.75 against
+// 0.91 ms for the 128^3 residual, and slower for DOT, whose per-block publication then
+// dominates: 1.40 against 1.31 ms with a second reduction stage.)
 template <typename T>
 void launch_nb_spmm(int K, int epi, const NbView &A, const T *X, const T *B, const T *pre, T *Y, const int32_t *done,
                     double *partials, int stride, hipStream_t s, const int32_t *order)
 {
     if (A.npos <= 0) return;
-    const int g = nb_spmm_grid<T>(K, A.npos);
+    const int g = epi == BEPI_DOT ? nb_spmm_grid<T>(K, A.npos) : nb_spmm_full_grid<T>(K, A.npos);
 #define MGX_NBS(KK, EE)                                                                                          \
     hipLaunchKernelGGL((k_nb_spmm<T, KK, EE>), dim3(g), dim3(kBlock), 0, s, A.npos, A.nrp, A.ncol, A.node,     \
                        (const T *)A.val, X, B, pre, Y, done, partials, stride, order)
