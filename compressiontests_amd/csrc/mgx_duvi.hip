@@ -37,6 +37,7 @@ constexpr int kDwRows = 8;                     // rows per wave tile
 constexpr int kDwLanes = 8;                    // lanes per row
 constexpr int kDwSeg = 12;                     // slots per lane per round (96 per row and round)
 constexpr int kDwCap = 768;                    // slots staged per wave tile
+constexpr int kDwDistNI = (kDwCap + 32 + 1023) / 1024;   // 64-slot rounds of 16 bytes for the distance bytes
 constexpr int kDwWaves = kBlock / 64;
 
 __device__ __forceinline__ void dw_fence()
@@ -93,9 +94,9 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
                                                       double *__restrict__ y, int clear, const int32_t *__restrict__ done)
 {
     __shared__ dw_u32x4 s_iq[kDwWaves][64 * DwIdx<VIB>::NI];  // value indices (not used in raw mode)
-    __shared__ dw_u32x4 s_dq[kDwWaves][64];                   // distance bytes
+    __shared__ dw_u32x4 s_dq[kDwWaves][64 * kDwDistNI];       // distance bytes
     __shared__ int32_t s_h[kDwWaves][kDwCap];                 // column + 1 at a block's first slot, else 0
-    static_assert(64 * 16 >= kDwCap + 32, "distance staging");
+    static_assert(64 * 16 * kDwDistNI >= kDwCap + 32, "distance staging");
     if (done && *done) return;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane / kDwLanes, l = lane % kDwLanes;
     const int ntiles = (nrows + kDwRows - 1) / kDwRows;
@@ -103,8 +104,13 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
     if (tile >= ntiles) return;                                // the whole wave leaves together
     const int r0 = tile * kDwRows, r1 = min(r0 + kDwRows, nrows);
     const int row = r0 + r;
+    // every offset the tile needs, loaded together (one global round trip)
     const int64_t d0 = row_dist[r0];
-    const int cnt = (int)(row_dist[r1] - d0);
+    const int64_t dr1 = row_dist[r1];
+    int64_t ds = d0, de = d0;                                  // an empty range past the last row
+    int rb0 = 0, rb1 = 0;
+    if (row < r1) { ds = row_dist[row]; de = row_dist[row + 1]; rb0 = row_blocks[row]; rb1 = row_blocks[row + 1]; }
+    const int cnt = (int)(dr1 - d0);
     if (cnt > kDwCap) {
         // long tile: one lane per row, decoding straight from global memory
         if (l == 0 && row < r1) {
@@ -123,11 +129,11 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
         }
         return;
     }
-    int s = 0, e = 0, rb0 = 0, rb1 = 0;
-    if (row < r1) {
-        s = (int)(row_dist[row] - d0); e = (int)(row_dist[row + 1] - d0);
-        rb0 = row_blocks[row]; rb1 = row_blocks[row + 1];
-    }
+    const int s = (int)(ds - d0), e = (int)(de - d0);
+    // the first two rounds of block columns (up to 2 * kDwLanes blocks per row), loaded before
+    // the staging so that the chain walk below starts without a global round trip
+    const int bcA = rb0 + l < rb1 ? block_col[rb0 + l] : 0;
+    const int bcB = rb0 + kDwLanes + l < rb1 ? block_col[rb0 + kDwLanes + l] : 0;
     // (1) raw values of this lane's slots in round 0 (they do not depend on the decode); the
     //     index stream and the distance bytes to LDS; cleared start marks
     double w0[kDwSeg];
@@ -137,7 +143,7 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
         for (int j = 0; j < kDwSeg; ++j) w0[j] = a + j < e ? values[d0 + a + j] : 0.0;
     }
     if (VIB != 0) dw_stage<DwIdx<VIB>::NI>((const char *)vi, d0 * VIB, (d0 + cnt) * VIB, nnz * VIB, s_iq[wv], lane);
-    dw_stage<1>((const char *)dist, d0, d0 + cnt, nnz, s_dq[wv], lane);
+    dw_stage<kDwDistNI>((const char *)dist, d0, d0 + cnt, nnz, s_dq[wv], lane);
     for (int k = lane; k < cnt; k += 64) s_h[wv][k] = 0;
     const uint8_t *sd = (const uint8_t *)s_dq[wv] + (d0 & 15);               // sd[k]: distance byte of slot d0 + k
     const uint8_t *si = (const uint8_t *)s_iq[wv] + ((d0 * (VIB ? VIB : 1)) & 15);
@@ -146,8 +152,8 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
     //     lane); the block columns come 8 at a time, one per lane
     {
         int d = s;
-        for (int bw = rb0; bw < rb1; bw += kDwLanes) {
-            const int bc = bw + l < rb1 ? block_col[bw + l] : 0;
+        for (int bw = rb0, it = 0; bw < rb1; bw += kDwLanes, ++it) {
+            const int bc = it == 0 ? bcA : it == 1 ? bcB : (bw + l < rb1 ? block_col[bw + l] : 0);
             const int nb = min(kDwLanes, rb1 - bw);
             for (int j = 0; j < nb; ++j) {
                 const int c = __shfl(bc, j, kDwLanes);
