@@ -1405,16 +1405,20 @@ static void solve_block_mixed(mgx_solver_s *S, int K, int j0, int kc, const doub
     std::vector<int> iters(K, 0), outer_n(K, 0), status(K, 0), failed(K, 0), stalled(K, 0);
     std::vector<double> best_r2(K, INFINITY);
     int steps = 0;
+    BCtl o;
     for (;;) {
         BCtl outer = block_ctl_init(K, kc, S->opt.stop, S->opt.tolerance, 1);
         for (int j = 0; j < kc; ++j) outer.frozen[j] = failed[j] != 0;
         put_ctl(S, S->bctl.p, outer);
-        level_spmm<double, double>(V, K, BEPI_RESID, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bR64.p, nullptr, nullptr,
-                                   s);
+        // FP64 true residual (b itself while x is still the zero start)
+        if (steps == 0 && !has_x0) launch_bcopy<double>(nk, S->bB64.p, S->bR64.p, nullptr, s);
+        else
+            level_spmm<double, double>(V, K, BEPI_RESID, V.A.v, S->bX64.p, S->bB64.p, nullptr, S->bR64.p, nullptr,
+                                       nullptr, s);
         launch_bnorms<double>(K, n, S->bB64.p, S->bR64.p, S->bctl.p, S->bpartials.p, s);
         MGX_HIP(hipMemcpyAsync(S->bctl_host, S->bctl.p, sizeof(BCtl), hipMemcpyDeviceToHost, s));
         MGX_HIP(hipStreamSynchronize(s));
-        const BCtl o = *S->bctl_host;
+        o = *S->bctl_host;
         for (int j = 0; j < kc; ++j) if (!failed[j]) status[j] = o.status[j];
         if (o.done || steps >= S->opt.max_iterations) break;
         // per-column stagnation of the FP64 true residual (see solve_mixed): such a column stops
@@ -1445,7 +1449,14 @@ static void solve_block_mixed(mgx_solver_s *S, int K, int j0, int kc, const doub
         launch_axpy_f32_into_f64(nk, S->bD32.p, S->bX64.p, nullptr, s);
         ++steps;
     }
-    for (int j = 0; j < kc; ++j) { st[j].iterations = iters[j]; st[j].outer_iterations = outer_n[j]; st[j].converged = status[j]; }
+    // the last true residual above is the one of the returned x
+    for (int j = 0; j < kc; ++j) {
+        st[j].iterations = iters[j];
+        st[j].outer_iterations = outer_n[j];
+        st[j].converged = status[j];
+        st[j].residual_norm = std::sqrt(o.rnorm2[j]);
+        st[j].rel_residual = o.bnorm2[j] > 0 ? std::sqrt(o.rnorm2[j] / o.bnorm2[j]) : std::sqrt(o.rnorm2[j]);
+    }
 }
 
 // true FP64 residual of every column of the block
@@ -1489,7 +1500,9 @@ static void solve_blocks(mgx_solver_s *S, const double *b, double *x, bool has_x
         float ms = 0;
         MGX_HIP(hipEventElapsedTime(&ms, S->ev0, S->ev1));
         launch_unpack_cols(n, K, kc, S->bX64.p, x + (size_t)j0 * n, n, s);
-        finish_stats_block(S, K, kc, st.data());
+        // the precision form ends on an FP64 true residual of x already; the FP64 block PCG's
+        // recurrence residual is not one
+        if (S->opt.precision != MGX_PRECISION_MIXED) finish_stats_block(S, K, kc, st.data());
         for (int j = 0; j < kc; ++j) {
             st[j].solve_ms = ms;
             st[j].spmv_launches = S->bprof_n;
