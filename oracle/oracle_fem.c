@@ -13,7 +13,7 @@
  *   IGeometricMultigridModel.cs:54-58,76-194  coarsening + restriction/interpolation
  *
  * Mesh conventions (the unvendored UniformCartesianMesh3D is not in the
- * reference; these are the assumptions, documented in DESIGN.md):
+ * reference; these are assumptions, the only ones consistent with the top3d Ke):
  *   node id n = ix + (nx+1)*(iy + (ny+1)*iz); element e = ex + nx*(ey + ny*ez);
  *   element corner order (0,0,0),(1,0,0),(1,1,0),(0,1,0),(0,0,1),(1,0,1),(1,1,1),(0,1,1);
  *   dofs (ux,uy,uz) per node; coordinates = index * (L / N).
