@@ -11,15 +11,20 @@
 // array = the identity), blocks nrp[p] .. nrp[p+1]-1 of position p, ncol[b] = the neighbour node
 // of block b, val[9b + 3a + c] = entry (row 3q+a, column 3j+c).  The SpMM copy is in node order
 // with the blocks of a node in column order; the Gauss-Seidel copy lists the nodes colour by
-// colour with each node's own (diagonal) block first.  val carries 16 bytes of slack at the end
-// (the staging loads below round the tile's byte range out to 16-byte boundaries).
+// colour with each node's blocks as [own, earlier colours, later colours] and split[p] between
+// the last two; the residual copy of a zero-start sweep is in node order with only [own, later
+// colours].  val carries 16 bytes of slack at the end (the staging loads below round a byte range
+// out to 16-byte boundaries).
 //
 // Kernels: a wavefront owns a tile of NW consecutive positions.  It copies the tile's blocks
-// (one contiguous range of ncol and of val) into its own LDS region with coalesced 16-byte
-// non-temporal loads, then G lanes handle one node: lane (s, v) takes the W columns v*W ..
-// v*W+W-1 (one 16-byte vector of X per dof row) and every S-th block of the node.  Waves proceed
-// independently (no block-wide barrier); LDS operations of one wavefront execute in order, so a
+// (one contiguous range of ncol and of val; node by node where only part of each node's blocks
+// is read) into its own LDS region with coalesced 16-byte non-temporal loads, then G lanes handle
+// one node: lane (s, v) takes the W columns v*W .. v*W+W-1 (one 16-byte vector of X per dof row)
+// and every S-th block of the node.  Waves proceed independently (no block-wide barrier, except
+// the final partials of the DOT form); LDS operations of one wavefront execute in order, so a
 // compiler barrier plus a wait for the wave's own LDS writes separates staging from use.
+//
+// Also here: the node transfers (R, P acting on the three dofs of a node alike).
 #include <type_traits>
 
 #include "mgx_device.h"
