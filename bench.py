@@ -17,7 +17,7 @@ FP64 GPU path (one right-hand side at a time) timed on the same system, the larg
 error of the batched solutions against FP64 solves to 1e-12 of every load case, and the
 autotuner's timings; roofline = the kernel with the most device time in profiled solves of the
 same batch (HIP events on the solver stream around each SpMM launch and around each level-0
-Gauss-Seidel sweep of colour launches), roofline_next = the other of
+Gauss-Seidel sweep of launches, one per colour pair or per colour), roofline_next = the other of
 the level-0 Gauss-Seidel and SpMM; cpu_baseline = the oracle's single-threaded C MG-PCG on the
 same hierarchy (a bounded sample of its iterations).
 
@@ -55,15 +55,16 @@ def spmm_bytes(nnz, n, k, val_bytes, layout="csr"):
     return (val_bytes + 4) * nnz + 4 * (n + 1) + 2 * val_bytes * k * n
 
 
-def gs_sweep_bytes(nnz, n, k, val_bytes, ncolors):
-    """Algorithmic bytes of the ncolors launches of one node-block multicolour Gauss-Seidel
-    sweep with k right-hand sides: the blocks (a value per nonzero, a column per block, the node
-    offsets and node map) once over the sweep; per colour launch all of X (the colour's nodes
-    have neighbours of every other colour, and their own rows are read for the update); B read
-    and X written once over the sweep (each colour its own rows)."""
+def gs_sweep_bytes(nnz, n, k, val_bytes, launches):
+    """Algorithmic bytes of one node-block multicolour Gauss-Seidel sweep with k right-hand sides
+    made of `launches` launches (one per colour, or one per colour pair): the blocks (a value per
+    nonzero, a column per block, the node offsets and node map) once over the sweep; per launch
+    all of X (a colour's nodes have neighbours of every other colour, and a pair launch reads
+    each line's neighbourhood for both of its colours); B read and X written once over the sweep
+    (each colour its own rows)."""
     matrix = val_bytes * nnz + 4 * (nnz // 9) + 8 * (n // 3) + 4
     vec = val_bytes * k * n
-    return matrix + ncolors * vec + 2 * vec
+    return matrix + launches * vec + 2 * vec
 
 
 def roofline(name, bytes_per_launch, total_ms, launches, pmc_tag, timing):
@@ -110,6 +111,8 @@ def main():
                     help="X bytes per strip of the level-0 tile order (-1: chosen by the library's timing)")
     ap.add_argument("--smoother", default="gs")
     ap.add_argument("--nu", type=int, default=1)
+    ap.add_argument("--gs-pairs", type=int, default=1,
+                    help="1: two Gauss-Seidel colours per launch where the colouring allows it")
     ap.add_argument("--tol", type=float, default=1e-8, help="relative residual of every FP64 true residual")
     ap.add_argument("--cpu-baseline", type=int, default=1, help="0: skip the CPU leg")
     ap.add_argument("--fp64-compare", type=int, default=2, help="load cases timed on the plain FP64 path")
@@ -137,7 +140,8 @@ def main():
     t0 = time.time()
     model = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
     F = model.rhs()[None, :] if nrhs == 1 else model.load_cases(nrhs, SEED_LOADS)
-    common = dict(levels=levels, smoother=args.smoother, nu=args.nu, check_every=args.check_every)
+    common = dict(levels=levels, smoother=args.smoother, nu=args.nu, check_every=args.check_every,
+                  gs_pairs=bool(args.gs_pairs))
     solver = HipMgPcgSolver(model, tolerance=args.tol, max_iterations=500, use_graph=bool(args.graph), nrhs=nrhs,
                             precision=args.precision, block_width=args.block_width,
                             strip_window=args.strip_window, **common)
@@ -241,15 +245,20 @@ def main():
     kind = ("spmm_nodeblock" if layout == "node-block" else "spmm") if width > 1 else "spmv"
     rooflines = [roofline(spmm_name, spmm_per_launch, spmv_ms, spmv_n,
                           f"{kind}_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed", "around each launch")]
-    ncol = solver.LevelColors(0)
-    if gs_n > 0 and ncol > 0:
-        rooflines.append(roofline(f"k_nb_gs<{tname}, K={width}> (one colour launch of the level-0 node-block "
-                                  f"multicolour Gauss-Seidel post-smoothing sweep; bytes averaged over the {ncol} "
-                                  f"colours of a sweep; the "
-                                  f"pre-smoothing sweep from x = 0 reads only earlier colours and is not counted)",
-                                  gs_sweep_bytes(nnz, ndof, width, vb, ncol) / ncol, gs_ms, gs_n,
-                                  f"gs_nodeblock_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed",
-                                  "around the back-to-back colour launches of each sweep"))
+    ncol, npair = solver.LevelColors(0), solver.GsPairs(0)
+    nl = ncol - npair                                   # launches per sweep
+    if gs_n > 0 and ncol > 0 and npair in (0, ncol // 2) and ncol % 2 == 0:
+        tag = "gs_pair_nodeblock" if npair else "gs_nodeblock"
+        what = (f"k_nb_gs_pair<{tname}, K={width}> (one launch of the level-0 node-block multicolour Gauss-Seidel "
+                f"post-smoothing sweep, two colours line by line; {nl} launches of a sweep, bytes averaged over them"
+                if npair else
+                f"k_nb_gs<{tname}, K={width}> (one colour launch of the level-0 node-block multicolour Gauss-Seidel "
+                f"post-smoothing sweep; bytes averaged over the {ncol} colours of a sweep")
+        rooflines.append(roofline(what + "; the pre-smoothing sweep from x = 0 reads only earlier colours and is "
+                                  "not counted)",
+                                  gs_sweep_bytes(nnz, ndof, width, vb, nl) / nl, gs_ms, gs_n,
+                                  f"{tag}_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed",
+                                  "around the back-to-back launches of each sweep"))
     # the dominant kernel: the most device time in the profiled solves
     rooflines.sort(key=lambda r: -r["total_ms"])
 

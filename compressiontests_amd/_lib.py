@@ -66,7 +66,8 @@ class Options(C.Structure):
                 ("level_iterations", C.c_int32), ("precision", C.c_int32), ("nrhs", C.c_int32),
                 ("duvi_tolerance", C.c_double), ("duvi_mode", C.c_int32), ("inner_iterations", C.c_int32),
                 ("inner_tolerance", C.c_double), ("profile", C.c_int32), ("check_every", C.c_int32),
-                ("use_graph", C.c_int32), ("block_width", C.c_int32), ("strip_window", C.c_double)]
+                ("use_graph", C.c_int32), ("block_width", C.c_int32), ("strip_window", C.c_double),
+                ("gs_pairs", C.c_int32)]
 
 
 class Stats(C.Structure):
@@ -154,6 +155,7 @@ _SIGS = {
     "mgx_solver_transfer_layout": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
     "mgx_solver_level_colors": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
     "mgx_solver_gs_order": (C.c_int, [_P, _I32, _P]),
+    "mgx_solver_gs_pairs": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
     "mgx_solver_strip_window": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
     "mgx_solver_release": (C.c_int, [_P]),
     "mgx_device_malloc": (C.c_int, [C.POINTER(_P), _U64]),

@@ -15,6 +15,7 @@
 #include <memory>
 #include <new>
 #include <type_traits>
+#include <unordered_map>
 
 #include "mgx_internal.h"
 
@@ -383,6 +384,9 @@ struct Level {
     DevNt Rn, Pn;                          // node transfers of R and P (block V-cycle)
     bool nt_unfit = false;                 // R or P does not act on the dofs of a node alike
     std::vector<int32_t> color_off;        // colour c = positions color_off[c] .. color_off[c+1]-1 of nbg
+    std::vector<int8_t> pair_ok;           // colours 2k, 2k+1: every edge between them lies on one mesh line
+    std::vector<DBuf<int32_t>> pair_rng;   // per such pair: 4 positions per line (launch_nb_gs_pair)
+    std::vector<int32_t> pair_lines;       // lines of pair_rng[k] (0: the pair runs colour by colour)
     std::vector<DBuf<int32_t>> color_nodes;
     std::vector<int32_t> color_count;
     DevDuvi dA, dR, dP;
@@ -672,6 +676,25 @@ static void initialize(mgx_solver_s *S)
             for (int c = 0; c < V.ncolors; ++c) {
                 V.color_nodes[c].upload(lists[c].data(), lists[c].size(), st);
                 V.color_count[c] = (int32_t)lists[c].size();
+            }
+            // colour pairs (2k, 2k+1) that one launch can relax line by line: on the free-node
+            // grid of a structured mesh (node q on line q / gx), every edge between the two
+            // colours must join two nodes of one line
+            V.pair_ok.assign(V.ncolors / 2, 0);
+            const int64_t lines = V.gy + 1, planes = V.gz + 1;
+            if (V.gx > 0 && (int64_t)V.gx * lines * planes * 3 == V.n) {
+                std::vector<int8_t> ok(V.ncolors / 2, 1);
+                const auto &rp = hrp[l];
+                const auto &ci = hci[l];
+                for (int32_t i = 0; i < V.n / 3; ++i) {
+                    const int c = color[i];
+                    if ((c ^ 1) >= V.ncolors) continue;
+                    for (int32_t k = rp[3 * i]; k < rp[3 * i + 1]; ++k) {
+                        const int32_t j = ci[k] / 3;
+                        if (color[j] == (c ^ 1) && j / V.gx != i / V.gx) ok[c >> 1] = 0;
+                    }
+                }
+                V.pair_ok = ok;
             }
             MGX_HIP(hipStreamSynchronize(st));
         }
@@ -1143,14 +1166,26 @@ static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, b
     const bool rec = start == NB_GS_GENERAL && S->prof_gs && &V == &S->lv[0] &&
                      (size_t)(2 * S->gs_used + 2) <= S->gs_events.size();
     if (rec) MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used], S->stream));
+    int launches = 0;
     for (int i = 0; i < V.ncolors; ++i) {
         const int c = backward ? V.ncolors - 1 - i : i;
         const int mode = start == NB_GS_GENERAL ? NB_GS_GENERAL
                          : i == 0              ? NB_GS_ZERO
                                                : (start == NB_GS_LOWER ? NB_GS_LOWER : NB_GS_GENERAL);
+        // colours c and the next one of the sweep in one launch, when they form a line pair
+        const int c2 = backward ? c - 1 : c + 1, k = c >> 1;
+        if (nbk && mode != NB_GS_ZERO && c2 >= 0 && (c2 >> 1) == k && k < (int)V.pair_lines.size() &&
+            V.pair_lines[k] > 0) {
+            launch_nb_gs_pair<T>(K, V.nbg.view, V.pair_rng[k].p, V.pair_lines[k], backward, b, x, backward,
+                                 (launches & 1) != 0, mode == NB_GS_LOWER, done, S->stream);
+            ++launches;
+            ++i;
+            continue;
+        }
+        ++launches;
         if (nbk)
-            launch_nb_gs<T>(K, V.nbg.view, V.color_off[c], V.color_off[c + 1], b, x, backward, (i & 1) != 0, done,
-                            S->stream, mode);
+            launch_nb_gs<T>(K, V.nbg.view, V.color_off[c], V.color_off[c + 1], b, x, backward,
+                            ((launches - 1) & 1) != 0, done, S->stream, mode);
         else if (V.node3)
             launch_gs_node<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done,
                               S->stream);
@@ -1160,7 +1195,7 @@ static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, b
     }
     if (rec) {
         MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used + 1], S->stream));
-        S->gs_launches[S->gs_used] = V.ncolors;
+        S->gs_launches[S->gs_used] = launches;
         S->gs_iter[S->gs_used++] = S->prof_used - 1;     // the profiled iteration (its A*p event pair)
     }
 }
@@ -1600,6 +1635,38 @@ static void order_for_cache(mgx_solver_s *S, int K)
             }
             all.insert(all.end(), nodes.begin(), nodes.end());
             V.color_off[c + 1] = (int32_t)all.size();
+        }
+        // line ranges of the colour pairs: each line's nodes must be consecutive in both lists;
+        // lines in the visiting order of the first colour
+        V.pair_rng.clear();
+        V.pair_rng.resize(V.pair_ok.size());
+        V.pair_lines.assign(V.pair_ok.size(), 0);
+        for (size_t k = 0; k < V.pair_ok.size() && S->opt.gs_pairs; ++k) {
+            if (!V.pair_ok[k]) continue;
+            std::vector<int32_t> rng;
+            std::unordered_map<int32_t, int32_t> at;   // line -> index into rng / 4
+            bool ok = true;
+            for (int h = 0; h < 2 && ok; ++h) {
+                const int c = 2 * (int)k + h;
+                int32_t prev = -1;
+                for (int32_t p = V.color_off[c]; p < V.color_off[c + 1] && ok; ++p) {
+                    const int32_t line = all[p] / V.gx;
+                    if (line == prev) { rng[4 * at[line] + 2 * h + 1] = p + 1; continue; }
+                    prev = line;
+                    auto it = at.find(line);
+                    if (it == at.end()) {
+                        it = at.emplace(line, (int32_t)(rng.size() / 4)).first;
+                        rng.insert(rng.end(), {0, 0, 0, 0});
+                    } else if (rng[4 * it->second + 2 * h + 1] != rng[4 * it->second + 2 * h]) {
+                        ok = false;                                // the line's nodes are split
+                    }
+                    rng[4 * it->second + 2 * h] = p;
+                    rng[4 * it->second + 2 * h + 1] = p + 1;
+                }
+            }
+            if (!ok || rng.empty()) continue;
+            V.pair_rng[k].upload(rng.data(), rng.size(), st);
+            V.pair_lines[k] = (int32_t)(rng.size() / 4);
         }
         V.nbg.reset();
         if (V.nb.built() && (int32_t)all.size() == V.n / 3) {
@@ -2162,6 +2229,7 @@ void mgx_options_default(mgx_options *o)
     o->use_graph = 1;
     o->block_width = 0;
     o->strip_window = -1;
+    o->gs_pairs = 1;
 }
 
 int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
@@ -2357,6 +2425,20 @@ int mgx_solver_level_colors(mgx_solver s, int32_t level, int32_t *ncolors)
         MGX_REQUIRE(s->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
         MGX_REQUIRE(level >= 0 && level < (int)s->lv.size(), MGX_ERR_INVALID_ARGUMENT, "level out of range");
         *ncolors = s->lv[level].ncolors;
+    });
+}
+
+int mgx_solver_gs_pairs(mgx_solver s, int32_t level, int32_t *pairs)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && pairs, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(s->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
+        MGX_REQUIRE(level >= 0 && level < (int)s->lv.size(), MGX_ERR_INVALID_ARGUMENT, "level out of range");
+        const Level &V = s->lv[level];
+        int32_t n = 0;
+        if (V.nbg.built())
+            for (int32_t m : V.pair_lines) n += m > 0;
+        *pairs = n;
     });
 }
 

@@ -21,8 +21,9 @@
 // is read) into its own LDS region with coalesced 16-byte non-temporal loads, then G lanes handle
 // one node: lane (s, v) takes the W columns v*W .. v*W+W-1 (one 16-byte vector of X per dof row)
 // and every S-th block of the node.  Waves proceed independently (no block-wide barrier, except
-// the final partials of the DOT form); LDS operations of one wavefront execute in order, so a
-// compiler barrier plus a wait for the wave's own LDS writes separates staging from use.
+// the final partials of the DOT form and the one between the two colours of a colour-pair
+// launch); LDS operations of one wavefront execute in order, so a compiler barrier plus a wait
+// for the wave's own LDS writes separates staging from use.
 //
 // Also here: the node transfers (R, P acting on the three dofs of a node alike).
 #include <type_traits>
@@ -476,6 +477,45 @@ __global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, cons
                             s_col[wv], lane);
 }
 
+// Two colours of a sweep in one launch.  Block b takes mesh line b: rng[4b] .. rng[4b+1]-1 are
+// its positions of colour a, rng[4b+2] .. rng[4b+3]-1 those of colour b; the block relaxes the
+// first colour's positions (a, or b when second_first), waits at a block barrier, then relaxes
+// the other colour's.  This is the sweep's order whenever every edge between nodes of the two
+// colours joins two nodes of one line (the host checks the matrix for it): a node of the first
+// colour reads only old values of the second colour (none is written before the barrier in its
+// line, and no other line's is a neighbour), and a node of the second colour reads the new values
+// of the first colour of its own line, written by this block before the barrier.  The second half
+// reads the same X neighbourhood as the first, from cache.
+constexpr int kPairBlock = 512;
+
+template <typename T, int K, bool LOWER>
+__global__ __launch_bounds__(kPairBlock) void k_nb_gs_pair(const int32_t *__restrict__ rng, int32_t nlines,
+                                                           int second_first, const int32_t *__restrict__ nrp,
+                                                           const int32_t *__restrict__ ncol,
+                                                           const int32_t *__restrict__ node,
+                                                           const int32_t *__restrict__ split, const T *__restrict__ val,
+                                                           const T *__restrict__ B, T *__restrict__ X, int backward,
+                                                           int reverse, const int32_t *__restrict__ done)
+{
+    using Sh = NbShape<T, K>;
+    constexpr int NWB = kPairBlock / 64;
+    __shared__ u32x4 s_val[NWB][Sh::VQ];
+    __shared__ u32x4 s_col[NWB][Sh::CQ];
+    if (done && *done) return;
+    const int b = xcd_remap(blockIdx.x, gridDim.x);
+    if (b >= nlines) return;                                     // the whole block leaves together
+    const int line = reverse ? nlines - 1 - b : b;
+    const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave-uniform
+    for (int h = 0; h < 2; ++h) {
+        const int o = 4 * line + 2 * (h ^ second_first);
+        const int p0 = rng[o], p1 = rng[o + 1];
+        for (int pa = p0 + wv * Sh::NW; pa < p1; pa += NWB * Sh::NW)
+            nb_gs_tile<T, K, LOWER>(pa, min(pa + Sh::NW, p1), nrp, ncol, node, split, val, B, X, backward, s_val[wv],
+                                    s_col[wv], lane);
+        if (h == 0) __syncthreads();
+    }
+}
+
 template <typename T>
 void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T *X, bool backward, bool reverse,
                   const int32_t *done, hipStream_t s, int start)
@@ -518,8 +558,40 @@ void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T 
 #undef MGX_NBG
     MGX_HIP(hipGetLastError());
 }
+template <typename T>
+void launch_nb_gs_pair(int K, const NbView &A, const int32_t *rng, int32_t nlines, bool second_first, const T *B,
+                       T *X, bool backward, bool reverse, bool lower, const int32_t *done, hipStream_t s)
+{
+    if (nlines <= 0) return;
+    if (lower && (!A.split || backward))
+        throw Error{MGX_ERR_INVALID_OPERATION, "lower-colour sweep needs a colour-sorted copy and a forward sweep"};
+    const int g = grid_rows(nlines, 1);
+#define MGX_NBP(KK)                                                                                              \
+    if (lower)                                                                                                   \
+        hipLaunchKernelGGL((k_nb_gs_pair<T, KK, true>), dim3(g), dim3(kPairBlock), 0, s, rng, nlines,           \
+                           second_first ? 1 : 0, A.nrp, A.ncol, A.node, A.split, (const T *)A.val, B, X, 0,      \
+                           reverse ? 1 : 0, done);                                                               \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_nb_gs_pair<T, KK, false>), dim3(g), dim3(kPairBlock), 0, s, rng, nlines,          \
+                           second_first ? 1 : 0, A.nrp, A.ncol, A.node, A.split, (const T *)A.val, B, X,         \
+                           backward ? 1 : 0, reverse ? 1 : 0, done)
+    switch (K) {
+    case 4: MGX_NBP(4); break;
+    case 8: MGX_NBP(8); break;
+    case 16: MGX_NBP(16); break;
+    case 32: MGX_NBP(32); break;
+    default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
+    }
+#undef MGX_NBP
+    MGX_HIP(hipGetLastError());
+}
+
 template void launch_nb_gs<float>(int, const NbView &, int32_t, int32_t, const float *, float *, bool, bool,
                                   const int32_t *, hipStream_t, int);
+template void launch_nb_gs_pair<float>(int, const NbView &, const int32_t *, int32_t, bool, const float *, float *,
+                                       bool, bool, bool, const int32_t *, hipStream_t);
+template void launch_nb_gs_pair<double>(int, const NbView &, const int32_t *, int32_t, bool, const double *,
+                                        double *, bool, bool, bool, const int32_t *, hipStream_t);
 template void launch_nb_gs<double>(int, const NbView &, int32_t, int32_t, const double *, double *, bool, bool,
                                    const int32_t *, hipStream_t, int);
 

@@ -156,6 +156,11 @@ class _Handle:
         L.call("mgx_solver_level_colors", C.c_void_p(self.h), level, C.byref(v))
         return v.value
 
+    def gs_pairs(self, level):
+        v = C.c_int32()
+        L.call("mgx_solver_gs_pairs", C.c_void_p(self.h), level, C.byref(v))
+        return v.value
+
     def release(self):
         if self.h:
             L.call("mgx_solver_release", C.c_void_p(self.h))
@@ -297,12 +302,14 @@ class HipMgPcgSolver:
     (4, 8, 16, 32), 1 = one at a time, 0 = chosen at Initialize by timing each width.
     strip_window: X window (bytes) of the strip order of the block kernels on structured levels,
     0 = whole planes, < 0 = chosen at Initialize by timing the candidates.
+    gs_pairs: block Gauss-Seidel relaxes two colours per launch, line by line, where the colouring
+    allows it (the same sweep); False = one launch per colour.
     precision "mixed": FP32 SpMV / V-cycle inside FP64 iterative refinement."""
 
     def __init__(self, model, levels=3, smoother="jacobi", omega=0.0, nu=2, tolerance=1e-8, stop="rel",
                  max_iterations=1000, hierarchy="galerkin", format="csr", profile=False, check_every=4, nrhs=1,
                  use_graph=True, precision="fp64", inner_tolerance=1e-4, inner_iterations=200, block_width=0,
-                 strip_window=-1.0):
+                 strip_window=-1.0, gs_pairs=True):
         self.model = model
         self.levels = levels
         opt = L.default_options(method=L.METHOD_MGPCG,
@@ -314,7 +321,8 @@ class HipMgPcgSolver:
                                 use_graph=int(bool(use_graph)),
                                 precision=L.PRECISION_MIXED if precision == "mixed" else L.PRECISION_FP64,
                                 inner_tolerance=inner_tolerance, inner_iterations=inner_iterations,
-                                block_width=block_width, strip_window=float(strip_window))
+                                block_width=block_width, strip_window=float(strip_window),
+                                gs_pairs=int(bool(gs_pairs)))
         self.opt = opt
         self.nrhs = nrhs
         h = _Handle(opt)
@@ -394,6 +402,11 @@ class HipMgPcgSolver:
     def LevelColors(self, level=0):
         """Colours of the level's multicolour Gauss-Seidel (0 without that smoother)."""
         return self._h.level_colors(level)
+
+    def GsPairs(self, level=0):
+        """Colour pairs of the level relaxed two colours per launch, line by line (0: one launch
+        per colour)."""
+        return self._h.gs_pairs(level)
 
     def solve_device(self, b_dev, x_dev):
         return self._h.solve_device(b_dev, x_dev, False, self.nrhs)

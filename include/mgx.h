@@ -254,6 +254,9 @@ typedef struct {
     double  strip_window;       /* batching form on structured levels: X window (bytes) of the strip order in
                                    which the block kernels visit the nodes, 0 = whole planes, < 0 = chosen at
                                    initialize by timing 6e6, 3e6, 1.5e6 and 0 (default) */
+    int32_t gs_pairs;           /* batching form, multicolour Gauss-Seidel on structured levels: 1 = relax two
+                                   colours per launch, line by line, where every edge between the two colours
+                                   lies on one mesh line (default; the same sweep), 0 = one launch per colour */
 } mgx_options;
 
 typedef struct {
@@ -268,9 +271,9 @@ typedef struct {
     double  spmv_ms;         /* their summed device time (HIP events on the solver stream) */
     int32_t block_width;     /* right-hand sides solved together with this one (1: alone); with
                                 block_width > 1, solve_ms is the device time of that whole block */
-    int32_t gs_launches;     /* batching form with options.profile: level-0 Gauss-Seidel colour launches timed
-                                (the general kernel; the first launch of a sweep from x = 0 reads no
-                                neighbours and is not counted) */
+    int32_t gs_launches;     /* batching form with options.profile: launches of the level-0 general Gauss-Seidel
+                                sweeps timed (one per colour, or per colour pair with options.gs_pairs; the
+                                sweeps from x = 0 are not timed) */
     double  gs_ms;           /* their summed device time (HIP events on the solver stream around each sweep) */
 } mgx_stats;
 
@@ -331,6 +334,12 @@ int mgx_solver_level_colors(mgx_solver s, int32_t level, int32_t *ncolors);
  * 1's, ...  The reference's GPU "Gauss-Seidel" (CsrGeometricMultigrid.cl:207-225) updates in
  * place with racy reads; this ordering is the documented deviation. */
 int mgx_solver_gs_order(mgx_solver s, int32_t level, int32_t *order);
+/* Colour pairs (2k, 2k+1) of the level that the batching form's Gauss-Seidel relaxes in one launch
+ * (options.gs_pairs): block i of the launch relaxes the pair's nodes on mesh line i, the first
+ * colour's, then after a block barrier the second colour's.  A pair qualifies when every edge
+ * between its two colours joins two nodes of one line, so the sweep is the one gs_order gives.
+ * 0 when the sweeps run one launch per colour.  No reference counterpart (batching form). */
+int mgx_solver_gs_pairs(mgx_solver s, int32_t level, int32_t *pairs);
 /* Replaces ReleaseOpenCLResources (OpenCLCsrGeometricMultigridSolver.cs:106-138). */
 int mgx_solver_release(mgx_solver s);
 
