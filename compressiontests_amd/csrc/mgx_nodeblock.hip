@@ -270,6 +270,9 @@ __global__ __launch_bounds__(kBlock) void k_nb_spmm(int32_t npos, const int32_t 
                 }
             }
         }
+        // the other forms run the full grid (launch_nb_spmm): every wave has at most one tile, and
+        // leaving here lets the compiler drop the loop-carried state (60 instead of 109 VGPRs)
+        if (EPI != BEPI_DOT) break;
     }
     if (EPI == BEPI_DOT) nb_publish<T, K>(dot, (double *)&s_val[0][0], partials, stride);
 }
@@ -309,7 +312,9 @@ int nb_spmm_full_grid(int K, int32_t npos)
 template int nb_spmm_full_grid<float>(int, int32_t);
 template int nb_spmm_full_grid<double>(int, int32_t);
 
-// One tile per wave (blocks dispatched in tile order), except the DOT form: it runs the bounded
+// One tile per wave (blocks dispatched in tile order; the kernel relies on it and processes at most
+// one tile per wave for these forms: 4m waves per XCD for ntiles <= 32m cover the XCD's
+// ceil(ntiles / 8) tiles), except the DOT form: it runs the bounded
 // grid of nb_spmm_grid (waves walk several tiles), whose block count is the number of partials
 // the caller's finalize reads.  (One tile per wave was faster for the other forms, 0.75 against
 // 0.91 ms for the 128^3 residual, and slower for DOT, whose per-block publication then
