@@ -373,21 +373,22 @@ def test_block_visiting_order_invariance(gpu):
         assert np.max(np.abs(X - res[0][0])) <= 1e-11 * np.max(np.abs(res[0][0]))
 
 
-@pytest.mark.parametrize("precision,nu", [("fp64", 1), ("fp64", 2), ("mixed", 1)])
-def test_block_gs_colour_pairs_same_sweep(gpu, precision, nu):
+@pytest.mark.parametrize("precision,nu,n,bw,window", [("fp64", 1, 24, 8, -1.0), ("fp64", 2, 24, 8, -1.0),
+                                                      ("mixed", 1, 24, 8, -1.0), ("mixed", 1, 32, 32, 1e5)])
+def test_block_gs_colour_pairs_same_sweep(gpu, precision, nu, n, bw, window):
     """Two colours per launch, line by line (every edge between colours 2k and 2k+1 of the
     greedy colouring joins nodes of one x-line), is the same sweep as one launch per colour: the
     same arithmetic for every node, so the same iterations and bit-identical solutions, for the
     zero-start sweep (lower-colour pairs, nu = 1) and the general one (nu = 2), in FP64 and in the
-    precision form.  The profiled level-0 general sweeps take half the launches."""
-    n = 24
+    precision form; the last case is the headline's kernel shape (FP32, 32 columns, strip order).
+    The profiled level-0 general sweeps take half the launches."""
     dens = 10 ** (4 * O.rng_uniform(20261207, n ** 3) - 2)
     m = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
-    F = m.load_cases(8, SEED_LOADS)
+    F = m.load_cases(bw, SEED_LOADS)
     res = []
     for pairs in (False, True):
-        s = HipMgPcgSolver(m, levels=3, smoother="gs", nu=nu, tolerance=1e-10, nrhs=8, block_width=8,
-                           precision=precision, profile=True, gs_pairs=pairs)
+        s = HipMgPcgSolver(m, levels=3, smoother="gs", nu=nu, tolerance=1e-10, nrhs=bw, block_width=bw,
+                           precision=precision, profile=True, gs_pairs=pairs, strip_window=window)
         s.Initialize(F)
         assert s.LevelLayout(0) == "node-block"
         assert s.GsPairs(0) == (s.LevelColors(0) // 2 if pairs else 0) and s.LevelColors(0) == 8
