@@ -124,6 +124,7 @@ _SIGS = {
     "mgx_csr_jacobi": (C.c_int, [_I32, _P, _P, _P, _P, _P, _P, _P, _P]),
     "mgx_csr_gs_color": (C.c_int, [_I32, _P, _P, _P, _P, _P, _P, _I32, _P]),
     "mgx_dot": (C.c_int, [_I32, _P, _P, _P, _P]),
+    "mgx_dot_noalloc": (C.c_int, [_I32, _P, _P, _P, _P]),
     "mgx_duvi_spmv": (C.c_int, [C.POINTER(Duvi), _P, _P, _I32, _P]),
     "mgx_duvi_encode": (C.c_int, [_I32, _P, _P, _P, _D, _I32, C.POINTER(DuviHost)]),
     "mgx_duvi_host_free": (None, [C.POINTER(DuviHost)]),

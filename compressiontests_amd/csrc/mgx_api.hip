@@ -13,6 +13,7 @@
 #include <cmath>
 #include <cstring>
 #include <memory>
+#include <mutex>
 #include <new>
 #include <type_traits>
 #include <unordered_map>
@@ -1820,6 +1821,35 @@ static void solve_device(mgx_solver_s *S, const double *b, double *x, bool has_x
 
 }  // namespace mgx
 
+// ====================================================================== allocation-free dot product
+// Device memory of mgx_dot_noalloc: the per-block partial sums of k_dot_partial, one double per
+// block.  dot_partial_count() caps that grid at 2048 blocks, so 2048 slots (16 KiB) cover every n an
+// int32 size can name.  Every device the library runs on holds its own copy of the array until the
+// process ends.  The array's name is a host-side stand-in for the device copy: host code uses the
+// address hipGetSymbolAddress returns for the current device, never the name itself.
+static constexpr int kNoAllocSlots = 2048;
+static __device__ __align__(256) double scratch[kNoAllocSlots];
+
+namespace mgx {
+static std::mutex g_scratch_mu;
+static std::unordered_map<int, double *> g_scratch_ptr;   // device ordinal -> its copy of `scratch`
+
+// address of the current device's `scratch`, looked up once per device on its first use
+static double *scratch_on_current_device()
+{
+    int dev = -1;
+    MGX_HIP(hipGetDevice(&dev));
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    auto it = g_scratch_ptr.find(dev);
+    if (it != g_scratch_ptr.end()) return it->second;
+    void *ptr = nullptr;
+    MGX_HIP(hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)));
+    MGX_REQUIRE(ptr != nullptr, MGX_ERR_DEVICE, "hipGetSymbolAddress returned no address for the dot scratch");
+    g_scratch_ptr[dev] = (double *)ptr;
+    return (double *)ptr;
+}
+}  // namespace mgx
+
 // ====================================================================== C ABI
 extern "C" {
 
@@ -1951,6 +1981,21 @@ int mgx_dot(int32_t n, const double *a, const double *b, double *result, void *s
         DBuf<double> partials; partials.alloc(3 * kPartialStride);
         launch_dot(n, a, b, result, partials.p, nullptr, s);
         MGX_HIP(hipStreamSynchronize(s));
+    });
+}
+
+int mgx_dot_noalloc(int32_t n, const double *a, const double *b, double *result, void *stream)
+{
+    return guard([&] {
+        MGX_REQUIRE(n >= 0 && result && (n == 0 || (a && b)), MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        const int parts = dot_partial_count(n);
+        MGX_REQUIRE(parts <= kNoAllocSlots, MGX_ERR_INVALID_ARGUMENT,
+                    "mgx_dot_noalloc: " + std::to_string(parts) + " partial sums exceed the " +
+                    std::to_string(kNoAllocSlots) + "-slot scratch");
+        double *ptr = scratch_on_current_device();
+        // k_dot_partial writes slots [0, parts) before k_finalize (same stream) reads them: nothing
+        // left over from an earlier call is read, so the slots need no clearing
+        launch_dot(n, a, b, result, ptr, nullptr, (hipStream_t)stream);
     });
 }
 

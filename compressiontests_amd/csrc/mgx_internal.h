@@ -74,6 +74,7 @@ void launch_gs_color_f32(int32_t nnodes, const int32_t *nodes, const int32_t *rp
                          const float *b, float *x, bool backward, const int32_t *done, hipStream_t s);
 void launch_dot(int32_t n, const double *a, const double *b, double *result, double *partials,
                 uint32_t *counter, hipStream_t s);
+int dot_partial_count(int64_t n);   // partial sums launch_dot writes (its first kernel's grid)
 void launch_dense_gemv(int32_t n, const double *M, const double *x, double *y, bool accumulate_into_y,
                        const int32_t *done, hipStream_t s);
 void launch_duvi_spmv(const mgx_duvi &A, const double *x, double *y, bool clear, const int32_t *done, hipStream_t s);

@@ -410,6 +410,8 @@ void launch_finalize(int nparts, const double *partials, int stride, Ctl *ctl, i
     MGX_HIP(hipGetLastError());
 }
 
+int dot_partial_count(int64_t n) { return grid_stream(n); }
+
 void launch_dot(int32_t n, const double *a, const double *b, double *result, double *partials, uint32_t *counter,
                 hipStream_t s)
 {

@@ -95,6 +95,18 @@ int mgx_csr_gs_color(int32_t nnodes, const int32_t *nodes, const int32_t *rowptr
 /* *result = a . b (device pointer result).  Replaces dot_partial + dot_finalize, CsrCG.cl:43-85. */
 int mgx_dot(int32_t n, const double *a, const double *b, double *result, void *stream);
 
+/* *result = a . b, the same kernels and the same result bits as mgx_dot, but the call allocates
+ * nothing: no hipMalloc / hipFree, no allocating library routine.  No counterpart in the reference;
+ * it is for callers that cannot allocate device memory (code under stream capture, tight serving
+ * loops, hosts that forbid allocation).  The per-block partial sums live in a 16 KiB array of
+ * device memory inside the library (2048 slots; dot sizes needing more partials are rejected with
+ * MGX_ERR_INVALID_ARGUMENT -- the grid is capped at 2048 blocks, so every int32 n fits), one copy
+ * per GPU, held until the process ends.  Asynchronous on `stream` (mgx_dot synchronizes): *result
+ * is ready when the stream reaches this point.  Every call on one GPU shares that GPU's array, so
+ * calls that overlap on one GPU (two streams, two host threads) race and must be serialized by
+ * the caller. */
+int mgx_dot_noalloc(int32_t n, const double *a, const double *b, double *result, void *stream);
+
 /* DuVi GPU block layout (OpenCLDuViGeometricMultigridSolver.cs:16-31) on the device.
  * vi_bytes: 1 (u8), 2 (u16), 4 (u32) value indices into values[nuniq], or 0 = "DU + raw
  * values" (values[nnz] in CSR order, no index stream). */
