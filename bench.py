@@ -14,12 +14,14 @@ exactly K steps.
 
 Printed JSON line: value = solves/s (load cases solved per second); config carries the plain
 FP64 GPU path (one right-hand side at a time) timed on the same system, the largest energy-norm
-error of the batched solutions against FP64 solves to 1e-12 of every load case, and the
-autotuner's timings; roofline = the kernel with the most device time in profiled solves of the
-same batch (HIP events on the solver stream around each SpMM launch and around each level-0
-Gauss-Seidel sweep of launches, one per colour pair or per colour), roofline_next = the other of
-the level-0 Gauss-Seidel and SpMM; cpu_baseline = the oracle's single-threaded C MG-PCG on the
-same hierarchy (a bounded sample of its iterations).
+error of the batched solutions against FP64 solves to 1e-12 of every load case (the run exits
+with status 3 after printing when it exceeds 1e-8), and the autotuner's timings; roofline = the
+kernel with the most device time in profiled solves of the same batch (HIP events on the solver
+stream around each SpMM launch and around each level-0 Gauss-Seidel sweep of launches, one per
+colour pair or per colour; a sweep's algorithmic bytes count X read once), roofline_next = the
+other of the level-0 Gauss-Seidel and SpMM, roofline_solve = the algorithmic bytes of every kernel
+of the solve per second (compressiontests_amd.traffic); cpu_baseline = the oracle's C MG-PCG
+on the same hierarchy, all cores (OpenMP build) and one core, a bounded sample of its iterations.
 
 --n 64 --rhs 1 --precision fp64 --moduli loguniform gives the single-right-hand-side FP64
 solve of the 64^3 system (moduli log-uniform in [1e-2, 1e2], tip load).
@@ -34,37 +36,10 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+ENERGY_BOUND = 1e-8                 # relative energy-norm error every load case must meet
 SEED_LOGUNIFORM = 20261203 + 2      # moduli log-uniform in [1e-2, 1e2]
 SEED_LOGNORMAL = 20261203 + 3       # moduli log-normal, sigma 1.5, contrast 1e3
 SEED_LOADS = 20261203 + 4           # top-face load cases
-
-
-def csr_spmv_bytes(nnz, nrows, ncols, val_bytes=8):
-    """Algorithmic bytes of one CSR SpMV: values + column indices, row offsets, x read once,
-    y written once."""
-    return (val_bytes + 4) * nnz + 4 * (nrows + 1) + val_bytes * (ncols + nrows)
-
-
-def spmm_bytes(nnz, n, k, val_bytes, layout="csr"):
-    """Algorithmic bytes of one SpMM with k right-hand sides: the matrix once, X read once and Y
-    written once (n x k each).  CSR: a value and a column index per nonzero and n + 1 row offsets;
-    node blocks: a value per nonzero, one column index per 3x3 block (nnz / 9) and n / 3 + 1 node
-    offsets."""
-    if layout == "node-block":
-        return val_bytes * nnz + 4 * (nnz // 9) + 4 * (n // 3 + 1) + 2 * val_bytes * k * n
-    return (val_bytes + 4) * nnz + 4 * (n + 1) + 2 * val_bytes * k * n
-
-
-def gs_sweep_bytes(nnz, n, k, val_bytes, launches):
-    """Algorithmic bytes of one node-block multicolour Gauss-Seidel sweep with k right-hand sides
-    made of `launches` launches (one per colour, or one per colour pair): the blocks (a value per
-    nonzero, a column per block, the node offsets and node map) once over the sweep; per launch
-    all of X (a colour's nodes have neighbours of every other colour, and a pair launch reads
-    each line's neighbourhood for both of its colours); B read and X written once over the sweep
-    (each colour its own rows)."""
-    matrix = val_bytes * nnz + 4 * (nnz // 9) + 8 * (n // 3) + 4
-    vec = val_bytes * k * n
-    return matrix + launches * vec + 2 * vec
 
 
 def roofline(name, bytes_per_launch, total_ms, launches, pmc_tag, timing):
@@ -84,6 +59,7 @@ def roofline(name, bytes_per_launch, total_ms, launches, pmc_tag, timing):
             "kernel": name + ", HIP events on the solver stream " + timing + " in profiled solves of the same batch",
             "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic, "bytes_per_launch": int(bytes_per_launch),
+            "traffic_ratio": round(traffic / bytes_per_launch, 3) if traffic else None,
             "avg_launch_ms": round(avg_ms, 5), "launches": launches, "total_ms": round(total_ms, 3)}
 
 
@@ -128,6 +104,7 @@ def main():
 
     import numpy as np
     from compressiontests_amd import _lib, kernels
+    from compressiontests_amd import traffic as T
     from compressiontests_amd.device import DeviceArray
     from compressiontests_amd.model import FemCantilever3D
     from compressiontests_amd.solvers import HipMgPcgSolver
@@ -182,7 +159,9 @@ def main():
 
     spmv_ms, spmv_n, gs_ms, gs_n = 0.0, 0, 0.0, 0
     xp = DeviceArray(F.size)
-    for _ in range(max(1, min(args.steps, 2))):
+    nprof = max(1, min(args.steps, 2))
+    nblocks = (nrhs + max(width, 1) - 1) // max(width, 1)
+    for _ in range(nprof):
         st = prof.solve_device(b_dev, xp)
         for j in range(0, nrhs, max(width, 1)):   # one entry per block: the block's launches
             spmv_ms += st[j].spmv_ms
@@ -233,14 +212,15 @@ def main():
     layout = solver.LevelLayout(0)
     tname = "float" if vb == 4 else "double"
     if width > 1:
-        spmm_per_launch = spmm_bytes(nnz, ndof, width, vb, layout)
+        spmm_per_launch = (T.nodeblock_spmm_bytes(nnz, ndof, width, vb) if layout == "node-block" else
+                           T.csr_spmv_bytes(nnz, ndof, ndof, vb) - vb * 2 * ndof + 2 * vb * width * ndof)
         spmm_name = {"node-block": f"k_nb_spmm<{tname}, K={width}, DOT> (level-0 node-block (3x3) SpMM Q = A P with "
                                    f"per-column P.Q)",
                      "node-triple": f"k_spmm_node<{tname}, K={width}, DOT> (level-0 node-triple CSR SpMM Q = A P "
                                     f"with per-column P.Q)"}.get(layout, f"k_spmm<{tname}, K={width}, DOT> (level-0 "
                                                                          f"CSR SpMM)")
     else:
-        spmm_per_launch = csr_spmv_bytes(nnz, ndof, ndof, vb)
+        spmm_per_launch = T.csr_spmv_bytes(nnz, ndof, ndof, vb)
         spmm_name = f"k_csr_stream<{tname}, DOT> (level-0 CSR-stream SpMV q = A p with p.q)"
     kind = ("spmm_nodeblock" if layout == "node-block" else "spmm") if width > 1 else "spmv"
     rooflines = [roofline(spmm_name, spmm_per_launch, spmv_ms, spmv_n,
@@ -254,13 +234,16 @@ def main():
                 if npair else
                 f"k_nb_gs<{tname}, K={width}> (one colour launch of the level-0 node-block multicolour Gauss-Seidel "
                 f"post-smoothing sweep; bytes averaged over the {ncol} colours of a sweep")
-        rooflines.append(roofline(what + "; the pre-smoothing sweep from x = 0 reads only earlier colours and is "
-                                  "not counted)",
-                                  gs_sweep_bytes(nnz, ndof, width, vb, nl) / nl, gs_ms, gs_n,
+        rooflines.append(roofline(what + "; a sweep's algorithmic bytes are the blocks, X read once, B read once "
+                                  "and X written once; the pre-smoothing sweep from x = 0 reads only earlier colours "
+                                  "and is not counted)",
+                                  T.nodeblock_gs_sweep_bytes(nnz, ndof, width, vb) / nl, gs_ms, gs_n,
                                   f"{tag}_{'f32' if vb == 4 else 'f64'}_k{width}_{n}cubed",
                                   "around the back-to-back launches of each sweep"))
     # the dominant kernel: the most device time in the profiled solves
     rooflines.sort(key=lambda r: -r["total_ms"])
+    solve_roof = solve_roofline(solver, hier, args, width, nrhs, solves_per_s, spmv_n / max(1, nprof * nblocks),
+                                int(max(outer)), T)
 
     cpu = None
     if args.cpu_baseline:
@@ -299,6 +282,8 @@ def main():
             "outer_iterations_max": int(max(outer)),
             "dof_per_s": round(ndof * solves_per_s, 1),
             "energy_norm_error": energy_err,
+            "energy_norm_bound": ENERGY_BOUND,
+            "energy_norm_bound_met": bool(energy_err <= ENERGY_BOUND),
             "energy_norm_checked_cases": cols,
             "rel_residual_max": float(max(rels)),
             "fp64_single_rhs_path": fp64,
@@ -307,16 +292,58 @@ def main():
         },
         "roofline": rooflines[0],
         "roofline_next": rooflines[1] if len(rooflines) > 1 else None,
+        "roofline_solve": solve_roof,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)
+    if energy_err > ENERGY_BOUND:
+        # the metric is solves/s to the energy-norm bound: a line whose batch missed it is not a result
+        print(f"# INVALID: energy-norm error {energy_err:.3e} above {ENERGY_BOUND:g}", file=sys.stderr, flush=True)
+        sys.exit(3)
+
+
+def solve_roofline(solver, hier, args, width, nrhs, solves_per_s, iterations, outer_steps, T):
+    """Whole-solve HBM roofline of the batching + precision form: the algorithmic bytes of every
+    kernel of one block solve (compressiontests_amd.traffic.block_mixed_solve_bytes, from the
+    level sizes, the block width, the block PCG iterations the profiled solves ran and the
+    refinement steps) times the blocks solved per second.  None for other configurations."""
+    L = len(hier)
+    if (width <= 1 or args.precision != "mixed" or args.smoother != "gs" or args.nu != 1
+            or any(solver.LevelLayout(l) != "node-block" for l in range(L - 1))
+            or any(solver.TransferLayout(l) != "node-transfer" for l in range(L - 1))):
+        return None
+    levels = []
+    for A, R, P in hier:
+        d = {"n": A.nrows, "nnz": A.nnz}
+        if R is not None:
+            d["r_nnz"], d["p_nnz"] = R.nnz, P.nnz
+        levels.append(d)
+    per_block = T.block_mixed_solve_bytes(levels, width, iterations, outer_steps)
+    achieved = per_block * (solves_per_s / width) / 1e9
+    return {"bound": "hbm", "scope": "every kernel of a block solve (block width %d, %.1f block PCG iterations, %d "
+                                     "refinement steps), algorithmic bytes, blocks of load cases solved per second"
+                                     % (width, iterations, outer_steps),
+            "bytes_per_load_case": int(per_block / width), "achieved": round(achieved, 1), "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": round(achieved / HBM_PEAK_GBS, 4)}
+
+
+def cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
 
 
 def cpu_baseline(hier, f, args, iterations):
-    """The oracle's single-threaded C restatement of the FP64 MG-PCG (one right-hand side, same
-    Galerkin hierarchy, smoother and first load case).  Bounded sample: the time of 2 PCG
-    iterations (a 3-iteration run minus a 1-iteration run, so the per-solve set-up cancels),
+    """The oracle's C restatement of the FP64 MG-PCG (one right-hand side, the same Galerkin
+    hierarchy, smoother and first load case), built -O3: the serial build on one core and the
+    OpenMP build on every core the process may use (bit-identical results).  Bounded sample: one
+    warm-up iteration, then the median of 5 timed iterations (each timed inside the oracle),
     scaled to the FP64 GPU solve's iteration count."""
+    import numpy as np
     from oracle import oracle as O
     lv = []
     for A, R, P in hier:
@@ -325,19 +352,23 @@ def cpu_baseline(hier, f, args, iterations):
             d["R"], d["P"] = _oracle_csr(O, R), _oracle_csr(O, P)
         lv.append(d)
     sm = 1 if args.smoother == "gs" else 0
-    times = {}
-    for it in (1, 3):
-        t = time.perf_counter()
-        O.mgpcg(lv, f, smoother=sm, omega=0.0, nu=args.nu, tol=0.0, max_it=it)
-        times[it] = time.perf_counter() - t
-        print(f"# cpu baseline: {it} iterations in {times[it]:.1f} s", file=sys.stderr, flush=True)
-    per_it = max(times[3] - times[1], 1e-9) / 2
-    solve_s = per_it * iterations
-    return {"value": round(1.0 / solve_s, 6), "unit": "solves/s", "cores": 1, "kind": "port",
-            "sample": f"2 FP64 MG-PCG iterations on one load case of the same {args.n}^3 system and hierarchy "
-                      f"(oracle C restatement, -O2, 1 thread; 3-iteration minus 1-iteration run), {per_it:.3f} "
-                      f"s/iteration, scaled to the FP64 GPU path's {iterations} iterations = {solve_s:.1f} s "
-                      f"per solve"}
+    res = {}
+    for omp in (False, True):
+        _, _, t = O.mgpcg_timed(lv, f, 6, smoother=sm, omega=0.0, nu=args.nu, omp=omp)
+        per_it = float(np.median(t[1:]))
+        res[omp] = (per_it, O.num_threads(omp))
+        print(f"# cpu baseline ({res[omp][1]} threads): {per_it:.3f} s/iteration", file=sys.stderr, flush=True)
+    per_it, cores = res[True]
+    one = res[False][0]
+    return {"value": round(1.0 / (per_it * iterations), 6), "unit": "solves/s", "cores": cores, "kind": "port",
+            "cpu_model": cpu_model(), "host_cpus": len(os.sched_getaffinity(0)),
+            "single_core": {"value": round(1.0 / (one * iterations), 6), "unit": "solves/s", "cores": 1,
+                            "s_per_iteration": round(one, 4)},
+            "s_per_iteration": round(per_it, 4),
+            "sample": f"FP64 MG-PCG on one load case of the same {args.n}^3 system and hierarchy (oracle C "
+                      f"restatement, -O3 -ffp-contract=off, OpenMP on {cores} threads; the single-core figure is the "
+                      f"serial build): 1 warm-up + median of 5 timed iterations, {per_it:.3f} s/iteration, scaled to "
+                      f"the FP64 GPU path's {iterations} iterations = {per_it * iterations:.1f} s per solve"}
 
 
 def _oracle_csr(O, M):

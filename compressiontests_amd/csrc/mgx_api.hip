@@ -1049,6 +1049,17 @@ static void vcycle32(mgx_solver_s *S, int l, const float *b, float *x, const int
 }
 
 constexpr int kStagnationSteps = 3;
+// Relative reduction asked of an inner FP32 solve: inner_tolerance, or less when less takes the
+// FP64 residual below the outer tolerance with a margin (an inner solve that lands the true
+// residual just above the tolerance would otherwise cost a whole further refinement step).
+constexpr double kInnerMargin = 0.3;
+static double inner_reduction(double inner_tol, int stop, double tol, double rnorm2, double bnorm2, double rmax)
+{
+    double need = 1.0;
+    if (stop == MGX_STOP_REL_L2 && rnorm2 > 0) need = tol * std::sqrt(bnorm2 / rnorm2);
+    else if (stop == MGX_STOP_MAXNORM && rmax > 0) need = tol / rmax;
+    return std::min(0.5, std::max(inner_tol, kInnerMargin * need));
+}
 
 // FP64 iterative refinement around an FP32 MG-PCG: per outer step the true residual
 // r = b - A x in FP64, an inner PCG on A d = r with FP32 SpMV / V-cycle / vectors and FP64
@@ -1086,6 +1097,8 @@ static void solve_mixed(mgx_solver_s *S, const double *b, double *x, bool has_x0
         if (S->ctl_host->rnorm2 < 0.25 * best_r2) { best_r2 = S->ctl_host->rnorm2; stalled = 0; }
         else if (++stalled >= kStagnationSteps) { status = 0; break; }
         // inner FP32 PCG on A d = r
+        inner.tol = inner_reduction(S->opt.inner_tolerance, outer.stop, outer.tol, S->ctl_host->rnorm2,
+                                    S->ctl_host->bnorm2, S->ctl_host->rmax);
         const int32_t *done = &ci->done;
         launch_convert_f64_f32(n, S->r.p, S->r32.p, nullptr, s);
         launch_fill_zero_f32(n, S->d32.p, nullptr, s);
@@ -1391,6 +1404,7 @@ static BCtl block_ctl_init(int K, int kc, int stop, double tol, int max_it)
     c.tol = tol;
     c.max_it = max_it;
     for (int j = kc; j < K; ++j) c.frozen[j] = 1;   // padding columns of the last block
+    for (int j = 0; j < K; ++j) c.ctol[j] = tol;
     return c;
 }
 
@@ -1469,6 +1483,9 @@ static void solve_block_mixed(mgx_solver_s *S, int K, int j0, int kc, const doub
         if (!active) break;
         BCtl in = block_ctl_init(K, kc, MGX_STOP_REL_L2, S->opt.inner_tolerance, S->opt.inner_iterations);
         for (int j = 0; j < K; ++j) in.frozen[j] = o.cdone[j] || (j < kc && failed[j]);
+        for (int j = 0; j < kc; ++j)
+            in.ctol[j] = inner_reduction(S->opt.inner_tolerance, S->opt.stop, S->opt.tolerance, o.rnorm2[j],
+                                         o.bnorm2[j], o.rmax[j]);
         put_ctl(S, S->bctl_in.p, in);
         launch_convert_f64_f32(nk, S->bR64.p, S->bR32.p, nullptr, s);
         launch_fill_zero_f32(nk, S->bD32.p, nullptr, s);

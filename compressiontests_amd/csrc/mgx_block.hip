@@ -708,7 +708,8 @@ __global__ __launch_bounds__(1024) void k_bfinalize(int nparts, const double *__
                     const int it = ctl->iter[c] + 1;
                     ctl->iter[c] = it;
                     if (!(m == m) || m > 1e50 || !(a == a)) { cd = 1; ctl->status[c] = -1; }
-                    else if (ctl->stop == MGX_STOP_REL_L2 ? (a <= ctl->tol * ctl->tol * ctl->bnorm2[c]) : (m <= ctl->tol)) {
+                    else if (ctl->stop == MGX_STOP_REL_L2 ? (a <= ctl->ctol[c] * ctl->ctol[c] * ctl->bnorm2[c])
+                                                          : (m <= ctl->ctol[c])) {
                         cd = 1; ctl->status[c] = 1;
                     } else if (it >= ctl->max_it) cd = 1;
                 }
@@ -722,7 +723,10 @@ __global__ __launch_bounds__(1024) void k_bfinalize(int nparts, const double *__
                 int st = 0;
                 cd = 0;
                 if (!(m == m) || m > 1e50) { cd = 1; st = -1; }
-                else if (ctl->stop == MGX_STOP_REL_L2 ? (b <= ctl->tol * ctl->tol * a) : (m <= ctl->tol)) { cd = 1; st = 1; }
+                else if (ctl->stop == MGX_STOP_REL_L2 ? (b <= ctl->ctol[c] * ctl->ctol[c] * a) : (m <= ctl->ctol[c])) {
+                    cd = 1;
+                    st = 1;
+                }
                 else if (ctl->frozen[c] || ctl->max_it <= 0) cd = 1;
                 ctl->status[c] = st;
                 break;

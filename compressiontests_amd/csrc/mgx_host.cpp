@@ -296,22 +296,54 @@ inline int32_t net_double_hash(double d)
     if (((bits - 1) & 0x7FFFFFFFFFFFFFFFULL) >= 0x7FF0000000000000ULL) bits &= 0x7FF0000000000000ULL;
     return (int32_t)((uint32_t)bits ^ (uint32_t)(bits >> 32));
 }
+// Entries of one hash value form a chain through `next`, newest first, from the slot of that
+// hash in an open-addressing table (linear probing, kept at most half full): the same candidate
+// order as the .NET Dictionary scan.
 struct Dedup {
     double tol;
-    std::unordered_map<int32_t, std::vector<int32_t>> chains;
     std::vector<double> keys;
-    explicit Dedup(double t) : tol(t) {}
+    std::vector<int32_t> next;       // previous entry with the same hash, -1 at the chain's end
+    std::vector<int32_t> slot_hash;  // hash of the slot's chain
+    std::vector<int32_t> slot_head;  // newest entry of the chain, -1 for a free slot
+    uint64_t mask = 0;
+    explicit Dedup(double t) : tol(t) { rehash(1 << 12); }
+    static uint64_t mix(int32_t h) { return (uint64_t)(uint32_t)h * 0x9E3779B97F4A7C15ULL >> 17; }
+    void rehash(uint64_t cap)
+    {
+        std::vector<int32_t> oh, ohead;
+        oh.swap(slot_hash);
+        ohead.swap(slot_head);
+        slot_hash.assign(cap, 0);
+        slot_head.assign(cap, -1);
+        mask = cap - 1;
+        for (size_t i = 0; i < ohead.size(); ++i) {
+            if (ohead[i] < 0) continue;
+            int32_t *hd = find(oh[i]);          // a free slot: every hash has one chain
+            slot_hash[hd - slot_head.data()] = oh[i];
+            *hd = ohead[i];
+        }
+    }
+    // the head of the chain of hash h, or the free slot where it would go
+    int32_t *find(int32_t h)
+    {
+        uint64_t i = mix(h) & mask;
+        while (slot_head[i] >= 0 && slot_hash[i] != h) i = (i + 1) & mask;
+        return &slot_head[i];
+    }
     int32_t get(double v)
     {
         const int32_t h = tol > 0 ? net_double_hash(std::nearbyint(v / (2 * tol))) : net_double_hash(v);
-        auto &ch = chains[h];
-        for (auto it = ch.rbegin(); it != ch.rend(); ++it) {
-            const double k = keys[*it];
-            if (tol > 0 ? std::fabs(k - v) < tol : (k == v || (k != k && v != v))) return *it;
+        int32_t *head = find(h);
+        for (int32_t e = *head; e >= 0; e = next[e]) {
+            const double k = keys[e];
+            if (tol > 0 ? std::fabs(k - v) < tol : (k == v || (k != k && v != v))) return e;
         }
         const int32_t idx = (int32_t)keys.size();
         keys.push_back(v);
-        ch.push_back(idx);
+        if (*head < 0) slot_hash[head - slot_head.data()] = h;
+        next.push_back(*head);
+        *head = idx;
+        if (2 * (uint64_t)keys.size() > mask) rehash(2 * (mask + 1));
         return idx;
     }
 };
@@ -353,11 +385,17 @@ int duvi_encode(int32_t nrows, const int32_t *rp, const int32_t *ci, const doubl
     out->block_col = (int32_t *)std::malloc(sizeof(int32_t) * (blocks.empty() ? 1 : blocks.size()));
     std::memcpy(out->block_col, blocks.data(), sizeof(int32_t) * blocks.size());
     // VI: :798-829, with the intended 16-bit switch and raw-value fallback
+    // mode 8 needs no table; mode 0 falls back to raw values once the table passes 65,536 entries,
+    // so the deduplication can stop there
     Dedup dd(tol);
-    std::vector<int32_t> idx(nnz);
-    for (int64_t k = 0; k < nnz; ++k) idx[k] = dd.get(v[k]);
+    std::vector<int32_t> idx(mode == 8 ? 0 : nnz);
+    bool raw = mode == 8;
+    for (int64_t k = 0; k < nnz && !raw; ++k) {
+        idx[k] = dd.get(v[k]);
+        if (mode == 0 && dd.keys.size() > 65536) raw = true;
+    }
     const int64_t nu = (int64_t)dd.keys.size();
-    int vib = mode == 8 ? 0 : (mode == 4 ? 4 : (nu <= 256 ? 1 : (nu <= 65536 ? 2 : 0)));
+    int vib = raw ? 0 : (mode == 4 ? 4 : (nu <= 256 ? 1 : (nu <= 65536 ? 2 : 0)));
     out->vi_bytes = vib;
     if (vib == 0) {
         out->nuniq = nnz;

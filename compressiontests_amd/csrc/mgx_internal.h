@@ -123,6 +123,7 @@ struct BCtl {
     int32_t k;                   // block width
     int32_t stop, max_it;
     double tol;
+    double ctol[kMaxBlock];      // per-column tolerance of the stop test (tol unless the host sets it)
 };
 enum BlockEpi : int { BEPI_SPMM = 0, BEPI_ADD = 1, BEPI_RESID = 2, BEPI_JACOBI = 3, BEPI_DOT = 4,
                       BEPI_UPRESID = 5 };   // node-block only, on an own-and-later-colours copy: residual after a

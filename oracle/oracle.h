@@ -9,8 +9,12 @@
  *
  * Every function names the reference file:line it restates.  Paths are
  * relative to the reference repository (nickreserved/CompressionTests).
- * Built with -O2 -ffp-contract=off so that products and sums round exactly as
- * the reference's managed (.NET, no FMA contraction) code does.
+ * Built with -O3 -ffp-contract=off so that products and sums round exactly as
+ * the reference's managed (.NET, no FMA contraction) code does.  A second build
+ * (liboracle_omp.so, -fopenmp) runs the row loops of the SpMV, residual, Jacobi
+ * and within-colour Gauss-Seidel in parallel: each row is computed by one thread
+ * exactly as in the serial build and the dot products stay serial, so both builds
+ * return bit-identical results; it serves as the all-core CPU baseline.
  *
  * Pinning (what checks each function against something other than itself) is
  * listed in the header comment of each source file.
@@ -22,6 +26,15 @@
 #ifdef __cplusplus
 extern "C" {
 #endif
+
+/* row-independent loops: parallel in the OpenMP build, plain loops otherwise */
+#ifdef _OPENMP
+#define ORC_PARALLEL_FOR _Pragma("omp parallel for schedule(static)")
+#else
+#define ORC_PARALLEL_FOR
+#endif
+/* threads the library runs its parallel loops on (1 in the serial build) */
+int orc_num_threads(void);
 
 /* ---------------- RNG (workload definition; SplitMix64-seeded xoshiro256**) */
 void orc_rng_uniform(uint64_t seed, int64_t n, double *out);  /* U[0,1), top 53 bits */
@@ -163,6 +176,12 @@ int orc_vcycle_apply(int nlevels, const orc_level *lv, int smoother, double omeg
                      const double *r, double *z);
 int orc_mgpcg(int nlevels, const orc_level *lv, int smoother, double omega, int nu,
               int max_it, double tol, int stop, const double *b, double *x, orc_stats *st);
+/* orc_mgpcg that also records the wall time (monotonic clock, seconds) of every PCG iteration
+   in iter_seconds[0 .. max_it) (set-up and the first V-cycle are not part of any entry): the
+   timed CPU baseline. */
+int orc_mgpcg_timed(int nlevels, const orc_level *lv, int smoother, double omega, int nu,
+                    int max_it, double tol, int stop, const double *b, double *x, orc_stats *st,
+                    double *iter_seconds);
 
 /* ---------------- SIMP compliance adjoint */
 void orc_simp_gradient(int nx, int ny, int nz, const double *Ke0, const double *rho, double penal,

@@ -14,7 +14,6 @@ from collections import namedtuple
 import numpy as np
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-_LIB = None
 
 Csr = namedtuple("Csr", "rowptr col val nrows ncols")
 
@@ -50,14 +49,19 @@ class OrcDuviUnits(C.Structure):
                 ("vi", C.POINTER(C.c_int32)), ("uniq", C.POINTER(C.c_double))]
 
 
-def lib():
-    global _LIB
-    if _LIB is None:
-        path = os.path.join(_HERE, "liboracle.so")
+_LIBS = {}
+
+
+def lib(omp=False):
+    """liboracle.so, or with omp=True its OpenMP build liboracle_omp.so (parallel row loops,
+    bit-identical results; the all-core CPU baseline)."""
+    name = "liboracle_omp.so" if omp else "liboracle.so"
+    L = _LIBS.get(name)
+    if L is None:
+        path = os.path.join(_HERE, name)
         if not os.path.exists(path):
-            raise RuntimeError("oracle/liboracle.so is not built (run __graft_entry__.build())")
-        _LIB = C.CDLL(path)
-        L = _LIB
+            raise RuntimeError(f"oracle/{name} is not built (run __graft_entry__.build())")
+        L = C.CDLL(path)
         L.orc_cantilever_nnz.restype = C.c_int64
         L.orc_galerkin_nnz.restype = C.c_int64
         L.orc_dedup.restype = C.c_int64
@@ -65,7 +69,13 @@ def lib():
         L.orc_dot.restype = C.c_double
         L.orc_power_bound.restype = C.c_double
         L.orc_splitmix64_next.restype = C.c_uint64
-    return _LIB
+        _LIBS[name] = L
+    return L
+
+
+def num_threads(omp=False):
+    """Threads the chosen build runs its parallel loops on (orc_num_threads)."""
+    return lib(omp).orc_num_threads()
 
 
 def _p(a):
@@ -441,15 +451,29 @@ def pcg_jacobi(A, b, max_it=10000, tol=1e-5, x0=None):
     return x, st
 
 
-def mgpcg(levels, b, smoother=0, omega=0.6, nu=2, max_it=1000, tol=1e-8, stop=1):
+def mgpcg(levels, b, smoother=0, omega=0.6, nu=2, max_it=1000, tol=1e-8, stop=1, omp=False):
     arr, keep = _levels(levels)
     x = np.zeros(levels[0]["A"].nrows)
     st = OrcStats()
-    rc = lib().orc_mgpcg(len(levels), arr, smoother, C.c_double(omega), nu, max_it, C.c_double(tol), stop,
-                         _p(np.ascontiguousarray(b, np.float64)), _p(x), C.byref(st))
+    rc = lib(omp).orc_mgpcg(len(levels), arr, smoother, C.c_double(omega), nu, max_it, C.c_double(tol), stop,
+                            _p(np.ascontiguousarray(b, np.float64)), _p(x), C.byref(st))
     if rc != 0:
         raise RuntimeError("orc_mgpcg failed: %d" % rc)
     return x, st
+
+
+def mgpcg_timed(levels, b, iterations, smoother=0, omega=0.6, nu=2, omp=False):
+    """orc_mgpcg_timed with the stop test disabled: runs exactly `iterations` PCG iterations and
+    returns (x, stats, seconds of each iteration)."""
+    arr, keep = _levels(levels)
+    x = np.zeros(levels[0]["A"].nrows)
+    st = OrcStats()
+    t = np.zeros(iterations)
+    rc = lib(omp).orc_mgpcg_timed(len(levels), arr, smoother, C.c_double(omega), nu, iterations, C.c_double(0.0), 1,
+                                  _p(np.ascontiguousarray(b, np.float64)), _p(x), C.byref(st), _p(t))
+    if rc != 0:
+        raise RuntimeError("orc_mgpcg_timed failed: %d" % rc)
+    return x, st, t
 
 
 def vcycle_apply(levels, r, smoother=0, omega=0.6, nu=2):

@@ -21,6 +21,26 @@
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
+#include <time.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+int orc_num_threads(void)
+{
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 1;
+#endif
+}
+
+static double now_seconds(void)
+{
+    struct timespec t;
+    clock_gettime(CLOCK_MONOTONIC, &t);
+    return (double)t.tv_sec + 1e-9 * (double)t.tv_nsec;
+}
 
 static double *dalloc(int64_t n) { return (double *)calloc(n > 0 ? n : 1, sizeof(double)); }
 
@@ -249,6 +269,13 @@ static void vcyc_free(vcyc *Vp)
 int orc_mgpcg(int nlevels, const orc_level *lv, int smoother, double omega, int nu,
               int max_it, double tol, int stop, const double *bvec, double *x, orc_stats *st)
 {
+    return orc_mgpcg_timed(nlevels, lv, smoother, omega, nu, max_it, tol, stop, bvec, x, st, NULL);
+}
+
+int orc_mgpcg_timed(int nlevels, const orc_level *lv, int smoother, double omega, int nu,
+                    int max_it, double tol, int stop, const double *bvec, double *x, orc_stats *st,
+                    double *iter_seconds)
+{
     vcyc V;
     if (vcyc_setup(&V, nlevels, lv, smoother, omega, nu) != 0) { vcyc_free(&V); return -2; }
     const int32_t n = lv[0].n;
@@ -266,10 +293,15 @@ int orc_mgpcg(int nlevels, const orc_level *lv, int smoother, double omega, int 
         memcpy(p, V.x[0], sizeof(double) * n);
         double rz = orc_dot(n, r, V.x[0]);
         for (it = 1; it <= max_it; ++it) {
+            const double t0 = iter_seconds ? now_seconds() : 0.0;
             orc_csr_spmv(n, rp, ci, v, p, q, 1);
             double pq = orc_dot(n, p, q);
             double alpha = rz / pq;
             double mx = 0; int bad = 0;
+            /* NaN never wins a max comparison, so the maximum is the same in any order */
+            #ifdef _OPENMP
+            #pragma omp parallel for schedule(static) reduction(max : mx) reduction(| : bad)
+            #endif
             for (int32_t i = 0; i < n; ++i) {
                 x[i] += alpha * p[i];
                 r[i] -= alpha * q[i];
@@ -279,13 +311,19 @@ int orc_mgpcg(int nlevels, const orc_level *lv, int smoother, double omega, int 
             }
             rn = sqrt(orc_dot(n, r, r));
             if (bad) { conv = -1; break; }
-            if ((stop == 1 && rn <= tol * bn) || (stop == 0 && mx <= tol)) { conv = 1; break; }
+            if ((stop == 1 && rn <= tol * bn) || (stop == 0 && mx <= tol)) {
+                conv = 1;
+                if (iter_seconds) iter_seconds[it - 1] = now_seconds() - t0;
+                break;
+            }
             memcpy(V.b[0], r, sizeof(double) * n);
             vcycle(&V, 0);
             double rz2 = orc_dot(n, r, V.x[0]);
             double beta = rz2 / rz;
+            ORC_PARALLEL_FOR
             for (int32_t i = 0; i < n; ++i) p[i] = V.x[0][i] + beta * p[i];
             rz = rz2;
+            if (iter_seconds) iter_seconds[it - 1] = now_seconds() - t0;
         }
         if (it > max_it) it = max_it;
     }

@@ -28,6 +28,7 @@
 void orc_csr_spmv(int32_t nrows, const int32_t *rp, const int32_t *ci, const double *v,
                   const double *x, double *y, int clear)
 {
+    ORC_PARALLEL_FOR
     for (int32_t r = 0; r < nrows; ++r) {
         double acc = clear ? 0.0 : y[r];
         for (int32_t k = rp[r]; k < rp[r + 1]; ++k) acc += v[k] * x[ci[k]];
@@ -38,6 +39,7 @@ void orc_csr_spmv(int32_t nrows, const int32_t *rp, const int32_t *ci, const dou
 void orc_csr_residual(int32_t nrows, const int32_t *rp, const int32_t *ci, const double *v,
                       const double *b, const double *x, double *r)
 {
+    ORC_PARALLEL_FOR
     for (int32_t i = 0; i < nrows; ++i) {
         double acc = 0.0;
         for (int32_t k = rp[i]; k < rp[i + 1]; ++k) acc += v[k] * x[ci[k]];
@@ -61,12 +63,14 @@ int orc_csr_residual_check(int32_t nrows, const int32_t *rp, const int32_t *ci, 
 
 void orc_jacobi_initial(int32_t n, const double *pre, const double *b, double *x)
 {
+    ORC_PARALLEL_FOR
     for (int32_t i = 0; i < n; ++i) x[i] = pre[i] * b[i];
 }
 
 void orc_csr_jacobi(int32_t n, const double *pre, const int32_t *rp, const int32_t *ci, const double *v,
                     const double *b, const double *x, double *y)
 {
+    ORC_PARALLEL_FOR
     for (int32_t i = 0; i < n; ++i) {
         double acc = 0.0;
         for (int32_t k = rp[i]; k < rp[i + 1]; ++k) acc += v[k] * x[ci[k]];
@@ -193,15 +197,19 @@ void orc_gs_multicolor(int32_t n, int bs, const int32_t *rp, const int32_t *ci, 
 {
     int32_t nb = n / bs;
     if (!backward) {
-        for (int c = 0; c < ncolors; ++c)
+        for (int c = 0; c < ncolors; ++c) {
+            ORC_PARALLEL_FOR
             for (int32_t k = 0; k < nb; ++k)
                 if (color[k] == c)
                     for (int a = 0; a < bs; ++a) gs_row(k * bs + a, rp, ci, v, b, x);
+        }
     } else {
-        for (int c = ncolors - 1; c >= 0; --c)
+        for (int c = ncolors - 1; c >= 0; --c) {
+            ORC_PARALLEL_FOR
             for (int32_t k = nb - 1; k >= 0; --k)
                 if (color[k] == c)
                     for (int a = bs - 1; a >= 0; --a) gs_row(k * bs + a, rp, ci, v, b, x);
+        }
     }
 }
 

@@ -130,3 +130,22 @@ def test_vcycle_is_symmetric():
         lhs, rhs = v @ Mu, u @ Mv
         assert abs(lhs - rhs) <= 1e-11 * max(abs(lhs), abs(rhs))
         assert u @ Mu > 0
+
+
+def test_openmp_build_is_bit_identical():
+    """liboracle_omp.so (the all-core CPU baseline) parallelises only row-independent loops and
+    keeps the dot products serial, so it must return the serial build's bits; the timed variant
+    runs the same iterations as the untimed one."""
+    n = 8
+    dens = 10 ** np.random.default_rng(6).uniform(-2, 2, n ** 3)
+    lv = galerkin_levels(n, 3, dens)
+    f = O.tip_load(n, n, n)
+    assert O.num_threads(omp=False) == 1
+    for sm in (0, 1):
+        x1, s1 = O.mgpcg(lv, f, smoother=sm, omega=0.6, nu=2, tol=1e-10)
+        x2, s2 = O.mgpcg(lv, f, smoother=sm, omega=0.6, nu=2, tol=1e-10, omp=True)
+        assert s1.iterations == s2.iterations and x1.tobytes() == x2.tobytes()
+        x3, s3 = O.mgpcg(lv, f, smoother=sm, omega=0.6, nu=2, tol=0.0, max_it=5)
+        x4, s4, t = O.mgpcg_timed(lv, f, 5, smoother=sm, omega=0.6, nu=2, omp=True)
+        assert s4.iterations == 5 and x3.tobytes() == x4.tobytes()
+        assert np.all(t > 0)
