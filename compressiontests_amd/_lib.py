@@ -126,6 +126,11 @@ _SIGS = {
     "mgx_dot": (C.c_int, [_I32, _P, _P, _P, _P]),
     "mgx_dot_noalloc": (C.c_int, [_I32, _P, _P, _P, _P]),
     "mgx_duvi_spmv": (C.c_int, [C.POINTER(Duvi), _P, _P, _I32, _P]),
+    "mgx_duvi_residual": (C.c_int, [C.POINTER(Duvi), _P, _P, _P, _P]),
+    "mgx_duvi_residual_check": (C.c_int, [C.POINTER(Duvi), _P, _P, _P, _D, _P, _P]),
+    "mgx_duvi_jacobi": (C.c_int, [C.POINTER(Duvi), _P, _P, _P, _P, _P]),
+    "mgx_duvi_gs_color": (C.c_int, [C.POINTER(Duvi), _I32, _P, _P, _P, _I32, _P]),
+    "mgx_duvi_spmv_f32": (C.c_int, [C.POINTER(Duvi), _P, _P, _P, _I32, _P]),
     "mgx_duvi_encode": (C.c_int, [_I32, _P, _P, _P, _D, _I32, C.POINTER(DuviHost)]),
     "mgx_duvi_host_free": (None, [C.POINTER(DuviHost)]),
     "mgx_nodeblock_from_csr": (C.c_int, [_I32, _P, _P, _P, _I32, C.POINTER(_P)]),

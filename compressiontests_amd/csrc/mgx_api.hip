@@ -136,6 +136,13 @@ struct DevDuvi {
         d.vi = h.vi_bytes ? vi.p : nullptr; d.values = values.p;
         mgx_duvi_host_free(&h);
     }
+    // FP32 copy of the value table (precision form; indices shared)
+    DBuf<float> values32;
+    void make_f32(hipStream_t s)
+    {
+        values32.alloc((size_t)d.nuniq);
+        if (d.nuniq) launch_convert_f64_f32(d.nuniq, values.p, values32.p, nullptr, s);
+    }
 };
 
 // Node-block (3x3) copy of a node-triple level matrix for the block kernels (mgx_nodeblock.hip):
@@ -311,38 +318,6 @@ struct mgx_nodetransfer_s {
 // small fill / combine kernels used by the drivers
 __global__ void k_set_i32(int32_t *p, int32_t v) { *p = v; }
 
-__global__ __launch_bounds__(kBlock) void k_resid_from_ax(int32_t n, const double *__restrict__ b,
-                                                          const double *__restrict__ ax, double *__restrict__ r,
-                                                          const int32_t *__restrict__ done)
-{
-    if (done && *done) return;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-        r[i] = b[i] - ax[i];
-}
-
-__global__ __launch_bounds__(kBlock) void k_resid_flag_from_ax(int32_t n, const double *__restrict__ b,
-                                                               const double *__restrict__ ax, double *__restrict__ r,
-                                                               double tol, int32_t *flag)
-{
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock) {
-        const double ri = b[i] - ax[i];
-        r[i] = ri;
-        const double c = fabs(ri);
-        if (ri != ri || c > 1e50) atomicOr(flag, 2);
-        else if (c > tol) atomicAnd(flag, 2);
-    }
-}
-
-__global__ __launch_bounds__(kBlock) void k_jacobi_from_ax(int32_t n, const double *__restrict__ pre,
-                                                           const double *__restrict__ b, const double *__restrict__ x,
-                                                           const double *__restrict__ ax, double *__restrict__ y,
-                                                           const int32_t *__restrict__ done)
-{
-    if (done && *done) return;
-    for (int64_t i = blockIdx.x * (int64_t)kBlock + threadIdx.x; i < n; i += (int64_t)gridDim.x * kBlock)
-        y[i] = x[i] + pre[i] * (b[i] - ax[i]);
-}
-
 __global__ __launch_bounds__(kBlock) void k_scale_into(int32_t n, const double *__restrict__ d,
                                                        const double *__restrict__ r, double *__restrict__ z,
                                                        const int32_t *__restrict__ done)
@@ -465,29 +440,31 @@ static void level_spmv(mgx_solver_s *S, Level &L, DevCsr &M, DevDuvi &D, const d
 static void level_residual(mgx_solver_s *S, Level &L, const double *b, const double *x, double *r, double *tmp,
                            const int32_t *done)
 {
-    if (S->opt.format == MGX_FORMAT_DUVI) {
-        launch_duvi_spmv(L.dA.d, x, tmp, true, done, S->stream);
-        hipLaunchKernelGGL(k_resid_from_ax, dim3(grid_vec(L.n)), dim3(kBlock), 0, S->stream, L.n, b, tmp, r, done);
-        MGX_HIP(hipGetLastError());
-    } else launch_residual(L.n, L.A.rp, L.A.ci, L.A.v, b, x, r, L.A.lanes, done, S->stream);
+    (void)tmp;
+    if (S->opt.format == MGX_FORMAT_DUVI)
+        launch_duvi<double>(L.dA.d, L.dA.d.values, DEPI_RESID, x, b, nullptr, r, nullptr, 0.0, done, S->stream);
+    else launch_residual(L.n, L.A.rp, L.A.ci, L.A.v, b, x, r, L.A.lanes, done, S->stream);
 }
 
 static void level_jacobi(mgx_solver_s *S, Level &L, const double *b, const double *x, double *y, double *tmp,
                          const int32_t *done)
 {
-    if (S->opt.format == MGX_FORMAT_DUVI) {
-        launch_duvi_spmv(L.dA.d, x, tmp, true, done, S->stream);
-        hipLaunchKernelGGL(k_jacobi_from_ax, dim3(grid_vec(L.n)), dim3(kBlock), 0, S->stream, L.n, L.pre.p, b, x, tmp,
-                           y, done);
-        MGX_HIP(hipGetLastError());
-    } else launch_jacobi(L.n, L.pre.p, L.A.rp, L.A.ci, L.A.v, b, x, y, L.A.lanes, done, S->stream);
+    (void)tmp;
+    if (S->opt.format == MGX_FORMAT_DUVI)
+        launch_duvi<double>(L.dA.d, L.dA.d.values, DEPI_JACOBI, x, b, L.pre.p, y, nullptr, 0.0, done, S->stream);
+    else launch_jacobi(L.n, L.pre.p, L.A.rp, L.A.ci, L.A.v, b, x, y, L.A.lanes, done, S->stream);
 }
 
 static void level_gs(mgx_solver_s *S, Level &L, const double *b, double *x, bool backward, const int32_t *done)
 {
     for (int i = 0; i < L.ncolors; ++i) {
         const int c = backward ? L.ncolors - 1 - i : i;
-        launch_gs_color(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v, b, x, backward, done, S->stream);
+        if (S->opt.format == MGX_FORMAT_DUVI)
+            launch_duvi_gs<double>(L.dA.d, L.dA.d.values, L.color_count[c], L.color_nodes[c].p, b, x, backward, done,
+                                   S->stream);
+        else
+            launch_gs_color(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v, b, x, backward, done,
+                            S->stream);
     }
 }
 
@@ -554,11 +531,8 @@ static void initialize(mgx_solver_s *S)
                     "Jacobi ping-pong needs an even number of smoother iterations");
     }
     if (S->opt.method == MGX_METHOD_MGPCG) MGX_REQUIRE(L >= 2, MGX_ERR_INVALID_ARGUMENT, "MG-PCG needs >= 2 levels");
-    MGX_REQUIRE(!(S->opt.format == MGX_FORMAT_DUVI && S->opt.smoother == MGX_SMOOTHER_GS), MGX_ERR_NOT_IMPLEMENTED,
-                "multicolour Gauss-Seidel runs on the CSR format only");
-    MGX_REQUIRE(S->opt.precision == MGX_PRECISION_FP64 ||
-                    (S->opt.method == MGX_METHOD_MGPCG && S->opt.format == MGX_FORMAT_CSR),
-                MGX_ERR_NOT_IMPLEMENTED, "the precision form is MG-PCG on CSR");
+    MGX_REQUIRE(S->opt.precision == MGX_PRECISION_FP64 || S->opt.method == MGX_METHOD_MGPCG, MGX_ERR_NOT_IMPLEMENTED,
+                "the precision form is MG-PCG");
     // upload / attach level matrices
     for (int l = 0; l < L; ++l) {
         Level &V = S->lv[l];
@@ -742,6 +716,10 @@ static void initialize(mgx_solver_s *S)
                     launch_convert_f64_f32(V.n, V.pre.p, V.pre32.p, nullptr, st);
                 }
             }
+            if (S->opt.format == MGX_FORMAT_DUVI) {
+                V.dA.make_f32(st);
+                if (l < L - 1) { V.dR.make_f32(st); V.dP.make_f32(st); }
+            }
             V.x32.alloc(V.n); V.b32.alloc(V.n); V.r32.alloc(V.n); V.t32.alloc(V.n); V.w32.alloc(V.n);
         }
         const int32_t n0 = S->lv[0].n;
@@ -816,11 +794,10 @@ static void solve_gmg(mgx_solver_s *S, const double *b, double *x, bool has_x0, 
                 if (down) {
                     if (cur == 0) {
                         hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, S->flag.p, 1);
-                        if (S->opt.format == MGX_FORMAT_DUVI) {
-                            launch_duvi_spmv(V.dA.d, X[0], tmp, true, nullptr, s);
-                            hipLaunchKernelGGL(k_resid_flag_from_ax, dim3(grid_vec(V.n)), dim3(kBlock), 0, s, V.n, B[0],
-                                               tmp, R, S->opt.tolerance, S->flag.p);
-                        } else
+                        if (S->opt.format == MGX_FORMAT_DUVI)
+                            launch_duvi<double>(V.dA.d, V.dA.d.values, DEPI_CHECK, X[0], B[0], nullptr, R, S->flag.p,
+                                                S->opt.tolerance, nullptr, s);
+                        else
                             launch_residual_flag(V.n, V.A.rp, V.A.ci, V.A.v, B[0], X[0], R, S->opt.tolerance, S->flag.p,
                                                  V.A.lanes, s);
                         MGX_HIP(hipMemcpyAsync(S->flag_host, S->flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
@@ -917,7 +894,7 @@ static void solve_pcg(mgx_solver_s *S, const double *b, double *x, bool has_x0, 
     MGX_HIP(hipMemcpyAsync(ctl, S->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, s));
     // r = b - A x0 (or b), x = x0 (or 0)
     if (has_x0) {
-        launch_residual(n, V.A.rp, V.A.ci, V.A.v, b, x, S->r.p, V.A.lanes, nullptr, s);
+        level_residual(S, V, b, x, S->r.p, nullptr, nullptr);
     } else {
         MGX_HIP(hipMemcpyAsync(S->r.p, b, sizeof(double) * n, hipMemcpyDeviceToDevice, s));
         launch_fill_zero(n, x, nullptr, s);
@@ -930,7 +907,11 @@ static void solve_pcg(mgx_solver_s *S, const double *b, double *x, bool has_x0, 
     S->prof_used = 0;
     auto one_iteration = [&](bool prof) {
         if (prof) MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used], s));
-        launch_spmv_dot_alpha(n, V.A.rp, V.A.ci, V.A.v, S->p.p, S->q.p, V.A.lanes, ctl, S->partials.p, s);
+        if (S->opt.format == MGX_FORMAT_DUVI) {
+            launch_duvi<double>(V.dA.d, V.dA.d.values, DEPI_SPMV, S->p.p, nullptr, nullptr, S->q.p, nullptr, 0.0, done, s);
+            launch_dot_alpha<double>(n, S->p.p, S->q.p, ctl, S->partials.p, s);
+        } else
+            launch_spmv_dot_alpha(n, V.A.rp, V.A.ci, V.A.v, S->p.p, S->q.p, V.A.lanes, ctl, S->partials.p, s);
         if (prof) { MGX_HIP(hipEventRecord(S->prof_events[2 * S->prof_used + 1], s)); ++S->prof_used; }
         launch_pcg_update(n, S->p.p, S->q.p, x, S->r.p, ctl, S->partials.p, s);
         apply_prec(S, S->r.p, S->z.p, done);
@@ -988,7 +969,7 @@ static void finish_stats(mgx_solver_s *S, const double *b, const double *x, mgx_
     // true residual ||b - A x|| (one extra SpMV, FP64)
     Level &V = S->lv[0];
     hipStream_t s = S->stream;
-    launch_residual(V.n, V.A.rp, V.A.ci, V.A.v, b, x, S->q.p, V.A.lanes, nullptr, s);
+    level_residual(S, V, b, x, S->q.p, nullptr, nullptr);
     double *res = S->partials.p + S->partials_tail;
     launch_dot(V.n, S->q.p, S->q.p, res, S->partials.p, &S->ctl.p->counter[5], s);
     launch_dot(V.n, b, b, res + 1, S->partials.p, &S->ctl.p->counter[5], s);
@@ -1000,13 +981,42 @@ static void finish_stats(mgx_solver_s *S, const double *b, const double *x, mgx_
 }
 
 // ---------------------------------------------------------------- precision form
+// FP32 twins of the level operations: the FP32 copy of the CSR values or of the DuVi value table
 static void level_gs32(mgx_solver_s *S, Level &L, const float *b, float *x, bool backward, const int32_t *done)
 {
     for (int i = 0; i < L.ncolors; ++i) {
         const int c = backward ? L.ncolors - 1 - i : i;
-        launch_gs_color_f32(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v32, b, x, backward, done,
-                            S->stream);
+        if (S->opt.format == MGX_FORMAT_DUVI)
+            launch_duvi_gs<float>(L.dA.d, L.dA.values32.p, L.color_count[c], L.color_nodes[c].p, b, x, backward, done,
+                                  S->stream);
+        else
+            launch_gs_color_f32(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v32, b, x, backward, done,
+                                S->stream);
     }
+}
+
+static void level_jacobi32(mgx_solver_s *S, Level &L, const float *b, const float *x, float *y, const int32_t *done)
+{
+    if (S->opt.format == MGX_FORMAT_DUVI)
+        launch_duvi<float>(L.dA.d, L.dA.values32.p, DEPI_JACOBI, x, b, L.pre32.p, y, nullptr, 0.0, done, S->stream);
+    else launch_jacobi_f32(L.n, L.pre32.p, L.A.rp, L.A.ci, L.A.v32, b, x, y, L.A.lanes, done, S->stream);
+}
+
+static void level_residual32(mgx_solver_s *S, Level &L, const float *b, const float *x, float *r, const int32_t *done)
+{
+    if (S->opt.format == MGX_FORMAT_DUVI)
+        launch_duvi<float>(L.dA.d, L.dA.values32.p, DEPI_RESID, x, b, nullptr, r, nullptr, 0.0, done, S->stream);
+    else launch_residual_f32(L.n, L.A.rp, L.A.ci, L.A.v32, b, x, r, L.A.lanes, done, S->stream);
+}
+
+// y = M x (clear) or y += M x with the level's R or P
+static void level_transfer32(mgx_solver_s *S, DevCsr &M, DevDuvi &D, const float *x, float *y, bool clear,
+                             const int32_t *done)
+{
+    if (S->opt.format == MGX_FORMAT_DUVI)
+        launch_duvi<float>(D.d, D.values32.p, clear ? DEPI_SPMV : DEPI_ADD, x, nullptr, nullptr, y, nullptr, 0.0, done,
+                           S->stream);
+    else launch_spmv_f32(M.nrows, M.rp, M.ci, M.v32, x, y, clear, M.lanes, done, S->stream);
 }
 
 // FP32 twin of vcycle(): FP32 values and vectors, FP64 coarse solve.
@@ -1027,21 +1037,21 @@ static void vcycle32(mgx_solver_s *S, int l, const float *b, float *x, const int
         cur = V.t32.p;
         for (int i = 1; i < nu; ++i) {
             float *o = (cur == x) ? V.t32.p : x;
-            launch_jacobi_f32(V.n, V.pre32.p, V.A.rp, V.A.ci, V.A.v32, b, cur, o, V.A.lanes, done, s);
+            level_jacobi32(S, V, b, cur, o, done);
             cur = o;
         }
     }
-    launch_residual_f32(V.n, V.A.rp, V.A.ci, V.A.v32, b, cur, V.r32.p, V.A.lanes, done, s);
+    level_residual32(S, V, b, cur, V.r32.p, done);
     Level &C = S->lv[l + 1];
-    launch_spmv_f32(V.R.nrows, V.R.rp, V.R.ci, V.R.v32, V.r32.p, C.b32.p, true, V.R.lanes, done, s);
+    level_transfer32(S, V.R, V.dR, V.r32.p, C.b32.p, true, done);
     vcycle32(S, l + 1, C.b32.p, C.x32.p, done);
-    launch_spmv_f32(V.P.nrows, V.P.rp, V.P.ci, V.P.v32, C.x32.p, cur, false, V.P.lanes, done, s);
+    level_transfer32(S, V.P, V.dP, C.x32.p, cur, false, done);
     if (S->opt.smoother == MGX_SMOOTHER_GS) {
         for (int i = 0; i < nu; ++i) level_gs32(S, V, b, x, true, done);
     } else {
         for (int i = 0; i < nu; ++i) {
             float *o = (cur == x) ? V.t32.p : x;
-            launch_jacobi_f32(V.n, V.pre32.p, V.A.rp, V.A.ci, V.A.v32, b, cur, o, V.A.lanes, done, s);
+            level_jacobi32(S, V, b, cur, o, done);
             cur = o;
         }
         if (cur != x) MGX_HIP(hipMemcpyAsync(x, cur, sizeof(float) * V.n, hipMemcpyDeviceToDevice, s));
@@ -1086,7 +1096,7 @@ static void solve_mixed(mgx_solver_s *S, const double *b, double *x, bool has_x0
     for (;;) {
         *S->ctl_host = outer;
         MGX_HIP(hipMemcpyAsync(co, S->ctl_host, sizeof(Ctl), hipMemcpyHostToDevice, s));
-        launch_residual(n, V.A.rp, V.A.ci, V.A.v, b, x, S->r.p, V.A.lanes, nullptr, s);
+        level_residual(S, V, b, x, S->r.p, nullptr, nullptr);
         launch_pcg_norms(n, b, S->r.p, co, S->partials.p, kPartialStride, s);
         MGX_HIP(hipMemcpyAsync(S->ctl_host, co, sizeof(Ctl), hipMemcpyDeviceToHost, s));
         MGX_HIP(hipStreamSynchronize(s));
@@ -1110,7 +1120,13 @@ static void solve_mixed(mgx_solver_s *S, const double *b, double *x, bool has_x0
         launch_update_p_f32(n, S->z32.p, S->p32.p, ci, true, s);
         for (int it = 0;; it += chunk) {
             for (int i = 0; i < chunk; ++i) {
-                launch_spmv_dot_alpha_f32(n, V.A.rp, V.A.ci, V.A.v32, S->p32.p, S->q32.p, V.A.lanes, ci, S->partials.p, s);
+                if (S->opt.format == MGX_FORMAT_DUVI) {
+                    launch_duvi<float>(V.dA.d, V.dA.values32.p, DEPI_SPMV, S->p32.p, nullptr, nullptr, S->q32.p, nullptr,
+                                       0.0, done, s);
+                    launch_dot_alpha<float>(n, S->p32.p, S->q32.p, ci, S->partials.p, s);
+                } else
+                    launch_spmv_dot_alpha_f32(n, V.A.rp, V.A.ci, V.A.v32, S->p32.p, S->q32.p, V.A.lanes, ci,
+                                              S->partials.p, s);
                 launch_pcg_update_f32(n, S->p32.p, S->q32.p, S->d32.p, S->r32.p, ci, S->partials.p, s);
                 vcycle32(S, 0, S->r32.p, S->z32.p, done);
                 launch_dot_beta_f32(n, S->r32.p, S->z32.p, ci, S->partials.p, false, s);
@@ -2023,6 +2039,60 @@ int mgx_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t clear, 
         MGX_REQUIRE(A->vi_bytes == 0 || A->vi_bytes == 1 || A->vi_bytes == 2 || A->vi_bytes == 4,
                     MGX_ERR_INVALID_ARGUMENT, "vi_bytes must be 0, 1, 2 or 4");
         launch_duvi_spmv(*A, x, y, clear != 0, nullptr, (hipStream_t)stream);
+    });
+}
+
+#define MGX_CHECK_DUVI(A)                                                                                    \
+    do {                                                                                                     \
+        MGX_REQUIRE((A) && (A)->nrows >= 0, MGX_ERR_INVALID_ARGUMENT, "invalid DuVi matrix");                \
+        MGX_REQUIRE((A)->vi_bytes == 0 || (A)->vi_bytes == 1 || (A)->vi_bytes == 2 || (A)->vi_bytes == 4,    \
+                    MGX_ERR_INVALID_ARGUMENT, "vi_bytes must be 0, 1, 2 or 4");                              \
+    } while (0)
+
+int mgx_duvi_residual(const mgx_duvi *A, const double *b, const double *x, double *r, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_DUVI(A);
+        launch_duvi<double>(*A, A->values, DEPI_RESID, x, b, nullptr, r, nullptr, 0.0, nullptr, (hipStream_t)stream);
+    });
+}
+
+int mgx_duvi_residual_check(const mgx_duvi *A, const double *b, const double *x, double *r, double tol,
+                            int32_t *flag, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_DUVI(A);
+        MGX_REQUIRE(flag, MGX_ERR_INVALID_ARGUMENT, "flag must not be NULL");
+        launch_duvi<double>(*A, A->values, DEPI_CHECK, x, b, nullptr, r, flag, tol, nullptr, (hipStream_t)stream);
+    });
+}
+
+int mgx_duvi_jacobi(const mgx_duvi *A, const double *pre, const double *b, const double *x, double *y, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_DUVI(A);
+        launch_duvi<double>(*A, A->values, DEPI_JACOBI, x, b, pre, y, nullptr, 0.0, nullptr, (hipStream_t)stream);
+    });
+}
+
+int mgx_duvi_gs_color(const mgx_duvi *A, int32_t nnodes, const int32_t *nodes, const double *b, double *x,
+                      int32_t backward, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_DUVI(A);
+        MGX_REQUIRE(nnodes >= 0 && (nnodes == 0 || nodes), MGX_ERR_INVALID_ARGUMENT, "invalid node list");
+        MGX_REQUIRE(A->nrows % 3 == 0, MGX_ERR_INVALID_ARGUMENT, "rows must come in node triples");
+        launch_duvi_gs<double>(*A, A->values, nnodes, nodes, b, x, backward != 0, nullptr, (hipStream_t)stream);
+    });
+}
+
+int mgx_duvi_spmv_f32(const mgx_duvi *A, const float *values32, const float *x, float *y, int32_t clear, void *stream)
+{
+    return guard([&] {
+        MGX_CHECK_DUVI(A);
+        MGX_REQUIRE(values32 || A->nuniq == 0, MGX_ERR_INVALID_ARGUMENT, "values32 must not be NULL");
+        launch_duvi<float>(*A, values32, clear ? DEPI_SPMV : DEPI_ADD, x, nullptr, nullptr, y, nullptr, 0.0, nullptr,
+                           (hipStream_t)stream);
     });
 }
 

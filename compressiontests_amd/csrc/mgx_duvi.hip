@@ -18,14 +18,27 @@
 // slot, and the lane decodes its slots again, gathering x and accumulating the products; (4) the
 // 8 lanes sum the row.  A tile of more than kDwCap slots falls back to one lane per row decoding
 // straight from global memory.
+//
+// The row result goes through one of the epilogues of the reference's DuVi kernels, fused into
+// the same pass (no intermediate A x vector): y = A x, y += A x (matrix_vector_product,
+// DuViGeometricMultigrid.cl:122-159), r = b - A x (residual, :220-254), the same with the
+// convergence flag (residual_with_check, :275-317), y = x + pre (b - A x) (jacobi_iteration,
+// :362-402).  Values and vectors are FP64, or FP32 for the precision form (an FP32 copy of the
+// value table; indices unchanged).
+//
+// k_duvi_gs is one colour of a multicolour Gauss-Seidel sweep on the same layout (the
+// reference's gauss_seidel_iteration, :422-462, updates x in place with racy reads; the
+// multicolour order is the documented deviation).
+#include <type_traits>
+
 #include "mgx_device.h"
 
 namespace mgx {
 
 typedef unsigned int dw_u32x4 __attribute__((ext_vector_type(4)));
 
-template <int VIB>
-__device__ __forceinline__ double duvi_value(const void *vi, const double *values, int64_t d)
+template <int VIB, typename T>
+__device__ __forceinline__ T duvi_value(const void *vi, const T *values, int64_t d)
 {
     if (VIB == 1) return values[((const uint8_t *)vi)[d]];
     if (VIB == 2) return values[((const uint16_t *)vi)[d]];
@@ -76,8 +89,8 @@ __device__ __forceinline__ void dw_stage(const char *__restrict__ src, int64_t l
 // staged index stream: 16-byte rounds of 64 slots per wave (none in raw mode)
 template <int VIB> struct DwIdx { static constexpr int NI = VIB == 0 ? 1 : (kDwCap * VIB / 16 + 2 + 63) / 64; };
 
-template <int VIB>
-__device__ __forceinline__ double dw_value(const uint8_t *si, const double *__restrict__ values, int64_t d0, int k)
+template <int VIB, typename T>
+__device__ __forceinline__ T dw_value(const uint8_t *si, const T *__restrict__ values, int64_t d0, int k)
 {
     if (VIB == 0) return values[d0 + k];
     if (VIB == 1) return values[si[k]];
@@ -85,19 +98,61 @@ __device__ __forceinline__ double dw_value(const uint8_t *si, const double *__re
     return values[((const uint32_t *)si)[k]];
 }
 
-template <int VIB>
+// operands of the fused epilogues
+template <typename T> struct DuviOp {
+    const T *values;          // value table (or the raw values)
+    const T *x, *b, *pre;     // b: RESID, JACOBI, CHECK; pre: JACOBI
+    T *y;
+    int32_t *flag;            // CHECK
+    double tol;               // CHECK
+};
+
+// result of one row through the epilogue; the CHECK form also returns the flag bits of the row
+// (2: NaN or |r| > 1e50, 1: |r| > tol)
+template <int EPI, typename T>
+__device__ __forceinline__ int duvi_store(const DuviOp<T> &op, int row, T acc)
+{
+    if (EPI == DEPI_SPMV) op.y[row] = acc;
+    else if (EPI == DEPI_ADD) op.y[row] = op.y[row] + acc;
+    else if (EPI == DEPI_RESID) op.y[row] = op.b[row] - acc;
+    else if (EPI == DEPI_JACOBI) op.y[row] = op.x[row] + op.pre[row] * (op.b[row] - acc);
+    else {
+        const T ri = op.b[row] - acc;
+        op.y[row] = ri;
+        const double c = fabs((double)ri);
+        if (ri != ri || c > 1e50) return 2;
+        if (c > op.tol) return 1;
+    }
+    return 0;
+}
+
+// the flag protocol of residual_with_check (bit 1 set on NaN / overflow, bit 0 cleared when a row
+// exceeds tol), one atomic per wavefront and kind; the final flag does not depend on the order
+template <int EPI, typename T>
+__device__ __forceinline__ void duvi_flag(const DuviOp<T> &op, int bits)
+{
+    if (EPI != DEPI_CHECK) return;
+    const bool bad = __any(bits & 2), over = __any(bits & 1);
+    if ((threadIdx.x & 63) == 0) {
+        if (bad) atomicOr(op.flag, 2);
+        if (over) atomicAnd(op.flag, 2);
+    }
+}
+
+template <int VIB, typename T, int EPI>
 __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz, const int32_t *__restrict__ row_blocks,
                                                       const int32_t *__restrict__ row_dist,
                                                       const int32_t *__restrict__ block_col,
                                                       const uint8_t *__restrict__ dist, const void *__restrict__ vi,
-                                                      const double *__restrict__ values, const double *__restrict__ x,
-                                                      double *__restrict__ y, int clear, const int32_t *__restrict__ done)
+                                                      const DuviOp<T> op, const int32_t *__restrict__ done)
 {
     __shared__ dw_u32x4 s_iq[kDwWaves][64 * DwIdx<VIB>::NI];  // value indices (not used in raw mode)
     __shared__ dw_u32x4 s_dq[kDwWaves][64 * kDwDistNI];       // distance bytes
     __shared__ int32_t s_h[kDwWaves][kDwCap];                 // column + 1 at a block's first slot, else 0
     static_assert(64 * 16 * kDwDistNI >= kDwCap + 32, "distance staging");
     if (done && *done) return;
+    const T *__restrict__ values = op.values;
+    const T *__restrict__ x = op.x;
     const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, r = lane / kDwLanes, l = lane % kDwLanes;
     const int ntiles = (nrows + kDwRows - 1) / kDwRows;
     const int tile = xcd_remap(blockIdx.x, gridDim.x) * kDwWaves + wv;
@@ -113,20 +168,22 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
     const int cnt = (int)(dr1 - d0);
     if (cnt > kDwCap) {
         // long tile: one lane per row, decoding straight from global memory
+        int bits = 0;
         if (l == 0 && row < r1) {
-            double acc = clear ? 0.0 : y[row];
+            T acc = 0;
             int64_t d = row_dist[row];
             for (int b = row_blocks[row]; b < row_blocks[row + 1]; ++b) {
                 int col = block_col[b];
                 const int64_t dend = d + dist[d] + 1;
                 for (;;) {
-                    acc += duvi_value<VIB>(vi, values, d) * x[col];
+                    acc += duvi_value<VIB, T>(vi, values, d) * x[col];
                     if (++d == dend) break;
                     col += dist[d] + 1;
                 }
             }
-            y[row] = acc;
+            bits = duvi_store<EPI, T>(op, row, acc);
         }
+        duvi_flag<EPI, T>(op, bits);
         return;
     }
     const int s = (int)(ds - d0), e = (int)(de - d0);
@@ -136,11 +193,11 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
     const int bcB = rb0 + kDwLanes + l < rb1 ? block_col[rb0 + kDwLanes + l] : 0;
     // (1) raw values of this lane's slots in round 0 (they do not depend on the decode); the
     //     index stream and the distance bytes to LDS; cleared start marks
-    double w0[kDwSeg];
+    T w0[kDwSeg];
     if (VIB == 0) {
         const int a = s + l * kDwSeg;
 #pragma unroll
-        for (int j = 0; j < kDwSeg; ++j) w0[j] = a + j < e ? values[d0 + a + j] : 0.0;
+        for (int j = 0; j < kDwSeg; ++j) w0[j] = a + j < e ? values[d0 + a + j] : T(0);
     }
     if (VIB != 0) dw_stage<DwIdx<VIB>::NI>((const char *)vi, d0 * VIB, (d0 + cnt) * VIB, nnz * VIB, s_iq[wv], lane);
     dw_stage<kDwDistNI>((const char *)dist, d0, d0 + cnt, nnz, s_dq[wv], lane);
@@ -164,7 +221,7 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
     }
     dw_fence();
     // (3) rounds of 96 slots per row
-    double acc = 0;
+    T acc = 0;
     int carry = 0;                                             // column before the round's first slot
     for (int base = s, round = 0; base < e; base += kDwLanes * kDwSeg, ++round) {
         const int a = base + l * kDwSeg, bnd = min(a + kDwSeg, e);
@@ -192,7 +249,7 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
         const int th = __shfl(h, kDwLanes - 1, kDwLanes), tv = __shfl(v, kDwLanes - 1, kDwLanes);
         carry = th ? tv : carry + tv;
         // decode again, gathering x
-        double xv[kDwSeg], w[kDwSeg];
+        T xv[kDwSeg], w[kDwSeg];
 #pragma unroll
         for (int j = 0; j < kDwSeg; ++j) {
             const int k = a + j;
@@ -202,9 +259,9 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
                 col = hc ? hc - 1 : col + sd[k] + 1;
                 c = col;
             }
-            xv[j] = k < bnd ? x[c] : 0.0;
+            xv[j] = k < bnd ? x[c] : T(0);
             if (VIB == 0 && round == 0) w[j] = w0[j];
-            else w[j] = k < bnd ? dw_value<VIB>(si, values, d0, k) : 0.0;
+            else w[j] = k < bnd ? dw_value<VIB, T>(si, values, d0, k) : T(0);
         }
 #pragma unroll
         for (int j = 0; j < kDwSeg; ++j) acc += w[j] * xv[j];
@@ -212,23 +269,156 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
     // (4) row sum over the 8 lanes
 #pragma unroll
     for (int o = kDwLanes / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kDwLanes);
-    if (l == 0 && row < r1) y[row] = clear ? acc : y[row] + acc;
+    int bits = 0;
+    if (l == 0 && row < r1) bits = duvi_store<EPI, T>(op, row, acc);
+    duvi_flag<EPI, T>(op, bits);
 }
+
+// One colour of a multicolour Gauss-Seidel sweep: the nodes nodes[0 .. nnodes) are pairwise
+// uncoupled; node q's rows 3q, 3q+1, 3q+2 are relaxed in that order (the reverse when backward)
+// with the current x.  A wavefront takes 2 nodes, 8 lanes per row (lanes 48-63 idle).  A row's
+// lanes walk its block chain together (one distance byte per block, the same address in all 8
+// lanes); lane l keeps the first slot of blocks l, l + 8, ... of each round of 8 and decodes
+// them one slot at a time, gathering x and recording the entries of the node's own three
+// columns.  The row sums (the node's own x still old) and the own 3x3 block meet in the node's
+// first lane, which relaxes the three rows in turn, correcting each row sum for the node's
+// values already updated: x_a = x_a + (b_a - sum_j A_aj x_j) / A_aa with x current.
+template <int VIB, typename T>
+__global__ __launch_bounds__(kBlock) void k_duvi_gs(int32_t nnodes, const int32_t *__restrict__ nodes,
+                                                    const int32_t *__restrict__ row_blocks,
+                                                    const int32_t *__restrict__ row_dist,
+                                                    const int32_t *__restrict__ block_col,
+                                                    const uint8_t *__restrict__ dist, const void *__restrict__ vi,
+                                                    const T *__restrict__ values, const T *__restrict__ b,
+                                                    T *__restrict__ x, int backward, const int32_t *__restrict__ done)
+{
+    if (done && *done) return;
+    const int wv = threadIdx.x >> 6, lane = threadIdx.x & 63, grp = lane >> 3, l = lane & 7;
+    const int ns = grp / 3, ra = grp - 3 * ns;                 // node slot (2: idle lanes), row of the node
+    const int64_t k = ((int64_t)xcd_remap(blockIdx.x, gridDim.x) * kDwWaves + wv) * 2 + ns;
+    if (k - ns >= nnodes) return;                              // the whole wave leaves together
+    const bool act = ns < 2 && k < nnodes;
+    const int q = act ? nodes[k] : 0;
+    T acc = 0, own[3] = {0, 0, 0};
+    if (act) {
+        const int row = 3 * q + ra;
+        const int rb1 = row_blocks[row + 1];
+        int64_t d = row_dist[row];
+        for (int bj = row_blocks[row]; bj < rb1; bj += kDwLanes) {
+            const int nb = min(kDwLanes, rb1 - bj);
+            int64_t my = -1;
+            for (int j = 0; j < nb; ++j) {
+                if (l == j) my = d;
+                d += dist[d] + 1;
+            }
+            if (my < 0) continue;
+            int col = block_col[bj + l];
+            int64_t dd = my;
+            const int64_t dend = dd + dist[dd] + 1;
+            for (;;) {
+                const T a = duvi_value<VIB, T>(vi, values, dd);
+                acc += a * x[col];
+                const int oc = col - 3 * q;
+                if (oc >= 0 && oc < 3) own[oc] = a;
+                if (++dd == dend) break;
+                col += dist[dd] + 1;
+            }
+        }
+    }
+#pragma unroll
+    for (int o = kDwLanes / 2; o > 0; o >>= 1) {
+        acc += __shfl_xor(acc, o, kDwLanes);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) own[c] += __shfl_xor(own[c], o, kDwLanes);
+    }
+    // the node's three row sums and its own block, in the node's first lane
+    const int base = 24 * min(ns, 1);
+    T S[3], A[3][3];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        S[a] = __shfl(acc, base + 8 * a);
+#pragma unroll
+        for (int c = 0; c < 3; ++c) A[a][c] = __shfl(own[c], base + 8 * a);
+    }
+    if (!act || ra != 0 || l != 0) return;
+    T xo[3], xn[3];
+    const size_t i0 = (size_t)3 * q;
+#pragma unroll
+    for (int a = 0; a < 3; ++a) { xo[a] = x[i0 + a]; xn[a] = xo[a]; }
+#pragma unroll
+    for (int t = 0; t < 3; ++t) {
+        const int a = backward ? 2 - t : t;
+        T r = b[i0 + a] - S[a];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) r -= A[a][c] * (xn[c] - xo[c]);   // zero for rows not yet relaxed
+        xn[a] = xo[a] + r / A[a][a];
+    }
+#pragma unroll
+    for (int a = 0; a < 3; ++a) x[i0 + a] = xn[a];
+}
+
+template <typename T>
+void launch_duvi(const mgx_duvi &A, const T *values, int epi, const T *x, const T *b, const T *pre, T *y,
+                 int32_t *flag, double tol, const int32_t *done, hipStream_t s)
+{
+    if (A.nrows <= 0) return;
+    if (epi == DEPI_CHECK && !std::is_same<T, double>::value)
+        throw Error{MGX_ERR_INVALID_ARGUMENT, "the residual check runs in FP64"};
+    const dim3 g(grid_rows((A.nrows + kDwRows - 1) / kDwRows, kDwWaves)), bl(kBlock);
+    const DuviOp<T> op{values, x, b, pre, y, flag, tol};
+#define MGX_DV(V, E) hipLaunchKernelGGL((k_duvi_wave<V, T, E>), g, bl, 0, s, A.nrows, A.nnz, A.row_blocks, A.row_dist, \
+                                        A.block_col, A.dist, A.vi, op, done)
+#define MGX_DVE(V)                                                                                               \
+    switch (epi) {                                                                                               \
+    case DEPI_SPMV: MGX_DV(V, DEPI_SPMV); break;                                                                 \
+    case DEPI_ADD: MGX_DV(V, DEPI_ADD); break;                                                                   \
+    case DEPI_RESID: MGX_DV(V, DEPI_RESID); break;                                                               \
+    case DEPI_JACOBI: MGX_DV(V, DEPI_JACOBI); break;                                                             \
+    default:                                                                                                     \
+        if constexpr (std::is_same<T, double>::value) MGX_DV(V, DEPI_CHECK);                                     \
+        break;                                                                                                   \
+    }
+    switch (A.vi_bytes) {
+    case 1: MGX_DVE(1); break;
+    case 2: MGX_DVE(2); break;
+    case 4: MGX_DVE(4); break;
+    default: MGX_DVE(0); break;
+    }
+#undef MGX_DVE
+#undef MGX_DV
+    MGX_HIP(hipGetLastError());
+}
+
+template <typename T>
+void launch_duvi_gs(const mgx_duvi &A, const T *values, int32_t nnodes, const int32_t *nodes, const T *b, T *x,
+                    bool backward, const int32_t *done, hipStream_t s)
+{
+    if (nnodes <= 0) return;
+    const dim3 g(grid_rows((nnodes + 1) / 2, kDwWaves)), bl(kBlock);
+#define MGX_DG(V) hipLaunchKernelGGL((k_duvi_gs<V, T>), g, bl, 0, s, nnodes, nodes, A.row_blocks, A.row_dist, \
+                                     A.block_col, A.dist, A.vi, values, b, x, backward ? 1 : 0, done)
+    switch (A.vi_bytes) {
+    case 1: MGX_DG(1); break;
+    case 2: MGX_DG(2); break;
+    case 4: MGX_DG(4); break;
+    default: MGX_DG(0); break;
+    }
+#undef MGX_DG
+    MGX_HIP(hipGetLastError());
+}
+
+template void launch_duvi<double>(const mgx_duvi &, const double *, int, const double *, const double *,
+                                  const double *, double *, int32_t *, double, const int32_t *, hipStream_t);
+template void launch_duvi<float>(const mgx_duvi &, const float *, int, const float *, const float *, const float *,
+                                 float *, int32_t *, double, const int32_t *, hipStream_t);
+template void launch_duvi_gs<double>(const mgx_duvi &, const double *, int32_t, const int32_t *, const double *,
+                                     double *, bool, const int32_t *, hipStream_t);
+template void launch_duvi_gs<float>(const mgx_duvi &, const float *, int32_t, const int32_t *, const float *, float *,
+                                    bool, const int32_t *, hipStream_t);
 
 void launch_duvi_spmv(const mgx_duvi &A, const double *x, double *y, bool clear, const int32_t *done, hipStream_t s)
 {
-    if (A.nrows <= 0) return;
-    const dim3 g(grid_rows((A.nrows + kDwRows - 1) / kDwRows, kDwWaves)), bl(kBlock);
-#define MGX_DV(V) hipLaunchKernelGGL(k_duvi_wave<V>, g, bl, 0, s, A.nrows, A.nnz, A.row_blocks, A.row_dist, A.block_col, \
-                                     A.dist, A.vi, A.values, x, y, clear ? 1 : 0, done)
-    switch (A.vi_bytes) {
-    case 1: MGX_DV(1); break;
-    case 2: MGX_DV(2); break;
-    case 4: MGX_DV(4); break;
-    default: MGX_DV(0); break;
-    }
-#undef MGX_DV
-    MGX_HIP(hipGetLastError());
+    launch_duvi<double>(A, A.values, clear ? DEPI_SPMV : DEPI_ADD, x, nullptr, nullptr, y, nullptr, 0.0, done, s);
 }
 
 }  // namespace mgx

@@ -78,6 +78,15 @@ int dot_partial_count(int64_t n);   // partial sums launch_dot writes (its first
 void launch_dense_gemv(int32_t n, const double *M, const double *x, double *y, bool accumulate_into_y,
                        const int32_t *done, hipStream_t s);
 void launch_duvi_spmv(const mgx_duvi &A, const double *x, double *y, bool clear, const int32_t *done, hipStream_t s);
+// DuVi row kernels with a fused epilogue (values: A's table, or an FP32 copy of it for T = float)
+enum DuviEpi : int { DEPI_SPMV = 0, DEPI_ADD = 1, DEPI_RESID = 2, DEPI_JACOBI = 3, DEPI_CHECK = 4 };
+template <typename T>
+void launch_duvi(const mgx_duvi &A, const T *values, int epi, const T *x, const T *b, const T *pre, T *y,
+                 int32_t *flag, double tol, const int32_t *done, hipStream_t s);
+// one colour of a multicolour Gauss-Seidel sweep on the DuVi layout (nodes of 3 rows)
+template <typename T>
+void launch_duvi_gs(const mgx_duvi &A, const T *values, int32_t nnodes, const int32_t *nodes, const T *b, T *x,
+                    bool backward, const int32_t *done, hipStream_t s);
 void launch_simp_gradient(int32_t nx, int32_t ny, int32_t nz, const double *Ke0, const double *rho, double penal,
                           double Emin, double E0, const double *u, double *grad, hipStream_t s);
 
@@ -98,6 +107,8 @@ void launch_update_p(int32_t n, const double *z, double *p, const Ctl *ctl, bool
 void launch_pcg_norms_f32(int32_t n, const float *b, const float *r, Ctl *ctl, double *partials, int stride, hipStream_t s);
 void launch_spmv_dot_alpha_f32(int32_t n, const int32_t *rp, const int32_t *ci, const float *v, const float *p,
                                float *q, int lanes, Ctl *ctl, double *partials, hipStream_t s);
+template <typename T>
+void launch_dot_alpha(int32_t n, const T *p, const T *q, Ctl *ctl, double *partials, hipStream_t s);
 void launch_pcg_update_f32(int32_t n, const float *p, const float *q, float *x, float *r, Ctl *ctl,
                            double *partials, hipStream_t s);
 void launch_dot_beta_f32(int32_t n, const float *r, const float *z, Ctl *ctl, double *partials, bool first, hipStream_t s);

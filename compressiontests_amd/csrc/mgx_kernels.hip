@@ -517,6 +517,18 @@ void launch_spmv_dot_alpha_f32(int32_t n, const int32_t *rp, const int32_t *ci, 
     launch_finalize(g, partials, 0, ctl, FIN_ALPHA, nullptr, s);
 }
 
+// alpha = rz / p.q from a q = A p computed by another kernel (the DuVi SpMV)
+template <typename T>
+void launch_dot_alpha(int32_t n, const T *p, const T *q, Ctl *ctl, double *partials, hipStream_t s)
+{
+    const int g = grid_stream(n);
+    hipLaunchKernelGGL(k_dot_partial<T>, dim3(g), dim3(kBlock), 0, s, n, p, q, &ctl->done, partials);
+    MGX_HIP(hipGetLastError());
+    launch_finalize(g, partials, 0, ctl, FIN_ALPHA, nullptr, s);
+}
+template void launch_dot_alpha<double>(int32_t, const double *, const double *, Ctl *, double *, hipStream_t);
+template void launch_dot_alpha<float>(int32_t, const float *, const float *, Ctl *, double *, hipStream_t);
+
 // x += alpha p; r -= alpha q; partials of ||r||^2 and max|r| (phase 1 of the stop test)
 template <typename T>
 __global__ __launch_bounds__(kBlock) void k_pcg_update(int32_t n, const T *__restrict__ p,

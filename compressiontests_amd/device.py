@@ -210,4 +210,30 @@ class DeviceDuvi:
         self.m = m
 
     def spmv(self, x, y, clear=True):
+        """y = A x (y += A x).  DuViGeometricMultigrid.cl:122-159."""
         L.call("mgx_duvi_spmv", C.byref(self.m), x.vp, y.vp, int(clear), None)
+
+    def residual(self, b, x, r):
+        """r = b - A x.  DuViGeometricMultigrid.cl:220-254."""
+        L.call("mgx_duvi_residual", C.byref(self.m), b.vp, x.vp, r.vp, None)
+
+    def residual_with_check(self, b, x, r, tolerance):
+        """r = b - A x and the reference flag (1 converged, bit 1 failed).  DuViGeometricMultigrid.cl:275-317."""
+        flag = DeviceArray.from_numpy(np.array([1], np.int32))
+        L.call("mgx_duvi_residual_check", C.byref(self.m), b.vp, x.vp, r.vp, C.c_double(tolerance), flag.vp, None)
+        return int(flag.numpy()[0])
+
+    def jacobi(self, pre, b, x, y):
+        """y = x + pre .* (b - A x).  DuViGeometricMultigrid.cl:362-402."""
+        L.call("mgx_duvi_jacobi", C.byref(self.m), pre.vp, b.vp, x.vp, y.vp, None)
+
+    def gauss_seidel_color(self, nodes, b, x, backward=False):
+        """One colour of the multicolour Gauss-Seidel sweep (replaces gauss_seidel_iteration,
+        DuViGeometricMultigrid.cl:422-462)."""
+        L.call("mgx_duvi_gs_color", C.byref(self.m), nodes.n, nodes.vp, b.vp, x.vp, int(backward), None)
+
+    def spmv_f32(self, x, y, clear=True):
+        """FP32 y = A x with an FP32 copy of the value table (made on first use)."""
+        if getattr(self, "values32", None) is None:
+            self.values32 = DeviceArray.from_numpy(self.keep[4].numpy().astype(np.float32))
+        L.call("mgx_duvi_spmv_f32", C.byref(self.m), self.values32.vp, x.vp, y.vp, int(clear), None)

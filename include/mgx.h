@@ -126,6 +126,36 @@ typedef struct {
  * Replaces matrix_vector_product, src/MGroup.OCL/DuViGeometricMultigrid.cl:122-159. */
 int mgx_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t clear, void *stream);
 
+/* r = b - A x on the DuVi layout, one fused pass.  Replaces residual,
+ * DuViGeometricMultigrid.cl:220-254. */
+int mgx_duvi_residual(const mgx_duvi *A, const double *b, const double *x, double *r, void *stream);
+
+/* r = b - A x and the convergence flag of residual_with_check (DuViGeometricMultigrid.cl:275-317),
+ * the same protocol as mgx_csr_residual_check: *flag (device) holds 1 on entry; bit 1 is set if
+ * any r_i is NaN or |r_i| > 1e50, bit 0 is cleared if any other |r_i| > tol. */
+int mgx_duvi_residual_check(const mgx_duvi *A, const double *b, const double *x, double *r, double tol,
+                            int32_t *flag, void *stream);
+
+/* y = x + pre .* (b - A x) on the DuVi layout, one fused pass.  Replaces jacobi_iteration,
+ * DuViGeometricMultigrid.cl:362-402. */
+int mgx_duvi_jacobi(const mgx_duvi *A, const double *pre, const double *b, const double *x, double *y,
+                    void *stream);
+
+/* One colour of a multicolour Gauss-Seidel sweep on the DuVi layout, rows in node triples: for
+ * each node k of nodes[0..nnodes), rows 3k..3k+2 are relaxed in order (reverse order when
+ * backward != 0) with the current x; nodes of one colour must not be coupled.  Replaces
+ * gauss_seidel_iteration, DuViGeometricMultigrid.cl:422-462, whose in-place update races (and
+ * whose 8/16-bit value-index types are swapped, :428-432); the multicolour order is the
+ * documented deviation. */
+int mgx_duvi_gs_color(const mgx_duvi *A, int32_t nnodes, const int32_t *nodes, const double *b, double *x,
+                      int32_t backward, void *stream);
+
+/* y = A x / y += A x with FP32 values and vectors: values32 is an FP32 copy (device) of A's value
+ * table (nuniq entries, or nnz raw values), which replaces A->values; the index streams are
+ * shared.  Precision form (no reference counterpart). */
+int mgx_duvi_spmv_f32(const mgx_duvi *A, const float *values32, const float *x, float *y, int32_t clear,
+                      void *stream);
+
 /* Host-side DuVi encoder: CSR -> block layout.  Restates FromDokRowMajor,
  * OpenCLDuViGeometricMultigridSolver.cs:753-851 (tolerance dedup of ToleranceComparer,
  * DuViCompressedSparseMatrix.cs:168-174).  mode 0: u8 if <= 256 unique values, u16 if

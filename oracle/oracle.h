@@ -123,6 +123,8 @@ int orc_duvi_block_encode(int32_t nrows, const int32_t *rp, const int32_t *ci, c
                           double tol, int mode, orc_duvi_block *out);
 void orc_duvi_block_free(orc_duvi_block *m);
 void orc_duvi_block_spmv(const orc_duvi_block *m, const double *x, double *y, int clear);
+/* decode to CSR (rp[nrows+1], ci[nnz], v[nnz]), DuVi2Mat (OpenCLDuViGeometricMultigridSolver.cs:394-415) */
+void orc_duvi_block_to_csr(const orc_duvi_block *m, int32_t *rp, int32_t *ci, double *v);
 /* .NET Dictionary<double,int>(ToleranceComparer) emulation: value -> index, first-seen order */
 int64_t orc_dedup(int64_t n, const double *v, double tol, int32_t *idx, double *uniq);
 

@@ -344,6 +344,16 @@ def duvi_block_spmv(d, x, y=None, clear=True):
     return y
 
 
+def duvi_block_to_csr(d, ncols):
+    """The CSR matrix a block-layout encoding stores (values after the tolerance merge)."""
+    m, keep = _block_struct(d)
+    rp = np.empty(d["nrows"] + 1, np.int32)
+    ci = np.empty(d["nnz"], np.int32)
+    v = np.empty(d["nnz"], np.float64)
+    lib().orc_duvi_block_to_csr(C.byref(m), _p(rp), _p(ci), _p(v))
+    return Csr(rp, ci, v, d["nrows"], ncols)
+
+
 def _units_to_dict(m):
     out = dict(
         nrows=m.nrows, ncols=m.ncols, nunits=m.nunits,

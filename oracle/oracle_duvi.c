@@ -213,6 +213,29 @@ void orc_duvi_block_spmv(const orc_duvi_block *m, const double *x, double *y, in
     }
 }
 
+/* The matrix the block layout stores, as CSR (the decode of DuVi2Mat,
+   OpenCLDuViGeometricMultigridSolver.cs:394-415, with the column rule of
+   DuViGeometricMultigrid.cl:84-98): rp[nrows+1], ci[nnz], v[nnz] with the decoded (merged)
+   values in slot order. */
+void orc_duvi_block_to_csr(const orc_duvi_block *m, int32_t *rp, int32_t *ci, double *v)
+{
+    for (int32_t r = 0; r <= m->nrows; ++r) rp[r] = m->row_dist[r];
+    for (int32_t r = 0; r < m->nrows; ++r) {
+        int64_t d = m->row_dist[r];
+        for (int64_t b = m->row_blocks[r]; b < m->row_blocks[r + 1]; ++b) {
+            int64_t col = m->block_col[b];
+            int64_t dend = d + m->dist[d] + 1;
+            for (;;) {
+                ci[d] = (int32_t)col;
+                v[d] = block_value(m, d);
+                ++d;
+                if (d == dend) break;
+                col += m->dist[d] + 1;
+            }
+        }
+    }
+}
+
 /* ---------------------------------------------------------------- CPU Unit stream */
 enum { DB_ONE = 0, DB_TWO = 1, DB_FOUR = 2 };
 #define LOW2 (1 << 7)

@@ -143,6 +143,18 @@ def test_config2_duvi_encode_and_spmv_64(k64, mode, vib):
     err = np.abs(y.numpy() - ref).max() / scale
     log(f"64^3 DuVi SpMV mode {mode}: max error {err:.2e} of max|a| max|x|")
     assert err <= 1e-12
+    # the fused residual and Jacobi sweep on the same encoding, against the oracle on the matrix
+    # the encoding stores (DuViGeometricMultigrid.cl:220-254, :362-402)
+    M = O.duvi_block_to_csr(o, A.ncols)
+    b = O.rng_normal(20261208, A.nrows)
+    pre = 0.6 * O.jacobi_preconditioner(M)
+    db, dx = DeviceArray.from_numpy(b), DeviceArray.from_numpy(x)
+    r = DeviceArray(A.nrows)
+    Dd.residual(db, dx, r)
+    assert np.abs(r.numpy() - O.residual(M, b, x)).max() <= 1e-12 * (scale + np.abs(b).max())
+    Dd.jacobi(DeviceArray.from_numpy(pre), db, dx, y)
+    err = np.abs(y.numpy() - O.jacobi(M, pre, b, x)) / (np.abs(x).max() + np.abs(pre) * (np.abs(b).max() + scale))
+    assert err.max() <= 1e-12
 
 
 @pytest.mark.slow

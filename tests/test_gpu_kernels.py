@@ -255,3 +255,91 @@ def test_spmv_64_parity(gpu):
     ref = O.spmv(A, x)
     err = np.abs(y.numpy() - ref) / (np.abs(A.val).max() * np.abs(x).max())
     assert err.max() <= 1e-12
+
+
+def duvi_case(mode, n=8, seed=9):
+    # mode 1: the homogeneous matrix (u8 indices); heterogeneous at 8^3 has < 65,536 distinct
+    # values, so mode 0 stores u16 indices
+    dens = None if mode == 1 else 10 ** (4 * O.rng_uniform(seed, n ** 3) - 2)
+    A = O.cantilever(n, n, n, dens=dens)
+    d = duvi_encode(A, 1e-10, 0 if mode == 1 else mode)
+    M = O.duvi_block_to_csr(O.duvi_block_encode(A, 1e-10, 0 if mode == 1 else mode), A.ncols)
+    return A, d, M
+
+
+@pytest.mark.parametrize("mode", [0, 4, 8])
+def test_duvi_fused_residual_jacobi_check(gpu, mode):
+    """The fused epilogues of the DuVi row kernel against the oracle on the matrix the encoding
+    stores: r = b - A x (DuViGeometricMultigrid.cl:220-254), y = x + pre (b - A x) (:362-402) and
+    the residual_with_check flag (:275-317) on a passing, a failing and a NaN case."""
+    A, d, M = duvi_case(mode)
+    Dd = DeviceDuvi(d)
+    rng = np.random.default_rng(40 + mode)
+    x, b = rng.normal(size=A.nrows), rng.normal(size=A.nrows)
+    pre = 0.6 * O.jacobi_preconditioner(M)
+    dx, db, dpre = DeviceArray.from_numpy(x), DeviceArray.from_numpy(b), DeviceArray.from_numpy(pre)
+    r, y = DeviceArray(A.nrows), DeviceArray(A.nrows)
+    Dd.residual(db, dx, r)
+    assert close(r.numpy(), O.residual(M, b, x))
+    Dd.jacobi(dpre, db, dx, y)
+    assert close(y.numpy(), O.jacobi(M, pre, b, x))
+    rr = O.residual(M, b, x)
+    assert Dd.residual_with_check(db, dx, r, 10 * np.abs(rr).max()) == 1
+    assert Dd.residual_with_check(db, dx, r, 0.5 * np.abs(rr).max()) == 0
+    bn = b.copy()
+    bn[A.nrows // 2] = np.nan
+    assert Dd.residual_with_check(DeviceArray.from_numpy(bn), dx, r, 1e300) == 3
+
+
+@pytest.mark.parametrize("mode", [0, 1, 4, 8])
+@pytest.mark.parametrize("backward", [False, True])
+def test_duvi_gs_color_matches_oracle(gpu, mode, backward):
+    """A multicolour Gauss-Seidel sweep on the DuVi layout, colour by colour, against the oracle's
+    multicolour sweep on the matrix the encoding stores (mode 1: the homogeneous matrix, u8
+    indices)."""
+    A, d, M = duvi_case(mode)
+    Dd = DeviceDuvi(d)
+    ncol, color = O.block_coloring(M)
+    rng = np.random.default_rng(50 + mode)
+    b, x0 = rng.normal(size=A.nrows), rng.normal(size=A.nrows)
+    dx, db = DeviceArray.from_numpy(x0), DeviceArray.from_numpy(b)
+    for c in (range(ncol - 1, -1, -1) if backward else range(ncol)):
+        nodes = DeviceArray.from_numpy(np.nonzero(color == c)[0].astype(np.int32))
+        Dd.gauss_seidel_color(nodes, db, dx, backward)
+    ref = O.gs_multicolor(M, ncol, color, b, x0, backward)
+    assert close(dx.numpy(), ref, 1e-11)
+
+
+@pytest.mark.parametrize("mode", [0, 4, 8])
+def test_duvi_spmv_f32(gpu, mode):
+    A, d, M = duvi_case(mode)
+    Dd = DeviceDuvi(d)
+    x = np.random.default_rng(60).normal(size=A.nrows)
+    y = DeviceArray(A.nrows, np.float32)
+    Dd.spmv_f32(DeviceArray.from_numpy(x.astype(np.float32)), y)
+    ref = O.spmv(M, x)
+    assert np.abs(y.numpy() - ref).max() <= 1e-5 * np.abs(M.val).max() * np.abs(x).max() * 9
+
+
+def test_duvi_fused_long_rows(gpu):
+    """Residual and Jacobi epilogues on rows whose tile exceeds the LDS staging (the one-lane-per-row
+    path) and on ragged rows, against the oracle."""
+    rng = np.random.default_rng(61)
+    rows = [list(range(2000, 9000)), [], [5, 261, 262, 518], list(range(0, 300))] + [[3, 4]] * 4
+    rp, ci = [0], []
+    for r_ in rows:
+        ci += r_
+        rp.append(len(ci))
+    A = O.Csr(np.array(rp, np.int32), np.array(ci, np.int32), rng.normal(size=len(ci)), len(rows), 10000)
+    d = duvi_encode(A, 1e-10, 8)
+    M = O.duvi_block_to_csr(O.duvi_block_encode(A, 1e-10, 8), A.ncols)
+    Dd = DeviceDuvi(d)
+    x, b = rng.normal(size=A.ncols), rng.normal(size=A.nrows)
+    pre = rng.uniform(0.1, 1, size=A.nrows)
+    r = DeviceArray(A.nrows)
+    Dd.residual(DeviceArray.from_numpy(b), DeviceArray.from_numpy(x), r)
+    assert close(r.numpy(), b - O.spmv(M, x))
+    # Jacobi needs x with nrows entries at the row indices: square the case by reading x[:nrows]
+    y = DeviceArray(A.nrows)
+    Dd.jacobi(DeviceArray.from_numpy(pre), DeviceArray.from_numpy(b), DeviceArray.from_numpy(x), y)
+    assert close(y.numpy(), x[:A.nrows] + pre * (b - O.spmv(M, x)))

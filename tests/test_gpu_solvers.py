@@ -56,8 +56,6 @@ def m16(gpu):
 @pytest.mark.parametrize("gs", [False, True])
 @pytest.mark.parametrize("cls", [HipCsrGeometricMultigridSolver, HipDuViGeometricMultigridSolver])
 def test_gmg_parity(m16, gs, cls):
-    if gs and cls is HipDuViGeometricMultigridSolver:
-        pytest.skip("multicolour GS runs on CSR; the DuVi GS of the reference is the racy variant")
     m, A, U, b = m16
     depth, it, tol = 2, 4, 1e-5
     solver = cls.CreateDeepV(m, GaussSeidel=gs, maxCircleIterations=20000, coarseRelaxation=True,
@@ -153,13 +151,19 @@ def test_nan_rhs_reports_failure(m16):
         HipCsrGeometricMultigridSolver(100, 1e-5, [False, True], [2]).Initialize(m)
 
 
-def test_duvi_mgpcg(m16):
+@pytest.mark.parametrize("smoother", ["jacobi", "gs"])
+def test_duvi_mgpcg(m16, smoother):
+    """FP64 MG-PCG on the DuVi layout (the level-0 products, the V-cycle's residual, smoother and
+    transfers all decode DuVi): the manufactured solution and the oracle's iteration count."""
     m, A, U, b = m16
-    s = HipMgPcgSolver(m, levels=3, smoother="jacobi", nu=2, tolerance=1e-9, format="duvi")
+    s = HipMgPcgSolver(m, levels=3, smoother=smoother, omega=0.6, nu=2, tolerance=1e-10, format="duvi")
     s.Initialize(b)
     x, st = s.Solve()
+    xo, so = O.mgpcg(oracle_galerkin_levels(16, 3), b, smoother=1 if smoother == "gs" else 0, omega=0.6, nu=2,
+                     tol=1e-10)
     assert st.HasConverged
-    assert energy_err(A, x, U) < 1e-7
+    assert abs(st.NumIterationsRequired - so.iterations) <= max(1, so.iterations // 10)
+    assert energy_err(A, x, U) < 1e-8
 
 
 def test_graph_replays_match_eager(gpu):
@@ -244,8 +248,28 @@ def test_mixed_precision_maxnorm_and_caps(m16):
     s2.Initialize(b)
     x2, st2 = s2.Solve()
     assert not st2.HasConverged and not st2.Failed and st2.OuterIterations == 2
-    with pytest.raises(L.MgxNotImplemented):
-        HipMgPcgSolver(m, levels=3, smoother="jacobi", nu=2, format="duvi", precision="mixed").Initialize(b)
+
+
+@pytest.mark.parametrize("smoother", ["gs", "jacobi"])
+def test_duvi_mixed_precision(gpu, smoother):
+    """Precision form on the DuVi layout: FP32 value table and vectors in the V-cycle and the inner
+    SpMV (DuVi residual, Jacobi, Gauss-Seidel and transfer kernels), FP64 DuVi true residual.
+    Against the oracle's FP64 MG-PCG on the CSR matrices (the DuVi values merge near-duplicates
+    within 1e-10), 16^3 with a 1e4 modulus contrast."""
+    n = 16
+    dens = 10 ** (4 * O.rng_uniform(20261205, n ** 3) - 2)
+    m = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
+    f = m.rhs()
+    A = O.cantilever(n, n, n, dens=dens)
+    xo, so = O.mgpcg(oracle_galerkin_levels(n, 3, dens), f, smoother=1, nu=2, tol=1e-12)
+    s = HipMgPcgSolver(m, levels=3, smoother=smoother, omega=0.0, nu=2, tolerance=1e-10, precision="mixed",
+                       format="duvi")
+    s.Initialize(f)
+    x, st = s.Solve()
+    assert st.HasConverged and not st.Failed and st.OuterIterations >= 2
+    assert st.RelativeResidual <= 1.01e-10
+    assert np.linalg.norm(f - S(A) @ x) <= 1.02e-10 * np.linalg.norm(f)
+    assert energy_err(A, x, xo) < 1e-8
 
 
 def test_mixed_precision_stagnation_stops(m16):

@@ -240,3 +240,21 @@ def test_cantilever_duvi_sizes():
     assert 2.5 < d["nblocks"] / A.nrows < 3.5
     x = np.random.default_rng(1).normal(size=A.nrows)
     assert np.allclose(O.duvi_block_spmv(d, x), O.spmv(A, x), rtol=1e-9, atol=1e-9)
+
+
+@pytest.mark.parametrize("mode", [0, 4, 8])
+def test_block_decode_to_csr(mode):
+    """The block layout decodes back to the CSR it encodes: exactly (indices and value bits) with
+    the exact dedup (tol = 0), and with the 1e-10 merge to the matrix whose product is the DuVi
+    SpMV's (the same slot order, so the same bits)."""
+    n = 4
+    dens = 10 ** (2 * O.rng_uniform(31, n ** 3) - 1)
+    A = O.cantilever(n, n, n, dens=dens)
+    exact = O.duvi_block_to_csr(O.duvi_block_encode(A, 0.0, mode), A.ncols)
+    assert np.array_equal(exact.rowptr, A.rowptr) and np.array_equal(exact.col, A.col)
+    assert exact.val.tobytes() == A.val.tobytes()
+    d = O.duvi_block_encode(A, 1e-10, mode)
+    M = O.duvi_block_to_csr(d, A.ncols)
+    assert np.array_equal(M.col, A.col) and np.abs(M.val - A.val).max() < 1e-10
+    x = O.rng_normal(32, A.ncols)
+    assert O.spmv(M, x).tobytes() == O.duvi_block_spmv(d, x).tobytes()
