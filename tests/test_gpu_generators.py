@@ -66,3 +66,24 @@ def test_model_rhs_and_free_dofs(gpu):
     free = m.FindFreeDofs()
     assert len(free) == m.NumDofsFree == A.nrows
     assert np.all((free // 3) % 5 != 0)
+
+
+def test_device_generators_reproduce_frozen_16_fixtures(gpu):
+    """The device assembly of the 16^3 K_ff and the device reference R / P against the frozen
+    fixtures (tests/golden/cantilever16.json): the same bytes as the oracle wrote there."""
+    import hashlib
+    import json
+    import os
+
+    def sha(a, dtype):
+        return hashlib.sha256(np.ascontiguousarray(a, dtype).astype(np.dtype(dtype).newbyteorder("<")).tobytes()).hexdigest()
+
+    fx = json.load(open(os.path.join(os.path.dirname(__file__), "golden", "cantilever16.json")))
+    K = D.cantilever_matrix(16, 16, 16).to_host()
+    R, P = D.transfer_reference(16, 16, 16)
+    for key, M in (("K_ff", K), ("R", R.to_host()), ("P", P.to_host())):
+        d = fx[key]
+        assert (M.nrows, M.ncols, len(M.col)) == (d["nrows"], d["ncols"], d["nnz"]), key
+        assert sha(M.rowptr, np.int32) == d["rowptr_sha256"], key
+        assert sha(M.col, np.int32) == d["col_sha256"], key
+        assert sha(M.val, np.float64) == d["val_sha256"], key

@@ -223,3 +223,33 @@ def test_reference_rp_against_dense_restatement():
             dense = np.zeros(len(ffree) * 3)
             dense[cols] = vals
             assert np.array_equal(dense[dof::3], Rd[cid, ffree])
+
+
+def _sha(a, dtype):
+    import hashlib
+    return hashlib.sha256(np.ascontiguousarray(a, dtype).astype(np.dtype(dtype).newbyteorder("<")).tobytes()).hexdigest()
+
+
+def test_frozen_16_fixtures():
+    """The 16^3 K_ff, U_f, b (DOK order) and reference R / P reproduce the frozen fixtures
+    (tests/golden/cantilever16.json, written by tests/golden/make_cantilever16.py) bit for bit;
+    the host model's manufactured displacements (compressiontests_amd/model.py) equal U_f."""
+    import json
+    import os
+    g = os.path.join(os.path.dirname(__file__), "golden")
+    fx = json.load(open(os.path.join(g, "cantilever16.json")))
+    U_fix = np.load(os.path.join(g, fx["U_f"]["file"]), allow_pickle=False)
+    assert _sha(U_fix, np.float64) == fx["U_f"]["sha256"]
+    n = 16
+    A = O.cantilever(n, n, n)
+    for key, M in (("K_ff", A),) + tuple(zip(("R", "P"), O.rp_reference(n, n, n))):
+        d = fx[key]
+        assert (M.nrows, M.ncols, len(M.col)) == (d["nrows"], d["ncols"], d["nnz"]), key
+        assert _sha(M.rowptr, np.int32) == d["rowptr_sha256"], key
+        assert _sha(M.col, np.int32) == d["col_sha256"], key
+        assert _sha(M.val, np.float64) == d["val_sha256"], key
+    U = O.manufactured(n, n, n)
+    assert U.tobytes() == U_fix.tobytes()
+    assert _sha(O.multiply_dok_order(n, n, n, A, U), np.float64) == fx["b_dok_order_sha256"]
+    from compressiontests_amd.model import FemCantilever3D
+    assert np.array_equal(FemCantilever3D((n, n, n)).CalcFreeDisplacements(), U_fix)
