@@ -95,6 +95,8 @@ def main():
     ap.add_argument("--check", type=int, default=32, help="load cases (the first ones) checked against FP64 1e-12 solves")
     ap.add_argument("--graph", type=int, default=1, help="replay PCG iterations as a captured hipGraph")
     ap.add_argument("--check-every", type=int, default=8)
+    ap.add_argument("--hierarchy-file", default="",
+                    help="on-disk hierarchy: loaded when the file exists, else built and written there")
     args = ap.parse_args()
 
     if args.gpus != 1:
@@ -105,7 +107,7 @@ def main():
     import numpy as np
     from compressiontests_amd import _lib, kernels
     from compressiontests_amd import traffic as T
-    from compressiontests_amd.device import DeviceArray
+    from compressiontests_amd.device import DeviceArray, load_hierarchy, save_hierarchy
     from compressiontests_amd.model import FemCantilever3D
     from compressiontests_amd.solvers import HipMgPcgSolver
 
@@ -119,11 +121,26 @@ def main():
     F = model.rhs()[None, :] if nrhs == 1 else model.load_cases(nrhs, SEED_LOADS)
     common = dict(levels=levels, smoother=args.smoother, nu=args.nu, check_every=args.check_every,
                   gs_pairs=bool(args.gs_pairs))
+    phases = {}
+    t_h = time.time()
+    hier_source = "built"
+    if args.hierarchy_file and os.path.exists(args.hierarchy_file):
+        hier = load_hierarchy(args.hierarchy_file)
+        hier_source = "file"
+    else:
+        hier = "galerkin"
     solver = HipMgPcgSolver(model, tolerance=args.tol, max_iterations=500, use_graph=bool(args.graph), nrhs=nrhs,
                             precision=args.precision, block_width=args.block_width,
-                            strip_window=args.strip_window, **common)
+                            strip_window=args.strip_window, hierarchy=hier, **common)
     hier = solver.hierarchy
+    phases["hierarchy_s"] = round(time.time() - t_h, 2)
+    if args.hierarchy_file and hier_source == "built":
+        t_w = time.time()
+        save_hierarchy(args.hierarchy_file, hier)
+        phases["hierarchy_write_s"] = round(time.time() - t_w, 2)
+    t_i = time.time()
     solver.Initialize(F)
+    phases["initialize_s"] = round(time.time() - t_i, 2)
     width, tune_ms = solver.BlockWidth
     strip = solver.StripWindow
     print(f"# setup done in {time.time() - t0:.1f} s, block width {width}", file=sys.stderr, flush=True)
@@ -131,8 +148,11 @@ def main():
     prof = HipMgPcgSolver(model, tolerance=args.tol, max_iterations=500, profile=True, nrhs=nrhs,
                           precision=args.precision, block_width=width, strip_window=strip[0], hierarchy=hier,
                           **common)
+    t_p = time.time()
     prof.Initialize(F)
+    phases["profile_initialize_s"] = round(time.time() - t_p, 2)
     setup_s = time.time() - t0
+    print(f"# setup phases {phases}", file=sys.stderr, flush=True)
     K = model.device_matrix()
     ndof, nnz = K.nrows, K.nnz
 
@@ -289,6 +309,8 @@ def main():
             "fp64_single_rhs_path": fp64,
             "speedup_vs_fp64_single_rhs": round(solves_per_s / fp64["solves_per_s"], 3) if fp64 else None,
             "setup_s": round(setup_s, 2),
+            "setup_phases_s": phases,
+            "hierarchy": hier_source,
         },
         "roofline": rooflines[0],
         "roofline_next": rooflines[1] if len(rooflines) > 1 else None,

@@ -93,6 +93,16 @@ class Duvi(C.Structure):
                 ("block_col", C.c_void_p), ("dist", C.c_void_p), ("vi", C.c_void_p), ("values", C.c_void_p)]
 
 
+class HierarchySummary(C.Structure):
+    _fields_ = [("version", C.c_int32), ("nlevels", C.c_int32), ("has_grid", C.c_int32), ("has_duvi", C.c_int32),
+                ("payload_bytes", C.c_int64)]
+
+
+class CsrOwned(C.Structure):
+    _fields_ = [("nrows", C.c_int32), ("ncols", C.c_int32), ("nnz", C.c_int64), ("rowptr", C.POINTER(C.c_int32)),
+                ("col", C.POINTER(C.c_int32)), ("val", C.POINTER(C.c_double))]
+
+
 class DuviHost(C.Structure):
     _fields_ = [("nrows", C.c_int32), ("vi_bytes", C.c_int32), ("nnz", C.c_int64), ("nblocks", C.c_int64),
                 ("nuniq", C.c_int64), ("row_blocks", C.POINTER(C.c_int32)), ("row_dist", C.POINTER(C.c_int32)),
@@ -176,6 +186,17 @@ _SIGS = {
     "mgx_time_copy_kernel": (C.c_int, [_P, _P, _U64, _I32, _I32, C.POINTER(_D)]),
     "mgx_dense_block_apply": (C.c_int, [_I32, _P, _I32, _P, _P, _I32, _P]),
     "mgx_time_read_kernel": (C.c_int, [_P, _U64, _P, _I32, _I32, C.POINTER(_D)]),
+    "mgx_hierarchy_write": (C.c_int, [C.c_char_p, _I32, C.POINTER(CsrHost), C.POINTER(CsrHost), C.POINTER(CsrHost),
+                                      _P, _I32, _D]),
+    "mgx_hierarchy_info": (C.c_int, [C.c_char_p, C.POINTER(HierarchySummary)]),
+    "mgx_hierarchy_read_csr": (C.c_int, [C.c_char_p, _I32, _I32, C.POINTER(CsrOwned)]),
+    "mgx_csr_owned_free": (None, [C.POINTER(CsrOwned)]),
+    "mgx_hierarchy_read_duvi": (C.c_int, [C.c_char_p, _I32, _I32, C.POINTER(DuviHost)]),
+    "mgx_hierarchy_read_grid": (C.c_int, [C.c_char_p, _I32, _P]),
+    "mgx_hierarchy_save": (C.c_int, [C.c_char_p, _I32, _P, _P, _P, _I32, _D]),
+    "mgx_hierarchy_load": (C.c_int, [C.c_char_p, _I32, C.POINTER(C.c_int32), _P, _P, _P]),
+    "mgx_matrix_grid": (C.c_int, [_P, _P]),
+    "mgx_matrix_set_grid": (C.c_int, [_P, _P]),
     "mgx_rng_uniform": (C.c_int, [_U64, _I64, _P]),
     "mgx_rng_normal": (C.c_int, [_U64, _I64, _P]),
 }

@@ -46,10 +46,11 @@ def build_lib(verbose=False):
     for src in ("mgx_kernels.hip", "mgx_block.hip", "mgx_nodeblock.hip", "mgx_duvi.hip", "mgx_api.hip"):
         obj = os.path.join(BUILD, src + ".o")
         jobs.append((obj, [hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]))
-    # host-only setup code: no device code, no FMA contraction (bit-exact weights)
-    obj = os.path.join(BUILD, "mgx_host.cpp.o")
-    jobs.append((obj, [hipcc, *HOST_FLAGS, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-c",
-                       os.path.join(CSRC, "mgx_host.cpp"), "-o", obj]))
+    # host-only code (setup, file I/O): no device code, no FMA contraction (bit-exact weights)
+    for src in ("mgx_host.cpp", "mgx_io.cpp"):
+        obj = os.path.join(BUILD, src + ".o")
+        jobs.append((obj, [hipcc, *HOST_FLAGS, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-c",
+                           os.path.join(CSRC, src), "-o", obj]))
     with concurrent.futures.ThreadPoolExecutor(max_workers=4) as ex:
         outs = list(ex.map(lambda j: _run(j[1]), jobs))
     if verbose:

@@ -2311,6 +2311,26 @@ int mgx_matrix_from_host(int32_t nrows, int32_t ncols, const int32_t *rowptr, co
     });
 }
 
+int mgx_matrix_grid(mgx_matrix m, int32_t *nxyz)
+{
+    return guard([&] {
+        MGX_REQUIRE(m && nxyz, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        nxyz[0] = m->gx; nxyz[1] = m->gy; nxyz[2] = m->gz;
+    });
+}
+
+int mgx_matrix_set_grid(mgx_matrix m, const int32_t *nxyz)
+{
+    return guard([&] {
+        MGX_REQUIRE(m && nxyz, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(nxyz[0] >= 0 && nxyz[1] >= 0 && nxyz[2] >= 0, MGX_ERR_INVALID_ARGUMENT, "negative grid size");
+        const int64_t free_nodes = (int64_t)nxyz[0] * (nxyz[1] + 1) * (nxyz[2] + 1);
+        MGX_REQUIRE(nxyz[0] == 0 || free_nodes * 3 == m->A.nrows, MGX_ERR_INVALID_ARGUMENT,
+                    "the grid's free dofs differ from the matrix rows");
+        m->gx = nxyz[0]; m->gy = nxyz[1]; m->gz = nxyz[2];
+    });
+}
+
 int mgx_matrix_view(mgx_matrix m, mgx_csr_view *view)
 {
     return guard([&] {

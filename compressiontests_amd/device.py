@@ -1,5 +1,6 @@
 """Device buffers and device-resident matrices for the Python side of the C ABI."""
 import ctypes as C
+import os
 from collections import namedtuple
 
 import numpy as np
@@ -167,6 +168,97 @@ def rng_normal(seed, n):
     return out
 
 
+# ------------------------------------------------------------------ on-disk hierarchy
+def _csr_host(M):
+    rp = np.ascontiguousarray(M.rowptr, np.int32)
+    ci = np.ascontiguousarray(M.col, np.int32)
+    v = np.ascontiguousarray(M.val, np.float64)
+    h = L.CsrHost(M.nrows, M.ncols, len(ci), rp.ctypes.data, ci.ctypes.data, v.ctypes.data)
+    return h, (rp, ci, v)
+
+
+def write_hierarchy(path, levels, grid=None, duvi_mode=-1, duvi_tol=1e-10):
+    """Host levels (finest first: dicts with A, and R, P above the coarsest; CSR with rowptr, col,
+    val, nrows, ncols) to a hierarchy file (mgx_hierarchy_write).  grid: per level (nx, ny, nz)."""
+    n = len(levels)
+    keep = []
+    A = (L.CsrHost * n)()
+    R = (L.CsrHost * max(n - 1, 1))()
+    P = (L.CsrHost * max(n - 1, 1))()
+    for l, d in enumerate(levels):
+        A[l], k = _csr_host(d["A"])
+        keep.append(k)
+        if l + 1 < n:
+            R[l], k1 = _csr_host(d["R"])
+            P[l], k2 = _csr_host(d["P"])
+            keep += [k1, k2]
+    g = None if grid is None else np.ascontiguousarray(grid, np.int32).reshape(n, 3)
+    L.call("mgx_hierarchy_write", os.fsencode(path), n, A, R, P, None if g is None else g.ctypes.data_as(C.c_void_p),
+           duvi_mode, C.c_double(duvi_tol))
+
+
+def hierarchy_info(path):
+    """(version, nlevels, has_grid, has_duvi, payload bytes) after verifying every checksum."""
+    s = L.HierarchySummary()
+    L.call("mgx_hierarchy_info", os.fsencode(path), C.byref(s))
+    return s.version, s.nlevels, bool(s.has_grid), bool(s.has_duvi), s.payload_bytes
+
+
+def read_hierarchy_csr(path, level, which):
+    """CSR of A (which 0), R (1) or P (2) of a level."""
+    m = L.CsrOwned()
+    L.call("mgx_hierarchy_read_csr", os.fsencode(path), level, which, C.byref(m))
+    try:
+        return CsrMatrix(np.ctypeslib.as_array(m.rowptr, (m.nrows + 1,)).copy(),
+                         np.ctypeslib.as_array(m.col, (max(m.nnz, 1),))[:m.nnz].copy(),
+                         np.ctypeslib.as_array(m.val, (max(m.nnz, 1),))[:m.nnz].copy(), m.nrows, m.ncols)
+    finally:
+        L.lib().mgx_csr_owned_free(C.byref(m))
+
+
+def read_hierarchy_duvi(path, level, which):
+    """DuVi block layout of A / R / P of a level (the dict duvi_encode returns)."""
+    h = L.DuviHost()
+    L.call("mgx_hierarchy_read_duvi", os.fsencode(path), level, which, C.byref(h))
+    return _duvi_host_to_dict(h)
+
+
+def read_hierarchy_grid(path, level):
+    g = np.zeros(3, np.int32)
+    L.call("mgx_hierarchy_read_grid", os.fsencode(path), level, g.ctypes.data_as(C.c_void_p))
+    return tuple(int(v) for v in g)
+
+
+def save_hierarchy(path, hier, duvi_mode=-1, duvi_tol=1e-10):
+    """Library-owned device levels [(A, R, P), ..., (A, None, None)] to a file (mgx_hierarchy_save)."""
+    n = len(hier)
+    A = (C.c_void_p * n)(*[h[0].handle for h in hier])
+    R = (C.c_void_p * max(n - 1, 1))(*[h[1].handle for h in hier[:-1]])
+    P = (C.c_void_p * max(n - 1, 1))(*[h[2].handle for h in hier[:-1]])
+    L.call("mgx_hierarchy_save", os.fsencode(path), n, A, R, P, duvi_mode, C.c_double(duvi_tol))
+
+
+def load_hierarchy(path, max_levels=16):
+    """Device levels [(A, R, P), ..., (A, None, None)] from a file (mgx_hierarchy_load)."""
+    A = (C.c_void_p * max_levels)()
+    R = (C.c_void_p * max_levels)()
+    P = (C.c_void_p * max_levels)()
+    n = C.c_int32()
+    L.call("mgx_hierarchy_load", os.fsencode(path), max_levels, C.byref(n), A, R, P)
+    out = []
+    for l in range(n.value):
+        last = l == n.value - 1
+        out.append((DeviceCsr(handle=A[l]), None if last else DeviceCsr(handle=R[l]),
+                    None if last else DeviceCsr(handle=P[l])))
+    return out
+
+
+def matrix_grid(M):
+    g = np.zeros(3, np.int32)
+    L.call("mgx_matrix_grid", C.c_void_p(M.handle), g.ctypes.data_as(C.c_void_p))
+    return tuple(int(v) for v in g)
+
+
 def duvi_encode(A, tol=1e-10, mode=0):
     """Host DuVi encoder of the library (OpenCLDuViGeometricMultigridSolver.cs:753-851)."""
     h = L.DuviHost()
@@ -175,6 +267,11 @@ def duvi_encode(A, tol=1e-10, mode=0):
     v = np.ascontiguousarray(A.val, np.float64)
     L.call("mgx_duvi_encode", A.nrows, rp.ctypes.data_as(C.c_void_p), ci.ctypes.data_as(C.c_void_p),
            v.ctypes.data_as(C.c_void_p), C.c_double(tol), mode, C.byref(h))
+    return _duvi_host_to_dict(h)
+
+
+def _duvi_host_to_dict(h):
+    """Copies a library-allocated mgx_duvi_host into numpy arrays and frees it."""
     nnz, nb, nu = h.nnz, h.nblocks, h.nuniq
     out = dict(nrows=h.nrows, nnz=nnz, nblocks=nb, nuniq=nu, vi_bytes=h.vi_bytes,
                row_blocks=np.ctypeslib.as_array(h.row_blocks, (h.nrows + 1,)).copy(),

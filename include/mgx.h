@@ -243,6 +243,11 @@ int mgx_matrix_galerkin_hierarchy(int32_t nx, int32_t ny, int32_t nz, double E, 
 int mgx_matrix_from_host(int32_t nrows, int32_t ncols, const int32_t *rowptr, const int32_t *col,
                          const double *val, mgx_matrix *out);
 int mgx_matrix_view(mgx_matrix m, mgx_csr_view *view);
+/* Structured grid of a matrix (elements per axis; zeros when unknown): set by the generators, read
+ * by the solver for its structured-mesh node orders; settable on a matrix uploaded from host CSR
+ * (its rows must be 3 x the free nodes nx (ny+1) (nz+1) of that grid). */
+int mgx_matrix_grid(mgx_matrix m, int32_t *nxyz);
+int mgx_matrix_set_grid(mgx_matrix m, const int32_t *nxyz);
 /* copy to host arrays sized from the view (rowptr[nrows+1], col[nnz], val[nnz]) */
 int mgx_matrix_download(mgx_matrix m, int32_t *rowptr, int32_t *col, double *val);
 int mgx_matrix_destroy(mgx_matrix m);
@@ -384,6 +389,49 @@ int mgx_solver_gs_order(mgx_solver s, int32_t level, int32_t *order);
 int mgx_solver_gs_pairs(mgx_solver s, int32_t level, int32_t *pairs);
 /* Replaces ReleaseOpenCLResources (OpenCLCsrGeometricMultigridSolver.cs:106-138). */
 int mgx_solver_release(mgx_solver s);
+
+/* ================================================================== on-disk hierarchy
+ * A multigrid hierarchy stored once and loaded without assembly, Galerkin products or DuVi
+ * encoding.  No reference counterpart: the reference rebuilds every level from the model's DOK
+ * matrices at each Initialize and keeps its DuVi encodings (the DuViMat types,
+ * OpenCLDuViGeometricMultigridSolver.cs:16-31, filled by FromDokRowMajor, :753-851) in memory
+ * only.  Versioned little-endian binary: a header, one section per matrix (CSR A of every level, R
+ * and P towards the next coarser one, optionally the DuVi block layout of each, and the level's
+ * structured grid), each with a 64-bit checksum of its payload, and a trailer checksumming the
+ * header and the section heads.  Every reader verifies what it reads and fails with
+ * MGX_ERR_INVALID_ARGUMENT on a bad magic, checksum or truncation, MGX_ERR_NOT_IMPLEMENTED on
+ * another format version. */
+#define MGX_HIERARCHY_VERSION 1
+/* Host levels, finest first: A[nlevels], R[nlevels-1] (level l -> l+1), P[nlevels-1]; grid:
+ * nlevels x 3 element counts (nx, ny, nz; zeros when unstructured) or NULL; duvi_mode: -1 stores
+ * no DuVi sections, else the mode of mgx_duvi_encode (0, 4, 8) with tolerance duvi_tol.  The
+ * file is written under path + ".part" and renamed when complete. */
+int mgx_hierarchy_write(const char *path, int32_t nlevels, const mgx_csr_host *A, const mgx_csr_host *R,
+                        const mgx_csr_host *P, const int32_t *grid, int32_t duvi_mode, double duvi_tol);
+typedef struct {
+    int32_t version, nlevels, has_grid, has_duvi;
+    int64_t payload_bytes;
+} mgx_hierarchy_summary;
+/* Reads and verifies the whole file (every checksum). */
+int mgx_hierarchy_info(const char *path, mgx_hierarchy_summary *info);
+/* CSR of one level's A (which = 0), R (1) or P (2), in host arrays allocated by the library. */
+typedef struct {
+    int32_t nrows, ncols;
+    int64_t nnz;
+    int32_t *rowptr, *col;
+    double *val;
+} mgx_csr_owned;
+int mgx_hierarchy_read_csr(const char *path, int32_t level, int32_t which, mgx_csr_owned *out);
+void mgx_csr_owned_free(mgx_csr_owned *m);
+/* DuVi block layout of one level's A / R / P (freed by mgx_duvi_host_free). */
+int mgx_hierarchy_read_duvi(const char *path, int32_t level, int32_t which, mgx_duvi_host *out);
+/* The level's structured grid (3 element counts). */
+int mgx_hierarchy_read_grid(const char *path, int32_t level, int32_t *nxyz);
+/* Library-owned device matrices in and out (the grid of each A travels with it). */
+int mgx_hierarchy_save(const char *path, int32_t nlevels, const mgx_matrix *A, const mgx_matrix *R,
+                       const mgx_matrix *P, int32_t duvi_mode, double duvi_tol);
+int mgx_hierarchy_load(const char *path, int32_t max_levels, int32_t *nlevels, mgx_matrix *A, mgx_matrix *R,
+                       mgx_matrix *P);
 
 /* ================================================================== plumbing
  * Device memory and timing helpers so that a host program (the C# binding, the Python test
