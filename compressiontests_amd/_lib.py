@@ -182,7 +182,7 @@ _SIGS = {
     "mgx_memset": (C.c_int, [_P, _I32, _U64]),
     "mgx_device_synchronize": (C.c_int, []),
     "mgx_time_csr_kernel": (C.c_int, [_I32, _I32, _P, _P, _P, _P, _P, _P, _P, _I32, _I32, C.POINTER(_D)]),
-    "mgx_time_duvi_spmv": (C.c_int, [C.POINTER(Duvi), _P, _P, _I32, _I32, C.POINTER(_D)]),
+    "mgx_time_duvi_spmv": (C.c_int, [C.POINTER(Duvi), _P, _P, _I32, _I32, _I32, C.POINTER(_D)]),
     "mgx_time_copy_kernel": (C.c_int, [_P, _P, _U64, _I32, _I32, C.POINTER(_D)]),
     "mgx_dense_block_apply": (C.c_int, [_I32, _P, _I32, _P, _P, _I32, _P]),
     "mgx_time_read_kernel": (C.c_int, [_P, _U64, _P, _I32, _I32, C.POINTER(_D)]),

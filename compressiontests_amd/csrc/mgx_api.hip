@@ -2851,10 +2851,16 @@ int mgx_time_read_kernel(const void *src, uint64_t bytes, void *sink, int32_t wa
     });
 }
 
-int mgx_time_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t warmup, int32_t repeats, double *avg_ms)
+int mgx_time_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t form, int32_t warmup, int32_t repeats,
+                       double *avg_ms)
 {
     return guard([&] {
-        MGX_REQUIRE(A && repeats >= 1 && warmup >= 0 && avg_ms, MGX_ERR_INVALID_ARGUMENT, "invalid arguments");
+        MGX_REQUIRE(A && repeats >= 1 && warmup >= 0 && avg_ms && (form == 0 || form == 1), MGX_ERR_INVALID_ARGUMENT,
+                    "invalid arguments");
+        struct WaveOnly {
+            explicit WaveOnly(bool on) { set_duvi_wave_only(on); }
+            ~WaveOnly() { set_duvi_wave_only(false); }
+        } wave_only(form == 1);
         hipStream_t s;
         MGX_HIP(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
         hipEvent_t e0, e1;

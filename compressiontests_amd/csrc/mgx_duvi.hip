@@ -98,6 +98,13 @@ __device__ __forceinline__ T dw_value(const uint8_t *si, const T *__restrict__ v
     return values[((const uint32_t *)si)[k]];
 }
 
+template <int VIB> __device__ __forceinline__ uint32_t dw_index(const uint8_t *si, int k)
+{
+    if (VIB == 1) return __builtin_nontemporal_load(si + k);
+    if (VIB == 2) return __builtin_nontemporal_load((const uint16_t *)si + k);
+    return __builtin_nontemporal_load((const uint32_t *)si + k);
+}
+
 // operands of the fused epilogues
 template <typename T> struct DuviOp {
     const T *values;          // value table (or the raw values)
@@ -274,6 +281,184 @@ __global__ __launch_bounds__(kBlock) void k_duvi_wave(int32_t nrows, int64_t nnz
     duvi_flag<EPI, T>(op, bits);
 }
 
+// ------------------------------------------------------------------ stream form (long rows)
+// For matrices with long rows (the stiffness levels: 9 blocks of up to 9 slots per row) a 256-thread
+// block takes a tile of RB = 32 consecutive rows, whose slots [d0, d1) and blocks [b0, b1) are
+// contiguous, the layout the CSR-stream kernel reads (64-row tiles take 43 KB of LDS, 3 blocks per
+// CU, and ran 1.2-1.8x slower at 128^3; 16-row tiles ran 1.4-1.5x slower):
+//   (1) each thread loads its share of the tile's row offsets, distance bytes (16-byte loads) and
+//       block columns, a fixed number of loads each; then the E slots t, t + 256, ... of the value
+//       stream: raw values, or value indices, whose table entries are loaded in step (3) beside x.
+//       A one-byte index table (256 values at most) is copied to LDS: looked up in global memory,
+//       every wave of the chip gathers from the same two or three cache lines;
+//   (2) the staged offsets, distance bytes and block columns go to LDS; the G = 256 / RB lanes of a
+//       row walk its block chain together (the count byte of each block) and lane g decodes the
+//       blocks g, g + G, ...: column of the block's first slot from block_col, then + distance + 1
+//       per slot, into an int32 column per slot;
+//   (3) every thread reads the columns of its E slots, gathers x and the table values of its value
+//       indices, and stores the products;
+//   (4) the G lanes of a row sum its products in the CSR-stream kernel's order.
+// The column, distance and block-column buffers alias the product buffer.  A tile with more
+// slots than the buffer or more blocks than fit (not the case for the stiffness matrices) is
+// decoded one row per thread from global memory.
+template <typename T> struct DsShape {
+    static constexpr int RB = 32;                                         // rows per tile
+    static constexpr int G = kBlock / RB;                                 // lanes per row
+    static constexpr int E = (RB * 84 + kBlock - 1) / kBlock;            // slots per thread
+    static constexpr int C = E * kBlock;                                  // slots per tile
+    static constexpr int NQ = (C + 32 + 15) / 16;                         // 16-byte rounds of distance bytes
+    static constexpr int NQT = (NQ + kBlock - 1) / kBlock;                // ... per thread
+    static constexpr int DIST_OFF = 4 * C;                                // after the int32 columns
+    static constexpr int BC_OFF = DIST_OFF + 16 * NQ;                     // block columns
+    static constexpr int NBT = 2;                                         // block-column loads per thread
+    static constexpr int BCAP = NBT * kBlock;                             // block columns per tile
+    static constexpr int BYTES_DECODE = BC_OFF + 4 * BCAP;
+    static constexpr int BYTES = BYTES_DECODE > (int)sizeof(T) * C ? BYTES_DECODE : (int)sizeof(T) * C;
+};
+
+template <int VIB, typename T, int EPI>
+__global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t nnz, int64_t nblocks, int32_t nvals,
+                                                        const int32_t *__restrict__ row_blocks,
+                                                        const int32_t *__restrict__ row_dist,
+                                                        const int32_t *__restrict__ block_col,
+                                                        const uint8_t *__restrict__ dist, const void *__restrict__ vi,
+                                                        const DuviOp<T> op, const int32_t *__restrict__ done)
+{
+    using Sh = DsShape<T>;
+    constexpr int RB = Sh::RB, G = Sh::G, E = Sh::E, C = Sh::C;
+    __shared__ __align__(16) unsigned char buf[Sh::BYTES];
+    __shared__ int32_t s_rd[RB + 1], s_rb[RB + 1];
+    __shared__ T s_tab[VIB == 1 ? 256 : 1];           // one-byte indices: the whole value table
+    if (done && *done) return;
+    const T *__restrict__ values = op.values;
+    const T *__restrict__ x = op.x;
+    const int t = threadIdx.x, lane = t % G, r = t / G;
+    const int ntiles = (nrows + RB - 1) / RB;
+    const int tile = xcd_remap(blockIdx.x, gridDim.x);
+    if (tile >= ntiles) return;                                    // the whole block leaves together
+    const int r0 = tile * RB, r1 = min(r0 + RB, nrows), nr = r1 - r0;
+    const int64_t d0 = row_dist[r0], d1 = row_dist[r1];
+    const int b0 = row_blocks[r0], b1 = row_blocks[r1];
+    const int cnt = (int)(d1 - d0), nb = b1 - b0;
+    const int row = r0 + r;
+    if (cnt > C || nb > Sh::BCAP) {
+        // long or fragmented tile: one thread per row, decoding straight from global memory
+        int bits = 0;
+        if (lane == 0 && row < r1) {
+            T acc = 0;
+            int64_t d = row_dist[row];
+            for (int bb = row_blocks[row]; bb < row_blocks[row + 1]; ++bb) {
+                int col = block_col[bb];
+                const int64_t dend = d + dist[d] + 1;
+                for (;;) {
+                    acc += duvi_value<VIB, T>(vi, values, d) * x[col];
+                    if (++d == dend) break;
+                    col += dist[d] + 1;
+                }
+            }
+            bits = duvi_store<EPI, T>(op, row, acc);
+        }
+        duvi_flag<EPI, T>(op, bits);
+        return;
+    }
+    // (1) staged inputs of the decode: a fixed number of loads per thread, then the value stream
+    int off = 0;
+    if (t <= nr) off = row_dist[r0 + t];
+    else if (t >= 128 && t - 128 <= nr) off = row_blocks[r0 + t - 128];
+    const int64_t a0 = d0 & ~(int64_t)15;
+    const int nq = (int)((d1 - a0 + 15) >> 4);
+    const int nin = (int)min<int64_t>((int64_t)nq, (nnz - a0) >> 4);   // 16-byte rounds wholly inside dist
+    dw_u32x4 q[Sh::NQT];
+    if (nin > 0) {                                   // rounds past nin reload round nin - 1 (unused)
+#pragma unroll
+        for (int i = 0; i < Sh::NQT; ++i)
+            q[i] = __builtin_nontemporal_load((const dw_u32x4 *)(dist + a0) + min(t + kBlock * i, nin - 1));
+    } else {
+#pragma unroll
+        for (int i = 0; i < Sh::NQT; ++i) q[i] = dw_u32x4{0, 0, 0, 0};
+    }
+    int bc[Sh::NBT];
+#pragma unroll
+    for (int i = 0; i < Sh::NBT; ++i) bc[i] = block_col[min(b0 + t + kBlock * i, (int)nblocks - 1)];
+    // raw values stream in now; value indices too, their table lookups wait for the gather of x
+    T w[E];
+    uint32_t vx[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        const int k = t + kBlock * j;
+        if constexpr (VIB == 0) w[j] = k < cnt ? __builtin_nontemporal_load(values + d0 + k) : T(0);
+        else vx[j] = k < cnt ? dw_index<VIB>((const uint8_t *)vi + d0 * VIB, k) : 0u;
+    }
+    // (2) to LDS, then the decode
+    int32_t *scol = (int32_t *)buf;
+    dw_u32x4 *sq = (dw_u32x4 *)(buf + Sh::DIST_OFF);
+    int32_t *sbc = (int32_t *)(buf + Sh::BC_OFF);
+    if constexpr (VIB == 1)
+        if (t < nvals) s_tab[t] = values[t];
+    if (t <= nr) s_rd[t] = (int)(off - d0);
+    else if (t >= 128 && t - 128 <= nr) s_rb[t - 128] = off - b0;
+#pragma unroll
+    for (int i = 0; i < Sh::NQT; ++i)
+        if (t + kBlock * i < nin) sq[t + kBlock * i] = q[i];
+    for (int i = max(nin, 0) + t; i < nq; i += kBlock) {               // the last bytes of dist, one by one
+        uint8_t *o = (uint8_t *)&sq[i];
+        for (int k = 0; k < 16; ++k) o[k] = a0 + 16 * i + k < nnz ? dist[a0 + 16 * i + k] : 0;
+    }
+#pragma unroll
+    for (int i = 0; i < Sh::NBT; ++i)
+        if (t + kBlock * i < nb) sbc[t + kBlock * i] = bc[i];
+    __syncthreads();
+    const uint8_t *sd = (const uint8_t *)sq + (d0 & 15);              // sd[k]: distance byte of slot d0 + k
+    if (row < r1) {
+        int sl = s_rd[r];
+        const int be = s_rb[r + 1];
+        for (int j = s_rb[r]; j < be; ++j) {
+            const int c = sd[sl];                                     // further slots of block j
+            if (((j - s_rb[r]) % G) == lane) {
+                int col = sbc[j];
+                scol[sl] = col;
+                for (int i = 1; i <= c; ++i) {
+                    col += sd[sl + i] + 1;
+                    scol[sl + i] = col;
+                }
+            }
+            sl += c + 1;
+        }
+    }
+    __syncthreads();
+    // (3) columns of this thread's slots, x, products
+    int c[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        const int k = t + kBlock * j;
+        c[j] = k < cnt ? scol[k] : 0;
+    }
+    __syncthreads();                                                  // the products overwrite the columns
+    T xv[E];
+#pragma unroll
+    for (int j = 0; j < E; ++j) xv[j] = x[c[j]];
+    if constexpr (VIB == 1) {
+#pragma unroll
+        for (int j = 0; j < E; ++j) w[j] = t + kBlock * j < cnt ? s_tab[vx[j]] : T(0);
+    } else if constexpr (VIB != 0) {
+#pragma unroll
+        for (int j = 0; j < E; ++j) w[j] = t + kBlock * j < cnt ? values[vx[j]] : T(0);
+    }
+    T *sp = (T *)buf;
+#pragma unroll
+    for (int j = 0; j < E; ++j) sp[t + kBlock * j] = w[j] * xv[j];
+    __syncthreads();
+    // (4) row sums
+    T a = 0;
+    if (row < r1)
+        for (int k = s_rd[r] + lane; k < s_rd[r + 1]; k += G) a += sp[k];
+#pragma unroll
+    for (int o = G / 2; o > 0; o >>= 1) a += __shfl_xor(a, o, G);
+    int bits = 0;
+    if (lane == 0 && row < r1) bits = duvi_store<EPI, T>(op, row, a);
+    duvi_flag<EPI, T>(op, bits);
+}
+
 // One colour of a multicolour Gauss-Seidel sweep: the nodes nodes[0 .. nnodes) are pairwise
 // uncoupled; node q's rows 3q, 3q+1, 3q+2 are relaxed in that order (the reverse when backward)
 // with the current x.  A wavefront takes 2 nodes, 8 lanes per row (lanes 48-63 idle).  A row's
@@ -357,6 +542,10 @@ __global__ __launch_bounds__(kBlock) void k_duvi_gs(int32_t nnodes, const int32_
     for (int a = 0; a < 3; ++a) x[i0 + a] = xn[a];
 }
 
+// the wave form for every matrix (timing comparisons of the two forms)
+static bool g_duvi_wave_only = false;
+void set_duvi_wave_only(bool on) { g_duvi_wave_only = on; }
+
 template <typename T>
 void launch_duvi(const mgx_duvi &A, const T *values, int epi, const T *x, const T *b, const T *pre, T *y,
                  int32_t *flag, double tol, const int32_t *done, hipStream_t s)
@@ -364,10 +553,20 @@ void launch_duvi(const mgx_duvi &A, const T *values, int epi, const T *x, const 
     if (A.nrows <= 0) return;
     if (epi == DEPI_CHECK && !std::is_same<T, double>::value)
         throw Error{MGX_ERR_INVALID_ARGUMENT, "the residual check runs in FP64"};
-    const dim3 g(grid_rows((A.nrows + kDwRows - 1) / kDwRows, kDwWaves)), bl(kBlock);
     const DuviOp<T> op{values, x, b, pre, y, flag, tol};
-#define MGX_DV(V, E) hipLaunchKernelGGL((k_duvi_wave<V, T, E>), g, bl, 0, s, A.nrows, A.nnz, A.row_blocks, A.row_dist, \
-                                        A.block_col, A.dist, A.vi, op, done)
+    // long rows (the stiffness levels) take the stream form, short rows (transfers) the wave form
+    const bool stream = A.nnz >= 16 * (int64_t)A.nrows && A.nblocks > 0 && !g_duvi_wave_only;
+    const dim3 g(stream ? grid_rows((A.nrows + DsShape<T>::RB - 1) / DsShape<T>::RB, 1)
+                        : grid_rows((A.nrows + kDwRows - 1) / kDwRows, kDwWaves)),
+        bl(kBlock);
+#define MGX_DV(V, E)                                                                                             \
+    if (stream)                                                                                                  \
+        hipLaunchKernelGGL((k_duvi_stream<V, T, E>), g, bl, 0, s, A.nrows, A.nnz, A.nblocks,                     \
+                           (int32_t)std::min<int64_t>(A.nuniq, 256), A.row_blocks, A.row_dist, A.block_col,        \
+                           A.dist, A.vi, op, done);                                                               \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_duvi_wave<V, T, E>), g, bl, 0, s, A.nrows, A.nnz, A.row_blocks, A.row_dist,        \
+                           A.block_col, A.dist, A.vi, op, done)
 #define MGX_DVE(V)                                                                                               \
     switch (epi) {                                                                                               \
     case DEPI_SPMV: MGX_DV(V, DEPI_SPMV); break;                                                                 \

@@ -83,6 +83,7 @@ enum DuviEpi : int { DEPI_SPMV = 0, DEPI_ADD = 1, DEPI_RESID = 2, DEPI_JACOBI = 
 template <typename T>
 void launch_duvi(const mgx_duvi &A, const T *values, int epi, const T *x, const T *b, const T *pre, T *y,
                  int32_t *flag, double tol, const int32_t *done, hipStream_t s);
+void set_duvi_wave_only(bool on);   // timing comparisons: the wave form for long rows too
 // one colour of a multicolour Gauss-Seidel sweep on the DuVi layout (nodes of 3 rows)
 template <typename T>
 void launch_duvi_gs(const mgx_duvi &A, const T *values, int32_t nnodes, const int32_t *nodes, const T *b, T *x,

@@ -11,7 +11,10 @@ non-temporal loads, nothing written), the practical HBM read ceiling.  `spmm64_k
 block SpMM Y = A X of the batching form, FP64 or FP32 values and vectors (suffix `_node`: the
 node-triple CSR kernel; `_nb`: the node-block kernel the block solver uses on the level
 matrices, on a 3x3 block copy built from the CSR; suffix `_dot`: the block-PCG form q = A p with the
-per-column p.q partials).
+per-column p.q partials).  `duvi` / `duvi_u32` / `duvi_raw` (DuVi block format: deduplicated
+values with the narrowest index, forced 32-bit indices, raw values): the kernel the library
+chooses; suffix `_wave` forces the one-wavefront-per-row form.  `--moduli uniform` gives the
+homogeneous cantilever, whose few distinct values take one-byte indices.
 """
 import argparse
 import ctypes as C
@@ -37,13 +40,15 @@ def main():
     ap.add_argument("--repeats", type=int, default=100)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--seed", type=int, default=20261203 + 2)
-    ap.add_argument("--moduli", default="loguniform", choices=["loguniform", "lognormal"])
+    ap.add_argument("--moduli", default="loguniform", choices=["loguniform", "lognormal", "uniform"])
     args = ap.parse_args()
     n = args.n
     if args.moduli == "loguniform":
         dens = 10 ** (4 * rng_uniform(args.seed, n ** 3) - 2)
-    else:   # log-normal, sigma 1.5, clipped to a 1e3 contrast
+    elif args.moduli == "lognormal":   # log-normal, sigma 1.5, clipped to a 1e3 contrast
         dens = np.exp(np.clip(1.5 * rng_normal(20261203 + 3, n ** 3), -0.5 * np.log(1e3), 0.5 * np.log(1e3)))
+    else:   # one modulus everywhere: few distinct values, so DuVi's value indices fit one byte
+        dens = None
     A = cantilever_matrix(n, n, n, dens=dens)
     N, nnz = A.nrows, A.nnz
     x = DeviceArray.from_numpy(rng_normal(args.seed + 1, N))
@@ -88,10 +93,11 @@ def main():
             else:
                 by = (vb + 4) * nnz + 4 * (N + 1) + 2 * vb * k * N
         elif kind.startswith("duvi"):
-            mode = {"duvi": 0, "duvi_u32": 4, "duvi_raw": 8}[kind]
+            wave = kind.endswith("_wave")
+            mode = {"duvi": 0, "duvi_u32": 4, "duvi_raw": 8}[kind[:-5] if wave else kind]
             d = duvi_encode(A.to_host(), 1e-10, mode)
             D = DeviceDuvi(d)
-            ms = kernels.time_duvi_spmv(D, x, y, warmup=args.warmup, repeats=args.repeats)
+            ms = kernels.time_duvi_spmv(D, x, y, form=1 if wave else 0, warmup=args.warmup, repeats=args.repeats)
             s_vi = d["vi_bytes"]
             by = nnz * (1 + (s_vi if s_vi else 8)) + 4 * d["nblocks"] + 8 * (N + 1) + (8 * d["nuniq"] if s_vi else 0) \
                 + 16 * N

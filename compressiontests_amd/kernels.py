@@ -145,9 +145,10 @@ def time_spmm(A, X, Y, k, warmup=5, repeats=100, node=False, dot=False, nodebloc
     return out.value
 
 
-def time_duvi_spmv(D, x, y, warmup=5, repeats=100):
+def time_duvi_spmv(D, x, y, form=0, warmup=5, repeats=100):
+    """Average ms per DuVi SpMV launch; form 0 = the kernel the library chooses, 1 = the wave form."""
     out = C.c_double()
-    L.call("mgx_time_duvi_spmv", C.byref(D.m), x.vp, y.vp, warmup, repeats, C.byref(out))
+    L.call("mgx_time_duvi_spmv", C.byref(D.m), x.vp, y.vp, form, warmup, repeats, C.byref(out))
     return out.value
 
 

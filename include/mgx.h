@@ -470,8 +470,10 @@ int mgx_time_copy_kernel(const void *src, void *dst, uint64_t bytes, int32_t war
  * non-temporal loads, nothing written; sink: 16 bytes of device memory the kernel may write): the
  * practical HBM read ceiling that the gather kernels are compared with.  No reference counterpart. */
 int mgx_time_read_kernel(const void *src, uint64_t bytes, void *sink, int32_t warmup, int32_t repeats, double *avg_ms);
-/* Same for the DuVi SpMV (kind 0 only). */
-int mgx_time_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t warmup, int32_t repeats, double *avg_ms);
+/* Same for the DuVi SpMV y = A x.  form 0: the kernel the library chooses (the stream form for
+ * matrices of 16 or more slots per row, else the wave form); 1: the wave form for any matrix. */
+int mgx_time_duvi_spmv(const mgx_duvi *A, const double *x, double *y, int32_t form, int32_t warmup, int32_t repeats,
+                       double *avg_ms);
 
 /* ================================================================== workload helpers
  * SplitMix64-seeded xoshiro256** (U[0,1) from the top 53 bits; N(0,1) by Box-Muller,

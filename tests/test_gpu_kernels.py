@@ -343,3 +343,44 @@ def test_duvi_fused_long_rows(gpu):
     y = DeviceArray(A.nrows)
     Dd.jacobi(DeviceArray.from_numpy(pre), DeviceArray.from_numpy(b), DeviceArray.from_numpy(x), y)
     assert close(y.numpy(), x[:A.nrows] + pre * (b - O.spmv(M, x)))
+
+
+def ragged_csr(rows, ncols, rng, mode):
+    rp, ci = [0], []
+    for r_ in rows:
+        ci += r_
+        rp.append(len(ci))
+    val = np.round(rng.normal(size=len(ci)), 1) if mode == 0 else rng.normal(size=len(ci))
+    return O.Csr(np.array(rp, np.int32), np.array(ci, np.int32), val, len(rows), ncols)
+
+
+@pytest.mark.parametrize("mode", [0, 4, 8])
+def test_duvi_stream_and_wave_forms(gpu, mode):
+    """The two DuVi SpMV kernels -- the tile-staged stream form the library picks for matrices of 16
+    or more slots per row, and the one-wavefront-per-row form -- on the same inputs, each against
+    the oracle: ragged tiles that fit the stream form's LDS staging (blocks split at 256 slots,
+    gaps of 255 / 256 / more, empty rows, a partial last tile), a tile of more slots than the
+    staging holds and a tile of more blocks than it holds (both decoded row by row), and the 8^3
+    cantilever of several tiles."""
+    rng = np.random.default_rng(70 + mode)
+    ncols = 250000
+    fit = [list(range(0, 300)), [], [5, 261, 262, 518, 60000], list(range(1000, 1600, 2))]
+    for _ in range(150):     # clustered columns: a few blocks per row, some gaps past 256
+        base = int(rng.integers(0, ncols - 3000))
+        fit.append((base + np.sort(rng.choice(3000, size=int(rng.integers(0, 90)), replace=False))).tolist())
+    many_slots = [list(range(10000, 16000))] + [[1, 2, 3]] * 63
+    many_blocks = [list(range(0, 240000, 300))] + [list(range(50, 90))] * 63
+    dens = 10 ** (4 * O.rng_uniform(9, 8 ** 3) - 2)
+    cases = [ragged_csr(fit, ncols, rng, mode), ragged_csr(many_slots + fit[:70], ncols, rng, mode),
+             ragged_csr(fit[:64] + many_blocks, ncols, rng, mode), O.cantilever(8, 8, 8, dens=dens)]
+    for A in cases:
+        assert len(A.col) >= 16 * A.nrows            # the library picks the stream form
+        d = duvi_encode(A, 1e-10, mode)
+        Dd = DeviceDuvi(d)
+        x = rng.normal(size=A.ncols)
+        ref = O.duvi_block_spmv(O.duvi_block_encode(A, 1e-10, mode), x)
+        dx = DeviceArray.from_numpy(x)
+        for form in (0, 1):
+            y = DeviceArray.from_numpy(np.full(A.nrows, np.nan))
+            K.time_duvi_spmv(Dd, dx, y, form=form, warmup=0, repeats=1)
+            assert close(y.numpy(), ref), (A.nrows, form)
