@@ -318,6 +318,21 @@ struct mgx_nodetransfer_s {
 // small fill / combine kernels used by the drivers
 __global__ void k_set_i32(int32_t *p, int32_t v) { *p = v; }
 
+// The stationary V-cycle's stop test on the device: after cycle `cycle`'s residual_with_check,
+// st = {done, cycle, flag} records the first cycle whose flag says converged (1) or failed (bit 1),
+// or that passed the cycle cap; every kernel queued after it sees done and returns at once, so the
+// host reads st only every few cycles and x stays as it was at that check.
+__global__ void k_gmg_check(const int32_t *flag, int32_t cycle, int32_t max_cycles, int32_t *st)
+{
+    if (st[0]) return;
+    const int32_t f = *flag;
+    if (f == 1 || (f & 2) == 2 || cycle > max_cycles) {
+        st[1] = cycle;
+        st[2] = f;
+        st[0] = 1;
+    }
+}
+
 __global__ __launch_bounds__(kBlock) void k_scale_into(int32_t n, const double *__restrict__ d,
                                                        const double *__restrict__ r, double *__restrict__ z,
                                                        const int32_t *__restrict__ done)
@@ -386,8 +401,8 @@ struct mgx_solver_s {
     mgx::DBuf<double> partials;
     mgx::DBuf<mgx::Ctl> ctl;
     mgx::Ctl *ctl_host = nullptr;          // pinned
-    mgx::DBuf<int32_t> flag;
-    int32_t *flag_host = nullptr;          // pinned
+    mgx::DBuf<int32_t> flag;               // [0] residual_with_check flag; [1..3] GMG state (k_gmg_check)
+    int32_t *flag_host = nullptr;          // pinned, 4 entries
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     int check_every = 4;
     size_t partials_tail = 0;
@@ -737,7 +752,7 @@ static void initialize(mgx_solver_s *S)
     S->partials_tail = 3 * maxgrid;
     S->ctl.alloc(1);
     MGX_HIP(hipMemsetAsync(S->ctl.p, 0, sizeof(Ctl), st));
-    S->flag.alloc(1);
+    S->flag.alloc(4);
     MGX_HIP(hipStreamSynchronize(st));
     if (S->gexec) { MGX_HIP(hipGraphExecDestroy(S->gexec)); S->gexec = nullptr; }
     S->graph_x = nullptr;
@@ -761,6 +776,11 @@ static void solve_gmg(mgx_solver_s *S, const double *b, double *x, bool has_x0, 
     double *tmp = S->lv[0].t.p;
     bool x_given = has_x0;
     const int jac = S->opt.smoother == MGX_SMOOTHER_JACOBI;
+    // the reference reads the flag back after every cycle (OpenCLCsrGeometricMultigridSolver.cs:458-489);
+    // here the device keeps the stop state and the host looks every check_every cycles
+    int32_t *gst = S->flag.p + 1;
+    const int32_t *dn = gst;
+    for (int i = 0; i < 3; ++i) hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, gst + i, 0);
     for (int cycle = 0;; ++cycle) {
         int cur = 0;
         for (int step = 0; step < 2 * depth; ++step) {
@@ -768,27 +788,27 @@ static void solve_gmg(mgx_solver_s *S, const double *b, double *x, bool has_x0, 
             Level &V = S->lv[cur];
             if (cur == L - 1) {
                 // coarse: b[last] <- A^-1 b[last] (OpenCLCsrGeometricMultigridSolver.cs:364-367), then x[last-1] += P b[last]
-                launch_dense_gemv(V.n, S->coarse_inv.p, B[cur], tmp, false, nullptr, s);
+                launch_dense_gemv(V.n, S->coarse_inv.p, B[cur], tmp, false, dn, s);
                 MGX_HIP(hipMemcpyAsync(B[cur], tmp, sizeof(double) * V.n, hipMemcpyDeviceToDevice, s));
                 Level &F = S->lv[cur - 1];
-                level_spmv(S, F, F.P, F.dP, B[cur], X[cur - 1], false, nullptr);
+                level_spmv(S, F, F.P, F.dP, B[cur], X[cur - 1], false, dn);
             } else {
                 const bool ld = step == 0 ? false : (step - 1 < depth);
                 const bool reset = (cur == 0 && !x_given) || (cur != 0 && ld);
                 const int it = S->opt.level_iterations;
                 if (!jac) {
-                    if (reset) launch_fill_zero(V.n, X[cur], nullptr, s);
-                    for (int i = 0; i < it; ++i) level_gs(S, V, B[cur], X[cur], false, nullptr);
+                    if (reset) launch_fill_zero(V.n, X[cur], dn, s);
+                    for (int i = 0; i < it; ++i) level_gs(S, V, B[cur], X[cur], false, dn);
                 } else {
                     int i = 0;
                     if (reset) {
-                        launch_jacobi_initial(V.n, V.pre.p, B[cur], R, nullptr, s);
-                        level_jacobi(S, V, B[cur], R, X[cur], tmp, nullptr);
+                        launch_jacobi_initial(V.n, V.pre.p, B[cur], R, dn, s);
+                        level_jacobi(S, V, B[cur], R, X[cur], tmp, dn);
                         i = 2;
                     }
                     for (; i < it; i += 2) {
-                        level_jacobi(S, V, B[cur], X[cur], R, tmp, nullptr);
-                        level_jacobi(S, V, B[cur], R, X[cur], tmp, nullptr);
+                        level_jacobi(S, V, B[cur], X[cur], R, tmp, dn);
+                        level_jacobi(S, V, B[cur], R, X[cur], tmp, dn);
                     }
                 }
                 if (down) {
@@ -796,27 +816,29 @@ static void solve_gmg(mgx_solver_s *S, const double *b, double *x, bool has_x0, 
                         hipLaunchKernelGGL(k_set_i32, dim3(1), dim3(1), 0, s, S->flag.p, 1);
                         if (S->opt.format == MGX_FORMAT_DUVI)
                             launch_duvi<double>(V.dA.d, V.dA.d.values, DEPI_CHECK, X[0], B[0], nullptr, R, S->flag.p,
-                                                S->opt.tolerance, nullptr, s);
+                                                S->opt.tolerance, dn, s);
                         else
                             launch_residual_flag(V.n, V.A.rp, V.A.ci, V.A.v, B[0], X[0], R, S->opt.tolerance, S->flag.p,
-                                                 V.A.lanes, s);
-                        MGX_HIP(hipMemcpyAsync(S->flag_host, S->flag.p, sizeof(int32_t), hipMemcpyDeviceToHost, s));
-                        MGX_HIP(hipStreamSynchronize(s));
+                                                 V.A.lanes, s, dn);
+                        hipLaunchKernelGGL(k_gmg_check, dim3(1), dim3(1), 0, s, S->flag.p, cycle,
+                                           S->opt.max_iterations, gst);
+                        MGX_HIP(hipGetLastError());
                         x_given = true;
-                        const int32_t f = *S->flag_host;
-                        const bool converged = f == 1, failed = (f & 2) == 2;
-                        if (converged || failed || cycle > S->opt.max_iterations) {
-                            st->iterations = cycle;
-                            st->converged = failed ? -1 : (converged ? 1 : 0);
-                            return;
+                        if ((cycle + 1) % S->check_every == 0 || cycle > S->opt.max_iterations) {
+                            MGX_HIP(hipMemcpyAsync(S->flag_host + 1, gst, 3 * sizeof(int32_t), hipMemcpyDeviceToHost, s));
+                            MGX_HIP(hipStreamSynchronize(s));
+                            if (S->flag_host[1]) {
+                                const int32_t f = S->flag_host[3];
+                                st->iterations = S->flag_host[2];
+                                st->converged = (f & 2) == 2 ? -1 : (f == 1 ? 1 : 0);
+                                return;
+                            }
                         }
-                    } else level_residual(S, V, B[cur], X[cur], R, tmp, nullptr);
-                    Level &C = S->lv[cur + 1];
-                    (void)C;
-                    level_spmv(S, V, V.R, V.dR, R, B[cur + 1], true, nullptr);
+                    } else level_residual(S, V, B[cur], X[cur], R, tmp, dn);
+                    level_spmv(S, V, V.R, V.dR, R, B[cur + 1], true, dn);
                 } else {
                     Level &F = S->lv[cur - 1];
-                    level_spmv(S, F, F.P, F.dP, X[cur], X[cur - 1], false, nullptr);
+                    level_spmv(S, F, F.P, F.dP, X[cur], X[cur - 1], false, dn);
                 }
             }
             cur += down ? 1 : -1;
@@ -2400,7 +2422,7 @@ int mgx_solver_create(const mgx_options *opt, mgx_solver *out)
         s->nrhs = opt->nrhs;
         MGX_HIP(hipStreamCreateWithFlags(&s->stream, hipStreamNonBlocking));
         MGX_HIP(hipHostMalloc((void **)&s->ctl_host, sizeof(Ctl), hipHostMallocDefault));
-        MGX_HIP(hipHostMalloc((void **)&s->flag_host, sizeof(int32_t), hipHostMallocDefault));
+        MGX_HIP(hipHostMalloc((void **)&s->flag_host, 4 * sizeof(int32_t), hipHostMallocDefault));
         MGX_HIP(hipHostMalloc((void **)&s->ctl_in_host, sizeof(Ctl), hipHostMallocDefault));
         MGX_HIP(hipHostMalloc((void **)&s->bctl_host, sizeof(BCtl), hipHostMallocDefault));
         MGX_HIP(hipEventCreate(&s->ev0));

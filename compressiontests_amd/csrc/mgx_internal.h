@@ -61,7 +61,7 @@ void launch_residual_f32(int32_t nrows, const int32_t *rp, const int32_t *ci, co
                          const float *b, const float *x, float *r, int lanes, const int32_t *done, hipStream_t s);
 void launch_residual_flag(int32_t nrows, const int32_t *rp, const int32_t *ci, const double *v,
                           const double *b, const double *x, double *r, double tol, int32_t *flag,
-                          int lanes, hipStream_t s);
+                          int lanes, hipStream_t s, const int32_t *done = nullptr);
 void launch_jacobi(int32_t nrows, const double *pre, const int32_t *rp, const int32_t *ci, const double *v,
                    const double *b, const double *x, double *y, int lanes, const int32_t *done, hipStream_t s);
 void launch_jacobi_f32(int32_t nrows, const float *pre, const int32_t *rp, const int32_t *ci, const float *v,

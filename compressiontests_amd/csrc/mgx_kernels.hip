@@ -256,9 +256,11 @@ template <int G>
 __global__ __launch_bounds__(kBlock) void k_residual_flag(int32_t nrows, const int32_t *__restrict__ rp,
                                                           const int32_t *__restrict__ ci, const double *__restrict__ v,
                                                           const double *__restrict__ b, const double *__restrict__ x,
-                                                          double *__restrict__ r, double tol, int32_t *flag)
+                                                          double *__restrict__ r, double tol, int32_t *flag,
+                                                          const int32_t *__restrict__ done)
 {
     constexpr int RPB = kBlock / G;
+    if (done && *done) return;
     const int blk = xcd_remap(blockIdx.x, gridDim.x);
     const int row = blk * RPB + threadIdx.x / G;
     const int lane = threadIdx.x % G;
@@ -274,11 +276,12 @@ __global__ __launch_bounds__(kBlock) void k_residual_flag(int32_t nrows, const i
 }
 
 void launch_residual_flag(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, const double *b,
-                          const double *x, double *r, double tol, int32_t *flag, int lanes, hipStream_t s)
+                          const double *x, double *r, double tol, int32_t *flag, int lanes, hipStream_t s,
+                          const int32_t *done)
 {
     if (n <= 0) return;
 #define MGX_RF(G) hipLaunchKernelGGL((k_residual_flag<G>), dim3(grid_rows(n, kBlock / G)), dim3(kBlock), 0, s, \
-                                     n, rp, ci, v, b, x, r, tol, flag)
+                                     n, rp, ci, v, b, x, r, tol, flag, done)
     switch (lanes) {
     case 2: MGX_RF(2); break;
     case 4: MGX_RF(4); break;

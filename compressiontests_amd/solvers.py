@@ -179,27 +179,31 @@ class HipCsrGeometricMultigridSolver:
 
     FORMAT = L.FORMAT_CSR
 
-    def __init__(self, maxCircleIterations, convergenceTolerance, levelDown, levelIterations, GaussSeidel=True):
+    def __init__(self, maxCircleIterations, convergenceTolerance, levelDown, levelIterations, GaussSeidel=True,
+                 checkEvery=None):
+        """checkEvery (no reference counterpart): the host reads the device's stop state every
+        checkEvery cycles (None: the library default, 4); the result is the same for every value."""
         self.MaxIterations = maxCircleIterations
         self.ConvergenceTolerance = convergenceTolerance
         self.LevelDown = list(levelDown)
         self.LevelIterations = list(levelIterations) if levelIterations is not None else [4]
         self.GaussSeidel = GaussSeidel
+        self.CheckEvery = checkEvery
         self._h = None
         self.LevelDoFs = None
 
     @classmethod
     def CreateSimpleV(cls, model, GaussSeidel=True, maxCircleIterations=MAX_CIRCLE_ITERATIONS, coarseRelaxation=True,
-                      convergenceTolerance=1e-6, levelIterations=4):
-        a = cls(maxCircleIterations, convergenceTolerance, [True, False], [levelIterations], GaussSeidel)
+                      convergenceTolerance=1e-6, levelIterations=4, checkEvery=None):
+        a = cls(maxCircleIterations, convergenceTolerance, [True, False], [levelIterations], GaussSeidel, checkEvery)
         a.Initialize(model, coarseRelaxation)
         return a
 
     @classmethod
     def CreateDeepV(cls, model, GaussSeidel=True, maxCircleIterations=MAX_CIRCLE_ITERATIONS, coarseRelaxation=True,
-                    convergenceTolerance=1e-6, depth=2, levelIterations=4):
+                    convergenceTolerance=1e-6, depth=2, levelIterations=4, checkEvery=None):
         a = cls(maxCircleIterations, convergenceTolerance, [True] * depth + [False] * depth, [levelIterations],
-                GaussSeidel)
+                GaussSeidel, checkEvery)
         a.Initialize(model, coarseRelaxation)
         return a
 
@@ -218,6 +222,8 @@ class HipCsrGeometricMultigridSolver:
                                 coarse_relaxation=int(bool(coarseRelaxation)), omega=0.0,
                                 max_iterations=self.MaxIterations, tolerance=self.ConvergenceTolerance,
                                 stop=L.STOP_MAXNORM, depth=depth, level_iterations=self.LevelIterations[0])
+        if self.CheckEvery is not None:
+            opt.check_every = int(self.CheckEvery)
         h = _Handle(opt)
         A = model.device_matrix()
         b = model.rhs()

@@ -292,7 +292,9 @@ typedef struct {
     int32_t inner_iterations;   /* mixed precision: max inner PCG iterations per refinement step */
     double  inner_tolerance;    /* mixed precision: inner relative reduction */
     int32_t profile;            /* 1: time every level-0 A*p SpMV with HIP events (mgx_stats.spmv_ms) */
-    int32_t check_every;        /* PCG iterations queued per host check of the device stop flag (default 4) */
+    int32_t check_every;        /* PCG iterations (GMG: V-cycles) queued per host check of the device stop
+                                   flag (default 4); the result does not depend on it.  The reference's GMG
+                                   reads its flag back every cycle (OpenCLCsrGeometricMultigridSolver.cs:458-489) */
     int32_t use_graph;          /* 1: replay the queued PCG iterations as one captured hipGraph (default) */
     int32_t block_width;        /* MG-PCG on CSR with nrhs > 1: right-hand sides advanced together by one
                                    batched PCG (its SpMM reads each matrix once per block); 4, 8, 16 or 32,

@@ -73,6 +73,35 @@ def test_gmg_parity(m16, gs, cls):
     assert len(times) == depth + 1
 
 
+@pytest.mark.parametrize("cls", [HipCsrGeometricMultigridSolver, HipDuViGeometricMultigridSolver])
+def test_gmg_device_stop_state(m16, cls):
+    """The V-cycle's residual_with_check verdict stays on the device and the host reads it every
+    checkEvery cycles (the reference reads it every cycle, OpenCLCsrGeometricMultigridSolver.cs:
+    458-489); the cycles queued past the stopping one return at once.  Reading every cycle and
+    every 5 cycles gives bit-identical solutions and the same cycle counts: converged, at the
+    cycle cap, and failed on a NaN initial guess."""
+    m, A, U, b = m16
+    x0 = np.zeros(A.nrows)
+    x0[7] = np.nan
+    out = []
+    for ce in (1, 5):
+        s = cls.CreateDeepV(m, GaussSeidel=True, maxCircleIterations=20000, convergenceTolerance=1e-5, depth=2,
+                            levelIterations=4, checkEvery=ce)
+        x, st, _ = s.Solve(None)
+        xn, stn, _ = s.Solve(x0)
+        s.ReleaseOpenCLResources()
+        c = cls.CreateDeepV(m, GaussSeidel=True, maxCircleIterations=6, convergenceTolerance=1e-12, depth=2,
+                            levelIterations=4, checkEvery=ce)
+        xc, stc, _ = c.Solve(None)
+        c.ReleaseOpenCLResources()
+        assert st.HasConverged and not stn.HasConverged and stn.Failed
+        assert not stc.HasConverged and not stc.Failed
+        out.append((x, st.NumIterationsRequired, stn.NumIterationsRequired, xc, stc.NumIterationsRequired))
+    assert np.array_equal(out[0][0], out[1][0]) and np.array_equal(out[0][3], out[1][3])
+    assert out[0][1:3] == out[1][1:3] and out[0][4] == out[1][4] == 7
+    assert out[0][2] == 0                      # the first check sees the NaN
+
+
 def test_pcg_jacobi_parity(m16):
     m, A, U, b = m16
     s = HipCsrPcgSolver(20000, 1e-5)
