@@ -2677,7 +2677,7 @@ int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, cons
 {
     return guard([&] {
         MGX_CHECK_ROWS(nrows, rowptr);
-        MGX_REQUIRE(kind >= 0 && kind <= 3 && repeats >= 1 && warmup >= 0 && avg_ms, MGX_ERR_INVALID_ARGUMENT,
+        MGX_REQUIRE(kind >= 0 && kind <= 4 && repeats >= 1 && warmup >= 0 && avg_ms, MGX_ERR_INVALID_ARGUMENT,
                     "invalid arguments");
         MGX_REQUIRE(kind != 1 || (b && pre), MGX_ERR_INVALID_ARGUMENT, "Jacobi needs b and pre");
         MGX_REQUIRE(kind != 2 || b, MGX_ERR_INVALID_ARGUMENT, "residual needs b");
@@ -2687,8 +2687,17 @@ int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, cons
         MGX_HIP(hipEventCreate(&e0));
         MGX_HIP(hipEventCreate(&e1));
         const int lanes = lanes_for(read_nnz(nrows, rowptr, s), nrows);
+        DBuf<Ctl> ctl;
+        DBuf<double> partials;
+        if (kind == 4) {
+            ctl.alloc(1);
+            MGX_HIP(hipMemsetAsync(ctl.p, 0, sizeof(Ctl), s));
+            partials.alloc((size_t)reduce_slots_for_rows(nrows, lanes) + 8);
+        }
         auto one = [&]() {
-            if (kind == 0) launch_spmv(nrows, rowptr, col, (const double *)val, (const double *)x, (double *)y, true, lanes, nullptr, s);
+            if (kind == 4) launch_spmv_dot_alpha(nrows, rowptr, col, (const double *)val, (const double *)x, (double *)y,
+                                                 lanes, ctl.p, partials.p, s);
+            else if (kind == 0) launch_spmv(nrows, rowptr, col, (const double *)val, (const double *)x, (double *)y, true, lanes, nullptr, s);
             else if (kind == 1) launch_jacobi(nrows, (const double *)pre, rowptr, col, (const double *)val, (const double *)b,
                                               (const double *)x, (double *)y, lanes, nullptr, s);
             else if (kind == 2) launch_residual(nrows, rowptr, col, (const double *)val, (const double *)b, (const double *)x,

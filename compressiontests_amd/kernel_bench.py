@@ -63,6 +63,9 @@ def main():
         elif kind == "jacobi":
             ms = kernels.time_csr_kernel(1, A, x, y, b=b, pre=pre, warmup=args.warmup, repeats=args.repeats)
             by = spmv_bytes(nnz, N) + 16 * N
+        elif kind == "spmv_dot":
+            ms = kernels.time_csr_kernel(4, A, x, y, warmup=args.warmup, repeats=args.repeats)
+            by = spmv_bytes(nnz, N)
         elif kind == "residual":
             ms = kernels.time_csr_kernel(2, A, x, y, b=b, warmup=args.warmup, repeats=args.repeats)
             by = spmv_bytes(nnz, N) + 8 * N

@@ -127,7 +127,8 @@ def dot(a, b, stream=None):
 
 
 def time_csr_kernel(kind, A, x, y, b=None, pre=None, warmup=5, repeats=100):
-    """Average device ms per launch (HIP events) of kind 0 SpMV / 1 Jacobi / 2 residual / 3 FP32 SpMV."""
+    """Average device ms per launch (HIP events) of kind 0 SpMV / 1 Jacobi / 2 residual / 3 FP32 SpMV /
+    4 SpMV with the x.y partials and their reduction (the PCG step q = A p, p.q)."""
     out = C.c_double()
     L.call("mgx_time_csr_kernel", kind, A.nrows, C.c_void_p(A.rowptr), C.c_void_p(A.col), C.c_void_p(A.val), x.vp,
            b.vp if b is not None else None, pre.vp if pre is not None else None, y.vp, warmup, repeats, C.byref(out))

@@ -448,7 +448,8 @@ int mgx_device_synchronize(void);
 /* Average device time (HIP events on one stream, after `warmup` untimed launches) of
  * `repeats` back-to-back launches of a CSR row kernel on device arrays:
  * kind 0: y = A x; 1: y = x + pre .* (b - A x); 2: y = b - A x; 3: the same as 0 with FP32
- * values and vectors (val, x, y point to float). */
+ * values and vectors (val, x, y point to float); 4: y = A x with the x.y partials and their
+ * reduction, the PCG step q = A p, p.q (square matrices). */
 int mgx_time_csr_kernel(int32_t kind, int32_t nrows, const int32_t *rowptr, const int32_t *col, const void *val,
                         const void *x, const void *b, const void *pre, void *y, int32_t warmup, int32_t repeats,
                         double *avg_ms);
