@@ -511,12 +511,18 @@ __global__ __launch_bounds__(kPairBlock) void k_nb_gs_pair(const int32_t *__rest
     if (b >= nlines) return;                                     // the whole block leaves together
     const int line = reverse ? nlines - 1 - b : b;
     const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;   // wave-uniform
+    // wave w takes tiles w, w + NWB, ... of each colour; the second colour's in reverse, so a wave
+    // starts it on the stretch of the line its last first-colour tile has just gathered
     for (int h = 0; h < 2; ++h) {
         const int o = 4 * line + 2 * (h ^ second_first);
         const int p0 = rng[o], p1 = rng[o + 1];
-        for (int pa = p0 + wv * Sh::NW; pa < p1; pa += NWB * Sh::NW)
-            nb_gs_tile<T, K, LOWER>(pa, min(pa + Sh::NW, p1), nrp, ncol, node, split, val, B, X, backward, s_val[wv],
-                                    s_col[wv], lane);
+        const int nt = (p1 - p0 + Sh::NW - 1) / Sh::NW;
+        if (wv < nt)
+            for (int i = 0, last = wv + (nt - 1 - wv) / NWB * NWB; wv + i * NWB <= last; ++i) {
+                const int pa = p0 + (h == 0 ? wv + i * NWB : last - i * NWB) * Sh::NW;
+                nb_gs_tile<T, K, LOWER>(pa, min(pa + Sh::NW, p1), nrp, ncol, node, split, val, B, X, backward,
+                                        s_val[wv], s_col[wv], lane);
+            }
         if (h == 0) __syncthreads();
     }
 }
