@@ -151,8 +151,11 @@ typedef struct {
     const int32_t *Arp, *Aci; const double *Av;   /* level matrix */
     const int32_t *Rrp, *Rci; const double *Rv;   /* restriction to level+1 (NULL on coarsest) */
     const int32_t *Prp, *Pci; const double *Pv;   /* interpolation from level+1 (NULL on coarsest) */
-    const int32_t *gs_order;        /* Gauss-Seidel node order of the forward sweep (n/3 nodes; NULL:
-                                       multicolour order of orc_block_coloring) */
+    const int32_t *gs_order;        /* Gauss-Seidel node order of the forward sweep (n/3 nodes, or the
+                                       ngroups groups; NULL: multicolour order of orc_block_coloring /
+                                       orc_group_coloring) */
+    const int32_t *gptr;            /* node groups of the Gauss-Seidel sweep (orc_row_groups); NULL: triples */
+    int32_t ngroups;
 } orc_level;
 
 typedef struct {
@@ -184,6 +187,24 @@ int orc_mgpcg(int nlevels, const orc_level *lv, int smoother, double omega, int 
 int orc_mgpcg_timed(int nlevels, const orc_level *lv, int smoother, double omega, int nu,
                     int max_it, double tol, int stop, const double *b, double *x, orc_stats *st,
                     double *iter_seconds);
+
+/* ---------------- Kirchhoff plate workload (oracle_plate.c) */
+void orc_plate_ke(double a, double b, double E, double nu, double t, double *Ke);
+void orc_plate_load(double a, double b, double q, double *fe);
+int32_t orc_plate_nfree(int nx, int ny, double Lx, double Ly);
+void orc_plate_free_dofs(int nx, int ny, double Lx, double Ly, int32_t *dofs);
+int64_t orc_plate_nnz(int nx, int ny, double Lx, double Ly);
+void orc_plate_assemble(int nx, int ny, double Lx, double Ly, double t, double E, double nu, double q,
+                        int32_t *rowptr, int32_t *col, double *val, double *b);
+void orc_plate_transfer_nnz(int nx, int ny, double Lx, double Ly, int64_t *nnzR, int64_t *nnzP);
+void orc_plate_transfer(int nx, int ny, double Lx, double Ly, int32_t *Rrowptr, int32_t *Rcol, double *Rval,
+                        int32_t *Prowptr, int32_t *Pcol, double *Pval);
+/* node groups: rows gptr[g] .. gptr[g+1]-1 (the free dofs of one mesh node) */
+int32_t orc_row_groups(int32_t n, const int32_t *rp, const int32_t *ci, int32_t *gptr);
+int orc_group_coloring(int32_t ngroups, const int32_t *gptr, const int32_t *rp, const int32_t *ci, int32_t *color);
+void orc_gs_groups(int32_t ngroups, const int32_t *gptr, const int32_t *rp, const int32_t *ci, const double *v,
+                   const int32_t *order, int ncolors, const int32_t *color, const double *b, double *x,
+                   int backward);
 
 /* ---------------- SIMP compliance adjoint */
 void orc_simp_gradient(int nx, int ny, int nz, const double *Ke0, const double *rho, double penal,

@@ -31,7 +31,7 @@ class OrcLevel(C.Structure):
                 ("Arp", C.c_void_p), ("Aci", C.c_void_p), ("Av", C.c_void_p),
                 ("Rrp", C.c_void_p), ("Rci", C.c_void_p), ("Rv", C.c_void_p),
                 ("Prp", C.c_void_p), ("Pci", C.c_void_p), ("Pv", C.c_void_p),
-                ("gs_order", C.c_void_p)]
+                ("gs_order", C.c_void_p), ("gptr", C.c_void_p), ("ngroups", C.c_int32)]
 
 
 class OrcDuviBlock(C.Structure):
@@ -69,6 +69,9 @@ def lib(omp=False):
         L.orc_dot.restype = C.c_double
         L.orc_power_bound.restype = C.c_double
         L.orc_splitmix64_next.restype = C.c_uint64
+        L.orc_plate_nnz.restype = C.c_int64
+        L.orc_plate_nfree.restype = C.c_int32
+        L.orc_row_groups.restype = C.c_int32
         _LIBS[name] = L
     return L
 
@@ -164,6 +167,74 @@ def rp_reference(nx, ny, nz):
     Prp = np.empty(nf + 1, np.int32); Pci = np.empty(npp.value, np.int32); Pv = np.empty(npp.value, np.float64)
     lib().orc_rp_reference(nx, ny, nz, _p(Rrp), _p(Rci), _p(Rv), _p(Prp), _p(Pci), _p(Pv))
     return Csr(Rrp, Rci, Rv, nc, nf), Csr(Prp, Pci, Pv, nf, nc)
+
+
+# ------------------------------------------------------------------ Kirchhoff plate
+def plate_ke(a, b, E=1.0, nu=0.3, t=1.0):
+    """Element stiffness of the 4-node plate rectangle of half-sides a, b (w, rx, ry per node)."""
+    Ke = np.empty((12, 12), np.float64)
+    lib().orc_plate_ke(C.c_double(a), C.c_double(b), C.c_double(E), C.c_double(nu), C.c_double(t), _p(Ke))
+    return Ke
+
+
+def plate_load(a, b, q):
+    fe = np.empty(12, np.float64)
+    lib().orc_plate_load(C.c_double(a), C.c_double(b), C.c_double(q), _p(fe))
+    return fe
+
+
+def plate_free_dofs(nx, ny, Lx, Ly):
+    n = lib().orc_plate_nfree(nx, ny, C.c_double(Lx), C.c_double(Ly))
+    d = np.empty(n, np.int32)
+    lib().orc_plate_free_dofs(nx, ny, C.c_double(Lx), C.c_double(Ly), _p(d))
+    return d
+
+
+def plate(nx, ny, Lx=3.0, Ly=2.0, t=0.1, E=52416.0, nu=0.3, q=-10.0):
+    """(K_ff, b) of FemPlate(nx x ny elements, Lx x Ly x t) with a uniform load q."""
+    n = lib().orc_plate_nfree(nx, ny, C.c_double(Lx), C.c_double(Ly))
+    nnz = lib().orc_plate_nnz(nx, ny, C.c_double(Lx), C.c_double(Ly))
+    rp = np.empty(n + 1, np.int32); ci = np.empty(nnz, np.int32); v = np.empty(nnz, np.float64)
+    b = np.empty(n, np.float64)
+    lib().orc_plate_assemble(nx, ny, C.c_double(Lx), C.c_double(Ly), C.c_double(t), C.c_double(E), C.c_double(nu),
+                             C.c_double(q), _p(rp), _p(ci), _p(v), _p(b))
+    return Csr(rp, ci, v, n, n), b
+
+
+def plate_transfer(nx, ny, Lx=3.0, Ly=2.0):
+    """(R, P) between the nx x ny plate and its coarser model."""
+    nr, npp = C.c_int64(), C.c_int64()
+    lib().orc_plate_transfer_nnz(nx, ny, C.c_double(Lx), C.c_double(Ly), C.byref(nr), C.byref(npp))
+    cx, cy = [(k >> 1) if k > 1 else k for k in (nx, ny)]
+    nf = lib().orc_plate_nfree(nx, ny, C.c_double(Lx), C.c_double(Ly))
+    nc = lib().orc_plate_nfree(cx, cy, C.c_double(Lx), C.c_double(Ly))
+    Rrp = np.empty(nc + 1, np.int32); Rci = np.empty(nr.value, np.int32); Rv = np.empty(nr.value, np.float64)
+    Prp = np.empty(nf + 1, np.int32); Pci = np.empty(npp.value, np.int32); Pv = np.empty(npp.value, np.float64)
+    lib().orc_plate_transfer(nx, ny, C.c_double(Lx), C.c_double(Ly), _p(Rrp), _p(Rci), _p(Rv), _p(Prp), _p(Pci),
+                             _p(Pv))
+    return Csr(Rrp, Rci, Rv, nc, nf), Csr(Prp, Pci, Pv, nf, nc)
+
+
+def row_groups(A):
+    """gptr of the Gauss-Seidel node groups (runs of <= 3 rows with identical column lists)."""
+    g = np.empty(A.nrows + 1, np.int32)
+    ng = lib().orc_row_groups(A.nrows, _p(A.rowptr), _p(A.col), _p(g))
+    return g[:ng + 1].copy()
+
+
+def group_coloring(A, gptr):
+    col = np.empty(len(gptr) - 1, np.int32)
+    nc = lib().orc_group_coloring(len(gptr) - 1, _p(gptr), _p(A.rowptr), _p(A.col), _p(col))
+    return nc, col
+
+
+def gs_groups(A, gptr, b, x, ncol=0, color=None, order=None, backward=False):
+    x = np.array(x, np.float64)
+    o = None if order is None else np.ascontiguousarray(order, np.int32)
+    lib().orc_gs_groups(len(gptr) - 1, _p(gptr), _p(A.rowptr), _p(A.col), _p(A.val), None if o is None else _p(o),
+                        ncol, None if color is None else _p(color), _p(np.ascontiguousarray(b, np.float64)), _p(x),
+                        int(backward))
+    return x
 
 
 def p_trilinear(nx, ny, nz):
@@ -426,9 +497,14 @@ def _levels(levels):
             arr[i].Rrp, arr[i].Rci, arr[i].Rv = _p(R.rowptr), _p(R.col), _p(R.val)
             arr[i].Prp, arr[i].Pci, arr[i].Pv = _p(P.rowptr), _p(P.col), _p(P.val)
             keep += [R, P]
+        if lv.get("gptr") is not None:
+            g = np.ascontiguousarray(lv["gptr"], np.int32)
+            assert g[0] == 0 and g[-1] == A.nrows
+            arr[i].gptr, arr[i].ngroups = _p(g), g.size - 1
+            keep.append(g)
         if lv.get("gs_order") is not None:
             o = np.ascontiguousarray(lv["gs_order"], np.int32)
-            assert o.size == A.nrows // 3
+            assert o.size == (A.nrows // 3 if lv.get("gptr") is None else len(lv["gptr"]) - 1)
             arr[i].gs_order = _p(o)
             keep.append(o)
     return arr, keep

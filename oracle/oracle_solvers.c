@@ -52,6 +52,23 @@ static double *coarse_factor(const orc_level *c)
     return L;
 }
 
+/* Gauss-Seidel node sets of a level: its groups when given (nodes of 2 or 3 free dofs), else
+   the dof triples */
+static int level_nodes(const orc_level *q) { return q->gptr ? q->ngroups : q->n / 3; }
+
+static int level_coloring(const orc_level *q, int32_t *color)
+{
+    if (q->gptr) return orc_group_coloring(q->ngroups, q->gptr, q->Arp, q->Aci, color);
+    return orc_block_coloring(q->n / 3, 3, q->Arp, q->Aci, color);
+}
+
+static void level_gs(const orc_level *q, int ncol, const int32_t *color, const double *b, double *x, int backward)
+{
+    if (q->gptr) orc_gs_groups(q->ngroups, q->gptr, q->Arp, q->Aci, q->Av, q->gs_order, ncol, color, b, x, backward);
+    else if (q->gs_order) orc_gs_ordered(q->n, 3, q->Arp, q->Aci, q->Av, q->gs_order, b, x, backward);
+    else orc_gs_multicolor(q->n, 3, q->Arp, q->Aci, q->Av, ncol, color, b, x, backward);
+}
+
 int orc_gmg_solve(int nlevels, const orc_level *lv, const double *const *pre, int smoother,
                   int depth, int level_iterations, int max_cycles, double tol,
                   const double *b, double *x0, int has_x0, orc_stats *st)
@@ -69,8 +86,8 @@ int orc_gmg_solve(int nlevels, const orc_level *lv, const double *const *pre, in
     for (int l = 1; l < L; ++l) { B[l] = dalloc(lv[l].n); if (l < L - 1) X[l] = dalloc(lv[l].n); }
     if (smoother == 1)
         for (int l = 0; l < L - 1; ++l) {
-            color[l] = (int32_t *)malloc(sizeof(int32_t) * (lv[l].n / 3));
-            ncol[l] = orc_block_coloring(lv[l].n / 3, 3, lv[l].Arp, lv[l].Aci, color[l]);
+            color[l] = (int32_t *)malloc(sizeof(int32_t) * (size_t)(level_nodes(&lv[l]) + 1));
+            ncol[l] = level_coloring(&lv[l], color[l]);
         }
     double *R = dalloc(lv[0].n);
     double *tmp = dalloc(lv[L - 1].n);
@@ -92,8 +109,7 @@ int orc_gmg_solve(int nlevels, const orc_level *lv, const double *const *pre, in
                 int it = level_iterations;
                 if (smoother == 1) {
                     if (reset) memset(X[cur], 0, sizeof(double) * q->n);
-                    for (int i = 0; i < it; ++i)
-                        orc_gs_multicolor(q->n, 3, q->Arp, q->Aci, q->Av, ncol[cur], color[cur], B[cur], X[cur], 0);
+                    for (int i = 0; i < it; ++i) level_gs(q, ncol[cur], color[cur], B[cur], X[cur], 0);
                 } else {
                     int i;
                     if (reset) {
@@ -197,8 +213,7 @@ static void vcycle(vcyc *V, int l)
     if (l == V->L - 1) { orc_dense_cholesky_solve(q->n, V->Lc, b, x); return; }
     memset(x, 0, sizeof(double) * q->n);
     for (int s = 0; s < V->nu; ++s) {
-        if (V->smoother == 1 && q->gs_order) orc_gs_ordered(q->n, 3, q->Arp, q->Aci, q->Av, q->gs_order, b, x, 0);
-        else if (V->smoother == 1) orc_gs_multicolor(q->n, 3, q->Arp, q->Aci, q->Av, V->ncol[l], V->color[l], b, x, 0);
+        if (V->smoother == 1) level_gs(q, V->ncol[l], V->color[l], b, x, 0);
         else if (s == 0) orc_jacobi_initial(q->n, V->pre[l], b, x);
         else { orc_csr_jacobi(q->n, V->pre[l], q->Arp, q->Aci, q->Av, b, x, r); memcpy(x, r, sizeof(double) * q->n); }
     }
@@ -207,8 +222,7 @@ static void vcycle(vcyc *V, int l)
     vcycle(V, l + 1);
     orc_csr_spmv(q->n, q->Prp, q->Pci, q->Pv, V->x[l + 1], x, 0);
     for (int s = 0; s < V->nu; ++s) {
-        if (V->smoother == 1 && q->gs_order) orc_gs_ordered(q->n, 3, q->Arp, q->Aci, q->Av, q->gs_order, b, x, 1);
-        else if (V->smoother == 1) orc_gs_multicolor(q->n, 3, q->Arp, q->Aci, q->Av, V->ncol[l], V->color[l], b, x, 1);
+        if (V->smoother == 1) level_gs(q, V->ncol[l], V->color[l], b, x, 1);
         else { orc_csr_jacobi(q->n, V->pre[l], q->Arp, q->Aci, q->Av, b, x, r); memcpy(x, r, sizeof(double) * q->n); }
     }
 }
@@ -243,8 +257,8 @@ static int vcyc_setup(vcyc *Vp, int nlevels, const orc_level *lv, int smoother, 
         V.x[l] = dalloc(lv[l].n); V.b[l] = dalloc(lv[l].n); V.r[l] = dalloc(lv[l].n);
         if (l < nlevels - 1) {
             if (smoother == 1) {
-                V.color[l] = (int32_t *)malloc(sizeof(int32_t) * (lv[l].n / 3));
-                V.ncol[l] = orc_block_coloring(lv[l].n / 3, 3, lv[l].Arp, lv[l].Aci, V.color[l]);
+                V.color[l] = (int32_t *)malloc(sizeof(int32_t) * (size_t)(level_nodes(&lv[l]) + 1));
+                V.ncol[l] = level_coloring(&lv[l], V.color[l]);
             } else {
                 V.pre[l] = dalloc(lv[l].n);
                 orc_jacobi_preconditioner(lv[l].n, lv[l].Arp, lv[l].Aci, lv[l].Av, V.pre[l]);

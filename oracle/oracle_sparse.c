@@ -213,6 +213,81 @@ void orc_gs_multicolor(int32_t n, int bs, const int32_t *rp, const int32_t *ci, 
     }
 }
 
+/* ---------------------------------------------------------------- node groups
+ * Multicolour Gauss-Seidel by mesh node when nodes carry different numbers of free dofs (the
+ * plate: 2 at supported nodes, 3 elsewhere).  A group is a maximal run of at most 3 consecutive
+ * rows with identical column lists -- the free dofs of one node, which couple to the same
+ * neighbours; on the cantilever these are the node triples of orc_block_coloring. */
+int32_t orc_row_groups(int32_t n, const int32_t *rp, const int32_t *ci, int32_t *gptr)
+{
+    int32_t ng = 0, r = 0;
+    while (r < n) {
+        gptr[ng++] = r;
+        int32_t e = r + 1;
+        while (e < n && e - r < 3 && rp[e + 1] - rp[e] == rp[r + 1] - rp[r] &&
+               memcmp(ci + rp[e], ci + rp[r], sizeof(int32_t) * (size_t)(rp[r + 1] - rp[r])) == 0)
+            ++e;
+        r = e;
+    }
+    gptr[ng] = n;
+    return ng;
+}
+
+/* greedy distance-1 colouring of the groups in natural order (orc_block_coloring with groups) */
+int orc_group_coloring(int32_t ngroups, const int32_t *gptr, const int32_t *rp, const int32_t *ci, int32_t *color)
+{
+    const int32_t n = gptr[ngroups];
+    int32_t *grp = (int32_t *)malloc(sizeof(int32_t) * (size_t)(n > 0 ? n : 1));
+    for (int32_t g = 0; g < ngroups; ++g)
+        for (int32_t r = gptr[g]; r < gptr[g + 1]; ++r) grp[r] = g;
+    int ncol = 0;
+    uint64_t used[4];
+    for (int32_t k = 0; k < ngroups; ++k) color[k] = -1;
+    for (int32_t k = 0; k < ngroups; ++k) {
+        memset(used, 0, sizeof used);
+        for (int32_t r = gptr[k]; r < gptr[k + 1]; ++r)
+            for (int32_t q = rp[r]; q < rp[r + 1]; ++q) {
+                const int32_t nb = grp[ci[q]];
+                if (nb != k && color[nb] >= 0 && color[nb] < 256) used[color[nb] >> 6] |= 1ULL << (color[nb] & 63);
+            }
+        int c = 0;
+        while (c < 256 && (used[c >> 6] >> (c & 63) & 1)) ++c;
+        color[k] = c;
+        if (c + 1 > ncol) ncol = c + 1;
+    }
+    free(grp);
+    return ncol;
+}
+
+/* Gauss-Seidel sweep over the groups: in `order` (group ids) when given, else colour by colour
+ * (groups ascending within a colour); a group's rows in order, all reversed when backward */
+void orc_gs_groups(int32_t ngroups, const int32_t *gptr, const int32_t *rp, const int32_t *ci, const double *v,
+                   const int32_t *order, int ncolors, const int32_t *color, const double *b, double *x,
+                   int backward)
+{
+    if (order) {
+        if (!backward) {
+            for (int32_t i = 0; i < ngroups; ++i)
+                for (int32_t r = gptr[order[i]]; r < gptr[order[i] + 1]; ++r) gs_row(r, rp, ci, v, b, x);
+        } else {
+            for (int32_t i = ngroups - 1; i >= 0; --i)
+                for (int32_t r = gptr[order[i] + 1] - 1; r >= gptr[order[i]]; --r) gs_row(r, rp, ci, v, b, x);
+        }
+        return;
+    }
+    if (!backward) {
+        for (int c = 0; c < ncolors; ++c)
+            for (int32_t k = 0; k < ngroups; ++k)
+                if (color[k] == c)
+                    for (int32_t r = gptr[k]; r < gptr[k + 1]; ++r) gs_row(r, rp, ci, v, b, x);
+    } else {
+        for (int c = ncolors - 1; c >= 0; --c)
+            for (int32_t k = ngroups - 1; k >= 0; --k)
+                if (color[k] == c)
+                    for (int32_t r = gptr[k + 1] - 1; r >= gptr[k]; --r) gs_row(r, rp, ci, v, b, x);
+    }
+}
+
 /* ---------------------------------------------------------------- dense coarse solve */
 void orc_csr_to_dense(int32_t n, const int32_t *rp, const int32_t *ci, const double *v, double *A)
 {
