@@ -153,6 +153,9 @@ _SIGS = {
     "mgx_matrix_cantilever": (C.c_int, [_I32, _I32, _I32, _D, _D, _P, C.POINTER(_P)]),
     "mgx_matrix_transfer_reference": (C.c_int, [_I32, _I32, _I32, C.POINTER(_P), C.POINTER(_P)]),
     "mgx_matrix_transfer_trilinear": (C.c_int, [_I32, _I32, _I32, C.POINTER(_P), C.POINTER(_P)]),
+    "mgx_plate_free_dofs": (C.c_int, [_I32, _I32, _D, _D, C.POINTER(C.c_int32), _P]),
+    "mgx_matrix_plate": (C.c_int, [_I32, _I32, _D, _D, _D, _D, _D, _D, C.POINTER(_P), _P]),
+    "mgx_matrix_transfer_plate": (C.c_int, [_I32, _I32, _D, _D, C.POINTER(_P), C.POINTER(_P)]),
     "mgx_matrix_galerkin_hierarchy": (C.c_int, [_I32, _I32, _I32, _D, _D, _P, _I32, C.POINTER(_P)]),
     "mgx_matrix_from_host": (C.c_int, [_I32, _I32, _P, _P, _P, C.POINTER(_P)]),
     "mgx_matrix_view": (C.c_int, [_P, C.POINTER(CsrView)]),
@@ -170,6 +173,7 @@ _SIGS = {
     "mgx_solver_level_layout": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
     "mgx_solver_transfer_layout": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
     "mgx_solver_level_colors": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
+    "mgx_solver_level_groups": (C.c_int, [_P, _I32, C.POINTER(C.c_int32), _P]),
     "mgx_solver_gs_order": (C.c_int, [_P, _I32, _P]),
     "mgx_solver_gs_pairs": (C.c_int, [_P, _I32, C.POINTER(C.c_int32)]),
     "mgx_solver_strip_window": (C.c_int, [_P, C.POINTER(C.c_double), C.POINTER(C.c_double)]),
@@ -191,6 +195,8 @@ _SIGS = {
     "mgx_hierarchy_info": (C.c_int, [C.c_char_p, C.POINTER(HierarchySummary)]),
     "mgx_hierarchy_read_csr": (C.c_int, [C.c_char_p, _I32, _I32, C.POINTER(CsrOwned)]),
     "mgx_csr_owned_free": (None, [C.POINTER(CsrOwned)]),
+    "mgx_plate_system_host": (C.c_int, [_I32, _I32, _D, _D, _D, _D, _D, _D, C.POINTER(CsrOwned), _P]),
+    "mgx_plate_transfer_host": (C.c_int, [_I32, _I32, _D, _D, C.POINTER(CsrOwned), C.POINTER(CsrOwned)]),
     "mgx_hierarchy_read_duvi": (C.c_int, [C.c_char_p, _I32, _I32, C.POINTER(DuviHost)]),
     "mgx_hierarchy_read_grid": (C.c_int, [C.c_char_p, _I32, _P]),
     "mgx_hierarchy_save": (C.c_int, [C.c_char_p, _I32, _P, _P, _P, _I32, _D]),

@@ -378,6 +378,8 @@ struct Level {
     std::vector<int8_t> pair_ok;           // colours 2k, 2k+1: every edge between them lies on one mesh line
     std::vector<DBuf<int32_t>> pair_rng;   // per such pair: 4 positions per line (launch_nb_gs_pair)
     std::vector<int32_t> pair_lines;       // lines of pair_rng[k] (0: the pair runs colour by colour)
+    DBuf<int32_t> gptr;                    // Gauss-Seidel node groups when the nodes are not row triples
+    int32_t ngroups = 0;                   // (0: node q = rows 3q .. 3q+2)
     std::vector<DBuf<int32_t>> color_nodes;
     std::vector<int32_t> color_count;
     DevDuvi dA, dR, dP;
@@ -476,10 +478,10 @@ static void level_gs(mgx_solver_s *S, Level &L, const double *b, double *x, bool
         const int c = backward ? L.ncolors - 1 - i : i;
         if (S->opt.format == MGX_FORMAT_DUVI)
             launch_duvi_gs<double>(L.dA.d, L.dA.d.values, L.color_count[c], L.color_nodes[c].p, b, x, backward, done,
-                                   S->stream);
+                                   S->stream, L.gptr.p);
         else
             launch_gs_color(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v, b, x, backward, done,
-                            S->stream);
+                            S->stream, L.gptr.p);
     }
 }
 
@@ -655,9 +657,21 @@ static void initialize(mgx_solver_s *S)
     if (S->opt.smoother == MGX_SMOOTHER_GS && S->opt.method != MGX_METHOD_PCG_JACOBI) {
         for (int l = 0; l < L - 1; ++l) {
             Level &V = S->lv[l];
-            MGX_REQUIRE(V.n % 3 == 0, MGX_ERR_INVALID_ARGUMENT, "Gauss-Seidel node blocks need 3 dofs per node");
-            std::vector<int32_t> color;
-            V.ncolors = block_coloring(V.n / 3, 3, hrp[l].data(), hci[l].data(), color);
+            // nodes: the row triples, or, where the nodes' free dofs differ in number (the plate's
+            // supported nodes), runs of up to 3 rows with identical column lists
+            std::vector<int32_t> color, gptr;
+            const int32_t ng = row_groups(V.n, hrp[l].data(), hci[l].data(), gptr);
+            bool triples = V.n % 3 == 0 && ng == V.n / 3;
+            for (int32_t g = 0; triples && g < ng; ++g) triples = gptr[g] == 3 * g;
+            V.gptr.reset();
+            V.ngroups = 0;
+            if (triples) {
+                V.ncolors = block_coloring(V.n / 3, 3, hrp[l].data(), hci[l].data(), color);
+            } else {
+                V.ncolors = group_coloring(ng, gptr.data(), hrp[l].data(), hci[l].data(), color);
+                V.gptr.upload(gptr.data(), gptr.size(), st);
+                V.ngroups = ng;
+            }
             std::vector<std::vector<int32_t>> lists(V.ncolors);
             for (int32_t k = 0; k < (int32_t)color.size(); ++k) lists[color[k]].push_back(k);
             V.color_nodes.clear();
@@ -672,7 +686,7 @@ static void initialize(mgx_solver_s *S)
             // colours must join two nodes of one line
             V.pair_ok.assign(V.ncolors / 2, 0);
             const int64_t lines = V.gy + 1, planes = V.gz + 1;
-            if (V.gx > 0 && (int64_t)V.gx * lines * planes * 3 == V.n) {
+            if (triples && V.gx > 0 && (int64_t)V.gx * lines * planes * 3 == V.n) {
                 std::vector<int8_t> ok(V.ncolors / 2, 1);
                 const auto &rp = hrp[l];
                 const auto &ci = hci[l];
@@ -1010,10 +1024,10 @@ static void level_gs32(mgx_solver_s *S, Level &L, const float *b, float *x, bool
         const int c = backward ? L.ncolors - 1 - i : i;
         if (S->opt.format == MGX_FORMAT_DUVI)
             launch_duvi_gs<float>(L.dA.d, L.dA.values32.p, L.color_count[c], L.color_nodes[c].p, b, x, backward, done,
-                                  S->stream);
+                                  S->stream, L.gptr.p);
         else
             launch_gs_color_f32(L.color_count[c], L.color_nodes[c].p, L.A.rp, L.A.ci, L.A.v32, b, x, backward, done,
-                                S->stream);
+                                S->stream, L.gptr.p);
     }
 }
 
@@ -1243,7 +1257,7 @@ static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, b
                               S->stream);
         else
             launch_gs_block<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done,
-                               S->stream);
+                               S->stream, V.gptr.p);
     }
     if (rec) {
         MGX_HIP(hipEventRecord(S->gs_events[2 * S->gs_used + 1], S->stream));
@@ -2267,6 +2281,82 @@ int mgx_matrix_transfer_reference(int32_t nx, int32_t ny, int32_t nz, mgx_matrix
     });
 }
 
+int mgx_plate_free_dofs(int32_t nx, int32_t ny, double Lx, double Ly, int32_t *count, int32_t *dofs)
+{
+    return guard([&] {
+        MGX_REQUIRE(count && nx > 0 && ny > 0 && Lx > 0 && Ly > 0, MGX_ERR_INVALID_ARGUMENT, "invalid plate");
+        std::vector<int32_t> d;
+        *count = plate_free_dofs(nx, ny, Lx, Ly, dofs ? &d : nullptr);
+        if (dofs) std::copy(d.begin(), d.end(), dofs);
+    });
+}
+
+int mgx_matrix_plate(int32_t nx, int32_t ny, double Lx, double Ly, double t, double E, double nu, double q,
+                     mgx_matrix *A, double *b)
+{
+    return guard([&] {
+        MGX_REQUIRE(A && nx > 0 && ny > 0 && Lx > 0 && Ly > 0 && t > 0 && E > 0 && nu > -1 && nu < 0.5,
+                    MGX_ERR_INVALID_ARGUMENT, "invalid plate");
+        HostCsr K;
+        std::vector<double> rhs;
+        plate_system(nx, ny, Lx, Ly, t, E, nu, q, K, b ? &rhs : nullptr);
+        auto m = std::make_unique<mgx_matrix_s>();
+        upload_host_csr(m->A, K, nullptr);
+        if (b) std::copy(rhs.begin(), rhs.end(), b);
+        *A = m.release();
+    });
+}
+
+static void to_owned(const HostCsr &h, mgx_csr_owned *out)
+{
+    const int64_t nnz = h.rowptr.empty() ? 0 : h.rowptr.back();
+    std::unique_ptr<int32_t[]> rp(new int32_t[h.rowptr.size()]), ci(new int32_t[nnz > 0 ? nnz : 1]);
+    std::unique_ptr<double[]> v(new double[nnz > 0 ? nnz : 1]);
+    std::copy(h.rowptr.begin(), h.rowptr.end(), rp.get());
+    std::copy(h.col.begin(), h.col.end(), ci.get());
+    std::copy(h.val.begin(), h.val.end(), v.get());
+    out->nrows = h.nrows; out->ncols = h.ncols; out->nnz = nnz;
+    out->rowptr = rp.release(); out->col = ci.release(); out->val = v.release();
+}
+
+int mgx_plate_system_host(int32_t nx, int32_t ny, double Lx, double Ly, double t, double E, double nu, double q,
+                          mgx_csr_owned *K, double *b)
+{
+    return guard([&] {
+        MGX_REQUIRE(K && nx > 0 && ny > 0 && Lx > 0 && Ly > 0 && t > 0 && E > 0 && nu > -1 && nu < 0.5,
+                    MGX_ERR_INVALID_ARGUMENT, "invalid plate");
+        HostCsr h;
+        std::vector<double> rhs;
+        plate_system(nx, ny, Lx, Ly, t, E, nu, q, h, b ? &rhs : nullptr);
+        if (b) std::copy(rhs.begin(), rhs.end(), b);
+        to_owned(h, K);
+    });
+}
+
+int mgx_plate_transfer_host(int32_t nx, int32_t ny, double Lx, double Ly, mgx_csr_owned *R, mgx_csr_owned *P)
+{
+    return guard([&] {
+        MGX_REQUIRE(R && P && nx > 0 && ny > 0 && Lx > 0 && Ly > 0, MGX_ERR_INVALID_ARGUMENT, "invalid plate");
+        HostCsr hr, hp;
+        plate_transfer(nx, ny, Lx, Ly, hr, hp);
+        to_owned(hr, R);
+        to_owned(hp, P);
+    });
+}
+
+int mgx_matrix_transfer_plate(int32_t nx, int32_t ny, double Lx, double Ly, mgx_matrix *R, mgx_matrix *P)
+{
+    return guard([&] {
+        MGX_REQUIRE(R && P && nx > 0 && ny > 0 && Lx > 0 && Ly > 0, MGX_ERR_INVALID_ARGUMENT, "invalid plate");
+        HostCsr hr, hp;
+        plate_transfer(nx, ny, Lx, Ly, hr, hp);
+        auto r = std::make_unique<mgx_matrix_s>(), p = std::make_unique<mgx_matrix_s>();
+        upload_host_csr(r->A, hr, nullptr);
+        upload_host_csr(p->A, hp, nullptr);
+        *R = r.release(); *P = p.release();
+    });
+}
+
 int mgx_matrix_transfer_trilinear(int32_t nx, int32_t ny, int32_t nz, mgx_matrix *R, mgx_matrix *P)
 {
     return guard([&] {
@@ -2589,6 +2679,19 @@ int mgx_solver_gs_order(mgx_solver s, int32_t level, int32_t *order)
                                   hipMemcpyDeviceToHost));
             off += V.color_count[c];
         }
+    });
+}
+
+int mgx_solver_level_groups(mgx_solver s, int32_t level, int32_t *count, int32_t *gptr)
+{
+    return guard([&] {
+        MGX_REQUIRE(s && count, MGX_ERR_INVALID_ARGUMENT, "null argument");
+        MGX_REQUIRE(s->initialized, MGX_ERR_INVALID_OPERATION, "You must call Initialize(model) first");
+        MGX_REQUIRE(level >= 0 && level < (int)s->lv.size(), MGX_ERR_INVALID_ARGUMENT, "level out of range");
+        const Level &V = s->lv[level];
+        *count = V.ngroups;
+        if (gptr && V.ngroups > 0)
+            MGX_HIP(hipMemcpy(gptr, V.gptr.p, sizeof(int32_t) * (V.ngroups + 1), hipMemcpyDeviceToHost));
     });
 }
 

@@ -464,11 +464,13 @@ template void launch_gs_node<float>(int, int32_t, const int32_t *, const int32_t
 // A block of 192 threads relaxes N nodes: thread (m, a, s, c) reduces row a of node m for
 // column c over the entries s, s + S, ...: sa = sum over columns outside the node, and the
 // node's 3x3 diagonal block (the same for every column).  Then one thread per (node, column)
-// relaxes the three dofs in order (reverse order when backward).
+// relaxes the three dofs in order (reverse order when backward).  With gptr, node q is the group
+// of one to three rows gptr[q] .. gptr[q+1] (threads of the missing rows idle).
 constexpr int kGsbBlock = 192;
 
 template <typename T, int K>
 __global__ __launch_bounds__(kGsbBlock) void k_gs_block(int32_t nnodes, const int32_t *__restrict__ nodes,
+                                                        const int32_t *__restrict__ gptr,
                                                         const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                         const T *__restrict__ v, const T *__restrict__ B,
                                                         T *__restrict__ X, int backward, const int32_t *__restrict__ done)
@@ -481,9 +483,13 @@ __global__ __launch_bounds__(kGsbBlock) void k_gs_block(int32_t nnodes, const in
     const int t = threadIdx.x, l = t % G, c = l % K, sl = l / K, a = (t / G) % 3, m = t / (3 * G);
     const int nd = base + m;
     T sa = 0, d0 = 0, d1 = 0, d2 = 0;
-    int r0 = 0;
+    int r0 = 0, nr = 0;
     if (nd < nnodes) {
-        r0 = 3 * nodes[nd];
+        const int q = nodes[nd];
+        r0 = gptr ? gptr[q] : 3 * q;
+        nr = gptr ? gptr[q + 1] - r0 : 3;
+    }
+    if (a < nr) {
         const int s = rp[r0 + a], e = rp[r0 + a + 1];
         for (int k = s + sl; k < e; k += U * S) {
             int cc[U];
@@ -497,13 +503,13 @@ __global__ __launch_bounds__(kGsbBlock) void k_gs_block(int32_t nnodes, const in
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const unsigned off = (unsigned)(cc[u] - r0);
-                xv[u] = (cc[u] >= 0 && off >= 3u) ? X[(size_t)cc[u] * K + c] : T(0);
+                xv[u] = (cc[u] >= 0 && off >= (unsigned)nr) ? X[(size_t)cc[u] * K + c] : T(0);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 if (cc[u] < 0) continue;
                 const unsigned off = (unsigned)(cc[u] - r0);
-                if (off < 3u) { if (off == 0) d0 += w[u]; else if (off == 1) d1 += w[u]; else d2 += w[u]; }
+                if (off < (unsigned)nr) { if (off == 0) d0 += w[u]; else if (off == 1) d1 += w[u]; else d2 += w[u]; }
                 else sa += w[u] * xv[u];
             }
         }
@@ -523,32 +529,40 @@ __global__ __launch_bounds__(kGsbBlock) void k_gs_block(int32_t nnodes, const in
     if (t < N * K) {
         const int m2 = t / K, c2 = t % K, nd2 = base + m2;
         if (nd2 < nnodes) {
-            const size_t q0 = (size_t)3 * nodes[nd2] * K + c2;
+            const int q = nodes[nd2], g0 = gptr ? gptr[q] : 3 * q, n2 = gptr ? gptr[q + 1] - g0 : 3;
+            const size_t q0 = (size_t)g0 * K + c2;
             const T(*D)[3] = dg[m2];
-            T x0 = X[q0], x1 = X[q0 + K], x2 = X[q0 + 2 * K];
-            const T b0 = B[q0] - sh[m2][0][c2], b1 = B[q0 + K] - sh[m2][1][c2], b2 = B[q0 + 2 * K] - sh[m2][2][c2];
-            if (!backward) {
-                x0 = (b0 - D[0][1] * x1 - D[0][2] * x2) / D[0][0];
-                x1 = (b1 - D[1][0] * x0 - D[1][2] * x2) / D[1][1];
-                x2 = (b2 - D[2][0] * x0 - D[2][1] * x1) / D[2][2];
-            } else {
-                x2 = (b2 - D[2][0] * x0 - D[2][1] * x1) / D[2][2];
-                x1 = (b1 - D[1][0] * x0 - D[1][2] * x2) / D[1][1];
-                x0 = (b0 - D[0][1] * x1 - D[0][2] * x2) / D[0][0];
+            T xv[3], bv[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                xv[r] = r < n2 ? X[q0 + r * K] : T(0);
+                bv[r] = r < n2 ? B[q0 + r * K] - sh[m2][r][c2] : T(0);
             }
-            X[q0] = x0; X[q0 + K] = x1; X[q0 + 2 * K] = x2;
+#pragma unroll
+            for (int t2 = 0; t2 < 3; ++t2) {
+                const int r = backward ? 2 - t2 : t2;
+                if (r >= n2) continue;
+                T y = bv[r];
+#pragma unroll
+                for (int cc2 = 0; cc2 < 3; ++cc2)
+                    if (cc2 != r) y -= D[r][cc2] * xv[cc2];     // zero past the group's rows
+                xv[r] = y / D[r][r];
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                if (r < n2) X[q0 + r * K] = xv[r];
         }
     }
 }
 
 template <typename T>
 void launch_gs_block(int K, int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const T *v,
-                     const T *B, T *X, bool backward, const int32_t *done, hipStream_t s)
+                     const T *B, T *X, bool backward, const int32_t *done, hipStream_t s, const int32_t *gptr)
 {
     if (nnodes <= 0) return;
 #define MGX_GSB(KK)                                                                                          \
     hipLaunchKernelGGL((k_gs_block<T, KK>), dim3(grid_rows(nnodes, kGsbBlock / (3 * BShape<KK>::G))),       \
-                       dim3(kGsbBlock), 0, s, nnodes, nodes, rp, ci, v, B, X, backward ? 1 : 0, done)
+                       dim3(kGsbBlock), 0, s, nnodes, nodes, gptr, rp, ci, v, B, X, backward ? 1 : 0, done)
     switch (K) {
     case 4: MGX_GSB(4); break;
     case 8: MGX_GSB(8); break;
@@ -560,9 +574,9 @@ void launch_gs_block(int K, int32_t nnodes, const int32_t *nodes, const int32_t 
     MGX_HIP(hipGetLastError());
 }
 template void launch_gs_block<double>(int, int32_t, const int32_t *, const int32_t *, const int32_t *, const double *,
-                                      const double *, double *, bool, const int32_t *, hipStream_t);
+                                      const double *, double *, bool, const int32_t *, hipStream_t, const int32_t *);
 template void launch_gs_block<float>(int, int32_t, const int32_t *, const int32_t *, const int32_t *, const float *,
-                                     const float *, float *, bool, const int32_t *, hipStream_t);
+                                     const float *, float *, bool, const int32_t *, hipStream_t, const int32_t *);
 
 // ------------------------------------------------------------------ block vector kernels
 // Every thread keeps one column: blockDim (256) and the grid stride are multiples of K.

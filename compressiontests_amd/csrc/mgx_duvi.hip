@@ -460,8 +460,8 @@ __global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t n
 }
 
 // One colour of a multicolour Gauss-Seidel sweep: the nodes nodes[0 .. nnodes) are pairwise
-// uncoupled; node q's rows 3q, 3q+1, 3q+2 are relaxed in that order (the reverse when backward)
-// with the current x.  A wavefront takes 2 nodes, 8 lanes per row (lanes 48-63 idle).  A row's
+// uncoupled; node q's rows 3q, 3q+1, 3q+2 (with gptr: the one to three rows gptr[q] ..
+// gptr[q+1]) are relaxed in that order (the reverse when backward) with the current x.  A wavefront takes 2 nodes, 8 lanes per row (lanes 48-63 idle).  A row's
 // lanes walk its block chain together (one distance byte per block, the same address in all 8
 // lanes); lane l keeps the first slot of blocks l, l + 8, ... of each round of 8 and decodes
 // them one slot at a time, gathering x and recording the entries of the node's own three
@@ -470,6 +470,7 @@ __global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t n
 // values already updated: x_a = x_a + (b_a - sum_j A_aj x_j) / A_aa with x current.
 template <int VIB, typename T>
 __global__ __launch_bounds__(kBlock) void k_duvi_gs(int32_t nnodes, const int32_t *__restrict__ nodes,
+                                                    const int32_t *__restrict__ gptr,
                                                     const int32_t *__restrict__ row_blocks,
                                                     const int32_t *__restrict__ row_dist,
                                                     const int32_t *__restrict__ block_col,
@@ -482,11 +483,13 @@ __global__ __launch_bounds__(kBlock) void k_duvi_gs(int32_t nnodes, const int32_
     const int ns = grp / 3, ra = grp - 3 * ns;                 // node slot (2: idle lanes), row of the node
     const int64_t k = ((int64_t)xcd_remap(blockIdx.x, gridDim.x) * kDwWaves + wv) * 2 + ns;
     if (k - ns >= nnodes) return;                              // the whole wave leaves together
-    const bool act = ns < 2 && k < nnodes;
-    const int q = act ? nodes[k] : 0;
+    const bool valid = ns < 2 && k < nnodes;
+    const int q = valid ? nodes[k] : 0;
+    const int r0 = gptr ? gptr[q] : 3 * q, nr = gptr ? (valid ? gptr[q + 1] - r0 : 0) : 3;
+    const bool act = valid && ra < nr;
     T acc = 0, own[3] = {0, 0, 0};
     if (act) {
-        const int row = 3 * q + ra;
+        const int row = r0 + ra;
         const int rb1 = row_blocks[row + 1];
         int64_t d = row_dist[row];
         for (int bj = row_blocks[row]; bj < rb1; bj += kDwLanes) {
@@ -503,8 +506,8 @@ __global__ __launch_bounds__(kBlock) void k_duvi_gs(int32_t nnodes, const int32_
             for (;;) {
                 const T a = duvi_value<VIB, T>(vi, values, dd);
                 acc += a * x[col];
-                const int oc = col - 3 * q;
-                if (oc >= 0 && oc < 3) own[oc] = a;
+                const int oc = col - r0;
+                if (oc >= 0 && oc < nr) own[oc] = a;
                 if (++dd == dend) break;
                 col += dist[dd] + 1;
             }
@@ -527,19 +530,21 @@ __global__ __launch_bounds__(kBlock) void k_duvi_gs(int32_t nnodes, const int32_
     }
     if (!act || ra != 0 || l != 0) return;
     T xo[3], xn[3];
-    const size_t i0 = (size_t)3 * q;
+    const size_t i0 = (size_t)r0;
 #pragma unroll
-    for (int a = 0; a < 3; ++a) { xo[a] = x[i0 + a]; xn[a] = xo[a]; }
+    for (int a = 0; a < 3; ++a) { xo[a] = a < nr ? x[i0 + a] : T(0); xn[a] = xo[a]; }
 #pragma unroll
     for (int t = 0; t < 3; ++t) {
         const int a = backward ? 2 - t : t;
+        if (a >= nr) continue;
         T r = b[i0 + a] - S[a];
 #pragma unroll
         for (int c = 0; c < 3; ++c) r -= A[a][c] * (xn[c] - xo[c]);   // zero for rows not yet relaxed
         xn[a] = xo[a] + r / A[a][a];
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) x[i0 + a] = xn[a];
+    for (int a = 0; a < 3; ++a)
+        if (a < nr) x[i0 + a] = xn[a];
 }
 
 // the wave form for every matrix (timing comparisons of the two forms)
@@ -590,11 +595,11 @@ void launch_duvi(const mgx_duvi &A, const T *values, int epi, const T *x, const 
 
 template <typename T>
 void launch_duvi_gs(const mgx_duvi &A, const T *values, int32_t nnodes, const int32_t *nodes, const T *b, T *x,
-                    bool backward, const int32_t *done, hipStream_t s)
+                    bool backward, const int32_t *done, hipStream_t s, const int32_t *gptr)
 {
     if (nnodes <= 0) return;
     const dim3 g(grid_rows((nnodes + 1) / 2, kDwWaves)), bl(kBlock);
-#define MGX_DG(V) hipLaunchKernelGGL((k_duvi_gs<V, T>), g, bl, 0, s, nnodes, nodes, A.row_blocks, A.row_dist, \
+#define MGX_DG(V) hipLaunchKernelGGL((k_duvi_gs<V, T>), g, bl, 0, s, nnodes, nodes, gptr, A.row_blocks, A.row_dist, \
                                      A.block_col, A.dist, A.vi, values, b, x, backward ? 1 : 0, done)
     switch (A.vi_bytes) {
     case 1: MGX_DG(1); break;
@@ -611,9 +616,9 @@ template void launch_duvi<double>(const mgx_duvi &, const double *, int, const d
 template void launch_duvi<float>(const mgx_duvi &, const float *, int, const float *, const float *, const float *,
                                  float *, int32_t *, double, const int32_t *, hipStream_t);
 template void launch_duvi_gs<double>(const mgx_duvi &, const double *, int32_t, const int32_t *, const double *,
-                                     double *, bool, const int32_t *, hipStream_t);
+                                     double *, bool, const int32_t *, hipStream_t, const int32_t *);
 template void launch_duvi_gs<float>(const mgx_duvi &, const float *, int32_t, const int32_t *, const float *, float *,
-                                    bool, const int32_t *, hipStream_t);
+                                    bool, const int32_t *, hipStream_t, const int32_t *);
 
 void launch_duvi_spmv(const mgx_duvi &A, const double *x, double *y, bool clear, const int32_t *done, hipStream_t s)
 {

@@ -242,6 +242,48 @@ int block_coloring(int32_t nblocks, int bs, const int32_t *rp, const int32_t *ci
     return ncol;
 }
 
+// Node groups for Gauss-Seidel when a level's nodes carry different numbers of free dofs (the
+// plate: w is constrained on its support lines): a group is a maximal run of at most 3
+// consecutive rows with identical column lists (the free dofs of one node couple to the same
+// neighbours).  gptr[g] .. gptr[g+1]-1 are the rows of group g.
+int32_t row_groups(int32_t n, const int32_t *rp, const int32_t *ci, std::vector<int32_t> &gptr)
+{
+    gptr.clear();
+    for (int32_t r = 0; r < n;) {
+        gptr.push_back(r);
+        const int32_t len = rp[r + 1] - rp[r];
+        int32_t e = r + 1;
+        while (e < n && e - r < 3 && rp[e + 1] - rp[e] == len && std::equal(ci + rp[e], ci + rp[e + 1], ci + rp[r]))
+            ++e;
+        r = e;
+    }
+    gptr.push_back(n);
+    return (int32_t)gptr.size() - 1;
+}
+
+// block_coloring over the groups (greedy, natural order)
+int group_coloring(int32_t ngroups, const int32_t *gptr, const int32_t *rp, const int32_t *ci,
+                   std::vector<int32_t> &color)
+{
+    std::vector<int32_t> group_of(gptr[ngroups]);
+    for (int32_t g = 0; g < ngroups; ++g) std::fill(group_of.begin() + gptr[g], group_of.begin() + gptr[g + 1], g);
+    color.assign(ngroups, -1);
+    int ncol = 0;
+    for (int32_t k = 0; k < ngroups; ++k) {
+        uint64_t used[4] = {0, 0, 0, 0};
+        for (int32_t r = gptr[k]; r < gptr[k + 1]; ++r)
+            for (int32_t q = rp[r]; q < rp[r + 1]; ++q) {
+                const int32_t nb = group_of[ci[q]];
+                if (nb != k && color[nb] >= 0 && color[nb] < 256) used[color[nb] >> 6] |= 1ULL << (color[nb] & 63);
+            }
+        int c = 0;
+        while (c < 256 && ((used[c >> 6] >> (c & 63)) & 1)) ++c;
+        color[k] = c;
+        ncol = std::max(ncol, c + 1);
+    }
+    return ncol;
+}
+
 // ------------------------------------------------------------------ coarse inverse
 // A (n x n, row-major, SPD) -> A^-1 by Cholesky; columns of the inverse solved in parallel.
 bool dense_spd_inverse(int32_t n, std::vector<double> &A)

@@ -68,10 +68,14 @@ void launch_jacobi_f32(int32_t nrows, const float *pre, const int32_t *rp, const
                        const float *b, const float *x, float *y, int lanes, const int32_t *done, hipStream_t s);
 void launch_jacobi_initial(int32_t n, const double *pre, const double *b, double *x, const int32_t *done, hipStream_t s);
 void launch_jacobi_initial_f32(int32_t n, const float *pre, const float *b, float *x, const int32_t *done, hipStream_t s);
+// one colour of a multicolour Gauss-Seidel sweep over nodes: node q = rows 3q .. 3q+2, or, with gptr,
+// rows gptr[q] .. gptr[q+1]-1 (1 to 3 rows: the plate's supported nodes have 2 free dofs)
 void launch_gs_color(int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const double *v,
-                     const double *b, double *x, bool backward, const int32_t *done, hipStream_t s);
+                     const double *b, double *x, bool backward, const int32_t *done, hipStream_t s,
+                     const int32_t *gptr = nullptr);
 void launch_gs_color_f32(int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const float *v,
-                         const float *b, float *x, bool backward, const int32_t *done, hipStream_t s);
+                         const float *b, float *x, bool backward, const int32_t *done, hipStream_t s,
+                         const int32_t *gptr = nullptr);
 void launch_dot(int32_t n, const double *a, const double *b, double *result, double *partials,
                 uint32_t *counter, hipStream_t s);
 int dot_partial_count(int64_t n);   // partial sums launch_dot writes (its first kernel's grid)
@@ -87,7 +91,7 @@ void set_duvi_wave_only(bool on);   // timing comparisons: the wave form for lon
 // one colour of a multicolour Gauss-Seidel sweep on the DuVi layout (nodes of 3 rows)
 template <typename T>
 void launch_duvi_gs(const mgx_duvi &A, const T *values, int32_t nnodes, const int32_t *nodes, const T *b, T *x,
-                    bool backward, const int32_t *done, hipStream_t s);
+                    bool backward, const int32_t *done, hipStream_t s, const int32_t *gptr = nullptr);
 void launch_simp_gradient(int32_t nx, int32_t ny, int32_t nz, const double *Ke0, const double *rho, double penal,
                           double Emin, double E0, const double *u, double *grad, hipStream_t s);
 
@@ -157,7 +161,8 @@ void launch_gs_node(int K, int32_t nnodes, const int32_t *nodes, const int32_t *
                     const T *B, T *X, bool backward, const int32_t *done, hipStream_t s);
 template <typename T>
 void launch_gs_block(int K, int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const T *v,
-                     const T *B, T *X, bool backward, const int32_t *done, hipStream_t s);
+                     const T *B, T *X, bool backward, const int32_t *done, hipStream_t s,
+                     const int32_t *gptr = nullptr);
 template <typename T>
 void launch_bnorms(int K, int32_t n, const T *B, const T *R, BCtl *ctl, double *partials, hipStream_t s);
 template <typename T>
@@ -269,7 +274,16 @@ struct HostCsr {
 void transfer_reference(int nx, int ny, int nz, HostCsr &R, HostCsr &P);
 void transfer_trilinear(int nx, int ny, int nz, HostCsr &R, HostCsr &P);
 void csr_transpose(const HostCsr &A, HostCsr &T);
+// the Kirchhoff plate workload (mgx_plate.cpp)
+void plate_element(double a, double b, double E, double nu, double t, double Ke[144]);
+int32_t plate_free_dofs(int nx, int ny, double Lx, double Ly, std::vector<int32_t> *dofs);
+void plate_system(int nx, int ny, double Lx, double Ly, double t, double E, double nu, double q, HostCsr &K,
+                  std::vector<double> *rhs);
+void plate_transfer(int nx, int ny, double Lx, double Ly, HostCsr &R, HostCsr &P);
 int block_coloring(int32_t nblocks, int bs, const int32_t *rp, const int32_t *ci, std::vector<int32_t> &color);
+int32_t row_groups(int32_t n, const int32_t *rp, const int32_t *ci, std::vector<int32_t> &gptr);
+int group_coloring(int32_t ngroups, const int32_t *gptr, const int32_t *rp, const int32_t *ci,
+                   std::vector<int32_t> &color);
 // dense SPD inverse via Cholesky; returns false if not SPD
 bool dense_spd_inverse(int32_t n, std::vector<double> &A);
 int duvi_encode(int32_t nrows, const int32_t *rp, const int32_t *ci, const double *v, double tol, int mode,

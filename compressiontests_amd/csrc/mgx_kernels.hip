@@ -681,15 +681,18 @@ void launch_axpy_f32_into_f64(int64_t n, const float *a, double *x, const int32_
 }
 
 // ------------------------------------------------------------------ multicolour Gauss-Seidel
-// A block of 192 threads relaxes 4 nodes (12 rows): each row is reduced by 16 lanes into
-// s_a = sum over columns outside the node and d_ab = the node's own 3x3 block, with all loads of
-// the row issued up front; the per-row results go through LDS and one thread per node relaxes
-// the three dofs in order (reverse order when backward).  Nodes of one colour are uncoupled, so
-// the x values read here are not written by this launch except the node's own.
+// A block of 192 threads relaxes 4 nodes (12 row slots): each row is reduced by 16 lanes into
+// s_a = sum over columns outside the node and d_ab = the node's own block, with all loads of the
+// row issued up front; the per-row results go through LDS and one thread per node relaxes its
+// dofs in order (reverse order when backward).  A node is 3 rows, or 1 to 3 with a group pointer
+// (nodes whose free dofs differ in number; slots past a node's rows stay idle).  Nodes of one
+// colour are uncoupled, so the x values read here are not written by this launch except the
+// node's own.
 constexpr int kGsBlock = 192, kGsNodes = 4;
 
 template <typename T>
 __global__ __launch_bounds__(kGsBlock) void k_gs_color(int32_t nnodes, const int32_t *__restrict__ nodes,
+                                                       const int32_t *__restrict__ gptr,
                                                        const int32_t *__restrict__ rp, const int32_t *__restrict__ ci,
                                                        const T *__restrict__ v, const T *__restrict__ b, T *__restrict__ x,
                                                        int backward, const int32_t *__restrict__ done)
@@ -703,9 +706,13 @@ __global__ __launch_bounds__(kGsBlock) void k_gs_color(int32_t nnodes, const int
     const int lane = threadIdx.x % G;
     const int grp = base + ln;
     T sa = 0, d0 = 0, d1 = 0, d2 = 0;
-    int r0 = 0;
+    int r0 = 0, gs = 3;
     if (grp < nnodes) {
-        r0 = 3 * nodes[grp];
+        const int g = nodes[grp];
+        r0 = gptr ? gptr[g] : 3 * g;
+        if (gptr) gs = gptr[g + 1] - r0;
+    }
+    if (grp < nnodes && a < gs) {
         const int s = rp[r0 + a], e = rp[r0 + a + 1];
         for (int k = s + lane; k < e; k += U * G) {
             int c[U];
@@ -719,13 +726,13 @@ __global__ __launch_bounds__(kGsBlock) void k_gs_color(int32_t nnodes, const int
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const unsigned off = (unsigned)(c[u] - r0);
-                xv[u] = (c[u] >= 0 && off >= 3u) ? x[c[u]] : T(0);
+                xv[u] = (c[u] >= 0 && off >= (unsigned)gs) ? x[c[u]] : T(0);
             }
 #pragma unroll
             for (int u = 0; u < U; ++u) {
                 const unsigned off = (unsigned)(c[u] - r0);
                 if (c[u] < 0) continue;
-                if (off < 3u) { if (off == 0) d0 += w[u]; else if (off == 1) d1 += w[u]; else d2 += w[u]; }
+                if (off < (unsigned)gs) { if (off == 0) d0 += w[u]; else if (off == 1) d1 += w[u]; else d2 += w[u]; }
                 else sa += w[u] * xv[u];
             }
         }
@@ -740,39 +747,50 @@ __global__ __launch_bounds__(kGsBlock) void k_gs_color(int32_t nnodes, const int
     if (lane == 0) { sh[ln][a][0] = sa; sh[ln][a][1] = d0; sh[ln][a][2] = d1; sh[ln][a][3] = d2; }
     __syncthreads();
     if (threadIdx.x < kGsNodes && base + (int)threadIdx.x < nnodes) {
+        // the node's rows in order (reverse when backward): x_a = (b_a - s_a - sum_{c != a} d_ac x_c) / d_aa
         const int m = threadIdx.x;
-        const int q0 = 3 * nodes[base + m];
+        const int g = nodes[base + m];
+        const int q0 = gptr ? gptr[g] : 3 * g, n = gptr ? gptr[g + 1] - q0 : 3;
         const T(*R)[4] = sh[m];
-        T x0 = x[q0], x1 = x[q0 + 1], x2 = x[q0 + 2];
-        const T b0 = b[q0] - R[0][0], b1 = b[q0 + 1] - R[1][0], b2 = b[q0 + 2] - R[2][0];
-        if (!backward) {
-            x0 = (b0 - R[0][2] * x1 - R[0][3] * x2) / R[0][1];
-            x1 = (b1 - R[1][1] * x0 - R[1][3] * x2) / R[1][2];
-            x2 = (b2 - R[2][1] * x0 - R[2][2] * x1) / R[2][3];
-        } else {
-            x2 = (b2 - R[2][1] * x0 - R[2][2] * x1) / R[2][3];
-            x1 = (b1 - R[1][1] * x0 - R[1][3] * x2) / R[1][2];
-            x0 = (b0 - R[0][2] * x1 - R[0][3] * x2) / R[0][1];
+        T xv[3], bv[3];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            xv[c] = c < n ? x[q0 + c] : T(0);
+            bv[c] = c < n ? b[q0 + c] - R[c][0] : T(0);
         }
-        x[q0] = x0; x[q0 + 1] = x1; x[q0 + 2] = x2;
+#pragma unroll
+        for (int t = 0; t < 3; ++t) {
+            const int r = backward ? 2 - t : t;
+            if (r >= n) continue;
+            T acc = bv[r];
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                if (c != r && c < n) acc -= R[r][c + 1] * xv[c];
+            xv[r] = acc / R[r][r + 1];
+        }
+#pragma unroll
+        for (int c = 0; c < 3; ++c)
+            if (c < n) x[q0 + c] = xv[c];
     }
 }
 
 void launch_gs_color(int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const double *v,
-                     const double *b, double *x, bool backward, const int32_t *done, hipStream_t s)
+                     const double *b, double *x, bool backward, const int32_t *done, hipStream_t s,
+                     const int32_t *gptr)
 {
     if (nnodes <= 0) return;
     const int nb = grid_rows(nnodes, kGsNodes);
-    hipLaunchKernelGGL(k_gs_color<double>, dim3(nb), dim3(kGsBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
+    hipLaunchKernelGGL(k_gs_color<double>, dim3(nb), dim3(kGsBlock), 0, s, nnodes, nodes, gptr, rp, ci, v, b, x,
                        backward ? 1 : 0, done);
     MGX_HIP(hipGetLastError());
 }
 void launch_gs_color_f32(int32_t nnodes, const int32_t *nodes, const int32_t *rp, const int32_t *ci, const float *v,
-                         const float *b, float *x, bool backward, const int32_t *done, hipStream_t s)
+                         const float *b, float *x, bool backward, const int32_t *done, hipStream_t s,
+                         const int32_t *gptr)
 {
     if (nnodes <= 0) return;
     const int nb = grid_rows(nnodes, kGsNodes);
-    hipLaunchKernelGGL(k_gs_color<float>, dim3(nb), dim3(kGsBlock), 0, s, nnodes, nodes, rp, ci, v, b, x,
+    hipLaunchKernelGGL(k_gs_color<float>, dim3(nb), dim3(kGsBlock), 0, s, nnodes, nodes, gptr, rp, ci, v, b, x,
                        backward ? 1 : 0, done);
     MGX_HIP(hipGetLastError());
 }

@@ -204,16 +204,63 @@ def hierarchy_info(path):
     return s.version, s.nlevels, bool(s.has_grid), bool(s.has_duvi), s.payload_bytes
 
 
-def read_hierarchy_csr(path, level, which):
-    """CSR of A (which 0), R (1) or P (2) of a level."""
-    m = L.CsrOwned()
-    L.call("mgx_hierarchy_read_csr", os.fsencode(path), level, which, C.byref(m))
+def _owned_to_csr(m):
     try:
         return CsrMatrix(np.ctypeslib.as_array(m.rowptr, (m.nrows + 1,)).copy(),
                          np.ctypeslib.as_array(m.col, (max(m.nnz, 1),))[:m.nnz].copy(),
                          np.ctypeslib.as_array(m.val, (max(m.nnz, 1),))[:m.nnz].copy(), m.nrows, m.ncols)
     finally:
         L.lib().mgx_csr_owned_free(C.byref(m))
+
+
+def read_hierarchy_csr(path, level, which):
+    """CSR of A (which 0), R (1) or P (2) of a level."""
+    m = L.CsrOwned()
+    L.call("mgx_hierarchy_read_csr", os.fsencode(path), level, which, C.byref(m))
+    return _owned_to_csr(m)
+
+
+# ------------------------------------------------------------------ Kirchhoff plate (FemPlate.cs)
+def plate_free_dofs(nx, ny, Lx, Ly):
+    """FindFreeDofs (FemPlate.cs:72-101): all-dof index 3 node + d of every free dof."""
+    n = C.c_int32()
+    L.call("mgx_plate_free_dofs", nx, ny, C.c_double(Lx), C.c_double(Ly), C.byref(n), None)
+    d = np.empty(n.value, np.int32)
+    L.call("mgx_plate_free_dofs", nx, ny, C.c_double(Lx), C.c_double(Ly), C.byref(n), d.ctypes.data_as(C.c_void_p))
+    return d
+
+
+def plate_system_host(nx, ny, Lx, Ly, t, E, nu, q):
+    """(K_ff as host CSR, load vector) of the plate, without a device."""
+    m = L.CsrOwned()
+    nfree = len(plate_free_dofs(nx, ny, Lx, Ly))
+    b = np.empty(nfree, np.float64)
+    L.call("mgx_plate_system_host", nx, ny, C.c_double(Lx), C.c_double(Ly), C.c_double(t), C.c_double(E),
+           C.c_double(nu), C.c_double(q), C.byref(m), b.ctypes.data_as(C.c_void_p))
+    return _owned_to_csr(m), b
+
+
+def plate_transfer_host(nx, ny, Lx, Ly):
+    r, p = L.CsrOwned(), L.CsrOwned()
+    L.call("mgx_plate_transfer_host", nx, ny, C.c_double(Lx), C.c_double(Ly), C.byref(r), C.byref(p))
+    return _owned_to_csr(r), _owned_to_csr(p)
+
+
+def plate_matrix(nx, ny, Lx, Ly, t, E, nu, q):
+    """(K_ff on the device, load vector on the host)."""
+    h = C.c_void_p()
+    nfree = len(plate_free_dofs(nx, ny, Lx, Ly))
+    b = np.empty(nfree, np.float64)
+    L.call("mgx_matrix_plate", nx, ny, C.c_double(Lx), C.c_double(Ly), C.c_double(t), C.c_double(E), C.c_double(nu),
+           C.c_double(q), C.byref(h), b.ctypes.data_as(C.c_void_p))
+    return DeviceCsr(handle=h.value), b
+
+
+def transfer_plate(nx, ny, Lx, Ly):
+    """(R, P) between the plate and its coarser model, on the device."""
+    r, p = C.c_void_p(), C.c_void_p()
+    L.call("mgx_matrix_transfer_plate", nx, ny, C.c_double(Lx), C.c_double(Ly), C.byref(r), C.byref(p))
+    return DeviceCsr(handle=r.value), DeviceCsr(handle=p.value)
 
 
 def read_hierarchy_duvi(path, level, which):

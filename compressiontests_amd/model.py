@@ -1,14 +1,19 @@
-"""The FE cantilever model of the reference (IGeometricMultigridModel + FemCantileverBase +
-FemCantilever3D), producing its matrices with the library's device-side generators.
+"""The FE models of the reference (IGeometricMultigridModel with FemCantileverBase +
+FemCantilever3D, and FemPlate), producing their matrices with the library's generators.
 
-Mesh conventions (the reference takes them from the unvendored UniformCartesianMesh3D):
+Cantilever mesh conventions (the reference takes them from the unvendored UniformCartesianMesh3D):
 node id = ix + (nx+1)(iy + (ny+1) iz); element e = ex + nx (ey + ny ez); element corners in
 VTK hexahedron order; 3 dofs (ux, uy, uz) per node; nodes with ix == 0 clamped
 (FemCantileverBase.cs:137-158).
+
+Plate conventions (UniformCartesianMesh2D): node id = ix + (nx+1) iy; 3 dofs (w, dw/dy, dw/dx)
+per node, node-major; w fixed on the lines x = 0, x = Lx/2, y = 0, y = Ly/2, y = Ly
+(FemPlate.cs:126-188).
 """
 import numpy as np
 
-from .device import DeviceArray, cantilever_matrix, rng_normal, transfer_reference
+from .device import (DeviceArray, cantilever_matrix, plate_free_dofs, plate_matrix, rng_normal,
+                     transfer_plate, transfer_reference)
 from . import kernels
 
 
@@ -117,8 +122,72 @@ class FemCantilever3D:
                                self.rhs_kind, self.TipLoad)
 
 
+class FemPlate:
+    """FemPlate (FemPlate.cs:20-189): a Kirchhoff plate of ACM rectangles (12-dof element of
+    PlateElementMatrixProvider) under the uniform load DistributedLoad, assembled node-major over
+    the free dofs.  lengthPerAxis = (Lx, Ly, thickness).
+
+    NumDofsFree is the count of the free dofs the supports leave; FemPlate.cs:118-122 states it
+    as NumDofsAll - 2 (ny+1) - 3 (nx+1) + 6, the same number whenever both element counts are
+    even (the lines x = Lx/2 and y = Ly/2 then hold nodes)."""
+
+    NumDofsPerNode = 3
+
+    def __init__(self, numElementsPerAxis, lengthPerAxis, E=1.0, v=0.3, q=0.0):
+        self.NumElements = tuple(int(n) for n in numElementsPerAxis)
+        if len(self.NumElements) != 2:
+            raise ValueError("FemPlate needs 2 element counts")
+        if len(lengthPerAxis) != 3:
+            raise ValueError("FemPlate needs (length x, length y, thickness)")
+        self.Lengths = tuple(float(v_) for v_ in lengthPerAxis[:2])
+        self.Thickness = float(lengthPerAxis[2])
+        self.ElasticityModulus = float(E)
+        self.PoissonRatio = float(v)
+        self.DistributedLoad = float(q)
+        nx, ny = self.NumElements
+        self.NumNodes = (nx + 1, ny + 1)
+        self.NumDofsAll = 3 * (nx + 1) * (ny + 1)
+        self._free = plate_free_dofs(nx, ny, *self.Lengths)
+        self.NumDofsFree = len(self._free)
+        self._K = None
+        self._b = None
+
+    def _system(self):
+        if self._K is None:
+            nx, ny = self.NumElements
+            self._K, self._b = plate_matrix(nx, ny, *self.Lengths, self.Thickness, self.ElasticityModulus,
+                                            self.PoissonRatio, self.DistributedLoad)
+        return self._K, self._b
+
+    def device_matrix(self):
+        """K_ff on the device (library-owned)."""
+        return self._system()[0]
+
+    def rhs(self):
+        """The load vector of PlateElementVectorAssembler (FemPlate.cs:65-67) over the free dofs."""
+        return self._system()[1].copy()
+
+    def CreateLinearSystem(self):
+        """(A, b): K_ff as host CSR and the load vector (FemPlate.cs:40-71)."""
+        K, b = self._system()
+        return K.to_host(), b.copy()
+
+    def FindFreeDofs(self):
+        """FemPlate.cs:72-101: the all-dof index 3 node + d of every free dof, ascending."""
+        return self._free.copy()
+
+    def CreateCoarserModel(self):
+        """FemPlate.cs:103-108: halve every element count > 1, same lengths and material."""
+        c = tuple(n >> 1 if n > 1 else n for n in self.NumElements)
+        return FemPlate(c, self.Lengths + (self.Thickness,), self.ElasticityModulus, self.PoissonRatio,
+                        self.DistributedLoad)
+
+
 def CreateCoarserModelAndSmoothenerMatrices(model):
     """IGeometricMultigridModel.cs:60-65: (coarser model, R, P) with R, P on the device."""
     coarser = model.CreateCoarserModel()
-    R, P = transfer_reference(*model.NumElements)
+    if isinstance(model, FemPlate):
+        R, P = transfer_plate(*model.NumElements, *model.Lengths)
+    else:
+        R, P = transfer_reference(*model.NumElements)
     return coarser, R, P

@@ -151,6 +151,16 @@ class _Handle:
         L.call("mgx_solver_gs_order", C.c_void_p(self.h), level, o.ctypes.data_as(C.c_void_p))
         return o
 
+    def level_groups(self, level):
+        """The level's Gauss-Seidel row groups: None for row triples, else the group starts."""
+        v = C.c_int32()
+        L.call("mgx_solver_level_groups", C.c_void_p(self.h), level, C.byref(v), None)
+        if v.value == 0:
+            return None
+        g = np.empty(v.value + 1, np.int32)
+        L.call("mgx_solver_level_groups", C.c_void_p(self.h), level, C.byref(v), g.ctypes.data_as(C.c_void_p))
+        return g
+
     def level_colors(self, level):
         v = C.c_int32()
         L.call("mgx_solver_level_colors", C.c_void_p(self.h), level, C.byref(v))
@@ -332,11 +342,14 @@ class HipMgPcgSolver:
         self.opt = opt
         self.nrhs = nrhs
         h = _Handle(opt)
-        nx, ny, nz = model.NumElements
         self.hierarchy = []             # (A, R, P) device matrices per level, finest first
         if isinstance(hierarchy, (list, tuple)):
             self.hierarchy = list(hierarchy)      # shared with another solver of the same model
         elif hierarchy == "galerkin":
+            if len(model.NumElements) != 3:
+                raise L.MgxNotImplemented("the Galerkin hierarchy is built for the 3D cantilever; "
+                                          "use hierarchy='reference' for the plate")
+            nx, ny, nz = model.NumElements
             Ks = galerkin_hierarchy(nx, ny, nz, levels, model.ElasticityModulus, model.PoissonRatio,
                                     model.ElementDensities)
             model._K = Ks[0]
@@ -401,9 +414,16 @@ class HipMgPcgSolver:
         return self._h.strip_window()
 
     def GsOrder(self, level=0):
-        """Node order of the level's forward Gauss-Seidel sweep as the solver runs it."""
+        """Node order of the level's forward Gauss-Seidel sweep as the solver runs it (row-group
+        numbers where the level's nodes are not all row triples: see GsGroups)."""
+        g = self._h.level_groups(level)
         A = self.hierarchy[level][0]
-        return self._h.gs_order(level, A.nrows // 3)
+        return self._h.gs_order(level, A.nrows // 3 if g is None else len(g) - 1)
+
+    def GsGroups(self, level=0):
+        """None when the level's Gauss-Seidel nodes are the row triples, else the starts of its
+        row groups (runs of up to 3 consecutive rows with one column list)."""
+        return self._h.level_groups(level)
 
     def LevelColors(self, level=0):
         """Colours of the level's multicolour Gauss-Seidel (0 without that smoother)."""

@@ -235,6 +235,21 @@ int mgx_matrix_cantilever(int32_t nx, int32_t ny, int32_t nz, double E, double n
 int mgx_matrix_transfer_reference(int32_t nx, int32_t ny, int32_t nz, mgx_matrix *R, mgx_matrix *P);
 /* Exact trilinear interpolation P (fine free x coarse free) and R = P^T. */
 int mgx_matrix_transfer_trilinear(int32_t nx, int32_t ny, int32_t nz, mgx_matrix *R, mgx_matrix *P);
+
+/* The Kirchhoff plate workload (FemPlate.cs): nx*ny rectangular plate elements (Adini-Clough-Melosh,
+ * PlateElementRectangle4Nodes3Dofs.cs:134-222) on Lx*Ly, thickness t; dofs (w, rx, ry) per node, node
+ * by node, w fixed on the lines x = 0, x = Lx/2, y = 0, y = Ly/2, y = Ly (FemPlate.cs:181-196).
+ * Free dofs (FindFreeDofs, FemPlate.cs:72-101): *count of them, and when dofs is not NULL their
+ * all-dof indices 3 node + d (host array of *count entries). */
+int mgx_plate_free_dofs(int32_t nx, int32_t ny, double Lx, double Ly, int32_t *count, int32_t *dofs);
+/* K_ff (sorted columns, element sums in ascending element order as the reference's DOK assembly) on
+ * the device, and, when b is not NULL, the load vector of a uniform load q (host, count entries;
+ * PlateElementVectorAssembler.cs:14-30).  Replaces FemPlate.CreateLinearSystem (FemPlate.cs:41-70). */
+int mgx_matrix_plate(int32_t nx, int32_t ny, double Lx, double Ly, double t, double E, double nu, double q,
+                     mgx_matrix *A, double *b);
+/* Restriction / interpolation between the plate and its coarser model (element counts halved):
+ * IGeometricMultigridModel.cs:76-194 on two axes, free-dof submatrices. */
+int mgx_matrix_transfer_plate(int32_t nx, int32_t ny, double Lx, double Ly, mgx_matrix *R, mgx_matrix *P);
 /* Galerkin hierarchy: K_0 from dens, K_{l+1} = P_l^T K_l P_l for trilinear P_l, computed
  * element by element on the device; levels[l] receives K_l (nlevels entries). */
 int mgx_matrix_galerkin_hierarchy(int32_t nx, int32_t ny, int32_t nz, double E, double nu,
@@ -376,13 +391,21 @@ int mgx_solver_transfer_layout(mgx_solver s, int32_t level, int32_t *layout);
 int mgx_solver_strip_window(mgx_solver s, double *bytes, double *tune_ms);
 /* Colours of the multicolour Gauss-Seidel node colouring of a level (0 without that smoother). */
 int mgx_solver_level_colors(mgx_solver s, int32_t level, int32_t *ncolors);
-/* The node order (n/3 nodes) of the level's forward Gauss-Seidel sweep as the solver runs it: the
- * sweep relaxes node order[0], then order[1], ... (rows 3q, 3q+1, 3q+2 of node q in turn), the
+/* The node order of the level's forward Gauss-Seidel sweep as the solver runs it: the sweep
+ * relaxes node order[0], then order[1], ... (rows 3q, 3q+1, 3q+2 of node q in turn), the
  * backward sweep the exact reverse; nodes that the kernels relax together (one colour) are
  * uncoupled, so the parallel sweep equals this sequential one: colour 0's nodes, then colour
- * 1's, ...  The reference's GPU "Gauss-Seidel" (CsrGeometricMultigrid.cl:207-225) updates in
+ * 1's, ...  A level whose rows do not all come in triples with one shared column list (the
+ * plate's supported nodes keep one or two free dofs) has row groups instead: maximal runs of up
+ * to 3 consecutive rows with identical column lists, numbered in row order, and order[] holds
+ * group numbers (mgx_solver_level_groups gives their count).  n/3 entries for triples, else one
+ * per group.  The reference's GPU "Gauss-Seidel" (CsrGeometricMultigrid.cl:207-225) updates in
  * place with racy reads; this ordering is the documented deviation. */
 int mgx_solver_gs_order(mgx_solver s, int32_t level, int32_t *order);
+/* Row groups of a level's Gauss-Seidel nodes: 0 when the nodes are the row triples, else the
+ * group count and, if gptr is not null, the group starts gptr[0 .. count] (group q is rows
+ * gptr[q] .. gptr[q+1]-1).  No counterpart in the reference, whose kernels take rows one by one. */
+int mgx_solver_level_groups(mgx_solver s, int32_t level, int32_t *count, int32_t *gptr);
 /* Colour pairs (2k, 2k+1) of the level that the batching form's Gauss-Seidel relaxes in one launch
  * (options.gs_pairs): block i of the launch relaxes the pair's nodes on mesh line i, the first
  * colour's, then after a block barrier the second colour's.  A pair qualifies when every edge
@@ -425,6 +448,11 @@ typedef struct {
 } mgx_csr_owned;
 int mgx_hierarchy_read_csr(const char *path, int32_t level, int32_t which, mgx_csr_owned *out);
 void mgx_csr_owned_free(mgx_csr_owned *m);
+/* The plate matrices of mgx_matrix_plate / mgx_matrix_transfer_plate in host arrays (freed by
+ * mgx_csr_owned_free); b (NULL or the free-dof count) receives the load vector. */
+int mgx_plate_system_host(int32_t nx, int32_t ny, double Lx, double Ly, double t, double E, double nu, double q,
+                          mgx_csr_owned *K, double *b);
+int mgx_plate_transfer_host(int32_t nx, int32_t ny, double Lx, double Ly, mgx_csr_owned *R, mgx_csr_owned *P);
 /* DuVi block layout of one level's A / R / P (freed by mgx_duvi_host_free). */
 int mgx_hierarchy_read_duvi(const char *path, int32_t level, int32_t which, mgx_duvi_host *out);
 /* The level's structured grid (3 element counts). */
