@@ -7,7 +7,8 @@ sequential-layout rule, natural alignment), and every [DllImport] must name a fu
 header with the same number and kind of parameters.  The solver classes must expose the
 reference's public API (OpenCLCsrGeometricMultigridSolver.cs:55-104,145,334,106;
 OpenCLDuViGeometricMultigridSolver.cs:77-104; OpenCLCsrPcgSolver.cs:40-45,84,250,47) with the
-OpenCL device and context arguments dropped."""
+OpenCL device and context arguments dropped, and the ISolver adapter must override what
+PcgSolver2 overrides (PcgSolver2.cs:20-155) and keep its Factory."""
 import os
 import re
 import shutil
@@ -175,3 +176,28 @@ def test_solve_returns_the_reference_tuple():
     assert ": IOpenCLGeometricMultigridSolver" in t
     p = open(os.path.join(CS, "HipCsrPcgSolver.cs")).read()
     assert re.search(r"public unsafe \(Vector, IterativeStatistics\) Solve\(Vector\? xInitialGuess\)", p)
+
+
+def test_pcg_solver2_adapter_keeps_the_reference_overrides():
+    """HipPcgSolver2 derives from the base class PcgSolver2 derives from (with the DOK matrix type),
+    overrides the same five members with the same signatures, throws the reference's exception on
+    non-convergence, keeps the nested Factory's members, and drives the library through Mgx."""
+    t = open(os.path.join(CS, "HipPcgSolver2.cs")).read()
+    assert re.search(r"public class HipPcgSolver2 : SingleSubdomainSolverBase<DokRowMajor>", t)
+    assert "namespace MGroup.Solvers.Iterative" in t
+    for sig in (r"public override void HandleMatrixWillBeSet\(\)", r"public override void Initialize\(\)",
+                r"public override void PreventFromOverwrittingSystemMatrices\(\)",
+                r"public override (?:unsafe )?void Solve\(\)",
+                r"protected override Matrix InverseSystemMatrixTimesOtherMatrix\(IReadOnlyMatrix otherMatrix\)"):
+        assert re.search(sig, t), sig
+    assert "throw new IterativeSolverNotConvergedException(" in t
+    f = t[t.index("public class Factory"):]
+    for member in ("public IDofOrderer DofOrderer", "public bool MatrixPatternWillNotBeModified",
+                   "public HipPcgSolver2 BuildSolver(GlobalAlgebraicModel<DokRowMajor> model)",
+                   "public GlobalAlgebraicModel<DokRowMajor> BuildAlgebraicModel(IModel model)"):
+        assert member in f, member
+    assert "new DokRowMajorMatrixAssembler(true)" in f
+    for call in ("Mgx.mgx_solver_create(", "Mgx.mgx_solver_set_rhs(", "Mgx.mgx_solver_initialize(",
+                 "Mgx.mgx_solver_solve(", "Mgx.mgx_solver_release(", "Mgx.SetLevel("):
+        assert call in t, call
+    assert "o.method = Mgx.METHOD_PCG_JACOBI;" in t and "o.stop = Mgx.STOP_REL_L2;" in t
