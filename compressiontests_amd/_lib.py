@@ -156,6 +156,8 @@ _SIGS = {
     "mgx_plate_free_dofs": (C.c_int, [_I32, _I32, _D, _D, C.POINTER(C.c_int32), _P]),
     "mgx_matrix_plate": (C.c_int, [_I32, _I32, _D, _D, _D, _D, _D, _D, C.POINTER(_P), _P]),
     "mgx_matrix_transfer_plate": (C.c_int, [_I32, _I32, _D, _D, C.POINTER(_P), C.POINTER(_P)]),
+    "mgx_matrix_cantilever2d": (C.c_int, [_I32, _I32, _D, _D, _P, C.POINTER(_P)]),
+    "mgx_matrix_transfer_cantilever2d": (C.c_int, [_I32, _I32, C.POINTER(_P), C.POINTER(_P)]),
     "mgx_matrix_galerkin_hierarchy": (C.c_int, [_I32, _I32, _I32, _D, _D, _P, _I32, C.POINTER(_P)]),
     "mgx_matrix_from_host": (C.c_int, [_I32, _I32, _P, _P, _P, C.POINTER(_P)]),
     "mgx_matrix_view": (C.c_int, [_P, C.POINTER(CsrView)]),
@@ -197,6 +199,8 @@ _SIGS = {
     "mgx_csr_owned_free": (None, [C.POINTER(CsrOwned)]),
     "mgx_plate_system_host": (C.c_int, [_I32, _I32, _D, _D, _D, _D, _D, _D, C.POINTER(CsrOwned), _P]),
     "mgx_plate_transfer_host": (C.c_int, [_I32, _I32, _D, _D, C.POINTER(CsrOwned), C.POINTER(CsrOwned)]),
+    "mgx_cantilever2d_system_host": (C.c_int, [_I32, _I32, _D, _D, _P, C.POINTER(CsrOwned)]),
+    "mgx_cantilever2d_transfer_host": (C.c_int, [_I32, _I32, C.POINTER(CsrOwned), C.POINTER(CsrOwned)]),
     "mgx_hierarchy_read_duvi": (C.c_int, [C.c_char_p, _I32, _I32, C.POINTER(DuviHost)]),
     "mgx_hierarchy_read_grid": (C.c_int, [C.c_char_p, _I32, _P]),
     "mgx_hierarchy_save": (C.c_int, [C.c_char_p, _I32, _P, _P, _P, _I32, _D]),

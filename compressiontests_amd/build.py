@@ -47,7 +47,7 @@ def build_lib(verbose=False):
         obj = os.path.join(BUILD, src + ".o")
         jobs.append((obj, [hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]))
     # host-only code (setup, file I/O): no device code, no FMA contraction (bit-exact weights)
-    for src in ("mgx_host.cpp", "mgx_io.cpp", "mgx_plate.cpp"):
+    for src in ("mgx_host.cpp", "mgx_io.cpp", "mgx_fem2d.cpp"):
         obj = os.path.join(BUILD, src + ".o")
         jobs.append((obj, [hipcc, *HOST_FLAGS, "-x", "c++", "-D__HIP_PLATFORM_AMD__", "-c",
                            os.path.join(CSRC, src), "-o", obj]))

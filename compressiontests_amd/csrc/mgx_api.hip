@@ -2357,6 +2357,52 @@ int mgx_matrix_transfer_plate(int32_t nx, int32_t ny, double Lx, double Ly, mgx_
     });
 }
 
+int mgx_matrix_cantilever2d(int32_t nx, int32_t ny, double E, double nu, const double *dens, mgx_matrix *A)
+{
+    return guard([&] {
+        MGX_REQUIRE(A && nx > 0 && ny > 0 && E > 0 && nu > -1 && nu < 1, MGX_ERR_INVALID_ARGUMENT, "invalid cantilever");
+        HostCsr K;
+        cant2d_system(nx, ny, E, nu, dens, K);
+        auto m = std::make_unique<mgx_matrix_s>();
+        upload_host_csr(m->A, K, nullptr);
+        *A = m.release();
+    });
+}
+
+int mgx_cantilever2d_system_host(int32_t nx, int32_t ny, double E, double nu, const double *dens, mgx_csr_owned *K)
+{
+    return guard([&] {
+        MGX_REQUIRE(K && nx > 0 && ny > 0 && E > 0 && nu > -1 && nu < 1, MGX_ERR_INVALID_ARGUMENT, "invalid cantilever");
+        HostCsr h;
+        cant2d_system(nx, ny, E, nu, dens, h);
+        to_owned(h, K);
+    });
+}
+
+int mgx_cantilever2d_transfer_host(int32_t nx, int32_t ny, mgx_csr_owned *R, mgx_csr_owned *P)
+{
+    return guard([&] {
+        MGX_REQUIRE(R && P && nx > 0 && ny > 0, MGX_ERR_INVALID_ARGUMENT, "invalid cantilever");
+        HostCsr hr, hp;
+        cant2d_transfer(nx, ny, hr, hp);
+        to_owned(hr, R);
+        to_owned(hp, P);
+    });
+}
+
+int mgx_matrix_transfer_cantilever2d(int32_t nx, int32_t ny, mgx_matrix *R, mgx_matrix *P)
+{
+    return guard([&] {
+        MGX_REQUIRE(R && P && nx > 0 && ny > 0, MGX_ERR_INVALID_ARGUMENT, "invalid cantilever");
+        HostCsr hr, hp;
+        cant2d_transfer(nx, ny, hr, hp);
+        auto r = std::make_unique<mgx_matrix_s>(), p = std::make_unique<mgx_matrix_s>();
+        upload_host_csr(r->A, hr, nullptr);
+        upload_host_csr(p->A, hp, nullptr);
+        *R = r.release(); *P = p.release();
+    });
+}
+
 int mgx_matrix_transfer_trilinear(int32_t nx, int32_t ny, int32_t nz, mgx_matrix *R, mgx_matrix *P)
 {
     return guard([&] {

@@ -274,12 +274,15 @@ struct HostCsr {
 void transfer_reference(int nx, int ny, int nz, HostCsr &R, HostCsr &P);
 void transfer_trilinear(int nx, int ny, int nz, HostCsr &R, HostCsr &P);
 void csr_transpose(const HostCsr &A, HostCsr &T);
-// the Kirchhoff plate workload (mgx_plate.cpp)
+// the 2D workloads (mgx_fem2d.cpp): the Kirchhoff plate and the plane-stress cantilever
 void plate_element(double a, double b, double E, double nu, double t, double Ke[144]);
 int32_t plate_free_dofs(int nx, int ny, double Lx, double Ly, std::vector<int32_t> *dofs);
 void plate_system(int nx, int ny, double Lx, double Ly, double t, double E, double nu, double q, HostCsr &K,
                   std::vector<double> *rhs);
 void plate_transfer(int nx, int ny, double Lx, double Ly, HostCsr &R, HostCsr &P);
+void cant2d_element(double E, double nu, double Ke[64]);
+void cant2d_system(int nx, int ny, double E, double nu, const double *dens, HostCsr &K);
+void cant2d_transfer(int nx, int ny, HostCsr &R, HostCsr &P);
 int block_coloring(int32_t nblocks, int bs, const int32_t *rp, const int32_t *ci, std::vector<int32_t> &color);
 int32_t row_groups(int32_t n, const int32_t *rp, const int32_t *ci, std::vector<int32_t> &gptr);
 int group_coloring(int32_t ngroups, const int32_t *gptr, const int32_t *rp, const int32_t *ci,

@@ -263,6 +263,45 @@ def transfer_plate(nx, ny, Lx, Ly):
     return DeviceCsr(handle=r.value), DeviceCsr(handle=p.value)
 
 
+# ------------------------------------------------------------------ plane-stress cantilever (FemCantilever2D.cs)
+def _dens_ptr(dens, n):
+    if dens is None:
+        return None, None
+    d = np.ascontiguousarray(dens, np.float64)
+    if d.size != n:
+        raise ValueError(f"{n} element densities expected, got {d.size}")
+    return d, d.ctypes.data_as(C.c_void_p)
+
+
+def cantilever2d_system_host(nx, ny, E=1.0, nu=0.3, dens=None):
+    """K_ff of the 2D cantilever as host CSR, without a device."""
+    m = L.CsrOwned()
+    keep, dp = _dens_ptr(dens, nx * ny)
+    L.call("mgx_cantilever2d_system_host", nx, ny, C.c_double(E), C.c_double(nu), dp, C.byref(m))
+    return _owned_to_csr(m)
+
+
+def cantilever2d_transfer_host(nx, ny):
+    r, p = L.CsrOwned(), L.CsrOwned()
+    L.call("mgx_cantilever2d_transfer_host", nx, ny, C.byref(r), C.byref(p))
+    return _owned_to_csr(r), _owned_to_csr(p)
+
+
+def cantilever2d_matrix(nx, ny, E=1.0, nu=0.3, dens=None):
+    """K_ff of the 2D cantilever on the device."""
+    h = C.c_void_p()
+    keep, dp = _dens_ptr(dens, nx * ny)
+    L.call("mgx_matrix_cantilever2d", nx, ny, C.c_double(E), C.c_double(nu), dp, C.byref(h))
+    return DeviceCsr(handle=h.value)
+
+
+def transfer_cantilever2d(nx, ny):
+    """(R, P) between the 2D cantilever and its coarser model, on the device."""
+    r, p = C.c_void_p(), C.c_void_p()
+    L.call("mgx_matrix_transfer_cantilever2d", nx, ny, C.byref(r), C.byref(p))
+    return DeviceCsr(handle=r.value), DeviceCsr(handle=p.value)
+
+
 def read_hierarchy_duvi(path, level, which):
     """DuVi block layout of A / R / P of a level (the dict duvi_encode returns)."""
     h = L.DuviHost()

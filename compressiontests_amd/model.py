@@ -6,14 +6,17 @@ node id = ix + (nx+1)(iy + (ny+1) iz); element e = ex + nx (ey + ny ez); element
 VTK hexahedron order; 3 dofs (ux, uy, uz) per node; nodes with ix == 0 clamped
 (FemCantileverBase.cs:137-158).
 
+2D cantilever (FemCantilever2D): node id = ix + (nx+1) iy, element corners counter-clockwise from
+(ex, ey), 2 dofs (ux, uy) per node, nodes with ix == 0 clamped.
+
 Plate conventions (UniformCartesianMesh2D): node id = ix + (nx+1) iy; 3 dofs (w, dw/dy, dw/dx)
 per node, node-major; w fixed on the lines x = 0, x = Lx/2, y = 0, y = Ly/2, y = Ly
 (FemPlate.cs:126-188).
 """
 import numpy as np
 
-from .device import (DeviceArray, cantilever_matrix, plate_free_dofs, plate_matrix, rng_normal,
-                     transfer_plate, transfer_reference)
+from .device import (DeviceArray, cantilever2d_matrix, cantilever_matrix, plate_free_dofs, plate_matrix, rng_normal,
+                     transfer_cantilever2d, transfer_plate, transfer_reference)
 from . import kernels
 
 
@@ -122,6 +125,86 @@ class FemCantilever3D:
                                self.rhs_kind, self.TipLoad)
 
 
+class FemCantilever2D:
+    """FemCantilever2D (FemCantilever2D.cs:6-64) with the base class behaviour of FemCantileverBase
+    (FemCantileverBase.cs:11-193): the plane-stress cantilever of 4-node elements, lengthPerAxis =
+    (length, height, depth), clamped at x = 0, with the reference's manufactured right-hand side
+    b = K_ff U_f (U_f from the Euler-Bernoulli beam, FemCantileverBase.cs:64-77, at z = y - height/2,
+    FemCantilever2D.cs:51-57)."""
+
+    NumDofsPerNode = 2
+
+    def __init__(self, numElementsPerAxis, lengthPerAxis=(20.0, 1.0, 1.0), ElementDensities=None,
+                 ElasticityModulus=1.0, PoissonRatio=0.3, DistributedLoad=1.0):
+        self.NumElements = tuple(int(n) for n in numElementsPerAxis)
+        if len(self.NumElements) != 2:
+            raise ValueError("FemCantilever2D needs 2 element counts")
+        if len(lengthPerAxis) != 3:
+            raise ValueError("FemCantilever2D needs (length, height, depth)")
+        self.Lengths = tuple(float(v) for v in lengthPerAxis[:2])
+        self.CantileverDepth = float(lengthPerAxis[2])
+        self.ElementDensities = None if ElementDensities is None else np.ascontiguousarray(ElementDensities, np.float64)
+        self.ElasticityModulus = ElasticityModulus
+        self.PoissonRatio = PoissonRatio
+        self.DistributedLoad = DistributedLoad
+        nx, ny = self.NumElements
+        self.NumNodes = (nx + 1, ny + 1)
+        self.NumDofsAll = 2 * (nx + 1) * (ny + 1)
+        self.NumDofsFree = self.NumDofsAll - self.NumDofsAll // (nx + 1)
+        H, D = self.Lengths[1], self.CantileverDepth
+        self.MomentOfInertia = H * D / 12 * H * H                 # FemCantilever2D.cs:24
+        self._K = None
+
+    def device_matrix(self):
+        """K_ff on the device (library-owned), element moduli = ElementDensities x E."""
+        if self._K is None:
+            nx, ny = self.NumElements
+            self._K = cantilever2d_matrix(nx, ny, self.ElasticityModulus, self.PoissonRatio, self.ElementDensities)
+        return self._K
+
+    def CalcFreeDisplacements(self):
+        """U_f of FemCantileverBase.cs:160-182 with FemCantilever2D.cs:51-57."""
+        nx, ny = self.NumElements
+        Lx, Ly = self.Lengths
+        x = np.arange(1, nx + 1, dtype=np.float64) * (Lx / nx)
+        z = np.arange(0, ny + 1, dtype=np.float64) * (Ly / ny) - Ly / 2
+        q, E, I, L = self.DistributedLoad, self.ElasticityModulus, self.MomentOfInertia, Lx
+        w = -q / (24 * E * I) * (np.power(x, 4) - 4 * L * np.power(x, 3) + 6 * L * L * x * x)
+        rot = -q / (6 * E * I) * (np.power(x, 3) - 3 * L * x * x + 3 * L * L * x)
+        U = np.empty((ny + 1, nx, 2))
+        U[..., 0] = -rot[None, :] * z[:, None]
+        U[..., 1] = w[None, :]
+        return U.reshape(-1)
+
+    def rhs(self):
+        K = self.device_matrix()
+        U = DeviceArray.from_numpy(self.CalcFreeDisplacements())
+        b = DeviceArray(self.NumDofsFree)
+        kernels.matrix_vector_product(K, U, b)
+        return b.numpy()
+
+    def CreateLinearSystem(self):
+        """(A, b): K_ff as host CSR and the manufactured right-hand side."""
+        return self.device_matrix().to_host(), self.rhs()
+
+    def CreateAndSolveLinearSystem(self):
+        A, b = self.CreateLinearSystem()
+        return A, self.CalcFreeDisplacements(), b
+
+    def FindFreeDofs(self):
+        nx1, ny1 = self.NumNodes
+        n = np.arange(nx1 * ny1)
+        free_nodes = n[(n % nx1) != 0]
+        return (2 * free_nodes[:, None] + np.arange(2)[None, :]).reshape(-1).astype(np.int32)
+
+    def CreateCoarserModel(self):
+        """FemCantileverBase.cs:186-191: halve every element count > 1, same lengths, default
+        densities."""
+        c = tuple(n >> 1 if n > 1 else n for n in self.NumElements)
+        return FemCantilever2D(c, self.Lengths + (self.CantileverDepth,), None, self.ElasticityModulus,
+                               self.PoissonRatio, self.DistributedLoad)
+
+
 class FemPlate:
     """FemPlate (FemPlate.cs:20-189): a Kirchhoff plate of ACM rectangles (12-dof element of
     PlateElementMatrixProvider) under the uniform load DistributedLoad, assembled node-major over
@@ -188,6 +271,8 @@ def CreateCoarserModelAndSmoothenerMatrices(model):
     coarser = model.CreateCoarserModel()
     if isinstance(model, FemPlate):
         R, P = transfer_plate(*model.NumElements, *model.Lengths)
+    elif isinstance(model, FemCantilever2D):
+        R, P = transfer_cantilever2d(*model.NumElements)
     else:
         R, P = transfer_reference(*model.NumElements)
     return coarser, R, P

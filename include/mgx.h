@@ -250,6 +250,16 @@ int mgx_matrix_plate(int32_t nx, int32_t ny, double Lx, double Ly, double t, dou
 /* Restriction / interpolation between the plate and its coarser model (element counts halved):
  * IGeometricMultigridModel.cs:76-194 on two axes, free-dof submatrices. */
 int mgx_matrix_transfer_plate(int32_t nx, int32_t ny, double Lx, double Ly, mgx_matrix *R, mgx_matrix *P);
+/* The plane-stress cantilever (FemCantilever2D.cs with FemCantileverBase.cs): nx*ny 4-node elements of
+ * the 99-line topology optimization code (FemCantilever2D.cs:29-49; the matrix does not depend on
+ * the element size), dofs (ux, uy) per node, node by node, every node with ix == 0 clamped
+ * (FemCantileverBase.cs:137-158); dens (host, nx*ny, e = ex + nx ey, may be NULL = 1) scales each
+ * element.  K_ff on the device with sorted columns, each entry summed in ascending element order as
+ * the reference's DOK assembly (FemCantileverBase.cs:87-104). */
+int mgx_matrix_cantilever2d(int32_t nx, int32_t ny, double E, double nu, const double *dens, mgx_matrix *A);
+/* Restriction / interpolation between the 2D cantilever and its coarser model: IGeometricMultigridModel.cs:
+ * 76-194 on two axes with two dofs per node, free-dof submatrices. */
+int mgx_matrix_transfer_cantilever2d(int32_t nx, int32_t ny, mgx_matrix *R, mgx_matrix *P);
 /* Galerkin hierarchy: K_0 from dens, K_{l+1} = P_l^T K_l P_l for trilinear P_l, computed
  * element by element on the device; levels[l] receives K_l (nlevels entries). */
 int mgx_matrix_galerkin_hierarchy(int32_t nx, int32_t ny, int32_t nz, double E, double nu,
@@ -453,6 +463,10 @@ void mgx_csr_owned_free(mgx_csr_owned *m);
 int mgx_plate_system_host(int32_t nx, int32_t ny, double Lx, double Ly, double t, double E, double nu, double q,
                           mgx_csr_owned *K, double *b);
 int mgx_plate_transfer_host(int32_t nx, int32_t ny, double Lx, double Ly, mgx_csr_owned *R, mgx_csr_owned *P);
+/* The 2D cantilever matrices of mgx_matrix_cantilever2d / mgx_matrix_transfer_cantilever2d in host arrays
+ * (freed by mgx_csr_owned_free). */
+int mgx_cantilever2d_system_host(int32_t nx, int32_t ny, double E, double nu, const double *dens, mgx_csr_owned *K);
+int mgx_cantilever2d_transfer_host(int32_t nx, int32_t ny, mgx_csr_owned *R, mgx_csr_owned *P);
 /* DuVi block layout of one level's A / R / P (freed by mgx_duvi_host_free). */
 int mgx_hierarchy_read_duvi(const char *path, int32_t level, int32_t which, mgx_duvi_host *out);
 /* The level's structured grid (3 element counts). */

@@ -188,7 +188,7 @@ int orc_mgpcg_timed(int nlevels, const orc_level *lv, int smoother, double omega
                     int max_it, double tol, int stop, const double *b, double *x, orc_stats *st,
                     double *iter_seconds);
 
-/* ---------------- Kirchhoff plate workload (oracle_plate.c) */
+/* ---------------- Kirchhoff plate workload (oracle_fem2d.c) */
 void orc_plate_ke(double a, double b, double E, double nu, double t, double *Ke);
 void orc_plate_load(double a, double b, double q, double *fe);
 int32_t orc_plate_nfree(int nx, int ny, double Lx, double Ly);
@@ -199,6 +199,18 @@ void orc_plate_assemble(int nx, int ny, double Lx, double Ly, double t, double E
 void orc_plate_transfer_nnz(int nx, int ny, double Lx, double Ly, int64_t *nnzR, int64_t *nnzP);
 void orc_plate_transfer(int nx, int ny, double Lx, double Ly, int32_t *Rrowptr, int32_t *Rcol, double *Rval,
                         int32_t *Prowptr, int32_t *Pcol, double *Pval);
+/* ---------------- plane-stress cantilever, FemCantilever2D (oracle_fem2d.c) */
+void orc_ke_q4(double E, double nu, double *Ke);
+int32_t orc_cant2d_nfree(int nx, int ny);
+int64_t orc_cant2d_nnz(int nx, int ny);
+void orc_cant2d_assemble(int nx, int ny, double E, double nu, const double *dens, int32_t *rowptr, int32_t *col,
+                         double *val);
+void orc_cant2d_manufactured(int nx, int ny, double Lx, double Ly, double Lz, double E, double q, double *Uf);
+void orc_cant2d_multiply_dok_order(int nx, int ny, const int32_t *rowptr, const int32_t *col, const double *val,
+                                   const double *x, double *y);
+void orc_cant2d_transfer_nnz(int nx, int ny, int64_t *nnzR, int64_t *nnzP);
+void orc_cant2d_transfer(int nx, int ny, int32_t *Rrowptr, int32_t *Rcol, double *Rval, int32_t *Prowptr,
+                         int32_t *Pcol, double *Pval);
 /* node groups: rows gptr[g] .. gptr[g+1]-1 (the free dofs of one mesh node) */
 int32_t orc_row_groups(int32_t n, const int32_t *rp, const int32_t *ci, int32_t *gptr);
 int orc_group_coloring(int32_t ngroups, const int32_t *gptr, const int32_t *rp, const int32_t *ci, int32_t *color);

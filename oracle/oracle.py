@@ -71,6 +71,8 @@ def lib(omp=False):
         L.orc_splitmix64_next.restype = C.c_uint64
         L.orc_plate_nnz.restype = C.c_int64
         L.orc_plate_nfree.restype = C.c_int32
+        L.orc_cant2d_nnz.restype = C.c_int64
+        L.orc_cant2d_nfree.restype = C.c_int32
         L.orc_row_groups.restype = C.c_int32
         _LIBS[name] = L
     return L
@@ -212,6 +214,53 @@ def plate_transfer(nx, ny, Lx=3.0, Ly=2.0):
     Prp = np.empty(nf + 1, np.int32); Pci = np.empty(npp.value, np.int32); Pv = np.empty(npp.value, np.float64)
     lib().orc_plate_transfer(nx, ny, C.c_double(Lx), C.c_double(Ly), _p(Rrp), _p(Rci), _p(Rv), _p(Prp), _p(Pci),
                              _p(Pv))
+    return Csr(Rrp, Rci, Rv, nc, nf), Csr(Prp, Pci, Pv, nf, nc)
+
+
+# ------------------------------------------------------------------ plane-stress cantilever
+def ke_q4(E=1.0, nu=0.3):
+    """Element stiffness of FemCantilever2D (8 x 8, dofs (ux, uy) of the corners (0,0), (1,0),
+    (1,1), (0,1))."""
+    Ke = np.empty((8, 8), np.float64)
+    lib().orc_ke_q4(C.c_double(E), C.c_double(nu), _p(Ke))
+    return Ke
+
+
+def cantilever2d(nx, ny, E=1.0, nu=0.3, dens=None):
+    """K_ff of FemCantilever2D with nx x ny elements (element densities dens, e = ex + nx ey)."""
+    n = lib().orc_cant2d_nfree(nx, ny)
+    nnz = lib().orc_cant2d_nnz(nx, ny)
+    rp = np.empty(n + 1, np.int32); ci = np.empty(nnz, np.int32); v = np.empty(nnz, np.float64)
+    d = None if dens is None else np.ascontiguousarray(dens, np.float64)
+    lib().orc_cant2d_assemble(nx, ny, C.c_double(E), C.c_double(nu), None if d is None else _p(d), _p(rp), _p(ci),
+                              _p(v))
+    return Csr(rp, ci, v, n, n)
+
+
+def manufactured2d(nx, ny, L=(20.0, 1.0, 1.0), E=1.0, q=1.0):
+    """U_f of FemCantilever2D (lengths (Lx, Ly, depth))."""
+    U = np.empty(lib().orc_cant2d_nfree(nx, ny), np.float64)
+    lib().orc_cant2d_manufactured(nx, ny, C.c_double(L[0]), C.c_double(L[1]), C.c_double(L[2]), C.c_double(E),
+                                  C.c_double(q), _p(U))
+    return U
+
+
+def multiply_dok_order2d(nx, ny, A, x):
+    y = np.empty(A.nrows, np.float64)
+    x = np.ascontiguousarray(x, np.float64)
+    lib().orc_cant2d_multiply_dok_order(nx, ny, _p(A.rowptr), _p(A.col), _p(A.val), _p(x), _p(y))
+    return y
+
+
+def cantilever2d_transfer(nx, ny):
+    """(R, P) between the nx x ny cantilever and its coarser model."""
+    nr, npp = C.c_int64(), C.c_int64()
+    lib().orc_cant2d_transfer_nnz(nx, ny, C.byref(nr), C.byref(npp))
+    cx, cy = [(k >> 1) if k > 1 else k for k in (nx, ny)]
+    nf, nc = lib().orc_cant2d_nfree(nx, ny), lib().orc_cant2d_nfree(cx, cy)
+    Rrp = np.empty(nc + 1, np.int32); Rci = np.empty(nr.value, np.int32); Rv = np.empty(nr.value, np.float64)
+    Prp = np.empty(nf + 1, np.int32); Pci = np.empty(npp.value, np.int32); Pv = np.empty(npp.value, np.float64)
+    lib().orc_cant2d_transfer(nx, ny, _p(Rrp), _p(Rci), _p(Rv), _p(Prp), _p(Pci), _p(Pv))
     return Csr(Rrp, Rci, Rv, nc, nf), Csr(Prp, Pci, Pv, nf, nc)
 
 

@@ -1,7 +1,10 @@
-/* oracle_plate.c -- CPU restatement of the reference's Kirchhoff plate workload (test
- * infrastructure: checker only, never the measured or shipped path).
+/* oracle_fem2d.c -- CPU restatement of the reference's two 2D workloads, the Kirchhoff plate and
+ * the plane-stress cantilever (test infrastructure: checker only, never the measured or shipped
+ * path).
  *
  *   FemPlate              tests/MGroup.LinearAlgebra.Tests/TestData/SparseLinearSystems/FemPlate.cs
+ *   FemCantilever2D       tests/MGroup.LinearAlgebra.Tests/TestData/SparseLinearSystems/FemCantilever2D.cs
+ *                         with FemCantileverBase.cs (assembly, free dofs, manufactured solution)
  *   plate element         tests/MGroup.FEM.Structural.Tests/FemExtentions/PlateElementRectangle4Nodes3Dofs.cs
  *   load vector           src/MGroup.Solvers/Assemblers/PlateElementVectorAssembler.cs
  *   matrix assembly       src/MGroup.Solvers/Assemblers/DokRowMajorMatrixAssembler.cs:30-57 with
@@ -12,7 +15,9 @@
  * = ix + (nx+1) iy at (ix Lx/nx, iy Ly/ny); element e = ex + nx ey with corners (ex,ey),
  * (ex+1,ey), (ex+1,ey+1), (ex,ey+1) -- the counter-clockwise order the element's rectangle check
  * demands (PlateElementRectangle4Nodes3Dofs.cs:33-39); dofs per node (w, rx, ry) in that order,
- * numbered node by node, constrained dofs left out (FemPlate.cs:72-101).
+ * numbered node by node, constrained dofs left out (FemPlate.cs:72-101).  The cantilever has the
+ * same mesh with dofs (ux, uy) per node and every node with ix == 0 clamped
+ * (FemCantileverBase.cs:137-158).
  */
 #include <math.h>
 #include <stdlib.h>
@@ -76,33 +81,57 @@ static int plate_fixed(int ix, int iy, int nx, int ny, double Lx, double Ly)
     return fabs(x) < tol || fabs(x - Lx / 2) < tol || fabs(y) < tol || fabs(y - Ly / 2) < tol || fabs(y - Ly) < tol;
 }
 
-/* free dof index of dof d of node (ix, iy), or -1 (node-major numbering of FemPlate.cs:72-101);
- * first[] = free index of each node's first free dof, fixed[] = its w is constrained */
-typedef struct { int nx, ny; int32_t *first; uint8_t *fixed; int32_t nfree; } plate_dofs;
+/* free dof index of dof d of node (ix, iy), or -1 (node-major numbering of FemPlate.cs:72-101 and
+ * FemCantileverBase.cs:137-158); dpn dofs per node, first[] = free index of each node's first free
+ * dof, fixed[] = bit d set when dof d of the node is constrained (plate: bit 0 at supported nodes;
+ * cantilever: both bits at ix == 0) */
+typedef struct { int nx, ny, dpn; int32_t *first; uint8_t *fixed; int32_t nfree; } plate_dofs;
+
+static void dofs_alloc(int nx, int ny, int dpn, plate_dofs *P)
+{
+    const int64_t nn = (int64_t)(nx + 1) * (ny + 1);
+    P->nx = nx; P->ny = ny; P->dpn = dpn;
+    P->first = (int32_t *)malloc(sizeof(int32_t) * nn);
+    P->fixed = (uint8_t *)malloc(nn);
+}
+
+static void dofs_number(plate_dofs *P)
+{
+    const int64_t nn = (int64_t)(P->nx + 1) * (P->ny + 1);
+    int32_t f = 0;
+    for (int64_t id = 0; id < nn; ++id) {
+        P->first[id] = f;
+        for (int d = 0; d < P->dpn; ++d) f += !((P->fixed[id] >> d) & 1);
+    }
+    P->nfree = f;
+}
 
 static void plate_dofs_make(int nx, int ny, double Lx, double Ly, plate_dofs *P)
 {
-    const int64_t nn = (int64_t)(nx + 1) * (ny + 1);
-    P->nx = nx; P->ny = ny;
-    P->first = (int32_t *)malloc(sizeof(int32_t) * nn);
-    P->fixed = (uint8_t *)malloc(nn);
-    int32_t f = 0;
+    dofs_alloc(nx, ny, 3, P);
     for (int iy = 0; iy <= ny; ++iy)
-        for (int ix = 0; ix <= nx; ++ix) {
-            const int64_t id = ix + (int64_t)(nx + 1) * iy;
-            P->fixed[id] = (uint8_t)plate_fixed(ix, iy, nx, ny, Lx, Ly);
-            P->first[id] = f;
-            f += P->fixed[id] ? 2 : 3;
-        }
-    P->nfree = f;
+        for (int ix = 0; ix <= nx; ++ix)
+            P->fixed[ix + (int64_t)(nx + 1) * iy] = (uint8_t)plate_fixed(ix, iy, nx, ny, Lx, Ly);
+    dofs_number(P);
+}
+
+static void cant2d_dofs_make(int nx, int ny, plate_dofs *P)
+{
+    dofs_alloc(nx, ny, 2, P);
+    for (int iy = 0; iy <= ny; ++iy)
+        for (int ix = 0; ix <= nx; ++ix) P->fixed[ix + (int64_t)(nx + 1) * iy] = ix == 0 ? 3 : 0;
+    dofs_number(P);
 }
 
 static void plate_dofs_free(plate_dofs *P) { free(P->first); free(P->fixed); }
 
 static inline int32_t plate_dof(const plate_dofs *P, int64_t id, int d)
 {
-    if (P->fixed[id]) return d == 0 ? -1 : P->first[id] + d - 1;
-    return P->first[id] + d;
+    const unsigned m = P->fixed[id];
+    if ((m >> d) & 1) return -1;
+    int32_t k = P->first[id];
+    for (int j = 0; j < d; ++j) k += !((m >> j) & 1);
+    return k;
 }
 
 int32_t orc_plate_nfree(int nx, int ny, double Lx, double Ly)
@@ -245,12 +274,11 @@ static double prp_influence(const prp_dims *d, const int *w, const double *cur)
     return 1 - sqrt(s);
 }
 
-void orc_plate_transfer_nnz(int nx, int ny, double Lx, double Ly, int64_t *nnzR, int64_t *nnzP)
+/* the transfers between the dof maps F (fine) and Cd (its coarser model) */
+static void transfer_nnz(const plate_dofs *F_, const plate_dofs *Cd_, int64_t *nnzR, int64_t *nnzP)
 {
-    prp_dims d; prp_setup(nx, ny, &d);
-    plate_dofs F, Cd;
-    plate_dofs_make(nx, ny, Lx, Ly, &F);
-    plate_dofs_make(d.nc[0] - 1, d.nc[1] - 1, Lx, Ly, &Cd);
+    prp_dims d; prp_setup(F_->nx, F_->ny, &d);
+    const plate_dofs F = *F_, Cd = *Cd_;
     int64_t nr = 0;
     for (int a1 = 0; a1 < d.nc[1]; ++a1)
         for (int a0 = 0; a0 < d.nc[0]; ++a0) {
@@ -258,7 +286,7 @@ void orc_plate_transfer_nnz(int nx, int ny, double Lx, double Ly, int64_t *nnzR,
             prp_window(&d, 0, a0, &s[0], &e[0], &cur[0]);
             prp_window(&d, 1, a1, &s[1], &e[1], &cur[1]);
             const int64_t cid = a0 + (int64_t)d.nc[0] * a1;
-            for (int dd = 0; dd < 3; ++dd) {
+            for (int dd = 0; dd < F.dpn; ++dd) {
                 if (plate_dof(&Cd, cid, dd) < 0) continue;
                 for (int w1 = s[1]; w1 < e[1]; ++w1)
                     for (int w0 = s[0]; w0 < e[0]; ++w0)
@@ -267,16 +295,13 @@ void orc_plate_transfer_nnz(int nx, int ny, double Lx, double Ly, int64_t *nnzR,
         }
     *nnzR = nr;
     *nnzP = nr;                                                         /* the same pairs, transposed */
-    plate_dofs_free(&F); plate_dofs_free(&Cd);
 }
 
-void orc_plate_transfer(int nx, int ny, double Lx, double Ly, int32_t *Rrowptr, int32_t *Rcol, double *Rval,
-                        int32_t *Prowptr, int32_t *Pcol, double *Pval)
+static void transfer_fill(const plate_dofs *F_, const plate_dofs *Cd_, int32_t *Rrowptr, int32_t *Rcol,
+                          double *Rval, int32_t *Prowptr, int32_t *Pcol, double *Pval)
 {
-    prp_dims d; prp_setup(nx, ny, &d);
-    plate_dofs F, Cd;
-    plate_dofs_make(nx, ny, Lx, Ly, &F);
-    plate_dofs_make(d.nc[0] - 1, d.nc[1] - 1, Lx, Ly, &Cd);
+    prp_dims d; prp_setup(F_->nx, F_->ny, &d);
+    const plate_dofs F = *F_, Cd = *Cd_;
     const int64_t nfn = (int64_t)d.nf[0] * d.nf[1], ncn = (int64_t)d.nc[0] * d.nc[1];
     double *sumR = (double *)calloc(ncn, sizeof(double)), *sumP = (double *)calloc(nfn, sizeof(double));
     int s[2], e[2], w[2], a[2]; double cur[2];
@@ -299,7 +324,7 @@ void orc_plate_transfer(int nx, int ny, double Lx, double Ly, int32_t *Rrowptr, 
         for (a[0] = 0; a[0] < d.nc[0]; ++a[0]) {
             const int64_t cid = a[0] + (int64_t)d.nc[0] * a[1];
             for (int i = 0; i < 2; ++i) prp_window(&d, i, a[i], &s[i], &e[i], &cur[i]);
-            for (int dd = 0; dd < 3; ++dd) {
+            for (int dd = 0; dd < F.dpn; ++dd) {
                 if (plate_dof(&Cd, cid, dd) < 0) continue;
                 Rrowptr[row++] = (int32_t)pos;
                 for (w[1] = s[1]; w[1] < e[1]; ++w[1])
@@ -328,7 +353,7 @@ void orc_plate_transfer(int nx, int ny, double Lx, double Ly, int32_t *Rrowptr, 
                     if (w[i] >= ss && w[i] < ee) { if (aa < lo[i]) lo[i] = aa; if (aa > hi[i]) hi[i] = aa; }
                 }
             }
-            for (int dd = 0; dd < 3; ++dd) {
+            for (int dd = 0; dd < F.dpn; ++dd) {
                 if (plate_dof(&F, fid, dd) < 0) continue;
                 Prowptr[row++] = (int32_t)pos;
                 for (a[1] = lo[1]; a[1] <= hi[1]; ++a[1])
@@ -344,5 +369,188 @@ void orc_plate_transfer(int nx, int ny, double Lx, double Ly, int32_t *Rrowptr, 
         }
     Prowptr[row] = (int32_t)pos;
     free(sumR); free(sumP);
+}
+
+void orc_plate_transfer_nnz(int nx, int ny, double Lx, double Ly, int64_t *nnzR, int64_t *nnzP)
+{
+    prp_dims d; prp_setup(nx, ny, &d);
+    plate_dofs F, Cd;
+    plate_dofs_make(nx, ny, Lx, Ly, &F);
+    plate_dofs_make(d.nc[0] - 1, d.nc[1] - 1, Lx, Ly, &Cd);
+    transfer_nnz(&F, &Cd, nnzR, nnzP);
+    plate_dofs_free(&F); plate_dofs_free(&Cd);
+}
+
+void orc_plate_transfer(int nx, int ny, double Lx, double Ly, int32_t *Rrowptr, int32_t *Rcol, double *Rval,
+                        int32_t *Prowptr, int32_t *Pcol, double *Pval)
+{
+    prp_dims d; prp_setup(nx, ny, &d);
+    plate_dofs F, Cd;
+    plate_dofs_make(nx, ny, Lx, Ly, &F);
+    plate_dofs_make(d.nc[0] - 1, d.nc[1] - 1, Lx, Ly, &Cd);
+    transfer_fill(&F, &Cd, Rrowptr, Rcol, Rval, Prowptr, Pcol, Pval);
+    plate_dofs_free(&F); plate_dofs_free(&Cd);
+}
+
+/* ================================================================ plane-stress cantilever */
+
+/* ElementStiffness, FemCantilever2D.cs:29-49 (the 4-node plane-stress element of "A 99 line
+ * topology optimization code", Sigmund 2001): eight distinct entries k[0..7] in the reference's
+ * operation order, placed by the reference's 8x8 index table, then every entry times
+ * E / (1 - nu^2) (ScaleIntoThis). */
+void orc_ke_q4(double E, double nu, double *Ke)
+{
+    const double v = nu;
+    const double k[8] = {0.5 - v / 6.0, 0.125 + v / 8.0, -0.25 - v / 12.0, -0.125 + 3 * v / 8.0,
+                         -0.25 + v / 12.0, -0.125 - v / 8.0, v / 6.0, 0.125 - 3 * v / 8.0};
+    static const int T[8][8] = {{0, 1, 2, 3, 4, 5, 6, 7}, {1, 0, 7, 6, 5, 4, 3, 2}, {2, 7, 0, 5, 6, 3, 4, 1},
+                                {3, 6, 5, 0, 7, 2, 1, 4}, {4, 5, 6, 7, 0, 1, 2, 3}, {5, 4, 3, 2, 1, 0, 7, 6},
+                                {6, 3, 4, 1, 2, 7, 0, 5}, {7, 2, 1, 4, 3, 6, 5, 0}};
+    const double s = E / (1 - v * v);
+    for (int i = 0; i < 8; ++i)
+        for (int j = 0; j < 8; ++j) Ke[i * 8 + j] = k[T[i][j]] * s;
+}
+
+int32_t orc_cant2d_nfree(int nx, int ny) { return 2 * nx * (ny + 1); }
+
+/* nnz of K_ff: the free dofs of the (up to 9) nodes sharing an element with the row's node */
+int64_t orc_cant2d_nnz(int nx, int ny)
+{
+    int64_t nnz = 0;
+    for (int iy = 0; iy <= ny; ++iy)
+        for (int ix = 1; ix <= nx; ++ix) {
+            int cx = 0, cy = 0;
+            for (int dd = -1; dd <= 1; ++dd) {
+                cx += ix + dd >= 1 && ix + dd <= nx;
+                cy += iy + dd >= 0 && iy + dd <= ny;
+            }
+            nnz += 4LL * cx * cy;
+        }
+    return nnz;
+}
+
+/* K_ff of AssembleGlobalMatrix + GetSubmatrix (FemCantileverBase.cs:87-104,137-158): entry (r, c)
+ * sums density_e * Ke over the elements holding both dofs in ascending element order
+ * e = ex + nx ey, each added to the running entry as sub + old (DokRowMajor.cs:182-207); rows and
+ * the sorted columns (BuildCsrArrays(true)) in free-dof order.  dens = NULL: every density 1. */
+void orc_cant2d_assemble(int nx, int ny, double E, double nu, const double *dens, int32_t *rowptr, int32_t *col,
+                         double *val)
+{
+    double Ke[64];
+    orc_ke_q4(E, nu, Ke);
+    plate_dofs P;
+    cant2d_dofs_make(nx, ny, &P);
+    static const int cx[4] = {0, 1, 1, 0}, cy[4] = {0, 0, 1, 1};
+    int64_t pos = 0, row = 0;
+    for (int iy = 0; iy <= ny; ++iy)
+        for (int ix = 1; ix <= nx; ++ix)
+            for (int a = 0; a < 2; ++a) {
+                rowptr[row++] = (int32_t)pos;
+                for (int jy = iy - 1; jy <= iy + 1; ++jy)
+                    for (int jx = ix - 1; jx <= ix + 1; ++jx) {
+                        if (jx < 1 || jx > nx || jy < 0 || jy > ny) continue;
+                        const int64_t jd = jx + (int64_t)(nx + 1) * jy;
+                        for (int b = 0; b < 2; ++b) {
+                            double acc = 0;
+                            for (int ey = (iy > jy ? iy : jy) - 1; ey <= (iy < jy ? iy : jy); ++ey)
+                                for (int ex = (ix > jx ? ix : jx) - 1; ex <= (ix < jx ? ix : jx); ++ex) {
+                                    if (ex < 0 || ex >= nx || ey < 0 || ey >= ny) continue;
+                                    int li = -1, lj = -1;
+                                    for (int c = 0; c < 4; ++c) {
+                                        if (ex + cx[c] == ix && ey + cy[c] == iy) li = c;
+                                        if (ex + cx[c] == jx && ey + cy[c] == jy) lj = c;
+                                    }
+                                    const double de = dens ? dens[ex + (int64_t)nx * ey] : 1.0;
+                                    acc = de * Ke[(2 * li + a) * 8 + 2 * lj + b] + acc;
+                                }
+                            col[pos] = plate_dof(&P, jd, b);
+                            val[pos] = acc;
+                            ++pos;
+                        }
+                    }
+            }
+    rowptr[row] = (int32_t)pos;
+    plate_dofs_free(&P);
+}
+
+/* U_f of CalcFreeDisplacements (FemCantileverBase.cs:160-182) with FemCantilever2D.cs:51-57: at
+ * node (ix Lx/nx, iy Ly/ny), x the first coordinate and z = y - Ly/2, (u, w) of the Euler-Bernoulli
+ * beam (:64-77) with I = Ly Lz / 12 Ly Ly (FemCantilever2D.cs:24, Lz the depth). */
+void orc_cant2d_manufactured(int nx, int ny, double Lx, double Ly, double Lz, double E, double q, double *Uf)
+{
+    const double I = Ly * Lz / 12 * Ly * Ly, L = Lx;
+    const double hx = Lx / nx, hy = Ly / ny;
+    int64_t pos = 0;
+    for (int iy = 0; iy <= ny; ++iy)
+        for (int ix = 1; ix <= nx; ++ix) {
+            const double x = ix * hx;
+            const double z = iy * hy - (Ly - 0) / 2;
+            const double w = -q / (24 * E * I) * (pow(x, 4) - 4 * L * pow(x, 3) + 6 * L * L * x * x);
+            const double rot = -q / (6 * E * I) * (pow(x, 3) - 3 * L * x * x + 3 * L * L * x);
+            const double u = -rot * z;
+            Uf[pos++] = u;
+            Uf[pos++] = w;
+        }
+}
+
+/* y = K_ff x with each row summed in the DOK insertion order (DokRowMajor.cs:660-674): a row's
+ * columns in the order the element loop first touched them (elements ascending, then the
+ * element's local dofs), FemCantileverBase.cs:95-101. */
+void orc_cant2d_multiply_dok_order(int nx, int ny, const int32_t *rowptr, const int32_t *col, const double *val,
+                                   const double *x, double *y)
+{
+    static const int cx[4] = {0, 1, 1, 0}, cy[4] = {0, 0, 1, 1};
+    int64_t row = 0;
+    for (int iy = 0; iy <= ny; ++iy)
+        for (int ix = 1; ix <= nx; ++ix)
+            for (int a = 0; a < 2; ++a, ++row) {
+                const int32_t s0 = rowptr[row], t0 = rowptr[row + 1];
+                int32_t seen[18];
+                int ns = 0;
+                double dot = 0;
+                for (int ey = iy - 1; ey <= iy; ++ey) {
+                    if (ey < 0 || ey >= ny) continue;
+                    for (int ex = ix - 1; ex <= ix; ++ex) {
+                        if (ex < 0 || ex >= nx) continue;
+                        for (int c = 0; c < 4; ++c) {
+                            const int jx = ex + cx[c], jy = ey + cy[c];
+                            if (jx == 0) continue;
+                            for (int b = 0; b < 2; ++b) {
+                                const int32_t cc = (int32_t)(2 * ((int64_t)jy * nx + (jx - 1)) + b);
+                                int dup = 0;
+                                for (int k = 0; k < ns; ++k) dup |= seen[k] == cc;
+                                if (dup) continue;
+                                seen[ns++] = cc;
+                                int32_t lo = s0, hi = t0 - 1;
+                                while (lo < hi) { const int32_t mid = (lo + hi) / 2; if (col[mid] < cc) lo = mid + 1; else hi = mid; }
+                                dot += val[lo] * x[cc];
+                            }
+                        }
+                    }
+                }
+                y[row] = dot;
+            }
+}
+
+void orc_cant2d_transfer_nnz(int nx, int ny, int64_t *nnzR, int64_t *nnzP)
+{
+    prp_dims d; prp_setup(nx, ny, &d);
+    plate_dofs F, Cd;
+    cant2d_dofs_make(nx, ny, &F);
+    cant2d_dofs_make(d.nc[0] - 1, d.nc[1] - 1, &Cd);
+    transfer_nnz(&F, &Cd, nnzR, nnzP);
+    plate_dofs_free(&F); plate_dofs_free(&Cd);
+}
+
+/* CreateRestrictionAndInterpolationMatrix (IGeometricMultigridModel.cs:76-194) on two axes with
+ * two dofs per node, then the free-dof submatrices of the clamped ix == 0 nodes */
+void orc_cant2d_transfer(int nx, int ny, int32_t *Rrowptr, int32_t *Rcol, double *Rval, int32_t *Prowptr,
+                         int32_t *Pcol, double *Pval)
+{
+    prp_dims d; prp_setup(nx, ny, &d);
+    plate_dofs F, Cd;
+    cant2d_dofs_make(nx, ny, &F);
+    cant2d_dofs_make(d.nc[0] - 1, d.nc[1] - 1, &Cd);
+    transfer_fill(&F, &Cd, Rrowptr, Rcol, Rval, Prowptr, Pcol, Pval);
     plate_dofs_free(&F); plate_dofs_free(&Cd);
 }

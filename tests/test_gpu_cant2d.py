@@ -1,0 +1,205 @@
+"""The plane-stress cantilever workload (FemCantilever2D.cs; the 2D cases of GMCantileverTests.cs and
+GMCantileverOpenCLTests.cs) on the GPU: the device generators against the oracle bit for bit; the
+stationary V-cycle (CSR and DuVi, multicolour Gauss-Seidel over the node pairs and Jacobi) on the
+reference's 256 x 16 cantilever against the oracle's V-cycle in the solver's own order, cycle
+counts within one; Jacobi PCG against the oracle's; MG-PCG against the oracle's MG-PCG and, on the
+reference's 4096 x 256 cantilever, against the manufactured solution itself; and K right-hand sides
+through the block kernels against single solves."""
+import numpy as np
+import pytest
+import scipy.sparse as sp
+
+from compressiontests_amd.device import transfer_cantilever2d
+from compressiontests_amd.model import FemCantilever2D
+from compressiontests_amd.solvers import (HipCsrGeometricMultigridSolver, HipCsrPcgSolver,
+                                          HipDuViGeometricMultigridSolver, HipMgPcgSolver)
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+LEN = (20.0, 1.0, 1.0)          # GMCantileverTests.cs:109 (length, height, depth)
+
+
+def same_csr(a, b):
+    return (a.nrows == b.nrows and a.ncols == b.ncols and np.array_equal(a.rowptr, b.rowptr)
+            and np.array_equal(a.col, b.col) and a.val.tobytes() == b.val.tobytes())
+
+
+def S(M):
+    return sp.csr_matrix((M.val, M.col, M.rowptr), shape=(M.nrows, M.ncols))
+
+
+def energy_err(A, x, xs):
+    e = x - xs
+    return np.sqrt(e @ (S(A) @ e)) / np.sqrt(xs @ (S(A) @ xs))
+
+
+def host_csr(M):
+    h = M.to_host()
+    return O.Csr(h.rowptr, h.col, h.val, h.nrows, h.ncols)
+
+
+def oracle_levels(nx, ny, depth):
+    lv, m = [], (nx, ny)
+    for _ in range(depth):
+        R, P = O.cantilever2d_transfer(*m)
+        lv.append(dict(A=O.cantilever2d(*m), R=R, P=P))
+        m = (m[0] >> 1 if m[0] > 1 else 1, m[1] >> 1 if m[1] > 1 else 1)
+    lv.append(dict(A=O.cantilever2d(*m)))
+    return lv
+
+
+@pytest.fixture(scope="module")
+def c256(gpu):
+    m = FemCantilever2D((256, 16), LEN)
+    A, U, b = m.CreateAndSolveLinearSystem()
+    return m, A, U, b
+
+
+def test_cant2d_device_generators(c256):
+    """K_ff (mgx_matrix_cantilever2d) and R, P (mgx_matrix_transfer_cantilever2d) on the device equal
+    the oracle's bit for bit (heterogeneous densities too); b = K_ff U_f on the device agrees with
+    the oracle's DOK-order product row by row within 32 ulp of sum |K_ij U_j| (the two sum in
+    different orders, and K_ff U_f cancels to near zero inside the beam)."""
+    m, A, U, b = c256
+    assert same_csr(A, O.cantilever2d(256, 16))
+    R, P = transfer_cantilever2d(256, 16)
+    Ro, Po = O.cantilever2d_transfer(256, 16)
+    assert same_csr(R.to_host(), Ro) and same_csr(P.to_host(), Po)
+    dens = 10 ** (2 * O.rng_uniform(5, 64 * 8) - 1)
+    h = FemCantilever2D((64, 8), LEN, ElementDensities=dens)
+    assert same_csr(h.device_matrix().to_host(), O.cantilever2d(64, 8, dens=dens))
+    bo = O.multiply_dok_order2d(256, 16, O.cantilever2d(256, 16), U)
+    scale = abs(S(A)) @ np.abs(U)
+    assert np.all(np.abs(b - bo) <= 32 * np.finfo(float).eps * scale)
+
+
+@pytest.mark.parametrize("cls", [HipCsrGeometricMultigridSolver, HipDuViGeometricMultigridSolver])
+@pytest.mark.parametrize("gs,depth,it", [(True, 1, 2), (True, 2, 4), (True, 3, 8), (True, 4, 2),
+                                         (False, 1, 4), (False, 2, 2), (False, 3, 4), (False, 4, 8)])
+def test_cant2d_gmg_parity(c256, cls, gs, depth, it):
+    """CheckCantileverSolutionDeepVWithOpenCL (GMCantileverOpenCLTests.cs:55-71 with :74-110:
+    tolerance 1e-5, 20000 cycles, coarse relaxation) on the 256 x 16 cantilever.  The Gauss-Seidel
+    nodes are the node pairs (row groups of 2), coloured as the oracle colours them, and the
+    oracle's V-cycle runs in the solver's own order."""
+    m, A, U, b = c256
+    tol = 1e-5
+    s = cls.CreateDeepV(m, GaussSeidel=gs, maxCircleIterations=20000, coarseRelaxation=True,
+                        convergenceTolerance=tol, depth=depth, levelIterations=it)
+    lv = oracle_levels(256, 16, depth)
+    for l in range(depth):
+        g = s._h.level_groups(l)
+        if not gs:
+            assert g is None
+            continue
+        assert np.array_equal(g, np.arange(0, lv[l]["A"].nrows + 1, 2))
+        nc, col = O.group_coloring(lv[l]["A"], g)
+        order = s._h.gs_order(l, len(g) - 1)
+        assert s._h.level_colors(l) == nc == 4
+        assert np.array_equal(order, np.argsort(col, kind="stable"))
+        lv[l]["gptr"], lv[l]["gs_order"] = g, order
+    x, st, times = s.Solve(None)
+    s.ReleaseOpenCLResources()
+    pre = None
+    if not gs:
+        lam = O.eigen_upper_bound(lv[-1]["A"])
+        pre = [O.jacobi_preconditioner(l["A"]) * (2 / lam) for l in lv[:-1]]
+    xo, so = O.gmg_solve(lv, b, pre=pre, smoother=1 if gs else 0, level_iterations=it, max_cycles=20000, tol=tol)
+    print(f"cantilever 256x16 {cls.__name__} gs={gs} depth={depth} it={it}: cycles {st.NumIterationsRequired} "
+          f"(oracle {so.iterations})", flush=True)
+    assert so.converged == 1 and st.HasConverged
+    assert abs(st.NumIterationsRequired - so.iterations) <= 1
+    assert np.abs(b - O.spmv(A, x)).max() <= tol
+    assert energy_err(A, x, xo) < 1e-6
+    assert len(times) == depth + 1
+
+
+def test_cant2d_pcg_jacobi_parity(c256):
+    """CheckCantileverSolutionCGWithOpenCL (GMCantileverOpenCLTests.cs:112-156) on 256 x 16.  The
+    slender beam (20 : 1) makes K_ff ill-conditioned, and over some 800 iterations the dot
+    products' different summation orders (device tree against the oracle's serial loop) shift the
+    Krylov sequence: the counts agree within 5 %."""
+    m, A, U, b = c256
+    s = HipCsrPcgSolver(20000, 1e-5)
+    s.Initialize(m)
+    x, st = s.Solve(None)
+    s.ReleaseOpenCLResources()
+    xo, so = O.pcg_jacobi(A, b, max_it=20000, tol=1e-5)
+    print(f"cantilever 256x16 PCG: iterations {st.NumIterationsRequired} (oracle {so.iterations})", flush=True)
+    assert st.HasConverged and so.converged == 1
+    assert abs(st.NumIterationsRequired - so.iterations) <= so.iterations // 20
+    assert np.abs(b - O.spmv(A, x)).max() <= 1e-5
+
+
+@pytest.mark.parametrize("smoother,fmt", [("gs", "csr"), ("jacobi", "csr"), ("gs", "duvi")])
+def test_cant2d_mgpcg_parity(c256, smoother, fmt):
+    """MG-PCG on the 256 x 16 cantilever over the reference's levels (R = P^T): iterations within
+    10 % of the oracle's MG-PCG in the solver's own Gauss-Seidel order, and the manufactured
+    solution to 1e-8 in the energy norm."""
+    m, A, U, b = c256
+    s = HipMgPcgSolver(m, levels=4, smoother=smoother, omega=0.0, nu=2, tolerance=1e-10, hierarchy="reference",
+                       format=fmt)
+    s.Initialize(b)
+    x, st = s.Solve()
+    lv = []
+    for l, (A_l, R_l, P_l) in enumerate(s.hierarchy):
+        d = {"A": host_csr(A_l)}
+        if R_l is not None:
+            d["R"], d["P"] = host_csr(R_l), host_csr(P_l)
+            if smoother == "gs":
+                d["gptr"], d["gs_order"] = s.GsGroups(l), s.GsOrder(l)
+        lv.append(d)
+    xo, so = O.mgpcg(lv, b, smoother=1 if smoother == "gs" else 0, omega=0.0, nu=2, tol=1e-10)
+    e = energy_err(A, x, U)
+    print(f"cantilever 256x16 MG-PCG {smoother} {fmt}: iterations {st.NumIterationsRequired} (oracle "
+          f"{so.iterations}), energy-norm error against U {e:.2e}", flush=True)
+    assert st.HasConverged and so.converged == 1
+    assert abs(st.NumIterationsRequired - so.iterations) <= max(1, round(0.1 * so.iterations))
+    assert e <= 1e-8 and energy_err(A, xo, U) <= 1e-8
+
+
+def test_cant2d_mgpcg_block_columns(c256):
+    """K = 4 right-hand sides through the block kernels (block Gauss-Seidel over the node pairs, the
+    block V-cycle and PCG): every column converges and agrees with its single solve."""
+    m, A, U, b = c256
+    rng = np.random.default_rng(20261204)
+    F = np.vstack([b] + [rng.normal(size=len(b)) for _ in range(3)])
+    s = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-9, hierarchy="reference", nrhs=4,
+                       block_width=4)
+    s.Initialize(F)
+    assert s.BlockWidth[0] == 4
+    X, st = s.Solve()
+    one = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-9, hierarchy=s.hierarchy)
+    for j in range(4):
+        one.Initialize(F[j])
+        x1, s1 = one.Solve()
+        e = energy_err(A, X[j], x1)
+        print(f"cantilever block column {j}: iterations {st[j].NumIterationsRequired} (single "
+              f"{s1.NumIterationsRequired}), energy-norm difference {e:.2e}", flush=True)
+        assert st[j].HasConverged and abs(st[j].NumIterationsRequired - s1.NumIterationsRequired) <= 1
+        assert e <= 1e-8
+
+
+def test_cant2d_4096x256_manufactured(gpu):
+    """The reference's largest 2D cantilever (GMCantileverTests.cs:108, 2.1 M dofs): MG-PCG with
+    Gauss-Seidel over 6 levels reproduces the manufactured solution U_f (K_ff U_f = b exactly) to
+    1e-7 in the energy norm.  The precision form may not claim a solution it does not have: in the
+    bent beam the rows of K_ff U_f cancel to a small fraction of sum |K_ij U_j| (most of b is zero),
+    finer than FP32 resolves, so its inner solves can stall on the bending mode; if it reports
+    convergence, the solution must meet the same bound."""
+    m = FemCantilever2D((4096, 256), LEN)
+    b = m.rhs()
+    U = m.CalcFreeDisplacements()
+    K = m.device_matrix().to_host()
+    for precision in ("fp64", "mixed"):
+        s = HipMgPcgSolver(m, levels=6, smoother="gs", nu=2, tolerance=1e-10, hierarchy="reference",
+                           precision=precision)
+        s.Initialize(b)
+        x, st = s.Solve()
+        e = energy_err(K, x, U)
+        print(f"cantilever 4096x256 MG-PCG {precision}: iterations {st.NumIterationsRequired}, "
+              f"{st.SolveMs:.1f} ms, energy-norm error {e:.2e}", flush=True)
+        if precision == "fp64":
+            assert st.HasConverged and e <= 1e-7
+        else:
+            assert not st.HasConverged or e <= 1e-7
