@@ -559,7 +559,7 @@ def _levels(levels):
     return arr, keep
 
 
-def gmg_solve(levels, b, pre=None, smoother=0, level_iterations=4, max_cycles=10000, tol=1e-6, x0=None):
+def gmg_solve(levels, b, pre=None, smoother=0, level_iterations=4, max_cycles=10000, tol=1e-6, x0=None, omp=False):
     arr, keep = _levels(levels)
     L = len(levels)
     pre_arr = (C.c_void_p * L)()
@@ -571,17 +571,17 @@ def gmg_solve(levels, b, pre=None, smoother=0, level_iterations=4, max_cycles=10
             pre_arr[i] = p.ctypes.data
     x = np.zeros(levels[0]["A"].nrows) if x0 is None else np.array(x0, np.float64)
     st = OrcStats()
-    rc = lib().orc_gmg_solve(L, arr, pre_arr, smoother, L - 1, level_iterations, max_cycles, C.c_double(tol),
+    rc = lib(omp).orc_gmg_solve(L, arr, pre_arr, smoother, L - 1, level_iterations, max_cycles, C.c_double(tol),
                              _p(np.ascontiguousarray(b, np.float64)), _p(x), int(x0 is not None), C.byref(st))
     if rc != 0:
         raise RuntimeError("orc_gmg_solve failed: %d" % rc)
     return x, st
 
 
-def pcg_jacobi(A, b, max_it=10000, tol=1e-5, x0=None):
+def pcg_jacobi(A, b, max_it=10000, tol=1e-5, x0=None, omp=False):
     x = np.zeros(A.nrows) if x0 is None else np.array(x0, np.float64)
     st = OrcStats()
-    lib().orc_pcg_jacobi(A.nrows, _p(A.rowptr), _p(A.col), _p(A.val), _p(np.ascontiguousarray(b, np.float64)),
+    lib(omp).orc_pcg_jacobi(A.nrows, _p(A.rowptr), _p(A.col), _p(A.val), _p(np.ascontiguousarray(b, np.float64)),
                          _p(x), int(x0 is not None), max_it, C.c_double(tol), C.byref(st))
     return x, st
 
