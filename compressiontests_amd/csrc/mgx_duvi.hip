@@ -410,20 +410,29 @@ __global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t n
     __syncthreads();
     const uint8_t *sd = (const uint8_t *)sq + (d0 & 15);              // sd[k]: distance byte of slot d0 + k
     if (row < r1) {
-        int sl = s_rd[r];
-        const int be = s_rb[r + 1];
-        for (int j = s_rb[r]; j < be; ++j) {
-            const int c = sd[sl];                                     // further slots of block j
-            if (((j - s_rb[r]) % G) == lane) {
-                int col = sbc[j];
-                scol[sl] = col;
-                for (int i = 1; i <= c; ++i) {
-                    col += sd[sl + i] + 1;
-                    scol[sl + i] = col;
-                }
+        // the row's lanes walk its block chain together (the count byte of each block) and lane g
+        // notes the first slots of its blocks g and g + G; then every lane decodes its blocks at
+        // once (a block past 2 G per row, none in the stiffness matrices, is decoded in the walk)
+        auto decode = [&](int j, int s0) {
+            int col = sbc[j];
+            const int c = sd[s0];
+            scol[s0] = col;
+            for (int i = 1; i <= c; ++i) {
+                col += sd[s0 + i] + 1;
+                scol[s0 + i] = col;
             }
-            sl += c + 1;
+        };
+        int sl = s_rd[r], own0 = -1, own1 = -1;
+        const int jb = s_rb[r], be = s_rb[r + 1];
+        for (int j = jb; j < be; ++j) {
+            const int k = j - jb;
+            if (k == lane) own0 = sl;
+            else if (k == lane + G) own1 = sl;
+            else if (k >= 2 * G && k % G == lane) decode(j, sl);
+            sl += sd[sl] + 1;
         }
+        if (own0 >= 0) decode(jb + lane, own0);
+        if (own1 >= 0) decode(jb + lane + G, own1);
     }
     __syncthreads();
     // (3) columns of this thread's slots, x, products
