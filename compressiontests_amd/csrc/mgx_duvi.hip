@@ -413,12 +413,21 @@ __global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t n
         // the row's lanes walk its block chain together (the count byte of each block) and lane g
         // notes the first slots of its blocks g and g + G; then every lane decodes its blocks at
         // once (a block past 2 G per row, none in the stiffness matrices, is decoded in the walk)
+        // (the distance bytes of a block's further slots are read four at a time, as 32-bit words
+        // of the staging area)
+        const uint32_t *sw = (const uint32_t *)buf;
+        const int boff = Sh::DIST_OFF + (int)(d0 & 15);              // byte offset of slot 0 in buf
         auto decode = [&](int j, int s0) {
             int col = sbc[j];
             const int c = sd[s0];
             scol[s0] = col;
+            int bo = boff + s0 + 1, wi = bo >> 2, left = 4 - (bo & 3);
+            uint32_t wv = sw[wi] >> (8 * (bo & 3));
             for (int i = 1; i <= c; ++i) {
-                col += sd[s0 + i] + 1;
+                if (left == 0) { wv = sw[++wi]; left = 4; }
+                col += (int)(wv & 0xffu) + 1;
+                wv >>= 8;
+                --left;
                 scol[s0 + i] = col;
             }
         };
@@ -428,11 +437,18 @@ __global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t n
             const int k = j - jb;
             if (k == lane) own0 = sl;
             else if (k == lane + G) own1 = sl;
-            else if (k >= 2 * G && k % G == lane) decode(j, sl);
             sl += sd[sl] + 1;
         }
         if (own0 >= 0) decode(jb + lane, own0);
         if (own1 >= 0) decode(jb + lane + G, own1);
+        if (be - jb > 2 * G) {                                        // blocks past 2 G: a second walk
+            sl = s_rd[r];
+            for (int j = jb; j < be; ++j) {
+                const int k = j - jb;
+                if (k >= 2 * G && k % G == lane) decode(j, sl);
+                sl += sd[sl] + 1;
+            }
+        }
     }
     __syncthreads();
     // (3) columns of this thread's slots, x, products
