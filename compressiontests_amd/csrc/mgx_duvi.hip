@@ -98,7 +98,7 @@ __device__ __forceinline__ T dw_value(const uint8_t *si, const T *__restrict__ v
     return values[((const uint32_t *)si)[k]];
 }
 
-template <int VIB> __device__ __forceinline__ uint32_t dw_index(const uint8_t *si, int k)
+template <int VIB> __device__ __forceinline__ uint32_t dw_index(const uint8_t *si, int64_t k)
 {
     if (VIB == 1) return __builtin_nontemporal_load(si + k);
     if (VIB == 2) return __builtin_nontemporal_load((const uint16_t *)si + k);
@@ -381,13 +381,23 @@ __global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t n
 #pragma unroll
     for (int i = 0; i < Sh::NBT; ++i) bc[i] = block_col[min(b0 + t + kBlock * i, (int)nblocks - 1)];
     // raw values stream in now; value indices too, their table lookups wait for the gather of x
+    // (every thread loads all E slots, clamped into the stream, and drops those past the tile: no
+    // branch per load)
     T w[E];
     uint32_t vx[E];
 #pragma unroll
     for (int j = 0; j < E; ++j) {
         const int k = t + kBlock * j;
-        if constexpr (VIB == 0) w[j] = k < cnt ? __builtin_nontemporal_load(values + d0 + k) : T(0);
-        else vx[j] = k < cnt ? dw_index<VIB>((const uint8_t *)vi + d0 * VIB, k) : 0u;
+        const int64_t ks = min<int64_t>(d0 + k, nnz - 1);
+        if constexpr (VIB == 0) w[j] = __builtin_nontemporal_load(values + ks);
+        else vx[j] = dw_index<VIB>((const uint8_t *)vi, ks);
+    }
+#pragma unroll
+    for (int j = 0; j < E; ++j) {
+        if (t + kBlock * j >= cnt) {
+            if constexpr (VIB == 0) w[j] = T(0);
+            else vx[j] = 0u;
+        }
     }
     // (2) to LDS, then the decode
     int32_t *scol = (int32_t *)buf;
@@ -456,7 +466,8 @@ __global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t n
 #pragma unroll
     for (int j = 0; j < E; ++j) {
         const int k = t + kBlock * j;
-        c[j] = k < cnt ? scol[k] : 0;
+        c[j] = scol[k];                                            // k < C: inside the buffer
+        if (k >= cnt) c[j] = 0;
     }
     __syncthreads();                                                  // the products overwrite the columns
     T xv[E];
@@ -464,10 +475,16 @@ __global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t n
     for (int j = 0; j < E; ++j) xv[j] = x[c[j]];
     if constexpr (VIB == 1) {
 #pragma unroll
-        for (int j = 0; j < E; ++j) w[j] = t + kBlock * j < cnt ? s_tab[vx[j]] : T(0);
+        for (int j = 0; j < E; ++j) {
+            w[j] = s_tab[vx[j]];                                   // index 0 past the tile
+            if (t + kBlock * j >= cnt) w[j] = T(0);
+        }
     } else if constexpr (VIB != 0) {
 #pragma unroll
-        for (int j = 0; j < E; ++j) w[j] = t + kBlock * j < cnt ? values[vx[j]] : T(0);
+        for (int j = 0; j < E; ++j) {
+            w[j] = values[vx[j]];                                  // index 0 past the tile
+            if (t + kBlock * j >= cnt) w[j] = T(0);
+        }
     }
     T *sp = (T *)buf;
 #pragma unroll
