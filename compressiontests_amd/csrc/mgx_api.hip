@@ -1212,6 +1212,12 @@ static bool nb_ready(const DevNb &nb)
     return nb.built() && nb.fp32 == std::is_same<T, float>::value;
 }
 
+// the value table of a DuVi level matrix in the block kernels' precision
+template <typename T> static const T *duvi_vals(const DevDuvi &D);
+template <> const double *duvi_vals<double>(const DevDuvi &D) { return D.values.p; }
+template <> const float *duvi_vals<float>(const DevDuvi &D) { return D.values32.p; }
+static bool has_duvi(const DevDuvi &D) { return D.d.nrows > 0; }
+
 template <typename T>
 static bool nt_ready(const DevNt &nt)
 {
@@ -1252,6 +1258,9 @@ static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, b
         if (nbk)
             launch_nb_gs<T>(K, V.nbg.view, V.color_off[c], V.color_off[c + 1], b, x, backward,
                             ((launches - 1) & 1) != 0, done, S->stream, mode);
+        else if (has_duvi(V.dA))
+            launch_duvi_gs_block<T>(K, V.color_count[c], V.color_nodes[c].p, V.node3 ? nullptr : V.gptr.p, V.dA.d,
+                                    duvi_vals<T>(V.dA), b, x, backward, done, S->stream);
         else if (V.node3)
             launch_gs_node<T>(K, V.color_count[c], V.color_nodes[c].p, V.A.rp, V.A.ci, Av, b, x, backward, done,
                               S->stream);
@@ -1273,6 +1282,10 @@ static void level_spmm(Level &V, int K, int epi, const TV *val, const TX *X, con
                        const int32_t *done, double *partials, hipStream_t s)
 {
     if constexpr (std::is_same<TV, TX>::value) {
+        if (has_duvi(V.dA)) {
+            launch_duvi_spmm<TX>(K, epi, V.dA.d, duvi_vals<TX>(V.dA), X, B, pre, Y, done, partials, kPartialStride, s);
+            return;
+        }
         if (nb_ready<TX>(V.nb)) {
             launch_nb_spmm<TX>(K, epi, V.nb.view, X, B, pre, Y, done, partials, kPartialStride, s,
                                V.tile_order_npb == nb_tile<TX>(K) ? V.tile_order.p : nullptr);
@@ -1293,6 +1306,7 @@ static void level_spmm(Level &V, int K, int epi, const TV *val, const TX *X, con
 template <typename T>
 static int level_spmm_parts(const Level &V, int K)
 {
+    if (has_duvi(V.dA)) return duvi_spmm_grid(K, V.n);
     if (nb_ready<T>(V.nb)) return nb_spmm_grid<T>(K, V.nb.view.npos);
     return V.node3 ? spmm_node_grid(K, V.n / 3) : spmm_grid(K, V.n);
 }
@@ -1337,11 +1351,17 @@ static void vcycle_block(mgx_solver_s *S, int K, int l, const T *b, T *x, const 
     else level_spmm<T, T>(V, K, BEPI_RESID, Av, cur, b, nullptr, B::r(V), done, nullptr, s);
     Level &C = S->lv[l + 1];
     if (nt_ready<T>(V.Rn)) launch_nt_apply<T>(K, false, V.Rn.view, B::r(V), B::b(C), done, s);
+    else if (has_duvi(V.dR))
+        launch_duvi_spmm<T>(K, BEPI_SPMM, V.dR.d, duvi_vals<T>(V.dR), B::r(V), nullptr, nullptr, B::b(C), done, nullptr,
+                            0, s);
     else
         launch_spmm<T, T>(K, BEPI_SPMM, V.R.nrows, V.R.rp, V.R.ci, B::val(V.R), B::r(V), nullptr, nullptr, B::b(C),
                           done, nullptr, 0, s);
     vcycle_block<T>(S, K, l + 1, B::b(C), B::x(C), done);
     if (nt_ready<T>(V.Pn)) launch_nt_apply<T>(K, true, V.Pn.view, B::x(C), cur, done, s);
+    else if (has_duvi(V.dP))
+        launch_duvi_spmm<T>(K, BEPI_ADD, V.dP.d, duvi_vals<T>(V.dP), B::x(C), nullptr, nullptr, cur, done, nullptr, 0,
+                            s);
     else
         launch_spmm<T, T>(K, BEPI_ADD, V.P.nrows, V.P.rp, V.P.ci, B::val(V.P), B::x(C), nullptr, nullptr, cur, done,
                           nullptr, 0, s);
@@ -1635,6 +1655,7 @@ static void order_for_cache(mgx_solver_s *S, int K)
         Level &V = S->lv[l];
         V.tile_order.reset();
         V.tile_order_npb = 0;
+        if (S->opt.format == MGX_FORMAT_DUVI) continue;   // the block kernels read the DuVi layout
         // node transfers (independent of K): built once
         if (!V.Rn.built() && !V.nt_unfit && V.R.v && V.P.v) {
             const bool ok = build_nt(V.Rn, V.R.nrows, V.R.ncols, V.R.rp, V.R.ci, V.R.v, mixed, st) &&
@@ -1820,7 +1841,7 @@ static const std::vector<double> kStripCandidates = {6.0e6, 3.0e6, 1.5e6, 0.0};
 static void choose_block_width(mgx_solver_s *S)
 {
     const int bw = S->opt.block_width;
-    const bool able = S->opt.method == MGX_METHOD_MGPCG && S->opt.format == MGX_FORMAT_CSR && S->nrhs > 1;
+    const bool able = S->opt.method == MGX_METHOD_MGPCG && S->nrhs > 1;
     const bool mixed = S->opt.precision == MGX_PRECISION_MIXED;
     S->tune_ms.assign(4, 0.0);
     S->strip_ms.assign(kStripCandidates.size(), 0.0);
@@ -1847,7 +1868,7 @@ static void choose_block_width(mgx_solver_s *S)
         S->bk = bestK;
     }
     // then the X window of the strip order at that width (structured levels only), unless fixed
-    if (S->opt.strip_window < 0 && S->lv[0].gx > 0 && S->lv[0].node3) {
+    if (S->opt.strip_window < 0 && S->lv[0].gx > 0 && S->lv[0].node3 && S->opt.format == MGX_FORMAT_CSR) {
         double bestw = S->strip_bytes, bestms = 0;
         for (size_t i = 0; i < kStripCandidates.size(); ++i) {
             S->strip_bytes = kStripCandidates[i];
@@ -2679,6 +2700,7 @@ int mgx_solver_level_layout(mgx_solver s, int32_t level, int32_t *layout)
         const Level &V = s->lv[level];
         const bool mixed = s->opt.precision == MGX_PRECISION_MIXED;
         if (s->bk <= 1) *layout = MGX_LAYOUT_CSR;
+        else if (has_duvi(V.dA)) *layout = MGX_LAYOUT_DUVI;
         else if (mixed ? nb_ready<float>(V.nb) : nb_ready<double>(V.nb)) *layout = MGX_LAYOUT_NODEBLOCK;
         else *layout = V.node3 ? MGX_LAYOUT_NODE_TRIPLE : MGX_LAYOUT_CSR;
     });
@@ -2694,7 +2716,8 @@ int mgx_solver_transfer_layout(mgx_solver s, int32_t level, int32_t *layout)
         const bool mixed = s->opt.precision == MGX_PRECISION_MIXED;
         const bool nt = mixed ? nt_ready<float>(V.Rn) && nt_ready<float>(V.Pn)
                               : nt_ready<double>(V.Rn) && nt_ready<double>(V.Pn);
-        *layout = (s->bk > 1 && nt) ? MGX_LAYOUT_NODE_TRANSFER : MGX_LAYOUT_CSR;
+        *layout = (s->bk > 1 && nt) ? MGX_LAYOUT_NODE_TRANSFER
+                  : (s->bk > 1 && has_duvi(V.dR) && has_duvi(V.dP)) ? MGX_LAYOUT_DUVI : MGX_LAYOUT_CSR;
     });
 }
 

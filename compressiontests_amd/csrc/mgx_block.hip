@@ -25,69 +25,7 @@
 
 namespace mgx {
 
-template <int K> struct BShape {
-    static constexpr int S = K >= 16 ? 1 : 16 / K;   // lanes sharing one row for one column
-    static constexpr int G = K * S;                  // lanes per row
-    static constexpr int RPB = kBlock / G;           // rows per tile
-    static constexpr int CAP = RPB * 96;             // staged entries per tile
-    static constexpr int U = S == 1 ? 8 : 4;         // entries in flight per lane
-};
-
-static inline int grid_tiles(int64_t ntiles)
-{
-    int64_t nb = std::min<int64_t>(std::max<int64_t>(ntiles, 1), 2048);
-    nb = (nb + 7) & ~int64_t(7);
-    return (int)nb;
-}
-
-// Per-column block reduction.  Thread t holds values of column t % K (blockDim = 256); the
-// K column totals of slot j go to partials[(j*K + c)*stride + blockIdx.x].  Slot MAXSLOT is a
-// NaN-propagating max, the others sums.  Fixed order: deterministic.
-template <int K, int NS, int MAXSLOT>
-__device__ __forceinline__ void publish_cols(double v0, double v1, double v2, double *__restrict__ partials, int stride)
-{
-    __shared__ double red[NS][kBlock];
-    const int t = threadIdx.x;
-    __syncthreads();
-    red[0][t] = v0;
-    if constexpr (NS > 1) red[1][t] = v1;
-    if constexpr (NS > 2) red[2][t] = v2;
-    __syncthreads();
-    if (t < K * NS) {
-        const int slot = t / K, c = t % K;
-        double a = 0;
-        for (int j = c; j < kBlock; j += K) {
-            const double x = red[slot][j];
-            a = (slot == MAXSLOT) ? nanmax(a, x) : a + x;
-        }
-        partials[(size_t)(slot * K + c) * stride + blockIdx.x] = a;
-    }
-}
-
 // ------------------------------------------------------------------ SpMM
-// One lane's share of row [s, e) for column c: entries s + sl, s + sl + S, ...
-template <int K, typename TV, typename TX, typename PI, typename PV>
-__device__ __forceinline__ TV row_cols(PI cis, PV vs, int s, int e, const TX *__restrict__ X, int c, int sl)
-{
-    using Sh = BShape<K>;
-    TV a0 = 0, a1 = 0;
-    for (int k = s + sl; k < e; k += Sh::U * Sh::S) {
-        int cc[Sh::U];
-        TV w[Sh::U];
-#pragma unroll
-        for (int u = 0; u < Sh::U; ++u) {
-            const int kk = k + u * Sh::S;
-            if (kk < e) { cc[u] = cis[kk]; w[u] = vs[kk]; } else { cc[u] = -1; w[u] = 0; }
-        }
-        TX xv[Sh::U];
-#pragma unroll
-        for (int u = 0; u < Sh::U; ++u) xv[u] = cc[u] >= 0 ? X[(size_t)cc[u] * K + c] : TX(0);
-#pragma unroll
-        for (int u = 0; u < Sh::U; ++u) { if (u & 1) a1 += w[u] * (TV)xv[u]; else a0 += w[u] * (TV)xv[u]; }
-    }
-    return a0 + a1;
-}
-
 template <typename TV, typename TX, int K, int EPI>
 __global__ __launch_bounds__(kBlock) void k_spmm(int32_t nrows, const int32_t *__restrict__ rp,
                                                  const int32_t *__restrict__ ci, const TV *__restrict__ v,

@@ -590,6 +590,347 @@ __global__ __launch_bounds__(kBlock) void k_duvi_gs(int32_t nnodes, const int32_
 }
 
 // the wave form for every matrix (timing comparisons of the two forms)
+// ------------------------------------------------------------------ batching form on the DuVi layout
+// K right-hand sides at once (block vectors n x K, row-major), the matrix read once per block of
+// columns.  A row's G lanes (the CSR block kernels' shape) turn the row into (column, value) pairs
+// in LDS: the row's distance bytes and values are staged, the lanes walk its block chain together
+// (the count byte of each block) and lane l decodes its own blocks l and l + G (further blocks in a
+// second walk); then the per-row arithmetic of the CSR block kernels runs on the staged pairs.
+// A tile or row longer than the staging area is decoded by every lane straight from global memory.
+
+// columns of a row's slots [s, s + len) into scol (tile-relative slots), sd[k] = distance byte of
+// slot k; jb .. je the row's blocks; lane l of the row's G lanes
+__device__ __forceinline__ void duvi_row_columns(const uint8_t *sd, int32_t *scol, int s, int jb, int je,
+                                                 const int32_t *__restrict__ block_col, int l, int G)
+{
+    auto decode = [&](int j, int p0) {
+        int col = block_col[j];
+        const int n = sd[p0];
+        scol[p0] = col;
+        for (int i = 1; i <= n; ++i) {
+            col += sd[p0 + i] + 1;
+            scol[p0 + i] = col;
+        }
+    };
+    int pos = s, own0 = -1, own1 = -1;
+    for (int j = jb; j < je; ++j) {
+        const int k = j - jb;
+        if (k == l) own0 = pos;
+        else if (k == l + G) own1 = pos;
+        pos += sd[pos] + 1;
+    }
+    if (own0 >= 0) decode(jb + l, own0);
+    if (own1 >= 0) decode(jb + l + G, own1);
+    if (je - jb > 2 * G) {
+        pos = s;
+        for (int j = jb; j < je; ++j) {
+            const int k = j - jb;
+            if (k >= 2 * G && k % G == l) decode(j, pos);
+            pos += sd[pos] + 1;
+        }
+    }
+}
+
+// one lane's share (slots s + sl, s + sl + S, ...) of row `row` for column c, decoded from global
+// memory (rows longer than the staging area)
+template <int VIB, int K, typename TV, typename TX>
+__device__ TV duvi_row_cols_global(int row, const int32_t *__restrict__ row_blocks, const int32_t *__restrict__ row_dist,
+                                   const int32_t *__restrict__ block_col, const uint8_t *__restrict__ dist,
+                                   const void *__restrict__ vi, const TV *__restrict__ values,
+                                   const TX *__restrict__ X, int c, int sl)
+{
+    constexpr int S = BShape<K>::S;
+    TV a = 0;
+    int64_t d = row_dist[row];
+    const int64_t ds = d;
+    for (int j = row_blocks[row]; j < row_blocks[row + 1]; ++j) {
+        int col = block_col[j];
+        const int64_t dend = d + dist[d] + 1;
+        for (;;) {
+            if ((int)(d - ds) % S == sl) a += duvi_value<VIB, TV>(vi, values, d) * (TV)X[(size_t)col * K + c];
+            if (++d == dend) break;
+            col += dist[d] + 1;
+        }
+    }
+    return a;
+}
+
+template <int VIB, typename TV, typename TX, int K, int EPI>
+__global__ __launch_bounds__(kBlock) void k_duvi_spmm(int32_t nrows, const int32_t *__restrict__ row_blocks,
+                                                      const int32_t *__restrict__ row_dist,
+                                                      const int32_t *__restrict__ block_col,
+                                                      const uint8_t *__restrict__ dist, const void *__restrict__ vi,
+                                                      const TV *__restrict__ values, const TX *__restrict__ X,
+                                                      const TX *__restrict__ B, const TV *__restrict__ pre,
+                                                      TX *__restrict__ Y, const int32_t *__restrict__ done,
+                                                      double *__restrict__ partials, int stride)
+{
+    using Sh = BShape<K>;
+    __shared__ int32_t s_ci[Sh::CAP];
+    __shared__ TV s_v[Sh::CAP];
+    __shared__ uint8_t s_d[Sh::CAP];
+    if (done && *done) return;
+    const int t = threadIdx.x, g = t / Sh::G, l = t % Sh::G, c = l % K, sl = l / K;
+    const int ntiles = (nrows + Sh::RPB - 1) / Sh::RPB;
+    const int per = gridDim.x >> 3, xcd = blockIdx.x & 7;
+    const int hi = (int)((int64_t)ntiles * (xcd + 1) / 8);
+    double dot = 0;
+    for (int tile = (int)((int64_t)ntiles * xcd / 8) + (int)(blockIdx.x >> 3); tile < hi; tile += per) {
+        const int r0 = tile * Sh::RPB, r1 = min(r0 + Sh::RPB, nrows);
+        const int64_t d0 = row_dist[r0], d1 = row_dist[r1];
+        const bool staged = d1 - d0 <= Sh::CAP;     // the same for the whole block
+        const int row = r0 + g;
+        if (staged) {
+            const int cnt = (int)(d1 - d0);
+            __syncthreads();                        // the previous tile's readers are done
+            for (int k = t; k < cnt; k += kBlock) {
+                s_d[k] = dist[d0 + k];
+                s_v[k] = duvi_value<VIB, TV>(vi, values, d0 + k);
+            }
+            __syncthreads();
+            if (row < r1)
+                duvi_row_columns(s_d, s_ci, (int)(row_dist[row] - d0), row_blocks[row], row_blocks[row + 1], block_col,
+                                 l, Sh::G);
+            __syncthreads();
+        }
+        if (row < r1) {
+            const int64_t s = row_dist[row], e = row_dist[row + 1];
+            TV acc = staged ? row_cols<K, TV, TX>(s_ci, s_v, (int)(s - d0), (int)(e - d0), X, c, sl)
+                            : duvi_row_cols_global<VIB, K, TV, TX>(row, row_blocks, row_dist, block_col, dist, vi,
+                                                                   values, X, c, sl);
+#pragma unroll
+            for (int o = K; o < Sh::G; o <<= 1) acc += __shfl_xor(acc, o, Sh::G);
+            if (sl == 0) {
+                const size_t i = (size_t)row * K + c;
+                if (EPI == BEPI_SPMM) Y[i] = (TX)acc;
+                else if (EPI == BEPI_ADD) Y[i] = (TX)((TV)Y[i] + acc);
+                else if (EPI == BEPI_RESID) Y[i] = (TX)((TV)B[i] - acc);
+                else if (EPI == BEPI_JACOBI) Y[i] = (TX)((TV)X[i] + pre[row] * ((TV)B[i] - acc));
+                else {
+                    const TX q = (TX)acc;
+                    Y[i] = q;
+                    dot += (double)X[i] * (double)q;
+                }
+            }
+        }
+    }
+    if (EPI == BEPI_DOT) publish_cols<K, 1, -1>(dot, 0, 0, partials, stride);
+}
+
+int duvi_spmm_grid(int K, int32_t nrows)
+{
+    const int rpb = kBlock / (K >= 16 ? K : 16);
+    return grid_tiles((nrows + rpb - 1) / rpb);
+}
+
+template <typename T>
+void launch_duvi_spmm(int K, int epi, const mgx_duvi &A, const T *values, const T *X, const T *B, const T *pre, T *Y,
+                      const int32_t *done, double *partials, int stride, hipStream_t s)
+{
+    if (A.nrows <= 0) return;
+    const int grid = duvi_spmm_grid(K, A.nrows);
+#define MGX_DSM(V, KK, EE)                                                                                      \
+    hipLaunchKernelGGL((k_duvi_spmm<V, T, T, KK, EE>), dim3(grid), dim3(kBlock), 0, s, A.nrows, A.row_blocks,   \
+                       A.row_dist, A.block_col, A.dist, A.vi, values, X, B, pre, Y, done, partials, stride)
+#define MGX_DSM_E(V, KK)                                         \
+    switch (epi) {                                               \
+    case BEPI_SPMM: MGX_DSM(V, KK, BEPI_SPMM); break;            \
+    case BEPI_ADD: MGX_DSM(V, KK, BEPI_ADD); break;              \
+    case BEPI_RESID: MGX_DSM(V, KK, BEPI_RESID); break;          \
+    case BEPI_JACOBI: MGX_DSM(V, KK, BEPI_JACOBI); break;        \
+    default: MGX_DSM(V, KK, BEPI_DOT); break;                    \
+    }
+#define MGX_DSM_K(V)                                                                           \
+    switch (K) {                                                                               \
+    case 4: MGX_DSM_E(V, 4); break;                                                            \
+    case 8: MGX_DSM_E(V, 8); break;                                                            \
+    case 16: MGX_DSM_E(V, 16); break;                                                          \
+    case 32: MGX_DSM_E(V, 32); break;                                                          \
+    default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};      \
+    }
+    switch (A.vi_bytes) {
+    case 1: MGX_DSM_K(1); break;
+    case 2: MGX_DSM_K(2); break;
+    case 4: MGX_DSM_K(4); break;
+    default: MGX_DSM_K(0); break;
+    }
+#undef MGX_DSM_K
+#undef MGX_DSM_E
+#undef MGX_DSM
+    MGX_HIP(hipGetLastError());
+}
+template void launch_duvi_spmm<double>(int, int, const mgx_duvi &, const double *, const double *, const double *,
+                                       const double *, double *, const int32_t *, double *, int, hipStream_t);
+template void launch_duvi_spmm<float>(int, int, const mgx_duvi &, const float *, const float *, const float *,
+                                      const float *, float *, const int32_t *, double *, int, hipStream_t);
+
+// Multicolour Gauss-Seidel for K columns on the DuVi layout: the node-group arithmetic of the CSR
+// kernel (a block of kDgbBlock threads relaxes N nodes; thread (m, a, s, c) sums row a of node m
+// for column c over the slots s, s + S, ... outside the node, and collects the node's diagonal
+// block; one thread per (node, column) then relaxes the node's rows in order, the reverse order
+// when backward), with each row's (column, value) pairs decoded into LDS first by its G lanes (all
+// in one wavefront).  With gptr, node q is the group of one to three rows gptr[q] .. gptr[q+1].
+constexpr int kDgbBlock = 192;
+constexpr int kDgbRow = 96;                          // staged slots per row
+
+template <int VIB, typename T, int K>
+__global__ __launch_bounds__(kDgbBlock) void k_duvi_gs_block(int32_t nnodes, const int32_t *__restrict__ nodes,
+                                                             const int32_t *__restrict__ gptr,
+                                                             const int32_t *__restrict__ row_blocks,
+                                                             const int32_t *__restrict__ row_dist,
+                                                             const int32_t *__restrict__ block_col,
+                                                             const uint8_t *__restrict__ dist,
+                                                             const void *__restrict__ vi, const T *__restrict__ values,
+                                                             const T *__restrict__ B, T *__restrict__ X, int backward,
+                                                             const int32_t *__restrict__ done)
+{
+    constexpr int S = BShape<K>::S, G = BShape<K>::G, N = kDgbBlock / (3 * G), U = S == 1 ? 8 : 4;
+    static_assert(64 % G == 0, "a row's lanes lie in one wavefront");
+    __shared__ T sh[N][3][K];
+    __shared__ T dg[N][3][3];
+    __shared__ int32_t s_ci[N][3][kDgbRow];
+    __shared__ T s_v[N][3][kDgbRow];
+    __shared__ uint8_t s_d[N][3][kDgbRow];
+    if (done && *done) return;
+    const int base = xcd_remap(blockIdx.x, gridDim.x) * N;
+    const int t = threadIdx.x, l = t % G, c = l % K, sl = l / K, a = (t / G) % 3, m = t / (3 * G);
+    const int nd = base + m;
+    T sa = 0, e0 = 0, e1 = 0, e2 = 0;
+    int r0 = 0, nr = 0;
+    if (nd < nnodes) {
+        const int q = nodes[nd];
+        r0 = gptr ? gptr[q] : 3 * q;
+        nr = gptr ? gptr[q + 1] - r0 : 3;
+    }
+    auto take = [&](int col, T w) {
+        const unsigned off = (unsigned)(col - r0);
+        if (off < (unsigned)nr) { if (off == 0) e0 += w; else if (off == 1) e1 += w; else e2 += w; }
+        else sa += w * X[(size_t)col * K + c];
+    };
+    if (a < nr) {
+        const int row = r0 + a;
+        const int64_t ds = row_dist[row], de = row_dist[row + 1];
+        const int len = (int)(de - ds);
+        if (len <= kDgbRow) {
+            uint8_t *sd = s_d[m][a];
+            int32_t *sc = s_ci[m][a];
+            T *sv = s_v[m][a];
+            for (int k = l; k < len; k += G) {
+                sd[k] = dist[ds + k];
+                sv[k] = duvi_value<VIB, T>(vi, values, ds + k);
+            }
+            dw_fence();                                          // the row's lanes: one wavefront
+            duvi_row_columns(sd, sc, 0, row_blocks[row], row_blocks[row + 1], block_col, l, G);
+            dw_fence();
+            for (int k = sl; k < len; k += U * S) {
+                int cc[U];
+                T w[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const int kk = k + u * S;
+                    cc[u] = kk < len ? sc[kk] : -1;
+                    w[u] = kk < len ? sv[kk] : T(0);
+                }
+                T xv[U];
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    const unsigned off = (unsigned)(cc[u] - r0);
+                    xv[u] = (cc[u] >= 0 && off >= (unsigned)nr) ? X[(size_t)cc[u] * K + c] : T(0);
+                }
+#pragma unroll
+                for (int u = 0; u < U; ++u) {
+                    if (cc[u] < 0) continue;
+                    const unsigned off = (unsigned)(cc[u] - r0);
+                    if (off < (unsigned)nr) { if (off == 0) e0 += w[u]; else if (off == 1) e1 += w[u]; else e2 += w[u]; }
+                    else sa += w[u] * xv[u];
+                }
+            }
+        } else {
+            int64_t d = ds;
+            for (int j = row_blocks[row]; j < row_blocks[row + 1]; ++j) {
+                int col = block_col[j];
+                const int64_t dend = d + dist[d] + 1;
+                for (;;) {
+                    if ((int)(d - ds) % S == sl) take(col, duvi_value<VIB, T>(vi, values, d));
+                    if (++d == dend) break;
+                    col += dist[d] + 1;
+                }
+            }
+        }
+    }
+#pragma unroll
+    for (int o = K; o < G; o <<= 1) {
+        sa += __shfl_xor(sa, o, G);
+        e0 += __shfl_xor(e0, o, G);
+        e1 += __shfl_xor(e1, o, G);
+        e2 += __shfl_xor(e2, o, G);
+    }
+    if (nd < nnodes && sl == 0) {
+        sh[m][a][c] = sa;
+        if (c == 0) { dg[m][a][0] = e0; dg[m][a][1] = e1; dg[m][a][2] = e2; }
+    }
+    __syncthreads();
+    if (t < N * K) {
+        const int m2 = t / K, c2 = t % K, nd2 = base + m2;
+        if (nd2 < nnodes) {
+            const int q = nodes[nd2], g0 = gptr ? gptr[q] : 3 * q, n2 = gptr ? gptr[q + 1] - g0 : 3;
+            const size_t q0 = (size_t)g0 * K + c2;
+            const T(*D)[3] = dg[m2];
+            T xv[3], bv[3];
+#pragma unroll
+            for (int r = 0; r < 3; ++r) {
+                xv[r] = r < n2 ? X[q0 + r * K] : T(0);
+                bv[r] = r < n2 ? B[q0 + r * K] - sh[m2][r][c2] : T(0);
+            }
+#pragma unroll
+            for (int t2 = 0; t2 < 3; ++t2) {
+                const int r = backward ? 2 - t2 : t2;
+                if (r >= n2) continue;
+                T y = bv[r];
+#pragma unroll
+                for (int cc2 = 0; cc2 < 3; ++cc2)
+                    if (cc2 != r) y -= D[r][cc2] * xv[cc2];     // zero past the group's rows
+                xv[r] = y / D[r][r];
+            }
+#pragma unroll
+            for (int r = 0; r < 3; ++r)
+                if (r < n2) X[q0 + r * K] = xv[r];
+        }
+    }
+}
+
+template <typename T>
+void launch_duvi_gs_block(int K, int32_t nnodes, const int32_t *nodes, const int32_t *gptr, const mgx_duvi &A,
+                          const T *values, const T *B, T *X, bool backward, const int32_t *done, hipStream_t s)
+{
+    if (nnodes <= 0) return;
+#define MGX_DGB(V, KK)                                                                                          \
+    hipLaunchKernelGGL((k_duvi_gs_block<V, T, KK>), dim3(grid_rows(nnodes, kDgbBlock / (3 * BShape<KK>::G))),  \
+                       dim3(kDgbBlock), 0, s, nnodes, nodes, gptr, A.row_blocks, A.row_dist, A.block_col, A.dist, \
+                       A.vi, values, B, X, backward ? 1 : 0, done)
+#define MGX_DGB_K(V)                                                                           \
+    switch (K) {                                                                               \
+    case 4: MGX_DGB(V, 4); break;                                                              \
+    case 8: MGX_DGB(V, 8); break;                                                              \
+    case 16: MGX_DGB(V, 16); break;                                                            \
+    case 32: MGX_DGB(V, 32); break;                                                            \
+    default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};      \
+    }
+    switch (A.vi_bytes) {
+    case 1: MGX_DGB_K(1); break;
+    case 2: MGX_DGB_K(2); break;
+    case 4: MGX_DGB_K(4); break;
+    default: MGX_DGB_K(0); break;
+    }
+#undef MGX_DGB_K
+#undef MGX_DGB
+    MGX_HIP(hipGetLastError());
+}
+template void launch_duvi_gs_block<double>(int, int32_t, const int32_t *, const int32_t *, const mgx_duvi &,
+                                           const double *, const double *, double *, bool, const int32_t *, hipStream_t);
+template void launch_duvi_gs_block<float>(int, int32_t, const int32_t *, const int32_t *, const mgx_duvi &,
+                                          const float *, const float *, float *, bool, const int32_t *, hipStream_t);
+
 static bool g_duvi_wave_only = false;
 void set_duvi_wave_only(bool on) { g_duvi_wave_only = on; }
 

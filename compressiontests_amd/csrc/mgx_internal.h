@@ -88,6 +88,16 @@ template <typename T>
 void launch_duvi(const mgx_duvi &A, const T *values, int epi, const T *x, const T *b, const T *pre, T *y,
                  int32_t *flag, double tol, const int32_t *done, hipStream_t s);
 void set_duvi_wave_only(bool on);   // timing comparisons: the wave form for long rows too
+// batching form on the DuVi layout: Y (op)= A X with K columns (BlockEpi epilogues; BEPI_DOT publishes
+// per-column X.Y partials of duvi_spmm_grid(K, nrows) blocks), and one colour of a multicolour
+// Gauss-Seidel sweep for K columns (node triples, or the row groups gptr)
+template <typename T>
+void launch_duvi_spmm(int K, int epi, const mgx_duvi &A, const T *values, const T *X, const T *B, const T *pre, T *Y,
+                      const int32_t *done, double *partials, int stride, hipStream_t s);
+int duvi_spmm_grid(int K, int32_t nrows);
+template <typename T>
+void launch_duvi_gs_block(int K, int32_t nnodes, const int32_t *nodes, const int32_t *gptr, const mgx_duvi &A,
+                          const T *values, const T *B, T *X, bool backward, const int32_t *done, hipStream_t s);
 // one colour of a multicolour Gauss-Seidel sweep on the DuVi layout (nodes of 3 rows)
 template <typename T>
 void launch_duvi_gs(const mgx_duvi &A, const T *values, int32_t nnodes, const int32_t *nodes, const T *b, T *x,

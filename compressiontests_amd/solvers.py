@@ -320,12 +320,13 @@ class HipMgPcgSolver:
     0 = whole planes, < 0 = chosen at Initialize by timing the candidates.
     gs_pairs: block Gauss-Seidel relaxes two colours per launch, line by line, where the colouring
     allows it (the same sweep); False = one launch per colour.
-    precision "mixed": FP32 SpMV / V-cycle inside FP64 iterative refinement."""
+    precision "mixed": FP32 SpMV / V-cycle inside FP64 iterative refinement.
+    duvi_mode (format "duvi"): 0 = the narrowest value index, 4 = 32-bit indices, 8 = raw values."""
 
     def __init__(self, model, levels=3, smoother="jacobi", omega=0.0, nu=2, tolerance=1e-8, stop="rel",
                  max_iterations=1000, hierarchy="galerkin", format="csr", profile=False, check_every=4, nrhs=1,
                  use_graph=True, precision="fp64", inner_tolerance=1e-4, inner_iterations=200, block_width=0,
-                 strip_window=-1.0, gs_pairs=True):
+                 strip_window=-1.0, gs_pairs=True, duvi_mode=0):
         self.model = model
         self.levels = levels
         opt = L.default_options(method=L.METHOD_MGPCG,
@@ -339,6 +340,7 @@ class HipMgPcgSolver:
                                 inner_tolerance=inner_tolerance, inner_iterations=inner_iterations,
                                 block_width=block_width, strip_window=float(strip_window),
                                 gs_pairs=int(bool(gs_pairs)))
+        opt.duvi_mode = duvi_mode
         self.opt = opt
         self.nrhs = nrhs
         h = _Handle(opt)
@@ -400,12 +402,12 @@ class HipMgPcgSolver:
         return self._h.block_width()
 
     def LevelLayout(self, level=0):
-        """Layout the block kernels read on a level: "csr", "node-triple" or "node-block"."""
-        return ("csr", "node-triple", "node-block")[self._h.level_layout(level)]
+        """Layout the block kernels read on a level: "csr", "node-triple", "node-block" or "duvi"."""
+        return {0: "csr", 1: "node-triple", 2: "node-block", 4: "duvi"}[self._h.level_layout(level)]
 
     def TransferLayout(self, level=0):
-        """Layout of the level's R and P in the block V-cycle: "csr" or "node-transfer"."""
-        return {0: "csr", 3: "node-transfer"}[self._h.transfer_layout(level)]
+        """Layout of the level's R and P in the block V-cycle: "csr", "node-transfer" or "duvi"."""
+        return {0: "csr", 3: "node-transfer", 4: "duvi"}[self._h.transfer_layout(level)]
 
     @property
     def StripWindow(self):

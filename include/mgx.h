@@ -393,6 +393,9 @@ int mgx_solver_level_layout(mgx_solver s, int32_t level, int32_t *layout);
 /* Layout of the level's R and P in the batching form's V-cycle: MGX_LAYOUT_NODE_TRANSFER when both
  * are node transfers (mgx_nodetransfer_from_csr), else MGX_LAYOUT_CSR.  No reference counterpart. */
 #define MGX_LAYOUT_NODE_TRANSFER 3
+/* The DuVi block layout (options.format = MGX_FORMAT_DUVI): the batching form's SpMM, Gauss-Seidel
+ * and transfers decode it, K columns per pass (level and transfer layouts). */
+#define MGX_LAYOUT_DUVI          4
 int mgx_solver_transfer_layout(mgx_solver s, int32_t level, int32_t *layout);
 /* X window (bytes) of the strip order in which the block kernels visit the nodes of structured
  * levels (0: whole planes) and, when the autotuner chose it, the ms per block PCG iteration it timed

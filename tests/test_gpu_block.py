@@ -243,6 +243,64 @@ def test_block_fp64_matches_oracle_per_column(het16, smoother, sm, nu, bw):
         assert energy_err(A, X[j], xo) < 1e-8, j
 
 
+@pytest.mark.parametrize("smoother,sm,nu,mode", [("gs", 1, 1, 0), ("gs", 1, 2, 8), ("jacobi", 0, 2, 4)])
+@pytest.mark.parametrize("bw", [4, 8])
+def test_block_duvi_matches_oracle_per_column(het16, smoother, sm, nu, mode, bw):
+    """The batching form on the DuVi layout: the block SpMM, the residual, the smoothers (K-column
+    multicolour Gauss-Seidel, Jacobi) and the transfers all decode the DuVi arrays of every level,
+    K columns per pass: mode 8 raw values, mode 4 32-bit indices, mode 0 the narrowest form (raw
+    values on level 0 with its > 65,536 distinct values, two-byte indices on the coarser levels,
+    one-byte indices on the transfers).  Every column matches the oracle's MG-PCG run in the
+    solver's node order (iterations +-10 %, 1e-8 in the energy norm) and the CSR block solve of
+    the same system (iterations +-1, 1e-9)."""
+    m, F, lv, A = het16
+    kw = dict(levels=3, smoother=smoother, omega=0.0, nu=nu, tolerance=1e-10, nrhs=6, block_width=bw)
+    s = HipMgPcgSolver(m, format="duvi", duvi_mode=mode, **kw)
+    s.Initialize(F)
+    assert s.BlockWidth[0] == bw
+    assert [s.LevelLayout(l) for l in range(2)] == ["duvi"] * 2
+    assert [s.TransferLayout(l) for l in range(2)] == ["duvi"] * 2
+    X, st = s.Solve()
+    c = HipMgPcgSolver(m, **kw)
+    c.Initialize(F)
+    Xc, sc = c.Solve()
+    if smoother == "gs":
+        lv = [dict(d, gs_order=s.GsOrder(l)) if "R" in d else d for l, d in enumerate(lv)]
+    for j in range(6):
+        xo, so = O.mgpcg(lv, F[j], smoother=sm, omega=0.0, nu=nu, tol=1e-10)
+        assert st[j].HasConverged and st[j].BlockWidth == bw
+        assert abs(st[j].NumIterationsRequired - so.iterations) <= max(1, so.iterations // 10), j
+        assert st[j].RelativeResidual <= 1.01e-10
+        assert energy_err(A, X[j], xo) < 1e-8, j
+        assert abs(st[j].NumIterationsRequired - sc[j].NumIterationsRequired) <= 1, j
+        assert energy_err(A, X[j], Xc[j]) < 1e-9, j
+
+
+@pytest.mark.parametrize("n", [16, 24])
+def test_block_duvi_homogeneous_u8_and_mixed(gpu, n):
+    """One modulus everywhere: 28 distinct stiffness values, so the encoder stores one-byte value
+    indices (the K-column kernels read the table from LDS).  The precision form (FP32 value
+    tables, FP64 true residual from the FP64 DuVi table) reaches the FP64 stop test on every
+    column and agrees with the FP64 DuVi block solve to 1e-8 in the energy norm."""
+    m = FemCantilever3D((n, n, n), rhs="tip")
+    F = m.load_cases(8, SEED_LOADS)
+    A = O.cantilever(n, n, n)
+    res = []
+    for precision in ("fp64", "mixed"):
+        s = HipMgPcgSolver(m, levels=3, smoother="gs", nu=1, tolerance=1e-10, nrhs=8, block_width=8, format="duvi",
+                           precision=precision)
+        s.Initialize(F)
+        assert s.LevelLayout(0) == "duvi"
+        X, st = s.Solve()
+        for j in range(8):
+            assert st[j].HasConverged and st[j].RelativeResidual <= 1.01e-10, (precision, j)
+            r = F[j] - S(A) @ X[j]
+            assert np.linalg.norm(r) <= 1.05e-10 * np.linalg.norm(F[j]), (precision, j)
+        res.append(X)
+    for j in range(8):
+        assert energy_err(A, res[1][j], res[0][j]) < 1e-8, j
+
+
 def test_block_columns_match_single_rhs_solves(het16):
     """A column of a block solve is the single-right-hand-side solve of that column up to
     rounding: each path matches the oracle run in its own Gauss-Seidel node order (iterations

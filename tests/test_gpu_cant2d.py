@@ -158,23 +158,26 @@ def test_cant2d_mgpcg_parity(c256, smoother, fmt):
     assert e <= 1e-8 and energy_err(A, xo, U) <= 1e-8
 
 
-def test_cant2d_mgpcg_block_columns(c256):
+@pytest.mark.parametrize("fmt", ["csr", "duvi"])
+def test_cant2d_mgpcg_block_columns(c256, fmt):
     """K = 4 right-hand sides through the block kernels (block Gauss-Seidel over the node pairs, the
-    block V-cycle and PCG): every column converges and agrees with its single solve."""
+    block V-cycle and PCG), on the CSR levels and on the DuVi layout: every column converges and
+    agrees with its single solve."""
     m, A, U, b = c256
     rng = np.random.default_rng(20261204)
     F = np.vstack([b] + [rng.normal(size=len(b)) for _ in range(3)])
     s = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-9, hierarchy="reference", nrhs=4,
-                       block_width=4)
+                       block_width=4, format=fmt)
     s.Initialize(F)
     assert s.BlockWidth[0] == 4
+    assert (s.LevelLayout(0) == "duvi") == (fmt == "duvi")
     X, st = s.Solve()
-    one = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-9, hierarchy=s.hierarchy)
+    one = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-9, hierarchy=s.hierarchy, format=fmt)
     for j in range(4):
         one.Initialize(F[j])
         x1, s1 = one.Solve()
         e = energy_err(A, X[j], x1)
-        print(f"cantilever block column {j}: iterations {st[j].NumIterationsRequired} (single "
+        print(f"cantilever block column {j} ({fmt}): iterations {st[j].NumIterationsRequired} (single "
               f"{s1.NumIterationsRequired}), energy-norm difference {e:.2e}", flush=True)
         assert st[j].HasConverged and abs(st[j].NumIterationsRequired - s1.NumIterationsRequired) <= 1
         assert e <= 1e-8

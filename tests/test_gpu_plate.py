@@ -190,28 +190,31 @@ def test_plate_mgpcg(p256, smoother, precision, fmt):
     assert e <= 1e-6
 
 
-def test_plate_mgpcg_block_columns(p256):
+@pytest.mark.parametrize("fmt", ["csr", "duvi"])
+def test_plate_mgpcg_block_columns(p256, fmt):
     """K = 4 right-hand sides through the block kernels (block Gauss-Seidel over row groups, the
-    block V-cycle and PCG), FP64: every column converges and agrees with the same column solved on
-    its own to the solve tolerance, in the same iteration count within one (the reported true
-    residual held to 1.2e-8 as in the single right-hand-side test)."""
+    block V-cycle and PCG), FP64, on the CSR levels and on the DuVi layout (the K-column DuVi
+    kernels with the row groups of 2 and 3 rows): every column converges and agrees with the same
+    column solved on its own to the solve tolerance, in the same iteration count within one (the
+    reported true residual held to 1.2e-8 as in the single right-hand-side test)."""
     m, b = p256
     n = len(b)
     rng = np.random.default_rng(20261204)
     F = np.vstack([b] + [rng.normal(size=n) * np.abs(b).max() for _ in range(3)])
     s = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-8, hierarchy="reference", nrhs=4,
-                       block_width=4)
+                       block_width=4, format=fmt)
     s.Initialize(F)
     assert s.BlockWidth[0] == 4
     assert s.GsGroups(0) is not None
+    assert (s.LevelLayout(0) == "duvi") == (fmt == "duvi")
     X, st = s.Solve()
-    one = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-8, hierarchy=s.hierarchy)
+    one = HipMgPcgSolver(m, levels=4, smoother="gs", nu=1, tolerance=1e-8, hierarchy=s.hierarchy, format=fmt)
     A0 = host_csr(s.hierarchy[0][0])
     for j in range(4):
         one.Initialize(F[j])
         x1, s1 = one.Solve()
         e = energy_err(A0, X[j], x1)
-        print(f"plate block column {j}: iterations {st[j].NumIterationsRequired} (single {s1.NumIterationsRequired}), "
+        print(f"plate block column {j} ({fmt}): iterations {st[j].NumIterationsRequired} (single {s1.NumIterationsRequired}), "
               f"energy-norm difference {e:.2e}", flush=True)
         assert st[j].HasConverged and st[j].RelativeResidual <= 1.2e-8
         assert abs(st[j].NumIterationsRequired - s1.NumIterationsRequired) <= 1
