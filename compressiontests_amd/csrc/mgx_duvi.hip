@@ -381,14 +381,14 @@ __global__ __launch_bounds__(kBlock) void k_duvi_stream(int32_t nrows, int64_t n
 #pragma unroll
     for (int i = 0; i < Sh::NBT; ++i) bc[i] = block_col[min(b0 + t + kBlock * i, (int)nblocks - 1)];
     // raw values stream in now; value indices too, their table lookups wait for the gather of x
-    // (every thread loads all E slots, clamped into the stream, and drops those past the tile: no
-    // branch per load)
+    // (every thread loads all E slots, those past the tile clamped to its last slot, and drops them:
+    // no branch per load, no bytes outside the tile)
     T w[E];
     uint32_t vx[E];
 #pragma unroll
     for (int j = 0; j < E; ++j) {
         const int k = t + kBlock * j;
-        const int64_t ks = min<int64_t>(d0 + k, nnz - 1);
+        const int64_t ks = min<int64_t>(d0 + min(k, max(cnt - 1, 0)), nnz - 1);
         if constexpr (VIB == 0) w[j] = __builtin_nontemporal_load(values + ks);
         else vx[j] = dw_index<VIB>((const uint8_t *)vi, ks);
     }
