@@ -365,14 +365,16 @@ template void launch_nb_spmm<double>(int, int, const NbView &, const double *, c
 // One wave tile [pa, pb) of the colour-ordered copy (pb - pa <= NW): stage its blocks, sum the
 // off-diagonal blocks, relax each node.  sv / sc: the wave's LDS staging region.
 // the node's 3x3 solve: x_0 from row 0 with the current x_1, x_2, then x_1, then x_2 (reverse
-// order when backward)
-template <typename T>
+// order when BW).  The direction is a template parameter: with a run-time flag the compiler merges
+// the two orders into stores through a selected address, a stack copy of x whose reload waits
+// for every memory load in flight.
+template <typename T, bool BW>
 __device__ __forceinline__ void nb_relax(const T *d, const typename NbVec<T>::type &r0,
                                          const typename NbVec<T>::type &r1, const typename NbVec<T>::type &r2,
                                          typename NbVec<T>::type &x0, typename NbVec<T>::type &x1,
-                                         typename NbVec<T>::type &x2, int backward)
+                                         typename NbVec<T>::type &x2)
 {
-    if (!backward) {
+    if constexpr (!BW) {
         x0 = (r0 - d[1] * x1 - d[2] * x2) / d[0];
         x1 = (r1 - d[3] * x0 - d[5] * x2) / d[4];
         x2 = (r2 - d[6] * x0 - d[7] * x1) / d[8];
@@ -383,11 +385,11 @@ __device__ __forceinline__ void nb_relax(const T *d, const typename NbVec<T>::ty
     }
 }
 
-template <typename T, int K, bool LOWER>
+template <typename T, int K, bool LOWER, bool BW>
 __device__ __forceinline__ void nb_gs_tile(int pa, int pb, const int32_t *__restrict__ nrp,
                                            const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
                                            const int32_t *__restrict__ split, const T *__restrict__ val,
-                                           const T *__restrict__ B, T *__restrict__ X, int backward, u32x4 *sv,
+                                           const T *__restrict__ B, T *__restrict__ X, u32x4 *sv,
                                            u32x4 *sc, int lane)
 {
     using Sh = NbShape<T, K>;
@@ -424,7 +426,7 @@ __device__ __forceinline__ void nb_gs_tile(int pa, int pb, const int32_t *__rest
     if (!LOWER) x0 = *(const VT *)(X + i0), x1 = *(const VT *)(X + i0 + K), x2 = *(const VT *)(X + i0 + 2 * K);
     const VT r0 = *(const VT *)(B + i0) - y[0], r1 = *(const VT *)(B + i0 + K) - y[1],
              r2 = *(const VT *)(B + i0 + 2 * K) - y[2];
-    nb_relax<T>(d, r0, r1, r2, x0, x1, x2, backward);
+    nb_relax<T, BW>(d, r0, r1, r2, x0, x1, x2);
     *(VT *)(X + i0) = x0;
     *(VT *)(X + i0 + K) = x1;
     *(VT *)(X + i0 + 2 * K) = x2;
@@ -434,10 +436,10 @@ __device__ __forceinline__ void nb_gs_tile(int pa, int pb, const int32_t *__rest
 // node's relaxation reads only its own block and its B rows (the off-diagonal sums are 0 and the
 // node's own X values 0, exactly what the general kernel would read).  Thread (p, v) handles the
 // W columns v*W .. v*W+W-1 of position p.
-template <typename T, int K>
+template <typename T, int K, bool BW>
 __global__ __launch_bounds__(kBlock) void k_nb_gs_zero(int32_t p0c, int32_t p1c, const int32_t *__restrict__ nrp,
                                                        const int32_t *__restrict__ node, const T *__restrict__ val,
-                                                       const T *__restrict__ B, T *__restrict__ X, int backward,
+                                                       const T *__restrict__ B, T *__restrict__ X,
                                                        const int32_t *__restrict__ done)
 {
     using VT = typename NbVec<T>::type;
@@ -455,18 +457,18 @@ __global__ __launch_bounds__(kBlock) void k_nb_gs_zero(int32_t p0c, int32_t p1c,
     const size_t i0 = (size_t)3 * q * K + c0;
     VT x0 = VT(0), x1 = VT(0), x2 = VT(0);
     const VT r0 = *(const VT *)(B + i0), r1 = *(const VT *)(B + i0 + K), r2 = *(const VT *)(B + i0 + 2 * K);
-    nb_relax<T>(d, r0, r1, r2, x0, x1, x2, backward);
+    nb_relax<T, BW>(d, r0, r1, r2, x0, x1, x2);
     *(VT *)(X + i0) = x0;
     *(VT *)(X + i0 + K) = x1;
     *(VT *)(X + i0 + 2 * K) = x2;
 }
 
-template <typename T, int K, bool LOWER>
+template <typename T, int K, bool LOWER, bool BW>
 __global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, const int32_t *__restrict__ nrp,
                                                   const int32_t *__restrict__ ncol, const int32_t *__restrict__ node,
                                                   const int32_t *__restrict__ split, const T *__restrict__ val,
-                                                  const T *__restrict__ B, T *__restrict__ X, int backward,
-                                                  int reverse, const int32_t *__restrict__ done)
+                                                  const T *__restrict__ B, T *__restrict__ X, int reverse,
+                                                  const int32_t *__restrict__ done)
 {
     using Sh = NbShape<T, K>;
     __shared__ u32x4 s_val[kNbWaves][Sh::VQ];
@@ -478,8 +480,8 @@ __global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, cons
     if (t0 >= ntiles) return;                                    // the whole wave leaves together
     const int tile = reverse ? ntiles - 1 - t0 : t0;
     const int pa = p0c + tile * Sh::NW;
-    nb_gs_tile<T, K, LOWER>(pa, min(pa + Sh::NW, p1c), nrp, ncol, node, split, val, B, X, backward, s_val[wv],
-                            s_col[wv], lane);
+    nb_gs_tile<T, K, LOWER, BW>(pa, min(pa + Sh::NW, p1c), nrp, ncol, node, split, val, B, X, s_val[wv],
+                                s_col[wv], lane);
 }
 
 // Two colours of a sweep in one launch.  Block b takes mesh line b: rng[4b] .. rng[4b+1]-1 are
@@ -493,14 +495,14 @@ __global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, cons
 // reads the same X neighbourhood as the first, from cache.
 constexpr int kPairBlock = 512;
 
-template <typename T, int K, bool LOWER>
+template <typename T, int K, bool LOWER, bool BW>
 __global__ __launch_bounds__(kPairBlock) void k_nb_gs_pair(const int32_t *__restrict__ rng, int32_t nlines,
                                                            int second_first, const int32_t *__restrict__ nrp,
                                                            const int32_t *__restrict__ ncol,
                                                            const int32_t *__restrict__ node,
                                                            const int32_t *__restrict__ split, const T *__restrict__ val,
-                                                           const T *__restrict__ B, T *__restrict__ X, int backward,
-                                                           int reverse, const int32_t *__restrict__ done)
+                                                           const T *__restrict__ B, T *__restrict__ X, int reverse,
+                                                           const int32_t *__restrict__ done)
 {
     using Sh = NbShape<T, K>;
     constexpr int NWB = kPairBlock / 64;
@@ -520,8 +522,8 @@ __global__ __launch_bounds__(kPairBlock) void k_nb_gs_pair(const int32_t *__rest
         if (wv < nt)
             for (int i = 0, last = wv + (nt - 1 - wv) / NWB * NWB; wv + i * NWB <= last; ++i) {
                 const int pa = p0 + (h == 0 ? wv + i * NWB : last - i * NWB) * Sh::NW;
-                nb_gs_tile<T, K, LOWER>(pa, min(pa + Sh::NW, p1), nrp, ncol, node, split, val, B, X, backward,
-                                        s_val[wv], s_col[wv], lane);
+                nb_gs_tile<T, K, LOWER, BW>(pa, min(pa + Sh::NW, p1), nrp, ncol, node, split, val, B, X,
+                                            s_val[wv], s_col[wv], lane);
             }
         if (h == 0) __syncthreads();
     }
@@ -537,8 +539,12 @@ void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T 
     if (start == NB_GS_ZERO) {
         const int g = grid_rows((int64_t)(p1 - p0) * (K / (16 / (int)sizeof(T))), kBlock);
 #define MGX_NBZ(KK)                                                                                              \
-    hipLaunchKernelGGL((k_nb_gs_zero<T, KK>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.node,              \
-                       (const T *)A.val, B, X, backward ? 1 : 0, done)
+    if (backward)                                                                                                \
+        hipLaunchKernelGGL((k_nb_gs_zero<T, KK, true>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.node,    \
+                           (const T *)A.val, B, X, done);                                                        \
+    else                                                                                                         \
+        hipLaunchKernelGGL((k_nb_gs_zero<T, KK, false>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.node,   \
+                           (const T *)A.val, B, X, done)
         switch (K) {
         case 4: MGX_NBZ(4); break;
         case 8: MGX_NBZ(8); break;
@@ -552,13 +558,13 @@ void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T 
     }
     const int nw = nb_tile<T>(K);
     const int g = grid_rows((p1 - p0 + nw - 1) / nw, kNbWaves);
+#define MGX_NBG1(KK, LO, BWD)                                                                                    \
+    hipLaunchKernelGGL((k_nb_gs<T, KK, LO, BWD>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.ncol, A.node,   \
+                       A.split, (const T *)A.val, B, X, reverse ? 1 : 0, done)
 #define MGX_NBG(KK)                                                                                              \
-    if (start == NB_GS_LOWER)                                                                                    \
-        hipLaunchKernelGGL((k_nb_gs<T, KK, true>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.ncol, A.node,  \
-                           A.split, (const T *)A.val, B, X, 0, reverse ? 1 : 0, done);                           \
-    else                                                                                                         \
-        hipLaunchKernelGGL((k_nb_gs<T, KK, false>), dim3(g), dim3(kBlock), 0, s, p0, p1, A.nrp, A.ncol, A.node, \
-                           A.split, (const T *)A.val, B, X, backward ? 1 : 0, reverse ? 1 : 0, done)
+    if (start == NB_GS_LOWER) MGX_NBG1(KK, true, false);                                                         \
+    else if (backward) MGX_NBG1(KK, false, true);                                                                \
+    else MGX_NBG1(KK, false, false)
     switch (K) {
     case 4: MGX_NBG(4); break;
     case 8: MGX_NBG(8); break;
@@ -567,6 +573,7 @@ void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T 
     default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
     }
 #undef MGX_NBG
+#undef MGX_NBG1
     MGX_HIP(hipGetLastError());
 }
 template <typename T>
@@ -577,15 +584,14 @@ void launch_nb_gs_pair(int K, const NbView &A, const int32_t *rng, int32_t nline
     if (lower && (!A.split || backward))
         throw Error{MGX_ERR_INVALID_OPERATION, "lower-colour sweep needs a colour-sorted copy and a forward sweep"};
     const int g = grid_rows(nlines, 1);
+#define MGX_NBP1(KK, LO, BWD)                                                                                    \
+    hipLaunchKernelGGL((k_nb_gs_pair<T, KK, LO, BWD>), dim3(g), dim3(kPairBlock), 0, s, rng, nlines,             \
+                       second_first ? 1 : 0, A.nrp, A.ncol, A.node, A.split, (const T *)A.val, B, X, reverse ? 1 : 0, \
+                       done)
 #define MGX_NBP(KK)                                                                                              \
-    if (lower)                                                                                                   \
-        hipLaunchKernelGGL((k_nb_gs_pair<T, KK, true>), dim3(g), dim3(kPairBlock), 0, s, rng, nlines,           \
-                           second_first ? 1 : 0, A.nrp, A.ncol, A.node, A.split, (const T *)A.val, B, X, 0,      \
-                           reverse ? 1 : 0, done);                                                               \
-    else                                                                                                         \
-        hipLaunchKernelGGL((k_nb_gs_pair<T, KK, false>), dim3(g), dim3(kPairBlock), 0, s, rng, nlines,          \
-                           second_first ? 1 : 0, A.nrp, A.ncol, A.node, A.split, (const T *)A.val, B, X,         \
-                           backward ? 1 : 0, reverse ? 1 : 0, done)
+    if (lower) MGX_NBP1(KK, true, false);                                                                        \
+    else if (backward) MGX_NBP1(KK, false, true);                                                                \
+    else MGX_NBP1(KK, false, false)
     switch (K) {
     case 4: MGX_NBP(4); break;
     case 8: MGX_NBP(8); break;
@@ -594,6 +600,7 @@ void launch_nb_gs_pair(int K, const NbView &A, const int32_t *rng, int32_t nline
     default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
     }
 #undef MGX_NBP
+#undef MGX_NBP1
     MGX_HIP(hipGetLastError());
 }
 
