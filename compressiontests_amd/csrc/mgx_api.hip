@@ -1249,7 +1249,7 @@ static void gs_block_sweep(mgx_solver_s *S, Level &V, int K, const T *b, T *x, b
         if (nbk && mode != NB_GS_ZERO && c2 >= 0 && (c2 >> 1) == k && k < (int)V.pair_lines.size() &&
             V.pair_lines[k] > 0) {
             launch_nb_gs_pair<T>(K, V.nbg.view, V.pair_rng[k].p, V.pair_lines[k], backward, b, x, backward,
-                                 (launches & 1) != 0, mode == NB_GS_LOWER, done, S->stream);
+                                 (launches & 1) != 0, mode == NB_GS_LOWER, done, S->stream, V.gx);
             ++launches;
             ++i;
             continue;

@@ -235,10 +235,12 @@ void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T 
                   const int32_t *done, hipStream_t s, int start = NB_GS_GENERAL);
 // two colours a, b of a sweep in one launch over nlines mesh lines: rng[4i .. 4i+3] = the
 // positions [a0, a1) of colour a and [b0, b1) of colour b on line i; colour b first when
-// second_first; lower: the NB_GS_LOWER form for both colours
+// second_first; lower: the NB_GS_LOWER form for both colours; line_nodes: nodes per mesh line (0:
+// unknown), which picks the block size
 template <typename T>
 void launch_nb_gs_pair(int K, const NbView &A, const int32_t *rng, int32_t nlines, bool second_first, const T *B,
-                       T *X, bool backward, bool reverse, bool lower, const int32_t *done, hipStream_t s);
+                       T *X, bool backward, bool reverse, bool lower, const int32_t *done, hipStream_t s,
+                       int32_t line_nodes = 0);
 // colour + upper (with diag_first): count / fill only the node's own block (slot 0) and the blocks
 // of later-coloured neighbours, in column order (the copy the zero-start sweep's residual reads)
 void nb_count(int32_t npos, const int32_t *node, const int32_t *rp, const int32_t *ci, bool diag_first, int32_t *cnt,

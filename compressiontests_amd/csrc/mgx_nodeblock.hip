@@ -493,10 +493,8 @@ __global__ __launch_bounds__(kBlock) void k_nb_gs(int32_t p0c, int32_t p1c, cons
 // line, and no other line's is a neighbour), and a node of the second colour reads the new values
 // of the first colour of its own line, written by this block before the barrier.  The second half
 // reads the same X neighbourhood as the first, from cache.
-constexpr int kPairBlock = 512;
-
-template <typename T, int K, bool LOWER, bool BW>
-__global__ __launch_bounds__(kPairBlock) void k_nb_gs_pair(const int32_t *__restrict__ rng, int32_t nlines,
+template <typename T, int K, bool LOWER, bool BW, int NB>
+__global__ __launch_bounds__(NB) void k_nb_gs_pair(const int32_t *__restrict__ rng, int32_t nlines,
                                                            int second_first, const int32_t *__restrict__ nrp,
                                                            const int32_t *__restrict__ ncol,
                                                            const int32_t *__restrict__ node,
@@ -505,7 +503,7 @@ __global__ __launch_bounds__(kPairBlock) void k_nb_gs_pair(const int32_t *__rest
                                                            const int32_t *__restrict__ done)
 {
     using Sh = NbShape<T, K>;
-    constexpr int NWB = kPairBlock / 64;
+    constexpr int NWB = NB / 64;
     __shared__ u32x4 s_val[NWB][Sh::VQ];
     __shared__ u32x4 s_col[NWB][Sh::CQ];
     if (done && *done) return;
@@ -578,25 +576,33 @@ void launch_nb_gs(int K, const NbView &A, int32_t p0, int32_t p1, const T *B, T 
 }
 template <typename T>
 void launch_nb_gs_pair(int K, const NbView &A, const int32_t *rng, int32_t nlines, bool second_first, const T *B,
-                       T *X, bool backward, bool reverse, bool lower, const int32_t *done, hipStream_t s)
+                       T *X, bool backward, bool reverse, bool lower, const int32_t *done, hipStream_t s,
+                       int32_t line_nodes)
 {
     if (nlines <= 0) return;
     if (lower && (!A.split || backward))
         throw Error{MGX_ERR_INVALID_OPERATION, "lower-colour sweep needs a colour-sorted copy and a forward sweep"};
     const int g = grid_rows(nlines, 1);
-#define MGX_NBP1(KK, LO, BWD)                                                                                    \
-    hipLaunchKernelGGL((k_nb_gs_pair<T, KK, LO, BWD>), dim3(g), dim3(kPairBlock), 0, s, rng, nlines,             \
+    // The FP32 general form at 32 columns on lines of 128 nodes or more (each of 16 waves then has
+    // a tile of each colour) runs 1024-thread blocks: 0.467 against 0.492 ms per launch at 128^3;
+    // with them the zero-start (lower) form made the batch slower.
+    const bool wide = sizeof(T) == 4 && K == 32 && !lower && line_nodes / 2 >= 16 * nb_tile<T>(K);
+#define MGX_NBP1(KK, LO, BWD, NB)                                                                                \
+    hipLaunchKernelGGL((k_nb_gs_pair<T, KK, LO, BWD, NB>), dim3(g), dim3(NB), 0, s, rng, nlines,                 \
                        second_first ? 1 : 0, A.nrp, A.ncol, A.node, A.split, (const T *)A.val, B, X, reverse ? 1 : 0, \
                        done)
-#define MGX_NBP(KK)                                                                                              \
-    if (lower) MGX_NBP1(KK, true, false);                                                                        \
-    else if (backward) MGX_NBP1(KK, false, true);                                                                \
-    else MGX_NBP1(KK, false, false)
+#define MGX_NBP(KK, NB)                                                                                          \
+    if (lower) MGX_NBP1(KK, true, false, NB);                                                                    \
+    else if (backward) MGX_NBP1(KK, false, true, NB);                                                            \
+    else MGX_NBP1(KK, false, false, NB)
     switch (K) {
-    case 4: MGX_NBP(4); break;
-    case 8: MGX_NBP(8); break;
-    case 16: MGX_NBP(16); break;
-    case 32: MGX_NBP(32); break;
+    case 4: MGX_NBP(4, 512); break;
+    case 8: MGX_NBP(8, 512); break;
+    case 16: MGX_NBP(16, 512); break;
+    case 32:
+        if (wide) MGX_NBP(32, 1024);
+        else MGX_NBP(32, 512);
+        break;
     default: throw Error{MGX_ERR_INVALID_ARGUMENT, "block width must be 4, 8, 16 or 32"};
     }
 #undef MGX_NBP
@@ -607,9 +613,9 @@ void launch_nb_gs_pair(int K, const NbView &A, const int32_t *rng, int32_t nline
 template void launch_nb_gs<float>(int, const NbView &, int32_t, int32_t, const float *, float *, bool, bool,
                                   const int32_t *, hipStream_t, int);
 template void launch_nb_gs_pair<float>(int, const NbView &, const int32_t *, int32_t, bool, const float *, float *,
-                                       bool, bool, bool, const int32_t *, hipStream_t);
+                                       bool, bool, bool, const int32_t *, hipStream_t, int32_t);
 template void launch_nb_gs_pair<double>(int, const NbView &, const int32_t *, int32_t, bool, const double *,
-                                        double *, bool, bool, bool, const int32_t *, hipStream_t);
+                                        double *, bool, bool, bool, const int32_t *, hipStream_t, int32_t);
 template void launch_nb_gs<double>(int, const NbView &, int32_t, int32_t, const double *, double *, bool, bool,
                                    const int32_t *, hipStream_t, int);
 

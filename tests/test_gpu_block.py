@@ -432,16 +432,19 @@ def test_block_visiting_order_invariance(gpu):
 
 
 @pytest.mark.parametrize("precision,nu,n,bw,window", [("fp64", 1, 24, 8, -1.0), ("fp64", 2, 24, 8, -1.0),
-                                                      ("mixed", 1, 24, 8, -1.0), ("mixed", 1, 32, 32, 1e5)])
+                                                      ("mixed", 1, 24, 8, -1.0), ("mixed", 1, 32, 32, 1e5),
+                                                      ("mixed", 2, (128, 8, 12), 32, -1.0)])
 def test_block_gs_colour_pairs_same_sweep(gpu, precision, nu, n, bw, window):
     """Two colours per launch, line by line (every edge between colours 2k and 2k+1 of the
     greedy colouring joins nodes of one x-line), is the same sweep as one launch per colour: the
     same arithmetic for every node, so the same iterations and bit-identical solutions, for the
     zero-start sweep (lower-colour pairs, nu = 1) and the general one (nu = 2), in FP64 and in the
-    precision form; the last case is the headline's kernel shape (FP32, 32 columns, strip order).
+    precision form; the last two cases are the headline's kernel shapes (FP32, 32 columns, strip
+    order), the last one on 128-node lines, where the general pair launch runs 1024-thread blocks.
     The profiled level-0 general sweeps take half the launches."""
-    dens = 10 ** (4 * O.rng_uniform(20261207, n ** 3) - 2)
-    m = FemCantilever3D((n, n, n), ElementDensities=dens, rhs="tip")
+    shape = n if isinstance(n, tuple) else (n, n, n)
+    dens = 10 ** (4 * O.rng_uniform(20261207, int(np.prod(shape))) - 2)
+    m = FemCantilever3D(shape, ElementDensities=dens, rhs="tip")
     F = m.load_cases(bw, SEED_LOADS)
     res = []
     for pairs in (False, True):
