@@ -802,11 +802,6 @@ __global__ __launch_bounds__(kDgbBlock) void k_duvi_gs_block(int32_t nnodes, con
         r0 = gptr ? gptr[q] : 3 * q;
         nr = gptr ? gptr[q + 1] - r0 : 3;
     }
-    auto take = [&](int col, T w) {
-        const unsigned off = (unsigned)(col - r0);
-        if (off < (unsigned)nr) { if (off == 0) e0 += w; else if (off == 1) e1 += w; else e2 += w; }
-        else sa += w * X[(size_t)col * K + c];
-    };
     if (a < nr) {
         const int row = r0 + a;
         const int64_t ds = row_dist[row], de = row_dist[row + 1];
@@ -851,7 +846,12 @@ __global__ __launch_bounds__(kDgbBlock) void k_duvi_gs_block(int32_t nnodes, con
                 int col = block_col[j];
                 const int64_t dend = d + dist[d] + 1;
                 for (;;) {
-                    if ((int)(d - ds) % S == sl) take(col, duvi_value<VIB, T>(vi, values, d));
+                    if ((int)(d - ds) % S == sl) {
+                        const T w = duvi_value<VIB, T>(vi, values, d);
+                        const unsigned off = (unsigned)(col - r0);
+                        if (off < (unsigned)nr) { if (off == 0) e0 += w; else if (off == 1) e1 += w; else e2 += w; }
+                        else sa += w * X[(size_t)col * K + c];
+                    }
                     if (++d == dend) break;
                     col += dist[d] + 1;
                 }
