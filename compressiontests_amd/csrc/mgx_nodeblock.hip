@@ -642,6 +642,12 @@ __global__ __launch_bounds__(kBlock) void k_nt_apply(int32_t nout, const int32_t
     const int64_t i = t / V;
     if (i >= nout) return;
     const int c0 = (int)(t - i * V) * W;
+    VT *yp = (VT *)(Y + (size_t)3 * i * K + c0);
+    VT yv[3] = {VT(0), VT(0), VT(0)};
+    if (ADD) {                                                   // issued with the first loads
+#pragma unroll
+        for (int a = 0; a < 3; ++a) yv[a] = yp[a * V];
+    }
     const int b1 = rp[i + 1];
     int b = rp[i];
     VT acc[3] = {VT(0), VT(0), VT(0)};
@@ -667,11 +673,7 @@ __global__ __launch_bounds__(kBlock) void k_nt_apply(int32_t nout, const int32_t
         for (int a = 0; a < 3; ++a) acc[a] += wv * *(const VT *)(X + ((size_t)3 * j + a) * K + c0);
     }
 #pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        VT *yp = (VT *)(Y + ((size_t)3 * i + a) * K + c0);
-        if (ADD) *yp = *yp + acc[a];
-        else *yp = acc[a];
-    }
+    for (int a = 0; a < 3; ++a) yp[a * V] = ADD ? yv[a] + acc[a] : acc[a];
 }
 
 template <typename T>
