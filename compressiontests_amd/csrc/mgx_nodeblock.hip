@@ -397,6 +397,9 @@ __device__ __forceinline__ void nb_gs_tile(int pa, int pb, const int32_t *__rest
     const int g = lane / Sh::G, l = lane % Sh::G, s = l / Sh::V, c0 = (l % Sh::V) * Sh::W;
     const int b0 = nrp[pa], b1 = nrp[pb];
     const int p = pa + g;
+    // the node's index is loaded with the tile's first loads, so the X and B rows of the relaxation
+    // wait for one memory round trip after the sums, not two
+    const int q = p < pb ? (node ? node[p] : p) : 0;
     VT y[3] = {VT(0), VT(0), VT(0)};
     const T *d;                                                  // the node's own block (slot 0) in LDS
     if (LOWER) {
@@ -420,7 +423,6 @@ __device__ __forceinline__ void nb_gs_tile(int pa, int pb, const int32_t *__rest
     }
     nb_reduce<T, K>(y);
     if (p >= pb || s != 0) return;
-    const int q = node ? node[p] : p;
     const size_t i0 = (size_t)3 * q * K + c0;
     VT x0 = VT(0), x1 = VT(0), x2 = VT(0);
     if (!LOWER) x0 = *(const VT *)(X + i0), x1 = *(const VT *)(X + i0 + K), x2 = *(const VT *)(X + i0 + 2 * K);
