@@ -229,6 +229,15 @@ __global__ __launch_bounds__(kBlock) void k_nb_spmm(int32_t npos, const int32_t 
         const int p0 = tile * Sh::NW, p1 = min(p0 + Sh::NW, npos);
         const int b0 = nrp[p0], b1 = nrp[p1];
         const int p = p0 + g;
+        // the DOT form's own X rows, loaded with the tile's first loads so that they do not add a
+        // memory round trip after the sums (its loop state already holds the form to 4 waves per
+        // SIMD; in the one-tile forms the extra registers would halve the waves)
+        const int qe = p < p1 ? (node ? node[p] : p) : 0;
+        VT own[3] = {VT(0), VT(0), VT(0)};
+        if (EPI == BEPI_DOT) {
+#pragma unroll
+            for (int a = 0; a < 3; ++a) own[a] = *(const VT *)(X + ((size_t)3 * qe + a) * K + c0);
+        }
         VT y[3] = {VT(0), VT(0), VT(0)};
         wave_lds_fence();                                        // the previous tile's LDS reads are issued
         stage16<Sh::VQ>((const char *)val, (size_t)b0 * 9 * sizeof(T), (size_t)b1 * 9 * sizeof(T), s_val[wv], lane);
@@ -241,7 +250,7 @@ __global__ __launch_bounds__(kBlock) void k_nb_spmm(int32_t npos, const int32_t 
             nb_accumulate<T, K>(lc, lv, nrp[p] - b0 + (EPI == BEPI_UPRESID ? 1 : 0) + s, nrp[p + 1] - b0, X, c0, y);
         nb_reduce<T, K>(y);
         if (p < p1 && s == 0) {
-            const int q = node ? node[p] : p;
+            const int q = qe;
             if (EPI == BEPI_UPRESID) {
                 // after a forward sweep from X = 0 (colours in order, each node's dofs 0, 1, 2 in
                 // order) rows are left with the pull of what was relaxed after them: the later
@@ -264,7 +273,7 @@ __global__ __launch_bounds__(kBlock) void k_nb_spmm(int32_t npos, const int32_t 
                 else if (EPI == BEPI_JACOBI) *yp = *(const VT *)(X + i) + pre[3 * q + a] * (*(const VT *)(B + i) - y[a]);
                 else {
                     *yp = y[a];
-                    const VT xp = *(const VT *)(X + i);
+                    const VT xp = own[a];
 #pragma unroll
                     for (int w = 0; w < W; ++w) dot[w] += (double)xp[w] * (double)y[a][w];
                 }
